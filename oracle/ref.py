@@ -1,0 +1,110 @@
+"""ctypes access to oracle/_ref/libtcgref.so: the reference's own Fortran
+integral routines (built by oracle/Makefile.ref from the reference sources).
+
+Test infrastructure only.  Used by the tests to pin the C restatement
+(oracle/tcg_oracle.c) and the GPU engine to the reference itself.  The
+library exists only where it was built; ``available()`` says whether it did.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_ref", "libtcgref.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_ci = ctypes.c_int
+
+
+def available() -> bool:
+    return os.path.exists(_LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(_LIB_PATH)
+        L.ref_init.argtypes = [_ci, _dp, _ip, _ci, _ci, _ci, _ci, _ci]
+        L.ref_set_basis.argtypes = [_ci, _dp, _ip, _ip, _dp, _dp]
+        L.ref_get_coef.argtypes = [_dp]
+        L.ref_normalize.argtypes = []
+        L.ref_setup_tables.argtypes = [ctypes.POINTER(_ci)]
+        L.ref_get_nodes.argtypes = [ctypes.POINTER(_ci), _dp, _dp, _ip]
+        L.ref_eri_block.argtypes = [_ci] * 8 + [_dp]
+        L.ref_eri_full.argtypes = [_dp]
+        L.ref_one_electron.argtypes = [_ci, _dp, _ip, _dp, _dp, _dp]
+        L.ref_nre.argtypes = [_ci, _dp, _ip, ctypes.POINTER(ctypes.c_double)]
+        L.ref_jk.argtypes = [_ci, _ci, _dp, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+class RefSystem:
+    """One torus system loaded into the reference's module state."""
+
+    def __init__(self, sysm, nq: int = 32, fast_quad: bool = True, sao: bool = True,
+                 physicist: bool = False, normalize: bool = True):
+        self.s = sysm
+        self.sao = sao
+        L = lib()
+        Lv = np.array(sysm.L, dtype=np.float64)
+        ncl = np.array(sysm.ncell, dtype=np.int32)
+        L.ref_init(sysm.ndp, Lv, ncl, sysm.nfuc, nq, int(fast_quad), int(sao), int(physicist))
+        # the reference's AO x/y/z are stored per AO; coordinates as Fortran (3, nbas)
+        xyz = np.ascontiguousarray(sysm.center, dtype=np.float64)
+        lx = np.ascontiguousarray(sysm.lxyz, dtype=np.int32)
+        L.ref_set_basis(sysm.nbas, xyz, lx, sysm.nprim(), sysm.expo, sysm.coef)
+        if normalize:
+            L.ref_normalize()
+        self.coef = np.zeros_like(sysm.coef)
+        L.ref_get_coef(self.coef)
+        n = _ci(0)
+        L.ref_setup_tables(ctypes.byref(n))
+        self.ncap = n.value
+
+    def nodes(self):
+        t = np.zeros(256)
+        w = np.zeros(256)
+        tn = np.zeros(256 * 3, dtype=np.int32)
+        nq = _ci(0)
+        lib().ref_get_nodes(ctypes.byref(nq), t, w, tn)
+        q = nq.value
+        return t[:q].copy(), w[:q].copy(), tn[:q * self.s.ndp].reshape(self.s.ndp, q).copy()
+
+    def eri_block(self, a, na, b, nb, c, nc, d, nd):
+        """Shell block; a..d are 0-based first AOs. Returns blk[na,nb,nc,nd]."""
+        out = np.zeros(na * nb * nc * nd)
+        lib().ref_eri_block(a + 1, na, b + 1, nb, c + 1, nc, d + 1, nd, out)
+        return out.reshape(nd, nc, nb, na).transpose(3, 2, 1, 0).copy()
+
+    def eri_full(self):
+        """ERI as the reference stores it: Fortran ERI(n1,nBas,nBas,nBas).
+
+        Returned as a C-ordered array E[l,k,j,i] (the same memory), i.e.
+        E[l,k,j,i] = ERI(i,j,k,l)."""
+        nb = self.s.nbas
+        n1 = self.s.nfuc if self.sao else nb
+        out = np.zeros(n1 * nb * nb * nb)
+        lib().ref_eri_full(out)
+        return out.reshape(nb, nb, nb, n1)
+
+    def one_electron(self):
+        nb = self.s.nbas
+        S = np.zeros(nb * nb)
+        T = np.zeros(nb * nb)
+        V = np.zeros(nb * nb)
+        geom = np.asfortranarray(self.s.geom).ravel(order="F").copy()
+        lib().ref_one_electron(self.s.natom, geom, self.s.Z.astype(np.int32), S, T, V)
+        f = lambda M: M.reshape(nb, nb).T.copy()
+        return f(S), f(T), f(V)
+
+    def nre(self) -> float:
+        geom = np.asfortranarray(self.s.geom).ravel(order="F").copy()
+        e = ctypes.c_double(0.0)
+        lib().ref_nre(self.s.natom, geom, self.s.Z.astype(np.int32), ctypes.byref(e))
+        return e.value

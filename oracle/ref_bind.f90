@@ -1,0 +1,274 @@
+!---------------------------------------------------------------------!
+! C-callable driver around the reference's own integral routines,     !
+! compiled together with the reference sources (read in place from    !
+! the reference tree by oracle/Makefile.ref) into oracle/_ref/.        !
+!                                                                     !
+! Test infrastructure only: it exists so that the tests can compare   !
+! the C restatement (oracle/tcg_oracle.c) and the GPU engine with the !
+! reference's own Fortran on the same inputs.  Nothing here is part   !
+! of the engine.  Every routine sets the reference's module state     !
+! (torus, unit cell, quadrature, keywords) explicitly, then calls the !
+! reference routine unchanged:                                        !
+!   ERI_shell_block_toroidal   src/Toroidal/ERI_shell_block_toroidal.f90:28
+!   ERI_integral_toroidal      src/Toroidal/ERI_toroidal.f90:36        !
+!   normalize_AO_toroidal_nD   src/basis/normalization/normalize_1D.f90:17
+!   overlap/kinetic/nuclear    src/Toroidal/{1D,2D,3D}/*.f90           !
+!   hartree_exchange_potential src/Energies/hartree_exchange_potential.f90:30
+!   NRE                        src/Energies/Nuclear_repulsion.f90:1    !
+!---------------------------------------------------------------------!
+module ref_bind
+
+      use iso_c_binding
+      use classification_ERI
+      use torus_init
+      use unitcell_module
+      use keywords
+      use atom_basis
+      use precomputed_bessel, only : nquad_point, fast_quad
+      use clifford_tools,     only : init_clifford, setup_t_tables, pair_cache_free, &
+                                     bessel_nmax_estimate, orbital_to_lxyz, t_bessel, &
+                                     nq_t, t_node, t_weight, t_nmax, ps_on, ps_noscreen
+
+      implicit none
+
+      type(ERI_function), allocatable, save :: AOl(:)
+      integer, save :: ndp_s = 1
+
+      interface
+        subroutine ERI_shell_block_toroidal(ndp,nA,SA,nB,SB,nC,SC,nD,SD,blk)
+          import :: ERI_function
+          integer, intent(in) :: ndp, nA, nB, nC, nD
+          type(ERI_function), intent(in) :: SA(nA), SB(nB), SC(nC), SD(nD)
+          double precision, intent(out) :: blk(nA,nB,nC,nD)
+        end subroutine
+        subroutine ERI_integral_toroidal(ndp,nBas,AO,two_electron_integrals)
+          import :: ERI_function, ERI_size
+          integer, intent(in) :: ndp, nBas
+          type(ERI_function), intent(in) :: AO(nBas)
+          double precision, intent(out) :: two_electron_integrals(ERI_size,nBas,nBas,nBas)
+        end subroutine
+      end interface
+
+contains
+
+      subroutine ref_init(ndp, Lv, ncl, nfuc_in, nq, fq, sao, phys) bind(C, name="ref_init")
+        integer(c_int), value      :: ndp, nfuc_in, nq, fq, sao, phys
+        real(c_double), intent(in) :: Lv(3)
+        integer(c_int), intent(in) :: ncl(3)
+        double precision, parameter :: pi = 3.14159265358979323846D00
+        ndp_s = ndp
+        Lx = Lv(1) ; Ly = Lv(2) ; Lz = Lv(3)
+        ax = 2.d0*pi/Lx
+        if (Ly > 0.d0) ay = 2.d0*pi/Ly
+        if (Lz > 0.d0) az = 2.d0*pi/Lz
+        nx = ncl(1) ; ny = ncl(2) ; nz = ncl(3)
+        nfuc = nfuc_in
+        N_cell = nx*ny*nz
+        c_SAO = (sao /= 0)
+        c_physicist = (phys /= 0)
+        c_prim_screen = .true.
+        c_symmetry = .false.
+        c_debug = .false.
+        c_fast_quad = (fq /= 0)
+        fast_quad = (fq /= 0)
+        nquad_point = nq
+        ps_on = .false. ; ps_noscreen = .false.
+        if (allocated(t_bessel)) deallocate(t_bessel)     ! force setup_t_tables to rebuild for this torus
+        call pair_cache_free()
+        call init_clifford()
+      end subroutine ref_init
+
+      subroutine label_of(l3, lab)
+        integer, intent(in) :: l3(3)
+        character(len=5), intent(out) :: lab
+        character(len=1), parameter :: sh(0:4) = (/ 's','p','d','f','g' /)
+        integer :: c, k
+        lab = ' '
+        lab(1:1) = sh(sum(l3))
+        k = 1
+        do c = 1, l3(1) ; k = k + 1 ; lab(k:k) = 'x' ; end do
+        do c = 1, l3(2) ; k = k + 1 ; lab(k:k) = 'y' ; end do
+        do c = 1, l3(3) ; k = k + 1 ; lab(k:k) = 'z' ; end do
+      end subroutine label_of
+
+      subroutine ref_set_basis(nbas, xyz, lxyz, nprim, expo, coef) bind(C, name="ref_set_basis")
+        integer(c_int), value      :: nbas
+        real(c_double), intent(in) :: xyz(3,nbas), expo(*), coef(*)
+        integer(c_int), intent(in) :: lxyz(3,nbas), nprim(nbas)
+        integer :: i, o
+        if (allocated(AOl)) deallocate(AOl)
+        allocate(AOl(nbas))
+        o = 0
+        do i = 1, nbas
+          AOl(i)%x = xyz(1,i) ; AOl(i)%y = xyz(2,i) ; AOl(i)%z = xyz(3,i)
+          call label_of(lxyz(:,i), AOl(i)%orbital)
+          AOl(i)%exponent    = expo(o+1:o+nprim(i))
+          AOl(i)%coefficient = coef(o+1:o+nprim(i))
+          o = o + nprim(i)
+        end do
+      end subroutine ref_set_basis
+
+      subroutine ref_get_coef(coef) bind(C, name="ref_get_coef")
+        real(c_double), intent(out) :: coef(*)
+        integer :: i, o, n
+        o = 0
+        do i = 1, size(AOl)
+          n = size(AOl(i)%coefficient)
+          coef(o+1:o+n) = AOl(i)%coefficient
+          o = o + n
+        end do
+      end subroutine ref_get_coef
+
+      subroutine ref_normalize() bind(C, name="ref_normalize")
+        select case (ndp_s)
+        case (1) ; call normalize_AO_toroidal_1D(size(AOl), AOl, .false.)
+        case (2) ; call normalize_AO_toroidal_2D(size(AOl), AOl, .false.)
+        case default ; call normalize_AO_toroidal_3D(size(AOl), AOl, .false.)
+        end select
+      end subroutine ref_normalize
+
+      ! t-tables as the ERI driver builds them (src/Toroidal/ERI_toroidal.f90:94-104)
+      subroutine ref_setup_tables(ncap_out) bind(C, name="ref_setup_tables")
+        integer(c_int), intent(out) :: ncap_out
+        integer :: i, lxyz(3), lmax, ncap
+        double precision :: emax, amin, a_dir(3)
+        lmax = 0 ; emax = 0.d0
+        do i = 1, size(AOl)
+          call orbital_to_lxyz(AOl(i)%orbital, lxyz)
+          lmax = max(lmax, maxval(lxyz(1:ndp_s)))
+          emax = max(emax, maxval(AOl(i)%exponent))
+        end do
+        a_dir = (/ ax, ay, az /)
+        amin = minval(a_dir(1:ndp_s))
+        ncap = bessel_nmax_estimate(4.d0*emax/(amin*amin)) + 2*lmax + 1
+        call setup_t_tables(ncap, ndp_s)
+        ncap_out = ncap
+      end subroutine ref_setup_tables
+
+      subroutine ref_get_nodes(nq, t, w, tnmax) bind(C, name="ref_get_nodes")
+        integer(c_int), intent(out) :: nq
+        real(c_double), intent(out) :: t(*), w(*)
+        integer(c_int), intent(out) :: tnmax(*)
+        integer :: k, d
+        nq = nq_t
+        t(1:nq_t) = t_node(1:nq_t)
+        w(1:nq_t) = t_weight(1:nq_t)
+        do d = 1, ndp_s
+          do k = 1, nq_t
+            tnmax(k + nq_t*(d-1)) = t_nmax(k,d)
+          end do
+        end do
+      end subroutine ref_get_nodes
+
+      ! one shell block; ia..id are 1-based first AOs, na..nd component counts
+      subroutine ref_eri_block(ia, na, ib, nb, ic, nc, id, nd, blk) bind(C, name="ref_eri_block")
+        integer(c_int), value       :: ia, na, ib, nb, ic, nc, id, nd
+        real(c_double), intent(out) :: blk(na,nb,nc,nd)
+        call ERI_shell_block_toroidal(ndp_s, na, AOl(ia:ia+na-1), nb, AOl(ib:ib+nb-1), &
+                                      nc, AOl(ic:ic+nc-1), nd, AOl(id:id+nd-1), blk)
+      end subroutine ref_eri_block
+
+      subroutine ref_eri_full(eri) bind(C, name="ref_eri_full")
+        real(c_double), intent(out) :: eri(*)
+        integer :: nb
+        nb = size(AOl)
+        if (c_SAO) then
+          ERI_size = nfuc
+        else
+          ERI_size = nb
+        end if
+        call ERI_integral_toroidal(ndp_s, nb, AOl, eri)
+      end subroutine ref_eri_full
+
+      subroutine ref_one_electron(natom, geom, zat, S, T, V) bind(C, name="ref_one_electron")
+        integer(c_int), value       :: natom
+        real(c_double), intent(in)  :: geom(natom,3)
+        integer(c_int), intent(in)  :: zat(natom)
+        real(c_double), intent(out) :: S(*), T(*), V(*)
+        type(atom), allocatable :: ats(:)
+        integer :: i, nb
+        nb = size(AOl)
+        allocate(ats(natom))
+        do i = 1, natom
+          ats(i)%charge = zat(i)
+        end do
+        select case (ndp_s)
+        case (1)
+          call overlap_matrix_toroidal_1D(nb, AOl, S)
+          call kinetic_matrix_toroidal_1D(nb, AOl, T)
+          call nuclear_attraction_matrix_toroidal_1D(natom, nb, geom, ats, AOl, V)
+        case (2)
+          call overlap_matrix_toroidal_2D(nb, AOl, S)
+          call kinetic_matrix_toroidal_2D(nb, AOl, T)
+          call nuclear_attraction_matrix_toroidal_2D(natom, nb, geom, ats, AOl, V)
+        case default
+          call overlap_matrix_toroidal_3D(nb, AOl, S)
+          call kinetic_matrix_toroidal_3D(nb, AOl, T)
+          call nuclear_attraction_matrix_toroidal_3D(natom, nb, geom, ats, AOl, V)
+        end select
+      end subroutine ref_one_electron
+
+      subroutine ref_nre(natom, geom, zat, E) bind(C, name="ref_nre")
+        integer(c_int), value       :: natom
+        real(c_double), intent(in)  :: geom(natom,3)
+        integer(c_int), intent(in)  :: zat(natom)
+        real(c_double), intent(out) :: E
+        type(atom), allocatable :: ats(:)
+        character(len=10) :: ctype
+        integer :: i
+        allocate(ats(natom))
+        do i = 1, natom
+          ats(i)%charge = zat(i)
+        end do
+        select case (ndp_s)
+        case (1) ; ctype = 'Tori1D'
+        case (2) ; ctype = 'Tori2D'
+        case default ; ctype = 'Tori3D'
+        end select
+        call NRE(ctype, natom, geom, ats, E)
+      end subroutine ref_nre
+
+      ! scaled Bessel function and vectors (src/module/bessel_iv_log.f90:320,
+      ! src/module/clifford_tools.f90:386,429), quadrature nodes (precompute_t.f90:906)
+      real(c_double) function ref_iv_scaled(v, x) bind(C, name="ref_iv_scaled")
+        use bessel_functions, only : iv_scaled
+        integer(c_int), value :: v
+        real(c_double), value :: x
+        ref_iv_scaled = iv_scaled(v, x)
+      end function ref_iv_scaled
+
+      subroutine ref_bessel_vector(n, A, fast, out) bind(C, name="ref_bessel_vector")
+        use clifford_tools, only : bessel_scaled_vector, bessel_scaled_vector_fast
+        integer(c_int), value :: n, fast
+        real(c_double), value :: A
+        real(c_double), intent(out) :: out(0:n)
+        if (fast /= 0) then
+          call bessel_scaled_vector_fast(n, A, out)
+        else
+          call bessel_scaled_vector(n, A, out)
+        end if
+      end subroutine ref_bessel_vector
+
+      subroutine ref_fast_quad_points(n, t, w) bind(C, name="ref_fast_quad_points")
+        use precomputed_bessel, only : fast_quad_points
+        integer(c_int), value :: n
+        real(c_double), intent(out) :: t(n), w(n)
+        call fast_quad_points(n, t, w)
+      end subroutine ref_fast_quad_points
+
+      subroutine ref_bary(d, e1, e2, x1, x2, pe, cen) bind(C, name="ref_bary")
+        use clifford_tools, only : bary_dir
+        integer(c_int), value :: d
+        real(c_double), value :: e1, e2, x1, x2
+        real(c_double), intent(out) :: pe, cen
+        call bary_dir(d, e1, e2, x1, x2, pe, cen)
+      end subroutine ref_bary
+
+      subroutine ref_jk(n1, nbas, P, eri, J, K) bind(C, name="ref_jk")
+        integer(c_int), value       :: n1, nbas
+        real(c_double), intent(in)  :: P(nbas,nbas), eri(*)
+        real(c_double), intent(out) :: J(nbas,nbas), K(nbas,nbas)
+        call hartree_exchange_potential(n1, nbas, P, eri, J, K)
+      end subroutine ref_jk
+
+end module ref_bind
