@@ -1,0 +1,92 @@
+/*
+ * tcg.h -- C ABI of the MI355X Clifford-Gaussian (toroidal) integral engine.
+ *
+ * Drop-in for the integral and energy calls of the reference's Tori1D/2D/3D
+ * path.  Plain pointers and sizes only; every array is owned by the caller,
+ * FP64, column-major (Fortran) exactly as the reference declares it.  Every
+ * entry point returns 0 on success and a negative status otherwise, with a
+ * message from tcg_last_error().  One context per torus system; a context is
+ * not re-entrant and must be used from one host thread at a time.  All calls
+ * are synchronous: results are in the caller's arrays on return.
+ *
+ * A Fortran caller binds these with ISO_C_BINDING.
+ */
+#ifndef TCG_H
+#define TCG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tcg_ctx tcg_ctx;
+
+enum {
+    TCG_OK = 0,
+    TCG_E_ARG = -1,        /* invalid argument */
+    TCG_E_STATE = -2,      /* call out of order (e.g. no basis yet) */
+    TCG_E_DEVICE = -3,     /* HIP runtime error */
+    TCG_E_LATTICE = -4,    /* shells do not repeat in every cell */
+    TCG_E_NOMEM = -5
+};
+
+/* Torus and quadrature.  Replaces the module state the reference reads:
+ *   torus_init (Lx,Ly,Lz,ax,ay,az)      src/module/Torus_init.f90:6-7
+ *   unitcell_module (nx,ny,nz)          src/module/unitcell.f90:2-4
+ *   nquad_point / fast_quad (32 or 40)  src/calculation_type/calc_call.f90:25-48
+ * ndp = number of periodic directions (1: Tori1D, 2: Tori2D, 3: Tori3D).
+ * L[d] is read for d < ndp; ncell = (nx, ny, nz). */
+int tcg_create(int ndp, const double L[3], const int ncell[3], int nquad, tcg_ctx **out);
+
+/* AO list as a structure of arrays.  Replaces type(ERI_function) AO(nBas)
+ * (src/module/ERI_mod.f90:5-12, built by src/basis/class.f90:1-70):
+ *   center[3*i+d]      centre of AO i
+ *   lxyz[3*i+d]        Cartesian powers of AO i (from AO%orbital)
+ *   prim_off[i]..prim_off[i+1]-1   its primitives in expo[] / coef[]
+ *   coef[]             NORMALISED contraction coefficients (the reference
+ *                      passes the normalised copy AOn, src/calculation_type/Tori1D.f90:80-82)
+ * nfuc = functions per unit cell. */
+int tcg_set_basis(tcg_ctx *ctx, int nbas, int nfuc, const double *center, const int *lxyz,
+                  const int *prim_off, const double *expo, const double *coef);
+
+/* Two-electron integrals.  Replaces
+ *   ERI_integral_toroidal(ndp,nBas,AO,ERI)   src/Toroidal/ERI_toroidal.f90:36
+ * eri: ERI(n1,nBas,nBas,nBas) with n1 = nfuc (sao != 0, keyword SAO) or nBas;
+ * chemist order ERI(i,j,k,l) = (ij|kl), or physicist ERI(i,k,j,l) when
+ * physicist != 0.  The tensor also stays resident on the GPU for
+ * tcg_jk_stored.  eri may be NULL: then it is kept on the GPU only. */
+int tcg_eri_build(tcg_ctx *ctx, int sao, int physicist, double *eri);
+
+/* Coulomb and exchange from the stored integrals.  Replaces
+ *   hartree_exchange_potential(n1,nBas,P,ERI,J,K)
+ *                                   src/Energies/hartree_exchange_potential.f90:30
+ * J(mu,nu) = sum P(la,si)(mu nu|la si), K(mu,nu) = -1/2 sum P(la,si)(mu la|si nu),
+ * rows mu <= n1 written, the rest zero; P, J, K are (nBas,nBas).
+ * eri == NULL uses the tensor resident from tcg_eri_build; otherwise eri is
+ * the caller's ERI(n1,nBas,nBas,nBas) (uploaded for this call). */
+int tcg_jk_stored(tcg_ctx *ctx, int n1, int nbas, const double *P, const double *eri, double *J, double *K);
+
+/* Counters of the last tcg_eri_build:
+ *   out[0] shell quartets evaluated   out[1] primitive quartets evaluated
+ *   out[2] rows (primitive quartet x direction x power combination)
+ *   out[3] algorithmic FP64 flops of the quadrature contraction
+ *   out[4] nonzero elements stored (n1 * nBas^3 counted over the array)
+ * n = number of slots the caller provides (up to 8). */
+int tcg_eri_stats(tcg_ctx *ctx, long long *out, int n);
+
+/* Wall times in milliseconds of the last tcg_eri_build sub-steps, the
+ * analogue of the reference's tstep table (src/module/clifford_tools.f90:816-852):
+ *   out[0] tables + pairs   out[1] Schwarz + primitive screen
+ *   out[2] quartet list     out[3] quartet kernel + scatter   out[4] total */
+int tcg_eri_times(tcg_ctx *ctx, double *out, int n);
+
+/* Device address of the resident ERI tensor (for callers that keep data on
+ * the GPU); *count = number of doubles. */
+int tcg_eri_device(tcg_ctx *ctx, const double **dptr, long long *count);
+
+const char *tcg_last_error(tcg_ctx *ctx);
+int tcg_destroy(tcg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,49 @@
+// Launch interface of the gfx950 kernels (tcg_eri.hip) used by the host
+// driver (tcg_host.cpp).  Pointers are device addresses.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "tcg_engine.hpp"
+
+namespace tcg {
+
+// Everything the quartet kernel reads.
+struct EriArgs {
+    int ndp, nq, nqp, ncap;           // periodic directions, nodes, nodes padded to 16, Bessel cap
+    double a[3], L[3];
+    double fac_const;                 // 2/sqrt(pi) prod_d L_d^2
+    const double *tab;                // [d][ncap+1+4][nqp] c_n exp(-C)I_n(C), 0 beyond t_nmax(k)
+    const int *tab_lim;               // [d][nqp/16]: last n with a nonzero entry in that node tile
+    const double *tnode, *tweight;    // [nqp] (0 in the padding)
+    const Shell *sh;
+    const int *ao_lxyz;               // [nbas][3]
+    const double *coef;               // normalised coefficients, per AO
+    const int *ao_poff;               // [nbas+1]
+    const PrimPair *pp;
+    const ShellPair *sp;
+    const double2 *vec;               // Fourier vectors (sign +1 form)
+    const double *herm;
+    const Quartet *q4;
+    int nq4;
+    int screen_prefactor;             // kernel prefactor screens (off in the primitive Schwarz pass)
+    int screen_prim;                  // primitive Schwarz screen (keyword prim_screen)
+    int max_ncomb;                    // largest (lA+1)(lB+1)(lC+1)(lD+1) over the quartets
+    // block mode
+    double *blk_out;
+    // scatter mode
+    double *eri;                      // ERI(n1,nbas,nbas,nbas)
+    int n1, nbas, nfuc, ncell, physicist;
+    const double *Qao;                // [nbas*nbas] function-pair Schwarz values
+    const int *relc;                  // [ncell*ncell]
+    // counters
+    unsigned long long *counters;     // [0] primitive quartets, [1] rows
+};
+
+size_t eri_lds_bytes(const EriArgs &a);
+hipError_t launch_eri_quartets(const EriArgs &a, hipStream_t st);
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
+size_t jk_work_doubles(int n1, int nbas);
+hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,
+                            double *jwork, hipStream_t st);
+
+}  // namespace tcg

@@ -1,0 +1,62 @@
+// Internal data layout of the engine, shared by the host setup (tcg_host.cpp)
+// and the gfx950 kernels (tcg_eri.hip).  Everything the kernels read is a
+// flat structure of arrays in HBM.
+#pragma once
+#include <cstdint>
+
+namespace tcg {
+
+constexpr int kMaxL = 4;            // up to g shells
+constexpr int kMaxNQ = 64;          // quadrature nodes, padded to node tiles of 16
+constexpr int kNodeTile = 16;       // nodes per MFMA column tile
+constexpr int kRowTile = 16;        // rows per MFMA tile
+constexpr int kMaxPrim = 16;        // primitives per shell
+constexpr double kScreenTol = 2.22e-16;  // prefactor screen (ERI_shell_block_toroidal.f90:42)
+constexpr double kPrimTol = 1e-14;       // primitive Schwarz (clifford_tools.f90:74)
+constexpr double kSchwarz = 1e-13;       // shell Schwarz (ERI_toroidal.f90:64)
+
+// One primitive pair (i of shell A, j of shell B) of a representative shell
+// pair (A in the unit cell).  Coefficient-free prefactor, barycentres,
+// Bessel lengths per periodic direction, offsets of its Fourier vectors and
+// Hermite coefficients.
+struct PrimPair {
+    double fac;          // prod_d exp(-2(e1+e2-p_d)/a_d^2) * prod_real exp(-mu R^2)
+    double cen[3];       // barycentre per direction (periodic: toroidal barycentre)
+    double q;            // e1 + e2
+    double cmax;         // max_comp |c_A(i)| * max_comp |c_B(j)|  (shell-wide max per primitive)
+    double ps;           // primitive Schwarz value sqrt|(ij|ij)| (max over components)
+    int N[3];            // Bessel length per periodic direction
+    int voff[3];         // offset (complex elements) of the Fourier vectors of direction d:
+                         //   vec[voff + combo*(N+1) + n], combo = ia*(lB+1)+ib
+    int eoff;            // offset of Hermite E: herm[eoff + ((dr*(lA+1)+ia)*(lB+1)+ib)*(2*kMaxL+1) + t]
+    int pi, pj;          // primitive indices
+};
+
+// Representative shell pair (A in the unit cell, B anywhere).
+struct ShellPair {
+    int first;           // first PrimPair
+    int count;           // n_prim(A) * n_prim(B), ordered i major
+};
+
+// One shell quartet to evaluate.
+struct Quartet {
+    double shift[3];     // translation of the ket pair: centre(SK) - centre(representative of SK)
+    int s[4];            // global shell indices SI, SJ, SK, SL
+    int bra;             // representative pair of (SI, SJ)   (SI is in the unit cell)
+    int ket;             // representative pair of (SK, SL) translated to the unit cell
+    int out;             // offset of the output block (block mode), or -1
+    int prim;            // >= 0: only primitive pair 'prim' of bra and ket (primitive Schwarz pass)
+};
+
+struct Shell {
+    int first_ao;        // first AO
+    int ncomp;           // Cartesian components
+    int l;               // angular momentum
+    int nprim;
+    int poff;            // offset into expo/coef of the first AO's primitives
+    int cell;            // cell index
+    int loc;             // local index inside the cell
+    double xyz[3];
+};
+
+}  // namespace tcg

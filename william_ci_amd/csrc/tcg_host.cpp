@@ -1,0 +1,672 @@
+// Host driver and C ABI of the engine (include/tcg.h).
+//
+// tcg_eri_build follows the steps of the reference's ERI driver
+// (src/Toroidal/ERI_toroidal.f90:36-562), regular-lattice path:
+//   1. t-tables (nodes, weights, c_n exp(-C)I_n(C) per periodic direction)
+//   2. shells, cell tables, representative shell pairs (first shell in the
+//      unit cell) with their primitive pairs and Fourier vectors
+//   3. Schwarz: diagonal blocks (SI SJ|SI SJ) on the GPU, replicated over
+//      the lattice translations
+//   4. primitive Schwarz table (keyword prim_screen, always on)
+//   5. translation x 8-fold unique shell quartets, Schwarz screened
+//   6. quartets on the GPU, scattered into the SAO/normal layout
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tcg.h"
+#include "tcg_device.hpp"
+#include "tcg_engine.hpp"
+#include "tcg_math.hpp"
+
+using namespace tcg;
+
+namespace {
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        n = count;
+        return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1));
+    }
+    hipError_t upload(const std::vector<T> &v) {
+        hipError_t e = alloc(v.size());
+        if (e != hipSuccess) return e;
+        if (!v.empty()) e = hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+        return e;
+    }
+};
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct tcg_ctx {
+    int ndp = 1;
+    double L[3] = {0, 0, 0}, a[3] = {0, 0, 0};
+    int ncl[3] = {1, 1, 1};
+    int ncell = 1;
+    int nq = 32, nqp = 32;
+    std::string err;
+    // basis
+    int nbas = 0, nfuc = 0;
+    std::vector<double> center, expo, coef;
+    std::vector<int> lxyz, poff;
+    // shells / lattice
+    std::vector<Shell> sh;
+    int nsh = 0, nshuc = 0;
+    std::vector<int> relc;
+    // tables
+    int ncap = 0;
+    std::vector<double> tnode, tweight;
+    // pair data
+    std::vector<PrimPair> pp;
+    std::vector<ShellPair> sp;
+    // device
+    hipStream_t st = nullptr;
+    DevBuf<double> d_tab, d_tnode, d_tw, d_coef, d_herm, d_blk, d_Q, d_eri, d_P, d_J, d_K, d_jwork;
+    DevBuf<int> d_tablim, d_lxyz, d_poff, d_relc;
+    DevBuf<Shell> d_sh;
+    DevBuf<PrimPair> d_pp;
+    DevBuf<ShellPair> d_sp;
+    DevBuf<double2> d_vec;
+    DevBuf<Quartet> d_q4;
+    DevBuf<unsigned long long> d_cnt;
+    size_t eri_count = 0;
+    int eri_n1 = 0;
+    long long stats[8] = {0};
+    double times[8] = {0};
+    int max_ncomb = 1;
+
+    int fail(int code, const char *fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPCHK(ctx, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return (ctx)->fail(TCG_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+static int lsum3(const int *l) { return l[0] + l[1] + l[2]; }
+
+// ---------------------------------------------------------------------------
+// 1. t-tables (src/module/clifford_tools.f90:620-709 with ncap from ERI_toroidal.f90:94-104)
+// ---------------------------------------------------------------------------
+static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim) {
+    int lmax = 0;
+    double emax = 0.0, amin = 1e300;
+    for (int i = 0; i < c->nbas; ++i) {
+        for (int d = 0; d < c->ndp; ++d) lmax = std::max(lmax, c->lxyz[3 * i + d]);
+        for (int p = c->poff[i]; p < c->poff[i + 1]; ++p) emax = std::max(emax, c->expo[p]);
+    }
+    for (int d = 0; d < c->ndp; ++d) amin = std::min(amin, c->a[d]);
+    c->ncap = nmax_est(4.0 * emax / (amin * amin)) + 2 * lmax + 1;
+    const int ncap = c->ncap, nqp = c->nqp, rows = ncap + 5;
+    c->tnode.assign(nqp, 0.0);
+    c->tweight.assign(nqp, 0.0);
+    mapped_gauss_legendre(c->nq, c->tnode.data(), c->tweight.data());
+    tab.assign((size_t)c->ndp * rows * nqp, 0.0);
+    const int nh = nqp / kNodeTile;
+    tablim.assign((size_t)c->ndp * nh, -1);
+    std::vector<double> col(ncap + 1);
+    for (int d = 0; d < c->ndp; ++d) {
+        for (int k = 0; k < c->nq; ++k) {
+            const double C = 2.0 * c->tnode[k] * c->tnode[k] / (c->a[d] * c->a[d]);
+            const int n0 = ncap + 30 + (int)std::sqrt(80.0 * C);
+            bessel_miller(ncap, C, n0, col.data());
+            int nmax = ncap;   // last order kept: first below 1e-16 relative, + 5
+            for (int n = 0; n <= ncap; ++n)
+                if (col[n] < 1e-16 * col[0]) { nmax = std::min(ncap, n + 5); break; }
+            for (int n = 0; n <= nmax; ++n) tab[((size_t)d * rows + n) * nqp + k] = (n == 0 ? 1.0 : 2.0) * col[n];
+            int &lim = tablim[d * nh + k / kNodeTile];
+            lim = std::max(lim, nmax);
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// 2. shells and lattice (ERI_toroidal.f90:106-156)
+// ---------------------------------------------------------------------------
+static int build_shells(tcg_ctx *c) {
+    c->sh.clear();
+    for (int i = 0; i < c->nbas;) {
+        const int lt = lsum3(&c->lxyz[3 * i]);
+        int nc = (lt + 1) * (lt + 2) / 2;
+        bool same = i + nc <= c->nbas;
+        for (int j = i + 1; same && j < i + nc; ++j) {
+            same = lsum3(&c->lxyz[3 * j]) == lt && c->center[3 * j] == c->center[3 * i] &&
+                   c->center[3 * j + 1] == c->center[3 * i + 1] && c->center[3 * j + 2] == c->center[3 * i + 2] &&
+                   c->poff[j + 1] - c->poff[j] == c->poff[i + 1] - c->poff[i];
+            for (int p = 0; same && p < c->poff[i + 1] - c->poff[i]; ++p)
+                same = c->expo[c->poff[j] + p] == c->expo[c->poff[i] + p];
+        }
+        if (!same) return c->fail(TCG_E_ARG, "AO %d: Cartesian components of a shell must be consecutive", i);
+        Shell s{};
+        s.first_ao = i;
+        s.ncomp = nc;
+        s.l = lt;
+        s.nprim = c->poff[i + 1] - c->poff[i];
+        s.poff = c->poff[i];
+        for (int d = 0; d < 3; ++d) s.xyz[d] = c->center[3 * i + d];
+        if (lt > 2) return c->fail(TCG_E_ARG, "shell at AO %d: l = %d; this engine handles s, p, d", i, lt);
+        if (s.nprim > kMaxPrim) return c->fail(TCG_E_ARG, "shell at AO %d: %d primitives (max %d)", i, s.nprim, kMaxPrim);
+        c->sh.push_back(s);
+        i += nc;
+    }
+    c->nsh = (int)c->sh.size();
+    c->nshuc = 0;
+    for (auto &s : c->sh)
+        if (s.first_ao < c->nfuc) c->nshuc++;
+    bool ok = c->nsh == c->nshuc * c->ncell && c->nbas == c->nfuc * c->ncell;
+    for (int S = 0; ok && S < c->nsh - c->nshuc; ++S)
+        ok = c->sh[S + c->nshuc].ncomp == c->sh[S].ncomp && c->sh[S + c->nshuc].first_ao == c->sh[S].first_ao + c->nfuc;
+    if (!ok || c->ncell > 4096)
+        return c->fail(TCG_E_LATTICE, "the shells do not repeat in every cell (nsh=%d, per cell %d, cells %d)", c->nsh,
+                       c->nshuc, c->ncell);
+    for (int S = 0; S < c->nsh; ++S) {
+        c->sh[S].cell = S / c->nshuc;
+        c->sh[S].loc = S % c->nshuc;
+    }
+    const int nc = c->ncell, n0 = c->ncl[0], n1 = c->ncl[1], n2 = c->ncl[2];
+    c->relc.resize((size_t)nc * nc);
+    for (int c1 = 0; c1 < nc; ++c1)
+        for (int c2 = 0; c2 < nc; ++c2) {
+            int dx = ((c2 % n0 - c1 % n0) % n0 + n0) % n0;
+            int dy = (((c2 / n0) % n1 - (c1 / n0) % n1) % n1 + n1) % n1;
+            int dz = ((c2 / (n0 * n1) - c1 / (n0 * n1)) % n2 + n2) % n2;
+            c->relc[(size_t)c1 * nc + c2] = dx + n0 * (dy + n1 * dz);
+        }
+    return 0;
+}
+
+// representative pair of (S1, S2): S1 shifted to the unit cell
+static inline int rep_pair(const tcg_ctx *c, int S1, int S2) {
+    const int c1 = c->sh[S1].cell, c2 = c->sh[S2].cell;
+    const int S2r = c->relc[(size_t)c1 * c->ncell + c2] * c->nshuc + c->sh[S2].loc;
+    return c->sh[S1].loc * c->nsh + S2r;
+}
+
+// ---------------------------------------------------------------------------
+// pair Fourier vector V_m = sum_k d_k exp(-A) I_{|m-k|}(A), m = 0..N
+// (src/module/clifford_tools.f90:467-549, sign +1)
+// ---------------------------------------------------------------------------
+static void pair_vector(int N, double A, int i, int l, double phA, double phB, double2 *V, std::vector<double> &iv) {
+    iv.resize(N + i + l + 1);
+    bessel_miller(N + i + l, A, N + i + l, iv.data());
+    if (i + l == 0) {
+        for (int m = 0; m <= N; ++m) V[m] = double2{iv[m], 0.0};
+        return;
+    }
+    std::complex<double> d[4 * kMaxL + 1];
+    sin_product_fourier(i, l, phA, phB, d);
+    const int K = i + l;
+    for (int m = 0; m <= N; ++m) {
+        std::complex<double> acc = 0.0;
+        for (int k = -K; k <= K; ++k) acc += d[k + K] * iv[std::abs(m - k)];
+        V[m] = double2{acc.real(), acc.imag()};
+    }
+}
+
+static int build_pairs(tcg_ctx *c, std::vector<double2> &vec, std::vector<double> &herm) {
+    c->pp.clear();
+    c->sp.assign((size_t)c->nshuc * c->nsh, ShellPair{0, 0});
+    vec.clear();
+    herm.clear();
+    std::vector<double> iv;
+    const int nreal = 3 - c->ndp;
+    for (int S1 = 0; S1 < c->nshuc; ++S1)
+        for (int S2 = 0; S2 < c->nsh; ++S2) {
+            const Shell &A = c->sh[S1], &B = c->sh[S2];
+            ShellPair &spr = c->sp[(size_t)S1 * c->nsh + S2];
+            spr.first = (int)c->pp.size();
+            spr.count = A.nprim * B.nprim;
+            for (int i = 0; i < A.nprim; ++i)
+                for (int j = 0; j < B.nprim; ++j) {
+                    PrimPair P{};
+                    P.pi = i;
+                    P.pj = j;
+                    const double e1 = c->expo[A.poff + i], e2 = c->expo[B.poff + j];
+                    const double esum = e1 + e2, mu = e1 * e2 / esum;
+                    P.q = esum;
+                    double fac = 1.0;
+                    for (int d = 0; d < 3; ++d) {
+                        if (d < c->ndp) {
+                            double pe, cen;
+                            clifford_pair(e1, e2, A.xyz[d], B.xyz[d], c->L[d], c->a[d], &pe, &cen);
+                            P.cen[d] = cen;
+                            fac *= std::exp(-2.0 * (esum - pe) / (c->a[d] * c->a[d]));
+                            const double Aarg = 2.0 * pe / (c->a[d] * c->a[d]);
+                            const int N = std::min(nmax_est(Aarg), c->ncap);
+                            P.N[d] = N;
+                            P.voff[d] = (int)vec.size();
+                            vec.resize(vec.size() + (size_t)(A.l + 1) * (B.l + 1) * (N + 1));
+                            for (int ia = 0; ia <= A.l; ++ia)
+                                for (int ib = 0; ib <= B.l; ++ib)
+                                    pair_vector(N, Aarg, ia, ib, c->a[d] * (cen - A.xyz[d]), c->a[d] * (cen - B.xyz[d]),
+                                                vec.data() + P.voff[d] + (size_t)(ia * (B.l + 1) + ib) * (N + 1), iv);
+                        } else {
+                            const double cen = (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum;
+                            P.cen[d] = cen;
+                            fac *= std::exp(-mu * (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]));
+                        }
+                    }
+                    P.fac = fac;
+                    P.eoff = (int)herm.size();
+                    herm.resize(herm.size() + (size_t)nreal * (A.l + 1) * (B.l + 1) * (2 * kMaxL + 1), 0.0);
+                    for (int dr = c->ndp; dr < 3; ++dr)
+                        for (int ia = 0; ia <= A.l; ++ia)
+                            for (int ib = 0; ib <= B.l; ++ib)
+                                hermite_E(ia, ib, esum, P.cen[dr] - A.xyz[dr], P.cen[dr] - B.xyz[dr],
+                                          herm.data() + P.eoff +
+                                              ((size_t)((dr - c->ndp) * (A.l + 1) + ia) * (B.l + 1) + ib) * (2 * kMaxL + 1));
+                    double ca = 0.0, cb = 0.0;
+                    for (int x = 0; x < A.ncomp; ++x) ca = std::max(ca, std::fabs(c->coef[c->poff[A.first_ao + x] + i]));
+                    for (int x = 0; x < B.ncomp; ++x) cb = std::max(cb, std::fabs(c->coef[c->poff[B.first_ao + x] + j]));
+                    P.cmax = ca * cb;
+                    P.ps = 1.0;
+                    c->pp.push_back(P);
+                }
+        }
+    if (vec.size() > (size_t)0x7fffffff) return c->fail(TCG_E_NOMEM, "pair vectors exceed 2^31 elements");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// device launch helper
+// ---------------------------------------------------------------------------
+static EriArgs base_args(tcg_ctx *c) {
+    EriArgs A{};
+    A.ndp = c->ndp;
+    A.nq = c->nq;
+    A.nqp = c->nqp;
+    A.ncap = c->ncap;
+    A.fac_const = 2.0 / std::sqrt(kPi);
+    for (int d = 0; d < 3; ++d) {
+        A.a[d] = c->a[d];
+        A.L[d] = c->L[d];
+    }
+    for (int d = 0; d < c->ndp; ++d) A.fac_const *= c->L[d] * c->L[d];
+    A.tab = c->d_tab.p;
+    A.tab_lim = c->d_tablim.p;
+    A.tnode = c->d_tnode.p;
+    A.tweight = c->d_tw.p;
+    A.sh = c->d_sh.p;
+    A.ao_lxyz = c->d_lxyz.p;
+    A.coef = c->d_coef.p;
+    A.ao_poff = c->d_poff.p;
+    A.pp = c->d_pp.p;
+    A.sp = c->d_sp.p;
+    A.vec = c->d_vec.p;
+    A.herm = c->d_herm.p;
+    A.screen_prefactor = 1;
+    A.screen_prim = 0;
+    A.max_ncomb = c->max_ncomb;
+    A.nbas = c->nbas;
+    A.nfuc = c->nfuc;
+    A.ncell = c->ncell;
+    A.relc = c->d_relc.p;
+    A.counters = c->d_cnt.p;
+    return A;
+}
+
+static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, std::vector<double> &out, int prefactor) {
+    HIPCHK(c, c->d_q4.upload(q4));
+    HIPCHK(c, c->d_blk.alloc(nout));
+    EriArgs A = base_args(c);
+    A.q4 = c->d_q4.p;
+    A.nq4 = (int)q4.size();
+    A.blk_out = c->d_blk.p;
+    A.screen_prefactor = prefactor;
+    HIPCHK(c, launch_eri_quartets(A, c->st));
+    out.resize(nout);
+    HIPCHK(c, hipMemcpyAsync(out.data(), c->d_blk.p, sizeof(double) * nout, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int tcg_create(int ndp, const double L[3], const int ncell[3], int nquad, tcg_ctx **out) {
+    if (!out) return TCG_E_ARG;
+    *out = nullptr;
+    if (ndp < 1 || ndp > 3 || !L || !ncell || nquad < 4 || nquad > kMaxNQ) return TCG_E_ARG;
+    for (int d = 0; d < ndp; ++d)
+        if (!(L[d] > 0.0)) return TCG_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TCG_E_DEVICE;
+    tcg_ctx *c = new tcg_ctx();
+    c->ndp = ndp;
+    for (int d = 0; d < 3; ++d) {
+        c->L[d] = L[d];
+        c->a[d] = (d < ndp) ? 2.0 * kPi / L[d] : 0.0;
+        c->ncl[d] = ncell[d] > 0 ? ncell[d] : 1;
+    }
+    c->ncell = c->ncl[0] * c->ncl[1] * c->ncl[2];
+    c->nq = nquad;
+    c->nqp = ((nquad + kNodeTile - 1) / kNodeTile) * kNodeTile;
+    if (hipStreamCreate(&c->st) != hipSuccess) {
+        delete c;
+        return TCG_E_DEVICE;
+    }
+    *out = c;
+    return TCG_OK;
+}
+
+int tcg_set_basis(tcg_ctx *c, int nbas, int nfuc, const double *center, const int *lxyz, const int *prim_off,
+                  const double *expo, const double *coef) {
+    if (!c) return TCG_E_ARG;
+    if (nbas <= 0 || nfuc <= 0 || !center || !lxyz || !prim_off || !expo || !coef)
+        return c->fail(TCG_E_ARG, "tcg_set_basis: null or empty argument");
+    if (nbas != nfuc * c->ncell) return c->fail(TCG_E_ARG, "nbas (%d) != nfuc (%d) * cells (%d)", nbas, nfuc, c->ncell);
+    if (prim_off[0] != 0) return c->fail(TCG_E_ARG, "prim_off[0] must be 0");
+    for (int i = 0; i < nbas; ++i)
+        if (prim_off[i + 1] <= prim_off[i]) return c->fail(TCG_E_ARG, "AO %d has no primitive", i);
+    const int np = prim_off[nbas];
+    c->nbas = nbas;
+    c->nfuc = nfuc;
+    c->center.assign(center, center + 3 * (size_t)nbas);
+    c->lxyz.assign(lxyz, lxyz + 3 * (size_t)nbas);
+    c->poff.assign(prim_off, prim_off + nbas + 1);
+    c->expo.assign(expo, expo + np);
+    c->coef.assign(coef, coef + np);
+    for (int p = 0; p < np; ++p)
+        if (!(expo[p] > 0.0)) return c->fail(TCG_E_ARG, "exponent %d is not positive", p);
+    return build_shells(c);
+}
+
+int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
+    if (!c) return TCG_E_ARG;
+    if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_build before tcg_set_basis");
+    if (!sao) return c->fail(TCG_E_ARG, "tcg_eri_build: this version builds the SAO layout ERI(nfuc,nBas,nBas,nBas) only");
+    const double t0 = now_ms();
+    // ---- 1-2: tables and pair data ----
+    std::vector<double> tab;
+    std::vector<int> tablim;
+    build_tables(c, tab, tablim);
+    std::vector<double2> vec;
+    std::vector<double> herm;
+    int rc = build_pairs(c, vec, herm);
+    if (rc) return rc;
+    c->max_ncomb = 1;
+    for (auto &s : c->sh) c->max_ncomb = std::max(c->max_ncomb, s.l + 1);
+    c->max_ncomb = c->max_ncomb * c->max_ncomb * c->max_ncomb * c->max_ncomb;
+    HIPCHK(c, c->d_tab.upload(tab));
+    HIPCHK(c, c->d_tablim.upload(tablim));
+    HIPCHK(c, c->d_tnode.upload(c->tnode));
+    HIPCHK(c, c->d_tw.upload(c->tweight));
+    HIPCHK(c, c->d_sh.upload(c->sh));
+    HIPCHK(c, c->d_lxyz.upload(c->lxyz));
+    HIPCHK(c, c->d_coef.upload(c->coef));
+    HIPCHK(c, c->d_poff.upload(c->poff));
+    HIPCHK(c, c->d_pp.upload(c->pp));
+    HIPCHK(c, c->d_sp.upload(c->sp));
+    HIPCHK(c, c->d_vec.upload(vec));
+    HIPCHK(c, c->d_herm.upload(herm));
+    HIPCHK(c, c->d_relc.upload(c->relc));
+    HIPCHK(c, c->d_cnt.alloc(4));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 4 * sizeof(unsigned long long), c->st));
+    const double t1 = now_ms();
+
+    // ---- 3: Schwarz from the diagonal blocks, replicated over translations ----
+    const int nsh = c->nsh, nshuc = c->nshuc, nb = c->nbas, nf = c->nfuc, nc = c->ncell;
+    std::vector<Quartet> q4;
+    std::vector<size_t> boff;
+    size_t nout = 0;
+    for (int SI = 0; SI < nshuc; ++SI)
+        for (int SJ = 0; SJ < nsh; ++SJ) {
+            Quartet q{};
+            q.s[0] = SI; q.s[1] = SJ; q.s[2] = SI; q.s[3] = SJ;
+            q.bra = q.ket = rep_pair(c, SI, SJ);
+            q.out = (int)nout;
+            q.prim = -1;
+            q4.push_back(q);
+            boff.push_back(nout);
+            const int m = c->sh[SI].ncomp * c->sh[SJ].ncomp;
+            nout += (size_t)m * m;
+        }
+    std::vector<double> blocks;
+    rc = run_blocks(c, q4, nout, blocks, 1);
+    if (rc) return rc;
+    std::vector<double> Q((size_t)nb * nb, 0.0), Qsh((size_t)nsh * nsh, 0.0);
+    for (size_t x = 0; x < q4.size(); ++x) {
+        const int SI = q4[x].s[0], SJ = q4[x].s[1];
+        const int nI = c->sh[SI].ncomp, nJ = c->sh[SJ].ncomp;
+        for (int a = 0; a < nI; ++a)
+            for (int b = 0; b < nJ; ++b) {
+                const double v = std::sqrt(std::fabs(blocks[boff[x] + a + nI * (b + nJ * (a + nI * b))]));
+                const int i = c->sh[SI].first_ao + a, j = c->sh[SJ].first_ao + b;
+                const int li = i % nf, cj = j / nf, lj = j % nf;
+                for (int ctr = 0; ctr < nc; ++ctr) {
+                    // translate (i in cell 0, j) by +ctr
+                    const int r1 = ctr * nf + li;
+                    const int r2 = c->relc[(size_t)c->relc[(size_t)ctr * nc] * nc + cj] * nf + lj;
+                    Q[(size_t)r1 + (size_t)nb * r2] = v;
+                    Q[(size_t)r2 + (size_t)nb * r1] = v;
+                }
+            }
+    }
+    for (int S1 = 0; S1 < nsh; ++S1)
+        for (int S2 = 0; S2 < nsh; ++S2) {
+            double m = 0.0;
+            for (int a = 0; a < c->sh[S1].ncomp; ++a)
+                for (int b = 0; b < c->sh[S2].ncomp; ++b)
+                    m = std::max(m, Q[(size_t)(c->sh[S1].first_ao + a) + (size_t)nb * (c->sh[S2].first_ao + b)]);
+            Qsh[(size_t)S1 + (size_t)nsh * S2] = m;
+        }
+    double Qmax = 0.0;
+    for (double v : Qsh) Qmax = std::max(Qmax, v);
+
+    // ---- 4: primitive Schwarz table, kernel prefactor screens off ----
+    q4.clear();
+    boff.clear();
+    nout = 0;
+    std::vector<int> pidx;
+    for (int SI = 0; SI < nshuc; ++SI)
+        for (int SJ = 0; SJ < nsh; ++SJ) {
+            const int rp = rep_pair(c, SI, SJ);
+            const ShellPair &s = c->sp[rp];
+            for (int p = 0; p < s.count; ++p) {
+                Quartet q{};
+                q.s[0] = SI; q.s[1] = SJ; q.s[2] = SI; q.s[3] = SJ;
+                q.bra = q.ket = rp;
+                q.out = (int)nout;
+                q.prim = p;
+                q4.push_back(q);
+                boff.push_back(nout);
+                pidx.push_back(s.first + p);
+                const int m = c->sh[SI].ncomp * c->sh[SJ].ncomp;
+                nout += (size_t)m * m;
+            }
+        }
+    rc = run_blocks(c, q4, nout, blocks, 0);
+    if (rc) return rc;
+    for (size_t x = 0; x < q4.size(); ++x) {
+        const int nI = c->sh[q4[x].s[0]].ncomp, nJ = c->sh[q4[x].s[1]].ncomp;
+        double m = 0.0;
+        for (int a = 0; a < nI; ++a)
+            for (int b = 0; b < nJ; ++b) m = std::max(m, std::sqrt(std::fabs(blocks[boff[x] + a + nI * (b + nJ * (a + nI * b))])));
+        c->pp[pidx[x]].ps = m;
+    }
+    HIPCHK(c, c->d_pp.upload(c->pp));
+    HIPCHK(c, c->d_Q.upload(Q));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 4 * sizeof(unsigned long long), c->st));
+    const double t2 = now_ms();
+
+    // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened ----
+    q4.clear();
+    static const int perm8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                    {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
+    for (int SI = 0; SI < nshuc; ++SI)
+        for (int SJ = 0; SJ < nsh; ++SJ) {
+            if (c->sh[SJ].loc < SI) continue;
+            const double qij = Qsh[(size_t)SI + (size_t)nsh * SJ];
+            if (qij * Qmax < kSchwarz) continue;
+            for (int SK = 0; SK < nsh; ++SK) {
+                if (c->sh[SK].loc < SI) continue;
+                for (int SL = 0; SL < nsh; ++SL) {
+                    if (c->sh[SL].loc < SI) continue;
+                    if (qij * Qsh[(size_t)SK + (size_t)nsh * SL] < kSchwarz) continue;
+                    const int q[4] = {SI, SJ, SK, SL};
+                    bool smaller = false;
+                    for (int ip = 1; ip < 8 && !smaller; ++ip) {
+                        int t[4], rr[4];
+                        for (int x = 0; x < 4; ++x) t[x] = q[perm8[ip][x]];
+                        const int c0 = c->sh[t[0]].cell;
+                        for (int x = 0; x < 4; ++x)
+                            rr[x] = c->relc[(size_t)c0 * nc + c->sh[t[x]].cell] * nshuc + c->sh[t[x]].loc;
+                        for (int x = 0; x < 4; ++x) {
+                            if (rr[x] < q[x]) { smaller = true; break; }
+                            if (rr[x] > q[x]) break;
+                        }
+                    }
+                    if (smaller) continue;
+                    Quartet qq{};
+                    for (int x = 0; x < 4; ++x) qq.s[x] = q[x];
+                    qq.bra = rep_pair(c, SI, SJ);
+                    qq.ket = rep_pair(c, SK, SL);
+                    const Shell &rk = c->sh[c->sh[SK].loc];   // representative of SK in the unit cell
+                    for (int d = 0; d < 3; ++d) qq.shift[d] = c->sh[SK].xyz[d] - rk.xyz[d];
+                    qq.out = -1;
+                    qq.prim = -1;
+                    q4.push_back(qq);
+                }
+            }
+        }
+    // heavy quartets first (longest Fourier vectors x most primitive pairs)
+    std::vector<std::pair<float, int>> order(q4.size());
+    for (size_t x = 0; x < q4.size(); ++x) {
+        const ShellPair &b = c->sp[q4[x].bra], &k = c->sp[q4[x].ket];
+        float cost = (float)b.count * k.count;
+        int nmax = 0;
+        for (int p = 0; p < b.count; ++p) nmax = std::max(nmax, c->pp[b.first + p].N[0]);
+        cost *= (float)(nmax + 16);
+        order[x] = {-cost, (int)x};
+    }
+    std::sort(order.begin(), order.end());
+    std::vector<Quartet> sorted(q4.size());
+    for (size_t x = 0; x < q4.size(); ++x) sorted[x] = q4[order[x].second];
+    c->stats[0] = (long long)sorted.size();
+    const double t3 = now_ms();
+
+    // ---- 6: quartets on the GPU, scattered into the ERI array ----
+    const int n1 = sao ? nf : nb;
+    const size_t count = (size_t)n1 * nb * nb * nb;
+    HIPCHK(c, c->d_eri.alloc(count));
+    HIPCHK(c, launch_fill(c->d_eri.p, count, 0.0, c->st));
+    HIPCHK(c, c->d_q4.upload(sorted));
+    EriArgs A = base_args(c);
+    A.q4 = c->d_q4.p;
+    A.nq4 = (int)sorted.size();
+    A.screen_prim = 1;
+    A.eri = c->d_eri.p;
+    A.n1 = n1;
+    A.physicist = physicist;
+    A.Qao = c->d_Q.p;
+    HIPCHK(c, launch_eri_quartets(A, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const double t4 = now_ms();
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpy(cnt, c->d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost));
+    c->stats[1] = (long long)cnt[0];
+    c->stats[2] = (long long)cnt[1];
+    c->eri_count = count;
+    c->eri_n1 = n1;
+    if (eri) HIPCHK(c, hipMemcpy(eri, c->d_eri.p, sizeof(double) * count, hipMemcpyDeviceToHost));
+    c->times[0] = t1 - t0;
+    c->times[1] = t2 - t1;
+    c->times[2] = t3 - t2;
+    c->times[3] = t4 - t3;
+    c->times[4] = now_ms() - t0;
+    return TCG_OK;
+}
+
+int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *eri, double *J, double *K) {
+    if (!c) return TCG_E_ARG;
+    if (!P || !J || !K || nbas <= 0 || n1 <= 0 || n1 > nbas) return c->fail(TCG_E_ARG, "tcg_jk_stored: bad arguments");
+    const size_t count = (size_t)n1 * nbas * nbas * nbas;
+    const double *src = c->d_eri.p;
+    if (eri) {
+        HIPCHK(c, c->d_eri.alloc(count));
+        HIPCHK(c, hipMemcpy(c->d_eri.p, eri, sizeof(double) * count, hipMemcpyHostToDevice));
+        c->eri_count = count;
+        c->eri_n1 = n1;
+        src = c->d_eri.p;
+    } else if (!src || c->eri_count != count || c->eri_n1 != n1) {
+        return c->fail(TCG_E_STATE, "tcg_jk_stored: no resident ERI of this shape (call tcg_eri_build)");
+    }
+    const size_t nn = (size_t)nbas * nbas;
+    HIPCHK(c, c->d_P.alloc(nn));
+    HIPCHK(c, c->d_J.alloc(nn));
+    HIPCHK(c, c->d_K.alloc(nn));
+    HIPCHK(c, c->d_jwork.alloc(jk_work_doubles(n1, nbas)));
+    HIPCHK(c, hipMemcpyAsync(c->d_P.p, P, sizeof(double) * nn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_fill(c->d_J.p, nn, 0.0, c->st));
+    HIPCHK(c, launch_fill(c->d_K.p, nn, 0.0, c->st));
+    HIPCHK(c, launch_jk_stored(src, c->d_P.p, c->d_J.p, c->d_K.p, n1, nbas, c->d_jwork.p, c->st));
+    HIPCHK(c, hipMemcpyAsync(J, c->d_J.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(K, c->d_K.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return TCG_OK;
+}
+
+int tcg_eri_stats(tcg_ctx *c, long long *out, int n) {
+    if (!c || !out) return TCG_E_ARG;
+    for (int i = 0; i < n && i < 8; ++i) out[i] = c->stats[i];
+    return TCG_OK;
+}
+
+int tcg_eri_times(tcg_ctx *c, double *out, int n) {
+    if (!c || !out) return TCG_E_ARG;
+    for (int i = 0; i < n && i < 8; ++i) out[i] = c->times[i];
+    return TCG_OK;
+}
+
+int tcg_eri_device(tcg_ctx *c, const double **dptr, long long *count) {
+    if (!c || !dptr || !count) return TCG_E_ARG;
+    *dptr = c->d_eri.p;
+    *count = (long long)c->eri_count;
+    return TCG_OK;
+}
+
+const char *tcg_last_error(tcg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int tcg_destroy(tcg_ctx *c) {
+    if (!c) return TCG_E_ARG;
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return TCG_OK;
+}
+
+}  // extern "C"

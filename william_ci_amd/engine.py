@@ -1,0 +1,134 @@
+"""Python binding of the engine's C ABI (include/tcg.h), with the names and
+argument meaning of the reference's Fortran entry points it replaces:
+
+  ERI_integral_toroidal       src/Toroidal/ERI_toroidal.f90:36
+  hartree_exchange_potential  src/Energies/hartree_exchange_potential.f90:30
+
+The engine is the HIP library william_ci_amd/libtcg.so.  There is no CPU
+fallback: if the library or a GPU is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtcg.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_ci = ctypes.c_int
+_vp = ctypes.c_void_p
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(f"engine library missing: {LIB_PATH} (build with python -m william_ci_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.tcg_create.argtypes = [_ci, _dp, _ip, _ci, ctypes.POINTER(_vp)]
+        L.tcg_set_basis.argtypes = [_vp, _ci, _ci, _dp, _ip, _ip, _dp, _dp]
+        L.tcg_eri_build.argtypes = [_vp, _ci, _ci, _vp]
+        L.tcg_jk_stored.argtypes = [_vp, _ci, _ci, _dp, _vp, _dp, _dp]
+        L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
+        L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
+        L.tcg_eri_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_longlong)]
+        L.tcg_last_error.argtypes = [_vp]
+        L.tcg_last_error.restype = ctypes.c_char_p
+        L.tcg_destroy.argtypes = [_vp]
+        _lib = L
+    return _lib
+
+
+class Engine:
+    """One torus system on the GPU (a tcg_ctx)."""
+
+    def __init__(self, sysm, coef: np.ndarray, nquad: int = 32):
+        self.s = sysm
+        L = lib()
+        h = _vp()
+        rc = L.tcg_create(sysm.ndp, np.array(sysm.L, dtype=np.float64), np.array(sysm.ncell, dtype=np.int32),
+                          nquad, ctypes.byref(h))
+        if rc != 0 or not h.value:
+            raise EngineError(f"tcg_create failed ({rc}): no HIP device or bad torus")
+        self.h = h
+        self._check(L.tcg_set_basis(h, sysm.nbas, sysm.nfuc, np.ascontiguousarray(sysm.center).ravel(),
+                                    np.ascontiguousarray(sysm.lxyz, dtype=np.int32).ravel(),
+                                    np.ascontiguousarray(sysm.prim_off, dtype=np.int32),
+                                    np.ascontiguousarray(sysm.expo), np.ascontiguousarray(coef, dtype=np.float64)),
+                    "tcg_set_basis")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().tcg_last_error(self.h).decode()
+            raise EngineError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().tcg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- ERI_integral_toroidal -------------------------------------------------
+    def eri_build(self, sao: bool = True, physicist: bool = False, to_host: bool = True):
+        """Builds the ERI; returns it as the reference stores it (Fortran
+        ERI(n1,nBas,nBas,nBas)) viewed C-ordered as E[l,k,j,i] = ERI(i,j,k,l),
+        or None when to_host is False (the tensor then stays on the GPU only)."""
+        nb = self.s.nbas
+        n1 = self.s.nfuc if sao else nb
+        out = np.empty(n1 * nb * nb * nb) if to_host else None
+        self._check(lib().tcg_eri_build(self.h, int(sao), int(physicist),
+                                        out.ctypes.data if to_host else None), "tcg_eri_build")
+        return out.reshape(nb, nb, nb, n1) if to_host else None
+
+    def stats(self):
+        out = np.zeros(8, dtype=np.int64)
+        lib().tcg_eri_stats(self.h, out, 8)
+        return {"shell_quartets": int(out[0]), "prim_quartets": int(out[1]), "rows": int(out[2])}
+
+    def times(self):
+        out = np.zeros(8)
+        lib().tcg_eri_times(self.h, out, 8)
+        return {"tables_pairs_ms": out[0], "schwarz_ms": out[1], "quartet_list_ms": out[2],
+                "quartets_ms": out[3], "total_ms": out[4]}
+
+    def eri_device(self):
+        p = _vp()
+        n = ctypes.c_longlong()
+        lib().tcg_eri_device(self.h, ctypes.byref(p), ctypes.byref(n))
+        return p.value, n.value
+
+    # -- hartree_exchange_potential ---------------------------------------------
+    def jk_stored(self, P: np.ndarray, n1: int | None = None, eri: np.ndarray | None = None):
+        """J, K as ordinary row-major numpy matrices (rows > n1 zero), from the
+        resident ERI (eri None) or from a host ERI in the reference layout."""
+        nb = self.s.nbas
+        n1 = self.s.nfuc if n1 is None else n1
+        Pf = np.ascontiguousarray(np.asarray(P, dtype=np.float64).T)
+        J = np.zeros(nb * nb)
+        K = np.zeros(nb * nb)
+        ep = None if eri is None else np.ascontiguousarray(eri).ctypes.data
+        self._check(lib().tcg_jk_stored(self.h, n1, nb, Pf.ravel(), ep, J, K), "tcg_jk_stored")
+        return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
+
+
+def ERI_integral_toroidal(sysm, coef, physicist: bool = False, sao: bool = True, nquad: int = 32):
+    """ERI(n1,nBas,nBas,nBas) of the normalised AO list (reference layout, as E[l,k,j,i])."""
+    eng = Engine(sysm, coef, nquad)
+    try:
+        return eng.eri_build(sao=sao, physicist=physicist)
+    finally:
+        eng.close()
