@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X toroidal ERI engine on one GPU.
+
+One step = one complete two-electron integral build of the workload (t-tables,
+pair data, Schwarz and primitive-Schwarz passes, unique-quartet list, quartet
+kernel, scatter into the SAO layout ERI(nfuc,nBas,nBas,nBas) resident in HBM),
+i.e. the work of the reference's ERI_integral_toroidal
+(src/Toroidal/ERI_toroidal.f90:36) for the Tori1D H64 chain (320 functions).
+
+Prints one JSON line.  ``value`` = nBas^4/8-equivalent unique quartets
+delivered per second (the unit of the workload's "1.3e9 unique quartets");
+the evaluated shell quartets per second are reported beside it.
+``roofline`` prices the quartet kernel's algorithmic FP64 flops (counted by
+the kernel itself) against MI355X's FP64 peak, with the kernel's duration
+measured by HIP events on the engine's stream.  ``cpu_baseline`` times the
+reference's own Fortran ERI driver (oracle/_ref, OpenMP) on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def _traffic_from_profiles(workload: str):
+    """HBM bytes per quartet-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", f"eri_quartets_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(sysm, coef_unused, seconds: float):
+    """The reference's own ERI driver (Fortran + OpenMP, oracle/_ref) on a
+    bounded sample: repeated full builds of the same workload for ~seconds."""
+    try:
+        from oracle import ref
+    except Exception:
+        ref = None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if ref is not None and ref.available():
+        r = ref.RefSystem(sysm)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            r.eri_full()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or el / n * (n + 1) > 3 * seconds:
+                break
+        nb = sysm.nbas
+        return {"value": n * nb ** 4 / 8 / el, "unit": "unique quartets/s (nBas^4/8 per build)",
+                "cores": threads, "kind": "reference",
+                "sample": f"{n} full ERI builds of {sysm.kind} {sysm.nbas}-function workload "
+                          f"(reference Fortran ERI_integral_toroidal, OpenMP, {el:.1f} s)",
+                "seconds_per_build": el / n}
+    from oracle import orc
+    from william_ci_amd.basis import normalize
+    o = orc.OracleSystem(sysm, normalize(sysm))
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.eri_full()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    nb = sysm.nbas
+    return {"value": n * nb ** 4 / 8 / el, "unit": "unique quartets/s (nBas^4/8 per build)",
+            "cores": threads, "kind": "port",
+            "sample": f"{n} full ERI builds (C restatement of the reference driver, OpenMP, {el:.1f} s)",
+            "seconds_per_build": el / n}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="h64_chain")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = (1 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 3}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is described in DESIGN.md but not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    from william_ci_amd import system as tsys
+    from william_ci_amd.basis import normalize
+    from william_ci_amd.engine import Engine
+
+    s = tsys.workload(args.workload)
+    coef = normalize(s)
+    eng = Engine(s, coef)
+    for _ in range(args.warmup):
+        eng.refresh_basis(coef)
+        eng.eri_build(to_host=False)
+    kms, steps = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        eng.refresh_basis(coef)               # every step rebuilds tables and pair data from the AO list
+        eng.eri_build(to_host=False)          # synchronous: returns after the GPU work completed
+        steps.append(time.perf_counter() - ts)
+        kms.append(eng.times()["kernel_ms"])
+    el = time.perf_counter() - t0
+    st = eng.stats()
+    tm = eng.times()
+
+    # the energy path on the same workload: one-electron integrals + RHF (Bloch
+    # basis, J/K from the resident ERI on the GPU); the ERI build is the last step's
+    from william_ci_amd import scf
+    t_rhf = time.perf_counter()
+    S, T, V = eng.one_electron()
+    t_one = time.perf_counter() - t_rhf
+    res = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
+    rhf_wall = time.perf_counter() - t_rhf
+    eng.close()
+
+    nb = s.nbas
+    unique = nb ** 4 / 8
+    ms_per_step = el / args.steps * 1e3
+    kernel_ms = sum(kms) / len(kms)
+    achieved = st["flops"] / (kernel_ms * 1e-3) / 1e12
+    traffic = _traffic_from_profiles(args.workload)
+    line = {
+        "metric": "Clifford ERI quartets/s (FP64 ERI build)",
+        "value": unique * args.steps / el,
+        "unit": "unique quartets/s (nBas^4/8 per build)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (deterministic H chain, no external data)",
+        "config": {"workload": args.workload, "kind": s.kind, "nbas": nb, "nfuc": s.nfuc,
+                   "cells": s.n_cell, "L": s.L[0], "nquad": 32, "layout": "SAO ERI(nfuc,nBas^3)",
+                   "parallelism": "single GPU"},
+        "shell_quartets_per_s": st["shell_quartets"] * args.steps / el,
+        "evaluated": {"shell_quartets": st["shell_quartets"], "prim_quartets": st["prim_quartets"],
+                      "rows": st["rows"]},
+        "step_ms": {k: round(v, 3) for k, v in tm.items()},
+        "rhf": {"energy": res.energy, "converged": res.converged, "iterations": res.iterations,
+                "orbital_error": res.orbital_error, "wall_s_after_eri": rhf_wall, "one_electron_s": t_one,
+                "jk_s": res.times["jk_s"], "eri_plus_rhf_wall_s": rhf_wall + ms_per_step / 1e3},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "k_eri_quartets", "kernel_ms": kernel_ms, "flops_per_launch": st["flops"]},
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(s, coef, args.cpu_seconds)
+        cb["gpu_speedup"] = line["value"] / cb["value"]
+        line["cpu_baseline"] = cb
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

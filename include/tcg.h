@@ -65,18 +65,28 @@ int tcg_eri_build(tcg_ctx *ctx, int sao, int physicist, double *eri);
  * the caller's ERI(n1,nBas,nBas,nBas) (uploaded for this call). */
 int tcg_jk_stored(tcg_ctx *ctx, int n1, int nbas, const double *P, const double *eri, double *J, double *K);
 
+/* One-electron matrices S, T, V (nBas,nBas): rows of the unit cell computed,
+ * values below 1e-14 zeroed, the other rows filled by translation.  Replaces
+ *   overlap_matrix_toroidal_nD            e.g. src/Toroidal/1D/1D_Overlap_toroidal.f90:23
+ *   kinetic_matrix_toroidal_nD            e.g. src/Toroidal/1D/1D_Kinetic_toroidal.f90:102
+ *   nuclear_attraction_matrix_toroidal_nD e.g. src/Toroidal/1D/1D_Nuclear_attraction_toroidal.f90:169
+ * xyz[3*c+d] nuclear positions, Z[c] charges (the supercell's atoms). */
+int tcg_one_electron(tcg_ctx *ctx, int natom, const double *xyz, const int *Z, double *S, double *T, double *V);
+
 /* Counters of the last tcg_eri_build:
  *   out[0] shell quartets evaluated   out[1] primitive quartets evaluated
  *   out[2] rows (primitive quartet x direction x power combination)
- *   out[3] algorithmic FP64 flops of the quadrature contraction
- *   out[4] nonzero elements stored (n1 * nBas^3 counted over the array)
+ *   out[3] algorithmic FP64 flops of the quartet kernel: per primitive quartet
+ *          ncomb*[sum_{d periodic}(2 sum_k min(nm_d, t_nmax(k,d)) + 10(nm_d+1))
+ *                 + sum_{d real} 2 nq (NN+1)] + 3 nq ncomp4
  * n = number of slots the caller provides (up to 8). */
 int tcg_eri_stats(tcg_ctx *ctx, long long *out, int n);
 
 /* Wall times in milliseconds of the last tcg_eri_build sub-steps, the
  * analogue of the reference's tstep table (src/module/clifford_tools.f90:816-852):
  *   out[0] tables + pairs   out[1] Schwarz + primitive screen
- *   out[2] quartet list     out[3] quartet kernel + scatter   out[4] total */
+ *   out[2] quartet list     out[3] quartet kernel + scatter   out[4] total
+ *   out[5] duration of the quartet kernel alone (HIP events on the engine's stream) */
 int tcg_eri_times(tcg_ctx *ctx, double *out, int n);
 
 /* Device address of the resident ERI tensor (for callers that keep data on

@@ -14,6 +14,7 @@ struct EriArgs {
     double fac_const;                 // 2/sqrt(pi) prod_d L_d^2
     const double *tab;                // [d][ncap+1+4][nqp] c_n exp(-C)I_n(C), 0 beyond t_nmax(k)
     const int *tab_lim;               // [d][nqp/16]: last n with a nonzero entry in that node tile
+    const int *tnmax;                 // [d][nqp]: truncation order of each node (-1 in the padding)
     const double *tnode, *tweight;    // [nqp] (0 in the padding)
     const Shell *sh;
     const int *ao_lxyz;               // [nbas][3]
@@ -36,12 +37,14 @@ struct EriArgs {
     const double *Qao;                // [nbas*nbas] function-pair Schwarz values
     const int *relc;                  // [ncell*ncell]
     // counters
-    unsigned long long *counters;     // [0] primitive quartets, [1] rows
+    unsigned long long *counters;     // [0] primitive quartets, [1] rows, [2] algorithmic flops
 };
 
 size_t eri_lds_bytes(const EriArgs &a);
 hipError_t launch_eri_quartets(const EriArgs &a, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
+hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
+                          const int *pair_rep, double *V, hipStream_t st);
 size_t jk_work_doubles(int n1, int nbas);
 hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,
                             double *jwork, hipStream_t st);

@@ -146,6 +146,20 @@ __global__ __launch_bounds__(64) void k_eri_quartets(EriArgs A) {
                             msgn[r] = sgn;
                             mdest[r] = valid ? gl * ncomb + combo : -1;
                         }
+                        if (A.counters) {
+                            // algorithmic flops of this row: the truncated node sums + the F products
+                            unsigned long long fl = 0;
+                            if (kq == 0 && valid) {
+                                long s = 0;
+                                for (int k = 0; k < nq; ++k) {
+                                    const int tk = A.tnmax[d * nqp + k];
+                                    s += (nm < tk ? nm : tk) + 1;
+                                }
+                                fl = (unsigned long long)(2 * s + 10 * (nm + 1));
+                            }
+                            for (int off = 32; off > 0; off >>= 1) fl += __shfl_xor(fl, off, 64);
+                            if (lane == 0) atomicAdd(&A.counters[2], fl);
+                        }
                         const int nmax_t = wave_max_int(nm);
                         int lim[kMaxNodeTiles];
                         int nend = -1;
@@ -248,6 +262,10 @@ __global__ __launch_bounds__(64) void k_eri_quartets(EriArgs A) {
                     }
                 }
                 __syncthreads();
+                if (A.counters && lane == 0)
+                    atomicAdd(&A.counters[2], (unsigned long long)ng *
+                                                  ((unsigned long long)ncomb * (3 - ndp) * 2 * nq * (NN + 1) +
+                                                   3ull * nq * nblk));
                 // ---------------- component contraction into the block ----------------
                 for (int task = lane; task < ng * nblk; task += 64) {
                     const int gl = task / nblk, comp = task % nblk;

@@ -85,7 +85,8 @@ struct tcg_ctx {
     // device
     hipStream_t st = nullptr;
     DevBuf<double> d_tab, d_tnode, d_tw, d_coef, d_herm, d_blk, d_Q, d_eri, d_P, d_J, d_K, d_jwork;
-    DevBuf<int> d_tablim, d_lxyz, d_poff, d_relc;
+    DevBuf<int> d_tablim, d_tnmax, d_lxyz, d_poff, d_relc;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf<Shell> d_sh;
     DevBuf<PrimPair> d_pp;
     DevBuf<ShellPair> d_sp;
@@ -97,6 +98,7 @@ struct tcg_ctx {
     long long stats[8] = {0};
     double times[8] = {0};
     int max_ncomb = 1;
+    bool prepared = false;
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -120,7 +122,7 @@ static int lsum3(const int *l) { return l[0] + l[1] + l[2]; }
 // ---------------------------------------------------------------------------
 // 1. t-tables (src/module/clifford_tools.f90:620-709 with ncap from ERI_toroidal.f90:94-104)
 // ---------------------------------------------------------------------------
-static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim) {
+static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim, std::vector<int> &tnmax) {
     int lmax = 0;
     double emax = 0.0, amin = 1e300;
     for (int i = 0; i < c->nbas; ++i) {
@@ -136,6 +138,7 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
     tab.assign((size_t)c->ndp * rows * nqp, 0.0);
     const int nh = nqp / kNodeTile;
     tablim.assign((size_t)c->ndp * nh, -1);
+    tnmax.assign((size_t)c->ndp * nqp, -1);
     std::vector<double> col(ncap + 1);
     for (int d = 0; d < c->ndp; ++d) {
         for (int k = 0; k < c->nq; ++k) {
@@ -146,6 +149,7 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
             for (int n = 0; n <= ncap; ++n)
                 if (col[n] < 1e-16 * col[0]) { nmax = std::min(ncap, n + 5); break; }
             for (int n = 0; n <= nmax; ++n) tab[((size_t)d * rows + n) * nqp + k] = (n == 0 ? 1.0 : 2.0) * col[n];
+            tnmax[(size_t)d * nqp + k] = nmax;
             int &lim = tablim[d * nh + k / kNodeTile];
             lim = std::max(lim, nmax);
         }
@@ -317,6 +321,7 @@ static EriArgs base_args(tcg_ctx *c) {
     for (int d = 0; d < c->ndp; ++d) A.fac_const *= c->L[d] * c->L[d];
     A.tab = c->d_tab.p;
     A.tab_lim = c->d_tablim.p;
+    A.tnmax = c->d_tnmax.p;
     A.tnode = c->d_tnode.p;
     A.tweight = c->d_tw.p;
     A.sh = c->d_sh.p;
@@ -351,6 +356,149 @@ static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, s
     HIPCHK(c, hipMemcpyAsync(out.data(), c->d_blk.p, sizeof(double) * nout, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
+}
+
+// tables and pair data (steps 1-2), uploaded once per basis
+static int prepare(tcg_ctx *c) {
+    if (c->prepared) return 0;
+    // ---- 1-2: tables and pair data ----
+    std::vector<double> tab;
+    std::vector<int> tablim, tnmax;
+    build_tables(c, tab, tablim, tnmax);
+    std::vector<double2> vec;
+    std::vector<double> herm;
+    int rc = build_pairs(c, vec, herm);
+    if (rc) return rc;
+    c->max_ncomb = 1;
+    for (auto &s : c->sh) c->max_ncomb = std::max(c->max_ncomb, s.l + 1);
+    c->max_ncomb = c->max_ncomb * c->max_ncomb * c->max_ncomb * c->max_ncomb;
+    HIPCHK(c, c->d_tab.upload(tab));
+    HIPCHK(c, c->d_tablim.upload(tablim));
+    HIPCHK(c, c->d_tnmax.upload(tnmax));
+    HIPCHK(c, c->d_tnode.upload(c->tnode));
+    HIPCHK(c, c->d_tw.upload(c->tweight));
+    HIPCHK(c, c->d_sh.upload(c->sh));
+    HIPCHK(c, c->d_lxyz.upload(c->lxyz));
+    HIPCHK(c, c->d_coef.upload(c->coef));
+    HIPCHK(c, c->d_poff.upload(c->poff));
+    HIPCHK(c, c->d_pp.upload(c->pp));
+    HIPCHK(c, c->d_sp.upload(c->sp));
+    HIPCHK(c, c->d_vec.upload(vec));
+    HIPCHK(c, c->d_herm.upload(herm));
+    HIPCHK(c, c->d_relc.upload(c->relc));
+    HIPCHK(c, c->d_cnt.alloc(4));
+    c->prepared = true;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// One-electron integrals (src/Toroidal/{1D,2D,3D}/*_toroidal.f90).
+// Periodic direction, pair of Clifford primitives with barycentre P,
+// u = a(x - P):  g_A^i g_B^l = kx a^-(i+l) sin^i(u+phA) sin^l(u+phB) e^{A(cos u - 1)}
+// and  Int_0^L e^{iku} e^{A(cos u - 1)} dx = L exp(-A) I_|k|(A), so every
+// overlap-type factor is L a^-(i+l) sum_k d_k exp(-A)I_|k|(A) with d_k the
+// Fourier coefficients of the trigonometric factor.  The kinetic factor uses
+//   d2 g^l/dx2 = l(l-1) g^(l-2) - l^2 a^2 g^l - 2 beta (2l+1) cos(a(x-B)) g^l + 4 beta^2 g^(l+2)
+// (src/module/clifford_tools.f90:295-330).  Real directions: Hermite E_0
+// overlap and m(m-1)S(m-2) - 2beta(2m+1)S(m) + 4beta^2 S(m+2).
+// ---------------------------------------------------------------------------
+static double clifford_overlap_factor(int i, int l, double A, double phA, double phB, double a, double L,
+                                      const std::vector<double> &iv, bool with_cos) {
+    if (l < 0) return 0.0;
+    const int K = i + l;
+    std::complex<double> d[4 * kMaxL + 8];
+    sin_product_fourier(i, l, phA, phB, d);
+    std::complex<double> acc = 0.0;
+    if (!with_cos) {
+        for (int k = -K; k <= K; ++k) acc += d[k + K] * iv[std::abs(k)];
+    } else {
+        // times cos(u + phB) = (e^{i phB} e^{iu} + e^{-i phB} e^{-iu})/2
+        const std::complex<double> ep(std::cos(phB), std::sin(phB));
+        for (int k = -K; k <= K; ++k) {
+            acc += 0.5 * ep * d[k + K] * iv[std::abs(k + 1)];
+            acc += 0.5 * std::conj(ep) * d[k + K] * iv[std::abs(k - 1)];
+        }
+    }
+    return acc.real() * L * std::pow(a, -(double)(i + l));
+}
+
+static double real_overlap_factor(int j, int m, double p, double PA, double PB) {
+    if (m < 0) return 0.0;
+    double E[2 * 8 + 3];
+    hermite_E(j, m, p, PA, PB, E);
+    return E[0] * std::sqrt(kPi / p);
+}
+
+static int one_electron_rows(tcg_ctx *c, std::vector<double> &S, std::vector<double> &T) {
+    const int nf = c->nfuc, nb = c->nbas;
+    std::vector<int> ao_shell(nb);
+    for (int sI = 0; sI < c->nsh; ++sI)
+        for (int x = 0; x < c->sh[sI].ncomp; ++x) ao_shell[c->sh[sI].first_ao + x] = sI;
+    std::vector<double> iv;
+    for (int i = 0; i < nf; ++i)
+        for (int j = 0; j < nb; ++j) {
+            const Shell &A = c->sh[ao_shell[i]], &B = c->sh[ao_shell[j]];
+            const int *li = &c->lxyz[3 * i], *lj = &c->lxyz[3 * j];
+            double sv = 0.0, tv = 0.0;
+            for (int pi = 0; pi < A.nprim; ++pi)
+                for (int pj = 0; pj < B.nprim; ++pj) {
+                    const double e1 = c->expo[A.poff + pi], e2 = c->expo[B.poff + pj];
+                    const double esum = e1 + e2, mu = e1 * e2 / esum;
+                    const double cc = c->coef[c->poff[i] + pi] * c->coef[c->poff[j] + pj];
+                    double Sd[3], Dd[3], pref = 1.0;
+                    for (int d = 0; d < 3; ++d) {
+                        const int l1 = li[d], l2 = lj[d];
+                        if (d < c->ndp) {
+                            double pe, cen;
+                            const double a = c->a[d], L = c->L[d];
+                            clifford_pair(e1, e2, A.xyz[d], B.xyz[d], L, a, &pe, &cen);
+                            pref *= std::exp(-2.0 * (esum - pe) / (a * a));
+                            const double Aarg = 2.0 * pe / (a * a);
+                            const int nmax = l1 + l2 + 3;
+                            iv.resize(nmax + 1);
+                            bessel_miller(nmax, Aarg, nmax + 30 + (int)std::sqrt(80.0 * Aarg), iv.data());
+                            const double phA = a * (cen - A.xyz[d]), phB = a * (cen - B.xyz[d]);
+                            Sd[d] = clifford_overlap_factor(l1, l2, Aarg, phA, phB, a, L, iv, false);
+                            double D = 0.0;
+                            if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * clifford_overlap_factor(l1, l2 - 2, Aarg, phA, phB, a, L, iv, false);
+                            if (l2 >= 1) D -= (double)(l2 * l2) * a * a * Sd[d];
+                            // cos(a(x - B)) g_B^l: the l+0 factor times cos(u + phB), with a^-(i+l) as for S(i,l)
+                            D -= 2.0 * e2 * (2 * l2 + 1) * clifford_overlap_factor(l1, l2, Aarg, phA, phB, a, L, iv, true);
+                            D += 4.0 * e2 * e2 * clifford_overlap_factor(l1, l2 + 2, Aarg, phA, phB, a, L, iv, false);
+                            Dd[d] = D;
+                        } else {
+                            const double P = (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum;
+                            pref *= std::exp(-mu * (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]));
+                            const double PA = P - A.xyz[d], PB = P - B.xyz[d];
+                            Sd[d] = real_overlap_factor(l1, l2, esum, PA, PB);
+                            double D = -2.0 * e2 * (2 * l2 + 1) * Sd[d] + 4.0 * e2 * e2 * real_overlap_factor(l1, l2 + 2, esum, PA, PB);
+                            if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * real_overlap_factor(l1, l2 - 2, esum, PA, PB);
+                            Dd[d] = D;
+                        }
+                    }
+                    sv += cc * pref * Sd[0] * Sd[1] * Sd[2];
+                    tv -= 0.5 * cc * pref * (Dd[0] * Sd[1] * Sd[2] + Sd[0] * Dd[1] * Sd[2] + Sd[0] * Sd[1] * Dd[2]);
+                }
+            S[(size_t)i + (size_t)nb * j] = sv;
+            T[(size_t)i + (size_t)nb * j] = tv;
+        }
+    return 0;
+}
+
+// rows of the other cells from the unit cell's (src/Tools/symmetric_matrix.f90:1-52)
+static void fill_translation(const tcg_ctx *c, double *M) {
+    const int nf = c->nfuc, nb = c->nbas, nc = c->ncell;
+    for (int i = 0; i < nf; ++i)
+        for (int j = 0; j < nb; ++j)
+            if (std::fabs(M[(size_t)i + (size_t)nb * j]) < 1e-14) M[(size_t)i + (size_t)nb * j] = 0.0;
+    for (int i = nf; i < nb; ++i) {
+        const int ci = i / nf, a = i % nf;
+        for (int j = 0; j < nb; ++j) {
+            const int cj = j / nf, b = j % nf;
+            const int src = c->relc[(size_t)ci * nc + cj] * nf + b;
+            M[(size_t)i + (size_t)nb * j] = M[(size_t)a + (size_t)nb * src];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -401,6 +549,7 @@ int tcg_set_basis(tcg_ctx *c, int nbas, int nfuc, const double *center, const in
     c->poff.assign(prim_off, prim_off + nbas + 1);
     c->expo.assign(expo, expo + np);
     c->coef.assign(coef, coef + np);
+    c->prepared = false;
     for (int p = 0; p < np; ++p)
         if (!(expo[p] > 0.0)) return c->fail(TCG_E_ARG, "exponent %d is not positive", p);
     return build_shells(c);
@@ -411,31 +560,8 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_build before tcg_set_basis");
     if (!sao) return c->fail(TCG_E_ARG, "tcg_eri_build: this version builds the SAO layout ERI(nfuc,nBas,nBas,nBas) only");
     const double t0 = now_ms();
-    // ---- 1-2: tables and pair data ----
-    std::vector<double> tab;
-    std::vector<int> tablim;
-    build_tables(c, tab, tablim);
-    std::vector<double2> vec;
-    std::vector<double> herm;
-    int rc = build_pairs(c, vec, herm);
+    int rc = prepare(c);
     if (rc) return rc;
-    c->max_ncomb = 1;
-    for (auto &s : c->sh) c->max_ncomb = std::max(c->max_ncomb, s.l + 1);
-    c->max_ncomb = c->max_ncomb * c->max_ncomb * c->max_ncomb * c->max_ncomb;
-    HIPCHK(c, c->d_tab.upload(tab));
-    HIPCHK(c, c->d_tablim.upload(tablim));
-    HIPCHK(c, c->d_tnode.upload(c->tnode));
-    HIPCHK(c, c->d_tw.upload(c->tweight));
-    HIPCHK(c, c->d_sh.upload(c->sh));
-    HIPCHK(c, c->d_lxyz.upload(c->lxyz));
-    HIPCHK(c, c->d_coef.upload(c->coef));
-    HIPCHK(c, c->d_poff.upload(c->poff));
-    HIPCHK(c, c->d_pp.upload(c->pp));
-    HIPCHK(c, c->d_sp.upload(c->sp));
-    HIPCHK(c, c->d_vec.upload(vec));
-    HIPCHK(c, c->d_herm.upload(herm));
-    HIPCHK(c, c->d_relc.upload(c->relc));
-    HIPCHK(c, c->d_cnt.alloc(4));
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 4 * sizeof(unsigned long long), c->st));
     const double t1 = now_ms();
 
@@ -594,13 +720,23 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     A.n1 = n1;
     A.physicist = physicist;
     A.Qao = c->d_Q.p;
+    if (!c->ev0) {
+        HIPCHK(c, hipEventCreate(&c->ev0));
+        HIPCHK(c, hipEventCreate(&c->ev1));
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
     HIPCHK(c, launch_eri_quartets(A, c->st));
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    float kms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
+    c->times[5] = kms;
     const double t4 = now_ms();
     unsigned long long cnt[4] = {0, 0, 0, 0};
     HIPCHK(c, hipMemcpy(cnt, c->d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost));
     c->stats[1] = (long long)cnt[0];
     c->stats[2] = (long long)cnt[1];
+    c->stats[3] = (long long)cnt[2];
     c->eri_count = count;
     c->eri_n1 = n1;
     if (eri) HIPCHK(c, hipMemcpy(eri, c->d_eri.p, sizeof(double) * count, hipMemcpyDeviceToHost));
@@ -641,6 +777,48 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
     return TCG_OK;
 }
 
+int tcg_one_electron(tcg_ctx *c, int natom, const double *xyz, const int *Z, double *S, double *T, double *V) {
+    if (!c) return TCG_E_ARG;
+    if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_one_electron before tcg_set_basis");
+    if (natom <= 0 || !xyz || !Z || !S || !T || !V) return c->fail(TCG_E_ARG, "tcg_one_electron: bad arguments");
+    int rc = prepare(c);
+    if (rc) return rc;
+    const int nb = c->nbas, nf = c->nfuc;
+    const size_t nn = (size_t)nb * nb;
+    std::fill(S, S + nn, 0.0);
+    std::fill(T, T + nn, 0.0);
+    std::fill(V, V + nn, 0.0);
+    // V on the GPU (rows of the unit cell)
+    std::vector<int> ao_shell(nb), pair_rep((size_t)nf * nb);
+    for (int sI = 0; sI < c->nsh; ++sI)
+        for (int x = 0; x < c->sh[sI].ncomp; ++x) ao_shell[c->sh[sI].first_ao + x] = sI;
+    for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < nf; ++i) pair_rep[(size_t)i + (size_t)nf * j] = rep_pair(c, ao_shell[i], ao_shell[j]);
+    std::vector<double> nxyz(xyz, xyz + 3 * (size_t)natom);
+    std::vector<int> nz(Z, Z + natom);
+    DevBuf<double> d_nxyz, d_V;
+    DevBuf<int> d_nz, d_aosh, d_prep;
+    HIPCHK(c, d_nxyz.upload(nxyz));
+    HIPCHK(c, d_nz.upload(nz));
+    HIPCHK(c, d_aosh.upload(ao_shell));
+    HIPCHK(c, d_prep.upload(pair_rep));
+    HIPCHK(c, d_V.alloc(nn));
+    HIPCHK(c, launch_fill(d_V.p, nn, 0.0, c->st));
+    EriArgs A = base_args(c);
+    HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, d_V.p, c->st));
+    // S and T on the host meanwhile
+    std::vector<double> Sv(nn, 0.0), Tv(nn, 0.0);
+    one_electron_rows(c, Sv, Tv);
+    HIPCHK(c, hipMemcpyAsync(V, d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::copy(Sv.begin(), Sv.end(), S);
+    std::copy(Tv.begin(), Tv.end(), T);
+    fill_translation(c, S);
+    fill_translation(c, T);
+    fill_translation(c, V);
+    return TCG_OK;
+}
+
 int tcg_eri_stats(tcg_ctx *c, long long *out, int n) {
     if (!c || !out) return TCG_E_ARG;
     for (int i = 0; i < n && i < 8; ++i) out[i] = c->stats[i];
@@ -664,6 +842,8 @@ const char *tcg_last_error(tcg_ctx *c) { return c ? c->err.c_str() : "null conte
 
 int tcg_destroy(tcg_ctx *c) {
     if (!c) return TCG_E_ARG;
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return TCG_OK;

@@ -1,0 +1,148 @@
+// Nuclear attraction matrix on the GPU (rows of the unit cell).
+//
+// Same t-quadrature as the reference's nuclear_attraction_integral_toroidal_nD
+// (src/Toroidal/1D/1D_Nuclear_attraction_toroidal.f90:26-158):
+//   V_ij = - sum_C Z_C  pref  sum_k w_k  prod_{d periodic} Ix_d(t_k)  prod_{d real} e_d s_d(t_k)
+//   Ix_d(t_k) = sum_n F_n T_d[n][k],   F_n = Re[ e^{i n a_d (P_d - C_d)} G_n ]
+// with G_n the bra pair Fourier vector (complex conjugate of the stored sign +1
+// vector).  One wavefront per (AO i in the unit cell, AO j).  Sixteen nuclei
+// are one MFMA row tile, so each F row costs one complex rotation per order
+// and the node sums run on v_mfma_f64_16x16x4f64 like the ERI kernel.
+#include <hip/hip_runtime.h>
+
+#include "tcg_device.hpp"
+
+namespace tcg {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ double overlap_real_1d(int j, int m, double p, double pa, double pb) {
+    // sum_{k,q} C(j,k) C(m,q) pa^(j-k) pb^(m-q) <x^(k+q)>,  <x^n> = (n-1)!!/(2p)^(n/2) sqrt(pi/p)
+    const double base = sqrt(3.14159265358979323846 / p);
+    double acc = 0.0;
+    double cj = 1.0;
+    for (int k = 0; k <= j; ++k) {
+        double cm = 1.0;
+        for (int q = 0; q <= m; ++q) {
+            const int n = k + q;
+            if ((n & 1) == 0) {
+                double mom = base;
+                for (int t = 1; t < n; t += 2) mom *= (double)t / (2.0 * p);
+                acc += cj * cm * pow(pa, (double)(j - k)) * pow(pb, (double)(m - q)) * mom;
+            }
+            cm = cm * (double)(m - q) / (double)(q + 1);
+        }
+        cj = cj * (double)(j - k) / (double)(k + 1);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(64) void k_nuclear(EriArgs A, const double *__restrict__ nuc_xyz, const int *__restrict__ nuc_z,
+                                                int natom, const int *__restrict__ ao_shell, const int *__restrict__ pair_rep,
+                                                double *__restrict__ Vout) {
+    __shared__ double Ix[3][16][kMaxNQ];
+    __shared__ double part[64];
+    const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
+    const int nf = A.nfuc, nb = A.nbas, nq = A.nq, nqp = A.nqp, ndp = A.ndp;
+    const int nh = nqp / kNodeTile;
+    const int tabrows = A.ncap + 5;
+    const int pidx = blockIdx.x;            // i + nf * j
+    if (pidx >= nf * nb) return;
+    const int i = pidx % nf, j = pidx / nf;
+    const Shell sA = A.sh[ao_shell[i]], sB = A.sh[ao_shell[j]];
+    const int lB = sB.l;
+    const int *li = A.ao_lxyz + 3 * i, *lj = A.ao_lxyz + 3 * j;
+    const ShellPair sp = A.sp[pair_rep[pidx]];
+    double total = 0.0;
+    for (int p = 0; p < sp.count; ++p) {
+        const PrimPair pp = A.pp[sp.first + p];
+        const double ci = A.coef[A.ao_poff[i] + pp.pi], cj = A.coef[A.ao_poff[j] + pp.pj];
+        double pref = ci * cj * pp.fac * 2.0 / sqrt(3.14159265358979323846);
+        for (int d = 0; d < ndp; ++d) pref *= pow(A.a[d], -(double)(li[d] + lj[d])) * A.L[d];
+        const double esum = pp.q;
+        for (int c0 = 0; c0 < natom; c0 += 16) {
+            const int c = c0 + r;
+            const bool cvalid = c < natom;
+            for (int d = 0; d < ndp; ++d) {
+                const int N = pp.N[d];
+                const double2 *Vp = A.vec + pp.voff[d] + (size_t)(li[d] * (lB + 1) + lj[d]) * (N + 1);
+                const double phi = cvalid ? A.a[d] * (pp.cen[d] - nuc_xyz[3 * c + d]) : 0.0;
+                int lim[kMaxNQ / kNodeTile];
+                int nend = -1;
+                for (int h = 0; h < nh; ++h) {
+                    const int lh = A.tab_lim[d * nh + h];
+                    lim[h] = lh < N ? lh : N;
+                    nend = lim[h] > nend ? lim[h] : nend;
+                }
+                dbl4 acc[kMaxNQ / kNodeTile];
+                for (int h = 0; h < kMaxNQ / kNodeTile; ++h) acc[h] = dbl4{0.0, 0.0, 0.0, 0.0};
+                double s4, c4, sn, cn;
+                sincos(4.0 * phi, &s4, &c4);
+                sincos((double)kq * phi, &sn, &cn);
+                const double *tab = A.tab + (size_t)d * tabrows * nqp;
+                for (int n0 = 0; n0 <= nend; n0 += 4) {
+                    const int n = n0 + kq;
+                    if (n0 > 0) {
+                        if ((n0 & 63) == 0) {
+                            sincos((double)n * phi, &sn, &cn);
+                        } else {
+                            const double tc = cn * c4 - sn * s4;
+                            sn = sn * c4 + cn * s4;
+                            cn = tc;
+                        }
+                    }
+                    double F = 0.0;
+                    if (cvalid && n <= N) {
+                        const double2 v = Vp[n];               // G_n = conj(v)
+                        F = v.x * cn + v.y * sn;               // Re[e^{in phi} conj(v)]
+                    }
+                    const double *trow = tab + (size_t)n * nqp + r;
+#pragma unroll
+                    for (int h = 0; h < kMaxNQ / kNodeTile; ++h)
+                        if (h < nh && n0 <= lim[h])
+                            acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, trow[h * kNodeTile], acc[h], 0, 0, 0);
+                }
+#pragma unroll
+                for (int h = 0; h < kMaxNQ / kNodeTile; ++h)
+                    if (h < nh)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) Ix[d][4 * v + kq][h * kNodeTile + r] = acc[h][v];
+            }
+            __syncthreads();
+            // per (nucleus, node): real directions, weights, product; summed per lane
+            double lsum = 0.0;
+            for (int task = lane; task < 16 * nq; task += 64) {
+                const int cr = task / nq, k = task % nq, cc = c0 + cr;
+                if (cc >= natom) continue;
+                double prod = A.tweight[k];
+                for (int d = 0; d < ndp; ++d) prod *= Ix[d][cr][k];
+                const double t2 = A.tnode[k] * A.tnode[k];
+                const double ppt = esum + t2, eta = t2 / ppt;
+                for (int d = ndp; d < 3; ++d) {
+                    const double pc = pp.cen[d] - nuc_xyz[3 * cc + d];
+                    const double pa = pp.cen[d] - sA.xyz[d], pb = pp.cen[d] - sB.xyz[d];
+                    prod *= exp(-esum * eta * pc * pc) * overlap_real_1d(li[d], lj[d], ppt, pa - eta * pc, pb - eta * pc);
+                }
+                lsum -= (double)nuc_z[cc] * prod;
+            }
+            total += pref * lsum;
+            __syncthreads();
+        }
+    }
+    part[lane] = total;
+    __syncthreads();
+    if (lane == 0) {
+        double s = 0.0;
+        for (int x = 0; x < 64; ++x) s += part[x];
+        Vout[i + (size_t)nb * j] = s;
+    }
+}
+
+hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
+                          const int *pair_rep, double *V, hipStream_t st) {
+    const int npairs = a.nfuc * a.nbas;
+    hipLaunchKernelGGL(k_nuclear, dim3(npairs), dim3(64), 0, st, a, nuc_xyz, nuc_z, natom, ao_shell, pair_rep, V);
+    return hipGetLastError();
+}
+
+}  // namespace tcg

@@ -38,6 +38,7 @@ def lib():
         L.tcg_set_basis.argtypes = [_vp, _ci, _ci, _dp, _ip, _ip, _dp, _dp]
         L.tcg_eri_build.argtypes = [_vp, _ci, _ci, _vp]
         L.tcg_jk_stored.argtypes = [_vp, _ci, _ci, _dp, _vp, _dp, _dp]
+        L.tcg_one_electron.argtypes = [_vp, _ci, _dp, _ip, _dp, _dp, _dp]
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
         L.tcg_eri_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_longlong)]
@@ -97,19 +98,42 @@ class Engine:
     def stats(self):
         out = np.zeros(8, dtype=np.int64)
         lib().tcg_eri_stats(self.h, out, 8)
-        return {"shell_quartets": int(out[0]), "prim_quartets": int(out[1]), "rows": int(out[2])}
+        return {"shell_quartets": int(out[0]), "prim_quartets": int(out[1]), "rows": int(out[2]),
+                "flops": int(out[3])}
 
     def times(self):
         out = np.zeros(8)
         lib().tcg_eri_times(self.h, out, 8)
         return {"tables_pairs_ms": out[0], "schwarz_ms": out[1], "quartet_list_ms": out[2],
-                "quartets_ms": out[3], "total_ms": out[4]}
+                "quartets_ms": out[3], "total_ms": out[4], "kernel_ms": out[5]}
 
     def eri_device(self):
         p = _vp()
         n = ctypes.c_longlong()
         lib().tcg_eri_device(self.h, ctypes.byref(p), ctypes.byref(n))
         return p.value, n.value
+
+    # -- overlap / kinetic / nuclear_attraction_matrix_toroidal_nD -------------
+    def one_electron(self):
+        """S, T, V of the supercell as ordinary (row-major) numpy matrices."""
+        s = self.s
+        nb = s.nbas
+        S = np.zeros(nb * nb)
+        T = np.zeros(nb * nb)
+        V = np.zeros(nb * nb)
+        self._check(lib().tcg_one_electron(self.h, s.natom, np.ascontiguousarray(s.geom).ravel(),
+                                           np.ascontiguousarray(s.Z, dtype=np.int32), S, T, V), "tcg_one_electron")
+        f = lambda M: M.reshape(nb, nb).T.copy()
+        return f(S), f(T), f(V)
+
+    def refresh_basis(self, coef):
+        """Re-sends the AO list (drops every table derived from it)."""
+        s = self.s
+        self._check(lib().tcg_set_basis(self.h, s.nbas, s.nfuc, np.ascontiguousarray(s.center).ravel(),
+                                        np.ascontiguousarray(s.lxyz, dtype=np.int32).ravel(),
+                                        np.ascontiguousarray(s.prim_off, dtype=np.int32),
+                                        np.ascontiguousarray(s.expo), np.ascontiguousarray(coef, dtype=np.float64)),
+                    "tcg_set_basis")
 
     # -- hartree_exchange_potential ---------------------------------------------
     def jk_stored(self, P: np.ndarray, n1: int | None = None, eri: np.ndarray | None = None):

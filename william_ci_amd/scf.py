@@ -1,0 +1,227 @@
+"""Restricted Hartree-Fock in the Bloch (SAO) basis on top of the engine.
+
+Host-side mirror of the reference's SCF driver, which stays in Fortran in the
+drop-in; the engine supplies the integrals and the Coulomb/exchange build:
+
+  RHF_SAO                src/Energies/SAO/RHF_SAO.f90:1-553
+  guess_AO_RHF_SAO       src/Energies/SAO/guess_RHF_SAO.f90:1 (+ guess_occupy_cut,
+                         src/Energies/SAO/guess_Huckel_SAO.f90:141-192)
+  transform_AO_to_SAO /  src/Tools/SAO_Transfer.f90:1,108
+  transform_SAO_to_AO
+  get_X_from_overlap     src/Energies/Energies_tools.f90:84 (canonical, drop <= 1e-6)
+  DIIS_extrapolation     src/Energies/DIIS_extrapolation.f90:1 (history 8, reset rcond < 1e-14)
+  NRE                    src/Energies/Nuclear_repulsion.f90:1 (chord distance)
+  HF_parameter           src/module/HF_constant.f90 (maxSCF 100, orbital error 1e-9)
+
+The dense algebra (nBas^2 to nBas^3, per-k nfuc x nfuc eigenproblems) runs
+in numpy; the J/K build, which dominates, runs on the GPU.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Callable, Optional
+
+import numpy as np
+
+MAX_SCF = 100
+THRESH = 1e-9
+MAX_DIIS = 8
+LINDEP = 1e-6
+LINDEP_SHIFT = 1e4
+
+
+def nuclear_repulsion(sysm) -> float:
+    """Sum Z_i Z_j / d_ij with the chord distance along the periodic directions."""
+    g, Z = sysm.geom, sysm.Z.astype(np.float64)
+    e = 0.0
+    for i in range(sysm.natom - 1):
+        d = g[i] - g[i + 1:]
+        for k in range(sysm.ndp):
+            a = 2 * math.pi / sysm.L[k]
+            d[:, k] = np.sqrt(2.0 * (1.0 - np.cos(a * d[:, k])) / (a * a))
+        e += float(np.sum(Z[i] * Z[i + 1:] / np.sqrt(np.sum(d * d, axis=1))))
+    return e
+
+
+def canonical_X(S: np.ndarray, thresh: float = LINDEP) -> np.ndarray:
+    """S^-1/2 (Lowdin) when all eigenvalues exceed thresh, else canonical with
+    the small-eigenvalue columns zeroed."""
+    e, U = np.linalg.eigh(S)
+    if np.all(e > thresh):
+        return (U / np.sqrt(e)) @ U.conj().T
+    X = U / np.sqrt(np.where(e > thresh, e, 1.0))
+    X[:, e <= thresh] = 0.0
+    return X
+
+
+def _shift_dropped(X: np.ndarray, Fp: np.ndarray) -> None:
+    dead = np.all(X == 0, axis=0)
+    idx = np.nonzero(dead)[0]
+    Fp[idx, idx] = LINDEP_SHIFT
+
+
+class Bloch:
+    """AO <-> SAO (Bloch) transforms of translation-symmetric supercell matrices."""
+
+    def __init__(self, sysm):
+        self.nf = sysm.nfuc
+        self.nc = sysm.ncell           # (nx, ny, nz), cells x fastest
+        self.N = sysm.n_cell
+
+    def to_k(self, M: np.ndarray) -> np.ndarray:
+        """M_k(a,b) = sum_R e^{+i k.R} M(a, b + R nf), from the unit-cell rows of M."""
+        nf, (nx, ny, nz) = self.nf, self.nc
+        B = M[:nf, :].reshape(nf, nz, ny, nx, nf).transpose(1, 2, 3, 0, 4)
+        Mk = np.fft.ifftn(B, axes=(0, 1, 2)) * self.N
+        return Mk.reshape(self.N, nf, nf)
+
+    def to_ao(self, Mk: np.ndarray) -> np.ndarray:
+        """M(a + R nf, b + R' nf) = Re (1/N) sum_k e^{+i k.(R-R')} M_k(a,b)."""
+        nf, (nx, ny, nz) = self.nf, self.nc
+        C = np.fft.ifftn(Mk.reshape(nz, ny, nx, nf, nf), axes=(0, 1, 2)).real   # C[dR]
+        cells = np.arange(self.N)
+        cx, cy, cz = cells % nx, (cells // nx) % ny, cells // (nx * ny)
+        dx = (cx[:, None] - cx[None, :]) % nx
+        dy = (cy[:, None] - cy[None, :]) % ny
+        dz = (cz[:, None] - cz[None, :]) % nz
+        blocks = C[dz, dy, dx]                         # (N, N, nf, nf): block (R, R')
+        return blocks.transpose(0, 2, 1, 3).reshape(self.N * nf, self.N * nf)
+
+
+def _occupy_cut(e: np.ndarray, c: np.ndarray, nO: int, deg_tol: float = 1e-8) -> np.ndarray:
+    """Density of the nO lowest orbitals, a degenerate level at the cut filled evenly."""
+    tol = deg_tol * max(1.0, abs(e[nO - 1]))
+    lo = nO - 1
+    while lo > 0 and abs(e[lo - 1] - e[nO - 1]) < tol:
+        lo -= 1
+    hi = nO - 1
+    while hi < len(e) - 1 and abs(e[hi + 1] - e[nO - 1]) < tol:
+        hi += 1
+    if hi == nO - 1:
+        return 2.0 * c[:, :nO] @ c[:, :nO].T
+    occ = 2.0 * (nO - lo) / (hi - lo + 1)
+    w = np.concatenate([np.full(lo, 2.0), np.full(hi - lo + 1, occ)])
+    cw = c[:, :hi + 1] * np.sqrt(w)
+    return cw @ cw.T
+
+
+class _DIIS:
+    def __init__(self, n: int):
+        self.err = []
+        self.F = []
+        self.n = n
+
+    def extrapolate(self, err: np.ndarray, F: np.ndarray) -> np.ndarray:
+        self.err.insert(0, err.ravel().copy())
+        self.F.insert(0, F.ravel().copy())
+        del self.err[self.n:], self.F[self.n:]
+        m = len(self.err)
+        E = np.array(self.err)
+        A = np.zeros((m + 1, m + 1))
+        A[:m, :m] = E @ E.T
+        A[:m, m] = A[m, :m] = -1.0
+        b = np.zeros(m + 1)
+        b[m] = -1.0
+        rcond = 1.0 / np.linalg.cond(A, 1) if np.all(np.isfinite(A)) else 0.0
+        if not np.isfinite(rcond) or rcond < 1e-14:
+            self.err.clear()
+            self.F.clear()
+            return F
+        w = np.linalg.solve(A, b)
+        return (w[:m] @ np.array(self.F)).reshape(F.shape)
+
+
+@dataclasses.dataclass
+class RHFResult:
+    energy: float                   # E_HF including nuclear repulsion
+    e_nuc: float
+    converged: bool
+    iterations: int
+    orbital_error: float
+    mo_energies: np.ndarray         # per k: (Ncell, nf), ascending
+    mo_coef: np.ndarray             # per k: (Ncell, nf, nf) AO-Bloch coefficients
+    P: np.ndarray                   # AO density (nBas, nBas)
+    occupied: np.ndarray            # (Ncell, nf) bool
+    times: dict
+
+
+def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.ndarray], tuple],
+            diis: bool = True, max_scf: int = MAX_SCF, thresh: float = THRESH,
+            log: Optional[Callable[[str], None]] = None) -> RHFResult:
+    """RHF_SAO: k-space RHF with the unit-cell rows of J and K from ``jk(P)``."""
+    t0 = time.perf_counter()
+    nb, nO, Nc, nf = sysm.nbas, sysm.n_occ, sysm.n_cell, sysm.nfuc
+    Hc = T + V
+    enuc = nuclear_repulsion(sysm)
+    X = canonical_X(S)
+    # core guess in the supercell, degenerate level at the cut shared evenly
+    cp = X.T @ Hc @ X
+    _shift_dropped(X, cp)
+    e0, U0 = np.linalg.eigh(cp)
+    P = _occupy_cut(e0, X @ U0, nO)
+
+    bl = Bloch(sysm)
+    Sk = bl.to_k(S)
+    Xk = np.zeros_like(Sk)
+    for k in range(Nc):
+        Xk[k] = canonical_X(Sk[k])
+    dead = [np.nonzero(np.all(Xk[k] == 0, axis=0))[0] for k in range(Nc)]
+    dii = _DIIS(MAX_DIIS)
+    E_old, conv, converged = 0.0, 1.0, False
+    n_stable, hist = 0, [np.inf] * 5
+    t_jk = 0.0
+    it = 0
+    ek = np.zeros((Nc, nf))
+    Ck = np.zeros((Nc, nf, nf), dtype=complex)
+    occ = np.zeros((Nc, nf), dtype=bool)
+    while it < max_scf:
+        it += 1
+        tj = time.perf_counter()
+        J, K = jk(P)
+        t_jk += time.perf_counter() - tj
+        F = Hc + J + K
+        F = bl.to_ao(bl.to_k(F))
+        ET = float(np.sum(P * T.T))
+        EV = float(np.sum(P * V.T))
+        EJ = 0.5 * float(np.sum(P * J.T)) * Nc
+        EK = 0.5 * float(np.sum(P * K.T)) * Nc
+        E = ET + EV + EJ + EK
+        err = F @ P @ S - S @ P @ F
+        err_o = X.T @ err @ X
+        if it > 1:
+            conv = float(np.abs(err_o).max())
+        if log:
+            log(f"  SCF {it:3d}  E = {E + enuc:.12f}  conv = {conv:.3e}")
+        if conv < thresh:
+            converged = True
+            break
+        if it > 2 and abs(E - E_old) < max(1e-10, 2e-10 * abs(E)):
+            n_stable += 1
+        else:
+            n_stable = 0
+        if n_stable >= 5 and conv < 1e-6 and conv > 0.5 * hist[it % 5]:
+            converged = True
+            break
+        hist[it % 5] = conv
+        E_old = E
+        if diis:
+            F = dii.extrapolate(err_o, F)
+        Fk = bl.to_k(F)
+        for k in range(Nc):
+            Fp = Xk[k].conj().T @ Fk[k] @ Xk[k]
+            Fp[dead[k], dead[k]] = LINDEP_SHIFT
+            ek[k], Up = np.linalg.eigh(Fp)
+            Ck[k] = Xk[k] @ Up
+        order = np.argsort(ek.ravel(), kind="stable")[:nO]
+        occ[:] = False
+        occ.ravel()[order] = True
+        Pk = np.zeros((Nc, nf, nf), dtype=complex)
+        for k in range(Nc):
+            c = Ck[k][:, occ[k]]
+            Pk[k] = 2.0 * c @ c.conj().T
+        P = bl.to_ao(Pk)
+    return RHFResult(energy=E + enuc, e_nuc=enuc, converged=converged, iterations=it, orbital_error=conv,
+                     mo_energies=ek.copy(), mo_coef=Ck.copy(), P=P, occupied=occ.copy(),
+                     times={"scf_s": time.perf_counter() - t0, "jk_s": t_jk})
