@@ -45,6 +45,10 @@ hipError_t launch_eri_quartets(const EriArgs &a, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
                           const int *pair_rep, double *V, hipStream_t st);
+hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, double *ivbuf, double2 *vec, hipStream_t st);
+hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, double Qmax, int nshuc,
+                                int nsh, int ncell, unsigned long long *count, Quartet *out,
+                                unsigned long long cap, hipStream_t st);
 size_t jk_work_doubles(int n1, int nbas);
 hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,
                             double *jwork, hipStream_t st);

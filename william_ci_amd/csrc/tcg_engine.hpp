@@ -48,6 +48,14 @@ struct Quartet {
     int prim;            // >= 0: only primitive pair 'prim' of bra and ket (primitive Schwarz pass)
 };
 
+// One pair Fourier-vector job: exp(-A) I_n(A) up to N + lA + lB, then the
+// vectors of all power pairs (ia <= lA, ib <= lB) at vec[voff + (ia*(lB+1)+ib)*(N+1)].
+struct VecJob {
+    double A, phA, phB;
+    int N, lA, lB, voff;
+    long ivoff;          // scratch for the Bessel values
+};
+
 struct Shell {
     int first_ao;        // first AO
     int ncomp;           // Cartesian components

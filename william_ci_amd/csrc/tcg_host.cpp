@@ -84,7 +84,7 @@ struct tcg_ctx {
     std::vector<ShellPair> sp;
     // device
     hipStream_t st = nullptr;
-    DevBuf<double> d_tab, d_tnode, d_tw, d_coef, d_herm, d_blk, d_Q, d_eri, d_P, d_J, d_K, d_jwork;
+    DevBuf<double> d_tab, d_tnode, d_tw, d_coef, d_herm, d_blk, d_Q, d_Qsh, d_eri, d_P, d_J, d_K, d_jwork;
     DevBuf<int> d_tablim, d_tnmax, d_lxyz, d_poff, d_relc;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf<Shell> d_sh;
@@ -240,12 +240,16 @@ static void pair_vector(int N, double A, int i, int l, double phA, double phB, d
     }
 }
 
-static int build_pairs(tcg_ctx *c, std::vector<double2> &vec, std::vector<double> &herm) {
+// Primitive-pair records on the host (barycentres, prefactors, Bessel lengths,
+// Hermite coefficients); the Fourier vectors are generated on the GPU from
+// the job list (tcg_setup.hip::k_pair_vectors).
+static int build_pairs(tcg_ctx *c, std::vector<VecJob> &jobs, size_t &nvec, size_t &niv, std::vector<double> &herm) {
     c->pp.clear();
     c->sp.assign((size_t)c->nshuc * c->nsh, ShellPair{0, 0});
-    vec.clear();
+    jobs.clear();
     herm.clear();
-    std::vector<double> iv;
+    nvec = 0;
+    niv = 0;
     const int nreal = 3 - c->ndp;
     for (int S1 = 0; S1 < c->nshuc; ++S1)
         for (int S2 = 0; S2 < c->nsh; ++S2) {
@@ -271,12 +275,19 @@ static int build_pairs(tcg_ctx *c, std::vector<double2> &vec, std::vector<double
                             const double Aarg = 2.0 * pe / (c->a[d] * c->a[d]);
                             const int N = std::min(nmax_est(Aarg), c->ncap);
                             P.N[d] = N;
-                            P.voff[d] = (int)vec.size();
-                            vec.resize(vec.size() + (size_t)(A.l + 1) * (B.l + 1) * (N + 1));
-                            for (int ia = 0; ia <= A.l; ++ia)
-                                for (int ib = 0; ib <= B.l; ++ib)
-                                    pair_vector(N, Aarg, ia, ib, c->a[d] * (cen - A.xyz[d]), c->a[d] * (cen - B.xyz[d]),
-                                                vec.data() + P.voff[d] + (size_t)(ia * (B.l + 1) + ib) * (N + 1), iv);
+                            P.voff[d] = (int)nvec;
+                            VecJob J;
+                            J.A = Aarg;
+                            J.phA = c->a[d] * (cen - A.xyz[d]);
+                            J.phB = c->a[d] * (cen - B.xyz[d]);
+                            J.N = N;
+                            J.lA = A.l;
+                            J.lB = B.l;
+                            J.voff = (int)nvec;
+                            J.ivoff = (long)niv;
+                            jobs.push_back(J);
+                            nvec += (size_t)(A.l + 1) * (B.l + 1) * (N + 1);
+                            niv += (size_t)(N + A.l + B.l + 1);
                         } else {
                             const double cen = (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum;
                             P.cen[d] = cen;
@@ -300,7 +311,7 @@ static int build_pairs(tcg_ctx *c, std::vector<double2> &vec, std::vector<double
                     c->pp.push_back(P);
                 }
         }
-    if (vec.size() > (size_t)0x7fffffff) return c->fail(TCG_E_NOMEM, "pair vectors exceed 2^31 elements");
+    if (nvec > (size_t)0x7fffffff) return c->fail(TCG_E_NOMEM, "pair vectors exceed 2^31 elements");
     return 0;
 }
 
@@ -365,9 +376,10 @@ static int prepare(tcg_ctx *c) {
     std::vector<double> tab;
     std::vector<int> tablim, tnmax;
     build_tables(c, tab, tablim, tnmax);
-    std::vector<double2> vec;
+    std::vector<VecJob> jobs;
+    size_t nvec = 0, niv = 0;
     std::vector<double> herm;
-    int rc = build_pairs(c, vec, herm);
+    int rc = build_pairs(c, jobs, nvec, niv, herm);
     if (rc) return rc;
     c->max_ncomb = 1;
     for (auto &s : c->sh) c->max_ncomb = std::max(c->max_ncomb, s.l + 1);
@@ -383,10 +395,18 @@ static int prepare(tcg_ctx *c) {
     HIPCHK(c, c->d_poff.upload(c->poff));
     HIPCHK(c, c->d_pp.upload(c->pp));
     HIPCHK(c, c->d_sp.upload(c->sp));
-    HIPCHK(c, c->d_vec.upload(vec));
+    {
+        DevBuf<VecJob> d_jobs;
+        DevBuf<double> d_iv;
+        HIPCHK(c, d_jobs.upload(jobs));
+        HIPCHK(c, d_iv.alloc(niv));
+        HIPCHK(c, c->d_vec.alloc(nvec));
+        HIPCHK(c, launch_pair_vectors(d_jobs.p, (int)jobs.size(), d_iv.p, c->d_vec.p, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
     HIPCHK(c, c->d_herm.upload(herm));
     HIPCHK(c, c->d_relc.upload(c->relc));
-    HIPCHK(c, c->d_cnt.alloc(4));
+    HIPCHK(c, c->d_cnt.alloc(8));
     c->prepared = true;
     return 0;
 }
@@ -562,7 +582,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     const double t0 = now_ms();
     int rc = prepare(c);
     if (rc) return rc;
-    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 4 * sizeof(unsigned long long), c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 3 * sizeof(unsigned long long), c->st));
     const double t1 = now_ms();
 
     // ---- 3: Schwarz from the diagonal blocks, replicated over translations ----
@@ -647,63 +667,23 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     }
     HIPCHK(c, c->d_pp.upload(c->pp));
     HIPCHK(c, c->d_Q.upload(Q));
-    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 4 * sizeof(unsigned long long), c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 3 * sizeof(unsigned long long), c->st));
     const double t2 = now_ms();
 
-    // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened ----
-    q4.clear();
-    static const int perm8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
-                                    {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
-    for (int SI = 0; SI < nshuc; ++SI)
-        for (int SJ = 0; SJ < nsh; ++SJ) {
-            if (c->sh[SJ].loc < SI) continue;
-            const double qij = Qsh[(size_t)SI + (size_t)nsh * SJ];
-            if (qij * Qmax < kSchwarz) continue;
-            for (int SK = 0; SK < nsh; ++SK) {
-                if (c->sh[SK].loc < SI) continue;
-                for (int SL = 0; SL < nsh; ++SL) {
-                    if (c->sh[SL].loc < SI) continue;
-                    if (qij * Qsh[(size_t)SK + (size_t)nsh * SL] < kSchwarz) continue;
-                    const int q[4] = {SI, SJ, SK, SL};
-                    bool smaller = false;
-                    for (int ip = 1; ip < 8 && !smaller; ++ip) {
-                        int t[4], rr[4];
-                        for (int x = 0; x < 4; ++x) t[x] = q[perm8[ip][x]];
-                        const int c0 = c->sh[t[0]].cell;
-                        for (int x = 0; x < 4; ++x)
-                            rr[x] = c->relc[(size_t)c0 * nc + c->sh[t[x]].cell] * nshuc + c->sh[t[x]].loc;
-                        for (int x = 0; x < 4; ++x) {
-                            if (rr[x] < q[x]) { smaller = true; break; }
-                            if (rr[x] > q[x]) break;
-                        }
-                    }
-                    if (smaller) continue;
-                    Quartet qq{};
-                    for (int x = 0; x < 4; ++x) qq.s[x] = q[x];
-                    qq.bra = rep_pair(c, SI, SJ);
-                    qq.ket = rep_pair(c, SK, SL);
-                    const Shell &rk = c->sh[c->sh[SK].loc];   // representative of SK in the unit cell
-                    for (int d = 0; d < 3; ++d) qq.shift[d] = c->sh[SK].xyz[d] - rk.xyz[d];
-                    qq.out = -1;
-                    qq.prim = -1;
-                    q4.push_back(qq);
-                }
-            }
-        }
-    // heavy quartets first (longest Fourier vectors x most primitive pairs)
-    std::vector<std::pair<float, int>> order(q4.size());
-    for (size_t x = 0; x < q4.size(); ++x) {
-        const ShellPair &b = c->sp[q4[x].bra], &k = c->sp[q4[x].ket];
-        float cost = (float)b.count * k.count;
-        int nmax = 0;
-        for (int p = 0; p < b.count; ++p) nmax = std::max(nmax, c->pp[b.first + p].N[0]);
-        cost *= (float)(nmax + 16);
-        order[x] = {-cost, (int)x};
-    }
-    std::sort(order.begin(), order.end());
-    std::vector<Quartet> sorted(q4.size());
-    for (size_t x = 0; x < q4.size(); ++x) sorted[x] = q4[order[x].second];
-    c->stats[0] = (long long)sorted.size();
+    // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened, on the GPU ----
+    HIPCHK(c, c->d_Qsh.upload(Qsh));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
+    HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, Qmax, nshuc, nsh, nc, c->d_cnt.p + 3, nullptr, 0,
+                                   c->st));
+    unsigned long long nq4 = 0;
+    HIPCHK(c, hipMemcpyAsync(&nq4, c->d_cnt.p + 3, sizeof nq4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (nq4 > (unsigned long long)0x7fffffff) return c->fail(TCG_E_NOMEM, "too many shell quartets (%llu)", nq4);
+    HIPCHK(c, c->d_q4.alloc(nq4));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
+    HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, Qmax, nshuc, nsh, nc, c->d_cnt.p + 3, c->d_q4.p,
+                                   nq4, c->st));
+    c->stats[0] = (long long)nq4;
     const double t3 = now_ms();
 
     // ---- 6: quartets on the GPU, scattered into the ERI array ----
@@ -711,10 +691,9 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     const size_t count = (size_t)n1 * nb * nb * nb;
     HIPCHK(c, c->d_eri.alloc(count));
     HIPCHK(c, launch_fill(c->d_eri.p, count, 0.0, c->st));
-    HIPCHK(c, c->d_q4.upload(sorted));
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
-    A.nq4 = (int)sorted.size();
+    A.nq4 = (int)nq4;
     A.screen_prim = 1;
     A.eri = c->d_eri.p;
     A.n1 = n1;
