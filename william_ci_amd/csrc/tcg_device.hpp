@@ -29,6 +29,7 @@ struct EriArgs {
     int screen_prefactor;             // kernel prefactor screens (off in the primitive Schwarz pass)
     int screen_prim;                  // primitive Schwarz screen (keyword prim_screen)
     int max_ncomb;                    // largest (lA+1)(lB+1)(lC+1)(lD+1) over the quartets
+    int max_blk;                      // largest nA nB nC nD over the quartets
     // block mode
     double *blk_out;
     // scatter mode
@@ -40,8 +41,6 @@ struct EriArgs {
     unsigned long long *counters;     // [0] primitive quartets, [1] rows, [2] algorithmic flops
 };
 
-size_t eri_lds_bytes(const EriArgs &a);
-hipError_t launch_eri_quartets(const EriArgs &a, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
                           const int *pair_rep, double *V, hipStream_t st);
@@ -49,6 +48,20 @@ hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, double *ivbuf, dou
 hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, double Qmax, int nshuc,
                                 int nsh, int ncell, unsigned long long *count, Quartet *out,
                                 unsigned long long cap, hipStream_t st);
+hipError_t launch_surv(const EriArgs &a, int pass, long long *scount, long long *rcount, const long long *soff,
+                       Surv *surv, hipStream_t st);
+hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                            RowRec *rows, unsigned *keys, int *vals, hipStream_t st);
+hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *order, long long nrows, double *rowval,
+                            hipStream_t st);
+size_t contract_lds_bytes(const EriArgs &a);
+hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                           const double *rowval, hipStream_t st);
+size_t scan_temp_bytes(int n);
+hipError_t scan_exclusive(void *tmp, size_t bytes, const long long *in, long long *out, int n, hipStream_t st);
+size_t sort_temp_bytes(int n, int end_bit);
+hipError_t sort_pairs(void *tmp, size_t bytes, const unsigned *kin, unsigned *kout, const int *vin, int *vout, int n,
+                      int end_bit, hipStream_t st);
 size_t jk_work_doubles(int n1, int nbas);
 hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,
                             double *jwork, hipStream_t st);

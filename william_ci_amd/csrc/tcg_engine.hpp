@@ -56,6 +56,22 @@ struct VecJob {
     long ivoff;          // scratch for the Bessel values
 };
 
+// A surviving primitive quartet: global primitive-pair indices and prefactor.
+struct Surv {
+    int pb, pk;
+    double fac;
+};
+
+// One row of the quadrature contraction: (primitive quartet, periodic
+// direction, power combination).
+struct RowRec {
+    double phi;          // a_d (P_d - Q_d)
+    double sgn;          // a_d^-(ia+ib+ic+id)
+    int gofs, hofs;      // offsets of the bra / ket Fourier vectors
+    int nm;              // last order: min(N_bra, N_ket)
+    int d;               // periodic direction
+};
+
 struct Shell {
     int first_ao;        // first AO
     int ncomp;           // Cartesian components

@@ -93,11 +93,20 @@ struct tcg_ctx {
     DevBuf<double2> d_vec;
     DevBuf<Quartet> d_q4;
     DevBuf<unsigned long long> d_cnt;
+    // row pipeline
+    DevBuf<long long> d_scount, d_rcount, d_soff, d_roff;
+    DevBuf<Surv> d_surv;
+    DevBuf<RowRec> d_rows;
+    DevBuf<unsigned> d_keys, d_keys2;
+    DevBuf<int> d_vals, d_order;
+    DevBuf<double> d_rowval;
+    DevBuf<char> d_tmp;
+    long long last_surv = 0, last_rows = 0;
     size_t eri_count = 0;
     int eri_n1 = 0;
     long long stats[8] = {0};
     double times[8] = {0};
-    int max_ncomb = 1;
+    int max_ncomb = 1, max_blk = 1;
     bool prepared = false;
 
     int fail(int code, const char *fmt, ...) {
@@ -346,12 +355,56 @@ static EriArgs base_args(tcg_ctx *c) {
     A.screen_prefactor = 1;
     A.screen_prim = 0;
     A.max_ncomb = c->max_ncomb;
+    A.max_blk = c->max_blk;
     A.nbas = c->nbas;
     A.nfuc = c->nfuc;
     A.ncell = c->ncell;
     A.relc = c->d_relc.p;
     A.counters = c->d_cnt.p;
     return A;
+}
+
+// Quartets of A.q4[0..A.nq4) through the row pipeline (tcg_rows.hip):
+// survivors -> rows -> sort by (direction, length) -> MFMA -> contraction.
+static int run_quartets(tcg_ctx *c, EriArgs A) {
+    const int nq4 = A.nq4;
+    c->last_surv = c->last_rows = 0;
+    if (nq4 <= 0) return 0;
+    HIPCHK(c, c->d_scount.alloc(nq4 + 1));
+    HIPCHK(c, c->d_rcount.alloc(nq4 + 1));
+    HIPCHK(c, c->d_soff.alloc(nq4 + 1));
+    HIPCHK(c, c->d_roff.alloc(nq4 + 1));
+    HIPCHK(c, hipMemsetAsync(c->d_scount.p + nq4, 0, sizeof(long long), c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_rcount.p + nq4, 0, sizeof(long long), c->st));
+    HIPCHK(c, launch_surv(A, 0, c->d_scount.p, c->d_rcount.p, nullptr, nullptr, c->st));
+    size_t tb = scan_temp_bytes(nq4 + 1);
+    HIPCHK(c, c->d_tmp.alloc(tb));
+    HIPCHK(c, scan_exclusive(c->d_tmp.p, tb, c->d_scount.p, c->d_soff.p, nq4 + 1, c->st));
+    HIPCHK(c, scan_exclusive(c->d_tmp.p, tb, c->d_rcount.p, c->d_roff.p, nq4 + 1, c->st));
+    long long ns = 0, nr = 0;
+    HIPCHK(c, hipMemcpyAsync(&ns, c->d_soff.p + nq4, sizeof ns, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(&nr, c->d_roff.p + nq4, sizeof nr, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->last_surv = ns;
+    c->last_rows = nr;
+    if (nr > 0x7fffffffLL) return c->fail(TCG_E_NOMEM, "too many quadrature rows (%lld) for one pass", nr);
+    if ((double)nr * A.nqp * 8.0 > 96e9) return c->fail(TCG_E_NOMEM, "row buffer would exceed 96 GB (%lld rows)", nr);
+    if (nr == 0) return 0;
+    HIPCHK(c, c->d_surv.alloc(ns));
+    HIPCHK(c, c->d_rows.alloc(nr));
+    HIPCHK(c, c->d_keys.alloc(nr));
+    HIPCHK(c, c->d_keys2.alloc(nr));
+    HIPCHK(c, c->d_vals.alloc(nr));
+    HIPCHK(c, c->d_order.alloc(nr));
+    HIPCHK(c, c->d_rowval.alloc((size_t)nr * A.nqp));
+    HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, c->d_soff.p, c->d_surv.p, c->st));
+    HIPCHK(c, launch_rows_emit(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
+    tb = sort_temp_bytes((int)nr, 18);
+    HIPCHK(c, c->d_tmp.alloc(tb));
+    HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, 18, c->st));
+    HIPCHK(c, launch_rows_mfma(A, c->d_rows.p, c->d_order.p, nr, c->d_rowval.p, c->st));
+    HIPCHK(c, launch_contract(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_rowval.p, c->st));
+    return 0;
 }
 
 static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, std::vector<double> &out, int prefactor) {
@@ -362,7 +415,8 @@ static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, s
     A.nq4 = (int)q4.size();
     A.blk_out = c->d_blk.p;
     A.screen_prefactor = prefactor;
-    HIPCHK(c, launch_eri_quartets(A, c->st));
+    int rc = run_quartets(c, A);
+    if (rc) return rc;
     out.resize(nout);
     HIPCHK(c, hipMemcpyAsync(out.data(), c->d_blk.p, sizeof(double) * nout, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -384,6 +438,9 @@ static int prepare(tcg_ctx *c) {
     c->max_ncomb = 1;
     for (auto &s : c->sh) c->max_ncomb = std::max(c->max_ncomb, s.l + 1);
     c->max_ncomb = c->max_ncomb * c->max_ncomb * c->max_ncomb * c->max_ncomb;
+    c->max_blk = 1;
+    for (auto &s : c->sh) c->max_blk = std::max(c->max_blk, s.ncomp);
+    c->max_blk = c->max_blk * c->max_blk * c->max_blk * c->max_blk;
     HIPCHK(c, c->d_tab.upload(tab));
     HIPCHK(c, c->d_tablim.upload(tablim));
     HIPCHK(c, c->d_tnmax.upload(tnmax));
@@ -704,7 +761,8 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
         HIPCHK(c, hipEventCreate(&c->ev1));
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, launch_eri_quartets(A, c->st));
+    rc = run_quartets(c, A);
+    if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float kms = 0.f;
@@ -713,8 +771,8 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     const double t4 = now_ms();
     unsigned long long cnt[4] = {0, 0, 0, 0};
     HIPCHK(c, hipMemcpy(cnt, c->d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost));
-    c->stats[1] = (long long)cnt[0];
-    c->stats[2] = (long long)cnt[1];
+    c->stats[1] = c->last_surv;
+    c->stats[2] = c->last_rows;
     c->stats[3] = (long long)cnt[2];
     c->eri_count = count;
     c->eri_n1 = n1;

@@ -1,0 +1,447 @@
+// ERI quartet pipeline, row form (gfx950).
+//
+// The arithmetic is that of the reference kernel
+// (src/Toroidal/ERI_shell_block_toroidal.f90:28-390); the work is regrouped so
+// that the matrix cores see full, uniform tiles:
+//
+//   k_surv        one wavefront per shell quartet: the primitive quartets that
+//                 survive the prefactor and primitive-Schwarz screens
+//                 (ERI_shell_block_toroidal.f90:191,216-219), counted, then listed.
+//   k_rows_emit   one ROW per (surviving primitive quartet, periodic direction d,
+//                 power combination): offsets of its bra/ket Fourier vectors,
+//                 length nm = min(N_bra, N_ket), phase phi, power sum; sort key
+//                 (d, nm).
+//   (radix sort of the rows by key, tcg_sort.hip)
+//   k_rows_mfma   one wavefront per 16 consecutive sorted rows:
+//                 I_d(t_k) = sum_n Re[conj(G_n) H_n e^{in phi}] T_d[n][k]
+//                 with v_mfma_f64_16x16x4f64 (F on the VALU as the A operand,
+//                 the t-table as B), loads two steps ahead of use.  Result:
+//                 32 node values per row, in HBM.
+//   k_contract    one wavefront per shell quartet: real directions (Hermite
+//                 form per node), sum_k w_k Ix Iy Iz, coefficients, the shell
+//                 block in LDS, then the 8-permutation scatter translated to
+//                 the unit cell (src/Toroidal/ERI_toroidal.f90:416-454) or the
+//                 block itself (Schwarz passes).
+#include <hip/hip_runtime.h>
+
+#include "tcg_device.hpp"
+
+namespace tcg {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__constant__ int c_perm8_rows[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                       {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
+
+
+__device__ __forceinline__ int wmax(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wmin(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ void quartet_shape(const EriArgs &A, const Quartet &Q, int l[4], int &ncomb) {
+    for (int x = 0; x < 4; ++x) l[x] = A.sh[Q.s[x]].l;
+    ncomb = (l[0] + 1) * (l[1] + 1) * (l[2] + 1) * (l[3] + 1);
+}
+
+// ---------------------------------------------------------------------------
+// survivors: pass 0 counts (scount, rcount), pass 1 writes at soff
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_surv(EriArgs A, int pass, long long *__restrict__ scount,
+                                             long long *__restrict__ rcount, const long long *__restrict__ soff,
+                                             Surv *__restrict__ surv) {
+    const int lane = threadIdx.x;
+    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+        const Quartet Q = A.q4[qi];
+        const ShellPair bp = A.sp[Q.bra], kp = A.sp[Q.ket];
+        const long ncand = (Q.prim >= 0) ? 1 : (long)bp.count * kp.count;
+        long n = 0;
+        const long base_out = pass ? soff[qi] : 0;
+        for (long base = 0; base < ncand; base += 64) {
+            const long c = base + lane;
+            bool ok = c < ncand;
+            int ij = 0, kl = 0;
+            double fac = 0.0;
+            if (ok) {
+                if (Q.prim >= 0) { ij = Q.prim; kl = Q.prim; }
+                else { ij = (int)(c / kp.count); kl = (int)(c % kp.count); }
+                const PrimPair &pb = A.pp[bp.first + ij];
+                const PrimPair &pk = A.pp[kp.first + kl];
+                fac = (A.fac_const * pb.fac) * pk.fac;
+                if (A.screen_prefactor && fabs(fac) * pb.cmax * pk.cmax <= kScreenTol) ok = false;
+                if (A.screen_prim && pb.ps * pk.ps < kPrimTol) ok = false;
+            }
+            const unsigned long long m = __ballot(ok);
+            if (pass && ok) {
+                const long pos = n + __popcll(m & ((1ull << lane) - 1ull));
+                Surv s;
+                s.pb = bp.first + ij;
+                s.pk = kp.first + kl;
+                s.fac = fac;
+                surv[base_out + pos] = s;
+            }
+            n += __popcll(m);
+        }
+        if (!pass && lane == 0) {
+            int l[4], ncomb;
+            quartet_shape(A, Q, l, ncomb);
+            scount[qi] = n;
+            rcount[qi] = n * A.ndp * ncomb;
+        }
+    }
+}
+
+hipError_t launch_surv(const EriArgs &a, int pass, long long *scount, long long *rcount, const long long *soff,
+                       Surv *surv, hipStream_t st) {
+    if (a.nq4 <= 0) return hipSuccess;
+    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
+    hipLaunchKernelGGL(k_surv, dim3(grid), dim3(64), 0, st, a, pass, scount, rcount, soff, surv);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// rows
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_rows_emit(EriArgs A, const long long *__restrict__ soff,
+                                                  const long long *__restrict__ roff, const Surv *__restrict__ surv,
+                                                  RowRec *__restrict__ rows, unsigned *__restrict__ keys,
+                                                  int *__restrict__ vals) {
+    const int lane = threadIdx.x;
+    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+        const Quartet Q = A.q4[qi];
+        int l[4], ncomb;
+        quartet_shape(A, Q, l, ncomb);
+        const long s0 = soff[qi], ns = soff[qi + 1] - s0;
+        const long r0 = roff[qi], nr = ns * A.ndp * ncomb;
+        for (long rl = lane; rl < nr; rl += 64) {
+            const int combo = (int)(rl % ncomb);
+            const int d = (int)((rl / ncomb) % A.ndp);
+            const long g = rl / ((long)ncomb * A.ndp);
+            const Surv S = surv[s0 + g];
+            const PrimPair &pb = A.pp[S.pb];
+            const PrimPair &pk = A.pp[S.pk];
+            int t = combo;
+            const int ia = t % (l[0] + 1); t /= (l[0] + 1);
+            const int ib = t % (l[1] + 1); t /= (l[1] + 1);
+            const int ic = t % (l[2] + 1);
+            const int id = t / (l[2] + 1);
+            const int bN = pb.N[d], kN = pk.N[d];
+            RowRec R;
+            R.nm = bN < kN ? bN : kN;
+            R.gofs = pb.voff[d] + (ia * (l[1] + 1) + ib) * (bN + 1);
+            R.hofs = pk.voff[d] + (ic * (l[3] + 1) + id) * (kN + 1);
+            const double ad = (d == 0) ? A.a[0] : (d == 1) ? A.a[1] : A.a[2];
+            const double shd = A.q4[qi].shift[d];
+            R.phi = ad * (pb.cen[d] - (pk.cen[d] + shd));
+            R.sgn = pow(ad, -(double)(ia + ib + ic + id));
+            R.d = d;
+            rows[r0 + rl] = R;
+            keys[r0 + rl] = ((unsigned)d << 16) | (unsigned)R.nm;
+            vals[r0 + rl] = (int)(r0 + rl);
+        }
+    }
+}
+
+hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                            RowRec *rows, unsigned *keys, int *vals, hipStream_t st) {
+    if (a.nq4 <= 0) return hipSuccess;
+    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
+    hipLaunchKernelGGL(k_rows_emit, dim3(grid), dim3(64), 0, st, a, soff, roff, surv, rows, keys, vals);
+    return hipGetLastError();
+}
+
+// one wavefront per tile of 16 sorted rows; several tiles per wavefront.
+// kNT = node tiles of 16 (2 for the 32-node rule, 3 for 40 nodes).
+template <int kNT>
+__global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__restrict__ rows,
+                                                  const int *__restrict__ order, long long nrows,
+                                                  double *__restrict__ rowval) {
+    const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
+    const int nqp = A.nqp, nh = kNT, nq = A.nq;
+    const int tabrows = A.ncap + 5;
+    const long long ntiles = (nrows + 15) / 16;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long long pos = tile * 16 + r;
+        const bool valid = pos < nrows;
+        int ridx = -1, nm = -1, d = 255;
+        double phi = 0.0, sgn = 1.0;
+        const double2 *Gp = A.vec, *Hp = A.vec;
+        if (valid) {
+            ridx = order[pos];
+            const RowRec R = rows[ridx];
+            nm = R.nm;
+            d = R.d;
+            sgn = R.sgn;
+            phi = R.phi;
+            Gp = A.vec + R.gofs;
+            Hp = A.vec + R.hofs;
+        }
+        if (A.counters && kq == 0 && valid) {
+            long s = 0;
+#pragma unroll 1
+            for (int k = 0; k < nq; ++k) {
+                const int tk = A.tnmax[d * nqp + k];
+                s += (nm < tk ? nm : tk) + 1;
+            }
+            atomicAdd(&A.counters[2], (unsigned long long)(2 * s + 10 * (nm + 1)));
+        }
+        const int dlo = wmin(valid ? d : 255), dhi = wmax(valid ? d : -1);
+        double s4, c4, sn0, cn0;
+        sincos(4.0 * phi, &s4, &c4);
+        sincos((double)kq * phi, &sn0, &cn0);
+#pragma unroll 1
+        for (int dd = dlo; dd <= dhi; ++dd) {
+            const int nmd = (d == dd) ? nm : -1;      // rows of another direction contribute 0
+            const int nmax_t = wmax(nmd);
+            int lim[kNT];
+            int nend = -1;
+#pragma unroll
+            for (int h = 0; h < kNT; ++h) {
+                lim[h] = -1;
+                if (h < nh) {
+                    const int lh = A.tab_lim[dd * nh + h];
+                    lim[h] = lh < nmax_t ? lh : nmax_t;
+                    nend = lim[h] > nend ? lim[h] : nend;
+                }
+            }
+            const double *tab = A.tab + (size_t)dd * tabrows * nqp + r;
+            dbl4 acc[kNT];
+#pragma unroll
+            for (int h = 0; h < kNT; ++h) acc[h] = dbl4{0.0, 0.0, 0.0, 0.0};
+            double cn = cn0, sn = sn0;
+            // loads two steps ahead of their use
+            double2 g0 = {0.0, 0.0}, h0 = {0.0, 0.0}, g1 = {0.0, 0.0}, h1 = {0.0, 0.0};
+            double b0[kNT], b1[kNT];
+#pragma unroll
+            for (int h = 0; h < kNT; ++h) { b0[h] = 0.0; b1[h] = 0.0; }
+            if (kq <= nmd) { g0 = Gp[kq]; h0 = Hp[kq]; }
+            if (4 + kq <= nmd) { g1 = Gp[4 + kq]; h1 = Hp[4 + kq]; }
+#pragma unroll
+            for (int h = 0; h < kNT; ++h) {
+                if (h < nh && 0 <= lim[h]) b0[h] = tab[(size_t)kq * nqp + h * kNodeTile];
+                if (h < nh && 4 <= lim[h]) b1[h] = tab[(size_t)(4 + kq) * nqp + h * kNodeTile];
+            }
+#pragma unroll 1
+            for (int n0 = 0; n0 <= nend; n0 += 4) {
+                const int n = n0 + kq;
+                const int n2 = n + 8;
+                double2 g2 = {0.0, 0.0}, h2 = {0.0, 0.0};
+                double b2[kNT];
+                if (n2 <= nmd) { g2 = Gp[n2]; h2 = Hp[n2]; }
+#pragma unroll
+                for (int h = 0; h < kNT; ++h) {
+                    b2[h] = 0.0;
+                    if (h < nh && n0 + 8 <= lim[h]) b2[h] = tab[(size_t)n2 * nqp + h * kNodeTile];
+                }
+                if (n0 > 0) {
+                    if ((n0 & 63) == 0) {
+                        sincos((double)n * phi, &sn, &cn);
+                    } else {
+                        const double tc = cn * c4 - sn * s4;
+                        sn = sn * c4 + cn * s4;
+                        cn = tc;
+                    }
+                }
+                const double re = g0.x * h0.x + g0.y * h0.y;   // conj(g) * h
+                const double im = g0.x * h0.y - g0.y * h0.x;
+                const double F = (n <= nmd) ? re * cn - im * sn : 0.0;
+#pragma unroll
+                for (int h = 0; h < kNT; ++h)
+                    if (h < nh && n0 <= lim[h]) acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, b0[h], acc[h], 0, 0, 0);
+                g0 = g1; h0 = h1; g1 = g2; h1 = h2;
+#pragma unroll
+                for (int h = 0; h < kNT; ++h) { b0[h] = b1[h]; b1[h] = b2[h]; }
+            }
+            // lane (r, kq) holds D[row = 4v + kq][node = 16h + r]
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * v + kq;
+                const int rrow = __shfl(ridx, row, 64);
+                const int rd = __shfl(d, row, 64);
+                const double sg = __shfl(sgn, row, 64);
+                if (rrow >= 0 && rd == dd) {
+#pragma unroll
+                    for (int h = 0; h < kNT; ++h)
+                        if (h < nh) rowval[(size_t)rrow * nqp + h * kNodeTile + r] = acc[h][v] * sg;
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *order, long long nrows, double *rowval,
+                            hipStream_t st) {
+    if (nrows <= 0) return hipSuccess;
+    const long long ntiles = (nrows + 15) / 16;
+    const int grid = ntiles < (1 << 20) ? (int)ntiles : (1 << 20);
+    switch (a.nqp / kNodeTile) {
+    case 1: hipLaunchKernelGGL(k_rows_mfma<1>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
+    case 2: hipLaunchKernelGGL(k_rows_mfma<2>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
+    case 3: hipLaunchKernelGGL(k_rows_mfma<3>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
+    default: hipLaunchKernelGGL(k_rows_mfma<4>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// contraction and output, one wavefront per shell quartet
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__restrict__ soff,
+                                                 const long long *__restrict__ roff, const Surv *__restrict__ surv,
+                                                 const double *__restrict__ rowval) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int nqp = A.nqp, nq = A.nq, ndp = A.ndp;
+    const int cap = A.max_ncomb > 16 ? A.max_ncomb : 16;
+    double *Ir = lds;                                     // [3 - ndp][cap][nqp]
+    double *blk = lds + (size_t)(3 - ndp) * cap * nqp;    // [max_blk]
+    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+        const Quartet Q = A.q4[qi];
+        const Shell sA = A.sh[Q.s[0]], sB = A.sh[Q.s[1]], sC = A.sh[Q.s[2]], sD = A.sh[Q.s[3]];
+        const int lA = sA.l, lB = sB.l, lC = sC.l, lD = sD.l;
+        const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
+        const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
+        const int nblk = nA * nB * nC * nD;
+        const int NN = lA + lB + lC + lD;
+        const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
+        __syncthreads();
+        for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
+        const int gpb = (cap / ncomb) > 0 ? cap / ncomb : 1;      // survivor groups per batch
+        for (long gb = 0; gb < ns; gb += gpb) {
+            const int ng = (ns - gb < gpb) ? (int)(ns - gb) : gpb;
+            // real directions per (group, node), all power combinations
+            for (int dr = ndp; dr < 3; ++dr) {
+                const int drr = dr - ndp;
+                for (int task = lane; task < ng * nq; task += 64) {
+                    const int gl = task / nq, k = task % nq;
+                    const Surv S = surv[s0 + gb + gl];
+                    const PrimPair &pb = A.pp[S.pb];
+                    const PrimPair &pk = A.pp[S.pk];
+                    const double p = pb.q, q = pk.q;
+                    const double tk = A.tnode[k], t2 = tk * tk;
+                    const double D2 = 1.0 / (p * q + (p + q) * t2);
+                    const double w = p * q * t2 * D2;
+                    const double rs = 3.14159265358979323846 * sqrt(D2);
+                    const double Rr = pb.cen[dr] - (pk.cen[dr] + A.q4[qi].shift[dr]);
+                    double gt[4 * 2 + 1];
+                    gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
+                    {
+                        double gm1 = 0.0, gc = gt[0];
+#pragma unroll
+                        for (int n = 0; n < 8; ++n) {
+                            const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
+                            gt[n + 1] = (n < NN) ? gp1 : 0.0;
+                            gm1 = gc;
+                            gc = gp1;
+                        }
+                    }
+                    const double *Eb = A.herm + pb.eoff + (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
+                    const double *Ek = A.herm + pk.eoff + (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
+                    for (int combo = 0; combo < ncomb; ++combo) {
+                        int t = combo;
+                        const int ia = t % (lA + 1); t /= (lA + 1);
+                        const int ib = t % (lB + 1); t /= (lB + 1);
+                        const int ic = t % (lC + 1);
+                        const int id = t / (lC + 1);
+                        const double *eb = Eb + (ia * (lB + 1) + ib) * (2 * kMaxL + 1);
+                        const double *ek = Ek + (ic * (lD + 1) + id) * (2 * kMaxL + 1);
+                        double v = 0.0;
+#pragma unroll
+                        for (int n = 0; n <= 4; ++n) {
+                            if (n > ia + ib) break;
+                            const double sgn_e = ((n & 1) ? -1.0 : 1.0) * eb[n];
+#pragma unroll
+                            for (int m = 0; m <= 4; ++m) {
+                                if (m > ic + id) break;
+                                v += sgn_e * ek[m] * gt[n + m];
+                            }
+                        }
+                        Ir[((size_t)drr * cap + gl * ncomb + combo) * nqp + k] = v;
+                    }
+                }
+            }
+            __syncthreads();
+            for (int task = lane; task < ng * nblk; task += 64) {
+                const int gl = task / nblk, comp = task % nblk;
+                int t = comp;
+                const int ca = t % nA; t /= nA;
+                const int cb = t % nB; t /= nB;
+                const int cc = t % nC;
+                const int cd = t / nC;
+                const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
+                const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
+                const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
+                const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
+                const double *I3[3];
+                for (int dd = 0; dd < 3; ++dd) {
+                    const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
+                    if (dd < ndp)
+                        I3[dd] = rowval + (size_t)(r0 + ((gb + gl) * ndp + dd) * ncomb + cx) * nqp;
+                    else
+                        I3[dd] = Ir + ((size_t)(dd - ndp) * cap + gl * ncomb + cx) * nqp;
+                }
+                double val = 0.0;
+                for (int k = 0; k < nq; ++k) val += (I3[0][k] * A.tweight[k]) * I3[1][k] * I3[2][k];
+                const Surv S = surv[s0 + gb + gl];
+                const PrimPair &pb = A.pp[S.pb];
+                const PrimPair &pk = A.pp[S.pk];
+                const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
+                const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
+                const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
+                const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
+                atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
+            }
+            if (A.counters && lane == 0)
+                atomicAdd(&A.counters[2], (unsigned long long)ng * ((unsigned long long)ncomb * (3 - ndp) * 2 * nq * (NN + 1) +
+                                                                     3ull * nq * nblk));
+            __syncthreads();
+        }
+        __syncthreads();
+        if (A.blk_out && Q.out >= 0) {
+            for (int i = lane; i < nblk; i += 64) A.blk_out[Q.out + i] = blk[i];
+        } else if (A.eri) {
+            const size_t nb = A.nbas, n1 = A.n1;
+            for (int task = lane; task < nblk * 8; task += 64) {
+                const int comp = task >> 3, ip = task & 7;
+                int t = comp;
+                const int ca = t % nA; t /= nA;
+                const int cb = t % nB; t /= nB;
+                const int cc = t % nC;
+                const int cd = t / nC;
+                const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
+                if (A.Qao[(size_t)f4[0] + nb * f4[1]] * A.Qao[(size_t)f4[2] + nb * f4[3]] < kSchwarz) continue;
+                int t4[4], pp[4];
+                for (int x = 0; x < 4; ++x) t4[x] = f4[c_perm8_rows[ip][x]];
+                const int c0 = t4[0] / A.nfuc;
+                for (int x = 0; x < 4; ++x) pp[x] = A.relc[c0 * A.ncell + t4[x] / A.nfuc] * A.nfuc + t4[x] % A.nfuc;
+                const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * (size_t)pp[3]))
+                                               : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * (size_t)pp[3]));
+                A.eri[idx] = blk[comp];
+            }
+        }
+    }
+}
+
+size_t contract_lds_bytes(const EriArgs &a) {
+    const int cap = a.max_ncomb > 16 ? a.max_ncomb : 16;
+    return sizeof(double) * ((size_t)(3 - a.ndp) * cap * a.nqp + a.max_blk);
+}
+
+hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                           const double *rowval, hipStream_t st) {
+    if (a.nq4 <= 0) return hipSuccess;
+    const size_t lds = contract_lds_bytes(a);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_contract, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
+    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(64), lds, st, a, soff, roff, surv, rowval);
+    return hipGetLastError();
+}
+
+}  // namespace tcg

@@ -1,0 +1,30 @@
+// Device-wide exclusive scan and radix sort (hipCUB) used by the row pipeline.
+#include <hipcub/hipcub.hpp>
+
+#include "tcg_device.hpp"
+
+namespace tcg {
+
+size_t scan_temp_bytes(int n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const long long *)nullptr, (long long *)nullptr, n);
+    return bytes;
+}
+
+hipError_t scan_exclusive(void *tmp, size_t bytes, const long long *in, long long *out, int n, hipStream_t st) {
+    return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, st);
+}
+
+size_t sort_temp_bytes(int n, int end_bit) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
+                                             (const int *)nullptr, (int *)nullptr, n, 0, end_bit);
+    return bytes;
+}
+
+hipError_t sort_pairs(void *tmp, size_t bytes, const unsigned *kin, unsigned *kout, const int *vin, int *vout, int n,
+                      int end_bit, hipStream_t st) {
+    return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout, n, 0, end_bit, st);
+}
+
+}  // namespace tcg
