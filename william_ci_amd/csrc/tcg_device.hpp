@@ -44,7 +44,8 @@ struct EriArgs {
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
                           const int *pair_rep, double *V, hipStream_t st);
-hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, double *ivbuf, double2 *vec, hipStream_t st);
+hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, const VecOut *vo, int nvo, const double2 *dco,
+                               double *ivbuf, double2 *vec, hipStream_t st);
 hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, double Qmax, int nshuc,
                                 int nsh, int ncell, unsigned long long *count, Quartet *out,
                                 unsigned long long cap, hipStream_t st);
@@ -55,13 +56,29 @@ hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long 
 hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *order, long long nrows, double *rowval,
                             hipStream_t st);
 size_t contract_lds_bytes(const EriArgs &a);
+hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv, int *sq,
+                            long long ns, double *realval, hipStream_t st);
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                           const double *rowval, hipStream_t st);
+                           const double *rowval, const double *realval, hipStream_t st);
 size_t scan_temp_bytes(int n);
 hipError_t scan_exclusive(void *tmp, size_t bytes, const long long *in, long long *out, int n, hipStream_t st);
 size_t sort_temp_bytes(int n, int end_bit);
 hipError_t sort_pairs(void *tmp, size_t bytes, const unsigned *kin, unsigned *kout, const int *vin, int *vout, int n,
                       int end_bit, hipStream_t st);
+size_t select_temp_bytes(long long n);
+hipError_t select_flagged(void *tmp, size_t bytes, const unsigned char *flags, long long *out, long long *nsel,
+                          long long n, hipStream_t st);
+size_t sort_ll_temp_bytes(long long n, int end_bit);
+hipError_t sort_pairs_ll(void *tmp, size_t bytes, const unsigned *kin, unsigned *kout, const long long *vin,
+                         long long *vout, long long n, int end_bit, hipStream_t st);
+size_t scan_int_temp_bytes(int n);
+hipError_t scan_int_exclusive(void *tmp, size_t bytes, const int *in, long long *out, int n, hipStream_t st);
+hipError_t launch_col_flags(const double *eri, int n1, int nbas, unsigned char *flags, int *cnt_d, int *cnt_b,
+                            unsigned *key_b, hipStream_t st);
+hipError_t launch_jk_sparse(const double *eri, const double *P, const long long *list_d, const long long *off_d,
+                            const long long *list_b, const long long *off_b, int n1, int nbas, double *J, double *K,
+                            hipStream_t st);
+hipError_t launch_gather_keys(const unsigned *keys, const long long *idx, long long n, unsigned *out, hipStream_t st);
 size_t jk_work_doubles(int n1, int nbas);
 hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,
                             double *jwork, hipStream_t st);

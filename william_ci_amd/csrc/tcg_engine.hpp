@@ -72,6 +72,12 @@ struct RowRec {
     int d;               // periodic direction
 };
 
+// One Fourier vector to assemble: its job (Bessel values), coefficient
+// offset (2K+1 complex d_k) and output offset.
+struct VecOut {
+    int job, dco, voff, K;
+};
+
 struct Shell {
     int first_ao;        // first AO
     int ncomp;           // Cartesian components

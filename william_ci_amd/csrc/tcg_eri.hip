@@ -107,4 +107,102 @@ hipError_t launch_jk_stored(const double *eri, const double *P, double *J, doubl
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Column-sparse J/K (the reference's sparse path,
+// src/Energies/hartree_exchange_potential.f90:54-155): the columns
+// ERI(:,b,c,d) with a non-zero element are listed once per tensor, ordered by
+// d (for K) and by b (for J); each SCF iteration then reads only them.
+//   K(a,d) = -1/2 sum_{(b,c) in list_d} ERI(a,b,c,d) P(b,c)
+//   J(a,b) =      sum_{(c,d) in list_b} ERI(a,b,c,d) P(c,d)
+// One block per d (K) or per b (J): no atomics, deterministic.
+// ---------------------------------------------------------------------------
+__global__ void k_col_flags(const double *__restrict__ eri, int n1, int nbas, unsigned char *__restrict__ flags,
+                            int *__restrict__ cnt_d, int *__restrict__ cnt_b, unsigned *__restrict__ key_b) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ncol = (size_t)nbas * nbas * nbas;
+    if (g >= ncol) return;
+    bool nz = false;
+    for (int a = 0; a < n1; ++a) nz |= eri[(size_t)a + (size_t)n1 * g] != 0.0;
+    flags[g] = nz ? 1 : 0;
+    const int b = (int)(g % nbas), d = (int)(g / ((size_t)nbas * nbas));
+    key_b[g] = (unsigned)b;
+    if (nz) {
+        atomicAdd(&cnt_d[d], 1);
+        atomicAdd(&cnt_b[b], 1);
+    }
+}
+
+__global__ void k_k_sparse(const double *__restrict__ eri, const double *__restrict__ P, const long long *__restrict__ list,
+                           const long long *__restrict__ off, int n1, int nbas, double *__restrict__ K) {
+    extern __shared__ double acc[];      // [n1]
+    const int d = blockIdx.x;
+    for (int a = threadIdx.x; a < n1; a += blockDim.x) acc[a] = 0.0;
+    __syncthreads();
+    double loc[32];
+    const int na = n1 < 32 ? n1 : 32;
+    for (int a = 0; a < na; ++a) loc[a] = 0.0;
+    for (long long i = off[d] + threadIdx.x; i < off[d + 1]; i += blockDim.x) {
+        const long long g = list[i];
+        const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas);
+        const double pbc = P[b + (size_t)nbas * c];
+        const double *e = eri + (size_t)n1 * g;
+        for (int a = 0; a < na; ++a) loc[a] += e[a] * pbc;
+        for (int a = na; a < n1; ++a) atomicAdd(&acc[a], e[a] * pbc);
+    }
+    for (int a = 0; a < na; ++a) atomicAdd(&acc[a], loc[a]);
+    __syncthreads();
+    for (int a = threadIdx.x; a < n1; a += blockDim.x) K[a + (size_t)nbas * d] = -0.5 * acc[a];
+}
+
+__global__ void k_j_sparse(const double *__restrict__ eri, const double *__restrict__ P, const long long *__restrict__ list,
+                           const long long *__restrict__ off, int n1, int nbas, double *__restrict__ J) {
+    extern __shared__ double acc[];      // [n1]
+    const int b = blockIdx.x;
+    for (int a = threadIdx.x; a < n1; a += blockDim.x) acc[a] = 0.0;
+    __syncthreads();
+    double loc[32];
+    const int na = n1 < 32 ? n1 : 32;
+    for (int a = 0; a < na; ++a) loc[a] = 0.0;
+    for (long long i = off[b] + threadIdx.x; i < off[b + 1]; i += blockDim.x) {
+        const long long g = list[i];
+        const int c = (int)((g / nbas) % nbas), d = (int)(g / ((long long)nbas * nbas));
+        const double pcd = P[c + (size_t)nbas * d];
+        const double *e = eri + (size_t)n1 * g;
+        for (int a = 0; a < na; ++a) loc[a] += e[a] * pcd;
+        for (int a = na; a < n1; ++a) atomicAdd(&acc[a], e[a] * pcd);
+    }
+    for (int a = 0; a < na; ++a) atomicAdd(&acc[a], loc[a]);
+    __syncthreads();
+    for (int a = threadIdx.x; a < n1; a += blockDim.x) J[a + (size_t)nbas * b] = acc[a];
+}
+
+hipError_t launch_col_flags(const double *eri, int n1, int nbas, unsigned char *flags, int *cnt_d, int *cnt_b,
+                            unsigned *key_b, hipStream_t st) {
+    const size_t ncol = (size_t)nbas * nbas * nbas;
+    hipLaunchKernelGGL(k_col_flags, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, eri, n1, nbas, flags, cnt_d,
+                       cnt_b, key_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_jk_sparse(const double *eri, const double *P, const long long *list_d, const long long *off_d,
+                            const long long *list_b, const long long *off_b, int n1, int nbas, double *J, double *K,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_k_sparse, dim3(nbas), dim3(256), sizeof(double) * n1, st, eri, P, list_d, off_d, n1, nbas, K);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_j_sparse, dim3(nbas), dim3(256), sizeof(double) * n1, st, eri, P, list_b, off_b, n1, nbas, J);
+    return hipGetLastError();
+}
+
+__global__ void k_gather_keys(const unsigned *__restrict__ keys, const long long *__restrict__ idx, long long n,
+                              unsigned *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = keys[idx[i]];
+}
+
+hipError_t launch_gather_keys(const unsigned *keys, const long long *idx, long long n, unsigned *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, idx, n, out);
+    return hipGetLastError();
+}
+
 }  // namespace tcg

@@ -99,9 +99,13 @@ struct tcg_ctx {
     DevBuf<RowRec> d_rows;
     DevBuf<unsigned> d_keys, d_keys2;
     DevBuf<int> d_vals, d_order;
-    DevBuf<double> d_rowval;
+    DevBuf<double> d_rowval, d_realval;
+    DevBuf<int> d_sq;
     DevBuf<char> d_tmp;
     long long last_surv = 0, last_rows = 0;
+    // column lists of the resident ERI for the sparse J/K
+    bool cols_ready = false;
+    DevBuf<long long> d_list_d, d_list_b, d_off_d, d_off_b;
     size_t eri_count = 0;
     int eri_n1 = 0;
     long long stats[8] = {0};
@@ -403,7 +407,13 @@ static int run_quartets(tcg_ctx *c, EriArgs A) {
     HIPCHK(c, c->d_tmp.alloc(tb));
     HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, 18, c->st));
     HIPCHK(c, launch_rows_mfma(A, c->d_rows.p, c->d_order.p, nr, c->d_rowval.p, c->st));
-    HIPCHK(c, launch_contract(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_rowval.p, c->st));
+    const int nreal = 3 - A.ndp;
+    if (nreal > 0) {
+        HIPCHK(c, c->d_sq.alloc(ns));
+        HIPCHK(c, c->d_realval.alloc((size_t)(nr / A.ndp) * nreal * A.nqp));
+        HIPCHK(c, launch_real_rows(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_sq.p, ns, c->d_realval.p, c->st));
+    }
+    HIPCHK(c, launch_contract(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
     return 0;
 }
 
@@ -453,12 +463,36 @@ static int prepare(tcg_ctx *c) {
     HIPCHK(c, c->d_pp.upload(c->pp));
     HIPCHK(c, c->d_sp.upload(c->sp));
     {
+        // Fourier coefficients d_k of every (job, power pair) on the host, vectors on the GPU
+        std::vector<VecOut> vo;
+        std::vector<double2> dco;
+        std::complex<double> dk[4 * kMaxL + 1];
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            const VecJob &J = jobs[j];
+            for (int ia = 0; ia <= J.lA; ++ia)
+                for (int ib = 0; ib <= J.lB; ++ib) {
+                    const int K = ia + ib;
+                    sin_product_fourier(ia, ib, J.phA, J.phB, dk);
+                    VecOut o;
+                    o.job = (int)j;
+                    o.dco = (int)dco.size();
+                    o.voff = J.voff + (ia * (J.lB + 1) + ib) * (J.N + 1);
+                    o.K = K;
+                    for (int k = 0; k <= 2 * K; ++k) dco.push_back(double2{dk[k].real(), dk[k].imag()});
+                    vo.push_back(o);
+                }
+        }
         DevBuf<VecJob> d_jobs;
+        DevBuf<VecOut> d_vo;
+        DevBuf<double2> d_dco;
         DevBuf<double> d_iv;
         HIPCHK(c, d_jobs.upload(jobs));
+        HIPCHK(c, d_vo.upload(vo));
+        HIPCHK(c, d_dco.upload(dco));
         HIPCHK(c, d_iv.alloc(niv));
         HIPCHK(c, c->d_vec.alloc(nvec));
-        HIPCHK(c, launch_pair_vectors(d_jobs.p, (int)jobs.size(), d_iv.p, c->d_vec.p, c->st));
+        HIPCHK(c, launch_pair_vectors(d_jobs.p, (int)jobs.size(), d_vo.p, (int)vo.size(), d_dco.p, d_iv.p, c->d_vec.p,
+                                      c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
     }
     HIPCHK(c, c->d_herm.upload(herm));
@@ -747,6 +781,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     const int n1 = sao ? nf : nb;
     const size_t count = (size_t)n1 * nb * nb * nb;
     HIPCHK(c, c->d_eri.alloc(count));
+    c->cols_ready = false;
     HIPCHK(c, launch_fill(c->d_eri.p, count, 0.0, c->st));
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
@@ -793,6 +828,7 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
     if (eri) {
         HIPCHK(c, c->d_eri.alloc(count));
         HIPCHK(c, hipMemcpy(c->d_eri.p, eri, sizeof(double) * count, hipMemcpyHostToDevice));
+        c->cols_ready = false;
         c->eri_count = count;
         c->eri_n1 = n1;
         src = c->d_eri.p;
@@ -800,14 +836,56 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
         return c->fail(TCG_E_STATE, "tcg_jk_stored: no resident ERI of this shape (call tcg_eri_build)");
     }
     const size_t nn = (size_t)nbas * nbas;
+    if (!c->cols_ready) {
+        // non-zero columns ERI(:,b,c,d), listed by d and by b (once per tensor)
+        const long long ncol = (long long)nbas * nbas * nbas;
+        DevBuf<unsigned char> flags;
+        DevBuf<int> cnt;
+        DevBuf<unsigned> keyb, keyb2;
+        DevBuf<long long> nsel, tmp_list;
+        HIPCHK(c, flags.alloc(ncol));
+        HIPCHK(c, cnt.alloc(2 * (size_t)nbas + 2));
+        HIPCHK(c, keyb.alloc(ncol));
+        HIPCHK(c, nsel.alloc(1));
+        HIPCHK(c, hipMemsetAsync(cnt.p, 0, sizeof(int) * (2 * (size_t)nbas + 2), c->st));
+        HIPCHK(c, launch_col_flags(src, n1, nbas, flags.p, cnt.p, cnt.p + nbas + 1, keyb.p, c->st));
+        size_t tb = std::max(select_temp_bytes(ncol), scan_int_temp_bytes(nbas + 1));
+        HIPCHK(c, c->d_tmp.alloc(tb));
+        HIPCHK(c, tmp_list.alloc(ncol));
+        HIPCHK(c, select_flagged(c->d_tmp.p, tb, flags.p, tmp_list.p, nsel.p, ncol, c->st));
+        long long nz = 0;
+        HIPCHK(c, hipMemcpyAsync(&nz, nsel.p, sizeof nz, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, c->d_off_d.alloc(nbas + 1));
+        HIPCHK(c, c->d_off_b.alloc(nbas + 1));
+        HIPCHK(c, scan_int_exclusive(c->d_tmp.p, tb, cnt.p, c->d_off_d.p, nbas + 1, c->st));
+        HIPCHK(c, scan_int_exclusive(c->d_tmp.p, tb, cnt.p + nbas + 1, c->d_off_b.p, nbas + 1, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, c->d_list_d.alloc(nz > 0 ? nz : 1));
+        HIPCHK(c, c->d_list_b.alloc(nz > 0 ? nz : 1));
+        if (nz > 0) {
+            HIPCHK(c, hipMemcpyAsync(c->d_list_d.p, tmp_list.p, sizeof(long long) * nz, hipMemcpyDeviceToDevice, c->st));
+            // keys b of the selected columns, then a stable sort by b
+            std::vector<long long> dummy;
+            HIPCHK(c, keyb2.alloc(nz));
+            DevBuf<unsigned> kb_sel;
+            HIPCHK(c, kb_sel.alloc(nz));
+            HIPCHK(c, launch_gather_keys(keyb.p, c->d_list_d.p, nz, kb_sel.p, c->st));
+            const int bits = (int)std::ceil(std::log2((double)nbas + 1.0)) + 1;
+            size_t tb2 = sort_ll_temp_bytes(nz, bits);
+            HIPCHK(c, c->d_tmp.alloc(tb2));
+            HIPCHK(c, sort_pairs_ll(c->d_tmp.p, tb2, kb_sel.p, keyb2.p, c->d_list_d.p, c->d_list_b.p, nz, bits, c->st));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->cols_ready = true;
+    }
     HIPCHK(c, c->d_P.alloc(nn));
     HIPCHK(c, c->d_J.alloc(nn));
     HIPCHK(c, c->d_K.alloc(nn));
-    HIPCHK(c, c->d_jwork.alloc(jk_work_doubles(n1, nbas)));
     HIPCHK(c, hipMemcpyAsync(c->d_P.p, P, sizeof(double) * nn, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, launch_fill(c->d_J.p, nn, 0.0, c->st));
     HIPCHK(c, launch_fill(c->d_K.p, nn, 0.0, c->st));
-    HIPCHK(c, launch_jk_stored(src, c->d_P.p, c->d_J.p, c->d_K.p, n1, nbas, c->d_jwork.p, c->st));
+    HIPCHK(c, launch_jk_sparse(src, c->d_P.p, c->d_list_d.p, c->d_off_d.p, c->d_list_b.p, c->d_off_b.p, n1, nbas,
+                               c->d_J.p, c->d_K.p, c->st));
     HIPCHK(c, hipMemcpyAsync(J, c->d_J.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(K, c->d_K.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));

@@ -17,8 +17,9 @@
 //                 with v_mfma_f64_16x16x4f64 (F on the VALU as the A operand,
 //                 the t-table as B), loads two steps ahead of use.  Result:
 //                 32 node values per row, in HBM.
-//   k_contract    one wavefront per shell quartet: real directions (Hermite
-//                 form per node), sum_k w_k Ix Iy Iz, coefficients, the shell
+//   k_real_rows   one thread per (primitive quartet, real direction, node):
+//                 the Hermite-form factors of all power combinations.
+//   k_contract    one wavefront per shell quartet: sum_k w_k Ix Iy Iz, coefficients, the shell
 //                 block in LDS, then the 8-permutation scatter translated to
 //                 the unit cell (src/Toroidal/ERI_toroidal.f90:416-454) or the
 //                 block itself (Schwarz passes).
@@ -288,17 +289,105 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
 }
 
 // ---------------------------------------------------------------------------
+// real directions: one thread per (surviving primitive quartet, real direction,
+// node): the McMurchie-Davidson factor of every power combination,
+//   I(t) = sum_N S_N g_N(t),  S_N = sum (-1)^tau E_tau E'_nu,
+//   g_0 = pi sqrt(D) e^{-w R^2}, g_{N+1} = 2 w R g_N - 2 N w g_{N-1}
+// (ERI_shell_block_toroidal.f90:228-236,302-335), written next to the
+// periodic rows of the same quartet: with R = roff[q] / ndp (the quartet's
+// first (group, combination) index) and g the group inside the quartet,
+//   realval[(R * nreal + (g * nreal + dr) * ncomb + combo) * nqp + k].
+// ---------------------------------------------------------------------------
+__global__ void k_real_rows(EriArgs A, const long long *__restrict__ soff, const long long *__restrict__ roff,
+                            const Surv *__restrict__ surv, const int *__restrict__ sq, long long ns,
+                            double *__restrict__ realval) {
+    const int nreal = 3 - A.ndp, nq = A.nq, nqp = A.nqp;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns * nreal * nq) return;
+    const int k = (int)(t % nq);
+    const int drr = (int)((t / nq) % nreal);
+    const long long g = t / ((long long)nq * nreal);
+    const int dr = A.ndp + drr;
+    const int qi = sq[g];
+    const Quartet *Q = A.q4 + qi;
+    const int lA = A.sh[Q->s[0]].l, lB = A.sh[Q->s[1]].l, lC = A.sh[Q->s[2]].l, lD = A.sh[Q->s[3]].l;
+    const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
+    const int NN = lA + lB + lC + lD;
+    const Surv S = surv[g];
+    const PrimPair &pb = A.pp[S.pb];
+    const PrimPair &pk = A.pp[S.pk];
+    const double p = pb.q, q = pk.q;
+    const double tk = A.tnode[k], t2 = tk * tk;
+    const double D2 = 1.0 / (p * q + (p + q) * t2);
+    const double w = p * q * t2 * D2;
+    const double rs = 3.14159265358979323846 * sqrt(D2);
+    const double Rr = pb.cen[dr] - (pk.cen[dr] + Q->shift[dr]);
+    double gt[4 * 2 + 1];
+    gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
+    {
+        double gm1 = 0.0, gc = gt[0];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
+            gt[n + 1] = (n < NN) ? gp1 : 0.0;
+            gm1 = gc;
+            gc = gp1;
+        }
+    }
+    const double *Eb = A.herm + pb.eoff + (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
+    const double *Ek = A.herm + pk.eoff + (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
+    const long long gl = g - soff[qi];
+    const long long R0 = roff[qi] / A.ndp;
+    double *out = realval + ((size_t)(R0 * nreal + (gl * nreal + drr) * ncomb)) * nqp + k;
+    for (int combo = 0; combo < ncomb; ++combo) {
+        int c = combo;
+        const int ia = c % (lA + 1); c /= (lA + 1);
+        const int ib = c % (lB + 1); c /= (lB + 1);
+        const int ic = c % (lC + 1);
+        const int id = c / (lC + 1);
+        const double *eb = Eb + (ia * (lB + 1) + ib) * (2 * kMaxL + 1);
+        const double *ek = Ek + (ic * (lD + 1) + id) * (2 * kMaxL + 1);
+        double v = 0.0;
+#pragma unroll
+        for (int n = 0; n <= 4; ++n) {
+            if (n > ia + ib) break;
+            const double se = ((n & 1) ? -1.0 : 1.0) * eb[n];
+#pragma unroll
+            for (int m = 0; m <= 4; ++m) {
+                if (m > ic + id) break;
+                v += se * ek[m] * gt[n + m];
+            }
+        }
+        out[(size_t)combo * nqp] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_surv_quartet(const long long *__restrict__ soff, int nq4, int *__restrict__ sq) {
+    for (int qi = blockIdx.x; qi < nq4; qi += gridDim.x)
+        for (long long g = soff[qi] + threadIdx.x; g < soff[qi + 1]; g += 64) sq[g] = qi;
+}
+
+hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv, int *sq,
+                            long long ns, double *realval, hipStream_t st) {
+    if (a.ndp >= 3 || ns <= 0) return hipSuccess;
+    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
+    hipLaunchKernelGGL(k_surv_quartet, dim3(grid), dim3(64), 0, st, soff, a.nq4, sq);
+    const long long n = ns * (3 - a.ndp) * a.nq;
+    hipLaunchKernelGGL(k_real_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, soff, roff, surv, sq, ns,
+                       realval);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // contraction and output, one wavefront per shell quartet
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__restrict__ soff,
                                                  const long long *__restrict__ roff, const Surv *__restrict__ surv,
-                                                 const double *__restrict__ rowval) {
-    extern __shared__ double lds[];
+                                                 const double *__restrict__ rowval,
+                                                 const double *__restrict__ realval) {
+    extern __shared__ double blk[];                        // [max_blk]
     const int lane = threadIdx.x;
-    const int nqp = A.nqp, nq = A.nq, ndp = A.ndp;
-    const int cap = A.max_ncomb > 16 ? A.max_ncomb : 16;
-    double *Ir = lds;                                     // [3 - ndp][cap][nqp]
-    double *blk = lds + (size_t)(3 - ndp) * cap * nqp;    // [max_blk]
+    const int nqp = A.nqp, nq = A.nq, ndp = A.ndp, nreal = 3 - ndp;
     for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
         const Quartet Q = A.q4[qi];
         const Shell sA = A.sh[Q.s[0]], sB = A.sh[Q.s[1]], sC = A.sh[Q.s[2]], sD = A.sh[Q.s[3]];
@@ -310,96 +399,42 @@ __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__r
         const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
         __syncthreads();
         for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
-        const int gpb = (cap / ncomb) > 0 ? cap / ncomb : 1;      // survivor groups per batch
-        for (long gb = 0; gb < ns; gb += gpb) {
-            const int ng = (ns - gb < gpb) ? (int)(ns - gb) : gpb;
-            // real directions per (group, node), all power combinations
-            for (int dr = ndp; dr < 3; ++dr) {
-                const int drr = dr - ndp;
-                for (int task = lane; task < ng * nq; task += 64) {
-                    const int gl = task / nq, k = task % nq;
-                    const Surv S = surv[s0 + gb + gl];
-                    const PrimPair &pb = A.pp[S.pb];
-                    const PrimPair &pk = A.pp[S.pk];
-                    const double p = pb.q, q = pk.q;
-                    const double tk = A.tnode[k], t2 = tk * tk;
-                    const double D2 = 1.0 / (p * q + (p + q) * t2);
-                    const double w = p * q * t2 * D2;
-                    const double rs = 3.14159265358979323846 * sqrt(D2);
-                    const double Rr = pb.cen[dr] - (pk.cen[dr] + A.q4[qi].shift[dr]);
-                    double gt[4 * 2 + 1];
-                    gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
-                    {
-                        double gm1 = 0.0, gc = gt[0];
+        __syncthreads();
+        for (long task = lane; task < ns * nblk; task += 64) {
+            const long g = task / nblk;
+            const int comp = (int)(task % nblk);
+            int t = comp;
+            const int ca = t % nA; t /= nA;
+            const int cb = t % nB; t /= nB;
+            const int cc = t % nC;
+            const int cd = t / nC;
+            const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
+            const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
+            const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
+            const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
+            const double *I3[3];
 #pragma unroll
-                        for (int n = 0; n < 8; ++n) {
-                            const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
-                            gt[n + 1] = (n < NN) ? gp1 : 0.0;
-                            gm1 = gc;
-                            gc = gp1;
-                        }
-                    }
-                    const double *Eb = A.herm + pb.eoff + (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
-                    const double *Ek = A.herm + pk.eoff + (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
-                    for (int combo = 0; combo < ncomb; ++combo) {
-                        int t = combo;
-                        const int ia = t % (lA + 1); t /= (lA + 1);
-                        const int ib = t % (lB + 1); t /= (lB + 1);
-                        const int ic = t % (lC + 1);
-                        const int id = t / (lC + 1);
-                        const double *eb = Eb + (ia * (lB + 1) + ib) * (2 * kMaxL + 1);
-                        const double *ek = Ek + (ic * (lD + 1) + id) * (2 * kMaxL + 1);
-                        double v = 0.0;
-#pragma unroll
-                        for (int n = 0; n <= 4; ++n) {
-                            if (n > ia + ib) break;
-                            const double sgn_e = ((n & 1) ? -1.0 : 1.0) * eb[n];
-#pragma unroll
-                            for (int m = 0; m <= 4; ++m) {
-                                if (m > ic + id) break;
-                                v += sgn_e * ek[m] * gt[n + m];
-                            }
-                        }
-                        Ir[((size_t)drr * cap + gl * ncomb + combo) * nqp + k] = v;
-                    }
-                }
+            for (int dd = 0; dd < 3; ++dd) {
+                const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
+                if (dd < ndp)
+                    I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
+                else
+                    I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
             }
-            __syncthreads();
-            for (int task = lane; task < ng * nblk; task += 64) {
-                const int gl = task / nblk, comp = task % nblk;
-                int t = comp;
-                const int ca = t % nA; t /= nA;
-                const int cb = t % nB; t /= nB;
-                const int cc = t % nC;
-                const int cd = t / nC;
-                const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
-                const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
-                const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
-                const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
-                const double *I3[3];
-                for (int dd = 0; dd < 3; ++dd) {
-                    const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
-                    if (dd < ndp)
-                        I3[dd] = rowval + (size_t)(r0 + ((gb + gl) * ndp + dd) * ncomb + cx) * nqp;
-                    else
-                        I3[dd] = Ir + ((size_t)(dd - ndp) * cap + gl * ncomb + cx) * nqp;
-                }
-                double val = 0.0;
-                for (int k = 0; k < nq; ++k) val += (I3[0][k] * A.tweight[k]) * I3[1][k] * I3[2][k];
-                const Surv S = surv[s0 + gb + gl];
-                const PrimPair &pb = A.pp[S.pb];
-                const PrimPair &pk = A.pp[S.pk];
-                const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
-                const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
-                const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
-                const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
-                atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
-            }
-            if (A.counters && lane == 0)
-                atomicAdd(&A.counters[2], (unsigned long long)ng * ((unsigned long long)ncomb * (3 - ndp) * 2 * nq * (NN + 1) +
-                                                                     3ull * nq * nblk));
-            __syncthreads();
+            double val = 0.0;
+            for (int k = 0; k < nq; ++k) val += (I3[0][k] * A.tweight[k]) * I3[1][k] * I3[2][k];
+            const Surv S = surv[s0 + g];
+            const PrimPair &pb = A.pp[S.pb];
+            const PrimPair &pk = A.pp[S.pk];
+            const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
+            const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
+            const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
+            const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
+            atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
         }
+        if (A.counters && lane == 0)
+            atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)ncomb * nreal * 2 * nq * (NN + 1) +
+                                                                3ull * nq * nblk));
         __syncthreads();
         if (A.blk_out && Q.out >= 0) {
             for (int i = lane; i < nblk; i += 64) A.blk_out[Q.out + i] = blk[i];
@@ -426,13 +461,10 @@ __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__r
     }
 }
 
-size_t contract_lds_bytes(const EriArgs &a) {
-    const int cap = a.max_ncomb > 16 ? a.max_ncomb : 16;
-    return sizeof(double) * ((size_t)(3 - a.ndp) * cap * a.nqp + a.max_blk);
-}
+size_t contract_lds_bytes(const EriArgs &a) { return sizeof(double) * (size_t)a.max_blk; }
 
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                           const double *rowval, hipStream_t st) {
+                           const double *rowval, const double *realval, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
     const size_t lds = contract_lds_bytes(a);
     if (lds > 64 * 1024) {
@@ -440,7 +472,7 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
         if (e != hipSuccess) return e;
     }
     const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
-    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(64), lds, st, a, soff, roff, surv, rowval);
+    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(64), lds, st, a, soff, roff, surv, rowval, realval);
     return hipGetLastError();
 }
 
