@@ -10,9 +10,11 @@ i.e. the work of the reference's ERI_integral_toroidal
 Prints one JSON line.  ``value`` = nBas^4/8-equivalent unique quartets
 delivered per second (the unit of the workload's "1.3e9 unique quartets");
 the evaluated shell quartets per second are reported beside it.
-``roofline`` prices the quartet kernel's algorithmic FP64 flops (counted by
-the kernel itself) against MI355X's FP64 peak, with the kernel's duration
-measured by HIP events on the engine's stream.  ``cpu_baseline`` times the
+``roofline`` prices the dominant kernel, the row MFMA kernel k_rows_mfma
+(I_d(t_k) = sum_n F_n T_d[n][k] on v_mfma_f64_16x16x4f64): its algorithmic
+FP64 flops (counted by the kernel itself, per row 2 sum_k min(nm, t_nmax(k))
++ 10 (nm+1)) over its duration, measured by HIP events around its launch on
+the engine's stream, against MI355X's FP64 matrix peak.  ``cpu_baseline`` times the
 reference's own Fortran ERI driver (oracle/_ref, OpenMP) on this host's cores.
 """
 from __future__ import annotations
@@ -34,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 def _traffic_from_profiles(workload: str):
     """HBM bytes per quartet-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", f"eri_quartets_{workload}_pmc.txt")
+    path = os.path.join(ROOT, "profiles", f"k_rows_mfma_{workload}_pmc.txt")
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
@@ -124,7 +126,7 @@ def main():
         eng.refresh_basis(coef)               # every step rebuilds tables and pair data from the AO list
         eng.eri_build(to_host=False)          # synchronous: returns after the GPU work completed
         steps.append(time.perf_counter() - ts)
-        kms.append(eng.times()["kernel_ms"])
+        kms.append(eng.times()["rows_kernel_ms"])
     el = time.perf_counter() - t0
     st = eng.stats()
     tm = eng.times()
@@ -143,7 +145,7 @@ def main():
     unique = nb ** 4 / 8
     ms_per_step = el / args.steps * 1e3
     kernel_ms = sum(kms) / len(kms)
-    achieved = st["flops"] / (kernel_ms * 1e-3) / 1e12
+    achieved = st["row_flops"] / (kernel_ms * 1e-3) / 1e12
     traffic = _traffic_from_profiles(args.workload)
     line = {
         "metric": "Clifford ERI quartets/s (FP64 ERI build)",
@@ -165,7 +167,8 @@ def main():
                 "jk_s": res.times["jk_s"], "eri_plus_rhf_wall_s": rhf_wall + ms_per_step / 1e3},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_eri_quartets", "kernel_ms": kernel_ms, "flops_per_launch": st["flops"]},
+                     "kernel": "k_rows_mfma", "kernel_ms": kernel_ms, "flops_per_launch": st["row_flops"],
+                     "all_quartet_flops": st["flops"]},
     }
     if not args.no_cpu_baseline:
         cb = cpu_baseline(s, coef, args.cpu_seconds)

@@ -79,6 +79,7 @@ int tcg_one_electron(tcg_ctx *ctx, int natom, const double *xyz, const int *Z, d
  *   out[3] algorithmic FP64 flops of the quartet kernel: per primitive quartet
  *          ncomb*[sum_{d periodic}(2 sum_k min(nm_d, t_nmax(k,d)) + 10(nm_d+1))
  *                 + sum_{d real} 2 nq (NN+1)] + 3 nq ncomp4
+ *   out[4] the periodic-direction part of out[3] (the row MFMA kernel alone)
  * n = number of slots the caller provides (up to 8). */
 int tcg_eri_stats(tcg_ctx *ctx, long long *out, int n);
 
@@ -86,7 +87,8 @@ int tcg_eri_stats(tcg_ctx *ctx, long long *out, int n);
  * analogue of the reference's tstep table (src/module/clifford_tools.f90:816-852):
  *   out[0] tables + pairs   out[1] Schwarz + primitive screen
  *   out[2] quartet list     out[3] quartet kernel + scatter   out[4] total
- *   out[5] duration of the quartet kernel alone (HIP events on the engine's stream) */
+ *   out[5] duration of the quartet phase on the GPU (HIP events on the engine's stream)
+ *   out[6] duration of the row MFMA kernel alone (HIP events around its launch) */
 int tcg_eri_times(tcg_ctx *ctx, double *out, int n);
 
 /* Device address of the resident ERI tensor (for callers that keep data on

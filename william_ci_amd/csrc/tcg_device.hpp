@@ -38,7 +38,7 @@ struct EriArgs {
     const double *Qao;                // [nbas*nbas] function-pair Schwarz values
     const int *relc;                  // [ncell*ncell]
     // counters
-    unsigned long long *counters;     // [0] primitive quartets, [1] rows, [2] algorithmic flops
+    unsigned long long *counters;     // [2] contraction flops, [3] quartet count, [4] row (MFMA) flops
 };
 
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);

@@ -86,7 +86,8 @@ struct tcg_ctx {
     hipStream_t st = nullptr;
     DevBuf<double> d_tab, d_tnode, d_tw, d_coef, d_herm, d_blk, d_Q, d_Qsh, d_eri, d_P, d_J, d_K, d_jwork;
     DevBuf<int> d_tablim, d_tnmax, d_lxyz, d_poff, d_relc;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    bool time_rows = false;    // record ev2/ev3 around the row MFMA kernel
     DevBuf<Shell> d_sh;
     DevBuf<PrimPair> d_pp;
     DevBuf<ShellPair> d_sp;
@@ -406,7 +407,9 @@ static int run_quartets(tcg_ctx *c, EriArgs A) {
     tb = sort_temp_bytes((int)nr, 18);
     HIPCHK(c, c->d_tmp.alloc(tb));
     HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, 18, c->st));
+    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev2, c->st));
     HIPCHK(c, launch_rows_mfma(A, c->d_rows.p, c->d_order.p, nr, c->d_rowval.p, c->st));
+    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev3, c->st));
     const int nreal = 3 - A.ndp;
     if (nreal > 0) {
         HIPCHK(c, c->d_sq.alloc(ns));
@@ -758,7 +761,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     }
     HIPCHK(c, c->d_pp.upload(c->pp));
     HIPCHK(c, c->d_Q.upload(Q));
-    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 3 * sizeof(unsigned long long), c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
     const double t2 = now_ms();
 
     // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened, on the GPU ----
@@ -794,21 +797,28 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     if (!c->ev0) {
         HIPCHK(c, hipEventCreate(&c->ev0));
         HIPCHK(c, hipEventCreate(&c->ev1));
+        HIPCHK(c, hipEventCreate(&c->ev2));
+        HIPCHK(c, hipEventCreate(&c->ev3));
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    c->time_rows = true;
     rc = run_quartets(c, A);
+    c->time_rows = false;
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    float kms = 0.f;
+    float kms = 0.f, rms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
     c->times[5] = kms;
+    if (c->last_rows > 0) HIPCHK(c, hipEventElapsedTime(&rms, c->ev2, c->ev3));
+    c->times[6] = rms;
     const double t4 = now_ms();
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HIPCHK(c, hipMemcpy(cnt, c->d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost));
     c->stats[1] = c->last_surv;
     c->stats[2] = c->last_rows;
-    c->stats[3] = (long long)cnt[2];
+    c->stats[3] = (long long)(cnt[2] + cnt[4]);
+    c->stats[4] = (long long)cnt[4];
     c->eri_count = count;
     c->eri_n1 = n1;
     if (eri) HIPCHK(c, hipMemcpy(eri, c->d_eri.p, sizeof(double) * count, hipMemcpyDeviceToHost));
@@ -959,6 +969,8 @@ int tcg_destroy(tcg_ctx *c) {
     if (!c) return TCG_E_ARG;
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return TCG_OK;

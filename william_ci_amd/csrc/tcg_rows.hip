@@ -188,7 +188,7 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
                 const int tk = A.tnmax[d * nqp + k];
                 s += (nm < tk ? nm : tk) + 1;
             }
-            atomicAdd(&A.counters[2], (unsigned long long)(2 * s + 10 * (nm + 1)));
+            atomicAdd(&A.counters[4], (unsigned long long)(2 * s + 10 * (nm + 1)));
         }
         const int dlo = wmin(valid ? d : 255), dhi = wmax(valid ? d : -1);
         double s4, c4, sn0, cn0;

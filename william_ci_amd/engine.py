@@ -99,13 +99,13 @@ class Engine:
         out = np.zeros(8, dtype=np.int64)
         lib().tcg_eri_stats(self.h, out, 8)
         return {"shell_quartets": int(out[0]), "prim_quartets": int(out[1]), "rows": int(out[2]),
-                "flops": int(out[3])}
+                "flops": int(out[3]), "row_flops": int(out[4])}
 
     def times(self):
         out = np.zeros(8)
         lib().tcg_eri_times(self.h, out, 8)
         return {"tables_pairs_ms": out[0], "schwarz_ms": out[1], "quartet_list_ms": out[2],
-                "quartets_ms": out[3], "total_ms": out[4], "kernel_ms": out[5]}
+                "quartets_ms": out[3], "total_ms": out[4], "kernel_ms": out[5], "rows_kernel_ms": out[6]}
 
     def eri_device(self):
         p = _vp()
