@@ -7,6 +7,11 @@ kernel, scatter into the SAO layout ERI(nfuc,nBas,nBas,nBas) resident in HBM),
 i.e. the work of the reference's ERI_integral_toroidal
 (src/Toroidal/ERI_toroidal.f90:36) for the Tori1D H64 chain (320 functions).
 
+The start-up fast_quad self-check runs first, as in the reference
+(src/calculation_type/Tori1D.f90:88): on the H chains it finds the 32-point
+rule 1.9e-9 away from the 64-point table and switches the run to the table,
+so both the GPU steps and the CPU baseline use the 64-point table.
+
 Prints one JSON line.  ``value`` = nBas^4/8-equivalent unique quartets
 delivered per second (the unit of the workload's "1.3e9 unique quartets");
 the evaluated shell quartets per second are reported beside it.
@@ -43,16 +48,18 @@ def _traffic_from_profiles(workload: str):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(sysm, coef_unused, seconds: float):
+def cpu_baseline(sysm, quad, seconds: float):
     """The reference's own ERI driver (Fortran + OpenMP, oracle/_ref) on a
-    bounded sample: repeated full builds of the same workload for ~seconds."""
+    bounded sample: repeated full builds of the same workload for ~seconds,
+    on the same quadrature rule as the GPU (quad = (fast, nquad, t, w))."""
+    fast, nquad, tq, wq = quad
     try:
         from oracle import ref
     except Exception:
         ref = None
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if ref is not None and ref.available():
-        r = ref.RefSystem(sysm)
+        r = ref.RefSystem(sysm, nq=nquad, fast_quad=fast)
         n, t0 = 0, time.perf_counter()
         while True:
             r.eri_full()
@@ -63,12 +70,13 @@ def cpu_baseline(sysm, coef_unused, seconds: float):
         nb = sysm.nbas
         return {"value": n * nb ** 4 / 8 / el, "unit": "unique quartets/s (nBas^4/8 per build)",
                 "cores": threads, "kind": "reference",
-                "sample": f"{n} full ERI builds of {sysm.kind} {sysm.nbas}-function workload "
+                "sample": f"{n} full ERI builds of {sysm.kind} {sysm.nbas}-function workload, {nquad}-point "
+                          f"{'fast' if fast else 'table'} rule "
                           f"(reference Fortran ERI_integral_toroidal, OpenMP, {el:.1f} s)",
                 "seconds_per_build": el / n}
     from oracle import orc
     from william_ci_amd.basis import normalize
-    o = orc.OracleSystem(sysm, normalize(sysm))
+    o = orc.OracleSystem(sysm, normalize(sysm), nq=nquad, nodes=None if fast else (tq, wq))
     n, t0 = 0, time.perf_counter()
     while True:
         o.eri_full()
@@ -91,6 +99,8 @@ def main():
     ap.add_argument("--workload", default="h64_chain")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-self-check", action="store_true",
+                    help="keep the 32-point fast rule (skip the start-up fast_quad self-check)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -116,6 +126,13 @@ def main():
     s = tsys.workload(args.workload)
     coef = normalize(s)
     eng = Engine(s, coef)
+    # start-up self-check of the fast rule (the reference runs it before any
+    # integral, src/calculation_type/Tori1D.f90:88): it keeps the 32-point rule
+    # or switches the whole run to the 64-point table
+    t_sc = time.perf_counter()
+    selfcheck = None if args.no_self_check else eng.fast_quad_self_check(64)
+    t_sc = time.perf_counter() - t_sc
+    quad = eng.quadrature()
     for _ in range(args.warmup):
         eng.refresh_basis(coef)
         eng.eri_build(to_host=False)
@@ -156,8 +173,11 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (deterministic H chain, no external data)",
         "config": {"workload": args.workload, "kind": s.kind, "nbas": nb, "nfuc": s.nfuc,
-                   "cells": s.n_cell, "L": s.L[0], "nquad": 32, "layout": "SAO ERI(nfuc,nBas^3)",
-                   "parallelism": "single GPU"},
+                   "cells": s.n_cell, "L": s.L[0], "nquad": quad[1],
+                   "quadrature": ("fast_quad %d-point mapped Gauss-Legendre" % quad[1]) if quad[0]
+                   else "%d-point table (fast_quad self-check: above 1e-10)" % quad[1],
+                   "layout": "SAO ERI(nfuc,nBas^3)", "parallelism": "single GPU"},
+        "self_check": None if selfcheck is None else dict(selfcheck, seconds=t_sc),
         "shell_quartets_per_s": st["shell_quartets"] * args.steps / el,
         "evaluated": {"shell_quartets": st["shell_quartets"], "prim_quartets": st["prim_quartets"],
                       "rows": st["rows"]},
@@ -171,7 +191,7 @@ def main():
                      "all_quartet_flops": st["flops"]},
     }
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(s, coef, args.cpu_seconds)
+        cb = cpu_baseline(s, quad, args.cpu_seconds)
         cb["gpu_speedup"] = line["value"] / cb["value"]
         line["cpu_baseline"] = cb
     else:

@@ -73,6 +73,30 @@ int tcg_jk_stored(tcg_ctx *ctx, int n1, int nbas, const double *P, const double 
  * xyz[3*c+d] nuclear positions, Z[c] charges (the supercell's atoms). */
 int tcg_one_electron(tcg_ctx *ctx, int natom, const double *xyz, const int *Z, double *S, double *T, double *V);
 
+/* Quadrature rule of every later call.  Replaces the module state
+ *   fast_quad, nquad_point            src/module/precompute_t.f90:9,13
+ * and the node source gauss_legendre_points (src/module/precompute_t.f90:484-895):
+ *   fast != 0: nquad mapped Gauss-Legendre points t = 0.6 u/(1-u) (4..128;
+ *              the reference uses 32, or 40 with nquad_128)  fast_quad_points :906-942
+ *   fast == 0: the literal 64- or 128-point table (nquad 64 or 128). */
+int tcg_set_quadrature(tcg_ctx *ctx, int fast, int nquad);
+
+/* Current rule; t, w (nquad each, may be NULL) receive its nodes and weights. */
+int tcg_get_quadrature(tcg_ctx *ctx, int *fast, int *nquad, double *t, double *w);
+
+/* Start-up self-check of the fast rule.  Replaces
+ *   fast_quad_self_check(ndp,nBas,AO,n_atoms,geometry,atoms)
+ *                                   src/Toroidal/fast_quad_self_check.f90:30-233
+ * Sample: every on-site shell quartet of the unit cell plus 50 LCG quartets
+ * (seed 12345), V(i,i) for the first function of each unit-cell shell plus
+ * 2 x 20 LCG pairs; each with the current fast rule and with the nq_full
+ * (64 or 128) table.  *deri, *dv = largest differences; if both are <= 1e-10
+ * the fast rule stays (*fast_on = 1), otherwise the table rule is left in
+ * place for the rest of the run (*fast_on = 0).  With the table rule already
+ * in place it returns at once with *fast_on = 0. */
+int tcg_fast_quad_self_check(tcg_ctx *ctx, int natom, const double *xyz, const int *Z, int nq_full,
+                             double *deri, double *dv, int *fast_on);
+
 /* Counters of the last tcg_eri_build:
  *   out[0] shell quartets evaluated   out[1] primitive quartets evaluated
  *   out[2] rows (primitive quartet x direction x power combination)

@@ -35,6 +35,7 @@ def lib():
         L.orc_nmax_est.argtypes = [_cd]
         L.orc_nmax_est.restype = _ci
         L.orc_fast_quad_points.argtypes = [_ci, _dp, _dp]
+        L.orc_set_nodes.argtypes = [_ci, _dp, _dp]
         L.orc_init.argtypes = [_ci, _dp, _ip, _ci, _ci]
         L.orc_set_basis.argtypes = [_ci, _dp, _ip, _ip, _dp, _dp]
         L.orc_setup_tables.restype = _ci
@@ -73,11 +74,18 @@ class OracleSystem:
     """A torus system loaded into the oracle.  ``coef`` must be the normalised
     contraction coefficients (the engine's and the reference's inputs)."""
 
-    def __init__(self, sysm, coef: np.ndarray, nq: int = 32):
+    def __init__(self, sysm, coef: np.ndarray, nq: int = 32, nodes=None):
+        """nodes = (t, w): an explicit rule (the 64/128-point tables) instead
+        of the nq-point fast rule."""
         self.s = sysm
         L = lib()
         L.orc_init(sysm.ndp, np.array(sysm.L, dtype=np.float64), np.array(sysm.ncell, dtype=np.int32),
                    sysm.nfuc, nq)
+        if nodes is not None:
+            t, w = (np.ascontiguousarray(x, dtype=np.float64) for x in nodes)
+            nq = len(t)
+            if L.orc_set_nodes(nq, t, w) != 0:
+                raise ValueError(f"oracle: {nq} nodes not supported")
         L.orc_set_basis(sysm.nbas, np.ascontiguousarray(sysm.center), np.ascontiguousarray(sysm.lxyz),
                         np.ascontiguousarray(sysm.prim_off), np.ascontiguousarray(sysm.expo),
                         np.ascontiguousarray(coef, dtype=np.float64))

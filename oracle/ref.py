@@ -40,6 +40,7 @@ def lib():
         L.ref_one_electron.argtypes = [_ci, _dp, _ip, _dp, _dp, _dp]
         L.ref_nre.argtypes = [_ci, _dp, _ip, ctypes.POINTER(ctypes.c_double)]
         L.ref_jk.argtypes = [_ci, _ci, _dp, _dp, _dp, _dp]
+        L.ref_fast_quad_self_check.argtypes = [_ci, _dp, _ip, _ci, ctypes.c_char_p, _dp]
         _lib = L
     return _lib
 
@@ -102,6 +103,24 @@ class RefSystem:
         lib().ref_one_electron(self.s.natom, geom, self.s.Z.astype(np.int32), S, T, V)
         f = lambda M: M.reshape(nb, nb).T.copy()
         return f(S), f(T), f(V)
+
+    def fast_quad_self_check(self, nq128: bool = False):
+        """The reference's fast_quad_self_check on this system (the fast
+        rule must be in place).  Returns its report (sample sizes, largest
+        ERI and V differences) and the rule it leaves in place."""
+        import re
+        import tempfile
+        geom = np.asfortranarray(self.s.geom).ravel(order="F").copy()
+        out = np.zeros(2)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "selfcheck.out")
+            lib().ref_fast_quad_self_check(self.s.natom, geom, self.s.Z.astype(np.int32), int(nq128),
+                                           path.encode(), out)
+            text = open(path).read()
+        m = re.search(r"self-check:\s*(\d+) ERI shell quartets, largest difference\s*([-0-9.E+]+)\s*;"
+                      r"\s*nuclear attraction \(sample of\s*(\d+)\)\s*([-0-9.E+]+)", text)
+        return {"n_quartets": int(m.group(1)), "eri_max_diff": float(m.group(2)), "n_v": int(m.group(3)),
+                "v_max_diff": float(m.group(4)), "fast_quad": bool(out[0]), "nquad": int(out[1]), "text": text}
 
     def nre(self) -> float:
         geom = np.asfortranarray(self.s.geom).ravel(order="F").copy()

@@ -15,6 +15,7 @@
 !   overlap/kinetic/nuclear    src/Toroidal/{1D,2D,3D}/*.f90           !
 !   hartree_exchange_potential src/Energies/hartree_exchange_potential.f90:30
 !   NRE                        src/Energies/Nuclear_repulsion.f90:1    !
+!   fast_quad_self_check       src/Toroidal/fast_quad_self_check.f90:30 !
 !---------------------------------------------------------------------!
 module ref_bind
 
@@ -207,6 +208,36 @@ contains
           call nuclear_attraction_matrix_toroidal_3D(natom, nb, geom, ats, AOl, V)
         end select
       end subroutine ref_one_electron
+
+      ! the start-up self-check; its report line goes to the file 'path'
+      ! (unit outfile); out2 = (fast_quad after the check, nquad_point after)
+      subroutine ref_fast_quad_self_check(natom, geom, zat, nq128, path, out2) bind(C, name="ref_fast_quad_self_check")
+        use files, only : outfile
+        integer(c_int), value       :: natom, nq128
+        real(c_double), intent(in)  :: geom(natom,3)
+        integer(c_int), intent(in)  :: zat(natom)
+        character(kind=c_char), intent(in) :: path(*)
+        real(c_double), intent(out) :: out2(2)
+        type(atom), allocatable :: ats(:)
+        character(len=512) :: fname
+        integer :: i
+        fname = ' '
+        i = 1
+        do while (path(i) /= c_null_char .and. i <= 512)
+          fname(i:i) = path(i)
+          i = i + 1
+        end do
+        allocate(ats(natom))
+        do i = 1, natom
+          ats(i)%charge = zat(i)
+        end do
+        c_nquad_128 = (nq128 /= 0)
+        open(unit=outfile, file=trim(fname), status='replace', action='write')
+        call fast_quad_self_check(ndp_s, size(AOl), AOl, natom, geom, ats)
+        close(outfile)
+        out2(1) = merge(1.d0, 0.d0, fast_quad)
+        out2(2) = dble(nquad_point)
+      end subroutine ref_fast_quad_self_check
 
       subroutine ref_nre(natom, geom, zat, E) bind(C, name="ref_nre")
         integer(c_int), value       :: natom

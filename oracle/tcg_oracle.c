@@ -18,6 +18,7 @@
  *   pair_vector              src/module/clifford_tools.f90:467-549 (without the memo:
  *                            a memo hit returns exactly what would be computed)
  *   orc_fast_quad_points     src/module/precompute_t.f90:906-942
+ *   orc_set_nodes            (explicit 64/128-point table, :495-891, as data)
  *   setup_tables             src/module/clifford_tools.f90:620-709
  *   orc_bary                 src/Toroidal/tools.f90:1-262, src/module/heaviside.f90
  *   hermite_E                src/module/clifford_tools.f90:757-785
@@ -365,6 +366,18 @@ int orc_init(int ndp, const double *L, const int *ncell, int nfuc, int nq) {
     if (nq < 1 || nq > MAXQ) return -1;
     s->nq = nq;
     orc_fast_quad_points(nq, s->t, s->w);
+    return 0;
+}
+
+/* an explicit rule (the 64/128-point tables of gauss_legendre_points,
+ * src/module/precompute_t.f90:495-891, taken as data); call before
+ * orc_setup_tables */
+int orc_set_nodes(int nq, const double *t, const double *w) {
+    osys *s = &S_;
+    if (nq < 1 || nq > MAXQ) return -1;
+    s->nq = nq;
+    memcpy(s->t, t, sizeof(double) * nq);
+    memcpy(s->w, w, sizeof(double) * nq);
     return 0;
 }
 

@@ -7,7 +7,7 @@
 namespace tcg {
 
 constexpr int kMaxL = 4;            // up to g shells
-constexpr int kMaxNQ = 64;          // quadrature nodes, padded to node tiles of 16
+constexpr int kMaxNQ = 128;         // quadrature nodes (the 128-point table), padded to node tiles of 16
 constexpr int kNodeTile = 16;       // nodes per MFMA column tile
 constexpr int kRowTile = 16;        // rows per MFMA tile
 constexpr int kMaxPrim = 16;        // primitives per shell

@@ -20,12 +20,15 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/tcg.h"
 #include "tcg_device.hpp"
 #include "tcg_engine.hpp"
 #include "tcg_math.hpp"
+#include "tcg_quad_tables.inc"
 
 using namespace tcg;
 
@@ -67,6 +70,7 @@ struct tcg_ctx {
     int ncl[3] = {1, 1, 1};
     int ncell = 1;
     int nq = 32, nqp = 32;
+    int fast = 1;                // 1: nq-point mapped Gauss-Legendre (fast_quad); 0: literal 64/128 table
     std::string err;
     // basis
     int nbas = 0, nfuc = 0;
@@ -148,7 +152,15 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
     const int ncap = c->ncap, nqp = c->nqp, rows = ncap + 5;
     c->tnode.assign(nqp, 0.0);
     c->tweight.assign(nqp, 0.0);
-    mapped_gauss_legendre(c->nq, c->tnode.data(), c->tweight.data());
+    if (c->fast) {
+        mapped_gauss_legendre(c->nq, c->tnode.data(), c->tweight.data());
+    } else {
+        // the literal tables of gauss_legendre_points (src/module/precompute_t.f90:495-891)
+        const double *t = c->nq == 64 ? kQuad64_t : kQuad128_t;
+        const double *w = c->nq == 64 ? kQuad64_w : kQuad128_w;
+        std::copy(t, t + c->nq, c->tnode.begin());
+        std::copy(w, w + c->nq, c->tweight.begin());
+    }
     tab.assign((size_t)c->ndp * rows * nqp, 0.0);
     const int nh = nqp / kNodeTile;
     tablim.assign((size_t)c->ndp * nh, -1);
@@ -615,6 +627,36 @@ static void fill_translation(const tcg_ctx *c, double *M) {
     }
 }
 
+// V(i, j), i in the unit cell, j anywhere, on the GPU (other rows zero);
+// host_work runs on the host while the kernel executes.
+template <class F>
+static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, std::vector<double> &V, F host_work) {
+    const int nb = c->nbas, nf = c->nfuc;
+    const size_t nn = (size_t)nb * nb;
+    std::vector<int> ao_shell(nb), pair_rep((size_t)nf * nb);
+    for (int sI = 0; sI < c->nsh; ++sI)
+        for (int x = 0; x < c->sh[sI].ncomp; ++x) ao_shell[c->sh[sI].first_ao + x] = sI;
+    for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < nf; ++i) pair_rep[(size_t)i + (size_t)nf * j] = rep_pair(c, ao_shell[i], ao_shell[j]);
+    std::vector<double> nxyz(xyz, xyz + 3 * (size_t)natom);
+    std::vector<int> nz(Z, Z + natom);
+    DevBuf<double> d_nxyz, d_V;
+    DevBuf<int> d_nz, d_aosh, d_prep;
+    HIPCHK(c, d_nxyz.upload(nxyz));
+    HIPCHK(c, d_nz.upload(nz));
+    HIPCHK(c, d_aosh.upload(ao_shell));
+    HIPCHK(c, d_prep.upload(pair_rep));
+    HIPCHK(c, d_V.alloc(nn));
+    HIPCHK(c, launch_fill(d_V.p, nn, 0.0, c->st));
+    EriArgs A = base_args(c);
+    HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, d_V.p, c->st));
+    if constexpr (!std::is_same<F, std::nullptr_t>::value) host_work();
+    V.assign(nn, 0.0);
+    HIPCHK(c, hipMemcpyAsync(V.data(), d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -623,7 +665,7 @@ extern "C" {
 int tcg_create(int ndp, const double L[3], const int ncell[3], int nquad, tcg_ctx **out) {
     if (!out) return TCG_E_ARG;
     *out = nullptr;
-    if (ndp < 1 || ndp > 3 || !L || !ncell || nquad < 4 || nquad > kMaxNQ) return TCG_E_ARG;
+    if (ndp < 1 || ndp > 3 || !L || !ncell || nquad < 4 || nquad > kMaxNQ) return TCG_E_ARG;   // fast rule
     for (int d = 0; d < ndp; ++d)
         if (!(L[d] > 0.0)) return TCG_E_ARG;
     int ndev = 0;
@@ -908,39 +950,117 @@ int tcg_one_electron(tcg_ctx *c, int natom, const double *xyz, const int *Z, dou
     if (natom <= 0 || !xyz || !Z || !S || !T || !V) return c->fail(TCG_E_ARG, "tcg_one_electron: bad arguments");
     int rc = prepare(c);
     if (rc) return rc;
-    const int nb = c->nbas, nf = c->nfuc;
+    const int nb = c->nbas;
     const size_t nn = (size_t)nb * nb;
-    std::fill(S, S + nn, 0.0);
-    std::fill(T, T + nn, 0.0);
-    std::fill(V, V + nn, 0.0);
-    // V on the GPU (rows of the unit cell)
-    std::vector<int> ao_shell(nb), pair_rep((size_t)nf * nb);
-    for (int sI = 0; sI < c->nsh; ++sI)
-        for (int x = 0; x < c->sh[sI].ncomp; ++x) ao_shell[c->sh[sI].first_ao + x] = sI;
-    for (int j = 0; j < nb; ++j)
-        for (int i = 0; i < nf; ++i) pair_rep[(size_t)i + (size_t)nf * j] = rep_pair(c, ao_shell[i], ao_shell[j]);
-    std::vector<double> nxyz(xyz, xyz + 3 * (size_t)natom);
-    std::vector<int> nz(Z, Z + natom);
-    DevBuf<double> d_nxyz, d_V;
-    DevBuf<int> d_nz, d_aosh, d_prep;
-    HIPCHK(c, d_nxyz.upload(nxyz));
-    HIPCHK(c, d_nz.upload(nz));
-    HIPCHK(c, d_aosh.upload(ao_shell));
-    HIPCHK(c, d_prep.upload(pair_rep));
-    HIPCHK(c, d_V.alloc(nn));
-    HIPCHK(c, launch_fill(d_V.p, nn, 0.0, c->st));
-    EriArgs A = base_args(c);
-    HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, d_V.p, c->st));
-    // S and T on the host meanwhile
-    std::vector<double> Sv(nn, 0.0), Tv(nn, 0.0);
-    one_electron_rows(c, Sv, Tv);
-    HIPCHK(c, hipMemcpyAsync(V, d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<double> Sv(nn, 0.0), Tv(nn, 0.0), Vv;
+    rc = nuclear_rows(c, natom, xyz, Z, Vv, [&]() { one_electron_rows(c, Sv, Tv); });
+    if (rc) return rc;
     std::copy(Sv.begin(), Sv.end(), S);
     std::copy(Tv.begin(), Tv.end(), T);
+    std::copy(Vv.begin(), Vv.end(), V);
     fill_translation(c, S);
     fill_translation(c, T);
     fill_translation(c, V);
+    return TCG_OK;
+}
+
+int tcg_set_quadrature(tcg_ctx *c, int fast, int nquad) {
+    if (!c) return TCG_E_ARG;
+    if (fast ? (nquad < 4 || nquad > kMaxNQ) : (nquad != 64 && nquad != 128))
+        return c->fail(TCG_E_ARG, "tcg_set_quadrature: fast=%d with %d points (fast: 4..%d; tables: 64 or 128)", fast,
+                       nquad, kMaxNQ);
+    if (c->fast != (fast != 0) || c->nq != nquad) {
+        c->fast = fast != 0;
+        c->nq = nquad;
+        c->nqp = ((nquad + kNodeTile - 1) / kNodeTile) * kNodeTile;
+        c->prepared = false;
+        c->cols_ready = false;
+    }
+    return TCG_OK;
+}
+
+int tcg_get_quadrature(tcg_ctx *c, int *fast, int *nquad, double *t, double *w) {
+    if (!c || !fast || !nquad) return TCG_E_ARG;
+    *fast = c->fast;
+    *nquad = c->nq;
+    if (t || w) {
+        if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_get_quadrature: nodes need tcg_set_basis first");
+        const int rc = prepare(c);
+        if (rc) return rc;
+        if (t) std::copy(c->tnode.begin(), c->tnode.begin() + c->nq, t);
+        if (w) std::copy(c->tweight.begin(), c->tweight.begin() + c->nq, w);
+    }
+    return TCG_OK;
+}
+
+int tcg_fast_quad_self_check(tcg_ctx *c, int natom, const double *xyz, const int *Z, int nq_full, double *deri,
+                             double *dv, int *fast_on) {
+    if (!c) return TCG_E_ARG;
+    if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_fast_quad_self_check before tcg_set_basis");
+    if (natom <= 0 || !xyz || !Z || !deri || !dv || !fast_on || (nq_full != 64 && nq_full != 128))
+        return c->fail(TCG_E_ARG, "tcg_fast_quad_self_check: bad arguments");
+    *deri = *dv = 0.0;
+    *fast_on = c->fast;
+    if (!c->fast) return TCG_OK;                  // nothing to check (fast_quad_self_check.f90:60)
+    const int nq_fast = c->nq, nsh = c->nsh, nshuc = std::max(c->nshuc, 1), nf = c->nfuc, nb = c->nbas;
+    // sample (fast_quad_self_check.f90:113-161): on-site quartets, 50 LCG quartets, V diagonals + 2 x 20 pairs
+    unsigned long long seed = 12345ull;
+    auto next = [&]() {
+        seed = (seed * 1103515245ull + 12345ull) % 2147483648ull;
+        return (long long)(seed / 16);
+    };
+    const int nrand = 50, nv_rand = 20;
+    std::vector<Quartet> q4;
+    std::vector<size_t> boff;
+    size_t nout = 0;
+    for (int it = 0; it < nshuc + nrand; ++it) {
+        int q[4];
+        if (it < nshuc) {
+            q[0] = q[1] = q[2] = q[3] = it;
+        } else {
+            q[0] = (int)(next() % nshuc);
+            for (int x = 1; x < 4; ++x) q[x] = (int)(next() % nsh);
+        }
+        Quartet Q{};
+        for (int x = 0; x < 4; ++x) Q.s[x] = q[x];
+        Q.bra = rep_pair(c, q[0], q[1]);
+        Q.ket = rep_pair(c, q[2], q[3]);
+        const int lK = c->sh[q[2]].loc;
+        for (int d = 0; d < 3; ++d) Q.shift[d] = c->sh[q[2]].xyz[d] - c->sh[lK].xyz[d];
+        Q.out = (int)nout;
+        Q.prim = -1;
+        q4.push_back(Q);
+        boff.push_back(nout);
+        nout += (size_t)c->sh[q[0]].ncomp * c->sh[q[1]].ncomp * c->sh[q[2]].ncomp * c->sh[q[3]].ncomp;
+    }
+    std::vector<std::pair<int, int>> pv;
+    for (int ip = 0; ip < nshuc; ++ip) pv.push_back({c->sh[ip].first_ao, c->sh[ip].first_ao});
+    for (int ip = 0; ip < 2 * nv_rand; ++ip) {
+        const int a = (int)(next() % nf);
+        const int b = (int)(next() % (ip < nv_rand ? nf : nb));
+        pv.push_back({a, b});
+    }
+    // both rules
+    std::vector<double> b1, b2, V1, V2;
+    int rc = prepare(c);
+    if (rc) return rc;
+    if ((rc = run_blocks(c, q4, nout, b1, 1))) return rc;
+    if ((rc = nuclear_rows(c, natom, xyz, Z, V1, nullptr))) return rc;
+    if ((rc = tcg_set_quadrature(c, 0, nq_full))) return rc;
+    if ((rc = prepare(c))) return rc;
+    if ((rc = run_blocks(c, q4, nout, b2, 1))) return rc;
+    if ((rc = nuclear_rows(c, natom, xyz, Z, V2, nullptr))) return rc;
+    double de = 0.0, dvv = 0.0;
+    for (size_t x = 0; x < nout; ++x) de = std::max(de, std::fabs(b1[x] - b2[x]));
+    for (auto &p : pv) {
+        const size_t ix = (size_t)p.first + (size_t)nb * p.second;
+        dvv = std::max(dvv, std::fabs(V1[ix] - V2[ix]));
+    }
+    const bool ok = de <= 1e-10 && dvv <= 1e-10;
+    if (ok && (rc = tcg_set_quadrature(c, 1, nq_fast))) return rc;
+    *deri = de;
+    *dv = dvv;
+    *fast_on = ok ? 1 : 0;
     return TCG_OK;
 }
 

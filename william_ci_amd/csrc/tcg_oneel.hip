@@ -16,6 +16,10 @@ namespace tcg {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// v[d] of a by-value kernel argument array without dynamic indexing (which
+// would copy the argument block to scratch memory)
+__device__ __forceinline__ double sel3(const double v[3], int d) { return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]); }
+
 __device__ double overlap_real_1d(int j, int m, double p, double pa, double pb) {
     // sum_{k,q} C(j,k) C(m,q) pa^(j-k) pb^(m-q) <x^(k+q)>,  <x^n> = (n-1)!!/(2p)^(n/2) sqrt(pi/p)
     const double base = sqrt(3.14159265358979323846 / p);
@@ -40,7 +44,7 @@ __device__ double overlap_real_1d(int j, int m, double p, double pa, double pb) 
 __global__ __launch_bounds__(64) void k_nuclear(EriArgs A, const double *__restrict__ nuc_xyz, const int *__restrict__ nuc_z,
                                                 int natom, const int *__restrict__ ao_shell, const int *__restrict__ pair_rep,
                                                 double *__restrict__ Vout) {
-    __shared__ double Ix[3][16][kMaxNQ];
+    extern __shared__ double Ix[];      // [3][16][nqp]
     __shared__ double part[64];
     const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
     const int nf = A.nfuc, nb = A.nbas, nq = A.nq, nqp = A.nqp, ndp = A.ndp;
@@ -49,16 +53,16 @@ __global__ __launch_bounds__(64) void k_nuclear(EriArgs A, const double *__restr
     const int pidx = blockIdx.x;            // i + nf * j
     if (pidx >= nf * nb) return;
     const int i = pidx % nf, j = pidx / nf;
-    const Shell sA = A.sh[ao_shell[i]], sB = A.sh[ao_shell[j]];
+    const Shell &sA = A.sh[ao_shell[i]], &sB = A.sh[ao_shell[j]];
     const int lB = sB.l;
     const int *li = A.ao_lxyz + 3 * i, *lj = A.ao_lxyz + 3 * j;
     const ShellPair sp = A.sp[pair_rep[pidx]];
     double total = 0.0;
     for (int p = 0; p < sp.count; ++p) {
-        const PrimPair pp = A.pp[sp.first + p];
+        const PrimPair &pp = A.pp[sp.first + p];
         const double ci = A.coef[A.ao_poff[i] + pp.pi], cj = A.coef[A.ao_poff[j] + pp.pj];
         double pref = ci * cj * pp.fac * 2.0 / sqrt(3.14159265358979323846);
-        for (int d = 0; d < ndp; ++d) pref *= pow(A.a[d], -(double)(li[d] + lj[d])) * A.L[d];
+        for (int d = 0; d < ndp; ++d) pref *= pow(sel3(A.a, d), -(double)(li[d] + lj[d])) * sel3(A.L, d);
         const double esum = pp.q;
         for (int c0 = 0; c0 < natom; c0 += 16) {
             const int c = c0 + r;
@@ -66,47 +70,56 @@ __global__ __launch_bounds__(64) void k_nuclear(EriArgs A, const double *__restr
             for (int d = 0; d < ndp; ++d) {
                 const int N = pp.N[d];
                 const double2 *Vp = A.vec + pp.voff[d] + (size_t)(li[d] * (lB + 1) + lj[d]) * (N + 1);
-                const double phi = cvalid ? A.a[d] * (pp.cen[d] - nuc_xyz[3 * c + d]) : 0.0;
-                int lim[kMaxNQ / kNodeTile];
-                int nend = -1;
-                for (int h = 0; h < nh; ++h) {
-                    const int lh = A.tab_lim[d * nh + h];
-                    lim[h] = lh < N ? lh : N;
-                    nend = lim[h] > nend ? lim[h] : nend;
-                }
-                dbl4 acc[kMaxNQ / kNodeTile];
-                for (int h = 0; h < kMaxNQ / kNodeTile; ++h) acc[h] = dbl4{0.0, 0.0, 0.0, 0.0};
-                double s4, c4, sn, cn;
+                const double phi = cvalid ? sel3(A.a, d) * (pp.cen[d] - nuc_xyz[3 * c + d]) : 0.0;
+                double s4, c4;
                 sincos(4.0 * phi, &s4, &c4);
-                sincos((double)kq * phi, &sn, &cn);
                 const double *tab = A.tab + (size_t)d * tabrows * nqp;
-                for (int n0 = 0; n0 <= nend; n0 += 4) {
-                    const int n = n0 + kq;
-                    if (n0 > 0) {
-                        if ((n0 & 63) == 0) {
-                            sincos((double)n * phi, &sn, &cn);
-                        } else {
-                            const double tc = cn * c4 - sn * s4;
-                            sn = sn * c4 + cn * s4;
-                            cn = tc;
+                // node tiles in groups of four (one accumulator tile each)
+                for (int hg = 0; hg < nh; hg += 4) {
+                    int lim[4];
+                    int nend = -1;
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        lim[h] = -1;
+                        if (hg + h < nh) {
+                            const int lh = A.tab_lim[d * nh + hg + h];
+                            lim[h] = lh < N ? lh : N;
+                            nend = lim[h] > nend ? lim[h] : nend;
                         }
                     }
-                    double F = 0.0;
-                    if (cvalid && n <= N) {
-                        const double2 v = Vp[n];               // G_n = conj(v)
-                        F = v.x * cn + v.y * sn;               // Re[e^{in phi} conj(v)]
+                    dbl4 acc[4];
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) acc[h] = dbl4{0.0, 0.0, 0.0, 0.0};
+                    double sn, cn;
+                    sincos((double)kq * phi, &sn, &cn);
+                    for (int n0 = 0; n0 <= nend; n0 += 4) {
+                        const int n = n0 + kq;
+                        if (n0 > 0) {
+                            if ((n0 & 63) == 0) {
+                                sincos((double)n * phi, &sn, &cn);
+                            } else {
+                                const double tc = cn * c4 - sn * s4;
+                                sn = sn * c4 + cn * s4;
+                                cn = tc;
+                            }
+                        }
+                        double F = 0.0;
+                        if (cvalid && n <= N) {
+                            const double2 v = Vp[n];               // G_n = conj(v)
+                            F = v.x * cn + v.y * sn;               // Re[e^{in phi} conj(v)]
+                        }
+                        const double *trow = tab + (size_t)n * nqp + hg * kNodeTile + r;
+#pragma unroll
+                        for (int h = 0; h < 4; ++h)
+                            if (n0 <= lim[h])
+                                acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, trow[h * kNodeTile], acc[h], 0, 0, 0);
                     }
-                    const double *trow = tab + (size_t)n * nqp + r;
 #pragma unroll
-                    for (int h = 0; h < kMaxNQ / kNodeTile; ++h)
-                        if (h < nh && n0 <= lim[h])
-                            acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, trow[h * kNodeTile], acc[h], 0, 0, 0);
+                    for (int h = 0; h < 4; ++h)
+                        if (hg + h < nh)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) Ix[((size_t)d * 16 + 4 * v + kq) * nqp + (hg + h) * kNodeTile + r] = acc[h][v];
                 }
-#pragma unroll
-                for (int h = 0; h < kMaxNQ / kNodeTile; ++h)
-                    if (h < nh)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) Ix[d][4 * v + kq][h * kNodeTile + r] = acc[h][v];
             }
             __syncthreads();
             // per (nucleus, node): real directions, weights, product; summed per lane
@@ -115,7 +128,7 @@ __global__ __launch_bounds__(64) void k_nuclear(EriArgs A, const double *__restr
                 const int cr = task / nq, k = task % nq, cc = c0 + cr;
                 if (cc >= natom) continue;
                 double prod = A.tweight[k];
-                for (int d = 0; d < ndp; ++d) prod *= Ix[d][cr][k];
+                for (int d = 0; d < ndp; ++d) prod *= Ix[((size_t)d * 16 + cr) * nqp + k];
                 const double t2 = A.tnode[k] * A.tnode[k];
                 const double ppt = esum + t2, eta = t2 / ppt;
                 for (int d = ndp; d < 3; ++d) {
@@ -141,7 +154,8 @@ __global__ __launch_bounds__(64) void k_nuclear(EriArgs A, const double *__restr
 hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
                           const int *pair_rep, double *V, hipStream_t st) {
     const int npairs = a.nfuc * a.nbas;
-    hipLaunchKernelGGL(k_nuclear, dim3(npairs), dim3(64), 0, st, a, nuc_xyz, nuc_z, natom, ao_shell, pair_rep, V);
+    const size_t lds = sizeof(double) * 3 * 16 * (size_t)a.nqp;
+    hipLaunchKernelGGL(k_nuclear, dim3(npairs), dim3(64), lds, st, a, nuc_xyz, nuc_z, natom, ao_shell, pair_rep, V);
     return hipGetLastError();
 }
 

@@ -159,10 +159,10 @@ hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long 
 // kNT = node tiles of 16 (2 for the 32-node rule, 3 for 40 nodes).
 template <int kNT>
 __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__restrict__ rows,
-                                                  const int *__restrict__ order, long long nrows,
+                                                  const int *__restrict__ order, long long nrows, int hbase,
                                                   double *__restrict__ rowval) {
     const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
-    const int nqp = A.nqp, nh = kNT, nq = A.nq;
+    const int nqp = A.nqp, nh = kNT, nq = A.nq, nht = A.nqp / kNodeTile;
     const int tabrows = A.ncap + 5;
     const long long ntiles = (nrows + 15) / 16;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
             Gp = A.vec + R.gofs;
             Hp = A.vec + R.hofs;
         }
-        if (A.counters && kq == 0 && valid) {
+        if (A.counters && kq == 0 && valid && hbase == 0) {
             long s = 0;
 #pragma unroll 1
             for (int k = 0; k < nq; ++k) {
@@ -204,12 +204,12 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
             for (int h = 0; h < kNT; ++h) {
                 lim[h] = -1;
                 if (h < nh) {
-                    const int lh = A.tab_lim[dd * nh + h];
+                    const int lh = A.tab_lim[dd * nht + hbase + h];
                     lim[h] = lh < nmax_t ? lh : nmax_t;
                     nend = lim[h] > nend ? lim[h] : nend;
                 }
             }
-            const double *tab = A.tab + (size_t)dd * tabrows * nqp + r;
+            const double *tab = A.tab + (size_t)dd * tabrows * nqp + hbase * kNodeTile + r;
             dbl4 acc[kNT];
 #pragma unroll
             for (int h = 0; h < kNT; ++h) acc[h] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
                 if (rrow >= 0 && rd == dd) {
 #pragma unroll
                     for (int h = 0; h < kNT; ++h)
-                        if (h < nh) rowval[(size_t)rrow * nqp + h * kNodeTile + r] = acc[h][v] * sg;
+                        if (h < nh) rowval[(size_t)rrow * nqp + (hbase + h) * kNodeTile + r] = acc[h][v] * sg;
                 }
             }
         }
@@ -279,13 +279,19 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
     if (nrows <= 0) return hipSuccess;
     const long long ntiles = (nrows + 15) / 16;
     const int grid = ntiles < (1 << 20) ? (int)ntiles : (1 << 20);
-    switch (a.nqp / kNodeTile) {
-    case 1: hipLaunchKernelGGL(k_rows_mfma<1>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
-    case 2: hipLaunchKernelGGL(k_rows_mfma<2>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
-    case 3: hipLaunchKernelGGL(k_rows_mfma<3>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
-    default: hipLaunchKernelGGL(k_rows_mfma<4>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, rowval); break;
+    const int nht = a.nqp / kNodeTile;
+    // up to four node tiles per launch (more nodes: one launch per group of four)
+    for (int h0 = 0; h0 < nht; h0 += 4) {
+        switch (nht - h0 < 4 ? nht - h0 : 4) {
+        case 1: hipLaunchKernelGGL(k_rows_mfma<1>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
+        case 2: hipLaunchKernelGGL(k_rows_mfma<2>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
+        case 3: hipLaunchKernelGGL(k_rows_mfma<3>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
+        default: hipLaunchKernelGGL(k_rows_mfma<4>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

@@ -42,6 +42,10 @@ def lib():
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
         L.tcg_eri_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_longlong)]
+        L.tcg_set_quadrature.argtypes = [_vp, _ci, _ci]
+        L.tcg_get_quadrature.argtypes = [_vp, ctypes.POINTER(_ci), ctypes.POINTER(_ci), _vp, _vp]
+        L.tcg_fast_quad_self_check.argtypes = [_vp, _ci, _dp, _ip, _ci, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ci)]
         L.tcg_last_error.argtypes = [_vp]
         L.tcg_last_error.restype = ctypes.c_char_p
         L.tcg_destroy.argtypes = [_vp]
@@ -112,6 +116,33 @@ class Engine:
         n = ctypes.c_longlong()
         lib().tcg_eri_device(self.h, ctypes.byref(p), ctypes.byref(n))
         return p.value, n.value
+
+    # -- quadrature rule (fast_quad / nquad_point) ------------------------------
+    def set_quadrature(self, fast: bool, nquad: int):
+        """fast: nquad mapped Gauss-Legendre points; else the 64/128 table."""
+        self._check(lib().tcg_set_quadrature(self.h, int(fast), int(nquad)), "tcg_set_quadrature")
+
+    def quadrature(self):
+        """(fast, nquad, t, w) of the rule in place."""
+        f, n = _ci(), _ci()
+        self._check(lib().tcg_get_quadrature(self.h, ctypes.byref(f), ctypes.byref(n), None, None),
+                    "tcg_get_quadrature")
+        t = np.zeros(n.value)
+        w = np.zeros(n.value)
+        self._check(lib().tcg_get_quadrature(self.h, ctypes.byref(f), ctypes.byref(n), t.ctypes.data,
+                                             w.ctypes.data), "tcg_get_quadrature")
+        return bool(f.value), n.value, t, w
+
+    def fast_quad_self_check(self, nq_full: int = 64):
+        """fast_quad_self_check: sample ERIs and V with the fast rule and the
+        nq_full table; keeps the fast rule only if both agree to 1e-10."""
+        s = self.s
+        de, dv, on = ctypes.c_double(), ctypes.c_double(), _ci()
+        self._check(lib().tcg_fast_quad_self_check(self.h, s.natom, np.ascontiguousarray(s.geom).ravel(),
+                                                   np.ascontiguousarray(s.Z, dtype=np.int32), int(nq_full),
+                                                   ctypes.byref(de), ctypes.byref(dv), ctypes.byref(on)),
+                    "tcg_fast_quad_self_check")
+        return {"eri_max_diff": de.value, "v_max_diff": dv.value, "fast_quad": bool(on.value)}
 
     # -- overlap / kinetic / nuclear_attraction_matrix_toroidal_nD -------------
     def one_electron(self):
