@@ -116,7 +116,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md but not implemented", file=sys.stderr)
+              "sharding the bra-pair task list over GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     from william_ci_amd import system as tsys

@@ -9,7 +9,7 @@
  * not re-entrant and must be used from one host thread at a time.  All calls
  * are synchronous: results are in the caller's arrays on return.
  *
- * A Fortran caller binds these with ISO_C_BINDING.
+ * A Fortran caller binds these with ISO_C_BINDING interfaces (bind(C, name=...)).
  */
 #ifndef TCG_H
 #define TCG_H
@@ -65,6 +65,18 @@ int tcg_eri_build(tcg_ctx *ctx, int sao, int physicist, double *eri);
  * the caller's ERI(n1,nBas,nBas,nBas) (uploaded for this call). */
 int tcg_jk_stored(tcg_ctx *ctx, int n1, int nbas, const double *P, const double *eri, double *J, double *K);
 
+/* Direct J/K: the integrals are recomputed at every call and contracted on
+ * the fly, nothing of size nBas^4 is stored.  Replaces
+ *   direct_init(ndp,nBas,AO)        src/module/eri_direct.f90:76-225
+ *   direct_JK(nBas,P,J,K)           src/module/eri_direct.f90:262-374
+ * tcg_direct_init: Schwarz, primitive Schwarz and the unique quartet list
+ * (again after every tcg_set_basis / tcg_set_quadrature).
+ * tcg_direct_jk: J(mu,nu) += E P(la,si), K(mu,si) -= E P(nu,la) / 2 over every
+ * stored element E = ERI(mu,nu,la,si) of the SAO layout, each counted once;
+ * rows mu <= nfuc written, the rest zero; P, J, K are (nBas,nBas). */
+int tcg_direct_init(tcg_ctx *ctx);
+int tcg_direct_jk(tcg_ctx *ctx, int nbas, const double *P, double *J, double *K);
+
 /* One-electron matrices S, T, V (nBas,nBas): rows of the unit cell computed,
  * values below 1e-14 zeroed, the other rows filled by translation.  Replaces
  *   overlap_matrix_toroidal_nD            e.g. src/Toroidal/1D/1D_Overlap_toroidal.f90:23
@@ -112,7 +124,8 @@ int tcg_eri_stats(tcg_ctx *ctx, long long *out, int n);
  *   out[0] tables + pairs   out[1] Schwarz + primitive screen
  *   out[2] quartet list     out[3] quartet kernel + scatter   out[4] total
  *   out[5] duration of the quartet phase on the GPU (HIP events on the engine's stream)
- *   out[6] duration of the row MFMA kernel alone (HIP events around its launch) */
+ *   out[6] duration of the row MFMA kernel alone (HIP events around its launch)
+ *   out[7] the t-table part of out[0] (host) */
 int tcg_eri_times(tcg_ctx *ctx, double *out, int n);
 
 /* Device address of the resident ERI tensor (for callers that keep data on

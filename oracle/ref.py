@@ -41,6 +41,7 @@ def lib():
         L.ref_nre.argtypes = [_ci, _dp, _ip, ctypes.POINTER(ctypes.c_double)]
         L.ref_jk.argtypes = [_ci, _ci, _dp, _dp, _dp, _dp]
         L.ref_fast_quad_self_check.argtypes = [_ci, _dp, _ip, _ci, ctypes.c_char_p, _dp]
+        L.ref_direct_jk.argtypes = [_dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -103,6 +104,16 @@ class RefSystem:
         lib().ref_one_electron(self.s.natom, geom, self.s.Z.astype(np.int32), S, T, V)
         f = lambda M: M.reshape(nb, nb).T.copy()
         return f(S), f(T), f(V)
+
+    def direct_jk(self, P):
+        """direct_init + direct_JK on the density P (row-major numpy, symmetric);
+        J, K returned as row-major numpy matrices (rows > nfuc zero)."""
+        nb = self.s.nbas
+        Pf = np.ascontiguousarray(np.asarray(P, dtype=np.float64).T)
+        J = np.zeros(nb * nb)
+        K = np.zeros(nb * nb)
+        lib().ref_direct_jk(Pf.ravel(), J, K)
+        return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
 
     def fast_quad_self_check(self, nq128: bool = False):
         """The reference's fast_quad_self_check on this system (the fast

@@ -16,6 +16,7 @@
 !   hartree_exchange_potential src/Energies/hartree_exchange_potential.f90:30
 !   NRE                        src/Energies/Nuclear_repulsion.f90:1    !
 !   fast_quad_self_check       src/Toroidal/fast_quad_self_check.f90:30 !
+!   direct_init / direct_JK    src/module/eri_direct.f90:76,262        !
 !---------------------------------------------------------------------!
 module ref_bind
 
@@ -208,6 +209,15 @@ contains
           call nuclear_attraction_matrix_toroidal_3D(natom, nb, geom, ats, AOl, V)
         end select
       end subroutine ref_one_electron
+
+      ! direct J/K on the density P(nBas,nBas): direct_init then direct_JK
+      subroutine ref_direct_jk(P, J, K) bind(C, name="ref_direct_jk")
+        use eri_direct, only : direct_init, direct_JK
+        real(c_double), intent(in)  :: P(size(AOl),size(AOl))
+        real(c_double), intent(out) :: J(size(AOl),size(AOl)), K(size(AOl),size(AOl))
+        call direct_init(ndp_s, size(AOl), AOl)
+        call direct_JK(size(AOl), P, J, K)
+      end subroutine ref_direct_jk
 
       ! the start-up self-check; its report line goes to the file 'path'
       ! (unit outfile); out2 = (fast_quad after the check, nquad_point after)

@@ -36,6 +36,9 @@ struct EriArgs {
     double *eri;                      // ERI(n1,nbas,nbas,nbas)
     int n1, nbas, nfuc, ncell, physicist;
     const double *Qao;                // [nbas*nbas] function-pair Schwarz values
+    // direct J/K mode
+    const double *Pden;               // density P(nbas,nbas), column-major
+    double *Jacc, *Kacc;              // J, K rows of the unit cell, (nfuc,nbas) column-major
     const int *relc;                  // [ncell*ncell]
     // counters
     unsigned long long *counters;     // [2] contraction flops, [3] quartet count, [4] row (MFMA) flops
@@ -58,8 +61,12 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
 size_t contract_lds_bytes(const EriArgs &a);
 hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv, int *sq,
                             long long ns, double *realval, hipStream_t st);
+hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                              const double *rowval, const double *realval, hipStream_t st);
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                            const double *rowval, const double *realval, hipStream_t st);
+hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, int nchunk, int *bounds, hipStream_t st);
+hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_t st);
 size_t scan_temp_bytes(int n);
 hipError_t scan_exclusive(void *tmp, size_t bytes, const long long *in, long long *out, int n, hipStream_t st);
 size_t sort_temp_bytes(int n, int end_bit);

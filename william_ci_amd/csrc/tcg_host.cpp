@@ -18,6 +18,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -108,6 +109,10 @@ struct tcg_ctx {
     DevBuf<int> d_sq;
     DevBuf<char> d_tmp;
     long long last_surv = 0, last_rows = 0;
+    double rows_ms = 0.0;        // row MFMA kernel time of the last run_quartets (time_rows)
+    double row_budget = 24e9;    // bytes of row buffers per chunk
+    bool row_budget_set = false; // TCG_ROW_BUDGET given (tests: small chunks on small systems)
+    DevBuf<long long> d_soffc, d_roffc;
     // column lists of the resident ERI for the sparse J/K
     bool cols_ready = false;
     DevBuf<long long> d_list_d, d_list_b, d_off_d, d_off_b;
@@ -117,6 +122,8 @@ struct tcg_ctx {
     double times[8] = {0};
     int max_ncomb = 1, max_blk = 1;
     bool prepared = false;
+    int nq4 = 0;                 // unique shell quartets in d_q4 (after setup_quartets)
+    bool direct_ready = false;   // d_q4 / d_Q / primitive Schwarz valid for the direct J/K
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -169,8 +176,7 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
     for (int d = 0; d < c->ndp; ++d) {
         for (int k = 0; k < c->nq; ++k) {
             const double C = 2.0 * c->tnode[k] * c->tnode[k] / (c->a[d] * c->a[d]);
-            const int n0 = ncap + 30 + (int)std::sqrt(80.0 * C);
-            bessel_miller(ncap, C, n0, col.data());
+            bessel_scaled_vector(ncap, C, col.data());
             int nmax = ncap;   // last order kept: first below 1e-16 relative, + 5
             for (int n = 0; n <= ncap; ++n)
                 if (col[n] < 1e-16 * col[0]) { nmax = std::min(ncap, n + 5); break; }
@@ -383,9 +389,51 @@ static EriArgs base_args(tcg_ctx *c) {
 
 // Quartets of A.q4[0..A.nq4) through the row pipeline (tcg_rows.hip):
 // survivors -> rows -> sort by (direction, length) -> MFMA -> contraction.
+// One chunk of quartets (A.q4[0..A.nq4)) whose survivor/row offsets so/ro
+// start at 0: survivors, rows, sort, MFMA rows, real rows, contraction.
+static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const long long *ro, long long ns,
+                     long long nr) {
+    HIPCHK(c, c->d_surv.alloc(ns));
+    HIPCHK(c, c->d_rows.alloc(nr));
+    HIPCHK(c, c->d_keys.alloc(nr));
+    HIPCHK(c, c->d_keys2.alloc(nr));
+    HIPCHK(c, c->d_vals.alloc(nr));
+    HIPCHK(c, c->d_order.alloc(nr));
+    HIPCHK(c, c->d_rowval.alloc((size_t)nr * A.nqp));
+    HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, so, c->d_surv.p, c->st));
+    HIPCHK(c, launch_rows_emit(A, so, ro, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
+    size_t tb = sort_temp_bytes((int)nr, 18);
+    HIPCHK(c, c->d_tmp.alloc(tb));
+    HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, 18, c->st));
+    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev2, c->st));
+    HIPCHK(c, launch_rows_mfma(A, c->d_rows.p, c->d_order.p, nr, c->d_rowval.p, c->st));
+    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev3, c->st));
+    const int nreal = 3 - A.ndp;
+    if (nreal > 0) {
+        HIPCHK(c, c->d_sq.alloc(ns));
+        HIPCHK(c, c->d_realval.alloc((size_t)(nr / A.ndp) * nreal * A.nqp));
+        HIPCHK(c, launch_real_rows(A, so, ro, c->d_surv.p, c->d_sq.p, ns, c->d_realval.p, c->st));
+    }
+    if (A.Jacc)
+        HIPCHK(c, launch_contract_jk(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
+    else
+        HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
+    if (c->time_rows) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventSynchronize(c->ev3));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev2, c->ev3));
+        c->rows_ms += ms;
+    }
+    return 0;
+}
+
+// Quartets of A.q4[0..A.nq4) through the row pipeline (tcg_rows.hip):
+// survivors -> rows -> sort by (direction, length) -> MFMA -> contraction,
+// in chunks of at most row_cap rows so that the row buffers stay bounded.
 static int run_quartets(tcg_ctx *c, EriArgs A) {
     const int nq4 = A.nq4;
     c->last_surv = c->last_rows = 0;
+    c->rows_ms = 0.0;
     if (nq4 <= 0) return 0;
     HIPCHK(c, c->d_scount.alloc(nq4 + 1));
     HIPCHK(c, c->d_rcount.alloc(nq4 + 1));
@@ -404,31 +452,40 @@ static int run_quartets(tcg_ctx *c, EriArgs A) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->last_surv = ns;
     c->last_rows = nr;
-    if (nr > 0x7fffffffLL) return c->fail(TCG_E_NOMEM, "too many quadrature rows (%lld) for one pass", nr);
-    if ((double)nr * A.nqp * 8.0 > 96e9) return c->fail(TCG_E_NOMEM, "row buffer would exceed 96 GB (%lld rows)", nr);
     if (nr == 0) return 0;
-    HIPCHK(c, c->d_surv.alloc(ns));
-    HIPCHK(c, c->d_rows.alloc(nr));
-    HIPCHK(c, c->d_keys.alloc(nr));
-    HIPCHK(c, c->d_keys2.alloc(nr));
-    HIPCHK(c, c->d_vals.alloc(nr));
-    HIPCHK(c, c->d_order.alloc(nr));
-    HIPCHK(c, c->d_rowval.alloc((size_t)nr * A.nqp));
-    HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, c->d_soff.p, c->d_surv.p, c->st));
-    HIPCHK(c, launch_rows_emit(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
-    tb = sort_temp_bytes((int)nr, 18);
-    HIPCHK(c, c->d_tmp.alloc(tb));
-    HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, 18, c->st));
-    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev2, c->st));
-    HIPCHK(c, launch_rows_mfma(A, c->d_rows.p, c->d_order.p, nr, c->d_rowval.p, c->st));
-    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev3, c->st));
+    // rows per chunk: row record, sort keys/values, node values, real-direction values
     const int nreal = 3 - A.ndp;
-    if (nreal > 0) {
-        HIPCHK(c, c->d_sq.alloc(ns));
-        HIPCHK(c, c->d_realval.alloc((size_t)(nr / A.ndp) * nreal * A.nqp));
-        HIPCHK(c, launch_real_rows(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_sq.p, ns, c->d_realval.p, c->st));
+    const double row_bytes = 48.0 + 8.0 * A.nqp * (1.0 + (double)nreal / A.ndp);
+    const long long row_floor = c->row_budget_set ? 1024 : (1ll << 20);
+    const long long row_cap =
+        std::min<long long>(1ll << 30, std::max<long long>(row_floor, (long long)(c->row_budget / row_bytes)));
+    if (nr <= row_cap) return run_chunk(c, A, c->d_soff.p, c->d_roff.p, ns, nr);
+    // chunk boundaries: first quartet whose row offset reaches k * row_cap
+    const int nchunk = (int)((nr + row_cap - 1) / row_cap);
+    DevBuf<int> d_bounds;
+    HIPCHK(c, d_bounds.alloc(nchunk + 1));
+    HIPCHK(c, launch_chunk_bounds(c->d_roff.p, nq4, row_cap, nchunk, d_bounds.p, c->st));
+    std::vector<int> bounds(nchunk + 1);
+    HIPCHK(c, hipMemcpyAsync(bounds.data(), d_bounds.p, sizeof(int) * (nchunk + 1), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, c->d_soffc.alloc(nq4 + 1));
+    HIPCHK(c, c->d_roffc.alloc(nq4 + 1));
+    for (int k = 0; k < nchunk; ++k) {
+        const int q0 = bounds[k], q1 = bounds[k + 1];
+        if (q1 <= q0) continue;
+        HIPCHK(c, launch_rebase(c->d_soff.p + q0, q1 - q0 + 1, c->d_soffc.p, c->st));
+        HIPCHK(c, launch_rebase(c->d_roff.p + q0, q1 - q0 + 1, c->d_roffc.p, c->st));
+        long long nsc = 0, nrc = 0;
+        HIPCHK(c, hipMemcpyAsync(&nsc, c->d_soffc.p + (q1 - q0), sizeof nsc, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(&nrc, c->d_roffc.p + (q1 - q0), sizeof nrc, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (nrc == 0) continue;
+        EriArgs B = A;
+        B.q4 = A.q4 + q0;
+        B.nq4 = q1 - q0;
+        const int rc = run_chunk(c, B, c->d_soffc.p, c->d_roffc.p, nsc, nrc);
+        if (rc) return rc;
     }
-    HIPCHK(c, launch_contract(A, c->d_soff.p, c->d_roff.p, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
     return 0;
 }
 
@@ -454,7 +511,9 @@ static int prepare(tcg_ctx *c) {
     // ---- 1-2: tables and pair data ----
     std::vector<double> tab;
     std::vector<int> tablim, tnmax;
+    const double tt0 = now_ms();
     build_tables(c, tab, tablim, tnmax);
+    c->times[7] = now_ms() - tt0;
     std::vector<VecJob> jobs;
     size_t nvec = 0, niv = 0;
     std::vector<double> herm;
@@ -627,100 +686,11 @@ static void fill_translation(const tcg_ctx *c, double *M) {
     }
 }
 
-// V(i, j), i in the unit cell, j anywhere, on the GPU (other rows zero);
-// host_work runs on the host while the kernel executes.
-template <class F>
-static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, std::vector<double> &V, F host_work) {
-    const int nb = c->nbas, nf = c->nfuc;
-    const size_t nn = (size_t)nb * nb;
-    std::vector<int> ao_shell(nb), pair_rep((size_t)nf * nb);
-    for (int sI = 0; sI < c->nsh; ++sI)
-        for (int x = 0; x < c->sh[sI].ncomp; ++x) ao_shell[c->sh[sI].first_ao + x] = sI;
-    for (int j = 0; j < nb; ++j)
-        for (int i = 0; i < nf; ++i) pair_rep[(size_t)i + (size_t)nf * j] = rep_pair(c, ao_shell[i], ao_shell[j]);
-    std::vector<double> nxyz(xyz, xyz + 3 * (size_t)natom);
-    std::vector<int> nz(Z, Z + natom);
-    DevBuf<double> d_nxyz, d_V;
-    DevBuf<int> d_nz, d_aosh, d_prep;
-    HIPCHK(c, d_nxyz.upload(nxyz));
-    HIPCHK(c, d_nz.upload(nz));
-    HIPCHK(c, d_aosh.upload(ao_shell));
-    HIPCHK(c, d_prep.upload(pair_rep));
-    HIPCHK(c, d_V.alloc(nn));
-    HIPCHK(c, launch_fill(d_V.p, nn, 0.0, c->st));
-    EriArgs A = base_args(c);
-    HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, d_V.p, c->st));
-    if constexpr (!std::is_same<F, std::nullptr_t>::value) host_work();
-    V.assign(nn, 0.0);
-    HIPCHK(c, hipMemcpyAsync(V.data(), d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    return 0;
-}
-
-// ---------------------------------------------------------------------------
-// C ABI
-// ---------------------------------------------------------------------------
-extern "C" {
-
-int tcg_create(int ndp, const double L[3], const int ncell[3], int nquad, tcg_ctx **out) {
-    if (!out) return TCG_E_ARG;
-    *out = nullptr;
-    if (ndp < 1 || ndp > 3 || !L || !ncell || nquad < 4 || nquad > kMaxNQ) return TCG_E_ARG;   // fast rule
-    for (int d = 0; d < ndp; ++d)
-        if (!(L[d] > 0.0)) return TCG_E_ARG;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TCG_E_DEVICE;
-    tcg_ctx *c = new tcg_ctx();
-    c->ndp = ndp;
-    for (int d = 0; d < 3; ++d) {
-        c->L[d] = L[d];
-        c->a[d] = (d < ndp) ? 2.0 * kPi / L[d] : 0.0;
-        c->ncl[d] = ncell[d] > 0 ? ncell[d] : 1;
-    }
-    c->ncell = c->ncl[0] * c->ncl[1] * c->ncl[2];
-    c->nq = nquad;
-    c->nqp = ((nquad + kNodeTile - 1) / kNodeTile) * kNodeTile;
-    if (hipStreamCreate(&c->st) != hipSuccess) {
-        delete c;
-        return TCG_E_DEVICE;
-    }
-    *out = c;
-    return TCG_OK;
-}
-
-int tcg_set_basis(tcg_ctx *c, int nbas, int nfuc, const double *center, const int *lxyz, const int *prim_off,
-                  const double *expo, const double *coef) {
-    if (!c) return TCG_E_ARG;
-    if (nbas <= 0 || nfuc <= 0 || !center || !lxyz || !prim_off || !expo || !coef)
-        return c->fail(TCG_E_ARG, "tcg_set_basis: null or empty argument");
-    if (nbas != nfuc * c->ncell) return c->fail(TCG_E_ARG, "nbas (%d) != nfuc (%d) * cells (%d)", nbas, nfuc, c->ncell);
-    if (prim_off[0] != 0) return c->fail(TCG_E_ARG, "prim_off[0] must be 0");
-    for (int i = 0; i < nbas; ++i)
-        if (prim_off[i + 1] <= prim_off[i]) return c->fail(TCG_E_ARG, "AO %d has no primitive", i);
-    const int np = prim_off[nbas];
-    c->nbas = nbas;
-    c->nfuc = nfuc;
-    c->center.assign(center, center + 3 * (size_t)nbas);
-    c->lxyz.assign(lxyz, lxyz + 3 * (size_t)nbas);
-    c->poff.assign(prim_off, prim_off + nbas + 1);
-    c->expo.assign(expo, expo + np);
-    c->coef.assign(coef, coef + np);
-    c->prepared = false;
-    for (int p = 0; p < np; ++p)
-        if (!(expo[p] > 0.0)) return c->fail(TCG_E_ARG, "exponent %d is not positive", p);
-    return build_shells(c);
-}
-
-int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
-    if (!c) return TCG_E_ARG;
-    if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_build before tcg_set_basis");
-    if (!sao) return c->fail(TCG_E_ARG, "tcg_eri_build: this version builds the SAO layout ERI(nfuc,nBas,nBas,nBas) only");
-    const double t0 = now_ms();
-    int rc = prepare(c);
-    if (rc) return rc;
-    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 3 * sizeof(unsigned long long), c->st));
-    const double t1 = now_ms();
-
+// Steps 3-5 of the driver (ERI_toroidal.f90:200-401): Schwarz values from
+// the diagonal blocks, the primitive Schwarz table, then the unique shell
+// quartets on the GPU (c->d_q4[0..c->nq4)).  Shared by the stored build and
+// the direct J/K (eri_direct.f90:76-225 does the same set-up).
+static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
     // ---- 3: Schwarz from the diagonal blocks, replicated over translations ----
     const int nsh = c->nsh, nshuc = c->nshuc, nb = c->nbas, nf = c->nfuc, nc = c->ncell;
     std::vector<Quartet> q4;
@@ -739,7 +709,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
             nout += (size_t)m * m;
         }
     std::vector<double> blocks;
-    rc = run_blocks(c, q4, nout, blocks, 1);
+    int rc = run_blocks(c, q4, nout, blocks, 1);
     if (rc) return rc;
     std::vector<double> Q((size_t)nb * nb, 0.0), Qsh((size_t)nsh * nsh, 0.0);
     for (size_t x = 0; x < q4.size(); ++x) {
@@ -804,7 +774,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     HIPCHK(c, c->d_pp.upload(c->pp));
     HIPCHK(c, c->d_Q.upload(Q));
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
-    const double t2 = now_ms();
+    t2 = now_ms();
 
     // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened, on the GPU ----
     HIPCHK(c, c->d_Qsh.upload(Qsh));
@@ -820,7 +790,119 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, Qmax, nshuc, nsh, nc, c->d_cnt.p + 3, c->d_q4.p,
                                    nq4, c->st));
     c->stats[0] = (long long)nq4;
-    const double t3 = now_ms();
+    c->nq4 = (int)nq4;
+    t3 = now_ms();
+    return 0;
+
+}
+
+// V(i, j), i in the unit cell, j anywhere, on the GPU (other rows zero);
+// host_work runs on the host while the kernel executes.
+template <class F>
+static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, std::vector<double> &V, F host_work) {
+    const int nb = c->nbas, nf = c->nfuc;
+    const size_t nn = (size_t)nb * nb;
+    std::vector<int> ao_shell(nb), pair_rep((size_t)nf * nb);
+    for (int sI = 0; sI < c->nsh; ++sI)
+        for (int x = 0; x < c->sh[sI].ncomp; ++x) ao_shell[c->sh[sI].first_ao + x] = sI;
+    for (int j = 0; j < nb; ++j)
+        for (int i = 0; i < nf; ++i) pair_rep[(size_t)i + (size_t)nf * j] = rep_pair(c, ao_shell[i], ao_shell[j]);
+    std::vector<double> nxyz(xyz, xyz + 3 * (size_t)natom);
+    std::vector<int> nz(Z, Z + natom);
+    DevBuf<double> d_nxyz, d_V;
+    DevBuf<int> d_nz, d_aosh, d_prep;
+    HIPCHK(c, d_nxyz.upload(nxyz));
+    HIPCHK(c, d_nz.upload(nz));
+    HIPCHK(c, d_aosh.upload(ao_shell));
+    HIPCHK(c, d_prep.upload(pair_rep));
+    HIPCHK(c, d_V.alloc(nn));
+    HIPCHK(c, launch_fill(d_V.p, nn, 0.0, c->st));
+    EriArgs A = base_args(c);
+    HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, d_V.p, c->st));
+    if constexpr (!std::is_same<F, std::nullptr_t>::value) host_work();
+    V.assign(nn, 0.0);
+    HIPCHK(c, hipMemcpyAsync(V.data(), d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int tcg_create(int ndp, const double L[3], const int ncell[3], int nquad, tcg_ctx **out) {
+    if (!out) return TCG_E_ARG;
+    *out = nullptr;
+    if (ndp < 1 || ndp > 3 || !L || !ncell || nquad < 4 || nquad > kMaxNQ) return TCG_E_ARG;   // fast rule
+    for (int d = 0; d < ndp; ++d)
+        if (!(L[d] > 0.0)) return TCG_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TCG_E_DEVICE;
+    tcg_ctx *c = new tcg_ctx();
+    c->ndp = ndp;
+    for (int d = 0; d < 3; ++d) {
+        c->L[d] = L[d];
+        c->a[d] = (d < ndp) ? 2.0 * kPi / L[d] : 0.0;
+        c->ncl[d] = ncell[d] > 0 ? ncell[d] : 1;
+    }
+    c->ncell = c->ncl[0] * c->ncl[1] * c->ncl[2];
+    c->nq = nquad;
+    c->nqp = ((nquad + kNodeTile - 1) / kNodeTile) * kNodeTile;
+    if (const char *e = getenv("TCG_ROW_BUDGET")) {
+        const double v = atof(e);
+        if (v > 0.0) {
+            c->row_budget = v;
+            c->row_budget_set = true;
+        }
+    }
+    if (hipStreamCreate(&c->st) != hipSuccess) {
+        delete c;
+        return TCG_E_DEVICE;
+    }
+    *out = c;
+    return TCG_OK;
+}
+
+int tcg_set_basis(tcg_ctx *c, int nbas, int nfuc, const double *center, const int *lxyz, const int *prim_off,
+                  const double *expo, const double *coef) {
+    if (!c) return TCG_E_ARG;
+    if (nbas <= 0 || nfuc <= 0 || !center || !lxyz || !prim_off || !expo || !coef)
+        return c->fail(TCG_E_ARG, "tcg_set_basis: null or empty argument");
+    if (nbas != nfuc * c->ncell) return c->fail(TCG_E_ARG, "nbas (%d) != nfuc (%d) * cells (%d)", nbas, nfuc, c->ncell);
+    if (prim_off[0] != 0) return c->fail(TCG_E_ARG, "prim_off[0] must be 0");
+    for (int i = 0; i < nbas; ++i)
+        if (prim_off[i + 1] <= prim_off[i]) return c->fail(TCG_E_ARG, "AO %d has no primitive", i);
+    const int np = prim_off[nbas];
+    c->nbas = nbas;
+    c->nfuc = nfuc;
+    c->center.assign(center, center + 3 * (size_t)nbas);
+    c->lxyz.assign(lxyz, lxyz + 3 * (size_t)nbas);
+    c->poff.assign(prim_off, prim_off + nbas + 1);
+    c->expo.assign(expo, expo + np);
+    c->coef.assign(coef, coef + np);
+    c->prepared = false;
+    c->direct_ready = false;
+    for (int p = 0; p < np; ++p)
+        if (!(expo[p] > 0.0)) return c->fail(TCG_E_ARG, "exponent %d is not positive", p);
+    return build_shells(c);
+}
+
+int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
+    if (!c) return TCG_E_ARG;
+    if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_build before tcg_set_basis");
+    if (!sao) return c->fail(TCG_E_ARG, "tcg_eri_build: this version builds the SAO layout ERI(nfuc,nBas,nBas,nBas) only");
+    const double t0 = now_ms();
+    int rc = prepare(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 3 * sizeof(unsigned long long), c->st));
+    const double t1 = now_ms();
+
+    double t2 = 0.0, t3 = 0.0;
+    rc = setup_quartets(c, t2, t3);
+    if (rc) return rc;
+    const int nb = c->nbas, nf = c->nfuc;
+    const unsigned long long nq4 = (unsigned long long)c->nq4;
 
     // ---- 6: quartets on the GPU, scattered into the ERI array ----
     const int n1 = sao ? nf : nb;
@@ -852,8 +934,8 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     float kms = 0.f, rms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
     c->times[5] = kms;
-    if (c->last_rows > 0) HIPCHK(c, hipEventElapsedTime(&rms, c->ev2, c->ev3));
-    c->times[6] = rms;
+    (void)rms;
+    c->times[6] = c->rows_ms;
     const double t4 = now_ms();
     unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HIPCHK(c, hipMemcpy(cnt, c->d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost));
@@ -944,6 +1026,57 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
     return TCG_OK;
 }
 
+int tcg_direct_init(tcg_ctx *c) {
+    if (!c) return TCG_E_ARG;
+    if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_direct_init before tcg_set_basis");
+    int rc = prepare(c);
+    if (rc) return rc;
+    double t2 = 0.0, t3 = 0.0;
+    rc = setup_quartets(c, t2, t3);
+    if (rc) return rc;
+    c->direct_ready = true;
+    return TCG_OK;
+}
+
+int tcg_direct_jk(tcg_ctx *c, int nbas, const double *P, double *J, double *K) {
+    if (!c) return TCG_E_ARG;
+    if (!c->direct_ready || !c->prepared) return c->fail(TCG_E_STATE, "tcg_direct_jk: direct_init not called");
+    if (nbas != c->nbas || !P || !J || !K) return c->fail(TCG_E_ARG, "tcg_direct_jk: bad arguments");
+    const int nb = c->nbas, nf = c->nfuc;
+    const size_t nn = (size_t)nb * nb, njk = (size_t)nf * nb;
+    HIPCHK(c, c->d_P.alloc(nn));
+    HIPCHK(c, c->d_J.alloc(njk));
+    HIPCHK(c, c->d_K.alloc(njk));
+    HIPCHK(c, hipMemcpyAsync(c->d_P.p, P, sizeof(double) * nn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_fill(c->d_J.p, njk, 0.0, c->st));
+    HIPCHK(c, launch_fill(c->d_K.p, njk, 0.0, c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
+    EriArgs A = base_args(c);
+    A.q4 = c->d_q4.p;
+    A.nq4 = c->nq4;
+    A.screen_prim = 1;
+    A.Qao = c->d_Q.p;
+    A.Pden = c->d_P.p;
+    A.Jacc = c->d_J.p;
+    A.Kacc = c->d_K.p;
+    const double t0 = now_ms();
+    int rc = run_quartets(c, A);
+    if (rc) return rc;
+    std::vector<double> Jr(njk), Kr(njk);
+    HIPCHK(c, hipMemcpyAsync(Jr.data(), c->d_J.p, sizeof(double) * njk, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(Kr.data(), c->d_K.p, sizeof(double) * njk, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::fill(J, J + nn, 0.0);
+    std::fill(K, K + nn, 0.0);
+    for (int q = 0; q < nb; ++q)
+        for (int p = 0; p < nf; ++p) {
+            J[p + (size_t)nb * q] = Jr[p + (size_t)nf * q];
+            K[p + (size_t)nb * q] = Kr[p + (size_t)nf * q];
+        }
+    c->times[3] = now_ms() - t0;
+    return TCG_OK;
+}
+
 int tcg_one_electron(tcg_ctx *c, int natom, const double *xyz, const int *Z, double *S, double *T, double *V) {
     if (!c) return TCG_E_ARG;
     if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_one_electron before tcg_set_basis");
@@ -975,6 +1108,7 @@ int tcg_set_quadrature(tcg_ctx *c, int fast, int nquad) {
         c->nqp = ((nquad + kNodeTile - 1) / kNodeTile) * kNodeTile;
         c->prepared = false;
         c->cols_ready = false;
+        c->direct_ready = false;
     }
     return TCG_OK;
 }

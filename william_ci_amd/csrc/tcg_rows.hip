@@ -25,6 +25,9 @@
 //                 block itself (Schwarz passes).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "tcg_device.hpp"
 
 namespace tcg {
@@ -385,7 +388,72 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
 }
 
 // ---------------------------------------------------------------------------
-// contraction and output, one wavefront per shell quartet
+// contraction, one wavefront per shell quartet: the block
+//   blk(a,b,c,d) = sum_g fac_g cA cB cC cD sum_k w_k Ix Iy Iz
+// accumulated into blk (zeroed by the caller) with LDS atomics, tasks
+// (primitive quartet g, component) spread over the 64 lanes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q, long s0, long ns, long r0,
+                                              const Surv *__restrict__ surv, const double *__restrict__ rowval,
+                                              const double *__restrict__ realval, double *blk, int lane) {
+    const int nqp = A.nqp, nq = A.nq, ndp = A.ndp, nreal = 3 - ndp;
+    const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
+    const int lA = sA.l, lB = sB.l, lC = sC.l, lD = sD.l;
+    const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
+    const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
+    const int nblk = nA * nB * nC * nD;
+    for (long task = lane; task < ns * nblk; task += 64) {
+        const long g = task / nblk;
+        const int comp = (int)(task % nblk);
+        int t = comp;
+        const int ca = t % nA; t /= nA;
+        const int cb = t % nB; t /= nB;
+        const int cc = t % nC;
+        const int cd = t / nC;
+        const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
+        const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
+        const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
+        const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
+        const double *I3[3];
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) {
+            const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
+            if (dd < ndp)
+                I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
+            else
+                I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
+        }
+        double val = 0.0;
+        for (int k = 0; k < nq; ++k) val += (I3[0][k] * A.tweight[k]) * I3[1][k] * I3[2][k];
+        const Surv S = surv[s0 + g];
+        const PrimPair &pb = A.pp[S.pb];
+        const PrimPair &pk = A.pp[S.pk];
+        const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
+        const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
+        const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
+        const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
+        atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
+    }
+    if (A.counters && lane == 0)
+        atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)ncomb * nreal * 2 * nq *
+                                                                (lA + lB + lC + lD + 1) + 3ull * nq * nblk));
+}
+
+// the stored position (ERI_toroidal.f90:433-449 index order, first index in
+// the unit cell) of component quartet f4 under permutation ip
+__device__ __forceinline__ void stored_position(const EriArgs &A, const int f4[4], int ip, int pp[4]) {
+    int t4[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) t4[x] = f4[c_perm8_rows[ip][x]];
+    const int c0 = t4[0] / A.nfuc;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) pp[x] = A.relc[c0 * A.ncell + t4[x] / A.nfuc] * A.nfuc + t4[x] % A.nfuc;
+}
+
+// ---------------------------------------------------------------------------
+// contraction and output, one wavefront per shell quartet: the block, then
+// written out (Schwarz passes) or scattered to the 8 permutations translated
+// to the unit cell (src/Toroidal/ERI_toroidal.f90:416-454).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__restrict__ soff,
                                                  const long long *__restrict__ roff, const Surv *__restrict__ surv,
@@ -393,54 +461,16 @@ __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__r
                                                  const double *__restrict__ realval) {
     extern __shared__ double blk[];                        // [max_blk]
     const int lane = threadIdx.x;
-    const int nqp = A.nqp, nq = A.nq, ndp = A.ndp, nreal = 3 - ndp;
     for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
         const Quartet Q = A.q4[qi];
-        const Shell sA = A.sh[Q.s[0]], sB = A.sh[Q.s[1]], sC = A.sh[Q.s[2]], sD = A.sh[Q.s[3]];
-        const int lA = sA.l, lB = sB.l, lC = sC.l, lD = sD.l;
+        const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
         const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
-        const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
         const int nblk = nA * nB * nC * nD;
-        const int NN = lA + lB + lC + lD;
         const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
         __syncthreads();
         for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
         __syncthreads();
-        for (long task = lane; task < ns * nblk; task += 64) {
-            const long g = task / nblk;
-            const int comp = (int)(task % nblk);
-            int t = comp;
-            const int ca = t % nA; t /= nA;
-            const int cb = t % nB; t /= nB;
-            const int cc = t % nC;
-            const int cd = t / nC;
-            const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
-            const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
-            const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
-            const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
-            const double *I3[3];
-#pragma unroll
-            for (int dd = 0; dd < 3; ++dd) {
-                const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
-                if (dd < ndp)
-                    I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
-                else
-                    I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
-            }
-            double val = 0.0;
-            for (int k = 0; k < nq; ++k) val += (I3[0][k] * A.tweight[k]) * I3[1][k] * I3[2][k];
-            const Surv S = surv[s0 + g];
-            const PrimPair &pb = A.pp[S.pb];
-            const PrimPair &pk = A.pp[S.pk];
-            const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
-            const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
-            const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
-            const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
-            atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
-        }
-        if (A.counters && lane == 0)
-            atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)ncomb * nreal * 2 * nq * (NN + 1) +
-                                                                3ull * nq * nblk));
+        quartet_block(A, Q, s0, ns, r0, surv, rowval, realval, blk, lane);
         __syncthreads();
         if (A.blk_out && Q.out >= 0) {
             for (int i = lane; i < nblk; i += 64) A.blk_out[Q.out + i] = blk[i];
@@ -455,10 +485,8 @@ __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__r
                 const int cd = t / nC;
                 const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
                 if (A.Qao[(size_t)f4[0] + nb * f4[1]] * A.Qao[(size_t)f4[2] + nb * f4[3]] < kSchwarz) continue;
-                int t4[4], pp[4];
-                for (int x = 0; x < 4; ++x) t4[x] = f4[c_perm8_rows[ip][x]];
-                const int c0 = t4[0] / A.nfuc;
-                for (int x = 0; x < 4; ++x) pp[x] = A.relc[c0 * A.ncell + t4[x] / A.nfuc] * A.nfuc + t4[x] % A.nfuc;
+                int pp[4];
+                stored_position(A, f4, ip, pp);
                 const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * (size_t)pp[3]))
                                                : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * (size_t)pp[3]));
                 A.eri[idx] = blk[comp];
@@ -468,6 +496,130 @@ __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__r
 }
 
 size_t contract_lds_bytes(const EriArgs &a) { return sizeof(double) * (size_t)a.max_blk; }
+
+// ---------------------------------------------------------------------------
+// Direct J/K (src/module/eri_direct.f90:262-374): every stored element of a
+// block once, J(p1,p2) += E P(p3,p4), K(p1,p4) -= E P(p2,p3) / 2.
+// "Once": a (component, permutation) pair counts only if no pair earlier in
+// the reference's loop order (components a fastest, then permutations 1..8)
+// reaches the same stored position -- the hash of the reference
+// (:331-359) expressed as a test on the 8 inverse images of the position.
+// Four wavefronts per workgroup, one shell quartet each; J and K of the
+// workgroup accumulate in LDS (flushed once at the end with global atomics)
+// when they fit, else directly in global memory.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool first_generator(const EriArgs &A, const Shell &sA, const Shell &sB, const Shell &sC,
+                                                const Shell &sD, int comp, int ip, const int pp[4]) {
+    constexpr int perm[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
+    const int key = comp * 8 + ip;
+    const int first[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
+    const int ncmp[4] = {sA.ncomp, sB.ncomp, sC.ncomp, sD.ncomp};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        int q[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) q[perm[j][x]] = pp[x];         // permutation j maps q to pp
+        const int c0 = q[0] / A.nfuc;
+        bool inside = true;
+        int off[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            const int qt = A.relc[c0 * A.ncell + q[x] / A.nfuc] * A.nfuc + q[x] % A.nfuc;
+            off[x] = qt - first[x];
+            inside = inside && off[x] >= 0 && off[x] < ncmp[x];
+        }
+        if (inside) {
+            const int c2 = off[0] + ncmp[0] * (off[1] + ncmp[1] * (off[2] + ncmp[2] * off[3]));
+            if (c2 * 8 + j < key) return false;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long *__restrict__ soff,
+                                                     const long long *__restrict__ roff,
+                                                     const Surv *__restrict__ surv, const double *__restrict__ rowval,
+                                                     const double *__restrict__ realval, int jk_lds) {
+    extern __shared__ double lds[];      // [4][max_blk] blocks, then J[nfuc*nbas], K[nfuc*nbas] when jk_lds
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nf = A.nfuc, nb = A.nbas;
+    const size_t njk = (size_t)nf * nb;
+    double *blk = lds + (size_t)wv * A.max_blk;
+    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk : A.Jacc;
+    double *Kl = jk_lds ? Jl + njk : A.Kacc;
+    if (jk_lds) {
+        for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
+    }
+    __syncthreads();
+    for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
+        const Quartet Q = A.q4[qi];
+        const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
+        const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
+        const int nblk = nA * nB * nC * nD;
+        const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
+        wave_sync();
+        for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
+        wave_sync();
+        quartet_block(A, Q, s0, ns, r0, surv, rowval, realval, blk, lane);
+        wave_sync();
+        for (int task = lane; task < nblk * 8; task += 64) {
+            const int comp = task >> 3, ip = task & 7;
+            int t = comp;
+            const int ca = t % nA; t /= nA;
+            const int cb = t % nB; t /= nB;
+            const int cc = t % nC;
+            const int cd = t / nC;
+            const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
+            if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz)
+                continue;
+            int pp[4];
+            stored_position(A, f4, ip, pp);
+            if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
+            const double val = blk[comp];
+            atomicAdd(&Jl[pp[0] + (size_t)nf * pp[1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
+            atomicAdd(&Kl[pp[0] + (size_t)nf * pp[3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
+        }
+    }
+    if (jk_lds) {
+        __syncthreads();
+        for (size_t i = threadIdx.x; i < njk; i += 256) {
+            if (Jl[i] != 0.0) atomicAdd(&A.Jacc[i], Jl[i]);
+            if (Kl[i] != 0.0) atomicAdd(&A.Kacc[i], Kl[i]);
+        }
+    }
+}
+
+hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                              const double *rowval, const double *realval, hipStream_t st) {
+    if (a.nq4 <= 0) return hipSuccess;
+    const size_t blk_bytes = sizeof(double) * 4 * (size_t)a.max_blk;
+    const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
+    const size_t lds_max = 160 * 1024;
+    // TCG_JK_GLOBAL=1 forces the global-memory accumulation (the path of systems
+    // whose J/K rows exceed the LDS) so that it can be tested on small systems
+    static const bool force_global = [] {
+        const char *e = getenv("TCG_JK_GLOBAL");
+        return e && e[0] == '1';
+    }();
+    const int jk_lds = (blk_bytes + jk_bytes <= lds_max && !force_global) ? 1 : 0;
+    const size_t lds = blk_bytes + (jk_lds ? jk_bytes : 0);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_contract_jk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    // persistent workgroups: as many as fit on the 256 CUs at this LDS size
+    const int per_cu = jk_lds ? (int)std::max<size_t>(1, std::min<size_t>(4, lds_max / lds)) : 4;
+    const int want = (a.nq4 + 3) / 4;
+    const int grid = std::min(want, 256 * per_cu);
+    hipLaunchKernelGGL(k_contract_jk, dim3(grid), dim3(256), lds, st, a, soff, roff, surv, rowval, realval, jk_lds);
+    return hipGetLastError();
+}
 
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                            const double *rowval, const double *realval, hipStream_t st) {
@@ -479,6 +631,41 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
     }
     const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
     hipLaunchKernelGGL(k_contract, dim3(grid), dim3(64), lds, st, a, soff, roff, surv, rowval, realval);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// chunking of long quartet lists
+// ---------------------------------------------------------------------------
+// bounds[k] = first quartet q with roff[q] >= k * cap (k < nchunk), bounds[nchunk] = nq4
+__global__ void k_chunk_bounds(const long long *__restrict__ roff, int nq4, long long cap, int nchunk,
+                               int *__restrict__ bounds) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nchunk) return;
+    if (k == nchunk) { bounds[k] = nq4; return; }
+    const long long target = (long long)k * cap;
+    int lo = 0, hi = nq4;                       // roff[0..nq4] is non-decreasing
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (roff[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    bounds[k] = lo;
+}
+
+// dst[i] = src[i] - src[0], i = 0..n-1
+__global__ void k_rebase(const long long *__restrict__ src, int n, long long *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] - src[0];
+}
+
+hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, int nchunk, int *bounds, hipStream_t st) {
+    hipLaunchKernelGGL(k_chunk_bounds, dim3((nchunk + 1 + 63) / 64), dim3(64), 0, st, roff, nq4, cap, nchunk, bounds);
+    return hipGetLastError();
+}
+
+hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rebase, dim3((n + 255) / 256), dim3(256), 0, st, src, n, dst);
     return hipGetLastError();
 }
 

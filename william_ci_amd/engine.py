@@ -42,6 +42,8 @@ def lib():
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
         L.tcg_eri_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_longlong)]
+        L.tcg_direct_init.argtypes = [_vp]
+        L.tcg_direct_jk.argtypes = [_vp, _ci, _dp, _dp, _dp]
         L.tcg_set_quadrature.argtypes = [_vp, _ci, _ci]
         L.tcg_get_quadrature.argtypes = [_vp, ctypes.POINTER(_ci), ctypes.POINTER(_ci), _vp, _vp]
         L.tcg_fast_quad_self_check.argtypes = [_vp, _ci, _dp, _ip, _ci, ctypes.POINTER(ctypes.c_double),
@@ -109,7 +111,8 @@ class Engine:
         out = np.zeros(8)
         lib().tcg_eri_times(self.h, out, 8)
         return {"tables_pairs_ms": out[0], "schwarz_ms": out[1], "quartet_list_ms": out[2],
-                "quartets_ms": out[3], "total_ms": out[4], "kernel_ms": out[5], "rows_kernel_ms": out[6]}
+                "quartets_ms": out[3], "total_ms": out[4], "kernel_ms": out[5], "rows_kernel_ms": out[6],
+                "t_tables_ms": out[7]}
 
     def eri_device(self):
         p = _vp()
@@ -177,6 +180,22 @@ class Engine:
         K = np.zeros(nb * nb)
         ep = None if eri is None else np.ascontiguousarray(eri).ctypes.data
         self._check(lib().tcg_jk_stored(self.h, n1, nb, Pf.ravel(), ep, J, K), "tcg_jk_stored")
+        return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
+
+
+    # -- direct_init / direct_JK -------------------------------------------------
+    def direct_init(self):
+        """Schwarz tables and the unique quartet list for the direct J/K."""
+        self._check(lib().tcg_direct_init(self.h), "tcg_direct_init")
+
+    def direct_jk(self, P: np.ndarray):
+        """J, K (row-major numpy, rows > nfuc zero) with the integrals
+        recomputed on the GPU and contracted on the fly."""
+        nb = self.s.nbas
+        Pf = np.ascontiguousarray(np.asarray(P, dtype=np.float64).T)
+        J = np.zeros(nb * nb)
+        K = np.zeros(nb * nb)
+        self._check(lib().tcg_direct_jk(self.h, nb, Pf.ravel(), J, K), "tcg_direct_jk")
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
 
 
