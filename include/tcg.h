@@ -50,7 +50,8 @@ int tcg_set_basis(tcg_ctx *ctx, int nbas, int nfuc, const double *center, const 
 
 /* Two-electron integrals.  Replaces
  *   ERI_integral_toroidal(ndp,nBas,AO,ERI)   src/Toroidal/ERI_toroidal.f90:36
- * eri: ERI(n1,nBas,nBas,nBas) with n1 = nfuc (sao != 0, keyword SAO) or nBas;
+ * eri: ERI(n1,nBas,nBas,nBas) with n1 = nfuc (sao != 0, keyword SAO) or nBas
+ * (rows of the other cells filled by translation, src/Tools/symmetric_matrix.f90:62);
  * chemist order ERI(i,j,k,l) = (ij|kl), or physicist ERI(i,k,j,l) when
  * physicist != 0.  The tensor also stays resident on the GPU for
  * tcg_jk_stored.  eri may be NULL: then it is kept on the GPU only. */
@@ -61,9 +62,19 @@ int tcg_eri_build(tcg_ctx *ctx, int sao, int physicist, double *eri);
  *                                   src/Energies/hartree_exchange_potential.f90:30
  * J(mu,nu) = sum P(la,si)(mu nu|la si), K(mu,nu) = -1/2 sum P(la,si)(mu la|si nu),
  * rows mu <= n1 written, the rest zero; P, J, K are (nBas,nBas).
- * eri == NULL uses the tensor resident from tcg_eri_build; otherwise eri is
- * the caller's ERI(n1,nBas,nBas,nBas) (uploaded for this call). */
+ * eri == NULL uses the tensor resident from tcg_eri_build (in its storage
+ * order); otherwise eri is the caller's chemist ERI(n1,nBas,nBas,nBas)
+ * (uploaded for this call). */
 int tcg_jk_stored(tcg_ctx *ctx, int n1, int nbas, const double *P, const double *eri, double *J, double *K);
+
+/* The same for a caller's array in either order (the reference reads the
+ * keyword physicist, src/Energies/hartree_exchange_potential.f90:124-146):
+ * physicist != 0 means eri holds ERI(i,k,j,l) = (ij|kl); then
+ * J(mu,la) = sum P(nu,si) ERI(mu,nu,la,si).  With eri == NULL the resident
+ * tensor is used in the order it was built and physicist is ignored
+ * (tcg_jk_stored behaves the same way). */
+int tcg_jk_stored_order(tcg_ctx *ctx, int n1, int nbas, const double *P, const double *eri, int physicist,
+                        double *J, double *K);
 
 /* Direct J/K: the integrals are recomputed at every call and contracted on
  * the fly, nothing of size nBas^4 is stored.  Replaces
@@ -76,6 +87,18 @@ int tcg_jk_stored(tcg_ctx *ctx, int n1, int nbas, const double *P, const double 
  * rows mu <= nfuc written, the rest zero; P, J, K are (nBas,nBas). */
 int tcg_direct_init(tcg_ctx *ctx);
 int tcg_direct_jk(tcg_ctx *ctx, int nbas, const double *P, double *J, double *K);
+
+/* Gamma-supercell MP2 from the resident full ERI (tcg_eri_build with sao = 0,
+ * chemist or physicist).  Replaces
+ *   AO_to_MO_ERI_inplace(nBas,c,ERI,physicist)   src/Tools/AO_to_MO/AO_to_MO.f90:20-111
+ *   MP2(nBas,nO,e,ERI,ENuc,EHF,physicist_MO)     src/Energies/MP2/MP2.f90:1-87
+ * C(nBas,nBas) MO coefficients (columns ascending in e), e(nBas) orbital
+ * energies, nocc occupied orbitals.  Only (ia|jb) is formed (four FP64
+ * matrix-core GEMMs, no second nBas^4 array); the resident ERI is unchanged.
+ * out[0] = E_c(MP2) = 2 E2d - E2x, out[1] = direct part 2 E2d, out[2] =
+ * exchange part -E2x (the three lines the reference prints), out[3] = GPU
+ * wall time in ms. */
+int tcg_mp2_gamma(tcg_ctx *ctx, int nbas, int nocc, const double *e, const double *C, double *out);
 
 /* One-electron matrices S, T, V (nBas,nBas): rows of the unit cell computed,
  * values below 1e-14 zeroed, the other rows filled by translation.  Replaces

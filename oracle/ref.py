@@ -42,6 +42,7 @@ def lib():
         L.ref_jk.argtypes = [_ci, _ci, _dp, _dp, _dp, _dp]
         L.ref_fast_quad_self_check.argtypes = [_ci, _dp, _ip, _ci, ctypes.c_char_p, _dp]
         L.ref_direct_jk.argtypes = [_dp, _dp, _dp]
+        L.ref_mp2.argtypes = [_ci, _ci, _dp, _dp, _dp, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -114,6 +115,23 @@ class RefSystem:
         K = np.zeros(nb * nb)
         lib().ref_direct_jk(Pf.ravel(), J, K)
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
+
+    def mp2(self, E_full, e, C, nocc):
+        """The reference's AO_to_MO_ERI_inplace + MP2 on the full chemist ERI
+        (C-ordered E[l,k,j,i] = ERI(i,j,k,l), as eri_full returns it with
+        sao=False), orbital energies e and coefficients C (row-major numpy,
+        columns = orbitals).  Returns the correlation energy it prints."""
+        import re
+        import tempfile
+        nb = self.s.nbas
+        eri = np.ascontiguousarray(E_full, dtype=np.float64).ravel().copy()
+        Cf = np.ascontiguousarray(np.asarray(C, dtype=np.float64).T).ravel()
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "mp2.out")
+            lib().ref_mp2(nb, int(nocc), np.ascontiguousarray(e, dtype=np.float64), Cf, eri, path.encode())
+            text = open(path).read()
+        m = re.search(r"MP2 correlation energy =\s*([-0-9.E+]+)", text)
+        return float(m.group(1))
 
     def fast_quad_self_check(self, nq128: bool = False):
         """The reference's fast_quad_self_check on this system (the fast

@@ -17,6 +17,8 @@
 !   NRE                        src/Energies/Nuclear_repulsion.f90:1    !
 !   fast_quad_self_check       src/Toroidal/fast_quad_self_check.f90:30 !
 !   direct_init / direct_JK    src/module/eri_direct.f90:76,262        !
+!   AO_to_MO_ERI_inplace, MP2  src/Tools/AO_to_MO/AO_to_MO.f90:20,     !
+!                              src/Energies/MP2/MP2.f90:1              !
 !---------------------------------------------------------------------!
 module ref_bind
 
@@ -209,6 +211,28 @@ contains
           call nuclear_attraction_matrix_toroidal_3D(natom, nb, geom, ats, AOl, V)
         end select
       end subroutine ref_one_electron
+
+      ! Gamma MP2: AO_to_MO_ERI_inplace on eri (overwritten), then MP2, whose
+      ! report goes to the file 'path' (unit outfile)
+      subroutine ref_mp2(nbas, nocc, e, cmo, eri, path) bind(C, name="ref_mp2")
+        use files, only : outfile
+        integer(c_int), value         :: nbas, nocc
+        real(c_double), intent(in)    :: e(nbas), cmo(nbas,nbas)
+        real(c_double), intent(inout) :: eri(nbas,nbas,nbas,nbas)
+        character(kind=c_char), intent(in) :: path(*)
+        character(len=512) :: fname
+        integer :: i
+        fname = ' '
+        i = 1
+        do while (path(i) /= c_null_char .and. i <= 512)
+          fname(i:i) = path(i)
+          i = i + 1
+        end do
+        open(unit=outfile, file=trim(fname), status='replace', action='write')
+        call AO_to_MO_ERI_inplace(nbas, cmo, eri, .false.)
+        call MP2(nbas, nocc, e, eri, 0.d0, 0.d0, .false.)
+        close(outfile)
+      end subroutine ref_mp2
 
       ! direct J/K on the density P(nBas,nBas): direct_init then direct_JK
       subroutine ref_direct_jk(P, J, K) bind(C, name="ref_direct_jk")

@@ -95,3 +95,28 @@ def test_chunked_pipeline_matches(name, monkeypatch):
     print(f"  rows {rows}  max |diff| {np.abs(E1 - E2).max():.2e}")
     assert np.abs(E1 - E2).max() <= 1e-14
     assert np.abs(J1 - J2).max() < 1e-12 and np.abs(K1 - K2).max() < 1e-12
+
+
+@pytest.mark.parametrize("sao", [True, False])
+def test_jk_stored_physicist_order(sao):
+    """J/K from a physicist-ordered tensor (hartree_exchange_potential's
+    physicist branch) equal the chemist ones."""
+    from william_ci_amd.engine import Engine
+    s = tsys.workload("h10_ring")
+    coef = normalised_coef(s)
+    print(f"case h10_ring physicist J/K sao={sao}", flush=True)
+    n1 = s.nfuc if sao else s.nbas
+    rng = np.random.default_rng(4)
+    P = rng.standard_normal((s.nbas, s.nbas))
+    P = P + P.T
+    eng = Engine(s, coef)
+    eng.eri_build(sao=sao, physicist=False, to_host=False)
+    Jc, Kc = eng.jk_stored(P, n1=n1)
+    Ep = eng.eri_build(sao=sao, physicist=True)
+    Jp, Kp = eng.jk_stored(P, n1=n1)
+    Jh, Kh = eng.jk_stored(P, n1=n1, eri=Ep, physicist=True)
+    eng.close()
+    Jo, Ko = orc.OracleSystem(s, coef).jk(P, Ep, physicist=True)
+    for J, K in ((Jp, Kp), (Jh, Kh), (Jo, Ko)):
+        assert np.abs(J - Jc).max() < 1e-12 * np.abs(Jc).max()
+        assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()

@@ -67,6 +67,11 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
                            const double *rowval, const double *realval, hipStream_t st);
 hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, int nchunk, int *bounds, hipStream_t st);
 hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_t st);
+hipError_t launch_translate_rows(double *eri, int nb, int nf, int ncell, const int *relc, hipStream_t st);
+hipError_t launch_dgemm_nn(int M, int N, int K, const double *A, long lda, long long sA, const double *B, long ldb,
+                           long long sB, double *C, long ldc, long long sC, long long batch, hipStream_t st);
+hipError_t launch_mp2_sums(const double *X, const double *e, int o, int v, long long si, long long sa, long long sj,
+                           long long sb, double *part, int nblocks, hipStream_t st);
 size_t scan_temp_bytes(int n);
 hipError_t scan_exclusive(void *tmp, size_t bytes, const long long *in, long long *out, int n, hipStream_t st);
 size_t sort_temp_bytes(int n, int end_bit);
@@ -80,11 +85,11 @@ hipError_t sort_pairs_ll(void *tmp, size_t bytes, const unsigned *kin, unsigned 
                          long long *vout, long long n, int end_bit, hipStream_t st);
 size_t scan_int_temp_bytes(int n);
 hipError_t scan_int_exclusive(void *tmp, size_t bytes, const int *in, long long *out, int n, hipStream_t st);
-hipError_t launch_col_flags(const double *eri, int n1, int nbas, unsigned char *flags, int *cnt_d, int *cnt_b,
-                            unsigned *key_b, hipStream_t st);
+hipError_t launch_col_flags(const double *eri, int n1, int nbas, int physicist, unsigned char *flags, int *cnt_d,
+                            int *cnt_t, unsigned *key_t, hipStream_t st);
 hipError_t launch_jk_sparse(const double *eri, const double *P, const long long *list_d, const long long *off_d,
-                            const long long *list_b, const long long *off_b, int n1, int nbas, double *J, double *K,
-                            hipStream_t st);
+                            const long long *list_t, const long long *off_t, int n1, int nbas, int physicist,
+                            double *J, double *K, hipStream_t st);
 hipError_t launch_gather_keys(const unsigned *keys, const long long *idx, long long n, unsigned *out, hipStream_t st);
 size_t jk_work_doubles(int n1, int nbas);
 hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,

@@ -112,86 +112,119 @@ hipError_t launch_jk_stored(const double *eri, const double *P, double *J, doubl
 // src/Energies/hartree_exchange_potential.f90:54-155): the columns
 // ERI(:,b,c,d) with a non-zero element are listed once per tensor, ordered by
 // d (for K) and by b (for J); each SCF iteration then reads only them.
-//   K(a,d) = -1/2 sum_{(b,c) in list_d} ERI(a,b,c,d) P(b,c)
-//   J(a,b) =      sum_{(c,d) in list_b} ERI(a,b,c,d) P(c,d)
-// One block per d (K) or per b (J): no atomics, deterministic.
+//   K(a,d) = -1/2 sum_{(b,c) in list_d} ERI(a,b,c,d) P(b,c)   (both orders; P symmetric)
+//   J(a,b) =      sum_{(c,d) in list_b} ERI(a,b,c,d) P(c,d)   (chemist)
+//   J(a,c) =      sum_{(b,d) in list_c} ERI(a,b,c,d) P(b,d)   (physicist)
+// One block per d (K) or per J column: no global atomics.
 // ---------------------------------------------------------------------------
-__global__ void k_col_flags(const double *__restrict__ eri, int n1, int nbas, unsigned char *__restrict__ flags,
-                            int *__restrict__ cnt_d, int *__restrict__ cnt_b, unsigned *__restrict__ key_b) {
+__global__ void k_col_flags(const double *__restrict__ eri, int n1, int nbas, int physicist,
+                            unsigned char *__restrict__ flags, int *__restrict__ cnt_d, int *__restrict__ cnt_t,
+                            unsigned *__restrict__ key_t) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t ncol = (size_t)nbas * nbas * nbas;
     if (g >= ncol) return;
     bool nz = false;
     for (int a = 0; a < n1; ++a) nz |= eri[(size_t)a + (size_t)n1 * g] != 0.0;
     flags[g] = nz ? 1 : 0;
-    const int b = (int)(g % nbas), d = (int)(g / ((size_t)nbas * nbas));
-    key_b[g] = (unsigned)b;
+    const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas), d = (int)(g / ((size_t)nbas * nbas));
+    const int t = physicist ? c : b;
+    key_t[g] = (unsigned)t;
     if (nz) {
         atomicAdd(&cnt_d[d], 1);
-        atomicAdd(&cnt_b[b], 1);
+        atomicAdd(&cnt_t[t], 1);
     }
 }
 
-__global__ void k_k_sparse(const double *__restrict__ eri, const double *__restrict__ P, const long long *__restrict__ list,
-                           const long long *__restrict__ off, int n1, int nbas, double *__restrict__ K) {
-    extern __shared__ double acc[];      // [n1]
-    const int d = blockIdx.x;
-    for (int a = threadIdx.x; a < n1; a += blockDim.x) acc[a] = 0.0;
+// Column list entries g = b + nbas (c + nbas d); K(a,d) for the block's d, J(a,t)
+// for the block's target t (t = b chemist, t = c physicist: J(a,c) += ERI(a,b,c,d)
+// P(b,d), src/Energies/hartree_exchange_potential.f90:136-146).
+// Few rows (n1 <= NA): threads over the list entries, register accumulators
+// per row, one LDS reduction per block.
+template <int NA>
+__global__ __launch_bounds__(256) void k_jk_sparse_small(const double *__restrict__ eri, const double *__restrict__ P,
+                                                         const long long *__restrict__ list,
+                                                         const long long *__restrict__ off, int n1, int nbas, int is_j,
+                                                         int physicist, double *__restrict__ out) {
+    __shared__ double acc[NA];
+    const int col = blockIdx.x;
+    for (int a = threadIdx.x; a < NA; a += blockDim.x) acc[a] = 0.0;
     __syncthreads();
-    double loc[32];
-    const int na = n1 < 32 ? n1 : 32;
-    for (int a = 0; a < na; ++a) loc[a] = 0.0;
-    for (long long i = off[d] + threadIdx.x; i < off[d + 1]; i += blockDim.x) {
+    double loc[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) loc[a] = 0.0;
+    for (long long i = off[col] + threadIdx.x; i < off[col + 1]; i += blockDim.x) {
         const long long g = list[i];
-        const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas);
-        const double pbc = P[b + (size_t)nbas * c];
+        const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas), d = (int)(g / ((long long)nbas * nbas));
+        double pw;
+        if (!is_j) pw = P[b + (size_t)nbas * c];
+        else pw = physicist ? P[b + (size_t)nbas * d] : P[c + (size_t)nbas * d];
         const double *e = eri + (size_t)n1 * g;
-        for (int a = 0; a < na; ++a) loc[a] += e[a] * pbc;
-        for (int a = na; a < n1; ++a) atomicAdd(&acc[a], e[a] * pbc);
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+            if (a < n1) loc[a] += e[a] * pw;
     }
-    for (int a = 0; a < na; ++a) atomicAdd(&acc[a], loc[a]);
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+        if (a < n1) atomicAdd(&acc[a], loc[a]);
     __syncthreads();
-    for (int a = threadIdx.x; a < n1; a += blockDim.x) K[a + (size_t)nbas * d] = -0.5 * acc[a];
+    const double f = is_j ? 1.0 : -0.5;
+    for (int a = threadIdx.x; a < n1; a += blockDim.x) out[a + (size_t)nbas * col] = f * acc[a];
 }
 
-__global__ void k_j_sparse(const double *__restrict__ eri, const double *__restrict__ P, const long long *__restrict__ list,
-                           const long long *__restrict__ off, int n1, int nbas, double *__restrict__ J) {
-    extern __shared__ double acc[];      // [n1]
-    const int b = blockIdx.x;
-    for (int a = threadIdx.x; a < n1; a += blockDim.x) acc[a] = 0.0;
-    __syncthreads();
-    double loc[32];
-    const int na = n1 < 32 ? n1 : 32;
-    for (int a = 0; a < na; ++a) loc[a] = 0.0;
-    for (long long i = off[b] + threadIdx.x; i < off[b + 1]; i += blockDim.x) {
-        const long long g = list[i];
-        const int c = (int)((g / nbas) % nbas), d = (int)(g / ((long long)nbas * nbas));
-        const double pcd = P[c + (size_t)nbas * d];
-        const double *e = eri + (size_t)n1 * g;
-        for (int a = 0; a < na; ++a) loc[a] += e[a] * pcd;
-        for (int a = na; a < n1; ++a) atomicAdd(&acc[a], e[a] * pcd);
+// Many rows (n1 > 32, the full layout): threads over the rows a (coalesced
+// column reads), the block walks its list.
+__global__ __launch_bounds__(256) void k_jk_sparse_rows(const double *__restrict__ eri, const double *__restrict__ P,
+                                                        const long long *__restrict__ list,
+                                                        const long long *__restrict__ off, int n1, int nbas, int is_j,
+                                                        int physicist, double *__restrict__ out) {
+    const int col = blockIdx.x;
+    const double f = is_j ? 1.0 : -0.5;
+    for (int a0 = 0; a0 < n1; a0 += blockDim.x) {
+        const int a = a0 + threadIdx.x;
+        double acc = 0.0;
+        for (long long i = off[col]; i < off[col + 1]; ++i) {
+            const long long g = list[i];
+            const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas), d = (int)(g / ((long long)nbas * nbas));
+            double pw;
+            if (!is_j) pw = P[b + (size_t)nbas * c];
+            else pw = physicist ? P[b + (size_t)nbas * d] : P[c + (size_t)nbas * d];
+            if (a < n1) acc += eri[(size_t)a + (size_t)n1 * g] * pw;
+        }
+        if (a < n1) out[a + (size_t)nbas * col] = f * acc;
     }
-    for (int a = 0; a < na; ++a) atomicAdd(&acc[a], loc[a]);
-    __syncthreads();
-    for (int a = threadIdx.x; a < n1; a += blockDim.x) J[a + (size_t)nbas * b] = acc[a];
 }
 
-hipError_t launch_col_flags(const double *eri, int n1, int nbas, unsigned char *flags, int *cnt_d, int *cnt_b,
-                            unsigned *key_b, hipStream_t st) {
+static hipError_t launch_one(const double *eri, const double *P, const long long *list, const long long *off, int n1,
+                             int nbas, int is_j, int physicist, double *out, hipStream_t st) {
+    if (n1 <= 8)
+        hipLaunchKernelGGL(k_jk_sparse_small<8>, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+                           physicist, out);
+    else if (n1 <= 16)
+        hipLaunchKernelGGL(k_jk_sparse_small<16>, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+                           physicist, out);
+    else if (n1 <= 32)
+        hipLaunchKernelGGL(k_jk_sparse_small<32>, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+                           physicist, out);
+    else
+        hipLaunchKernelGGL(k_jk_sparse_rows, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+                           physicist, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_col_flags(const double *eri, int n1, int nbas, int physicist, unsigned char *flags, int *cnt_d,
+                            int *cnt_t, unsigned *key_t, hipStream_t st) {
     const size_t ncol = (size_t)nbas * nbas * nbas;
-    hipLaunchKernelGGL(k_col_flags, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, eri, n1, nbas, flags, cnt_d,
-                       cnt_b, key_b);
+    hipLaunchKernelGGL(k_col_flags, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, eri, n1, nbas, physicist,
+                       flags, cnt_d, cnt_t, key_t);
     return hipGetLastError();
 }
 
 hipError_t launch_jk_sparse(const double *eri, const double *P, const long long *list_d, const long long *off_d,
-                            const long long *list_b, const long long *off_b, int n1, int nbas, double *J, double *K,
-                            hipStream_t st) {
-    hipLaunchKernelGGL(k_k_sparse, dim3(nbas), dim3(256), sizeof(double) * n1, st, eri, P, list_d, off_d, n1, nbas, K);
-    hipError_t e = hipGetLastError();
+                            const long long *list_t, const long long *off_t, int n1, int nbas, int physicist,
+                            double *J, double *K, hipStream_t st) {
+    hipError_t e = launch_one(eri, P, list_d, off_d, n1, nbas, 0, physicist, K, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_j_sparse, dim3(nbas), dim3(256), sizeof(double) * n1, st, eri, P, list_b, off_b, n1, nbas, J);
-    return hipGetLastError();
+    return launch_one(eri, P, list_t, off_t, n1, nbas, 1, physicist, J, st);
 }
 
 __global__ void k_gather_keys(const unsigned *__restrict__ keys, const long long *__restrict__ idx, long long n,

@@ -118,6 +118,8 @@ struct tcg_ctx {
     DevBuf<long long> d_list_d, d_list_b, d_off_d, d_off_b;
     size_t eri_count = 0;
     int eri_n1 = 0;
+    bool eri_physicist = false;
+    int cols_phys = 0;           // storage order the column lists were built for
     long long stats[8] = {0};
     double times[8] = {0};
     int max_ncomb = 1, max_blk = 1;
@@ -891,7 +893,6 @@ int tcg_set_basis(tcg_ctx *c, int nbas, int nfuc, const double *center, const in
 int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     if (!c) return TCG_E_ARG;
     if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_build before tcg_set_basis");
-    if (!sao) return c->fail(TCG_E_ARG, "tcg_eri_build: this version builds the SAO layout ERI(nfuc,nBas,nBas,nBas) only");
     const double t0 = now_ms();
     int rc = prepare(c);
     if (rc) return rc;
@@ -936,6 +937,9 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     c->times[5] = kms;
     (void)rms;
     c->times[6] = c->rows_ms;
+    // no SAO: rows of the other cells by translation (ERI_toroidal.f90:549-552)
+    if (!sao) HIPCHK(c, launch_translate_rows(c->d_eri.p, nb, nf, c->ncell, c->d_relc.p, c->st));
+    c->eri_physicist = physicist != 0;
     const double t4 = now_ms();
     unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HIPCHK(c, hipMemcpy(cnt, c->d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost));
@@ -955,6 +959,11 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
 }
 
 int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *eri, double *J, double *K) {
+    return tcg_jk_stored_order(c, n1, nbas, P, eri, 0, J, K);
+}
+
+int tcg_jk_stored_order(tcg_ctx *c, int n1, int nbas, const double *P, const double *eri, int phys_arg, double *J,
+                        double *K) {
     if (!c) return TCG_E_ARG;
     if (!P || !J || !K || nbas <= 0 || n1 <= 0 || n1 > nbas) return c->fail(TCG_E_ARG, "tcg_jk_stored: bad arguments");
     const size_t count = (size_t)n1 * nbas * nbas * nbas;
@@ -969,6 +978,11 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
     } else if (!src || c->eri_count != count || c->eri_n1 != n1) {
         return c->fail(TCG_E_STATE, "tcg_jk_stored: no resident ERI of this shape (call tcg_eri_build)");
     }
+    // a caller's array follows the caller's order; the resident one the order it was built in
+    const int physicist = eri ? (phys_arg != 0) : (c->eri_physicist ? 1 : 0);
+    if (eri) c->eri_physicist = physicist != 0;
+    if (eri && c->cols_phys != physicist) c->cols_ready = false;
+    if (!eri && c->cols_ready && c->cols_phys != physicist) c->cols_ready = false;
     const size_t nn = (size_t)nbas * nbas;
     if (!c->cols_ready) {
         // non-zero columns ERI(:,b,c,d), listed by d and by b (once per tensor)
@@ -982,7 +996,7 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
         HIPCHK(c, keyb.alloc(ncol));
         HIPCHK(c, nsel.alloc(1));
         HIPCHK(c, hipMemsetAsync(cnt.p, 0, sizeof(int) * (2 * (size_t)nbas + 2), c->st));
-        HIPCHK(c, launch_col_flags(src, n1, nbas, flags.p, cnt.p, cnt.p + nbas + 1, keyb.p, c->st));
+        HIPCHK(c, launch_col_flags(src, n1, nbas, physicist, flags.p, cnt.p, cnt.p + nbas + 1, keyb.p, c->st));
         size_t tb = std::max(select_temp_bytes(ncol), scan_int_temp_bytes(nbas + 1));
         HIPCHK(c, c->d_tmp.alloc(tb));
         HIPCHK(c, tmp_list.alloc(ncol));
@@ -1011,6 +1025,7 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
         }
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->cols_ready = true;
+        c->cols_phys = physicist;
     }
     HIPCHK(c, c->d_P.alloc(nn));
     HIPCHK(c, c->d_J.alloc(nn));
@@ -1019,7 +1034,7 @@ int tcg_jk_stored(tcg_ctx *c, int n1, int nbas, const double *P, const double *e
     HIPCHK(c, launch_fill(c->d_J.p, nn, 0.0, c->st));
     HIPCHK(c, launch_fill(c->d_K.p, nn, 0.0, c->st));
     HIPCHK(c, launch_jk_sparse(src, c->d_P.p, c->d_list_d.p, c->d_off_d.p, c->d_list_b.p, c->d_off_b.p, n1, nbas,
-                               c->d_J.p, c->d_K.p, c->st));
+                               physicist, c->d_J.p, c->d_K.p, c->st));
     HIPCHK(c, hipMemcpyAsync(J, c->d_J.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(K, c->d_K.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1074,6 +1089,74 @@ int tcg_direct_jk(tcg_ctx *c, int nbas, const double *P, double *J, double *K) {
             K[p + (size_t)nb * q] = Kr[p + (size_t)nf * q];
         }
     c->times[3] = now_ms() - t0;
+    return TCG_OK;
+}
+
+int tcg_mp2_gamma(tcg_ctx *c, int nbas, int nocc, const double *e, const double *C, double *out) {
+    if (!c) return TCG_E_ARG;
+    if (nbas != c->nbas || nocc <= 0 || nocc >= nbas || !e || !C || !out)
+        return c->fail(TCG_E_ARG, "tcg_mp2_gamma: bad arguments");
+    if (c->eri_n1 != nbas || !c->d_eri.p)
+        return c->fail(TCG_E_STATE, "tcg_mp2_gamma needs the full ERI resident (tcg_eri_build with sao = 0)");
+    const int N = nbas, o = nocc, v = nbas - nocc;
+    const size_t N2 = (size_t)N * N, N3 = N2 * N;
+    // occupied / virtual columns and the transposed virtual block, column-major
+    std::vector<double> Co((size_t)N * o), Cv((size_t)N * v), CvT((size_t)v * N);
+    for (int p = 0; p < o; ++p)
+        for (int m = 0; m < N; ++m) Co[m + (size_t)N * p] = C[m + (size_t)N * p];
+    for (int a = 0; a < v; ++a)
+        for (int m = 0; m < N; ++m) Cv[m + (size_t)N * a] = C[m + (size_t)N * (o + a)];
+    for (int m = 0; m < N; ++m)
+        for (int a = 0; a < v; ++a) CvT[a + (size_t)v * m] = Cv[m + (size_t)N * a];
+    // positions 4, 3, 2, 1 of the stored array contracted in that order.
+    // chemist ERI(mu,nu,la,si) = (mu nu|la si): 4 <- occ (j), 3 <- vir (b), 2 <- occ (i), 1 <- vir (a)
+    //   X(a,i,b,j) = (ai|bj) = (ia|jb)
+    // physicist ERI(mu,la,nu,si) = (mu nu|la si): 4 <- occ (j), 3 <- occ (i), 2 <- vir (b), 1 <- vir (a)
+    //   X(a,b,i,j) = (ia|jb)
+    const bool phys = c->eri_physicist;
+    const int d4 = o, d3 = phys ? o : v, d2 = phys ? v : o, d1 = v;
+    DevBuf<double> dCo, dCv, dCvT, dY1, dY2, dY3, dX, de, dpart;
+    HIPCHK(c, dCo.upload(Co));
+    HIPCHK(c, dCv.upload(Cv));
+    HIPCHK(c, dCvT.upload(CvT));
+    HIPCHK(c, dY1.alloc(N3 * d4));
+    HIPCHK(c, dY2.alloc(N2 * d3 * d4));
+    HIPCHK(c, dY3.alloc((size_t)N * d2 * d3 * d4));
+    HIPCHK(c, dX.alloc((size_t)d1 * d2 * d3 * d4));
+    const double *C4 = dCo.p, *C3 = phys ? dCo.p : dCv.p, *C2 = phys ? dCv.p : dCo.p;
+    hipStream_t st = c->st;
+    const double t0 = now_ms();
+    // Y1(mu,nu,la,q4) = sum_si ERI(mu,nu,la,si) C4(si,q4)
+    HIPCHK(c, launch_dgemm_nn((int)N3, d4, N, c->d_eri.p, (long)N3, 0, C4, N, 0, dY1.p, (long)N3, 0, 1, st));
+    // Y2(mu,nu,q3,q4) = sum_la Y1(mu,nu,la,q4) C3(la,q3), batched over q4
+    HIPCHK(c, launch_dgemm_nn((int)N2, d3, N, dY1.p, (long)N2, (long long)N3, C3, N, 0, dY2.p, (long)N2,
+                              (long long)(N2 * d3), d4, st));
+    // Y3(mu,q2,q3,q4) = sum_nu Y2(mu,nu,q3,q4) C2(nu,q2), batched over (q3,q4)
+    HIPCHK(c, launch_dgemm_nn(N, d2, N, dY2.p, N, (long long)N2, C2, N, 0, dY3.p, N, (long long)N * d2,
+                              (long long)d3 * d4, st));
+    // X(q1,q2,q3,q4) = sum_mu Cv(mu,q1) Y3(mu,q2,q3,q4)
+    HIPCHK(c, launch_dgemm_nn(d1, d2 * d3 * d4, N, dCvT.p, d1, 0, dY3.p, N, 0, dX.p, d1, 0, 1, st));
+    std::vector<double> ev(e, e + nbas);
+    HIPCHK(c, de.upload(ev));
+    const int nblk = 1024;
+    HIPCHK(c, dpart.alloc(2 * nblk));
+    // strides of (i, a, j, b) in X
+    long long si, sa, sj, sb;
+    if (!phys) { sa = 1; si = v; sb = (long long)v * o; sj = (long long)v * o * v; }
+    else { sa = 1; sb = v; si = (long long)v * v; sj = (long long)v * v * o; }
+    HIPCHK(c, launch_mp2_sums(dX.p, de.p, o, v, si, sa, sj, sb, dpart.p, nblk, st));
+    std::vector<double> part(2 * nblk);
+    HIPCHK(c, hipMemcpyAsync(part.data(), dpart.p, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    double E2d = 0.0, E2x = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        E2d += part[2 * b];
+        E2x += part[2 * b + 1];
+    }
+    out[0] = 2.0 * E2d - E2x;
+    out[1] = 2.0 * E2d;
+    out[2] = -E2x;
+    out[3] = now_ms() - t0;
     return TCG_OK;
 }
 

@@ -38,6 +38,8 @@ def lib():
         L.tcg_set_basis.argtypes = [_vp, _ci, _ci, _dp, _ip, _ip, _dp, _dp]
         L.tcg_eri_build.argtypes = [_vp, _ci, _ci, _vp]
         L.tcg_jk_stored.argtypes = [_vp, _ci, _ci, _dp, _vp, _dp, _dp]
+        L.tcg_jk_stored_order.argtypes = [_vp, _ci, _ci, _dp, _vp, _ci, _dp, _dp]
+        L.tcg_mp2_gamma.argtypes = [_vp, _ci, _ci, _dp, _dp, _dp]
         L.tcg_one_electron.argtypes = [_vp, _ci, _dp, _ip, _dp, _dp, _dp]
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
@@ -170,18 +172,32 @@ class Engine:
                     "tcg_set_basis")
 
     # -- hartree_exchange_potential ---------------------------------------------
-    def jk_stored(self, P: np.ndarray, n1: int | None = None, eri: np.ndarray | None = None):
+    def jk_stored(self, P: np.ndarray, n1: int | None = None, eri: np.ndarray | None = None,
+                  physicist: bool = False):
         """J, K as ordinary row-major numpy matrices (rows > n1 zero), from the
-        resident ERI (eri None) or from a host ERI in the reference layout."""
+        resident ERI (eri None, in the order it was built) or from a host ERI
+        in the reference layout (chemist, or physicist when asked)."""
         nb = self.s.nbas
         n1 = self.s.nfuc if n1 is None else n1
         Pf = np.ascontiguousarray(np.asarray(P, dtype=np.float64).T)
         J = np.zeros(nb * nb)
         K = np.zeros(nb * nb)
         ep = None if eri is None else np.ascontiguousarray(eri).ctypes.data
-        self._check(lib().tcg_jk_stored(self.h, n1, nb, Pf.ravel(), ep, J, K), "tcg_jk_stored")
+        self._check(lib().tcg_jk_stored_order(self.h, n1, nb, Pf.ravel(), ep, int(physicist), J, K),
+                    "tcg_jk_stored")
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
 
+    # -- AO_to_MO_ERI_inplace + MP2 ---------------------------------------------
+    def mp2_gamma(self, e: np.ndarray, C: np.ndarray, nocc: int):
+        """Gamma-supercell MP2 from the resident full ERI (eri_build(sao=False)).
+        C: (nBas, nBas) MO coefficients (ordinary numpy, columns = orbitals,
+        ascending e).  Returns dict(ec, direct, exchange, gpu_ms)."""
+        nb = self.s.nbas
+        Cf = np.ascontiguousarray(np.asarray(C, dtype=np.float64).T)   # column-major C(mu,p)
+        out = np.zeros(4)
+        self._check(lib().tcg_mp2_gamma(self.h, nb, int(nocc), np.ascontiguousarray(e, dtype=np.float64),
+                                         Cf.ravel(), out), "tcg_mp2_gamma")
+        return {"ec": out[0], "direct": out[1], "exchange": out[2], "gpu_ms": out[3]}
 
     # -- direct_init / direct_JK -------------------------------------------------
     def direct_init(self):

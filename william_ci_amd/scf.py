@@ -1,9 +1,11 @@
-"""Restricted Hartree-Fock in the Bloch (SAO) basis on top of the engine.
+"""Restricted Hartree-Fock on top of the engine: in the Bloch (SAO) basis
+(rhf_sao) and in the supercell AO basis (rhf, the Gamma-point form).
 
 Host-side mirror of the reference's SCF driver, which stays in Fortran in the
 drop-in; the engine supplies the integrals and the Coulomb/exchange build:
 
   RHF_SAO                src/Energies/SAO/RHF_SAO.f90:1-553
+  RHF                    src/Energies/RHF.f90:1-369 (+ guess_AO_RHF, src/Energies/guess_RHF.f90:1)
   guess_AO_RHF_SAO       src/Energies/SAO/guess_RHF_SAO.f90:1 (+ guess_occupy_cut,
                          src/Energies/SAO/guess_Huckel_SAO.f90:141-192)
   transform_AO_to_SAO /  src/Tools/SAO_Transfer.f90:1,108
@@ -225,3 +227,65 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
     return RHFResult(energy=E + enuc, e_nuc=enuc, converged=converged, iterations=it, orbital_error=conv,
                      mo_energies=ek.copy(), mo_coef=Ck.copy(), P=P, occupied=occ.copy(),
                      times={"scf_s": time.perf_counter() - t0, "jk_s": t_jk})
+
+
+@dataclasses.dataclass
+class RHFGammaResult:
+    energy: float                   # E_HF including nuclear repulsion
+    e_nuc: float
+    converged: bool
+    iterations: int
+    orbital_error: float
+    mo_energies: np.ndarray         # (nBas,), ascending
+    mo_coef: np.ndarray             # (nBas, nBas), columns = orbitals
+    P: np.ndarray
+    times: dict
+
+
+def rhf(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.ndarray], tuple],
+        diis: bool = True, max_scf: int = MAX_SCF, thresh: float = THRESH,
+        log: Optional[Callable[[str], None]] = None) -> RHFGammaResult:
+    """RHF in the supercell AO basis (src/Energies/RHF.f90:1-369), J and K of
+    every row from ``jk(P)`` (the stored full ERI, hartree_exchange_potential
+    with n1 = nBas).  Core guess (guess_AO_RHF, src/Energies/guess_RHF.f90),
+    energy before the convergence test, orbital error X^T (FPS - SPF) X,
+    DIIS on that error, removed directions shifted to the top."""
+    t0 = time.perf_counter()
+    nO = sysm.n_occ
+    Hc = T + V
+    enuc = nuclear_repulsion(sysm)
+    X = canonical_X(S)
+    cp = X.T @ Hc @ X
+    _shift_dropped(X, cp)
+    e, U = np.linalg.eigh(cp)
+    C = X @ U
+    P = 2.0 * C[:, :nO] @ C[:, :nO].T
+    dii = _DIIS(MAX_DIIS)
+    conv, converged, E = 1.0, False, 0.0
+    t_jk = 0.0
+    it = 0
+    while it < max_scf:
+        it += 1
+        tj = time.perf_counter()
+        J, K = jk(P)
+        t_jk += time.perf_counter() - tj
+        F = Hc + J + K
+        E = float(np.sum(P * T.T) + np.sum(P * V.T) + 0.5 * np.sum(P * J.T) + 0.5 * np.sum(P * K.T))
+        err = X.T @ (F @ P @ S - S @ P @ F) @ X
+        if it > 1:
+            conv = float(np.abs(err).max())
+        if log:
+            log(f"{it:3d} E = {E + enuc:.12f} conv = {conv:.3e}")
+        if conv < thresh:
+            converged = True
+            break
+        if diis:
+            F = dii.extrapolate(err, F)
+        Fp = X.T @ F @ X
+        _shift_dropped(X, Fp)
+        e, U = np.linalg.eigh(Fp)
+        C = X @ U
+        P = 2.0 * C[:, :nO] @ C[:, :nO].T
+    return RHFGammaResult(energy=E + enuc, e_nuc=enuc, converged=converged, iterations=it, orbital_error=conv,
+                          mo_energies=e, mo_coef=C, P=P,
+                          times={"total_s": time.perf_counter() - t0, "jk_s": t_jk})
