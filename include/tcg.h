@@ -100,6 +100,20 @@ int tcg_direct_jk(tcg_ctx *ctx, int nbas, const double *P, double *J, double *K)
  * wall time in ms. */
 int tcg_mp2_gamma(tcg_ctx *ctx, int nbas, int nocc, const double *e, const double *C, double *out);
 
+/* k-point MP2 from the resident SAO ERI (tcg_eri_build with sao = 1,
+ * chemist).  Replaces
+ *   transform_ERI_AO_to_kMO(nBas,ERI,coeff_SAO,p_lo,p_hi,q_lo,q_hi,ERI_kMO)
+ *                                src/Energies/SAO/MP2/AO_to_MO_SAO.f90:34-283
+ *   kMP2_calc(N_cell,blk,p_lo,p_hi,q_lo,q_hi,ERI_kMO,ENuc,EHF,EcMP2)
+ *                                src/Energies/SAO/MP2/MP2_SAO.f90:6-151
+ * coeff_sao: coeff_SAO(nBas,nBas) complex, column-major, as (re, im) pairs,
+ * k-block diagonal (blocks of nfuc in the cell order, x fastest); e(nBas) and
+ * occ(nBas) = 2 or 0 per band, block by block (the K_block fields).  The
+ * (ov|ov) ranges follow set_cc_dims (src/module/cc_dims.f90:68-88).
+ * out[0] = E_c, out[1] = 2 E2d, out[2] = -E2x, out[3] = number of terms left
+ * out for |D| < 1e-8, out[4] = GPU wall time in ms. */
+int tcg_kmp2(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out);
+
 /* One-electron matrices S, T, V (nBas,nBas): rows of the unit cell computed,
  * values below 1e-14 zeroed, the other rows filled by translation.  Replaces
  *   overlap_matrix_toroidal_nD            e.g. src/Toroidal/1D/1D_Overlap_toroidal.f90:23

@@ -43,6 +43,8 @@ def lib():
         L.ref_fast_quad_self_check.argtypes = [_ci, _dp, _ip, _ci, ctypes.c_char_p, _dp]
         L.ref_direct_jk.argtypes = [_dp, _dp, _dp]
         L.ref_mp2.argtypes = [_ci, _ci, _dp, _dp, _dp, ctypes.c_char_p]
+        L.ref_kmp2.argtypes = [_ci, np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS"), _dp, _ip,
+                               _dp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -132,6 +134,27 @@ class RefSystem:
             text = open(path).read()
         m = re.search(r"MP2 correlation energy =\s*([-0-9.E+]+)", text)
         return float(m.group(1))
+
+    def kmp2(self, E_sao, Ck, ek, occ):
+        """The reference's transform_ERI_AO_to_kMO + kMP2_calc on the SAO ERI
+        (C-ordered E[l,k,j,i] = ERI(i,j,k,l), i < nfuc), per-k coefficients
+        Ck[k] (nfuc x nfuc, columns = bands), energies ek[k, band] and
+        occupations occ[k, band] (2 or 0).  Returns E_c(MP2)."""
+        import tempfile
+        nb, nf = self.s.nbas, self.s.nfuc
+        N = nb // nf
+        csao = np.zeros((nb, nb), dtype=complex)
+        for k in range(N):
+            csao[k * nf:(k + 1) * nf, k * nf:(k + 1) * nf] = Ck[k]
+        cf = np.ascontiguousarray(csao.T)              # Fortran column-major
+        eri = np.ascontiguousarray(E_sao, dtype=np.float64).ravel()
+        ec = ctypes.c_double(0.0)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "kmp2.out")
+            lib().ref_kmp2(nb, cf.ravel(), np.ascontiguousarray(np.asarray(ek, dtype=np.float64).ravel()),
+                           np.ascontiguousarray(np.asarray(occ, dtype=np.int32).ravel()), eri, path.encode(),
+                           ctypes.byref(ec))
+        return ec.value
 
     def fast_quad_self_check(self, nq128: bool = False):
         """The reference's fast_quad_self_check on this system (the fast

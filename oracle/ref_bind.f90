@@ -19,6 +19,8 @@
 !   direct_init / direct_JK    src/module/eri_direct.f90:76,262        !
 !   AO_to_MO_ERI_inplace, MP2  src/Tools/AO_to_MO/AO_to_MO.f90:20,     !
 !                              src/Energies/MP2/MP2.f90:1              !
+!   transform_ERI_AO_to_kMO,   src/Energies/SAO/MP2/AO_to_MO_SAO.f90:34 !
+!   kMP2_calc                  src/Energies/SAO/MP2/MP2_SAO.f90:6      !
 !---------------------------------------------------------------------!
 module ref_bind
 
@@ -233,6 +235,65 @@ contains
         call MP2(nbas, nocc, e, eri, 0.d0, 0.d0, .false.)
         close(outfile)
       end subroutine ref_mp2
+
+      ! k-MP2 from the SAO ERI(nfuc,nBas,nBas,nBas): k blocks built from the
+      ! block-diagonal coefficients, energies and occupations (2 or 0) given per
+      ! k (ordered as the cells), the (ov|ov) ranges as set_cc_dims sets them
+      ! (src/module/cc_dims.f90:68-88), transform_ERI_AO_to_kMO, kMP2_calc.
+      ! Its report goes to the file 'path' (unit outfile).
+      subroutine ref_kmp2(nbas, csao, ek, occ, eri, path, ec) bind(C, name="ref_kmp2")
+        use files, only : outfile
+        use k_blocks, only : K_block
+        integer(c_int), value          :: nbas
+        complex(c_double_complex), intent(in) :: csao(nbas,nbas)
+        real(c_double), intent(in)     :: ek(nbas), eri(nfuc,nbas,nbas,nbas)
+        integer(c_int), intent(in)     :: occ(nbas)
+        character(kind=c_char), intent(in) :: path(*)
+        real(c_double), intent(out)    :: ec
+        type(K_block), allocatable :: blk(:)
+        complex(8), allocatable :: ekmo(:,:,:,:,:,:,:)
+        character(len=512) :: fname
+        integer :: i, k, o_hi, omin, nok
+        interface
+          subroutine transform_ERI_AO_to_kMO(nBas, ERI_AO, coeff_SAO, p_lo, p_hi, q_lo, q_hi, ERI_out)
+            use unitcell_module, only : nfuc, N_cell
+            integer, intent(in) :: nBas, p_lo, p_hi, q_lo, q_hi
+            real(8), intent(in) :: ERI_AO(nfuc,nBas,nBas,nBas)
+            complex(8), intent(in) :: coeff_SAO(nBas,nBas)
+            complex(8), intent(out) :: ERI_out(p_lo:p_hi,q_lo:q_hi,p_lo:p_hi,q_lo:q_hi,N_cell,N_cell,N_cell)
+          end subroutine
+          subroutine kMP2_calc(N_cell, blk, p_lo, p_hi, q_lo, q_hi, ERI_kMO, ENuc, EHF, EcMP2)
+            use k_blocks, only : K_block
+            integer, intent(in) :: N_cell, p_lo, p_hi, q_lo, q_hi
+            type(K_block), intent(in) :: blk(N_cell)
+            complex(8), intent(in) :: ERI_kMO(p_lo:p_hi,q_lo:q_hi,p_lo:p_hi,q_lo:q_hi,N_cell,N_cell,N_cell)
+            real(8), intent(in) :: ENuc, EHF
+            real(8), intent(out) :: EcMP2
+          end subroutine
+        end interface
+        fname = ' '
+        i = 1
+        do while (path(i) /= c_null_char .and. i <= 512)
+          fname(i:i) = path(i)
+          i = i + 1
+        end do
+        allocate(blk(N_cell))
+        o_hi = 0 ; omin = nfuc
+        do k = 1, N_cell
+          blk(k)%dim = nfuc
+          allocate(blk(k)%occupation(nfuc), blk(k)%energy(nfuc), blk(k)%Coeff(nfuc,nfuc))
+          blk(k)%occupation = occ((k-1)*nfuc+1:k*nfuc)
+          blk(k)%energy = ek((k-1)*nfuc+1:k*nfuc)
+          blk(k)%Coeff = csao((k-1)*nfuc+1:k*nfuc, (k-1)*nfuc+1:k*nfuc)
+          nok = count(blk(k)%occupation == 2)
+          o_hi = max(o_hi, nok) ; omin = min(omin, nok)
+        end do
+        allocate(ekmo(1:o_hi, omin+1:nfuc, 1:o_hi, omin+1:nfuc, N_cell, N_cell, N_cell))
+        open(unit=outfile, file=trim(fname), status='replace', action='write')
+        call transform_ERI_AO_to_kMO(nbas, eri, csao, 1, o_hi, omin+1, nfuc, ekmo)
+        call kMP2_calc(N_cell, blk, 1, o_hi, omin+1, nfuc, ekmo, 0.d0, 0.d0, ec)
+        close(outfile)
+      end subroutine ref_kmp2
 
       ! direct J/K on the density P(nBas,nBas): direct_init then direct_JK
       subroutine ref_direct_jk(P, J, K) bind(C, name="ref_direct_jk")

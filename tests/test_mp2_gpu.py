@@ -75,3 +75,47 @@ def test_mp2_matches_reference(physicist):
           f"gpu {got['gpu_ms']:.2f} ms")
     assert abs(got["ec"] - want) < 1e-10
     assert abs(got["direct"] + got["exchange"] - got["ec"]) < 1e-15
+
+
+@needs_ref
+def test_kmp2_matches_reference():
+    from oracle import ref
+    from william_ci_amd import scf
+    from william_ci_amd.engine import Engine
+    s = tsys.workload("h10_ring")
+    print("case h10_ring k-MP2", flush=True)
+    r = ref.RefSystem(s)
+    eng = Engine(s, r.coef)
+    S, T, V = eng.one_electron()
+    eng.eri_build(sao=True, to_host=False)
+    res = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
+    got = eng.kmp2(res.mo_coef, res.mo_energies, res.occupied)
+    eng.close()
+    occ = np.where(res.occupied, 2, 0)
+    want = r.kmp2(r.eri_full(), res.mo_coef, res.mo_energies, occ)
+    print(f"  E_kMP2 engine {got['ec']:.14f}  reference {want:.14f}  diff {got['ec'] - want:.2e}  "
+          f"gpu {got['gpu_ms']:.2f} ms  zero-D {got['zero_denominators']}")
+    assert abs(got["ec"] - want) < 1e-10
+    assert got["zero_denominators"] == 0
+
+
+def test_kmp2_equals_gamma_mp2_2d():
+    """2D lattice: k-point MP2 on the Bloch RHF equals the Gamma-supercell MP2
+    on the supercell RHF (closed shell, same state)."""
+    from william_ci_amd import scf
+    from william_ci_amd.engine import Engine
+    s = tsys.lattice_2d_h2(3, 3.6, 10.8)
+    coef = normalised_coef(s)
+    print("case h2_3x3_2d k-MP2 vs Gamma MP2", flush=True)
+    eng = Engine(s, coef)
+    S, T, V = eng.one_electron()
+    eng.eri_build(sao=True, to_host=False)
+    rk = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
+    ek = eng.kmp2(rk.mo_coef, rk.mo_energies, rk.occupied)
+    eng.eri_build(sao=False, to_host=False)
+    rg = scf.rhf(s, S, T, V, lambda P: eng.jk_stored(P, n1=s.nbas))
+    eg = eng.mp2_gamma(rg.mo_energies, rg.mo_coef, s.n_occ)
+    eng.close()
+    print(f"  E_HF k {rk.energy:.12f} gamma {rg.energy:.12f}  E_MP2 k {ek['ec']:.12f} gamma {eg['ec']:.12f}")
+    assert abs(rk.energy - rg.energy) < 1e-8
+    assert abs(ek["ec"] - eg["ec"]) < 1e-8

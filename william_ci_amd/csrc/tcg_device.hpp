@@ -68,8 +68,14 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
 hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, int nchunk, int *bounds, hipStream_t st);
 hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_t st);
 hipError_t launch_translate_rows(double *eri, int nb, int nf, int ncell, const int *relc, hipStream_t st);
-hipError_t launch_dgemm_nn(int M, int N, int K, const double *A, long lda, long long sA, const double *B, long ldb,
-                           long long sB, double *C, long ldc, long long sC, long long batch, hipStream_t st);
+hipError_t launch_dgemm_nn(int M, int N, int K, double alpha, const double *A, long lda, long long sA, const double *B,
+                           long ldb, long long sB, double beta, double *C, long ldc, long long sC, long long batch,
+                           hipStream_t st);
+hipError_t launch_kmo(const double *Sre, const double *Sim, const double2 *Ck, int nf, int N, int nx, int ny, int nz,
+                      int o_hi, int v_lo, double2 *out, hipStream_t st);
+hipError_t launch_kmp2_sums(const double2 *ekmo, const double *ek, const int *occ, int nf, int N, int nx, int ny,
+                            int nz, int o_hi, int v_lo, double *part, int nblocks, unsigned long long *nzero,
+                            hipStream_t st);
 hipError_t launch_mp2_sums(const double *X, const double *e, int o, int v, long long si, long long sa, long long sj,
                            long long sb, double *part, int nblocks, hipStream_t st);
 size_t scan_temp_bytes(int n);

@@ -1127,15 +1127,15 @@ int tcg_mp2_gamma(tcg_ctx *c, int nbas, int nocc, const double *e, const double 
     hipStream_t st = c->st;
     const double t0 = now_ms();
     // Y1(mu,nu,la,q4) = sum_si ERI(mu,nu,la,si) C4(si,q4)
-    HIPCHK(c, launch_dgemm_nn((int)N3, d4, N, c->d_eri.p, (long)N3, 0, C4, N, 0, dY1.p, (long)N3, 0, 1, st));
+    HIPCHK(c, launch_dgemm_nn((int)N3, d4, N, 1.0, c->d_eri.p, (long)N3, 0, C4, N, 0, 0.0, dY1.p, (long)N3, 0, 1, st));
     // Y2(mu,nu,q3,q4) = sum_la Y1(mu,nu,la,q4) C3(la,q3), batched over q4
-    HIPCHK(c, launch_dgemm_nn((int)N2, d3, N, dY1.p, (long)N2, (long long)N3, C3, N, 0, dY2.p, (long)N2,
+    HIPCHK(c, launch_dgemm_nn((int)N2, d3, N, 1.0, dY1.p, (long)N2, (long long)N3, C3, N, 0, 0.0, dY2.p, (long)N2,
                               (long long)(N2 * d3), d4, st));
     // Y3(mu,q2,q3,q4) = sum_nu Y2(mu,nu,q3,q4) C2(nu,q2), batched over (q3,q4)
-    HIPCHK(c, launch_dgemm_nn(N, d2, N, dY2.p, N, (long long)N2, C2, N, 0, dY3.p, N, (long long)N * d2,
+    HIPCHK(c, launch_dgemm_nn(N, d2, N, 1.0, dY2.p, N, (long long)N2, C2, N, 0, 0.0, dY3.p, N, (long long)N * d2,
                               (long long)d3 * d4, st));
     // X(q1,q2,q3,q4) = sum_mu Cv(mu,q1) Y3(mu,q2,q3,q4)
-    HIPCHK(c, launch_dgemm_nn(d1, d2 * d3 * d4, N, dCvT.p, d1, 0, dY3.p, N, 0, dX.p, d1, 0, 1, st));
+    HIPCHK(c, launch_dgemm_nn(d1, d2 * d3 * d4, N, 1.0, dCvT.p, d1, 0, dY3.p, N, 0, 0.0, dX.p, d1, 0, 1, st));
     std::vector<double> ev(e, e + nbas);
     HIPCHK(c, de.upload(ev));
     const int nblk = 1024;
@@ -1157,6 +1157,132 @@ int tcg_mp2_gamma(tcg_ctx *c, int nbas, int nocc, const double *e, const double 
     out[1] = 2.0 * E2d;
     out[2] = -E2x;
     out[3] = now_ms() - t0;
+    return TCG_OK;
+}
+
+int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out) {
+    if (!c) return TCG_E_ARG;
+    if (nbas != c->nbas || !coeff_sao || !e || !occ || !out) return c->fail(TCG_E_ARG, "tcg_kmp2: bad arguments");
+    if (c->eri_n1 != c->nfuc || !c->d_eri.p || c->eri_physicist)
+        return c->fail(TCG_E_STATE, "tcg_kmp2 needs the chemist SAO ERI resident (tcg_eri_build with sao = 1)");
+    const int nf = c->nfuc, N = c->ncell, nb = c->nbas;
+    // k blocks, occupation ranges as set_cc_dims sets them (src/module/cc_dims.f90:68-88)
+    std::vector<double2> Ck((size_t)N * nf * nf);
+    std::vector<double> ek((size_t)N * nf);
+    std::vector<int> oc((size_t)N * nf);
+    int o_hi = 0, omin = nf;
+    double offblk = 0.0, cmax = 0.0;
+    for (int col = 0; col < nb; ++col)
+        for (int row = 0; row < nb; ++row) {
+            const double re = coeff_sao[2 * ((size_t)row + (size_t)nb * col)];
+            const double im = coeff_sao[2 * ((size_t)row + (size_t)nb * col) + 1];
+            const double m = std::hypot(re, im);
+            cmax = std::max(cmax, m);
+            if (row / nf != col / nf) offblk = std::max(offblk, m);
+            else Ck[(size_t)(row / nf) * nf * nf + (row % nf) + (size_t)nf * (col % nf)] = double2{re, im};
+        }
+    if (offblk > 1e-10 * std::max(1.0, cmax))
+        return c->fail(TCG_E_ARG, "tcg_kmp2: coefficients not k-block ordered (off-block %.3e)", offblk);
+    for (int k = 0; k < N; ++k) {
+        int nok = 0;
+        for (int p = 0; p < nf; ++p) {
+            ek[(size_t)k * nf + p] = e[(size_t)k * nf + p];
+            oc[(size_t)k * nf + p] = occ[(size_t)k * nf + p];
+            const int o = occ[(size_t)k * nf + p];
+            if (o != 0 && o != 2) return c->fail(TCG_E_ARG, "tcg_kmp2: occupation must be 0 or 2");
+            if (o == 2) {
+                if (p != nok) return c->fail(TCG_E_ARG, "tcg_kmp2: occupied bands are not the lowest ones in a k block");
+                ++nok;
+            }
+        }
+        o_hi = std::max(o_hi, nok);
+        omin = std::min(omin, nok);
+    }
+    const int v_lo = omin;                     // first virtual band (0-based)
+    if (o_hi == 0 || v_lo >= nf) return c->fail(TCG_E_ARG, "tcg_kmp2: no occupied or no virtual bands");
+    const size_t nbl = (size_t)nb, M1 = (size_t)nf * nbl * nbl * nf;     // rows (a, j2, j3, d)
+    // phase matrix e^{+i k.R}, cells and k points x fastest
+    std::vector<double> cosm((size_t)N * N), sinm((size_t)N * N), cosn((size_t)N * N), sinn((size_t)N * N);
+    const int nx = c->ncl[0], ny = c->ncl[1], nz = c->ncl[2];
+    for (int k = 0; k < N; ++k)
+        for (int R = 0; R < N; ++R) {
+            const double th = 2.0 * kPi * ((double)(k % nx) * (R % nx) / nx + (double)((k / nx) % ny) * ((R / nx) % ny) / ny +
+                                           (double)(k / (nx * ny)) * (R / (nx * ny)) / nz);
+            cosm[R + (size_t)N * k] = std::cos(th);
+            sinm[R + (size_t)N * k] = std::sin(th);
+            cosn[R + (size_t)N * k] = std::cos(th) / N;
+            sinn[R + (size_t)N * k] = std::sin(th) / N;
+        }
+    DevBuf<double> dcos, dsin, dcosn, dsinn, X1r, X1i, X2r, X2i, Sr, Si, de, dpart;
+    DevBuf<double2> dCk, dekmo;
+    DevBuf<int> docc;
+    DevBuf<unsigned long long> dnz;
+    HIPCHK(c, dcos.upload(cosm));
+    HIPCHK(c, dsin.upload(sinm));
+    HIPCHK(c, dcosn.upload(cosn));
+    HIPCHK(c, dsinn.upload(sinn));
+    HIPCHK(c, dCk.upload(Ck));
+    HIPCHK(c, de.upload(ek));
+    HIPCHK(c, docc.upload(oc));
+    const size_t nX = M1 * N;                  // = nf^2 nb^2 (all elements, k in place of R)
+    HIPCHK(c, X1r.alloc(nX));
+    HIPCHK(c, X1i.alloc(nX));
+    hipStream_t st = c->st;
+    const double t0 = now_ms();
+    // 1) R4 -> k4: X1(a,j2,j3,d,k4) = sum_R4 ERI(a,j2,j3,d R4) e^{+i k4 R4}
+    HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dcos.p, N, 0, 0.0, X1r.p, (long)M1, 0, 1, st));
+    HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dsin.p, N, 0, 0.0, X1i.p, (long)M1, 0, 1, st));
+    // 2) R3 -> k3 (e^{-i k3 R3}), batched over (d, k4): rows (a, j2, c), X2(a,j2,c,k3,d,k4)
+    HIPCHK(c, X2r.alloc(nX));
+    HIPCHK(c, X2i.alloc(nX));
+    {
+        const long long rows = (long long)nf * nb * nf, bs = (long long)nf * nb * nb, nbatch = (long long)nf * N;
+        // re = X1r cos + X1i sin ; im = X1i cos - X1r sin
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1r.p, (long)rows, bs, dcos.p, N, 0, 0.0, X2r.p, (long)rows, bs, nbatch, st));
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1i.p, (long)rows, bs, dsin.p, N, 0, 1.0, X2r.p, (long)rows, bs, nbatch, st));
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1i.p, (long)rows, bs, dcos.p, N, 0, 0.0, X2i.p, (long)rows, bs, nbatch, st));
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, -1.0, X1r.p, (long)rows, bs, dsin.p, N, 0, 1.0, X2i.p, (long)rows, bs, nbatch, st));
+    }
+    X1r.release();
+    X1i.release();
+    // 3) R2 -> k2 (e^{+i k2 R2} / N), batched over (c, k3, d, k4): rows (a, b), S(a,b,k2,c,k3,d,k4)
+    HIPCHK(c, Sr.alloc(nX));
+    HIPCHK(c, Si.alloc(nX));
+    {
+        const long long rows = (long long)nf * nf, bs = (long long)nf * nb, nbatch = (long long)nf * N * nf * N;
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2r.p, (long)rows, bs, dcosn.p, N, 0, 0.0, Sr.p, (long)rows, bs, nbatch, st));
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, -1.0, X2i.p, (long)rows, bs, dsinn.p, N, 0, 1.0, Sr.p, (long)rows, bs, nbatch, st));
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2r.p, (long)rows, bs, dsinn.p, N, 0, 0.0, Si.p, (long)rows, bs, nbatch, st));
+        HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2i.p, (long)rows, bs, dcosn.p, N, 0, 1.0, Si.p, (long)rows, bs, nbatch, st));
+    }
+    X2r.release();
+    X2i.release();
+    // 4) quarter transforms with the k-block coefficients, (ov|ov) only
+    const int nv = nf - v_lo;
+    const size_t nkmo = (size_t)o_hi * nv * o_hi * nv * N * N * N;
+    HIPCHK(c, dekmo.alloc(nkmo));
+    HIPCHK(c, launch_kmo(Sr.p, Si.p, dCk.p, nf, N, nx, ny, nz, o_hi, v_lo, dekmo.p, st));
+    // 5) kMP2_calc
+    const int nblk = 1024;
+    HIPCHK(c, dpart.alloc(2 * nblk));
+    HIPCHK(c, dnz.alloc(1));
+    HIPCHK(c, launch_fill((double *)dnz.p, 1, 0.0, st));
+    HIPCHK(c, launch_kmp2_sums(dekmo.p, de.p, docc.p, nf, N, nx, ny, nz, o_hi, v_lo, dpart.p, nblk, dnz.p, st));
+    std::vector<double> part(2 * nblk);
+    unsigned long long nzero = 0;
+    HIPCHK(c, hipMemcpyAsync(part.data(), dpart.p, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(&nzero, dnz.p, sizeof nzero, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    double E2d = 0.0, E2x = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        E2d += part[2 * b];
+        E2x += part[2 * b + 1];
+    }
+    out[0] = 2.0 * E2d - E2x;
+    out[1] = 2.0 * E2d;
+    out[2] = -E2x;
+    out[3] = (double)nzero;
+    out[4] = now_ms() - t0;
     return TCG_OK;
 }
 

@@ -5,7 +5,7 @@
 //                     the unit cell: (i j|k l) = (i0 j'|k' l'), all indices
 //                     shifted by the lattice vector taking i to the unit cell
 //                     (src/Tools/symmetric_matrix.f90:62-131).
-//   k_dgemm_nn        strided-batched C_b = A_b B_b, column-major, FP64 on
+//   k_dgemm_nn        strided-batched C_b = alpha A_b B_b + beta C_b, column-major, FP64 on
 //                     v_mfma_f64_16x16x4f64: 64x64 tiles per workgroup of four
 //                     wavefronts (32x32 each, 2x2 MFMA tiles), K in steps of 16
 //                     staged through LDS.
@@ -51,10 +51,10 @@ hipError_t launch_translate_rows(double *eri, int nb, int nf, int ncell, const i
 // ---------------------------------------------------------------------------
 constexpr int kTM = 64, kTN = 64, kTK = 16, kPad = 2;
 
-__global__ __launch_bounds__(256) void k_dgemm_nn(int M, int N, int K, const double *__restrict__ A, long lda,
-                                                  long long sA, const double *__restrict__ B, long ldb, long long sB,
-                                                  double *__restrict__ C, long ldc, long long sC, int tilesM,
-                                                  int tilesN) {
+__global__ __launch_bounds__(256) void k_dgemm_nn(int M, int N, int K, double alpha, const double *__restrict__ A,
+                                                  long lda, long long sA, const double *__restrict__ B, long ldb,
+                                                  long long sB, double beta, double *__restrict__ C, long ldc,
+                                                  long long sC, int tilesM, int tilesN) {
     __shared__ double As[kTK][kTM + kPad];
     __shared__ double Bs[kTK][kTN + kPad];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -110,18 +110,22 @@ __global__ __launch_bounds__(256) void k_dgemm_nn(int M, int N, int K, const dou
             for (int v = 0; v < 4; ++v) {
                 const int gm = m0 + wm + 16 * x + 4 * v + (lane >> 4);
                 const int gn = n0 + wn + 16 * y + (lane & 15);
-                if (gm < M && gn < N) C[gm + ldc * gn] = acc[x][y][v];
+                if (gm < M && gn < N) {
+                    double *cp = C + gm + ldc * gn;
+                    *cp = beta == 0.0 ? alpha * acc[x][y][v] : alpha * acc[x][y][v] + beta * *cp;
+                }
             }
 }
 
-hipError_t launch_dgemm_nn(int M, int N, int K, const double *A, long lda, long long sA, const double *B, long ldb,
-                           long long sB, double *C, long ldc, long long sC, long long batch, hipStream_t st) {
+hipError_t launch_dgemm_nn(int M, int N, int K, double alpha, const double *A, long lda, long long sA, const double *B,
+                           long ldb, long long sB, double beta, double *C, long ldc, long long sC, long long batch,
+                           hipStream_t st) {
     if (M <= 0 || N <= 0 || batch <= 0) return hipSuccess;
     const int tilesM = (M + kTM - 1) / kTM, tilesN = (N + kTN - 1) / kTN;
     const long long blocks = (long long)tilesM * tilesN * batch;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_dgemm_nn, dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc,
-                       sC, tilesM, tilesN);
+    hipLaunchKernelGGL(k_dgemm_nn, dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, alpha, A, lda, sA, B, ldb, sB,
+                       beta, C, ldc, sC, tilesM, tilesN);
     return hipGetLastError();
 }
 
@@ -166,6 +170,150 @@ __global__ __launch_bounds__(256) void k_mp2_sums(const double *__restrict__ X, 
 hipError_t launch_mp2_sums(const double *X, const double *e, int o, int v, long long si, long long sa, long long sj,
                            long long sb, double *part, int nblocks, hipStream_t st) {
     hipLaunchKernelGGL(k_mp2_sums, dim3(nblocks), dim3(256), 0, st, X, e, o, v, si, sa, sj, sb, part);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k-point MP2 (src/Energies/SAO/MP2/AO_to_MO_SAO.f90:34-283, MP2_SAO.f90:6-151)
+// after the lattice Fourier transforms (GEMMs on the host side):
+//   S(a,b,k2,c,k3,d,k4) = 1/N sum ERI(a, b R2, c R3, d R4) e^{+i k2 R2 - i k3 R3 + i k4 R4}
+// k_kmo: ERI_kMO(p,q,r,s,k1,k2,k3) = sum conj C_k1(a,p) C_k2(b,q) conj C_k3(c,r)
+//        C_k4(d,s) S, k4 = k1 - k2 + k3; one thread per element of the (ov|ov)
+//        block (p, r < o_hi; q, s >= v_lo).
+// k_kmp2_sums: kb = ki - ka + kj, |D| < 1e-8 left out and counted.
+// Cells and k points are numbered x fastest.
+// ---------------------------------------------------------------------------
+struct KGrid {
+    int nx, ny, nz;
+    __device__ int add(int k1, int k2, int k3) const {    // k1 - k2 + k3
+        const int x = ((k1 % nx) - (k2 % nx) + (k3 % nx) + 2 * nx) % nx;
+        const int y = (((k1 / nx) % ny) - ((k2 / nx) % ny) + ((k3 / nx) % ny) + 2 * ny) % ny;
+        const int z = ((k1 / (nx * ny)) - (k2 / (nx * ny)) + (k3 / (nx * ny)) + 2 * nz) % nz;
+        return x + nx * (y + ny * z);
+    }
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {   // conj(a) b
+    return double2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+
+__global__ __launch_bounds__(256) void k_kmo(const double *__restrict__ Sre, const double *__restrict__ Sim,
+                                             const double2 *__restrict__ Ck, int nf, int N, KGrid g, int o_hi,
+                                             int v_lo, double2 *__restrict__ out) {
+    const int nv = nf - v_lo;
+    const long long per = (long long)o_hi * nv * o_hi * nv;
+    const long long total = per * N * N * N;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r_ = t;
+        const int p = (int)(r_ % o_hi); r_ /= o_hi;
+        const int q = v_lo + (int)(r_ % nv); r_ /= nv;
+        const int r = (int)(r_ % o_hi); r_ /= o_hi;
+        const int s = v_lo + (int)(r_ % nv); r_ /= nv;
+        const int k1 = (int)(r_ % N); r_ /= N;
+        const int k2 = (int)(r_ % N);
+        const int k3 = (int)(r_ / N);
+        const int k4 = g.add(k1, k2, k3);
+        const double2 *C1 = Ck + (size_t)k1 * nf * nf, *C2 = Ck + (size_t)k2 * nf * nf;
+        const double2 *C3 = Ck + (size_t)k3 * nf * nf, *C4 = Ck + (size_t)k4 * nf * nf;
+        // S(a, b, k2, c, k3, d, k4): strides 1, nf, nf^2, nf^2 N, nf^3 N, nf^3 N^2, nf^4 N^2
+        const size_t sb = nf, sc = (size_t)nf * nf * N, sd = (size_t)nf * nf * nf * N * N;
+        const size_t base = (size_t)nf * nf * k2 + (size_t)nf * nf * nf * N * k3 + (size_t)nf * nf * nf * nf * N * N * k4;
+        double2 acc = {0.0, 0.0};
+        for (int d = 0; d < nf; ++d) {
+            double2 accd = {0.0, 0.0};
+            for (int c = 0; c < nf; ++c) {
+                double2 accc = {0.0, 0.0};
+                for (int b = 0; b < nf; ++b) {
+                    double2 accb = {0.0, 0.0};
+                    const size_t o0 = base + b * sb + c * sc + d * sd;
+                    for (int a = 0; a < nf; ++a) {
+                        const double2 sv = {Sre[o0 + a], Sim[o0 + a]};
+                        const double2 m = cmulc(C1[a + (size_t)nf * p], sv);
+                        accb.x += m.x;
+                        accb.y += m.y;
+                    }
+                    const double2 m = cmul(C2[b + (size_t)nf * q], accb);
+                    accc.x += m.x;
+                    accc.y += m.y;
+                }
+                const double2 m = cmulc(C3[c + (size_t)nf * r], accc);
+                accd.x += m.x;
+                accd.y += m.y;
+            }
+            const double2 m = cmul(C4[d + (size_t)nf * s], accd);
+            acc.x += m.x;
+            acc.y += m.y;
+        }
+        out[t] = acc;
+    }
+}
+
+hipError_t launch_kmo(const double *Sre, const double *Sim, const double2 *Ck, int nf, int N, int nx, int ny, int nz,
+                      int o_hi, int v_lo, double2 *out, hipStream_t st) {
+    KGrid g{nx, ny, nz};
+    hipLaunchKernelGGL(k_kmo, dim3(4096), dim3(256), 0, st, Sre, Sim, Ck, nf, N, g, o_hi, v_lo, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_kmp2_sums(const double2 *__restrict__ ekmo, const double *__restrict__ ek,
+                                                   const int *__restrict__ occ, int nf, int N, KGrid g, int o_hi,
+                                                   int v_lo, double *__restrict__ part,
+                                                   unsigned long long *__restrict__ nzero) {
+    __shared__ double sd[256], sx[256];
+    const int nv = nf - v_lo;
+    const long long total = (long long)N * N * N * o_hi * o_hi * nv * nv;
+    double d = 0.0, x = 0.0;
+    unsigned long long nz = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r_ = t;
+        const int i = (int)(r_ % o_hi); r_ /= o_hi;
+        const int a = v_lo + (int)(r_ % nv); r_ /= nv;
+        const int j = (int)(r_ % o_hi); r_ /= o_hi;
+        const int b = v_lo + (int)(r_ % nv); r_ /= nv;
+        const int ki = (int)(r_ % N); r_ /= N;
+        const int ka = (int)(r_ % N);
+        const int kj = (int)(r_ / N);
+        const int kb = g.add(ki, ka, kj);
+        if (occ[ki * nf + i] != 2 || occ[kj * nf + j] != 2 || occ[ka * nf + a] != 0 || occ[kb * nf + b] != 0) continue;
+        const double D = ek[ka * nf + a] + ek[kb * nf + b] - ek[ki * nf + i] - ek[kj * nf + j];
+        // ERI_kMO(p,q,r,s,k1,k2,k3): dims (o_hi, nv, o_hi, nv, N, N, N)
+        const size_t blk = (size_t)o_hi * nv * o_hi * nv;
+        const double2 gd = ekmo[(size_t)i + o_hi * ((size_t)(a - v_lo) + nv * ((size_t)j + (size_t)o_hi * (b - v_lo))) +
+                                blk * ((size_t)ki + (size_t)N * ((size_t)ka + (size_t)N * kj))];
+        if (fabs(D) < 1e-8) {
+            ++nz;
+            continue;
+        }
+        const double2 gx = ekmo[(size_t)i + o_hi * ((size_t)(b - v_lo) + nv * ((size_t)j + (size_t)o_hi * (a - v_lo))) +
+                                blk * ((size_t)ki + (size_t)N * ((size_t)kb + (size_t)N * kj))];
+        d -= (gd.x * gd.x + gd.y * gd.y) / D;
+        x -= (gd.x * gx.x + gd.y * gx.y) / D;
+    }
+    if (nz) atomicAdd(nzero, nz);
+    sd[threadIdx.x] = d;
+    sx[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+            sx[threadIdx.x] += sx[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sd[0];
+        part[2 * blockIdx.x + 1] = sx[0];
+    }
+}
+
+hipError_t launch_kmp2_sums(const double2 *ekmo, const double *ek, const int *occ, int nf, int N, int nx, int ny,
+                            int nz, int o_hi, int v_lo, double *part, int nblocks, unsigned long long *nzero,
+                            hipStream_t st) {
+    KGrid g{nx, ny, nz};
+    hipLaunchKernelGGL(k_kmp2_sums, dim3(nblocks), dim3(256), 0, st, ekmo, ek, occ, nf, N, g, o_hi, v_lo, part, nzero);
     return hipGetLastError();
 }
 

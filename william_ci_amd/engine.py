@@ -40,6 +40,7 @@ def lib():
         L.tcg_jk_stored.argtypes = [_vp, _ci, _ci, _dp, _vp, _dp, _dp]
         L.tcg_jk_stored_order.argtypes = [_vp, _ci, _ci, _dp, _vp, _ci, _dp, _dp]
         L.tcg_mp2_gamma.argtypes = [_vp, _ci, _ci, _dp, _dp, _dp]
+        L.tcg_kmp2.argtypes = [_vp, _ci, _dp, _dp, _ip, _dp]
         L.tcg_one_electron.argtypes = [_vp, _ci, _dp, _ip, _dp, _dp, _dp]
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
@@ -198,6 +199,27 @@ class Engine:
         self._check(lib().tcg_mp2_gamma(self.h, nb, int(nocc), np.ascontiguousarray(e, dtype=np.float64),
                                          Cf.ravel(), out), "tcg_mp2_gamma")
         return {"ec": out[0], "direct": out[1], "exchange": out[2], "gpu_ms": out[3]}
+
+    # -- transform_ERI_AO_to_kMO + kMP2_calc --------------------------------------
+    def kmp2(self, mo_coef: np.ndarray, mo_energies: np.ndarray, occupied: np.ndarray):
+        """k-point MP2 from the resident SAO ERI (eri_build(sao=True)).
+        mo_coef[k] (nfuc x nfuc, columns = bands), mo_energies[k, band],
+        occupied[k, band] (bool), k ordered as the cells (x fastest) -- the
+        fields of scf.rhf_sao's result.  Returns dict(ec, direct, exchange,
+        zero_denominators, gpu_ms)."""
+        s = self.s
+        nb, nf = s.nbas, s.nfuc
+        N = nb // nf
+        C = np.zeros((nb, nb), dtype=complex)
+        for k in range(N):
+            C[k * nf:(k + 1) * nf, k * nf:(k + 1) * nf] = mo_coef[k]
+        cf = np.ascontiguousarray(C.T).view(np.float64).ravel()           # column-major, (re, im) pairs
+        e = np.ascontiguousarray(np.asarray(mo_energies, dtype=np.float64).reshape(N * nf))
+        occ = np.ascontiguousarray(np.where(np.asarray(occupied).reshape(N * nf), 2, 0).astype(np.int32))
+        out = np.zeros(5)
+        self._check(lib().tcg_kmp2(self.h, nb, cf, e, occ, out), "tcg_kmp2")
+        return {"ec": out[0], "direct": out[1], "exchange": out[2], "zero_denominators": int(out[3]),
+                "gpu_ms": out[4]}
 
     # -- direct_init / direct_JK -------------------------------------------------
     def direct_init(self):
