@@ -156,6 +156,11 @@ def main():
     t_one = time.perf_counter() - t_rhf
     res = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
     rhf_wall = time.perf_counter() - t_rhf
+    # MP2 on the RHF orbitals (SAO run: k-point MP2 from the resident ERI,
+    # main.f90:823-847 with transform_ERI_AO_to_kMO + kMP2_calc)
+    t_mp2 = time.perf_counter()
+    mp2 = eng.kmp2(res.mo_coef, res.mo_energies, res.occupied)
+    mp2_wall = time.perf_counter() - t_mp2
     eng.close()
 
     nb = s.nbas
@@ -185,6 +190,10 @@ def main():
         "rhf": {"energy": res.energy, "converged": res.converged, "iterations": res.iterations,
                 "orbital_error": res.orbital_error, "wall_s_after_eri": rhf_wall, "one_electron_s": t_one,
                 "jk_s": res.times["jk_s"], "eri_plus_rhf_wall_s": rhf_wall + ms_per_step / 1e3},
+        "mp2": {"kind": "k-point MP2 (transform_ERI_AO_to_kMO + kMP2_calc)", "ec": mp2["ec"],
+                "e_total": res.energy + mp2["ec"], "zero_denominators": mp2["zero_denominators"],
+                "gpu_ms": mp2["gpu_ms"], "wall_s": mp2_wall,
+                "eri_rhf_mp2_wall_s": ms_per_step / 1e3 + rhf_wall + mp2_wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "k_rows_mfma", "kernel_ms": kernel_ms, "flops_per_launch": st["row_flops"],

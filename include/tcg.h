@@ -146,6 +146,45 @@ int tcg_get_quadrature(tcg_ctx *ctx, int *fast, int *nquad, double *t, double *w
 int tcg_fast_quad_self_check(tcg_ctx *ctx, int natom, const double *xyz, const int *Z, int nq_full,
                              double *deri, double *dv, int *fast_on);
 
+/* Shell blocks of a list of shell quartets, with the current rule.  Replaces
+ *   ERI_shell_block_toroidal(ndp,nA,SA,nB,SB,nC,SC,nD,SD,blk)
+ *                                   src/Toroidal/ERI_shell_block_toroidal.f90:28-390
+ * shells[4*x .. 4*x+3]: the four shells (0-based, in the order of the AO list)
+ * of quartet x; out[offsets[x] ..] = blk(nA,nB,nC,nD) column-major with the
+ * kernel's prefactor screens (no Schwarz screen); offsets has n + 1 entries. */
+int tcg_eri_blocks(tcg_ctx *ctx, int n, const int *shells, double *out, long long *offsets);
+
+/* Learned per-quartet quadrature orders.  No counterpart in the reference
+ * (its rule is one for the whole run).  A network 6 -> 64 -> 64 -> 2 (BF16
+ * matrix-core kernels, FP32 accumulation) predicts for every surviving
+ * primitive quartet log10 of its largest integral and the index of the
+ * mapped Gauss-Legendre order (t = 0.6 u/(1-u)) it needs; a shell quartet
+ * takes the largest index over its primitive quartets plus margin (rounded
+ * up), and the current (plain) rule when any of its features leaves the
+ * training range or the index passes the last order.
+ *   params: tcg_mlp_param_count() floats, W1[6][64] b1 W2[64][64] b2 W3[64][2] b3
+ *   norm:   24 floats, feature mean[6], std[6], lowest[6], highest[6]
+ *   orders: norders (<= 15) ascending node counts (4..128)
+ * Trained on Tori1D systems: tcg_eri_build_learned refuses ndp > 1.
+ * rule_counts (norders + 1 entries, may be NULL): shell quartets per order,
+ * the last entry those on the plain rule. */
+int tcg_set_learned(tcg_ctx *ctx, const float *params, const float *norm, const int *orders, int norders,
+                    float margin);
+int tcg_eri_build_learned(tcg_ctx *ctx, int sao, int physicist, double *eri, long long *rule_counts);
+
+/* The network itself (no context): parameter count, inference, one
+ * forward + backward (loss = mean over samples of w0 (y0 - t0)^2 +
+ * w1 (y1 - t1)^2; grad receives dloss/dparams) and Adam training on the GPU
+ * (epochs over the n samples in shuffled mini-batches of batch; params is
+ * updated in place; loss_hist, may be NULL, gets the mean batch loss of
+ * every epoch).  X: n x 6 features, T and Y: n x 2, all row-major FP32. */
+int tcg_mlp_param_count(void);
+int tcg_mlp_forward(const float *params, const float *X, int n, float *Y);
+int tcg_mlp_grad(const float *params, const float *X, const float *T, int n, const float w[2], float *grad,
+                 float *loss);
+int tcg_mlp_train(float *params, const float *X, const float *T, int n, int epochs, int batch, float lr,
+                  unsigned seed, const float w[2], float *loss_hist);
+
 /* Counters of the last tcg_eri_build:
  *   out[0] shell quartets evaluated   out[1] primitive quartets evaluated
  *   out[2] rows (primitive quartet x direction x power combination)

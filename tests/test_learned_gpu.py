@@ -1,0 +1,117 @@
+"""GPU checks of the learned-order network kernels (tcg_mlp_forward,
+tcg_mlp_grad, tcg_mlp_train) against the numpy network (oracle/mlp.py with
+BF16 rounding at the same places), and of the learned ERI build.
+
+Bars: outputs within 1e-3 of the output scale; gradients within 2e-2 in
+relative norm per parameter block (BF16 operands, FP32 accumulation, a few
+roundings that fall on the other side); training lowers the loss of a
+smooth target at least tenfold.  The learned build with an infinite margin
+is the plain build; with the committed model its ERIs stay within 1e-8 of
+the plain rule and the RHF energy within 1 microhartree.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mlp as omlp
+from tests_support import normalised_coef
+from william_ci_amd import system as tsys
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed=0):
+    from william_ci_amd import learned
+    p = learned.init_params(np.random.default_rng(seed))
+    rng = np.random.default_rng(seed + 1)
+    o = 6 * 64
+    p[o:o + 64] = rng.normal(0, 0.1, 64)               # non-zero biases
+    return p
+
+
+def test_forward_matches_oracle():
+    from william_ci_amd import learned
+    rng = np.random.default_rng(2)
+    p = _params()
+    X = rng.normal(size=(1000, 6)).astype(np.float32)
+    print("case mlp forward n=1000", flush=True)
+    Y = learned.forward(p, X)
+    Yo, _ = omlp.forward(p.astype(np.float64), X.astype(np.float64), emulate=True)
+    scale = np.abs(Yo).max()
+    print(f"  max |dY| {np.abs(Y - Yo).max():.3e} (scale {scale:.3f})")
+    assert np.abs(Y - Yo).max() <= 1e-3 * scale
+
+
+def test_gradient_matches_oracle():
+    from william_ci_amd import learned
+    rng = np.random.default_rng(3)
+    p = _params(4)
+    X = rng.normal(size=(517, 6)).astype(np.float32)        # not a multiple of 64
+    T = rng.normal(size=(517, 2)).astype(np.float32)
+    print("case mlp gradient n=517", flush=True)
+    g, loss = learned.grad(p, X, T)
+    lo, go = omlp.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64), emulate=True)
+    assert abs(loss - lo) <= 1e-4 * lo
+    off = 0
+    for name, shp in omlp.SHAPES:
+        n = int(np.prod(shp))
+        a, b = g[off:off + n], go[off:off + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"  {name}: rel {rel:.2e}")
+        assert rel < 2e-2, name
+        off += n
+
+
+def test_training_lowers_loss():
+    from william_ci_amd import learned
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, size=(20000, 6)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) + X[:, 1] * X[:, 2], np.abs(X[:, 3]) * 3 + X[:, 4]], axis=1).astype(np.float32)
+    print("case mlp training n=20000", flush=True)
+    p, hist = learned.train(learned.init_params(np.random.default_rng(0)), X, T, epochs=30, batch=512, lr=3e-3,
+                            w=(1.0, 1.0))
+    print(f"  loss {hist[0]:.4f} -> {hist[-1]:.5f}")
+    assert hist[-1] < 0.1 * hist[0]
+    # the trained network through the inference kernel agrees with the training loss
+    Y = learned.forward(p, X)
+    assert np.mean(np.sum((Y - T) ** 2, axis=1)) < 0.2 * hist[0]
+
+
+def test_learned_build_with_infinite_margin_is_plain():
+    from william_ci_amd import learned
+    from william_ci_amd.engine import Engine
+    s = tsys.workload("h10_ring")
+    print("case learned build, margin 100 (all plain)", flush=True)
+    eng = Engine(s, normalised_coef(s))
+    eng.set_quadrature(False, 64)
+    E0 = eng.eri_build()
+    m = learned.Model(params=_params(), norm=np.concatenate([np.zeros(6), np.ones(6), -1e30 * np.ones(6),
+                                                             1e30 * np.ones(6)]).astype(np.float32), margin=100.0)
+    m.install(eng)
+    E1, counts = learned.eri_build_learned(eng, m)
+    eng.close()
+    print(f"  counts {counts.tolist()}  max |diff| {np.abs(E1 - E0).max():.2e}")
+    assert counts[-1] == counts.sum() and np.abs(E1 - E0).max() <= 1e-15
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(__file__)), "william_ci_amd", "data",
+                                                    "quartet_orders.npz")), reason="no trained model committed")
+def test_learned_build_energy_within_microhartree():
+    from william_ci_amd import learned, scf
+    from william_ci_amd.engine import Engine
+    s = tsys.workload("h10_ring")
+    print("case learned build h10_ring with the trained model", flush=True)
+    eng = Engine(s, normalised_coef(s))
+    S, T, V = eng.one_electron()
+    eng.set_quadrature(False, 64)
+    E0 = eng.eri_build()
+    e0 = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P)).energy
+    m = learned.Model.load()
+    m.install(eng)
+    E1, counts = learned.eri_build_learned(eng, m)
+    e1 = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P)).energy
+    eng.close()
+    print(f"  counts {counts.tolist()}  max |dERI| {np.abs(E1 - E0).max():.2e}  dE {e1 - e0:.2e}")
+    assert np.abs(E1 - E0).max() <= 1e-8
+    assert abs(e1 - e0) <= 1e-6

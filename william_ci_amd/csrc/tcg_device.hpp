@@ -78,6 +78,19 @@ hipError_t launch_kmp2_sums(const double2 *ekmo, const double *ek, const int *oc
                             hipStream_t st);
 hipError_t launch_mp2_sums(const double *X, const double *e, int o, int v, long long si, long long sa, long long sj,
                            long long sb, double *part, int nblocks, hipStream_t st);
+int mlp_param_count();
+hipError_t launch_features(const EriArgs &a, const long long *soff, const Surv *surv, const double *expo,
+                           const float *norm, float *feat, unsigned char *ood, hipStream_t st);
+hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const unsigned char *ood, float margin,
+                        int plain, int *qrule, hipStream_t st);
+hipError_t launch_rule_keys(int nq4, const int *qrule, unsigned *keys, int *idx, unsigned long long *hist,
+                            hipStream_t st);
+hipError_t launch_gather_q4(int n, const int *order, const Quartet *in, Quartet *out, hipStream_t st);
+hipError_t launch_mlp_fwd(const float *P, const float *X, int n, float *Y, hipStream_t st);
+hipError_t launch_mlp_step(const float *P, const float *X, const float *T, int n, float inv_total, float w0, float w1,
+                           float *G, float *loss, hipStream_t st);
+hipError_t launch_mlp_adam(float *P, float *G, float *M, float *V, float lr, float b1, float b2, float eps, float c1,
+                           float c2, hipStream_t st);
 size_t scan_temp_bytes(int n);
 hipError_t scan_exclusive(void *tmp, size_t bytes, const long long *in, long long *out, int n, hipStream_t st);
 size_t sort_temp_bytes(int n, int end_bit);

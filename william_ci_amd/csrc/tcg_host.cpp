@@ -23,6 +23,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <memory>
 #include <vector>
 
 #include "../../include/tcg.h"
@@ -64,6 +65,14 @@ double now_ms() {
 }
 
 }  // namespace
+
+// t-tables of an additional quadrature rule (the learned per-quartet orders)
+struct RuleTables {
+    int fast = 1, nq = 0, nqp = 0;
+    std::vector<double> tnode, tweight;
+    DevBuf<double> d_tab, d_tnode, d_tw;
+    DevBuf<int> d_tablim, d_tnmax;
+};
 
 struct tcg_ctx {
     int ndp = 1;
@@ -125,6 +134,22 @@ struct tcg_ctx {
     int max_ncomb = 1, max_blk = 1;
     bool prepared = false;
     int nq4 = 0;                 // unique shell quartets in d_q4 (after setup_quartets)
+    std::vector<std::unique_ptr<RuleTables>> rules;   // cached per (fast, nq); cleared with the tables
+    // learned per-quartet orders
+    std::vector<float> mlp_params, mlp_norm;           // network, feature mean/std and limits
+    std::vector<int> learned_orders;                   // index -> mapped Gauss-Legendre order
+    float learned_margin = 1.0f;
+    DevBuf<float> d_mlp, d_feat, d_pred;
+    DevBuf<int> d_qrule;
+    DevBuf<unsigned> d_rkey, d_rkey2;
+    DevBuf<int> d_ridx, d_ridx2;
+    DevBuf<Quartet> d_q4b;
+    DevBuf<double> d_expo;
+    long long learned_counts[16] = {0};
+    bool learned_on = false;
+    DevBuf<unsigned char> d_ood;
+    DevBuf<float> d_mlpnorm;
+    DevBuf<unsigned long long> d_hist;
     bool direct_ready = false;   // d_q4 / d_Q / primitive Schwarz valid for the direct J/K
 
     int fail(int code, const char *fmt, ...) {
@@ -149,26 +174,22 @@ static int lsum3(const int *l) { return l[0] + l[1] + l[2]; }
 // ---------------------------------------------------------------------------
 // 1. t-tables (src/module/clifford_tools.f90:620-709 with ncap from ERI_toroidal.f90:94-104)
 // ---------------------------------------------------------------------------
-static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim, std::vector<int> &tnmax) {
-    int lmax = 0;
-    double emax = 0.0, amin = 1e300;
-    for (int i = 0; i < c->nbas; ++i) {
-        for (int d = 0; d < c->ndp; ++d) lmax = std::max(lmax, c->lxyz[3 * i + d]);
-        for (int p = c->poff[i]; p < c->poff[i + 1]; ++p) emax = std::max(emax, c->expo[p]);
-    }
-    for (int d = 0; d < c->ndp; ++d) amin = std::min(amin, c->a[d]);
-    c->ncap = nmax_est(4.0 * emax / (amin * amin)) + 2 * lmax + 1;
-    const int ncap = c->ncap, nqp = c->nqp, rows = ncap + 5;
-    c->tnode.assign(nqp, 0.0);
-    c->tweight.assign(nqp, 0.0);
-    if (c->fast) {
-        mapped_gauss_legendre(c->nq, c->tnode.data(), c->tweight.data());
+// Nodes, weights and t-table of one rule (fast: nq mapped Gauss-Legendre
+// points, else the literal 64/128 table) for the context's torus and ncap.
+static void fill_rule(const tcg_ctx *c, int fast, int nq, int nqp, std::vector<double> &tnode,
+                      std::vector<double> &tweight, std::vector<double> &tab, std::vector<int> &tablim,
+                      std::vector<int> &tnmax) {
+    const int ncap = c->ncap, rows = ncap + 5;
+    tnode.assign(nqp, 0.0);
+    tweight.assign(nqp, 0.0);
+    if (fast) {
+        mapped_gauss_legendre(nq, tnode.data(), tweight.data());
     } else {
         // the literal tables of gauss_legendre_points (src/module/precompute_t.f90:495-891)
-        const double *t = c->nq == 64 ? kQuad64_t : kQuad128_t;
-        const double *w = c->nq == 64 ? kQuad64_w : kQuad128_w;
-        std::copy(t, t + c->nq, c->tnode.begin());
-        std::copy(w, w + c->nq, c->tweight.begin());
+        const double *t = nq == 64 ? kQuad64_t : kQuad128_t;
+        const double *w = nq == 64 ? kQuad64_w : kQuad128_w;
+        std::copy(t, t + nq, tnode.begin());
+        std::copy(w, w + nq, tweight.begin());
     }
     tab.assign((size_t)c->ndp * rows * nqp, 0.0);
     const int nh = nqp / kNodeTile;
@@ -176,8 +197,8 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
     tnmax.assign((size_t)c->ndp * nqp, -1);
     std::vector<double> col(ncap + 1);
     for (int d = 0; d < c->ndp; ++d) {
-        for (int k = 0; k < c->nq; ++k) {
-            const double C = 2.0 * c->tnode[k] * c->tnode[k] / (c->a[d] * c->a[d]);
+        for (int k = 0; k < nq; ++k) {
+            const double C = 2.0 * tnode[k] * tnode[k] / (c->a[d] * c->a[d]);
             bessel_scaled_vector(ncap, C, col.data());
             int nmax = ncap;   // last order kept: first below 1e-16 relative, + 5
             for (int n = 0; n <= ncap; ++n)
@@ -188,6 +209,18 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
             lim = std::max(lim, nmax);
         }
     }
+}
+
+static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim, std::vector<int> &tnmax) {
+    int lmax = 0;
+    double emax = 0.0, amin = 1e300;
+    for (int i = 0; i < c->nbas; ++i) {
+        for (int d = 0; d < c->ndp; ++d) lmax = std::max(lmax, c->lxyz[3 * i + d]);
+        for (int p = c->poff[i]; p < c->poff[i + 1]; ++p) emax = std::max(emax, c->expo[p]);
+    }
+    for (int d = 0; d < c->ndp; ++d) amin = std::min(amin, c->a[d]);
+    c->ncap = nmax_est(4.0 * emax / (amin * amin)) + 2 * lmax + 1;
+    fill_rule(c, c->fast, c->nq, c->nqp, c->tnode, c->tweight, tab, tablim, tnmax);
     return 0;
 }
 
@@ -389,6 +422,40 @@ static EriArgs base_args(tcg_ctx *c) {
     return A;
 }
 
+// tables of rule (fast, nq), built on first use for the current basis
+static RuleTables *get_rule(tcg_ctx *c, int fast, int nq, int *rc) {
+    for (auto &r : c->rules)
+        if (r->fast == fast && r->nq == nq) return r.get();
+    auto rt = std::make_unique<RuleTables>();
+    rt->fast = fast;
+    rt->nq = nq;
+    rt->nqp = ((nq + kNodeTile - 1) / kNodeTile) * kNodeTile;
+    std::vector<double> tab;
+    std::vector<int> tablim, tnmax;
+    fill_rule(c, fast, nq, rt->nqp, rt->tnode, rt->tweight, tab, tablim, tnmax);
+    hipError_t e = rt->d_tab.upload(tab);
+    if (e == hipSuccess) e = rt->d_tablim.upload(tablim);
+    if (e == hipSuccess) e = rt->d_tnmax.upload(tnmax);
+    if (e == hipSuccess) e = rt->d_tnode.upload(rt->tnode);
+    if (e == hipSuccess) e = rt->d_tw.upload(rt->tweight);
+    if (e != hipSuccess) {
+        *rc = c->fail(TCG_E_DEVICE, "rule tables: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    c->rules.push_back(std::move(rt));
+    return c->rules.back().get();
+}
+
+static void use_rule(EriArgs &A, const RuleTables *rt) {
+    A.nq = rt->nq;
+    A.nqp = rt->nqp;
+    A.tab = rt->d_tab.p;
+    A.tab_lim = rt->d_tablim.p;
+    A.tnmax = rt->d_tnmax.p;
+    A.tnode = rt->d_tnode.p;
+    A.tweight = rt->d_tw.p;
+}
+
 // Quartets of A.q4[0..A.nq4) through the row pipeline (tcg_rows.hip):
 // survivors -> rows -> sort by (direction, length) -> MFMA -> contraction.
 // One chunk of quartets (A.q4[0..A.nq4)) whose survivor/row offsets so/ro
@@ -515,6 +582,7 @@ static int prepare(tcg_ctx *c) {
     std::vector<int> tablim, tnmax;
     const double tt0 = now_ms();
     build_tables(c, tab, tablim, tnmax);
+    c->rules.clear();
     c->times[7] = now_ms() - tt0;
     std::vector<VecJob> jobs;
     size_t nvec = 0, niv = 0;
@@ -535,6 +603,7 @@ static int prepare(tcg_ctx *c) {
     HIPCHK(c, c->d_sh.upload(c->sh));
     HIPCHK(c, c->d_lxyz.upload(c->lxyz));
     HIPCHK(c, c->d_coef.upload(c->coef));
+    HIPCHK(c, c->d_expo.upload(c->expo));
     HIPCHK(c, c->d_poff.upload(c->poff));
     HIPCHK(c, c->d_pp.upload(c->pp));
     HIPCHK(c, c->d_sp.upload(c->sp));
@@ -686,6 +755,85 @@ static void fill_translation(const tcg_ctx *c, double *M) {
             M[(size_t)i + (size_t)nb * j] = M[(size_t)a + (size_t)nb * src];
         }
     }
+}
+
+// The final quartet pass with learned orders: survivors of the whole list,
+// their features and the network's predictions, one rule per shell quartet
+// (the largest asked for, plain rule outside the training range), then the
+// row pipeline once per rule on the quartets of that rule.
+static int learned_pass(tcg_ctx *c, EriArgs A) {
+    const int nq4 = A.nq4;
+    const int nord = (int)c->learned_orders.size();
+    for (int r = 0; r <= nord && r < 16; ++r) c->learned_counts[r] = 0;
+    if (nq4 <= 0) return 0;
+    HIPCHK(c, c->d_scount.alloc(nq4 + 1));
+    HIPCHK(c, c->d_rcount.alloc(nq4 + 1));
+    HIPCHK(c, c->d_soff.alloc(nq4 + 1));
+    HIPCHK(c, c->d_roff.alloc(nq4 + 1));
+    HIPCHK(c, hipMemsetAsync(c->d_scount.p + nq4, 0, sizeof(long long), c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_rcount.p + nq4, 0, sizeof(long long), c->st));
+    HIPCHK(c, launch_surv(A, 0, c->d_scount.p, c->d_rcount.p, nullptr, nullptr, c->st));
+    size_t tb = scan_temp_bytes(nq4 + 1);
+    HIPCHK(c, c->d_tmp.alloc(tb));
+    HIPCHK(c, scan_exclusive(c->d_tmp.p, tb, c->d_scount.p, c->d_soff.p, nq4 + 1, c->st));
+    long long ns = 0;
+    HIPCHK(c, hipMemcpyAsync(&ns, c->d_soff.p + nq4, sizeof ns, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, c->d_qrule.alloc(nq4));
+    if (ns > 0) {
+        if (ns > 0x7fffffffLL) return c->fail(TCG_E_NOMEM, "learned orders: too many primitive quartets (%lld)", ns);
+        HIPCHK(c, c->d_surv.alloc(ns));
+        HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, c->d_soff.p, c->d_surv.p, c->st));
+        HIPCHK(c, c->d_feat.alloc((size_t)ns * 6));
+        HIPCHK(c, c->d_pred.alloc((size_t)ns * 2));
+        HIPCHK(c, c->d_ood.alloc(ns));
+        HIPCHK(c, launch_features(A, c->d_soff.p, c->d_surv.p, c->d_expo.p, c->d_mlpnorm.p, c->d_feat.p, c->d_ood.p,
+                                  c->st));
+        HIPCHK(c, launch_mlp_fwd(c->d_mlp.p, c->d_feat.p, (int)ns, c->d_pred.p, c->st));
+    }
+    HIPCHK(c, launch_rules(nq4, c->d_soff.p, c->d_pred.p, c->d_ood.p, c->learned_margin, nord, c->d_qrule.p, c->st));
+    // order the quartets by rule
+    HIPCHK(c, c->d_rkey.alloc(nq4));
+    HIPCHK(c, c->d_rkey2.alloc(nq4));
+    HIPCHK(c, c->d_ridx.alloc(nq4));
+    HIPCHK(c, c->d_ridx2.alloc(nq4));
+    HIPCHK(c, c->d_hist.alloc(16));
+    HIPCHK(c, launch_fill((double *)c->d_hist.p, 16, 0.0, c->st));
+    HIPCHK(c, launch_rule_keys(nq4, c->d_qrule.p, c->d_rkey.p, c->d_ridx.p, c->d_hist.p, c->st));
+    tb = sort_temp_bytes(nq4, 8);
+    HIPCHK(c, c->d_tmp.alloc(tb));
+    HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_rkey.p, c->d_rkey2.p, c->d_ridx.p, c->d_ridx2.p, nq4, 8, c->st));
+    HIPCHK(c, c->d_q4b.alloc(nq4));
+    HIPCHK(c, launch_gather_q4(nq4, c->d_ridx2.p, A.q4, c->d_q4b.p, c->st));
+    unsigned long long hist[16];
+    HIPCHK(c, hipMemcpyAsync(hist, c->d_hist.p, sizeof hist, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    long long start = 0, surv = 0, rows = 0;
+    double rows_ms = 0.0;
+    for (int r = 0; r <= nord; ++r) {
+        const long long cnt = (long long)hist[r];
+        c->learned_counts[r] = cnt;
+        if (cnt == 0) continue;
+        EriArgs B = A;
+        B.q4 = c->d_q4b.p + start;
+        B.nq4 = (int)cnt;
+        if (r < nord) {
+            int rc = 0;
+            RuleTables *rt = get_rule(c, 1, c->learned_orders[r], &rc);
+            if (!rt) return rc;
+            use_rule(B, rt);
+        }
+        const int rc = run_quartets(c, B);
+        if (rc) return rc;
+        surv += c->last_surv;
+        rows += c->last_rows;
+        rows_ms += c->rows_ms;
+        start += cnt;
+    }
+    c->last_surv = surv;
+    c->last_rows = rows;
+    c->rows_ms = rows_ms;
+    return 0;
 }
 
 // Steps 3-5 of the driver (ERI_toroidal.f90:200-401): Schwarz values from
@@ -927,7 +1075,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
     c->time_rows = true;
-    rc = run_quartets(c, A);
+    rc = c->learned_on ? learned_pass(c, A) : run_quartets(c, A);
     c->time_rows = false;
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
@@ -1283,6 +1431,72 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
     out[2] = -E2x;
     out[3] = (double)nzero;
     out[4] = now_ms() - t0;
+    return TCG_OK;
+}
+
+int tcg_set_learned(tcg_ctx *c, const float *params, const float *norm, const int *orders, int norders, float margin) {
+    if (!c) return TCG_E_ARG;
+    if (!params || !norm || !orders || norders <= 0 || norders > 15)
+        return c->fail(TCG_E_ARG, "tcg_set_learned: bad arguments");
+    for (int r = 0; r < norders; ++r)
+        if (orders[r] < 4 || orders[r] > kMaxNQ) return c->fail(TCG_E_ARG, "tcg_set_learned: order %d", orders[r]);
+    const int np = tcg_mlp_param_count();
+    c->mlp_params.assign(params, params + np);
+    c->mlp_norm.assign(norm, norm + 24);
+    c->learned_orders.assign(orders, orders + norders);
+    c->learned_margin = margin;
+    HIPCHK(c, c->d_mlp.upload(c->mlp_params));
+    HIPCHK(c, c->d_mlpnorm.upload(c->mlp_norm));
+    return TCG_OK;
+}
+
+int tcg_eri_build_learned(tcg_ctx *c, int sao, int physicist, double *eri, long long *rule_counts) {
+    if (!c) return TCG_E_ARG;
+    if (c->learned_orders.empty()) return c->fail(TCG_E_STATE, "tcg_eri_build_learned before tcg_set_learned");
+    if (c->ndp != 1) return c->fail(TCG_E_ARG, "tcg_eri_build_learned: the network is trained on Tori1D systems");
+    c->learned_on = true;
+    const int rc = tcg_eri_build(c, sao, physicist, eri);
+    c->learned_on = false;
+    if (rc) return rc;
+    if (rule_counts)
+        for (size_t r = 0; r <= c->learned_orders.size(); ++r) rule_counts[r] = c->learned_counts[r];
+    return TCG_OK;
+}
+
+int tcg_eri_blocks(tcg_ctx *c, int n, const int *shells, double *out, long long *offsets) {
+    if (!c) return TCG_E_ARG;
+    if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_blocks before tcg_set_basis");
+    if (n <= 0 || !shells || !out || !offsets) return c->fail(TCG_E_ARG, "tcg_eri_blocks: bad arguments");
+    int rc = prepare(c);
+    if (rc) return rc;
+    std::vector<Quartet> q4(n);
+    size_t nout = 0;
+    for (int x = 0; x < n; ++x) {
+        int s4[4];
+        for (int y = 0; y < 4; ++y) {
+            s4[y] = shells[4 * x + y];
+            if (s4[y] < 0 || s4[y] >= c->nsh) return c->fail(TCG_E_ARG, "tcg_eri_blocks: shell %d out of range", s4[y]);
+        }
+        // translate the quartet so that its first shell is in the unit cell
+        const int c0 = c->sh[s4[0]].cell;
+        int t4[4];
+        for (int y = 0; y < 4; ++y) t4[y] = c->relc[(size_t)c0 * c->ncell + c->sh[s4[y]].cell] * c->nshuc + c->sh[s4[y]].loc;
+        Quartet &Q = q4[x];
+        for (int y = 0; y < 4; ++y) Q.s[y] = t4[y];
+        Q.bra = rep_pair(c, t4[0], t4[1]);
+        Q.ket = rep_pair(c, t4[2], t4[3]);
+        const int lK = c->sh[t4[2]].loc;
+        for (int d = 0; d < 3; ++d) Q.shift[d] = c->sh[t4[2]].xyz[d] - c->sh[lK].xyz[d];
+        Q.out = (int)nout;
+        Q.prim = -1;
+        offsets[x] = (long long)nout;
+        nout += (size_t)c->sh[t4[0]].ncomp * c->sh[t4[1]].ncomp * c->sh[t4[2]].ncomp * c->sh[t4[3]].ncomp;
+    }
+    offsets[n] = (long long)nout;
+    std::vector<double> blocks;
+    rc = run_blocks(c, q4, nout, blocks, 1);
+    if (rc) return rc;
+    std::copy(blocks.begin(), blocks.end(), out);
     return TCG_OK;
 }
 

@@ -1,0 +1,148 @@
+// Learned per-quartet quadrature orders (gfx950).
+//
+//   k_features   six features of every surviving primitive quartet (one
+//                wavefront per shell quartet, lanes over its survivors),
+//                normalised; a quartet outside the training range is flagged.
+//   k_rules      the rule of each shell quartet: the largest order index the
+//                network asks for over its primitive quartets, one step of
+//                margin added; any flagged primitive quartet -> the plain rule.
+//   k_rule_hist  shell quartets per rule.
+//   k_gather_q4  the quartet list reordered by rule.
+//
+// Features of the primitive quartet (e1 A, e2 B | e3 C, e4 D) on a torus of
+// length L (periodic direction x, chord distances, a = 2 pi / L):
+//   f0 = log10 rho,  rho = P Q / (P + Q),  P = e1 + e2,  Q = e3 + e4
+//   f1 = log10 (rho L^2)
+//   f2 = log10 (1 + rho R^2), R = distance of the pair centres (x: circular
+//        weighted mean, chord; y, z: weighted mean)
+//   f3 = (lA + lB + lC + lD) / 4
+//   f4 = log10 (max(e) L^2)
+//   f5 = max(log10 (K_AB K_CD), -30) / 10,  K_AB = exp(-e1 e2 / P d_AB^2)
+#include <hip/hip_runtime.h>
+
+#include "tcg_device.hpp"
+
+namespace tcg {
+
+__device__ __forceinline__ double chord2(double a, double dx) { return 2.0 * (1.0 - cos(a * dx)) / (a * a); }
+
+__device__ void pair_centre(double a, double e1, const double *A, double e2, const double *B, double *P) {
+    const double s = e1 * sin(a * A[0]) + e2 * sin(a * B[0]);
+    const double cc = e1 * cos(a * A[0]) + e2 * cos(a * B[0]);
+    P[0] = atan2(s, cc) / a;
+    P[1] = (e1 * A[1] + e2 * B[1]) / (e1 + e2);
+    P[2] = (e1 * A[2] + e2 * B[2]) / (e1 + e2);
+}
+
+__device__ double pair_d2(double a, const double *A, const double *B) {
+    const double dy = A[1] - B[1], dz = A[2] - B[2];
+    return chord2(a, A[0] - B[0]) + dy * dy + dz * dz;
+}
+
+__global__ __launch_bounds__(64) void k_features(EriArgs A, const long long *__restrict__ soff,
+                                                 const Surv *__restrict__ surv, const double *__restrict__ expo,
+                                                 const float *__restrict__ norm, float *__restrict__ feat,
+                                                 unsigned char *__restrict__ ood) {
+    const double a = A.a[0], L = A.L[0];
+    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+        const Quartet Q = A.q4[qi];
+        const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
+        const int lt = sA.l + sB.l + sC.l + sD.l;
+        for (long long g = soff[qi] + threadIdx.x; g < soff[qi + 1]; g += 64) {
+            const Surv S = surv[g];
+            const PrimPair &pb = A.pp[S.pb], &pk = A.pp[S.pk];
+            const double e1 = expo[A.ao_poff[sA.first_ao] + pb.pi], e2 = expo[A.ao_poff[sB.first_ao] + pb.pj];
+            const double e3 = expo[A.ao_poff[sC.first_ao] + pk.pi], e4 = expo[A.ao_poff[sD.first_ao] + pk.pj];
+            const double P = e1 + e2, Qe = e3 + e4, rho = P * Qe / (P + Qe);
+            double Pc[3], Qc[3];
+            pair_centre(a, e1, sA.xyz, e2, sB.xyz, Pc);
+            pair_centre(a, e3, sC.xyz, e4, sD.xyz, Qc);
+            const double R2 = pair_d2(a, Pc, Qc);
+            const double emax = fmax(fmax(e1, e2), fmax(e3, e4));
+            const double lk = -(e1 * e2 / P * pair_d2(a, sA.xyz, sB.xyz) + e3 * e4 / Qe * pair_d2(a, sC.xyz, sD.xyz)) /
+                              2.302585092994046;
+            double f[6];
+            f[0] = log10(rho);
+            f[1] = log10(rho * L * L);
+            f[2] = log10(1.0 + rho * R2);
+            f[3] = 0.25 * lt;
+            f[4] = log10(emax * L * L);
+            f[5] = fmax(lk, -30.0) / 10.0;
+            // norm: mean[6], std[6], lo[6], hi[6]
+            bool out = false;
+            for (int k = 0; k < 6; ++k) {
+                out = out || f[k] < norm[12 + k] || f[k] > norm[18 + k];
+                feat[g * 6 + k] = (float)((f[k] - norm[k]) / norm[6 + k]);
+            }
+            ood[g] = out ? 1 : 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_rules(int nq4, const long long *__restrict__ soff, const float *__restrict__ pred,
+                                              const unsigned char *__restrict__ ood, float margin, int plain,
+                                              int *__restrict__ qrule) {
+    for (int qi = blockIdx.x; qi < nq4; qi += gridDim.x) {
+        int r = 0;
+        for (long long g = soff[qi] + threadIdx.x; g < soff[qi + 1]; g += 64) {
+            int rg = plain;
+            if (!ood[g]) {
+                const float y = pred[g * 2 + 1] + margin;
+                rg = y <= 0.f ? 0 : (int)ceilf(y);
+                rg = rg > plain ? plain : rg;
+            }
+            r = r > rg ? r : rg;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(r, off, 64);
+            r = r > o ? r : o;
+        }
+        if (threadIdx.x == 0) qrule[qi] = r;
+    }
+}
+
+__global__ void k_rule_keys(int nq4, const int *__restrict__ qrule, unsigned *__restrict__ keys, int *__restrict__ idx,
+                            unsigned long long *__restrict__ hist) {
+    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq4) return;
+    keys[qi] = (unsigned)qrule[qi];
+    idx[qi] = qi;
+    atomicAdd(&hist[qrule[qi]], 1ull);
+}
+
+__global__ void k_gather_q4(int n, const int *__restrict__ order, const Quartet *__restrict__ in,
+                            Quartet *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[order[i]];
+}
+
+hipError_t launch_features(const EriArgs &a, const long long *soff, const Surv *surv, const double *expo,
+                           const float *norm, float *feat, unsigned char *ood, hipStream_t st) {
+    if (a.nq4 <= 0) return hipSuccess;
+    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
+    hipLaunchKernelGGL(k_features, dim3(grid), dim3(64), 0, st, a, soff, surv, expo, norm, feat, ood);
+    return hipGetLastError();
+}
+
+hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const unsigned char *ood, float margin,
+                        int plain, int *qrule, hipStream_t st) {
+    if (nq4 <= 0) return hipSuccess;
+    const int grid = nq4 < (1 << 20) ? nq4 : (1 << 20);
+    hipLaunchKernelGGL(k_rules, dim3(grid), dim3(64), 0, st, nq4, soff, pred, ood, margin, plain, qrule);
+    return hipGetLastError();
+}
+
+hipError_t launch_rule_keys(int nq4, const int *qrule, unsigned *keys, int *idx, unsigned long long *hist,
+                            hipStream_t st) {
+    if (nq4 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rule_keys, dim3((nq4 + 255) / 256), dim3(256), 0, st, nq4, qrule, keys, idx, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_q4(int n, const int *order, const Quartet *in, Quartet *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_q4, dim3((n + 255) / 256), dim3(256), 0, st, n, order, in, out);
+    return hipGetLastError();
+}
+
+}  // namespace tcg
