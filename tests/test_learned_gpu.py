@@ -115,3 +115,26 @@ def test_learned_build_energy_within_microhartree():
     print(f"  counts {counts.tolist()}  max |dERI| {np.abs(E1 - E0).max():.2e}  dE {e1 - e0:.2e}")
     assert np.abs(E1 - E0).max() <= 1e-8
     assert abs(e1 - e0) <= 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(__file__)), "william_ci_amd", "data",
+                                                    "quartet_orders.npz")), reason="no trained model committed")
+def test_learned_build_chunked_equals_one_pass(monkeypatch):
+    """Rules stay attached to their quartets when the list is split into chunks."""
+    from william_ci_amd import learned
+    from william_ci_amd.engine import Engine
+    s = tsys.workload("h10_ring")
+    coef = normalised_coef(s)
+    print("case learned build chunked", flush=True)
+    m = learned.Model.load()
+    eng = Engine(s, coef)
+    m.install(eng)
+    E1, c1 = learned.eri_build_learned(eng, m)
+    rows = eng.stats()["rows"]
+    eng.close()
+    monkeypatch.setenv("TCG_ROW_BUDGET", str(max(2e5, rows * 1200.0 / 5)))
+    eng = Engine(s, coef)
+    m.install(eng)
+    E2, c2 = learned.eri_build_learned(eng, m)
+    eng.close()
+    assert np.array_equal(c1, c2) and np.abs(E1 - E2).max() <= 1e-14

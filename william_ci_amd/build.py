@@ -14,7 +14,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed FP32 instructions stay off (they are not used by this FP64 code either)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-I", os.path.join(HERE, "..", "include")]
+         "-I", os.path.join(HERE, "..", "include"), "-pthread"]
 
 
 def build(verbose: bool = False) -> str:

@@ -12,10 +12,14 @@ struct EriArgs {
     int ndp, nq, nqp, ncap;           // periodic directions, nodes, nodes padded to 16, Bessel cap
     double a[3], L[3];
     double fac_const;                 // 2/sqrt(pi) prod_d L_d^2
-    const double *tab;                // [d][ncap+1+4][nqp] c_n exp(-C)I_n(C), 0 beyond t_nmax(k)
-    const int *tab_lim;               // [d][nqp/16]: last n with a nonzero entry in that node tile
-    const int *tnmax;                 // [d][nqp]: truncation order of each node (-1 in the padding)
-    const double *tnode, *tweight;    // [nqp] (0 in the padding)
+    // quadrature rules (rule 0 = the run's rule; more with learned per-quartet orders),
+    // every rule padded to nqp nodes:
+    const double *tab;                // [rule][d][ncap+1+4][nqp] c_n exp(-C)I_n(C), 0 beyond t_nmax(k)
+    const int *tab_lim;               // [rule][d][nqp/16]: last n with a nonzero entry in that node tile
+    const int *tnmax;                 // [rule][d][nqp]: truncation order of each node (-1 in the padding)
+    const double *tnode, *tweight;    // [rule][nqp] (0 in the padding)
+    const int *rule_nq;               // [rule] node count (nullptr: one rule of nq nodes)
+    const int *qrule;                 // [nq4] rule of each quartet (nullptr: rule 0)
     const Shell *sh;
     const int *ao_lxyz;               // [nbas][3]
     const double *coef;               // normalised coefficients, per AO
@@ -79,13 +83,12 @@ hipError_t launch_kmp2_sums(const double2 *ekmo, const double *ek, const int *oc
 hipError_t launch_mp2_sums(const double *X, const double *e, int o, int v, long long si, long long sa, long long sj,
                            long long sb, double *part, int nblocks, hipStream_t st);
 int mlp_param_count();
-hipError_t launch_features(const EriArgs &a, const long long *soff, const Surv *surv, const double *expo,
+hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, const Surv *surv, const double *expo,
                            const float *norm, float *feat, unsigned char *ood, hipStream_t st);
+hipError_t launch_surv_owner(const long long *soff, int nq4, int *owner, hipStream_t st);
 hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const unsigned char *ood, float margin,
                         int plain, int *qrule, hipStream_t st);
-hipError_t launch_rule_keys(int nq4, const int *qrule, unsigned *keys, int *idx, unsigned long long *hist,
-                            hipStream_t st);
-hipError_t launch_gather_q4(int n, const int *order, const Quartet *in, Quartet *out, hipStream_t st);
+hipError_t launch_rule_hist(int nq4, const int *qrule, unsigned long long *hist, hipStream_t st);
 hipError_t launch_mlp_fwd(const float *P, const float *X, int n, float *Y, hipStream_t st);
 hipError_t launch_mlp_step(const float *P, const float *X, const float *T, int n, float inv_total, float w0, float w1,
                            float *G, float *loss, hipStream_t st);

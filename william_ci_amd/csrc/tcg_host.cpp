@@ -21,6 +21,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <atomic>
+#include <thread>
 #include <type_traits>
 #include <utility>
 #include <memory>
@@ -65,14 +67,6 @@ double now_ms() {
 }
 
 }  // namespace
-
-// t-tables of an additional quadrature rule (the learned per-quartet orders)
-struct RuleTables {
-    int fast = 1, nq = 0, nqp = 0;
-    std::vector<double> tnode, tweight;
-    DevBuf<double> d_tab, d_tnode, d_tw;
-    DevBuf<int> d_tablim, d_tnmax;
-};
 
 struct tcg_ctx {
     int ndp = 1;
@@ -134,16 +128,15 @@ struct tcg_ctx {
     int max_ncomb = 1, max_blk = 1;
     bool prepared = false;
     int nq4 = 0;                 // unique shell quartets in d_q4 (after setup_quartets)
-    std::vector<std::unique_ptr<RuleTables>> rules;   // cached per (fast, nq); cleared with the tables
+    std::vector<int> rule_nq;    // node count of each rule in the tables (rule 0 = the run's rule)
+    int nqp_all = 32;            // node stride of the tables (largest padded node count)
+    DevBuf<int> d_rule_nq;
     // learned per-quartet orders
     std::vector<float> mlp_params, mlp_norm;           // network, feature mean/std and limits
     std::vector<int> learned_orders;                   // index -> mapped Gauss-Legendre order
     float learned_margin = 1.0f;
     DevBuf<float> d_mlp, d_feat, d_pred;
     DevBuf<int> d_qrule;
-    DevBuf<unsigned> d_rkey, d_rkey2;
-    DevBuf<int> d_ridx, d_ridx2;
-    DevBuf<Quartet> d_q4b;
     DevBuf<double> d_expo;
     long long learned_counts[16] = {0};
     bool learned_on = false;
@@ -174,12 +167,9 @@ static int lsum3(const int *l) { return l[0] + l[1] + l[2]; }
 // ---------------------------------------------------------------------------
 // 1. t-tables (src/module/clifford_tools.f90:620-709 with ncap from ERI_toroidal.f90:94-104)
 // ---------------------------------------------------------------------------
-// Nodes, weights and t-table of one rule (fast: nq mapped Gauss-Legendre
-// points, else the literal 64/128 table) for the context's torus and ncap.
-static void fill_rule(const tcg_ctx *c, int fast, int nq, int nqp, std::vector<double> &tnode,
-                      std::vector<double> &tweight, std::vector<double> &tab, std::vector<int> &tablim,
-                      std::vector<int> &tnmax) {
-    const int ncap = c->ncap, rows = ncap + 5;
+// Nodes and weights of one rule (fast: nq mapped Gauss-Legendre points, else
+// the literal 64/128 table), padded to nqp.
+static void rule_nodes(int fast, int nq, int nqp, std::vector<double> &tnode, std::vector<double> &tweight) {
     tnode.assign(nqp, 0.0);
     tweight.assign(nqp, 0.0);
     if (fast) {
@@ -191,27 +181,16 @@ static void fill_rule(const tcg_ctx *c, int fast, int nq, int nqp, std::vector<d
         std::copy(t, t + nq, tnode.begin());
         std::copy(w, w + nq, tweight.begin());
     }
-    tab.assign((size_t)c->ndp * rows * nqp, 0.0);
-    const int nh = nqp / kNodeTile;
-    tablim.assign((size_t)c->ndp * nh, -1);
-    tnmax.assign((size_t)c->ndp * nqp, -1);
-    std::vector<double> col(ncap + 1);
-    for (int d = 0; d < c->ndp; ++d) {
-        for (int k = 0; k < nq; ++k) {
-            const double C = 2.0 * tnode[k] * tnode[k] / (c->a[d] * c->a[d]);
-            bessel_scaled_vector(ncap, C, col.data());
-            int nmax = ncap;   // last order kept: first below 1e-16 relative, + 5
-            for (int n = 0; n <= ncap; ++n)
-                if (col[n] < 1e-16 * col[0]) { nmax = std::min(ncap, n + 5); break; }
-            for (int n = 0; n <= nmax; ++n) tab[((size_t)d * rows + n) * nqp + k] = (n == 0 ? 1.0 : 2.0) * col[n];
-            tnmax[(size_t)d * nqp + k] = nmax;
-            int &lim = tablim[d * nh + k / kNodeTile];
-            lim = std::max(lim, nmax);
-        }
-    }
 }
 
-static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim, std::vector<int> &tnmax) {
+// Tables of every rule the quartets may use: rule 0 = the run's rule
+// (fast_quad / nquad_point), rules 1.. = the learned orders; all padded to
+// the largest node count c->nqp_all.  Layout [rule][d][n][node] (t-table),
+// [rule][d][node tile] (last nonzero order), [rule][d][node] (t_nmax),
+// [rule][node] (nodes, weights).  The Bessel columns are independent: they
+// are spread over host threads.
+static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim, std::vector<int> &tnmax,
+                        std::vector<double> &tnode, std::vector<double> &tweight) {
     int lmax = 0;
     double emax = 0.0, amin = 1e300;
     for (int i = 0; i < c->nbas; ++i) {
@@ -220,7 +199,57 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
     }
     for (int d = 0; d < c->ndp; ++d) amin = std::min(amin, c->a[d]);
     c->ncap = nmax_est(4.0 * emax / (amin * amin)) + 2 * lmax + 1;
-    fill_rule(c, c->fast, c->nq, c->nqp, c->tnode, c->tweight, tab, tablim, tnmax);
+    c->rule_nq.assign(1, c->nq);
+    for (int n : c->learned_orders) c->rule_nq.push_back(n);
+    int nqp = c->nqp;
+    for (int n : c->rule_nq) nqp = std::max(nqp, ((n + kNodeTile - 1) / kNodeTile) * kNodeTile);
+    c->nqp_all = nqp;
+    const int nr = (int)c->rule_nq.size(), ndp = c->ndp, ncap = c->ncap, rows = ncap + 5, nh = nqp / kNodeTile;
+    tnode.assign((size_t)nr * nqp, 0.0);
+    tweight.assign((size_t)nr * nqp, 0.0);
+    for (int r = 0; r < nr; ++r) {
+        std::vector<double> tn, tw;
+        rule_nodes(r == 0 ? c->fast : 1, c->rule_nq[r], nqp, tn, tw);
+        std::copy(tn.begin(), tn.end(), tnode.begin() + (size_t)r * nqp);
+        std::copy(tw.begin(), tw.end(), tweight.begin() + (size_t)r * nqp);
+        if (r == 0) {
+            c->tnode = tn;
+            c->tweight = tw;
+        }
+    }
+    tab.assign((size_t)nr * ndp * rows * nqp, 0.0);
+    tnmax.assign((size_t)nr * ndp * nqp, -1);
+    // columns (rule, d, node), each a scaled Bessel vector up to ncap
+    std::vector<int> cols;
+    for (int r = 0; r < nr; ++r)
+        for (int d = 0; d < ndp; ++d)
+            for (int k = 0; k < c->rule_nq[r]; ++k) cols.push_back((r * ndp + d) * nqp + k);
+    std::atomic<size_t> next(0);
+    auto work = [&] {
+        std::vector<double> col(ncap + 1);
+        for (size_t i = next++; i < cols.size(); i = next++) {
+            const int code = cols[i], k = code % nqp, rd = code / nqp, d = rd % ndp, r = rd / ndp;
+            const double t = tnode[(size_t)r * nqp + k];
+            const double C = 2.0 * t * t / (c->a[d] * c->a[d]);
+            bessel_scaled_vector(ncap, C, col.data());
+            int nmax = ncap;   // last order kept: first below 1e-16 relative, + 5
+            for (int n = 0; n <= ncap; ++n)
+                if (col[n] < 1e-16 * col[0]) { nmax = std::min(ncap, n + 5); break; }
+            for (int n = 0; n <= nmax; ++n) tab[((size_t)rd * rows + n) * nqp + k] = (n == 0 ? 1.0 : 2.0) * col[n];
+            tnmax[(size_t)rd * nqp + k] = nmax;
+        }
+    };
+    const unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < nthr && i < cols.size(); ++i) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    tablim.assign((size_t)nr * ndp * nh, -1);
+    for (int rd = 0; rd < nr * ndp; ++rd)
+        for (int k = 0; k < nqp; ++k) {
+            int &lim = tablim[(size_t)rd * nh + k / kNodeTile];
+            lim = std::max(lim, tnmax[(size_t)rd * nqp + k]);
+        }
     return 0;
 }
 
@@ -388,8 +417,10 @@ static int build_pairs(tcg_ctx *c, std::vector<VecJob> &jobs, size_t &nvec, size
 static EriArgs base_args(tcg_ctx *c) {
     EriArgs A{};
     A.ndp = c->ndp;
-    A.nq = c->nq;
-    A.nqp = c->nqp;
+    A.nq = c->nq;                  // rule 0
+    A.nqp = c->nqp_all;            // node stride of every rule
+    A.rule_nq = c->d_rule_nq.p;
+    A.qrule = nullptr;
     A.ncap = c->ncap;
     A.fac_const = 2.0 / std::sqrt(kPi);
     for (int d = 0; d < 3; ++d) {
@@ -422,40 +453,6 @@ static EriArgs base_args(tcg_ctx *c) {
     return A;
 }
 
-// tables of rule (fast, nq), built on first use for the current basis
-static RuleTables *get_rule(tcg_ctx *c, int fast, int nq, int *rc) {
-    for (auto &r : c->rules)
-        if (r->fast == fast && r->nq == nq) return r.get();
-    auto rt = std::make_unique<RuleTables>();
-    rt->fast = fast;
-    rt->nq = nq;
-    rt->nqp = ((nq + kNodeTile - 1) / kNodeTile) * kNodeTile;
-    std::vector<double> tab;
-    std::vector<int> tablim, tnmax;
-    fill_rule(c, fast, nq, rt->nqp, rt->tnode, rt->tweight, tab, tablim, tnmax);
-    hipError_t e = rt->d_tab.upload(tab);
-    if (e == hipSuccess) e = rt->d_tablim.upload(tablim);
-    if (e == hipSuccess) e = rt->d_tnmax.upload(tnmax);
-    if (e == hipSuccess) e = rt->d_tnode.upload(rt->tnode);
-    if (e == hipSuccess) e = rt->d_tw.upload(rt->tweight);
-    if (e != hipSuccess) {
-        *rc = c->fail(TCG_E_DEVICE, "rule tables: %s", hipGetErrorString(e));
-        return nullptr;
-    }
-    c->rules.push_back(std::move(rt));
-    return c->rules.back().get();
-}
-
-static void use_rule(EriArgs &A, const RuleTables *rt) {
-    A.nq = rt->nq;
-    A.nqp = rt->nqp;
-    A.tab = rt->d_tab.p;
-    A.tab_lim = rt->d_tablim.p;
-    A.tnmax = rt->d_tnmax.p;
-    A.tnode = rt->d_tnode.p;
-    A.tweight = rt->d_tw.p;
-}
-
 // Quartets of A.q4[0..A.nq4) through the row pipeline (tcg_rows.hip):
 // survivors -> rows -> sort by (direction, length) -> MFMA -> contraction.
 // One chunk of quartets (A.q4[0..A.nq4)) whose survivor/row offsets so/ro
@@ -471,9 +468,13 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     HIPCHK(c, c->d_rowval.alloc((size_t)nr * A.nqp));
     HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, so, c->d_surv.p, c->st));
     HIPCHK(c, launch_rows_emit(A, so, ro, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
-    size_t tb = sort_temp_bytes((int)nr, 18);
+    int rbits = 0;
+    while ((1 << rbits) < (int)c->rule_nq.size()) ++rbits;
+    const int end_bit = 18 + (A.qrule ? rbits : 0);
+    size_t tb = sort_temp_bytes((int)nr, end_bit);
     HIPCHK(c, c->d_tmp.alloc(tb));
-    HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, 18, c->st));
+    HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, end_bit,
+                         c->st));
     if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev2, c->st));
     HIPCHK(c, launch_rows_mfma(A, c->d_rows.p, c->d_order.p, nr, c->d_rowval.p, c->st));
     if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev3, c->st));
@@ -552,6 +553,7 @@ static int run_quartets(tcg_ctx *c, EriArgs A) {
         EriArgs B = A;
         B.q4 = A.q4 + q0;
         B.nq4 = q1 - q0;
+        if (A.qrule) B.qrule = A.qrule + q0;
         const int rc = run_chunk(c, B, c->d_soffc.p, c->d_roffc.p, nsc, nrc);
         if (rc) return rc;
     }
@@ -578,11 +580,10 @@ static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, s
 static int prepare(tcg_ctx *c) {
     if (c->prepared) return 0;
     // ---- 1-2: tables and pair data ----
-    std::vector<double> tab;
+    std::vector<double> tab, rtnode, rtw;
     std::vector<int> tablim, tnmax;
     const double tt0 = now_ms();
-    build_tables(c, tab, tablim, tnmax);
-    c->rules.clear();
+    build_tables(c, tab, tablim, tnmax, rtnode, rtw);
     c->times[7] = now_ms() - tt0;
     std::vector<VecJob> jobs;
     size_t nvec = 0, niv = 0;
@@ -598,8 +599,9 @@ static int prepare(tcg_ctx *c) {
     HIPCHK(c, c->d_tab.upload(tab));
     HIPCHK(c, c->d_tablim.upload(tablim));
     HIPCHK(c, c->d_tnmax.upload(tnmax));
-    HIPCHK(c, c->d_tnode.upload(c->tnode));
-    HIPCHK(c, c->d_tw.upload(c->tweight));
+    HIPCHK(c, c->d_tnode.upload(rtnode));
+    HIPCHK(c, c->d_tw.upload(rtw));
+    HIPCHK(c, c->d_rule_nq.upload(c->rule_nq));
     HIPCHK(c, c->d_sh.upload(c->sh));
     HIPCHK(c, c->d_lxyz.upload(c->lxyz));
     HIPCHK(c, c->d_coef.upload(c->coef));
@@ -787,52 +789,25 @@ static int learned_pass(tcg_ctx *c, EriArgs A) {
         HIPCHK(c, c->d_feat.alloc((size_t)ns * 6));
         HIPCHK(c, c->d_pred.alloc((size_t)ns * 2));
         HIPCHK(c, c->d_ood.alloc(ns));
-        HIPCHK(c, launch_features(A, c->d_soff.p, c->d_surv.p, c->d_expo.p, c->d_mlpnorm.p, c->d_feat.p, c->d_ood.p,
-                                  c->st));
+        HIPCHK(c, c->d_sq.alloc(ns));
+        HIPCHK(c, launch_surv_owner(c->d_soff.p, nq4, c->d_sq.p, c->st));
+        HIPCHK(c, launch_features(A, c->d_sq.p, ns, c->d_surv.p, c->d_expo.p, c->d_mlpnorm.p, c->d_feat.p,
+                                  c->d_ood.p, c->st));
         HIPCHK(c, launch_mlp_fwd(c->d_mlp.p, c->d_feat.p, (int)ns, c->d_pred.p, c->st));
     }
     HIPCHK(c, launch_rules(nq4, c->d_soff.p, c->d_pred.p, c->d_ood.p, c->learned_margin, nord, c->d_qrule.p, c->st));
-    // order the quartets by rule
-    HIPCHK(c, c->d_rkey.alloc(nq4));
-    HIPCHK(c, c->d_rkey2.alloc(nq4));
-    HIPCHK(c, c->d_ridx.alloc(nq4));
-    HIPCHK(c, c->d_ridx2.alloc(nq4));
+    // quartets per rule (rule 0 = the plain rule)
     HIPCHK(c, c->d_hist.alloc(16));
     HIPCHK(c, launch_fill((double *)c->d_hist.p, 16, 0.0, c->st));
-    HIPCHK(c, launch_rule_keys(nq4, c->d_qrule.p, c->d_rkey.p, c->d_ridx.p, c->d_hist.p, c->st));
-    tb = sort_temp_bytes(nq4, 8);
-    HIPCHK(c, c->d_tmp.alloc(tb));
-    HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_rkey.p, c->d_rkey2.p, c->d_ridx.p, c->d_ridx2.p, nq4, 8, c->st));
-    HIPCHK(c, c->d_q4b.alloc(nq4));
-    HIPCHK(c, launch_gather_q4(nq4, c->d_ridx2.p, A.q4, c->d_q4b.p, c->st));
+    HIPCHK(c, launch_rule_hist(nq4, c->d_qrule.p, c->d_hist.p, c->st));
     unsigned long long hist[16];
     HIPCHK(c, hipMemcpyAsync(hist, c->d_hist.p, sizeof hist, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    long long start = 0, surv = 0, rows = 0;
-    double rows_ms = 0.0;
-    for (int r = 0; r <= nord; ++r) {
-        const long long cnt = (long long)hist[r];
-        c->learned_counts[r] = cnt;
-        if (cnt == 0) continue;
-        EriArgs B = A;
-        B.q4 = c->d_q4b.p + start;
-        B.nq4 = (int)cnt;
-        if (r < nord) {
-            int rc = 0;
-            RuleTables *rt = get_rule(c, 1, c->learned_orders[r], &rc);
-            if (!rt) return rc;
-            use_rule(B, rt);
-        }
-        const int rc = run_quartets(c, B);
-        if (rc) return rc;
-        surv += c->last_surv;
-        rows += c->last_rows;
-        rows_ms += c->rows_ms;
-        start += cnt;
-    }
-    c->last_surv = surv;
-    c->last_rows = rows;
-    c->rows_ms = rows_ms;
+    // one pass over all quartets, each with its rule (rows sorted by rule, direction, length)
+    A.qrule = c->d_qrule.p;
+    const int rc = run_quartets(c, A);
+    if (rc) return rc;
+    for (int r = 0; r < nord; ++r) c->learned_counts[r] = (long long)hist[r + 1];
+    c->learned_counts[nord] = (long long)hist[0];
     return 0;
 }
 
@@ -1445,6 +1420,8 @@ int tcg_set_learned(tcg_ctx *c, const float *params, const float *norm, const in
     c->mlp_norm.assign(norm, norm + 24);
     c->learned_orders.assign(orders, orders + norders);
     c->learned_margin = margin;
+    c->prepared = false;          // the tables get the learned rules
+    c->direct_ready = false;
     HIPCHK(c, c->d_mlp.upload(c->mlp_params));
     HIPCHK(c, c->d_mlpnorm.upload(c->mlp_norm));
     return TCG_OK;

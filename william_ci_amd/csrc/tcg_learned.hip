@@ -1,13 +1,14 @@
 // Learned per-quartet quadrature orders (gfx950).
 //
 //   k_features   six features of every surviving primitive quartet (one
-//                wavefront per shell quartet, lanes over its survivors),
-//                normalised; a quartet outside the training range is flagged.
+//                thread each), normalised; a quartet outside the training
+//                range is flagged.
 //   k_rules      the rule of each shell quartet: the largest order index the
 //                network asks for over its primitive quartets, one step of
 //                margin added; any flagged primitive quartet -> the plain rule.
+//                Rule numbers of the tables: 0 = the plain rule, r + 1 =
+//                learned order r.
 //   k_rule_hist  shell quartets per rule.
-//   k_gather_q4  the quartet list reordered by rule.
 //
 // Features of the primitive quartet (e1 A, e2 B | e3 C, e4 D) on a torus of
 // length L (periodic direction x, chord distances, a = 2 pi / L):
@@ -39,16 +40,16 @@ __device__ double pair_d2(double a, const double *A, const double *B) {
     return chord2(a, A[0] - B[0]) + dy * dy + dz * dz;
 }
 
-__global__ __launch_bounds__(64) void k_features(EriArgs A, const long long *__restrict__ soff,
-                                                 const Surv *__restrict__ surv, const double *__restrict__ expo,
-                                                 const float *__restrict__ norm, float *__restrict__ feat,
-                                                 unsigned char *__restrict__ ood) {
+__global__ __launch_bounds__(256) void k_features(EriArgs A, const int *__restrict__ owner, long long ns,
+                                                  const Surv *__restrict__ surv, const double *__restrict__ expo,
+                                                  const float *__restrict__ norm, float *__restrict__ feat,
+                                                  unsigned char *__restrict__ ood) {
     const double a = A.a[0], L = A.L[0];
-    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
-        const Quartet Q = A.q4[qi];
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < ns; g += (long long)gridDim.x * blockDim.x) {
+        const Quartet &Q = A.q4[owner[g]];
         const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
         const int lt = sA.l + sB.l + sC.l + sD.l;
-        for (long long g = soff[qi] + threadIdx.x; g < soff[qi + 1]; g += 64) {
+        {
             const Surv S = surv[g];
             const PrimPair &pb = A.pp[S.pb], &pk = A.pp[S.pk];
             const double e1 = expo[A.ao_poff[sA.first_ao] + pb.pi], e2 = expo[A.ao_poff[sB.first_ao] + pb.pj];
@@ -97,30 +98,27 @@ __global__ __launch_bounds__(64) void k_rules(int nq4, const long long *__restri
             const int o = __shfl_xor(r, off, 64);
             r = r > o ? r : o;
         }
-        if (threadIdx.x == 0) qrule[qi] = r;
+        if (threadIdx.x == 0) qrule[qi] = r >= plain ? 0 : r + 1;
     }
 }
 
-__global__ void k_rule_keys(int nq4, const int *__restrict__ qrule, unsigned *__restrict__ keys, int *__restrict__ idx,
-                            unsigned long long *__restrict__ hist) {
-    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= nq4) return;
-    keys[qi] = (unsigned)qrule[qi];
-    idx[qi] = qi;
-    atomicAdd(&hist[qrule[qi]], 1ull);
+__global__ __launch_bounds__(256) void k_rule_hist(int nq4, const int *__restrict__ qrule,
+                                                   unsigned long long *__restrict__ hist) {
+    __shared__ unsigned h[16];
+    if (threadIdx.x < 16) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq4; qi += gridDim.x * blockDim.x)
+        atomicAdd(&h[qrule[qi] & 15], 1u);
+    __syncthreads();
+    if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
-__global__ void k_gather_q4(int n, const int *__restrict__ order, const Quartet *__restrict__ in,
-                            Quartet *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[order[i]];
-}
-
-hipError_t launch_features(const EriArgs &a, const long long *soff, const Surv *surv, const double *expo,
+hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, const Surv *surv, const double *expo,
                            const float *norm, float *feat, unsigned char *ood, hipStream_t st) {
-    if (a.nq4 <= 0) return hipSuccess;
-    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
-    hipLaunchKernelGGL(k_features, dim3(grid), dim3(64), 0, st, a, soff, surv, expo, norm, feat, ood);
+    if (ns <= 0) return hipSuccess;
+    const long long blocks = (ns + 255) / 256;
+    const int grid = blocks < (1 << 20) ? (int)blocks : (1 << 20);
+    hipLaunchKernelGGL(k_features, dim3(grid), dim3(256), 0, st, a, owner, ns, surv, expo, norm, feat, ood);
     return hipGetLastError();
 }
 
@@ -132,16 +130,10 @@ hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const
     return hipGetLastError();
 }
 
-hipError_t launch_rule_keys(int nq4, const int *qrule, unsigned *keys, int *idx, unsigned long long *hist,
-                            hipStream_t st) {
+hipError_t launch_rule_hist(int nq4, const int *qrule, unsigned long long *hist, hipStream_t st) {
     if (nq4 <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rule_keys, dim3((nq4 + 255) / 256), dim3(256), 0, st, nq4, qrule, keys, idx, hist);
-    return hipGetLastError();
-}
-
-hipError_t launch_gather_q4(int n, const int *order, const Quartet *in, Quartet *out, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_q4, dim3((n + 255) / 256), dim3(256), 0, st, n, order, in, out);
+    const int grid = (nq4 + 255) / 256 < 1024 ? (nq4 + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_rule_hist, dim3(grid), dim3(256), 0, st, nq4, qrule, hist);
     return hipGetLastError();
 }
 

@@ -47,6 +47,10 @@ __device__ __forceinline__ int wmin(int v) {
     return v;
 }
 
+// rule of a quartet and its node count
+__device__ __forceinline__ int quartet_rule(const EriArgs &A, int qi) { return A.qrule ? A.qrule[qi] : 0; }
+__device__ __forceinline__ int rule_nodes(const EriArgs &A, int r) { return A.rule_nq ? A.rule_nq[r] : A.nq; }
+
 __device__ __forceinline__ void quartet_shape(const EriArgs &A, const Quartet &Q, int l[4], int &ncomb) {
     for (int x = 0; x < 4; ++x) l[x] = A.sh[Q.s[x]].l;
     ncomb = (l[0] + 1) * (l[1] + 1) * (l[2] + 1) * (l[3] + 1);
@@ -142,9 +146,10 @@ __global__ __launch_bounds__(64) void k_rows_emit(EriArgs A, const long long *__
             const double shd = A.q4[qi].shift[d];
             R.phi = ad * (pb.cen[d] - (pk.cen[d] + shd));
             R.sgn = pow(ad, -(double)(ia + ib + ic + id));
-            R.d = d;
+            const int rule = quartet_rule(A, qi);
+            R.d = d | (rule << 8);
             rows[r0 + rl] = R;
-            keys[r0 + rl] = ((unsigned)d << 16) | (unsigned)R.nm;
+            keys[r0 + rl] = ((unsigned)rule << 18) | ((unsigned)d << 16) | (unsigned)R.nm;
             vals[r0 + rl] = (int)(r0 + rl);
         }
     }
@@ -165,13 +170,13 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
                                                   const int *__restrict__ order, long long nrows, int hbase,
                                                   double *__restrict__ rowval) {
     const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
-    const int nqp = A.nqp, nh = kNT, nq = A.nq, nht = A.nqp / kNodeTile;
+    const int nqp = A.nqp, nh = kNT, nht = A.nqp / kNodeTile;
     const int tabrows = A.ncap + 5;
     const long long ntiles = (nrows + 15) / 16;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long long pos = tile * 16 + r;
         const bool valid = pos < nrows;
-        int ridx = -1, nm = -1, d = 255;
+        int ridx = -1, nm = -1, d = 0x7fffffff;         // d: direction | rule << 8
         double phi = 0.0, sgn = 1.0;
         const double2 *Gp = A.vec, *Hp = A.vec;
         if (valid) {
@@ -186,20 +191,22 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
         }
         if (A.counters && kq == 0 && valid && hbase == 0) {
             long s = 0;
+            const int rl = d >> 8, dl = d & 0xff, nql = rule_nodes(A, rl);
 #pragma unroll 1
-            for (int k = 0; k < nq; ++k) {
-                const int tk = A.tnmax[d * nqp + k];
+            for (int k = 0; k < nql; ++k) {
+                const int tk = A.tnmax[(rl * A.ndp + dl) * nqp + k];
                 s += (nm < tk ? nm : tk) + 1;
             }
             atomicAdd(&A.counters[4], (unsigned long long)(2 * s + 10 * (nm + 1)));
         }
-        const int dlo = wmin(valid ? d : 255), dhi = wmax(valid ? d : -1);
         double s4, c4, sn0, cn0;
         sincos(4.0 * phi, &s4, &c4);
         sincos((double)kq * phi, &sn0, &cn0);
+        // the (direction, rule) combinations present in the tile, ascending
 #pragma unroll 1
-        for (int dd = dlo; dd <= dhi; ++dd) {
-            const int nmd = (d == dd) ? nm : -1;      // rows of another direction contribute 0
+        for (int dd = wmin(d); dd != 0x7fffffff; dd = wmin(d > dd ? d : 0x7fffffff)) {
+            const int nmd = (d == dd) ? nm : -1;      // rows of another combination contribute 0
+            const int tr = (dd >> 8) * A.ndp + (dd & 0xff);   // table block of this rule and direction
             const int nmax_t = wmax(nmd);
             int lim[kNT];
             int nend = -1;
@@ -207,12 +214,12 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
             for (int h = 0; h < kNT; ++h) {
                 lim[h] = -1;
                 if (h < nh) {
-                    const int lh = A.tab_lim[dd * nht + hbase + h];
+                    const int lh = A.tab_lim[tr * nht + hbase + h];
                     lim[h] = lh < nmax_t ? lh : nmax_t;
                     nend = lim[h] > nend ? lim[h] : nend;
                 }
             }
-            const double *tab = A.tab + (size_t)dd * tabrows * nqp + hbase * kNodeTile + r;
+            const double *tab = A.tab + (size_t)tr * tabrows * nqp + hbase * kNodeTile + r;
             dbl4 acc[kNT];
 #pragma unroll
             for (int h = 0; h < kNT; ++h) acc[h] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -310,14 +317,16 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
 __global__ void k_real_rows(EriArgs A, const long long *__restrict__ soff, const long long *__restrict__ roff,
                             const Surv *__restrict__ surv, const int *__restrict__ sq, long long ns,
                             double *__restrict__ realval) {
-    const int nreal = 3 - A.ndp, nq = A.nq, nqp = A.nqp;
+    const int nreal = 3 - A.ndp, nqp = A.nqp;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ns * nreal * nq) return;
-    const int k = (int)(t % nq);
-    const int drr = (int)((t / nq) % nreal);
-    const long long g = t / ((long long)nq * nreal);
+    if (t >= ns * nreal * nqp) return;
+    const int k = (int)(t % nqp);
+    const int drr = (int)((t / nqp) % nreal);
+    const long long g = t / ((long long)nqp * nreal);
     const int dr = A.ndp + drr;
     const int qi = sq[g];
+    const int rule = quartet_rule(A, qi);
+    if (k >= rule_nodes(A, rule)) return;
     const Quartet *Q = A.q4 + qi;
     const int lA = A.sh[Q->s[0]].l, lB = A.sh[Q->s[1]].l, lC = A.sh[Q->s[2]].l, lD = A.sh[Q->s[3]].l;
     const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
@@ -326,7 +335,7 @@ __global__ void k_real_rows(EriArgs A, const long long *__restrict__ soff, const
     const PrimPair &pb = A.pp[S.pb];
     const PrimPair &pk = A.pp[S.pk];
     const double p = pb.q, q = pk.q;
-    const double tk = A.tnode[k], t2 = tk * tk;
+    const double tk = A.tnode[rule * nqp + k], t2 = tk * tk;
     const double D2 = 1.0 / (p * q + (p + q) * t2);
     const double w = p * q * t2 * D2;
     const double rs = 3.14159265358979323846 * sqrt(D2);
@@ -381,7 +390,7 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
     if (a.ndp >= 3 || ns <= 0) return hipSuccess;
     const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
     hipLaunchKernelGGL(k_surv_quartet, dim3(grid), dim3(64), 0, st, soff, a.nq4, sq);
-    const long long n = ns * (3 - a.ndp) * a.nq;
+    const long long n = ns * (3 - a.ndp) * a.nqp;
     hipLaunchKernelGGL(k_real_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, soff, roff, surv, sq, ns,
                        realval);
     return hipGetLastError();
@@ -393,10 +402,12 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
 // accumulated into blk (zeroed by the caller) with LDS atomics, tasks
 // (primitive quartet g, component) spread over the 64 lanes.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q, long s0, long ns, long r0,
-                                              const Surv *__restrict__ surv, const double *__restrict__ rowval,
-                                              const double *__restrict__ realval, double *blk, int lane) {
-    const int nqp = A.nqp, nq = A.nq, ndp = A.ndp, nreal = 3 - ndp;
+__device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q, int rule, long s0, long ns,
+                                              long r0, const Surv *__restrict__ surv,
+                                              const double *__restrict__ rowval, const double *__restrict__ realval,
+                                              double *blk, int lane) {
+    const int nqp = A.nqp, nq = rule_nodes(A, rule), ndp = A.ndp, nreal = 3 - ndp;
+    const double *tw = A.tweight + (size_t)rule * nqp;
     const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
     const int lA = sA.l, lB = sB.l, lC = sC.l, lD = sD.l;
     const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
@@ -424,7 +435,7 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
                 I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
         }
         double val = 0.0;
-        for (int k = 0; k < nq; ++k) val += (I3[0][k] * A.tweight[k]) * I3[1][k] * I3[2][k];
+        for (int k = 0; k < nq; ++k) val += (I3[0][k] * tw[k]) * I3[1][k] * I3[2][k];
         const Surv S = surv[s0 + g];
         const PrimPair &pb = A.pp[S.pb];
         const PrimPair &pk = A.pp[S.pk];
@@ -470,7 +481,7 @@ __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__r
         __syncthreads();
         for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
         __syncthreads();
-        quartet_block(A, Q, s0, ns, r0, surv, rowval, realval, blk, lane);
+        quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, lane);
         __syncthreads();
         if (A.blk_out && Q.out >= 0) {
             for (int i = lane; i < nblk; i += 64) A.blk_out[Q.out + i] = blk[i];
@@ -566,7 +577,7 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         wave_sync();
         for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
         wave_sync();
-        quartet_block(A, Q, s0, ns, r0, surv, rowval, realval, blk, lane);
+        quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, lane);
         wave_sync();
         for (int task = lane; task < nblk * 8; task += 64) {
             const int comp = task >> 3, ip = task & 7;
@@ -666,6 +677,13 @@ hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, in
 hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_rebase, dim3((n + 255) / 256), dim3(256), 0, st, src, n, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_surv_owner(const long long *soff, int nq4, int *owner, hipStream_t st) {
+    if (nq4 <= 0) return hipSuccess;
+    const int grid = nq4 < (1 << 20) ? nq4 : (1 << 20);
+    hipLaunchKernelGGL(k_surv_quartet, dim3(grid), dim3(64), 0, st, soff, nq4, owner);
     return hipGetLastError();
 }
 
