@@ -114,6 +114,50 @@ int tcg_mp2_gamma(tcg_ctx *ctx, int nbas, int nocc, const double *e, const doubl
  * out for |D| < 1e-8, out[4] = GPU wall time in ms. */
 int tcg_kmp2(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out);
 
+/* Density fitting (keyword density_fitting, SAO path).  A DF context holds
+ * the fitted three-index tensor df_B(nfuc, nBas, naux) of one torus system
+ * on the GPU.
+ *
+ * tcg_df_create replaces
+ *   df_init(ndp,n_atoms,atoms,geometry,nBas,AO,iout)   src/module/df_toroidal.f90:352-709
+ * (metric, pivoted Cholesky chol_f :1174-1205 with tolerance 1e-10, the
+ * 3-index integrals with the df_schwarz 1e-13 screen, B as a cell
+ * convolution).  The auxiliary functions come as an AO list like
+ * tcg_set_basis's (normalised coefficients), nau_cell (<= 64) per cell, the
+ * cells in the order of the orbital basis; the exponent-0 dummy of
+ * dummy_at (:837-844) is added by the library.  ctx supplies the torus, the
+ * orbital basis and the quadrature rule. */
+typedef struct tcg_df tcg_df;
+int tcg_df_create(tcg_ctx *ctx, int nau_cell, const double *aux_center, const int *aux_lxyz,
+                  const int *aux_prim_off, const double *aux_expo, const double *aux_coef, tcg_df **out);
+
+/* info[0] = nau, info[1] = naux, info[2] = aux functions skipped by the
+ * pivoted Cholesky (largest over k); *dmin = smallest pivot kept (the three
+ * lines df_init prints, :601-603). */
+int tcg_df_info(tcg_df *df, long long *info, double *dmin);
+
+/* Copy of df_B(nfuc, nBas, naux), column-major (module variable df_B). */
+int tcg_df_b(tcg_df *df, double *B);
+
+/* Coulomb and exchange from the fitted integrals.  Replaces
+ *   df_JK(nBas,P,J,K)                 src/module/df_toroidal.f90:850-957
+ * J(mu0,nu) = sum (mu0 nu|la si) P(la,si), K(mu0,si) = -1/2 sum (mu0 nu|la si) P(nu,la),
+ * rows of the unit cell filled, the other rows zero (spread_rows :958-971);
+ * exchange reads P through its first nfuc rows, as the k-space exchange does. */
+int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K);
+
+/* k-point MP2 from the fitted integrals.  Replaces
+ *   df_kMO(nBas,coeff_SAO,p_lo,p_hi,q_lo,q_hi,ERI_out,iout)   src/module/df_toroidal.f90:981-1050
+ *   df_L_kap                                                 :1053-1112
+ * followed by kMP2_calc (src/Energies/SAO/MP2/MP2_SAO.f90:6-151); arguments
+ * and out[0..4] as tcg_kmp2. */
+int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out);
+
+/* out[0] metric + Cholesky, out[1] Schwarz + 3-index integrals, out[2] B (ms). */
+int tcg_df_times(tcg_df *df, double *out);
+const char *tcg_df_last_error(tcg_df *df);
+int tcg_df_destroy(tcg_df *df);
+
 /* One-electron matrices S, T, V (nBas,nBas): rows of the unit cell computed,
  * values below 1e-14 zeroed, the other rows filled by translation.  Replaces
  *   overlap_matrix_toroidal_nD            e.g. src/Toroidal/1D/1D_Overlap_toroidal.f90:23

@@ -3,9 +3,23 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "tcg_engine.hpp"
 
+struct tcg_ctx;
+
 namespace tcg {
+
+// What the density-fitting driver reads of a context (host pointers).
+struct CtxView {
+    int ndp, ncell[3], nbas, nfuc, fast, nq;
+    double L[3];
+    const double *center, *expo, *coef;
+    const int *lxyz, *poff;
+};
+int ctx_view(const tcg_ctx *c, CtxView *v);
+int ctx_shells(tcg_ctx *c, std::vector<int> &first_ao, std::vector<int> &ncomp, std::vector<int> &cell);
 
 // Everything the quartet kernel reads.
 struct EriArgs {

@@ -1,0 +1,355 @@
+// Density fitting on the GPU (gfx950): the kernels of tcg_df.cpp.
+//
+//   k_metric_factor  per k point: pivoted Cholesky of the Hermitian metric
+//                    V(k) (nau <= 64) with the stopping rule of LAPACK's
+//                    ?pstrf (stop when the largest remaining diagonal <= tol),
+//                    then f = P (L^-1)^H on the kept columns
+//                    (src/module/df_toroidal.f90:1174-1205); only for the
+//                    first k of each pair (k, -k).
+//   k_metric_mirror  f(-k) = conj f(k) for the second k of each pair
+//                    (:589-592), so that B = 1/N Re sum_k W(k) e^{-ikR} is a
+//                    factor of the fitted integrals (W(-k) = conj W(k)).
+//   k_df_gamma       G(Q, R) = sum_x B(x, Q) Ps(x, R)  (x = (mu0, nu) pairs)
+//   k_df_jvec        J(x) = sum_Q B(x, Q) gamma(Q)
+//   k_df_k           K(mu0, si) = -1/2 sum_{la, Q} X(mu0, la, Q) B(la0, si - R_la, Q - R_la),
+//                    X(mu0, la, Q) = sum_nu B(mu0, nu, Q) P(nu, la)
+//   k_df_l           L(p, q, Q, k1, k2) = sum_ab conj C_k1(a, p) C_k2(b, q) F(a, b, k2, Q, k2 - k1)
+//   k_df_eri         ERI(p,q,r,s,k1,k2,k3) = 1/N sum_Q L(p,q,Q,k1,k2) L(r,s,Q,k3,k4)
+// Cells and k points are numbered x fastest; cdiff[c1 * N + c2] = c1 - c2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "tcg_device.hpp"
+
+namespace tcg {
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ double2 zmulc(double2 a, double2 b) {   // a conj(b)
+    return double2{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return double2{a.x, -a.y}; }
+
+// V: [k][nau*nau] (re, im separate, column-major), f out likewise.
+__global__ __launch_bounds__(64) void k_metric_factor(const double *__restrict__ Vre, const double *__restrict__ Vim,
+                                                      int n, const int *__restrict__ mate, double tol,
+                                                      double *__restrict__ fre, double *__restrict__ fim,
+                                                      int *__restrict__ rank, double *__restrict__ dmin) {
+    __shared__ double2 A[64][65];
+    __shared__ double2 Li[64][65];
+    __shared__ int perm[64];
+    __shared__ double red_v[64];
+    __shared__ int red_i[64];
+    const int k = blockIdx.x, t = threadIdx.x;
+    if (mate[k] < k) return;                     // the conjugate of an earlier k: k_metric_mirror
+    const size_t off = (size_t)k * n * n;
+    // Hermitian part of V(k)
+    for (int j = 0; j < n; ++j)
+        if (t < n) {
+            const double2 a = {Vre[off + t + (size_t)n * j], Vim[off + t + (size_t)n * j]};
+            const double2 b = {Vre[off + j + (size_t)n * t], Vim[off + j + (size_t)n * t]};
+            A[t][j] = double2{0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+        }
+    if (t < n) perm[t] = t;
+    __syncthreads();
+    int m = n;
+    for (int j = 0; j < n; ++j) {
+        // pivot: largest remaining diagonal (first one on ties)
+        red_v[t] = (t >= j && t < n) ? A[t][t].x : -1e300;
+        red_i[t] = t;
+        __syncthreads();
+        for (int s = 32; s > 0; s >>= 1) {
+            if (t < s) {
+                const double v2 = red_v[t + s];
+                const int i2 = red_i[t + s];
+                if (v2 > red_v[t] || (v2 == red_v[t] && i2 < red_i[t])) {
+                    red_v[t] = v2;
+                    red_i[t] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        const double dmax = red_v[0];
+        const int p = red_i[0];
+        __syncthreads();
+        if (!(dmax > tol)) {
+            m = j;
+            break;
+        }
+        // symmetric swap of rows/columns j and p
+        if (p != j) {
+            if (t < n) {
+                const double2 x = A[t][j];
+                A[t][j] = A[t][p];
+                A[t][p] = x;
+            }
+            __syncthreads();
+            if (t < n) {
+                const double2 x = A[j][t];
+                A[j][t] = A[p][t];
+                A[p][t] = x;
+            }
+            if (t == 0) {
+                const int x = perm[j];
+                perm[j] = perm[p];
+                perm[p] = x;
+            }
+            __syncthreads();
+        }
+        const double ljj = sqrt(A[j][j].x);
+        __syncthreads();
+        if (t == j) A[j][j] = double2{ljj, 0.0};
+        if (t > j && t < n) A[t][j] = double2{A[t][j].x / ljj, A[t][j].y / ljj};
+        __syncthreads();
+        // trailing update A(i, c) -= L(i, j) conj(L(c, j)), i, c > j
+        if (t > j && t < n) {
+            const double2 lij = A[t][j];
+            for (int c = j + 1; c < n; ++c) {
+                const double2 u = zmulc(lij, A[c][j]);
+                A[t][c] = double2{A[t][c].x - u.x, A[t][c].y - u.y};
+            }
+        }
+        __syncthreads();
+    }
+    // L^-1 of the leading m x m block, one column per lane (forward substitution)
+    if (t < m) {
+        for (int i = 0; i < m; ++i) {
+            double2 s = {i == t ? 1.0 : 0.0, 0.0};
+            for (int c = t; c < i; ++c) {
+                const double2 u = zmul(A[i][c], Li[c][t]);
+                s.x -= u.x;
+                s.y -= u.y;
+            }
+            const double d = A[i][i].x;
+            Li[i][t] = i < t ? double2{0.0, 0.0} : double2{s.x / d, s.y / d};
+        }
+    }
+    __syncthreads();
+    // f(perm(i), j) = conj(Linv(j, i)), i, j < m; zero elsewhere
+    for (int j = 0; j < n; ++j)
+        if (t < n) {
+            fre[off + t + (size_t)n * j] = 0.0;
+            fim[off + t + (size_t)n * j] = 0.0;
+        }
+    __syncthreads();
+    for (int j = 0; j < m; ++j)
+        if (t < m) {
+            const double2 v = zconj(Li[j][t]);
+            fre[off + perm[t] + (size_t)n * j] = v.x;
+            fim[off + perm[t] + (size_t)n * j] = v.y;
+        }
+    if (t == 0) {
+        rank[k] = m;
+        dmin[k] = m > 0 ? A[m - 1][m - 1].x * A[m - 1][m - 1].x : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_metric_mirror(int n, int nk, const int *__restrict__ mate,
+                                                       double *__restrict__ fre, double *__restrict__ fim,
+                                                       int *__restrict__ rank, double *__restrict__ dmin) {
+    const size_t nn = (size_t)n * n;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nn * nk; g += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(g / nn);
+        const int mk = mate[k];
+        if (mk >= k) continue;
+        const size_t e = g % nn;
+        fre[g] = fre[(size_t)mk * nn + e];
+        fim[g] = -fim[(size_t)mk * nn + e];
+        if (e == 0) {
+            rank[k] = rank[mk];
+            dmin[k] = dmin[mk];
+        }
+    }
+}
+
+hipError_t launch_metric_factor(const double *Vre, const double *Vim, int n, int nk, const int *mate, double tol,
+                                double *fre, double *fim, int *rank, double *dmin, hipStream_t st) {
+    if (n > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_metric_factor, dim3(nk), dim3(64), 0, st, Vre, Vim, n, mate, tol, fre, fim, rank, dmin);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t tot = (size_t)n * n * nk;
+    const int grid = (int)std::min<size_t>((tot + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_metric_mirror, dim3(grid), dim3(256), 0, st, n, nk, mate, fre, fim, rank, dmin);
+    return hipGetLastError();
+}
+
+// G(Q, R) = sum_x B(x, Q) Ps(x, R); one block per Q, threads over x
+__global__ __launch_bounds__(256) void k_df_gamma(const double *__restrict__ B, const double *__restrict__ Ps, long nx,
+                                                  int naux, int N, double *__restrict__ G) {
+    __shared__ double red[256];
+    const int Q = blockIdx.x;
+    for (int R = 0; R < N; ++R) {
+        double s = 0.0;
+        for (long x = threadIdx.x; x < nx; x += 256) s += B[x + nx * (size_t)Q] * Ps[x + nx * (size_t)R];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) G[Q + (size_t)naux * R] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_df_jvec(const double *__restrict__ B, const double *__restrict__ gam, long nx,
+                                                 int naux, double *__restrict__ J) {
+    const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    double s = 0.0;
+    for (int Q = 0; Q < naux; ++Q) s += B[x + nx * (size_t)Q] * gam[Q];
+    J[x] = s;
+}
+
+// K(mu0, si): one block per (mu0, si), threads over (la, q, RQ)
+__global__ __launch_bounds__(256) void k_df_k(const double *__restrict__ X, const double *__restrict__ B, int nf,
+                                              int nb, int nau, int N, const int *__restrict__ cdiff,
+                                              double *__restrict__ K) {
+    __shared__ double red[256];
+    const int mu = blockIdx.x % nf, si = blockIdx.x / nf;
+    const int csi = si / nf, lsi = si % nf;
+    const int naux = nau * N;
+    const size_t total = (size_t)nb * naux;
+    double s = 0.0;
+    for (size_t w = threadIdx.x; w < total; w += 256) {
+        const int la = (int)(w % nb), Q = (int)(w / nb);
+        const int R = la / nf, la0 = la % nf;
+        const int q = Q % nau, RQ = Q / nau;
+        // B^Q(la, si) = B(la0, si - R, (q, RQ - R))
+        const int sis = cdiff[csi * N + R] * nf + lsi;
+        const int Qs = q + nau * cdiff[RQ * N + R];
+        s += X[mu + (size_t)nf * (la + (size_t)nb * Q)] * B[la0 + (size_t)nf * (sis + (size_t)nb * Qs)];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) K[mu + (size_t)nf * si] = -0.5 * red[0];
+}
+
+hipError_t launch_df_jk(const double *B, const double *Ps, const double *X, int nf, int nb, int nau, int N,
+                        const int *cdiff, double *G, const double *gam, double *J, double *K, int what,
+                        hipStream_t st) {
+    const long nx = (long)nf * nb;
+    const int naux = nau * N;
+    if (what == 0) {
+        hipLaunchKernelGGL(k_df_gamma, dim3(naux), dim3(256), 0, st, B, Ps, nx, naux, N, G);
+    } else if (what == 1) {
+        hipLaunchKernelGGL(k_df_jvec, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, B, gam, nx, naux, J);
+    } else {
+        hipLaunchKernelGGL(k_df_k, dim3(nf * nb), dim3(256), 0, st, X, B, nf, nb, nau, N, cdiff, K);
+    }
+    return hipGetLastError();
+}
+
+// L(p, q, Q, k1, k2), p < o_hi, v_lo <= q < nf; F(a, b, k2, Q, kap) complex (Fr, Fi)
+__global__ __launch_bounds__(256) void k_df_l(const double *__restrict__ Fr, const double *__restrict__ Fi,
+                                              const double2 *__restrict__ Ck, int nf,
+                                              int nau, int N, int o_hi, int v_lo, const int *__restrict__ cdiff,
+                                              double2 *__restrict__ L) {
+    const int nv = nf - v_lo;
+    const size_t total = (size_t)o_hi * nv * nau * N * N;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        size_t r = t;
+        const int p = (int)(r % o_hi); r /= o_hi;
+        const int q = v_lo + (int)(r % nv); r /= nv;
+        const int Q = (int)(r % nau); r /= nau;
+        const int k1 = (int)(r % N);
+        const int k2 = (int)(r / N);
+        const int kap = cdiff[k2 * N + k1];
+        const double2 *C1 = Ck + (size_t)k1 * nf * nf, *C2 = Ck + (size_t)k2 * nf * nf;
+        const size_t fb = (size_t)nf * nf * (k2 + (size_t)N * (Q + (size_t)nau * kap));
+        double2 acc = {0.0, 0.0};
+        for (int b = 0; b < nf; ++b) {
+            double2 s = {0.0, 0.0};
+            for (int a = 0; a < nf; ++a) {
+                const size_t fi = fb + a + (size_t)nf * b;
+                const double2 u = zmul(zconj(C1[a + (size_t)nf * p]), double2{Fr[fi], Fi[fi]});
+                s.x += u.x;
+                s.y += u.y;
+            }
+            const double2 u = zmul(C2[b + (size_t)nf * q], s);
+            acc.x += u.x;
+            acc.y += u.y;
+        }
+        L[t] = acc;
+    }
+}
+
+// ERI(p,q,r,s,k1,k2,k3) = 1/N sum_Q L(p,q,Q,k1,k2) L(r,s,Q,k3,k4), k4 = k1 - k2 + k3.
+// One block per (k1, k2): L(:,:,:,k1,k2) staged in LDS, every k3 from it.
+__global__ __launch_bounds__(256) void k_df_eri_lds(const double2 *__restrict__ L, int nau, int N, int per,
+                                                    const int *__restrict__ cdiff, const int *__restrict__ cadd,
+                                                    double2 *__restrict__ out) {
+    extern __shared__ double2 L1s[];             // per * nau
+    const int k1 = blockIdx.x % N, k2 = blockIdx.x / N;
+    const double2 *L1 = L + (size_t)per * nau * (k1 + (size_t)N * k2);
+    for (int i = threadIdx.x; i < per * nau; i += blockDim.x) L1s[i] = L1[i];
+    __syncthreads();
+    const int dk = cdiff[k1 * N + k2];
+    const double invN = 1.0 / N;
+    const int pp = per * per;
+    for (int w = threadIdx.x; w < pp * N; w += blockDim.x) {
+        const int k3 = w / pp, e = w - k3 * pp;
+        const int pq = e % per, rs = e / per;
+        const int k4 = cadd[dk * N + k3];
+        const double2 *L2 = L + (size_t)per * nau * (k3 + (size_t)N * k4);
+        double2 acc = {0.0, 0.0};
+        for (int Q = 0; Q < nau; ++Q) {
+            const double2 u = zmul(L1s[pq + per * Q], L2[rs + per * Q]);
+            acc.x += u.x;
+            acc.y += u.y;
+        }
+        out[pq + (size_t)per * (rs + (size_t)per * (k1 + (size_t)N * (k2 + (size_t)N * k3)))] =
+            double2{acc.x * invN, acc.y * invN};
+    }
+}
+
+__global__ __launch_bounds__(256) void k_df_eri(const double2 *__restrict__ L, int nau, int N, int o_hi, int nv,
+                                                const int *__restrict__ cdiff, const int *__restrict__ cadd,
+                                                double2 *__restrict__ out) {
+    const size_t per = (size_t)o_hi * nv;
+    const size_t total = per * per * N * N * N;
+    const double invN = 1.0 / N;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        size_t r_ = t;
+        const size_t pq = r_ % per; r_ /= per;
+        const size_t rs = r_ % per; r_ /= per;
+        const int k1 = (int)(r_ % N); r_ /= N;
+        const int k2 = (int)(r_ % N);
+        const int k3 = (int)(r_ / N);
+        const int k4 = cadd[cdiff[k1 * N + k2] * N + k3];
+        const double2 *L1 = L + pq + per * (size_t)nau * (k1 + (size_t)N * k2);
+        const double2 *L2 = L + rs + per * (size_t)nau * (k3 + (size_t)N * k4);
+        double2 acc = {0.0, 0.0};
+        for (int Q = 0; Q < nau; ++Q) {
+            const double2 u = zmul(L1[per * Q], L2[per * Q]);
+            acc.x += u.x;
+            acc.y += u.y;
+        }
+        out[t] = double2{acc.x * invN, acc.y * invN};
+    }
+}
+
+hipError_t launch_df_l(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,
+                       int v_lo, const int *cdiff, double2 *L, hipStream_t st) {
+    hipLaunchKernelGGL(k_df_l, dim3(2048), dim3(256), 0, st, Fr, Fi, Ck, nf, nau, N, o_hi, v_lo, cdiff, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_df_eri(const double2 *L, int nau, int N, int o_hi, int nv, const int *cdiff, const int *cadd,
+                         double2 *out, hipStream_t st) {
+    const int per = o_hi * nv;
+    const size_t lds = sizeof(double2) * (size_t)per * nau;
+    if (lds <= 65536) {
+        hipLaunchKernelGGL(k_df_eri_lds, dim3(N * N), dim3(256), lds, st, L, nau, N, per, cdiff, cadd, out);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_df_eri, dim3(4096), dim3(256), 0, st, L, nau, N, o_hi, nv, cdiff, cadd, out);
+    return hipGetLastError();
+}
+
+}  // namespace tcg

@@ -359,7 +359,9 @@ static int build_pairs(tcg_ctx *c, std::vector<VecJob> &jobs, size_t &nvec, size
                     P.pi = i;
                     P.pj = j;
                     const double e1 = c->expo[A.poff + i], e2 = c->expo[B.poff + j];
-                    const double esum = e1 + e2, mu = e1 * e2 / esum;
+                    // esum = 0: two constant functions (the exponent-0 s function of the
+                    // density-fitting integrals, df_toroidal.f90:837-844)
+                    const double esum = e1 + e2, mu = esum > 0.0 ? e1 * e2 / esum : 0.0;
                     P.q = esum;
                     double fac = 1.0;
                     for (int d = 0; d < 3; ++d) {
@@ -385,7 +387,8 @@ static int build_pairs(tcg_ctx *c, std::vector<VecJob> &jobs, size_t &nvec, size
                             nvec += (size_t)(A.l + 1) * (B.l + 1) * (N + 1);
                             niv += (size_t)(N + A.l + B.l + 1);
                         } else {
-                            const double cen = (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum;
+                            const double cen = esum > 0.0 ? (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum
+                                                          : 0.5 * (A.xyz[d] + B.xyz[d]);
                             P.cen[d] = cen;
                             fac *= std::exp(-mu * (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]));
                         }
@@ -951,6 +954,41 @@ static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, 
     return 0;
 }
 
+// Read access for the density-fitting driver (tcg_df.cpp), same library.
+namespace tcg {
+int ctx_view(const tcg_ctx *c, CtxView *v) {
+    if (!c || !v) return TCG_E_ARG;
+    v->ndp = c->ndp;
+    for (int d = 0; d < 3; ++d) {
+        v->L[d] = c->L[d];
+        v->ncell[d] = c->ncl[d];
+    }
+    v->nbas = c->nbas;
+    v->nfuc = c->nfuc;
+    v->fast = c->fast;
+    v->nq = c->nq;
+    v->center = c->center.data();
+    v->lxyz = c->lxyz.data();
+    v->poff = c->poff.data();
+    v->expo = c->expo.data();
+    v->coef = c->coef.data();
+    return TCG_OK;
+}
+
+int ctx_shells(tcg_ctx *c, std::vector<int> &first_ao, std::vector<int> &ncomp, std::vector<int> &cell) {
+    if (!c || c->nbas == 0) return TCG_E_STATE;
+    first_ao.clear();
+    ncomp.clear();
+    cell.clear();
+    for (const Shell &s : c->sh) {
+        first_ao.push_back(s.first_ao);
+        ncomp.push_back(s.ncomp);
+        cell.push_back(s.cell);
+    }
+    return TCG_OK;
+}
+}  // namespace tcg
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1009,7 +1047,11 @@ int tcg_set_basis(tcg_ctx *c, int nbas, int nfuc, const double *center, const in
     c->prepared = false;
     c->direct_ready = false;
     for (int p = 0; p < np; ++p)
-        if (!(expo[p] > 0.0)) return c->fail(TCG_E_ARG, "exponent %d is not positive", p);
+        if (!(expo[p] >= 0.0) || !std::isfinite(expo[p])) return c->fail(TCG_E_ARG, "exponent %d is negative", p);
+    for (int i = 0; i < nbas; ++i)
+        for (int p = prim_off[i]; p < prim_off[i + 1]; ++p)
+            if (expo[p] == 0.0 && (lxyz[3 * i] + lxyz[3 * i + 1] + lxyz[3 * i + 2]) != 0)
+                return c->fail(TCG_E_ARG, "exponent 0 (the constant function) only for s functions");
     return build_shells(c);
 }
 
