@@ -21,6 +21,14 @@ FP64 flops (counted by the kernel itself, per row 2 sum_k min(nm, t_nmax(k))
 + 10 (nm+1)) over its duration, measured by HIP events around its launch on
 the engine's stream, against MI355X's FP64 matrix peak.  ``cpu_baseline`` times the
 reference's own Fortran ERI driver (oracle/_ref, OpenMP) on this host's cores.
+
+Beside the timed ERI steps the line reports the energy path on the same
+workload: RHF (k-space, J/K from the resident ERI) and k-point MP2 wall times;
+``learned`` repeats ERI + RHF + MP2 with the learned per-quartet orders (build
+time, energy differences to the plain path, speed-ups); ``df_mp2`` runs the
+density-fitted path (41 even-tempered auxiliary functions per H): df_init,
+DF-RHF, DF-kMP2, and the DF correlation energy on the exact orbitals against the
+exact k-MP2.
 """
 from __future__ import annotations
 
@@ -91,6 +99,73 @@ def cpu_baseline(sysm, quad, seconds: float):
             "seconds_per_build": el / n}
 
 
+def _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total, ms_step_plain):
+    """The same ERI + RHF + MP2 with learned per-quartet orders (the committed
+    network): build time, energies against the plain path, speed-up."""
+    from william_ci_amd import learned, scf
+    model = learned.Model.load()
+    model.install(eng)
+    for _ in range(args.warmup):
+        eng.refresh_basis(coef)
+        learned.eri_build_learned(eng, model, to_host=False)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.refresh_basis(coef)
+        _, counts = learned.eri_build_learned(eng, model, to_host=False)
+    step = (time.perf_counter() - t0) / args.steps
+    tm = eng.times()
+    t1 = time.perf_counter()
+    rl = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
+    t2 = time.perf_counter()
+    ml = eng.kmp2(rl.mo_coef, rl.mo_energies, rl.occupied)
+    t3 = time.perf_counter()
+    total = step + (t2 - t1) + (t3 - t2)
+    return {"orders": list(model.orders), "margin": model.margin,
+            "shell_quartets_per_rule": {"plain": int(counts[-1]),
+                                        **{str(o): int(c) for o, c in zip(model.orders, counts[:-1])}},
+            "ms_per_step": step * 1e3, "rows_kernel_ms": tm["rows_kernel_ms"],
+            "speedup_eri": ms_step_plain / step,
+            "rhf_energy": rl.energy, "rhf_iterations": rl.iterations, "mp2_ec": ml["ec"],
+            "dE_rhf": rl.energy - res.energy, "dE_total": (rl.energy + ml["ec"]) - (res.energy + mp2["ec"]),
+            "eri_rhf_mp2_wall_s": total, "speedup_eri_rhf_mp2": plain_total / total}
+
+
+def _df_leg(s, eng, S, T, V, res, mp2):
+    """Density-fitted MP2 on the same chain (keyword density_fitting with the
+    41-function even-tempered AuxBasis): df_init, DF-RHF, DF-kMP2; and DF-kMP2
+    on the exact-integral orbitals against the exact k-MP2."""
+    from william_ci_amd import df, scf
+    aux = df.aux_list(s, df.read_aux(df.AUX_H_ETB41))
+    t0 = time.perf_counter()
+    d = df.DF(eng, aux)
+    t1 = time.perf_counter()
+    njk, tjk = 0, 0.0
+
+    def jk(P):
+        nonlocal njk, tjk
+        ta = time.perf_counter()
+        out = d.jk(P)
+        tjk += time.perf_counter() - ta
+        njk += 1
+        return out
+    rd = scf.rhf_sao(s, S, T, V, jk)
+    t2 = time.perf_counter()
+    md = d.kmp2(rd.mo_coef, rd.mo_energies, rd.occupied)
+    t3 = time.perf_counter()
+    on_exact = d.kmp2(res.mo_coef, res.mo_energies, res.occupied)
+    info, times = d.info(), d.times()
+    d.close()
+    return {"aux": "AuxBasis_H_etb41 (s 8: 0.1-40, p 5: 0.2-10, d 3: 0.3-3; 41 per H)",
+            "nau": info["nau"], "naux": info["naux"], "dropped": info["dropped"],
+            "df_init_s": t1 - t0, "df_init_ms": {k: round(v, 3) for k, v in times.items()},
+            "rhf_energy": rd.energy, "rhf_iterations": rd.iterations, "rhf_wall_s": t2 - t1,
+            "df_jk_ms": tjk / max(njk, 1) * 1e3,
+            "kmp2_ec": md["ec"], "kmp2_gpu_ms": md["gpu_ms"], "kmp2_wall_s": t3 - t2,
+            "e_total": rd.energy + md["ec"], "df_init_rhf_mp2_wall_s": t3 - t0,
+            "ec_on_exact_orbitals": on_exact["ec"], "ec_exact": mp2["ec"],
+            "fit_error_ec": on_exact["ec"] - mp2["ec"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +176,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-self-check", action="store_true",
                     help="keep the 32-point fast rule (skip the start-up fast_quad self-check)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-order leg")
+    ap.add_argument("--no-df", action="store_true", help="skip the density-fitted MP2 leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -151,9 +228,11 @@ def main():
     # the energy path on the same workload: one-electron integrals + RHF (Bloch
     # basis, J/K from the resident ERI on the GPU); the ERI build is the last step's
     from william_ci_amd import scf
-    t_rhf = time.perf_counter()
+    t_one = time.perf_counter()
     S, T, V = eng.one_electron()
-    t_one = time.perf_counter() - t_rhf
+    t_one = time.perf_counter() - t_one
+    scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P), max_scf=2)      # untimed: first-call costs of the host loop
+    t_rhf = time.perf_counter()
     res = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
     rhf_wall = time.perf_counter() - t_rhf
     # MP2 on the RHF orbitals (SAO run: k-point MP2 from the resident ERI,
@@ -161,6 +240,11 @@ def main():
     t_mp2 = time.perf_counter()
     mp2 = eng.kmp2(res.mo_coef, res.mo_energies, res.occupied)
     mp2_wall = time.perf_counter() - t_mp2
+    plain_total = ms_step_plain = el / args.steps
+    plain_total += rhf_wall + mp2_wall
+    learned_part = None if args.no_learned else _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total,
+                                                             ms_step_plain)
+    df_part = None if args.no_df else _df_leg(s, eng, S, T, V, res, mp2)
     eng.close()
 
     nb = s.nbas
@@ -188,7 +272,7 @@ def main():
                       "rows": st["rows"]},
         "step_ms": {k: round(v, 3) for k, v in tm.items()},
         "rhf": {"energy": res.energy, "converged": res.converged, "iterations": res.iterations,
-                "orbital_error": res.orbital_error, "wall_s_after_eri": rhf_wall, "one_electron_s": t_one,
+                "orbital_error": res.orbital_error, "wall_s": rhf_wall, "one_electron_s": t_one,
                 "jk_s": res.times["jk_s"], "eri_plus_rhf_wall_s": rhf_wall + ms_per_step / 1e3},
         "mp2": {"kind": "k-point MP2 (transform_ERI_AO_to_kMO + kMP2_calc)", "ec": mp2["ec"],
                 "e_total": res.energy + mp2["ec"], "zero_denominators": mp2["zero_denominators"],
@@ -199,6 +283,8 @@ def main():
                      "kernel": "k_rows_mfma", "kernel_ms": kernel_ms, "flops_per_launch": st["row_flops"],
                      "all_quartet_flops": st["flops"]},
     }
+    line["learned"] = learned_part
+    line["df_mp2"] = df_part
     if not args.no_cpu_baseline:
         cb = cpu_baseline(s, quad, args.cpu_seconds)
         cb["gpu_speedup"] = line["value"] / cb["value"]

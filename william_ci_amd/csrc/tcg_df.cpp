@@ -12,8 +12,8 @@
 //   pivoted Cholesky per k (tolerance 1e-10) gives f(k) = (L^-1)^H, and
 //   B(R') = 1/N Re sum_k [sum_RP T3(RP) e^{ik RP}] f(k) e^{-ik R'}
 //   (df_toroidal.f90:570-700); every product is an FP64 matrix-core GEMM.
-// * df_JK (:850-957): J through gamma_Q; K in real space,
-//   K(mu0,si) = -1/2 sum_Q sum_{nu,la} B^Q_{mu0 nu} P(nu,la) B^Q_{la si}.
+// * df_JK (:850-957): J through gamma_Q; K in k space,
+//   Kt(k) = -1/(2N) sum_{k1,Q} Bh_Q(k1, k-k1) P(k1) Bh_Q(k1, k-k1)^H, back to cells.
 // * df_kMO (:981-1112): L^Q_pq(k1,k2) and (ov|ov) = 1/N sum_Q L L, then the
 //   k-MP2 sums of tcg_kmp2.
 #include <hip/hip_runtime.h>
@@ -32,12 +32,15 @@ using namespace tcg;
 namespace tcg {
 hipError_t launch_metric_factor(const double *Vre, const double *Vim, int n, int nk, const int *mate, double tol,
                                 double *fre, double *fim, int *rank, double *dmin, hipStream_t st);
-hipError_t launch_df_jk(const double *B, const double *Ps, const double *X, int nf, int nb, int nau, int N,
-                        const int *cdiff, double *G, const double *gam, double *J, double *K, int what, hipStream_t st);
+hipError_t launch_df_gamma(const double *B, const double *Ps, int nf, int nb, int nau, int N, double *G,
+                           hipStream_t st);
+hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *J, hipStream_t st);
 hipError_t launch_df_l(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,
                        int v_lo, const int *cdiff, double2 *L, hipStream_t st);
 hipError_t launch_df_eri(const double2 *L, int nau, int N, int o_hi, int nv, const int *cdiff, const int *cadd,
                          double2 *out, hipStream_t st);
+hipError_t launch_df_kt(const double *Fr, const double *Fi, const double2 *Pk, int nf, int nau, int N, const int *cdiff,
+                        double2 *Y, double2 *Kt, hipStream_t st);
 }  // namespace tcg
 
 namespace {
@@ -76,6 +79,10 @@ struct tcg_df {
     std::vector<int> cdiff, cadd;
     DBuf<int> d_cdiff, d_cadd;
     DBuf<double> d_B;                    // df_B(nfuc, nBas, naux), column-major
+    // F(a, b0, k, Q, kap) = sum_{Rb, RQ} df_B(a, b0 + Rb, Q RQ) e^{ik Rb} e^{-i kap RQ} (real, imaginary)
+    DBuf<double> d_Fr, d_Fi;
+    bool f_ready = false;
+    DBuf<double2> d_Pk, d_Y, d_Kt;       // exchange work arrays (kept between SCF iterations)
     std::vector<int> rank;
     double dmin = 0.0;
     double times[4] = {0, 0, 0, 0};
@@ -122,6 +129,35 @@ static hipError_t zgemm_split(int M, int Nn, int K, const double *Ar, const doub
     if (e == hipSuccess) e = launch_dgemm_nn(M, Nn, K, sb, Ar, lda, sA, Bi, ldb, sB, 0.0, Ci, ldc, sC, batch, st);
     if (e == hipSuccess) e = launch_dgemm_nn(M, Nn, K, 1.0, Ai, lda, sA, Br, ldb, sB, 1.0, Ci, ldc, sC, batch, st);
     return e;
+}
+
+// F of df_L_kap (df_toroidal.f90:1071-1090) for every kap at once, kept on the GPU: the
+// exchange (Bh(k1,k2) = F(k1, kap = -k2), :898-953) and the MP2 L vectors read it.
+static int build_F(tcg_df *df) {
+    if (df->f_ready) return TCG_OK;
+    const int nf = df->nf, N = df->N, nau = df->nau, naux = nau * N;
+    std::vector<double> cphi, sphi;
+    phase_tables(df->ncl, N, 1.0, cphi, sphi);
+    DBuf<double> dC, dS, Y1r, Y1i;
+    DCHK(df, dC.put(cphi));
+    DCHK(df, dS.put(sphi));
+    // df_B viewed (a, b0, Rb, Q, RQ): first Rb -> k, batched over (Q, RQ)
+    const size_t nff = (size_t)nf * nf, nY = nff * N * naux;
+    DCHK(df, Y1r.alloc(nY));
+    DCHK(df, Y1i.alloc(nY));
+    DCHK(df, launch_dgemm_nn((int)nff, N, N, 1.0, df->d_B.p, (long)nff, (long long)(nff * N), dC.p, N, 0, 0.0, Y1r.p,
+                             (long)nff, (long long)(nff * N), naux, df->st));
+    DCHK(df, launch_dgemm_nn((int)nff, N, N, 1.0, df->d_B.p, (long)nff, (long long)(nff * N), dS.p, N, 0, 0.0, Y1i.p,
+                             (long)nff, (long long)(nff * N), naux, df->st));
+    // then RQ -> kap with e^{-i kap RQ}: rows (a, b0, k, Q), columns RQ
+    DCHK(df, df->d_Fr.alloc(nY));
+    DCHK(df, df->d_Fi.alloc(nY));
+    const long rows = (long)(nff * N * nau);
+    DCHK(df, zgemm_split((int)rows, N, N, Y1r.p, Y1i.p, rows, 0, dC.p, dS.p, N, 0, df->d_Fr.p, df->d_Fi.p, rows, 0, 1,
+                         true, df->st));
+    DCHK(df, hipStreamSynchronize(df->st));
+    df->f_ready = true;
+    return TCG_OK;
 }
 
 extern "C" {
@@ -393,10 +429,10 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K) {
             for (int l0 = 0; l0 < nf; ++l0)
                 Ps[(size_t)l0 + (size_t)nf * s + nx * R] = P[(size_t)(R * nf + l0) + (size_t)nb * col];
         }
-    DBuf<double> dPs, dG, dgam, dJ, dK, dP, dX;
+    DBuf<double> dPs, dG, dgam, dJ;
     DCHK(df, dPs.put(Ps));
     DCHK(df, dG.alloc((size_t)naux * N));
-    DCHK(df, launch_df_jk(df->d_B.p, dPs.p, nullptr, nf, nb, nau, N, nullptr, dG.p, nullptr, nullptr, nullptr, 0, df->st));
+    DCHK(df, launch_df_gamma(df->d_B.p, dPs.p, nf, nb, nau, N, dG.p, df->st));
     std::vector<double> G((size_t)naux * N), gam(naux, 0.0);
     DCHK(df, hipMemcpyAsync(G.data(), dG.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, df->st));
     DCHK(df, hipStreamSynchronize(df->st));
@@ -407,34 +443,56 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K) {
                 gam[(size_t)df->cadd[(size_t)rq * N + R] * nau + q] += G[(size_t)rq * nau + q + (size_t)naux * R];
     DCHK(df, dgam.put(gam));
     DCHK(df, dJ.alloc(nx));
-    DCHK(df, launch_df_jk(df->d_B.p, nullptr, nullptr, nf, nb, nau, N, nullptr, nullptr, dgam.p, dJ.p, nullptr, 1, df->st));
-    // K: X_Q = B_Q P (batched over Q), then the translated contraction.  P enters through its
-    // first nfuc rows, translated: P(nu0 + R, la) = P(nu0, la - R), as the k-space exchange of
-    // the reference reads it (df_toroidal.f90:898-903).
-    std::vector<double> Pv((size_t)nb * nb);
-    for (int la = 0; la < nb; ++la)
-        for (int nu = 0; nu < nb; ++nu) {
-            const int rn = nu / nf, rl = la / nf;
-            Pv[(size_t)nu + (size_t)nb * la] = P[(size_t)(nu % nf) + (size_t)nb * (df->cdiff[(size_t)rl * N + rn] * nf + la % nf)];
+    DCHK(df, launch_df_jvec(df->d_B.p, dgam.p, nf, nb, naux, dJ.p, df->st));
+    // K in k space (df_toroidal.f90:893-954): P(k) = sum_R P(0, R) e^{ikR} (its Hermitian part),
+    // Kt(kk) = -1/(2N) sum_{k1,Q} M P(k1) M^H with M = Bh_Q(k1, kk - k1) = F(k1, Q, kap = k1 - kk),
+    // K(mu0, s0 + R) = 1/N Re sum_kk Kt(kk) e^{-i kk R}.  Only the first nfuc rows of P enter.
+    int rc = build_F(df);
+    if (rc) return rc;
+    std::vector<double> cphi, sphi;
+    phase_tables(df->ncl, N, 1.0, cphi, sphi);
+    const size_t nff = (size_t)nf * nf;
+    std::vector<double2> Pk(nff * N, double2{0.0, 0.0}), Kt(nff * N);
+    for (int k = 0; k < N; ++k) {
+        double2 *pk = Pk.data() + nff * k;
+        for (int R = 0; R < N; ++R) {
+            const double c = cphi[R + (size_t)N * k], sn = sphi[R + (size_t)N * k];
+            for (int b = 0; b < nf; ++b)
+                for (int a = 0; a < nf; ++a) {
+                    const double p = P[(size_t)a + (size_t)nb * (R * nf + b)];
+                    pk[a + nf * b].x += p * c;
+                    pk[a + nf * b].y += p * sn;
+                }
         }
-    DCHK(df, dP.put(Pv));
-    DCHK(df, dX.alloc(nx * naux));
-    DCHK(df, launch_dgemm_nn(nf, nb, nb, 1.0, df->d_B.p, nf, (long long)nx, dP.p, nb, 0, 0.0, dX.p, nf, (long long)nx,
-                             naux, df->st));
-    DCHK(df, dK.alloc(nx));
-    DCHK(df, launch_df_jk(df->d_B.p, nullptr, dX.p, nf, nb, nau, N, df->d_cdiff.p, nullptr, nullptr, nullptr, dK.p, 2,
-                          df->st));
-    std::vector<double> Jr(nx), Kr(nx);
+        for (int b = 0; b < nf; ++b)
+            for (int a = 0; a <= b; ++a) {
+                const double2 x = pk[a + nf * b], y = pk[b + nf * a];
+                pk[a + nf * b] = double2{0.5 * (x.x + y.x), 0.5 * (x.y - y.y)};
+                pk[b + nf * a] = double2{pk[a + nf * b].x, -pk[a + nf * b].y};
+            }
+    }
+    DCHK(df, df->d_Pk.put(Pk));
+    DCHK(df, df->d_Y.alloc(nff * nau * N * N));
+    DCHK(df, df->d_Kt.alloc(nff * N));
+    DCHK(df, launch_df_kt(df->d_Fr.p, df->d_Fi.p, df->d_Pk.p, nf, nau, N, df->d_cdiff.p, df->d_Y.p, df->d_Kt.p, df->st));
+    DCHK(df, hipMemcpyAsync(Kt.data(), df->d_Kt.p, sizeof(double2) * Kt.size(), hipMemcpyDeviceToHost, df->st));
+    std::vector<double> Jr(nx);
     DCHK(df, hipMemcpyAsync(Jr.data(), dJ.p, sizeof(double) * nx, hipMemcpyDeviceToHost, df->st));
-    DCHK(df, hipMemcpyAsync(Kr.data(), dK.p, sizeof(double) * nx, hipMemcpyDeviceToHost, df->st));
     DCHK(df, hipStreamSynchronize(df->st));
     std::fill(J, J + (size_t)nb * nb, 0.0);
     std::fill(K, K + (size_t)nb * nb, 0.0);
     for (int c = 0; c < nb; ++c)
-        for (int r = 0; r < nf; ++r) {
-            J[r + (size_t)nb * c] = Jr[r + (size_t)nf * c];
-            K[r + (size_t)nb * c] = Kr[r + (size_t)nf * c];
-        }
+        for (int r = 0; r < nf; ++r) J[r + (size_t)nb * c] = Jr[r + (size_t)nf * c];
+    for (int R = 0; R < N; ++R)
+        for (int b = 0; b < nf; ++b)
+            for (int a = 0; a < nf; ++a) {
+                double v = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    const double2 kt = Kt[nff * k + a + nf * b];
+                    v += kt.x * cphi[R + (size_t)N * k] + kt.y * sphi[R + (size_t)N * k];
+                }
+                K[a + (size_t)nb * (R * nf + b)] = v / N;
+            }
     return TCG_OK;
 }
 
@@ -466,33 +524,13 @@ int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, 
     }
     const int v_lo = omin, nv = nf - v_lo;
     if (o_hi == 0 || nv <= 0) return df->fail(TCG_E_ARG, "tcg_df_kmp2: no occupied or no virtual bands");
-    std::vector<double> cphi, sphi;
-    phase_tables(df->ncl, N, 1.0, cphi, sphi);
-    DBuf<double> dC, dS, Y1r, Y1i, Fr, Fi;
-    DCHK(df, dC.put(cphi));
-    DCHK(df, dS.put(sphi));
-    // F(a, b0, k, Q, kap) = sum_{Rb, RQ} B(a, b0 + Rb, (Q, RQ)) e^{ik Rb} e^{-i kap RQ}
-    // df_B viewed (a, b0, Rb, Q, RQ): first Rb -> k, batched over (Q, RQ)
-    const size_t nff = (size_t)nf * nf, nY = nff * N * naux;
-    DCHK(df, Y1r.alloc(nY));
-    DCHK(df, Y1i.alloc(nY));
-    DCHK(df, launch_dgemm_nn((int)nff, N, N, 1.0, df->d_B.p, (long)nff, (long long)(nff * N), dC.p, N, 0, 0.0, Y1r.p,
-                             (long)nff, (long long)(nff * N), naux, df->st));
-    DCHK(df, launch_dgemm_nn((int)nff, N, N, 1.0, df->d_B.p, (long)nff, (long long)(nff * N), dS.p, N, 0, 0.0, Y1i.p,
-                             (long)nff, (long long)(nff * N), naux, df->st));
-    // then RQ -> kap with e^{-i kap RQ}: rows (a, b0, k, Q), columns RQ
-    DCHK(df, Fr.alloc(nY));
-    DCHK(df, Fi.alloc(nY));
-    const long rows = (long)(nff * N * nau);
-    DCHK(df, zgemm_split((int)rows, N, N, Y1r.p, Y1i.p, rows, 0, dC.p, dS.p, N, 0, Fr.p, Fi.p, rows, 0, 1, true,
-                         df->st));
-    Y1r.release();
-    Y1i.release();
+    int rc = build_F(df);
+    if (rc) return rc;
     DBuf<double2> dCk, dL, dE;
     DCHK(df, dCk.put(Ck));
     const size_t nL = (size_t)o_hi * nv * nau * N * N;
     DCHK(df, dL.alloc(nL));
-    DCHK(df, launch_df_l(Fr.p, Fi.p, dCk.p, nf, nau, N, o_hi, v_lo, df->d_cdiff.p, dL.p, df->st));
+    DCHK(df, launch_df_l(df->d_Fr.p, df->d_Fi.p, dCk.p, nf, nau, N, o_hi, v_lo, df->d_cdiff.p, dL.p, df->st));
     const size_t nE = (size_t)o_hi * nv * o_hi * nv * N * N * N;
     DCHK(df, dE.alloc(nE));
     DCHK(df, launch_df_eri(dL.p, nau, N, o_hi, nv, df->d_cdiff.p, df->d_cadd.p, dE.p, df->st));

@@ -11,8 +11,8 @@
 //                    factor of the fitted integrals (W(-k) = conj W(k)).
 //   k_df_gamma       G(Q, R) = sum_x B(x, Q) Ps(x, R)  (x = (mu0, nu) pairs)
 //   k_df_jvec        J(x) = sum_Q B(x, Q) gamma(Q)
-//   k_df_k           K(mu0, si) = -1/2 sum_{la, Q} X(mu0, la, Q) B(la0, si - R_la, Q - R_la),
-//                    X(mu0, la, Q) = sum_nu B(mu0, nu, Q) P(nu, la)
+//   k_df_kt_y/_k     the exchange in k space (df_toroidal.f90:893-954):
+//                    Kt(kk) = -1/(2N) sum_{k1,Q} M P(k1) M^H,  M = Bh_Q(k1, kk - k1)
 //   k_df_l           L(p, q, Q, k1, k2) = sum_ab conj C_k1(a, p) C_k2(b, q) F(a, b, k2, Q, k2 - k1)
 //   k_df_eri         ERI(p,q,r,s,k1,k2,k3) = 1/N sum_Q L(p,q,Q,k1,k2) L(r,s,Q,k3,k4)
 // Cells and k points are numbered x fastest; cdiff[c1 * N + c2] = c1 - c2.
@@ -202,46 +202,83 @@ __global__ __launch_bounds__(256) void k_df_jvec(const double *__restrict__ B, c
     J[x] = s;
 }
 
-// K(mu0, si): one block per (mu0, si), threads over (la, q, RQ)
-__global__ __launch_bounds__(256) void k_df_k(const double *__restrict__ X, const double *__restrict__ B, int nf,
-                                              int nb, int nau, int N, const int *__restrict__ cdiff,
-                                              double *__restrict__ K) {
-    __shared__ double red[256];
-    const int mu = blockIdx.x % nf, si = blockIdx.x / nf;
-    const int csi = si / nf, lsi = si % nf;
-    const int naux = nau * N;
-    const size_t total = (size_t)nb * naux;
-    double s = 0.0;
-    for (size_t w = threadIdx.x; w < total; w += 256) {
-        const int la = (int)(w % nb), Q = (int)(w / nb);
-        const int R = la / nf, la0 = la % nf;
-        const int q = Q % nau, RQ = Q / nau;
-        // B^Q(la, si) = B(la0, si - R, (q, RQ - R))
-        const int sis = cdiff[csi * N + R] * nf + lsi;
-        const int Qs = q + nau * cdiff[RQ * N + R];
-        s += X[mu + (size_t)nf * (la + (size_t)nb * Q)] * B[la0 + (size_t)nf * (sis + (size_t)nb * Qs)];
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) K[mu + (size_t)nf * si] = -0.5 * red[0];
+hipError_t launch_df_gamma(const double *B, const double *Ps, int nf, int nb, int nau, int N, double *G,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_df_gamma, dim3(nau * N), dim3(256), 0, st, B, Ps, (long)nf * nb, nau * N, N, G);
+    return hipGetLastError();
 }
 
-hipError_t launch_df_jk(const double *B, const double *Ps, const double *X, int nf, int nb, int nau, int N,
-                        const int *cdiff, double *G, const double *gam, double *J, double *K, int what,
-                        hipStream_t st) {
+hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *J, hipStream_t st) {
     const long nx = (long)nf * nb;
-    const int naux = nau * N;
-    if (what == 0) {
-        hipLaunchKernelGGL(k_df_gamma, dim3(naux), dim3(256), 0, st, B, Ps, nx, naux, N, G);
-    } else if (what == 1) {
-        hipLaunchKernelGGL(k_df_jvec, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, B, gam, nx, naux, J);
-    } else {
-        hipLaunchKernelGGL(k_df_k, dim3(nf * nb), dim3(256), 0, st, X, B, nf, nb, nau, N, cdiff, K);
+    hipLaunchKernelGGL(k_df_jvec, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, B, gam, nx, naux, J);
+    return hipGetLastError();
+}
+
+// Y(mu, nu', Q, k1, kk) = sum_nu M(mu, nu) P(k1)(nu, nu'), M(mu, nu) = F(mu, nu, k1, Q, kap = k1 - kk)
+__global__ __launch_bounds__(256) void k_df_kt_y(const double *__restrict__ Fr, const double *__restrict__ Fi,
+                                                 const double2 *__restrict__ Pk, int nf, int nau, int N,
+                                                 const int *__restrict__ cdiff, double2 *__restrict__ Y) {
+    const size_t nff = (size_t)nf * nf;
+    const size_t total = nff * nau * N * N;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        size_t r = t;
+        const int mu = (int)(r % nf); r /= nf;
+        const int nu2 = (int)(r % nf); r /= nf;
+        const int Q = (int)(r % nau); r /= nau;
+        const int k1 = (int)(r % N);
+        const int kk = (int)(r / N);
+        const int kap = cdiff[k1 * N + kk];
+        const size_t fb = (size_t)mu + nff * (k1 + (size_t)N * (Q + (size_t)nau * kap));
+        const double2 *P1 = Pk + nff * k1 + (size_t)nf * nu2;
+        double2 acc = {0.0, 0.0};
+        for (int nu = 0; nu < nf; ++nu) {
+            const double2 u = zmul(double2{Fr[fb + (size_t)nf * nu], Fi[fb + (size_t)nf * nu]}, P1[nu]);
+            acc.x += u.x;
+            acc.y += u.y;
+        }
+        Y[t] = acc;
     }
+}
+
+// Kt(kk)(mu, mu') = scale sum_{k1, Q, nu'} Y(mu, nu', Q, k1, kk) conj M(mu', nu'); one block per (mu, mu', kk)
+__global__ __launch_bounds__(256) void k_df_kt_k(const double *__restrict__ Fr, const double *__restrict__ Fi,
+                                                 const double2 *__restrict__ Y, int nf, int nau, int N,
+                                                 const int *__restrict__ cdiff, double scale,
+                                                 double2 *__restrict__ Kt) {
+    __shared__ double2 red[256];
+    const size_t nff = (size_t)nf * nf;
+    const int mu = blockIdx.x % nf, mu2 = (blockIdx.x / nf) % nf, kk = blockIdx.x / (nf * nf);
+    const int terms = N * nau * nf;
+    double2 acc = {0.0, 0.0};
+    for (int w = threadIdx.x; w < terms; w += blockDim.x) {
+        const int nu2 = w % nf, Q = (w / nf) % nau, k1 = w / (nf * nau);
+        const int kap = cdiff[k1 * N + kk];
+        const size_t fi = (size_t)mu2 + nf * (nu2 + nf * (k1 + (size_t)N * (Q + (size_t)nau * kap)));
+        const double2 y = Y[(size_t)mu + nf * (nu2 + nf * (Q + (size_t)nau * (k1 + (size_t)N * kk)))];
+        const double2 u = zmulc(y, double2{Fr[fi], Fi[fi]});
+        acc.x += u.x;
+        acc.y += u.y;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[threadIdx.x].x += red[threadIdx.x + w].x;
+            red[threadIdx.x].y += red[threadIdx.x + w].y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Kt[nff * kk + mu + (size_t)nf * mu2] = double2{scale * red[0].x, scale * red[0].y};
+}
+
+hipError_t launch_df_kt(const double *Fr, const double *Fi, const double2 *Pk, int nf, int nau, int N, const int *cdiff,
+                        double2 *Y, double2 *Kt, hipStream_t st) {
+    const size_t total = (size_t)nf * nf * nau * N * N;
+    const int grid = (int)std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_df_kt_y, dim3(grid), dim3(256), 0, st, Fr, Fi, Pk, nf, nau, N, cdiff, Y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_df_kt_k, dim3(nf * nf * N), dim3(256), 0, st, Fr, Fi, Y, nf, nau, N, cdiff, -0.5 / N, Kt);
     return hipGetLastError();
 }
 
