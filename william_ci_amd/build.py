@@ -14,6 +14,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed FP32 instructions stay off (they are not used by this FP64 code either)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # matrix-core accumulators in ordinary VGPRs (no AGPR copies around every MFMA)
+         "-Xarch_device", "-mllvm=-amdgpu-mfma-vgpr-form=1",
          "-I", os.path.join(HERE, "..", "include"), "-pthread"]
 
 

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
                                                   const int *__restrict__ order, long long nrows, int hbase,
                                                   double *__restrict__ rowval) {
     const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
-    const int nqp = A.nqp, nh = kNT, nht = A.nqp / kNodeTile;
+    const int nqp = A.nqp, nht = A.nqp / kNodeTile;
     const int tabrows = A.ncap + 5;
     const long long ntiles = (nrows + 15) / 16;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -208,48 +208,54 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
             const int nmd = (d == dd) ? nm : -1;      // rows of another combination contribute 0
             const int tr = (dd >> 8) * A.ndp + (dd & 0xff);   // table block of this rule and direction
             const int nmax_t = wmax(nmd);
-            int lim[kNT];
-            int nend = -1;
+            // node tiles as slots sorted by their last order, descending (uniform values);
+            // padding tiles of a rule with fewer nodes have lim -1 and never run
+            int sl[kNT], sp[kNT];
 #pragma unroll
             for (int h = 0; h < kNT; ++h) {
-                lim[h] = -1;
-                if (h < nh) {
-                    const int lh = A.tab_lim[tr * nht + hbase + h];
-                    lim[h] = lh < nmax_t ? lh : nmax_t;
-                    nend = lim[h] > nend ? lim[h] : nend;
-                }
+                const int lh = A.tab_lim[tr * nht + hbase + h];
+                sl[h] = lh < nmax_t ? lh : nmax_t;
+                sp[h] = h;
             }
+#pragma unroll
+            for (int i = 1; i < kNT; ++i)
+#pragma unroll
+                for (int j = i; j > 0; --j) {
+                    const bool sw = sl[j - 1] < sl[j];
+                    const int a = sl[j - 1], b = sl[j], pa = sp[j - 1], pb = sp[j];
+                    sl[j - 1] = sw ? b : a;
+                    sl[j] = sw ? a : b;
+                    sp[j - 1] = sw ? pb : pa;
+                    sp[j] = sw ? pa : pb;
+                }
             const double *tab = A.tab + (size_t)tr * tabrows * nqp + hbase * kNodeTile + r;
+            int toff[kNT];
+#pragma unroll
+            for (int j = 0; j < kNT; ++j) toff[j] = sp[j] * kNodeTile;
             dbl4 acc[kNT];
 #pragma unroll
-            for (int h = 0; h < kNT; ++h) acc[h] = dbl4{0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < kNT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
             double cn = cn0, sn = sn0;
-            // loads two steps ahead of their use
-            double2 g0 = {0.0, 0.0}, h0 = {0.0, 0.0}, g1 = {0.0, 0.0}, h1 = {0.0, 0.0};
-            double b0[kNT], b1[kNT];
+            // Two register sets, each consumed by one step and refilled for the step after
+            // next (loads two steps ahead, no register copies).  Rows past nm and orders past
+            // the table read in-bounds entries (order 0 of the row, the zero rows at the
+            // table's end) whose products the selection of F or the zero table make vanish.
+            const int tlast = tabrows - 1;
+            auto gidx = [&](int n) { return n <= nmd ? n : 0; };
+            auto tidx = [&](int n) { return (size_t)(n < tlast ? n : tlast) * nqp; };
+            double2 gP = Gp[gidx(kq)], hP = Hp[gidx(kq)];
+            double2 gQ = Gp[gidx(4 + kq)], hQ = Hp[gidx(4 + kq)];
+            double bP[kNT], bQ[kNT];
 #pragma unroll
-            for (int h = 0; h < kNT; ++h) { b0[h] = 0.0; b1[h] = 0.0; }
-            if (kq <= nmd) { g0 = Gp[kq]; h0 = Hp[kq]; }
-            if (4 + kq <= nmd) { g1 = Gp[4 + kq]; h1 = Hp[4 + kq]; }
-#pragma unroll
-            for (int h = 0; h < kNT; ++h) {
-                if (h < nh && 0 <= lim[h]) b0[h] = tab[(size_t)kq * nqp + h * kNodeTile];
-                if (h < nh && 4 <= lim[h]) b1[h] = tab[(size_t)(4 + kq) * nqp + h * kNodeTile];
+            for (int j = 0; j < kNT; ++j) {
+                bP[j] = tab[tidx(kq) + toff[j]];
+                bQ[j] = tab[tidx(4 + kq) + toff[j]];
             }
-#pragma unroll 1
-            for (int n0 = 0; n0 <= nend; n0 += 4) {
-                const int n = n0 + kq;
-                const int n2 = n + 8;
-                double2 g2 = {0.0, 0.0}, h2 = {0.0, 0.0};
-                double b2[kNT];
-                if (n2 <= nmd) { g2 = Gp[n2]; h2 = Hp[n2]; }
-#pragma unroll
-                for (int h = 0; h < kNT; ++h) {
-                    b2[h] = 0.0;
-                    if (h < nh && n0 + 8 <= lim[h]) b2[h] = tab[(size_t)n2 * nqp + h * kNodeTile];
-                }
-                if (n0 > 0) {
-                    if ((n0 & 63) == 0) {
+            // one step of 4 orders m..m+3 (lane order m + kq) with the set (g, h, b); c slots active
+            auto step = [&](int m, int c, double2 &g, double2 &hh, double *b) {
+                const int n = m + kq;
+                if (m > 0) {
+                    if ((m & 63) == 0) {
                         sincos((double)n * phi, &sn, &cn);
                     } else {
                         const double tc = cn * c4 - sn * s4;
@@ -257,15 +263,30 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
                         cn = tc;
                     }
                 }
-                const double re = g0.x * h0.x + g0.y * h0.y;   // conj(g) * h
-                const double im = g0.x * h0.y - g0.y * h0.x;
+                const double re = g.x * hh.x + g.y * hh.y;   // conj(g) * h
+                const double im = g.x * hh.y - g.y * hh.x;
                 const double F = (n <= nmd) ? re * cn - im * sn : 0.0;
 #pragma unroll
-                for (int h = 0; h < kNT; ++h)
-                    if (h < nh && n0 <= lim[h]) acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, b0[h], acc[h], 0, 0, 0);
-                g0 = g1; h0 = h1; g1 = g2; h1 = h2;
+                for (int j = 0; j < kNT; ++j)
+                    if (j < c) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, b[j], acc[j], 0, 0, 0);
+                const int n2 = n + 8;
+                g = Gp[gidx(n2)];
+                hh = Hp[gidx(n2)];
+                const size_t t2 = tidx(n2);
 #pragma unroll
-                for (int h = 0; h < kNT; ++h) { b0[h] = b1[h]; b1[h] = b2[h]; }
+                for (int j = 0; j < kNT; ++j) b[j] = j < c ? tab[t2 + toff[j]] : 0.0;
+            };
+            // stage c: the c slots with the largest limits are active, up to order sl[c-1]
+            // (steps in pairs: one extra step past a limit only adds zero products)
+            int n0 = 0;
+#pragma unroll
+            for (int c = kNT; c >= 1; --c) {
+                const int lend = sl[c - 1];
+#pragma unroll 1
+                for (; n0 <= lend; n0 += 8) {
+                    step(n0, c, gP, hP, bP);
+                    step(n0 + 4, c, gQ, hQ, bQ);
+                }
             }
             // lane (r, kq) holds D[row = 4v + kq][node = 16h + r]
 #pragma unroll
@@ -276,8 +297,8 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
                 const double sg = __shfl(sgn, row, 64);
                 if (rrow >= 0 && rd == dd) {
 #pragma unroll
-                    for (int h = 0; h < kNT; ++h)
-                        if (h < nh) rowval[(size_t)rrow * nqp + (hbase + h) * kNodeTile + r] = acc[h][v] * sg;
+                    for (int j = 0; j < kNT; ++j)
+                        rowval[(size_t)rrow * nqp + hbase * kNodeTile + toff[j] + r] = acc[j][v] * sg;
                 }
             }
         }
