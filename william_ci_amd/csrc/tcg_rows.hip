@@ -199,8 +199,12 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
             }
             atomicAdd(&A.counters[4], (unsigned long long)(2 * s + 10 * (nm + 1)));
         }
-        double s4, c4, sn0, cn0;
+        // e^{i n phi}: rotation by e^{4 i phi} inside each block of 64 orders; at a block start
+        // the lane's base phase advances by e^{64 i phi} (both from sincos once per tile),
+        // so the rounding error stays within a few tens of ulps for n up to ~1300
+        double s4, c4, sn0, cn0, s64, c64;
         sincos(4.0 * phi, &s4, &c4);
+        sincos(64.0 * phi, &s64, &c64);
         sincos((double)kq * phi, &sn0, &cn0);
         // the (direction, rule) combinations present in the tile, ascending
 #pragma unroll 1
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
             dbl4 acc[kNT];
 #pragma unroll
             for (int j = 0; j < kNT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
-            double cn = cn0, sn = sn0;
+            double cn = cn0, sn = sn0, cb = cn0, sb = sn0;
             // Two register sets, each consumed by one step and refilled for the step after
             // next (loads two steps ahead, no register copies).  Rows past nm and orders past
             // the table read in-bounds entries (order 0 of the row, the zero rows at the
@@ -256,7 +260,11 @@ __global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__res
                 const int n = m + kq;
                 if (m > 0) {
                     if ((m & 63) == 0) {
-                        sincos((double)n * phi, &sn, &cn);
+                        const double tb = cb * c64 - sb * s64;
+                        sb = sb * c64 + cb * s64;
+                        cb = tb;
+                        cn = cb;
+                        sn = sb;
                     } else {
                         const double tc = cn * c4 - sn * s4;
                         sn = sn * c4 + cn * s4;
