@@ -429,7 +429,8 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
 // contraction, one wavefront per shell quartet: the block
 //   blk(a,b,c,d) = sum_g fac_g cA cB cC cD sum_k w_k Ix Iy Iz
 // accumulated into blk (zeroed by the caller) with LDS atomics, tasks
-// (primitive quartet g, component) spread over the 64 lanes.
+// (primitive quartet g, component) spread over the 64 lanes; the node rows
+// (16-byte aligned, nqp a multiple of 16) are read two nodes per load.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q, int rule, long s0, long ns,
                                               long r0, const Surv *__restrict__ surv,
@@ -442,6 +443,8 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
     const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
     const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
     const int nblk = nA * nB * nC * nD;
+    const int nq2 = nq >> 1;
+    const double2 *tw2 = reinterpret_cast<const double2 *>(tw);
     for (long task = lane; task < ns * nblk; task += 64) {
         const long g = task / nblk;
         const int comp = (int)(task % nblk);
@@ -463,8 +466,17 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
             else
                 I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
         }
-        double val = 0.0;
-        for (int k = 0; k < nq; ++k) val += (I3[0][k] * tw[k]) * I3[1][k] * I3[2][k];
+        const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
+        const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
+        const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
+        double v0 = 0.0, v1 = 0.0;
+        for (int k2 = 0; k2 < nq2; ++k2) {
+            const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
+            v0 += (x.x * w.x) * y.x * z.x;
+            v1 += (x.y * w.y) * y.y * z.y;
+        }
+        double val = v0 + v1;
+        if (nq & 1) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
         const Surv S = surv[s0 + g];
         const PrimPair &pb = A.pp[S.pb];
         const PrimPair &pk = A.pp[S.pk];
