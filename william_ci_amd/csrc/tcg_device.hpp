@@ -89,6 +89,8 @@ hipError_t launch_translate_rows(double *eri, int nb, int nf, int ncell, const i
 hipError_t launch_dgemm_nn(int M, int N, int K, double alpha, const double *A, long lda, long long sA, const double *B,
                            long ldb, long long sB, double beta, double *C, long ldc, long long sC, long long batch,
                            hipStream_t st);
+hipError_t launch_dft_lattice(const double *Xr, const double *Xi, long long I, long long outer, int N, const double *cosm,
+                              const double *sinm, double sgn, double scale, double *Yr, double *Yi, hipStream_t st);
 hipError_t launch_kmo(const double *Sre, const double *Sim, const double2 *Ck, int nf, int N, int nx, int ny, int nz,
                       int o_hi, int v_lo, double2 *out, hipStream_t st);
 hipError_t launch_kmp2_sums(const double2 *ekmo, const double *ek, const int *occ, int nf, int N, int nx, int ny,

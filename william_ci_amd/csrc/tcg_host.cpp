@@ -1394,19 +1394,28 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
     HIPCHK(c, X1i.alloc(nX));
     hipStream_t st = c->st;
     const double t0 = now_ms();
+    // Each lattice transform is one pass of k_dft_lattice (N <= 64), else real GEMMs.
+    const bool dft = N <= 64;
     // 1) R4 -> k4: X1(a,j2,j3,d,k4) = sum_R4 ERI(a,j2,j3,d R4) e^{+i k4 R4}
-    HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dcos.p, N, 0, 0.0, X1r.p, (long)M1, 0, 1, st));
-    HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dsin.p, N, 0, 0.0, X1i.p, (long)M1, 0, 1, st));
+    if (dft) {
+        HIPCHK(c, launch_dft_lattice(c->d_eri.p, nullptr, (long long)M1, 1, N, dcos.p, dsin.p, 1.0, 1.0, X1r.p, X1i.p, st));
+    } else {
+        HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dcos.p, N, 0, 0.0, X1r.p, (long)M1, 0, 1, st));
+        HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dsin.p, N, 0, 0.0, X1i.p, (long)M1, 0, 1, st));
+    }
     // 2) R3 -> k3 (e^{-i k3 R3}), batched over (d, k4): rows (a, j2, c), X2(a,j2,c,k3,d,k4)
     HIPCHK(c, X2r.alloc(nX));
     HIPCHK(c, X2i.alloc(nX));
     {
         const long long rows = (long long)nf * nb * nf, bs = (long long)nf * nb * nb, nbatch = (long long)nf * N;
         // re = X1r cos + X1i sin ; im = X1i cos - X1r sin
+        if (dft) HIPCHK(c, launch_dft_lattice(X1r.p, X1i.p, rows, nbatch, N, dcos.p, dsin.p, -1.0, 1.0, X2r.p, X2i.p, st));
+        else {
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1r.p, (long)rows, bs, dcos.p, N, 0, 0.0, X2r.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1i.p, (long)rows, bs, dsin.p, N, 0, 1.0, X2r.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1i.p, (long)rows, bs, dcos.p, N, 0, 0.0, X2i.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, -1.0, X1r.p, (long)rows, bs, dsin.p, N, 0, 1.0, X2i.p, (long)rows, bs, nbatch, st));
+        }
     }
     X1r.release();
     X1i.release();
@@ -1415,10 +1424,13 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
     HIPCHK(c, Si.alloc(nX));
     {
         const long long rows = (long long)nf * nf, bs = (long long)nf * nb, nbatch = (long long)nf * N * nf * N;
+        if (dft) HIPCHK(c, launch_dft_lattice(X2r.p, X2i.p, rows, nbatch, N, dcos.p, dsin.p, 1.0, 1.0 / N, Sr.p, Si.p, st));
+        else {
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2r.p, (long)rows, bs, dcosn.p, N, 0, 0.0, Sr.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, -1.0, X2i.p, (long)rows, bs, dsinn.p, N, 0, 1.0, Sr.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2r.p, (long)rows, bs, dsinn.p, N, 0, 0.0, Si.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2i.p, (long)rows, bs, dcosn.p, N, 0, 1.0, Si.p, (long)rows, bs, nbatch, st));
+        }
     }
     X2r.release();
     X2i.release();

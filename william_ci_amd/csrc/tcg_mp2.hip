@@ -15,6 +15,8 @@
 //                     fixed order).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "tcg_device.hpp"
 
 namespace tcg {
@@ -250,9 +252,174 @@ __global__ __launch_bounds__(256) void k_kmo(const double *__restrict__ Sre, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_dft_lattice: the lattice Fourier transform along one cell index,
+//   Y[o][k][i] = scale sum_R X[o][R][i] (cos + i sgn sin)(k R),   R, k < N <= 64,
+// X, Y viewed as [outer][N][inner] (inner fastest); the rows (i, o) are
+// flattened, 16 per wavefront.  On the matrix cores the transposed product
+// Y^T = W^T X^T is formed (16 k x 4 R times 4 R x 16 rows), so that a lane
+// loads and stores 16 consecutive rows; W (cos, sgn sin) sits in LDS for the
+// workgroup's lifetime.  kCplx: X has an imaginary part.
+// ---------------------------------------------------------------------------
+template <bool kCplx>
+__global__ __launch_bounds__(256) void k_dft_lattice(const double *__restrict__ Xr, const double *__restrict__ Xi,
+                                                     long long I, long long outer, int N,
+                                                     const double *__restrict__ cosm, const double *__restrict__ sinm,
+                                                     double sgn, double scale, double *__restrict__ Yr,
+                                                     double *__restrict__ Yi) {
+    __shared__ double Wc[64][65];
+    __shared__ double Ws[64][65];
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int R = e >> 6, k = e & 63;
+        const bool in = R < N && k < N;
+        Wc[R][k] = in ? cosm[R + (size_t)N * k] : 0.0;
+        Ws[R][k] = in ? sgn * sinm[R + (size_t)N * k] : 0.0;
+    }
+    __syncthreads();
+    const long long rows = I * outer, ntile = (rows + 15) / 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    const int nk = (N + 15) >> 4;
+    for (long long tile = (long long)blockIdx.x * 4 + wv; tile < ntile; tile += (long long)gridDim.x * 4) {
+        const long long r = tile * 16 + lr;
+        const bool rv = r < rows;
+        const long long rr = rv ? r : rows - 1;
+        const long long inner = rr % I, ob = rr / I;
+        const size_t base = (size_t)inner + (size_t)I * N * ob;          // (row, R) at base + I R
+        dbl4 re[4], im[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            re[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+            im[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
+        for (int R0 = 0; R0 < N; R0 += 4) {
+            const int R = R0 + lq;
+            const double ar = R < N ? Xr[base + (size_t)I * R] : 0.0;
+            const double ai = (kCplx && R < N) ? Xi[base + (size_t)I * R] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (t < nk) {
+                    const double wc = Wc[R][16 * t + lr], ws = Ws[R][16 * t + lr];
+                    re[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wc, ar, re[t], 0, 0, 0);
+                    im[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ws, ar, im[t], 0, 0, 0);
+                    if (kCplx) {
+                        re[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ws, ai, re[t], 0, 0, 0);
+                        im[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wc, ai, im[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (rv) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = 16 * t + 4 * v + lq;
+                    if (t < nk && k < N) {
+                        Yr[base + (size_t)I * k] = scale * re[t][v];
+                        Yi[base + (size_t)I * k] = scale * im[t][v];
+                    }
+                }
+        }
+    }
+}
+
+hipError_t launch_dft_lattice(const double *Xr, const double *Xi, long long I, long long outer, int N, const double *cosm,
+                              const double *sinm, double sgn, double scale, double *Yr, double *Yi, hipStream_t st) {
+    if (N > 64 || I <= 0 || outer <= 0) return hipErrorInvalidValue;
+    const long long ntile = (I * outer + 15) / 16;
+    const int grid = (int)std::min<long long>((ntile + 3) / 4, 2048);
+    if (Xi)
+        hipLaunchKernelGGL(k_dft_lattice<true>, dim3(grid), dim3(256), 0, st, Xr, Xi, I, outer, N, cosm, sinm, sgn,
+                           scale, Yr, Yi);
+    else
+        hipLaunchKernelGGL(k_dft_lattice<false>, dim3(grid), dim3(256), 0, st, Xr, Xi, I, outer, N, cosm, sinm, sgn,
+                           scale, Yr, Yi);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k_kmo_block: the four quarter transforms of one S block (k2, k3, k4) in LDS
+// (nf <= 8), one workgroup per block: S is read once, coalesced by 25-element
+// runs; the (ov|ov) block of k1 = k4 + k2 - k3 is written out.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kmo_block(const double *__restrict__ Sre, const double *__restrict__ Sim,
+                                                   const double2 *__restrict__ Ck, int nf, int N, KGrid g, int o_hi,
+                                                   int v_lo, double2 *__restrict__ out) {
+    extern __shared__ double2 sm[];
+    const int nf2 = nf * nf, nf4 = nf2 * nf2, nv = nf - v_lo, per = o_hi * nv;
+    double2 *T0 = sm, *T1 = sm + nf4;
+    const int kb = blockIdx.x;
+    const int k2 = kb % N, k3 = (kb / N) % N, k4 = kb / (N * N);
+    const int k1 = g.add(k4, k3, k2);                    // k4 - k3 + k2
+    const size_t sc = (size_t)nf2 * N, sd = (size_t)nf2 * nf * N * N;
+    const size_t base = (size_t)nf2 * k2 + (size_t)nf2 * nf * N * k3 + (size_t)nf4 * N * N * k4;
+    for (int e = threadIdx.x; e < nf4; e += blockDim.x) {          // T0[a + nf (b + nf (c + nf d))]
+        const int ab = e % nf2, c = (e / nf2) % nf, d = e / (nf2 * nf);
+        const size_t o = base + ab + sc * c + sd * d;
+        T0[e] = double2{Sre[o], Sim[o]};
+    }
+    __syncthreads();
+    const double2 *C1 = Ck + (size_t)k1 * nf2, *C2 = Ck + (size_t)k2 * nf2;
+    const double2 *C3 = Ck + (size_t)k3 * nf2, *C4 = Ck + (size_t)k4 * nf2;
+    // T1(p, b, c, d) = sum_a conj C1(a, p) T0(a, b, c, d), p < o_hi
+    for (int e = threadIdx.x; e < o_hi * nf * nf2; e += blockDim.x) {
+        const int p = e % o_hi, bcd = e / o_hi;
+        double2 acc = {0.0, 0.0};
+        for (int a = 0; a < nf; ++a) {
+            const double2 m = cmulc(C1[a + nf * p], T0[a + nf * bcd]);
+            acc.x += m.x;
+            acc.y += m.y;
+        }
+        T1[e] = acc;                                      // T1[p + o_hi (b + nf (c + nf d))]
+    }
+    __syncthreads();
+    // T0(p, q, c, d) = sum_b C2(b, q) T1(p, b, c, d), q >= v_lo
+    for (int e = threadIdx.x; e < per * nf2; e += blockDim.x) {
+        const int p = e % o_hi, q = (e / o_hi) % nv, cd = e / per;
+        double2 acc = {0.0, 0.0};
+        for (int b = 0; b < nf; ++b) {
+            const double2 m = cmul(C2[b + nf * (v_lo + q)], T1[p + o_hi * (b + nf * cd)]);
+            acc.x += m.x;
+            acc.y += m.y;
+        }
+        T0[e] = acc;                                      // T0[pq + per (c + nf d)]
+    }
+    __syncthreads();
+    // T1(pq, r, d) = sum_c conj C3(c, r) T0(pq, c, d), r < o_hi
+    for (int e = threadIdx.x; e < per * o_hi * nf; e += blockDim.x) {
+        const int pq = e % per, r = (e / per) % o_hi, d = e / (per * o_hi);
+        double2 acc = {0.0, 0.0};
+        for (int c = 0; c < nf; ++c) {
+            const double2 m = cmulc(C3[c + nf * r], T0[pq + per * (c + nf * d)]);
+            acc.x += m.x;
+            acc.y += m.y;
+        }
+        T1[e] = acc;                                      // T1[pq + per (r + o_hi d)]
+    }
+    __syncthreads();
+    // out(pq, rs) = sum_d C4(d, s) T1(pq, r, d), s >= v_lo
+    const size_t ob = (size_t)per * per * (k1 + (size_t)N * (k2 + (size_t)N * k3));
+    for (int e = threadIdx.x; e < per * per; e += blockDim.x) {
+        const int pq = e % per, rs = e / per, r = rs % o_hi, s = rs / o_hi;
+        double2 acc = {0.0, 0.0};
+        for (int d = 0; d < nf; ++d) {
+            const double2 m = cmul(C4[d + nf * (v_lo + s)], T1[pq + per * (r + o_hi * d)]);
+            acc.x += m.x;
+            acc.y += m.y;
+        }
+        out[ob + e] = acc;
+    }
+}
+
 hipError_t launch_kmo(const double *Sre, const double *Sim, const double2 *Ck, int nf, int N, int nx, int ny, int nz,
                       int o_hi, int v_lo, double2 *out, hipStream_t st) {
     KGrid g{nx, ny, nz};
+    if (nf <= 8) {
+        const size_t lds = 2 * sizeof(double2) * (size_t)nf * nf * nf * nf;
+        hipLaunchKernelGGL(k_kmo_block, dim3((unsigned)((size_t)N * N * N)), dim3(256), lds, st, Sre, Sim, Ck, nf, N, g,
+                           o_hi, v_lo, out);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_kmo, dim3(4096), dim3(256), 0, st, Sre, Sim, Ck, nf, N, g, o_hi, v_lo, out);
     return hipGetLastError();
 }

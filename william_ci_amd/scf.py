@@ -190,7 +190,8 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
         EJ = 0.5 * float(np.sum(P * J.T)) * Nc
         EK = 0.5 * float(np.sum(P * K.T)) * Nc
         E = ET + EV + EJ + EK
-        err = F @ P @ S - S @ P @ F
+        M = F @ P @ S
+        err = M - M.T                     # S P F = (F P S)^T: F, P, S are symmetric
         err_o = X.T @ err @ X
         if it > 1:
             conv = float(np.abs(err_o).max())
