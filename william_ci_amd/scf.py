@@ -71,6 +71,12 @@ class Bloch:
         self.nf = sysm.nfuc
         self.nc = sysm.ncell           # (nx, ny, nz), cells x fastest
         self.N = sysm.n_cell
+        nx, ny, nz = self.nc
+        cells = np.arange(self.N)
+        cx, cy, cz = cells % nx, (cells // nx) % ny, cells // (nx * ny)
+        self._dx = (cx[:, None] - cx[None, :]) % nx      # cell differences R - R'
+        self._dy = (cy[:, None] - cy[None, :]) % ny
+        self._dz = (cz[:, None] - cz[None, :]) % nz
 
     def to_k(self, M: np.ndarray) -> np.ndarray:
         """M_k(a,b) = sum_R e^{+i k.R} M(a, b + R nf), from the unit-cell rows of M."""
@@ -83,12 +89,7 @@ class Bloch:
         """M(a + R nf, b + R' nf) = Re (1/N) sum_k e^{+i k.(R-R')} M_k(a,b)."""
         nf, (nx, ny, nz) = self.nf, self.nc
         C = np.fft.ifftn(Mk.reshape(nz, ny, nx, nf, nf), axes=(0, 1, 2)).real   # C[dR]
-        cells = np.arange(self.N)
-        cx, cy, cz = cells % nx, (cells // nx) % ny, cells // (nx * ny)
-        dx = (cx[:, None] - cx[None, :]) % nx
-        dy = (cy[:, None] - cy[None, :]) % ny
-        dz = (cz[:, None] - cz[None, :]) % nz
-        blocks = C[dz, dy, dx]                         # (N, N, nf, nf): block (R, R')
+        blocks = C[self._dz, self._dy, self._dx]       # (N, N, nf, nf): block (R, R')
         return blocks.transpose(0, 2, 1, 3).reshape(self.N * nf, self.N * nf)
 
 
@@ -170,6 +171,9 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
     for k in range(Nc):
         Xk[k] = canonical_X(Sk[k])
     dead = [np.nonzero(np.all(Xk[k] == 0, axis=0))[0] for k in range(Nc)]
+    dead_k = np.concatenate([np.full(len(d), k) for k, d in enumerate(dead)]).astype(int)
+    dead_i = np.concatenate(dead).astype(int) if dead_k.size else np.zeros(0, dtype=int)
+    XkH = Xk.conj().transpose(0, 2, 1)
     dii = _DIIS(MAX_DIIS)
     E_old, conv, converged = 0.0, 1.0, False
     n_stable, hist = 0, [np.inf] * 5
@@ -212,18 +216,15 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
         if diis:
             F = dii.extrapolate(err_o, F)
         Fk = bl.to_k(F)
-        for k in range(Nc):
-            Fp = Xk[k].conj().T @ Fk[k] @ Xk[k]
-            Fp[dead[k], dead[k]] = LINDEP_SHIFT
-            ek[k], Up = np.linalg.eigh(Fp)
-            Ck[k] = Xk[k] @ Up
+        Fp = XkH @ Fk @ Xk                           # all k blocks at once
+        Fp[dead_k, dead_i, dead_i] = LINDEP_SHIFT
+        ek, Up = np.linalg.eigh(Fp)
+        Ck = Xk @ Up
         order = np.argsort(ek.ravel(), kind="stable")[:nO]
         occ[:] = False
         occ.ravel()[order] = True
-        Pk = np.zeros((Nc, nf, nf), dtype=complex)
-        for k in range(Nc):
-            c = Ck[k][:, occ[k]]
-            Pk[k] = 2.0 * c @ c.conj().T
+        Cocc = Ck * occ[:, None, :]
+        Pk = 2.0 * Cocc @ Cocc.conj().transpose(0, 2, 1)
         P = bl.to_ao(Pk)
     return RHFResult(energy=E + enuc, e_nuc=enuc, converged=converged, iterations=it, orbital_error=conv,
                      mo_energies=ek.copy(), mo_coef=Ck.copy(), P=P, occupied=occ.copy(),
