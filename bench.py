@@ -123,7 +123,7 @@ def _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total, ms_step_pla
     return {"orders": list(model.orders), "margin": model.margin,
             "shell_quartets_per_rule": {"plain": int(counts[-1]),
                                         **{str(o): int(c) for o, c in zip(model.orders, counts[:-1])}},
-            "ms_per_step": step * 1e3, "rows_kernel_ms": tm["rows_kernel_ms"],
+            "ms_per_step": step * 1e3, "step_ms": {k: round(v, 3) for k, v in tm.items()},
             "speedup_eri": ms_step_plain / step,
             "rhf_energy": rl.energy, "rhf_iterations": rl.iterations, "mp2_ec": ml["ec"],
             "dE_rhf": rl.energy - res.energy, "dE_total": (rl.energy + ml["ec"]) - (res.energy + mp2["ec"]),

@@ -64,7 +64,7 @@ struct EriArgs {
 
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
-                          const int *pair_rep, double *V, hipStream_t st);
+                          const int *pair_rep, int max_prim_pairs, double *V, hipStream_t st);
 hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, const VecOut *vo, int nvo, const double2 *dco,
                                double *ivbuf, double2 *vec, hipStream_t st);
 hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, double Qmax, int nshuc,

@@ -2,6 +2,8 @@
 // The quartet pipeline itself is in tcg_rows.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "tcg_device.hpp"
 
 namespace tcg {
@@ -117,21 +119,32 @@ hipError_t launch_jk_stored(const double *eri, const double *P, double *J, doubl
 //   J(a,c) =      sum_{(b,d) in list_c} ERI(a,b,c,d) P(b,d)   (physicist)
 // One block per d (K) or per J column: no global atomics.
 // ---------------------------------------------------------------------------
-__global__ void k_col_flags(const double *__restrict__ eri, int n1, int nbas, int physicist,
-                            unsigned char *__restrict__ flags, int *__restrict__ cnt_d, int *__restrict__ cnt_t,
-                            unsigned *__restrict__ key_t) {
-    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// flags and keys per column; the per-d and per-target counts are gathered in
+// LDS histograms by persistent workgroups (one global atomic per bin and
+// workgroup instead of one per non-zero column).
+__global__ __launch_bounds__(256) void k_col_flags(const double *__restrict__ eri, int n1, int nbas, int physicist,
+                                                   unsigned char *__restrict__ flags, int *__restrict__ cnt_d,
+                                                   int *__restrict__ cnt_t, unsigned *__restrict__ key_t) {
+    extern __shared__ int hist[];                 // [nbas] per d, [nbas] per target
+    for (int i = threadIdx.x; i < 2 * nbas; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
     const size_t ncol = (size_t)nbas * nbas * nbas;
-    if (g >= ncol) return;
-    bool nz = false;
-    for (int a = 0; a < n1; ++a) nz |= eri[(size_t)a + (size_t)n1 * g] != 0.0;
-    flags[g] = nz ? 1 : 0;
-    const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas), d = (int)(g / ((size_t)nbas * nbas));
-    const int t = physicist ? c : b;
-    key_t[g] = (unsigned)t;
-    if (nz) {
-        atomicAdd(&cnt_d[d], 1);
-        atomicAdd(&cnt_t[t], 1);
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < ncol; g += (size_t)gridDim.x * blockDim.x) {
+        bool nz = false;
+        for (int a = 0; a < n1; ++a) nz |= eri[(size_t)a + (size_t)n1 * g] != 0.0;
+        flags[g] = nz ? 1 : 0;
+        const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas), d = (int)(g / ((size_t)nbas * nbas));
+        const int t = physicist ? c : b;
+        key_t[g] = (unsigned)t;
+        if (nz) {
+            atomicAdd(&hist[d], 1);
+            atomicAdd(&hist[nbas + t], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbas; i += blockDim.x) {
+        if (hist[i]) atomicAdd(&cnt_d[i], hist[i]);
+        if (hist[nbas + i]) atomicAdd(&cnt_t[i], hist[nbas + i]);
     }
 }
 
@@ -214,8 +227,10 @@ static hipError_t launch_one(const double *eri, const double *P, const long long
 hipError_t launch_col_flags(const double *eri, int n1, int nbas, int physicist, unsigned char *flags, int *cnt_d,
                             int *cnt_t, unsigned *key_t, hipStream_t st) {
     const size_t ncol = (size_t)nbas * nbas * nbas;
-    hipLaunchKernelGGL(k_col_flags, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, eri, n1, nbas, physicist,
-                       flags, cnt_d, cnt_t, key_t);
+    if ((size_t)nbas * 2 * sizeof(int) > 65536) return hipErrorInvalidValue;     // nbas <= 8192
+    const int grid = (int)std::min<size_t>((ncol + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_col_flags, dim3(grid), dim3(256), sizeof(int) * 2 * (size_t)nbas, st, eri, n1, nbas,
+                       physicist, flags, cnt_d, cnt_t, key_t);
     return hipGetLastError();
 }
 

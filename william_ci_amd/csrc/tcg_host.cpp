@@ -946,7 +946,9 @@ static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, 
     HIPCHK(c, d_V.alloc(nn));
     HIPCHK(c, launch_fill(d_V.p, nn, 0.0, c->st));
     EriArgs A = base_args(c);
-    HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, d_V.p, c->st));
+    int maxpp = 1;
+    for (int rp : pair_rep) maxpp = std::max(maxpp, c->sp[rp].count);
+    HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, maxpp, d_V.p, c->st));
     if constexpr (!std::is_same<F, std::nullptr_t>::value) host_work();
     V.assign(nn, 0.0);
     HIPCHK(c, hipMemcpyAsync(V.data(), d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
