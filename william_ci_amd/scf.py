@@ -85,6 +85,11 @@ class Bloch:
         Mk = np.fft.ifftn(B, axes=(0, 1, 2)) * self.N
         return Mk.reshape(self.N, nf, nf)
 
+    def max_abs_ao(self, Mk: np.ndarray) -> float:
+        """Largest |element| of to_ao(Mk) without forming the matrix."""
+        nf, (nx, ny, nz) = self.nf, self.nc
+        return float(np.abs(np.fft.ifftn(Mk.reshape(nz, ny, nx, nf, nf), axes=(0, 1, 2)).real).max())
+
     def to_ao(self, Mk: np.ndarray) -> np.ndarray:
         """M(a + R nf, b + R' nf) = Re (1/N) sum_k e^{+i k.(R-R')} M_k(a,b)."""
         nf, (nx, ny, nz) = self.nf, self.nc
@@ -110,6 +115,27 @@ def _occupy_cut(e: np.ndarray, c: np.ndarray, nO: int, deg_tol: float = 1e-8) ->
     return cw @ cw.T
 
 
+def _occupy_cut_k(ek: np.ndarray, ck: np.ndarray, nO: int, bl: "Bloch", deg_tol: float = 1e-8):
+    """_occupy_cut over the union of the k spectra: ek (Nk, nf), ck (Nk, nf, nf)
+    columns = orbitals; returns the AO density and its k blocks."""
+    flat = ek.ravel()
+    order = np.argsort(flat, kind="stable")
+    e = flat[order]
+    tol = deg_tol * max(1.0, abs(e[nO - 1]))
+    lo = nO - 1
+    while lo > 0 and abs(e[lo - 1] - e[nO - 1]) < tol:
+        lo -= 1
+    hi = nO - 1
+    while hi < len(e) - 1 and abs(e[hi + 1] - e[nO - 1]) < tol:
+        hi += 1
+    w = np.zeros(len(e))
+    w[order[:lo]] = 2.0
+    w[order[lo:hi + 1]] = 2.0 * (nO - lo) / (hi - lo + 1)
+    cw = ck * np.sqrt(w.reshape(ek.shape))[:, None, :]
+    Pk = cw @ cw.conj().transpose(0, 2, 1)
+    return bl.to_ao(Pk), Pk
+
+
 class _DIIS:
     def __init__(self, n: int):
         self.err = []
@@ -123,7 +149,7 @@ class _DIIS:
         m = len(self.err)
         E = np.array(self.err)
         A = np.zeros((m + 1, m + 1))
-        A[:m, :m] = E @ E.T
+        A[:m, :m] = (E.conj() @ E.T).real
         A[:m, m] = A[m, :m] = -1.0
         b = np.zeros(m + 1)
         b[m] = -1.0
@@ -158,18 +184,31 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
     nb, nO, Nc, nf = sysm.nbas, sysm.n_occ, sysm.n_cell, sysm.nfuc
     Hc = T + V
     enuc = nuclear_repulsion(sysm)
-    X = canonical_X(S)
-    # core guess in the supercell, degenerate level at the cut shared evenly
-    cp = X.T @ Hc @ X
-    _shift_dropped(X, cp)
-    e0, U0 = np.linalg.eigh(cp)
-    P = _occupy_cut(e0, X @ U0, nO)
-
     bl = Bloch(sysm)
     Sk = bl.to_k(S)
-    Xk = np.zeros_like(Sk)
-    for k in range(Nc):
-        Xk[k] = canonical_X(Sk[k])
+    sk, Uk = np.linalg.eigh(Sk)
+    kspace = bool(np.all(sk > LINDEP))
+    if kspace:
+        # No linear dependence: X = S^-1/2 (Lowdin), F, P, S are block-circulant, so the
+        # whole loop runs block by block in k space -- the same numbers as the supercell
+        # forms: (X^T err X)(k) = X(k)^H err(k) X(k), its largest element is the largest
+        # element of its inverse lattice transform, and Frobenius products (DIIS) are sums
+        # over k (Parseval).  The core guess fills a degenerate level at the cut evenly
+        # over its whole space, as in the supercell.
+        Xk = (Uk / np.sqrt(sk)[:, None, :]) @ Uk.conj().transpose(0, 2, 1)
+        Hk = bl.to_k(Hc)
+        e0k, U0k = np.linalg.eigh(Xk.conj().transpose(0, 2, 1) @ Hk @ Xk)
+        P, Pk = _occupy_cut_k(e0k, Xk @ U0k, nO, bl)
+    else:
+        X = canonical_X(S)
+        # core guess in the supercell, degenerate level at the cut shared evenly
+        cp = X.T @ Hc @ X
+        _shift_dropped(X, cp)
+        e0, U0 = np.linalg.eigh(cp)
+        P = _occupy_cut(e0, X @ U0, nO)
+        Xk = np.zeros_like(Sk)
+        for k in range(Nc):
+            Xk[k] = canonical_X(Sk[k])
     dead = [np.nonzero(np.all(Xk[k] == 0, axis=0))[0] for k in range(Nc)]
     dead_k = np.concatenate([np.full(len(d), k) for k, d in enumerate(dead)]).astype(int)
     dead_i = np.concatenate(dead).astype(int) if dead_k.size else np.zeros(0, dtype=int)
@@ -187,18 +226,25 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
         tj = time.perf_counter()
         J, K = jk(P)
         t_jk += time.perf_counter() - tj
-        F = Hc + J + K
-        F = bl.to_ao(bl.to_k(F))
         ET = float(np.sum(P * T.T))
         EV = float(np.sum(P * V.T))
         EJ = 0.5 * float(np.sum(P * J.T)) * Nc
         EK = 0.5 * float(np.sum(P * K.T)) * Nc
         E = ET + EV + EJ + EK
-        M = F @ P @ S
-        err = M - M.T                     # S P F = (F P S)^T: F, P, S are symmetric
-        err_o = X.T @ err @ X
-        if it > 1:
-            conv = float(np.abs(err_o).max())
+        if kspace:
+            Fk = bl.to_k(Hc + J + K)
+            Mk = Fk @ Pk @ Sk
+            err_o = XkH @ (Mk - Mk.conj().transpose(0, 2, 1)) @ Xk
+            if it > 1:
+                conv = bl.max_abs_ao(err_o)
+        else:
+            F = Hc + J + K
+            F = bl.to_ao(bl.to_k(F))
+            M = F @ P @ S
+            err = M - M.T                     # S P F = (F P S)^T: F, P, S are symmetric
+            err_o = X.T @ err @ X
+            if it > 1:
+                conv = float(np.abs(err_o).max())
         if log:
             log(f"  SCF {it:3d}  E = {E + enuc:.12f}  conv = {conv:.3e}")
         if conv < thresh:
@@ -213,9 +259,13 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
             break
         hist[it % 5] = conv
         E_old = E
-        if diis:
-            F = dii.extrapolate(err_o, F)
-        Fk = bl.to_k(F)
+        if kspace:
+            if diis:
+                Fk = dii.extrapolate(err_o, Fk)
+        else:
+            if diis:
+                F = dii.extrapolate(err_o, F)
+            Fk = bl.to_k(F)
         Fp = XkH @ Fk @ Xk                           # all k blocks at once
         Fp[dead_k, dead_i, dead_i] = LINDEP_SHIFT
         ek, Up = np.linalg.eigh(Fp)
