@@ -52,19 +52,50 @@ __global__ void k_bessel_jobs(const VecJob *__restrict__ jobs, int njobs, double
     const double r = 2.0 / J.A;
     double up = 0.0, cur = 1e-280;     // I_{k+1}, I_k
     iv[M] = cur;
+    // A rescale by 1e-250 at index m applies to iv[m..M]; the positions are kept and the
+    // factors applied in the final pass (the same multiplications in the same order as
+    // rescaling the stored values at once), up to kMaxRescale of them.
+    constexpr int kMaxRescale = 48;
+    int rpos[kMaxRescale];
+    int nres = 0;
     for (int k = M; k >= 1; --k) {
         const double dn = up + (double)k * r * cur;
         iv[k - 1] = dn;
         up = cur;
         cur = dn;
         if (dn > 1e250) {
-            for (int m = k - 1; m <= M; ++m) iv[m] *= 1e-250;
+            if (nres < kMaxRescale) {
+                rpos[nres++] = k - 1;
+            } else {
+                for (int m = k - 1; m <= M; ++m) iv[m] *= 1e-250;
+            }
             up *= 1e-250;
             cur *= 1e-250;
         }
     }
-    const double s = d_i0_scaled(J.A) / iv[0];
-    for (int n = 0; n <= M; ++n) iv[n] *= s;
+    // a value at index n takes the rescales recorded at positions <= n; positions were
+    // recorded in decreasing order, so those are rpos[q..nres)
+    double v0 = iv[0];
+    for (int t = 0; t < nres; ++t)
+        if (rpos[t] <= 0) v0 *= 1e-250;
+    const double s = d_i0_scaled(J.A) / v0;
+    // sixteen loads in flight at a time (the pass would otherwise wait on every load)
+    int q = nres;
+    for (int n0 = 0; n0 <= M; n0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = n0 + j <= M ? iv[n0 + j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int n = n0 + j;
+            if (n <= M) {
+                while (q > 0 && rpos[q - 1] <= n) --q;
+                double x = v[j];
+                for (int t = q; t < nres; ++t) x *= 1e-250;
+                iv[n] = x * s;
+            }
+        }
+    }
 }
 
 // V_m = sum_k d_k exp(-A) I_|m-k|(A), m = 0..N: one block per vector, the
