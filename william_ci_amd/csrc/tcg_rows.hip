@@ -16,8 +16,8 @@
 //                 I_d(t_k) = sum_n Re[conj(G_n) H_n e^{in phi}] T_d[n][k]
 //                 with v_mfma_f64_16x16x4f64 (F on the VALU as the A operand,
 //                 the t-table as B), loads two steps ahead of use.  Result:
-//                 32 node values per row, in HBM.
-//   k_real_rows   one thread per (primitive quartet, real direction, node):
+//                 the node values of every row (nqp per row), in HBM.
+//   k_real_rows   one workgroup per (primitive quartet, real direction), a thread per node:
 //                 the Hermite-form factors of all power combinations.
 //   k_contract    one wavefront per shell quartet: sum_k w_k Ix Iy Iz, coefficients, the shell
 //                 block in LDS, then the 8-permutation scatter translated to
@@ -343,15 +343,15 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
 // first (group, combination) index) and g the group inside the quartet,
 //   realval[(R * nreal + (g * nreal + dr) * ncomb + combo) * nqp + k].
 // ---------------------------------------------------------------------------
-__global__ void k_real_rows(EriArgs A, const long long *__restrict__ soff, const long long *__restrict__ roff,
+__global__ __launch_bounds__(128) void k_real_rows(EriArgs A, const long long *__restrict__ soff, const long long *__restrict__ roff,
                             const Surv *__restrict__ surv, const int *__restrict__ sq, long long ns,
                             double *__restrict__ realval) {
+    // one workgroup per (survivor, real direction), one thread per node
     const int nreal = 3 - A.ndp, nqp = A.nqp;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ns * nreal * nqp) return;
-    const int k = (int)(t % nqp);
-    const int drr = (int)((t / nqp) % nreal);
-    const long long g = t / ((long long)nqp * nreal);
+    const int k = threadIdx.x;
+    if (k >= nqp) return;
+    const int drr = nreal == 1 ? 0 : (int)(blockIdx.x & 1);
+    const long long g = nreal == 1 ? (long long)blockIdx.x : (long long)(blockIdx.x >> 1);
     const int dr = A.ndp + drr;
     const int qi = sq[g];
     const int rule = quartet_rule(A, qi);
@@ -419,9 +419,10 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
     if (a.ndp >= 3 || ns <= 0) return hipSuccess;
     const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
     hipLaunchKernelGGL(k_surv_quartet, dim3(grid), dim3(64), 0, st, soff, a.nq4, sq);
-    const long long n = ns * (3 - a.ndp) * a.nqp;
-    hipLaunchKernelGGL(k_real_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, soff, roff, surv, sq, ns,
-                       realval);
+    const long long nb = ns * (3 - a.ndp);          // workgroups: (survivor, real direction)
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    const int bs = (a.nqp + 63) / 64 * 64;
+    hipLaunchKernelGGL(k_real_rows, dim3((unsigned)nb), dim3(bs), 0, st, a, soff, roff, surv, sq, ns, realval);
     return hipGetLastError();
 }
 
@@ -445,9 +446,10 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
     const int nblk = nA * nB * nC * nD;
     const int nq2 = nq >> 1;
     const double2 *tw2 = reinterpret_cast<const double2 *>(tw);
-    for (long task = lane; task < ns * nblk; task += 64) {
-        const long g = task / nblk;
-        const int comp = (int)(task % nblk);
+    const int ntask = (int)ns * nblk;             // <= 16^4 primitive quartets x 6^4 components
+    for (int task = lane; task < ntask; task += 64) {
+        const int g = task / nblk;
+        const int comp = task - g * nblk;
         int t = comp;
         const int ca = t % nA; t /= nA;
         const int cb = t % nB; t /= nB;
