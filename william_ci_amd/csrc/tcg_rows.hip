@@ -600,12 +600,15 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                                                      const long long *__restrict__ roff,
                                                      const Surv *__restrict__ surv, const double *__restrict__ rowval,
                                                      const double *__restrict__ realval, int jk_lds) {
-    extern __shared__ double lds[];      // [4][max_blk] blocks, then J[nfuc*nbas], K[nfuc*nbas] when jk_lds
+    // LDS: [4][max_blk] blocks, [4][2][36] per-wave J/K sub-blocks, then J[nfuc*nbas],
+    // K[nfuc*nbas] when jk_lds
+    extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nf = A.nfuc, nb = A.nbas;
     const size_t njk = (size_t)nf * nb;
     double *blk = lds + (size_t)wv * A.max_blk;
-    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk : A.Jacc;
+    double *Jw = lds + 4 * (size_t)A.max_blk + 72 * wv, *Kw = Jw + 36;
+    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 72 : A.Jacc;
     double *Kl = jk_lds ? Jl + njk : A.Kacc;
     if (jk_lds) {
         for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
@@ -622,22 +625,70 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         wave_sync();
         quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, lane);
         wave_sync();
-        for (int task = lane; task < nblk * 8; task += 64) {
-            const int comp = task >> 3, ip = task & 7;
-            int t = comp;
-            const int ca = t % nA; t /= nA;
-            const int cb = t % nB; t /= nB;
-            const int cc = t % nC;
-            const int cd = t / nC;
-            const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
-            if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz)
-                continue;
-            int pp[4];
-            stored_position(A, f4, ip, pp);
-            if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
-            const double val = blk[comp];
-            atomicAdd(&Jl[pp[0] + (size_t)nf * pp[1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
-            atomicAdd(&Kl[pp[0] + (size_t)nf * pp[3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
+        if (jk_lds) {
+            // J/K in LDS: every counted element adds there directly
+            for (int task = lane; task < nblk * 8; task += 64) {
+                const int comp = task >> 3, ip = task & 7;
+                int t = comp;
+                const int ca = t % nA; t /= nA;
+                const int cb = t % nB; t /= nB;
+                const int cc = t % nC;
+                const int cd = t / nC;
+                const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
+                if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz)
+                    continue;
+                int pp[4];
+                stored_position(A, f4, ip, pp);
+                if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
+                const double val = blk[comp];
+                atomicAdd(&Jl[pp[0] + (size_t)nf * pp[1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
+                atomicAdd(&Kl[pp[0] + (size_t)nf * pp[3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
+            }
+            continue;
+        }
+        // J/K in global memory: one permutation at a time, the counted elements of the
+        // block add into the (n0 x n1) J and (n0 x n3) K sub-blocks the permutation maps
+        // them to (LDS), which then go out with one atomic per entry instead of one per
+        // element
+        const int ncmp[4] = {nA, nB, nC, nD};
+        const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
+        for (int ip = 0; ip < 8; ++ip) {
+            const int x0 = c_perm8_rows[ip][0], x1 = c_perm8_rows[ip][1], x3 = c_perm8_rows[ip][3];
+            const int n0 = ncmp[x0], n1 = ncmp[x1], n3 = ncmp[x3];
+            if (lane < 36) {
+                Jw[lane] = 0.0;
+                Kw[lane] = 0.0;
+            }
+            wave_sync();
+            for (int comp = lane; comp < nblk; comp += 64) {
+                int t = comp;
+                const int ca = t % nA; t /= nA;
+                const int cb = t % nB; t /= nB;
+                const int cc = t % nC;
+                const int cd = t / nC;
+                const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
+                if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz)
+                    continue;
+                int pp[4];
+                stored_position(A, f4, ip, pp);
+                if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
+                const int c4[4] = {ca, cb, cc, cd};
+                const double val = blk[comp];
+                atomicAdd(&Jw[c4[x0] * n1 + c4[x1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
+                atomicAdd(&Kw[c4[x0] * n3 + c4[x3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
+            }
+            wave_sync();
+            int pb[4];                                     // stored position of components (0,0,0,0)
+            stored_position(A, f0, ip, pb);
+            for (int e = lane; e < n0 * n1; e += 64) {
+                const double v = Jw[e];
+                if (v != 0.0) atomicAdd(&Jl[(pb[0] + e / n1) + (size_t)nf * (pb[1] + e % n1)], v);
+            }
+            for (int e = lane; e < n0 * n3; e += 64) {
+                const double v = Kw[e];
+                if (v != 0.0) atomicAdd(&Kl[(pb[0] + e / n3) + (size_t)nf * (pb[3] + e % n3)], v);
+            }
+            wave_sync();
         }
     }
     if (jk_lds) {
@@ -652,7 +703,7 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                               const double *rowval, const double *realval, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const size_t blk_bytes = sizeof(double) * 4 * (size_t)a.max_blk;
+    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 72);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
     const size_t lds_max = 160 * 1024;
     // TCG_JK_GLOBAL=1 forces the global-memory accumulation (the path of systems
