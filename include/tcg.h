@@ -123,7 +123,7 @@ int tcg_kmp2(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, c
  * (metric, pivoted Cholesky chol_f :1174-1205 with tolerance 1e-10, the
  * 3-index integrals with the df_schwarz 1e-13 screen, B as a cell
  * convolution).  The auxiliary functions come as an AO list like
- * tcg_set_basis's (normalised coefficients), nau_cell (<= 64) per cell, the
+ * tcg_set_basis's (normalised coefficients), nau_cell (<= 256) per cell, the
  * cells in the order of the orbital basis; the exponent-0 dummy of
  * dummy_at (:837-844) is added by the library.  ctx supplies the torus, the
  * orbital basis and the quadrature rule. */

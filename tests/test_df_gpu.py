@@ -158,3 +158,42 @@ def test_df_mp2_h64_chain_etb41():
     assert abs(got["ec"] - exact["ec"]) < 5e-4 * abs(exact["ec"])
     nf = s.nfuc
     assert np.abs(Jd[:nf] - Jx[:nf]).max() < 1e-3 and np.abs(Kd[:nf] - Kx[:nf]).max() < 1e-3
+
+
+def test_df_more_than_64_aux_per_cell():
+    """Two H per cell with the 41-function set: 82 auxiliary functions per cell,
+    the global-memory metric factorisation; fitted integrals and J/K against
+    the oracle."""
+    from oracle import df as odf
+    from william_ci_amd import df as wdf
+    from william_ci_amd.engine import Engine
+    s = tsys.build_system("Tori1D\n3 3.6 10.8\nSAO\n$$\nH 0 0 0\nH 1.4 0 0\n$$\n", tsys.H_BASIS)
+    coef = normalised_coef(s)
+    aux = wdf.aux_list(s, wdf.read_aux(wdf.AUX_H_ETB41))
+    print("case H2 chain x3, 82 aux per cell", flush=True)
+    eng = Engine(s, coef)
+    d = wdf.DF(eng, aux)
+    B, ranks, dmin = _oracle_fit(s, coef, aux)
+    Bg = d.B().transpose(2, 1, 0)
+    Eg, Eo = odf.eri_sao(Bg, s.ncell), odf.eri_sao(B, s.ncell)
+    err = np.abs(Eg - Eo).max() / np.abs(Eo).max()
+    info = d.info()
+    nf, nb, N = s.nfuc, s.nbas, s.n_cell
+    cdiff, cadd = odf.cell_ops(s.ncell)
+    rng = np.random.default_rng(4)
+    P0 = rng.normal(size=(nf, nb))
+    P = np.zeros((nb, nb))
+    for R in range(N):
+        for nu in range(nb):
+            P[R * nf:(R + 1) * nf, cadd[nu // nf, R] * nf + nu % nf] = P0[:, nu]
+    P = 0.5 * (P + P.T)
+    Jg, Kg = d.jk(P)
+    Jo, Ko = odf.jk(B, P, s.ncell)
+    d.close()
+    eng.close()
+    print(f"  nau {info['nau']} dropped {info['dropped']} (oracle {int(d.nau - ranks.min())})  fitted ERI rel diff "
+          f"{err:.2e}  J {np.abs(Jg - Jo).max():.2e}  K {np.abs(Kg - Ko).max():.2e}")
+    assert info["nau"] == 82
+    assert info["dropped"] == int(82 - ranks.min())
+    assert err < 1e-9
+    assert np.abs(Jg - Jo).max() < 1e-9 and np.abs(Kg - Ko).max() < 1e-9

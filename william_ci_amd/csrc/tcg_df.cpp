@@ -31,7 +31,7 @@ using namespace tcg;
 
 namespace tcg {
 hipError_t launch_metric_factor(const double *Vre, const double *Vim, int n, int nk, const int *mate, double tol,
-                                double *fre, double *fim, int *rank, double *dmin, hipStream_t st);
+                                double2 *work, double *fre, double *fim, int *rank, double *dmin, hipStream_t st);
 hipError_t launch_df_gamma(const double *B, const double *Ps, int nf, int nb, int nau, int N, double *G,
                            hipStream_t st);
 hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *J, hipStream_t st);
@@ -164,7 +164,7 @@ extern "C" {
 
 int tcg_df_create(tcg_ctx *ctx, int nau_cell, const double *aux_center, const int *aux_lxyz,
                   const int *aux_prim_off, const double *aux_expo, const double *aux_coef, tcg_df **out) {
-    if (!ctx || !out || nau_cell <= 0 || nau_cell > 64 || !aux_center || !aux_lxyz || !aux_prim_off || !aux_expo ||
+    if (!ctx || !out || nau_cell <= 0 || nau_cell > 256 || !aux_center || !aux_lxyz || !aux_prim_off || !aux_expo ||
         !aux_coef)
         return TCG_E_ARG;
     *out = nullptr;
@@ -305,7 +305,10 @@ int tcg_df_create(tcg_ctx *ctx, int nau_cell, const double *aux_center, const in
     DCHK(df, dfi.alloc((size_t)nau * nau * N));
     DCHK(df, drank.alloc(N));
     DCHK(df, ddmin.alloc(N));
-    DCHK(df, launch_metric_factor(dVr.p, dVi.p, nau, N, dmate.p, kMetricTol, dfr.p, dfi.p, drank.p, ddmin.p, df->st));
+    DBuf<double2> dwork;
+    if (nau > 64) DCHK(df, dwork.alloc(2 * (size_t)nau * nau * N));
+    DCHK(df, launch_metric_factor(dVr.p, dVi.p, nau, N, dmate.p, kMetricTol, dwork.p, dfr.p, dfi.p, drank.p, ddmin.p,
+                                  df->st));
     df->rank.assign(N, 0);
     std::vector<double> dm(N);
     DCHK(df, hipMemcpyAsync(df->rank.data(), drank.p, sizeof(int) * N, hipMemcpyDeviceToHost, df->st));

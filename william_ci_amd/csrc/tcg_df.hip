@@ -1,7 +1,7 @@
 // Density fitting on the GPU (gfx950): the kernels of tcg_df.cpp.
 //
 //   k_metric_factor  per k point: pivoted Cholesky of the Hermitian metric
-//                    V(k) (nau <= 64) with the stopping rule of LAPACK's
+//                    V(k) (nau <= 64 in LDS; k_metric_factor_g up to 256) with the stopping rule of LAPACK's
 //                    ?pstrf (stop when the largest remaining diagonal <= tol),
 //                    then f = P (L^-1)^H on the kept columns
 //                    (src/module/df_toroidal.f90:1174-1205); only for the
@@ -144,6 +144,120 @@ __global__ __launch_bounds__(64) void k_metric_factor(const double *__restrict__
     }
 }
 
+// The same factorisation for 64 < n <= 256: one workgroup of 256 threads per k, the
+// working matrix and L^-1 in global memory (Aw, Lw: [k][n][n], row-major per k).
+__global__ __launch_bounds__(256) void k_metric_factor_g(const double *__restrict__ Vre, const double *__restrict__ Vim,
+                                                         int n, const int *__restrict__ mate, double tol,
+                                                         double2 *__restrict__ Aw, double2 *__restrict__ Lw,
+                                                         double *__restrict__ fre, double *__restrict__ fim,
+                                                         int *__restrict__ rank, double *__restrict__ dmin) {
+    __shared__ int perm[256];
+    __shared__ double red_v[256];
+    __shared__ int red_i[256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    if (mate[k] < k) return;
+    const size_t off = (size_t)k * n * n;
+    double2 *A = Aw + off, *Li = Lw + off;              // A[r * n + c]
+    for (int j = 0; j < n; ++j)
+        if (t < n) {
+            const double2 a = {Vre[off + t + (size_t)n * j], Vim[off + t + (size_t)n * j]};
+            const double2 b = {Vre[off + j + (size_t)n * t], Vim[off + j + (size_t)n * t]};
+            A[(size_t)t * n + j] = double2{0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+        }
+    if (t < n) perm[t] = t;
+    __syncthreads();
+    int m = n;
+    for (int j = 0; j < n; ++j) {
+        red_v[t] = (t >= j && t < n) ? A[(size_t)t * n + t].x : -1e300;
+        red_i[t] = t;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) {
+                const double v2 = red_v[t + s];
+                const int i2 = red_i[t + s];
+                if (v2 > red_v[t] || (v2 == red_v[t] && i2 < red_i[t])) {
+                    red_v[t] = v2;
+                    red_i[t] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        const double dmax = red_v[0];
+        const int p = red_i[0];
+        __syncthreads();
+        if (!(dmax > tol)) {
+            m = j;
+            break;
+        }
+        if (p != j) {
+            if (t < n) {
+                const double2 x = A[(size_t)t * n + j];
+                A[(size_t)t * n + j] = A[(size_t)t * n + p];
+                A[(size_t)t * n + p] = x;
+            }
+            __syncthreads();
+            if (t < n) {
+                const double2 x = A[(size_t)j * n + t];
+                A[(size_t)j * n + t] = A[(size_t)p * n + t];
+                A[(size_t)p * n + t] = x;
+            }
+            if (t == 0) {
+                const int x = perm[j];
+                perm[j] = perm[p];
+                perm[p] = x;
+            }
+            __syncthreads();
+        }
+        const double ljj = sqrt(A[(size_t)j * n + j].x);
+        __syncthreads();
+        if (t == j) A[(size_t)j * n + j] = double2{ljj, 0.0};
+        if (t > j && t < n) {
+            const double2 v = A[(size_t)t * n + j];
+            A[(size_t)t * n + j] = double2{v.x / ljj, v.y / ljj};
+        }
+        __syncthreads();
+        if (t > j && t < n) {
+            const double2 lij = A[(size_t)t * n + j];
+            for (int c = j + 1; c < n; ++c) {
+                const double2 u = zmulc(lij, A[(size_t)c * n + j]);
+                const double2 x = A[(size_t)t * n + c];
+                A[(size_t)t * n + c] = double2{x.x - u.x, x.y - u.y};
+            }
+        }
+        __syncthreads();
+    }
+    if (t < m) {
+        for (int i = 0; i < m; ++i) {
+            double2 sacc = {i == t ? 1.0 : 0.0, 0.0};
+            for (int c = t; c < i; ++c) {
+                const double2 u = zmul(A[(size_t)i * n + c], Li[(size_t)c * n + t]);
+                sacc.x -= u.x;
+                sacc.y -= u.y;
+            }
+            const double d = A[(size_t)i * n + i].x;
+            Li[(size_t)i * n + t] = i < t ? double2{0.0, 0.0} : double2{sacc.x / d, sacc.y / d};
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j)
+        if (t < n) {
+            fre[off + t + (size_t)n * j] = 0.0;
+            fim[off + t + (size_t)n * j] = 0.0;
+        }
+    __syncthreads();
+    for (int j = 0; j < m; ++j)
+        if (t < m) {
+            const double2 v = zconj(Li[(size_t)j * n + t]);
+            fre[off + perm[t] + (size_t)n * j] = v.x;
+            fim[off + perm[t] + (size_t)n * j] = v.y;
+        }
+    if (t == 0) {
+        rank[k] = m;
+        const double lm = m > 0 ? A[(size_t)(m - 1) * n + (m - 1)].x : 0.0;
+        dmin[k] = lm * lm;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_metric_mirror(int n, int nk, const int *__restrict__ mate,
                                                        double *__restrict__ fre, double *__restrict__ fim,
                                                        int *__restrict__ rank, double *__restrict__ dmin) {
@@ -163,9 +277,16 @@ __global__ __launch_bounds__(256) void k_metric_mirror(int n, int nk, const int 
 }
 
 hipError_t launch_metric_factor(const double *Vre, const double *Vim, int n, int nk, const int *mate, double tol,
-                                double *fre, double *fim, int *rank, double *dmin, hipStream_t st) {
-    if (n > 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_metric_factor, dim3(nk), dim3(64), 0, st, Vre, Vim, n, mate, tol, fre, fim, rank, dmin);
+                                double2 *work, double *fre, double *fim, int *rank, double *dmin, hipStream_t st) {
+    if (n <= 64) {
+        hipLaunchKernelGGL(k_metric_factor, dim3(nk), dim3(64), 0, st, Vre, Vim, n, mate, tol, fre, fim, rank, dmin);
+    } else if (n <= 256 && work) {
+        double2 *Aw = work, *Lw = work + (size_t)nk * n * n;
+        hipLaunchKernelGGL(k_metric_factor_g, dim3(nk), dim3(256), 0, st, Vre, Vim, n, mate, tol, Aw, Lw, fre, fim,
+                           rank, dmin);
+    } else {
+        return hipErrorInvalidValue;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t tot = (size_t)n * n * nk;
