@@ -44,7 +44,7 @@ struct EriArgs {
     const double *herm;
     const Quartet *q4;
     int nq4;
-    int screen_prefactor;             // kernel prefactor screens (off in the primitive Schwarz pass)
+    int screen_prefactor;             // kernel prefactor screens (shell quartets only, q.prim < 0)
     int screen_prim;                  // primitive Schwarz screen (keyword prim_screen)
     int max_ncomb;                    // largest (lA+1)(lB+1)(lC+1)(lD+1) over the quartets
     int max_blk;                      // largest nA nB nC nD over the quartets
@@ -67,8 +67,11 @@ hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nu
                           const int *pair_rep, int max_prim_pairs, double *V, hipStream_t st);
 hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, const VecOut *vo, int nvo, const double2 *dco,
                                double *ivbuf, double2 *vec, hipStream_t st);
-hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, double Qmax, int nshuc,
-                                int nsh, int ncell, unsigned long long *count, Quartet *out,
+hipError_t launch_schwarz_tables(const Quartet *q4, int nquart, int nshell, const Shell *sh, const ShellPair *sp,
+                                 const double *blk, const int *relc, int nf, int nb, int nc, int nsh, double *Q,
+                                 double *Qsh, PrimPair *pp, unsigned long long *qmax_bits, hipStream_t st);
+hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, const unsigned long long *qmax_bits,
+                                int nshuc, int nsh, int ncell, unsigned long long *count, Quartet *out,
                                 unsigned long long cap, hipStream_t st);
 hipError_t launch_surv(const EriArgs &a, int pass, long long *scount, long long *rcount, const long long *soff,
                        Surv *surv, hipStream_t st);
@@ -77,7 +80,7 @@ hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long 
 hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *order, long long nrows, double *rowval,
                             hipStream_t st);
 size_t contract_lds_bytes(const EriArgs &a);
-hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv, int *sq,
+hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                             long long ns, double *realval, hipStream_t st);
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                               const double *rowval, const double *realval, hipStream_t st);

@@ -101,7 +101,7 @@ struct tcg_ctx {
     DevBuf<ShellPair> d_sp;
     DevBuf<double2> d_vec;
     DevBuf<Quartet> d_q4;
-    DevBuf<unsigned long long> d_cnt;
+    DevBuf<unsigned long long> d_cnt, d_qmax;
     // row pipeline
     DevBuf<long long> d_scount, d_rcount, d_soff, d_roff;
     DevBuf<Surv> d_surv;
@@ -138,12 +138,26 @@ struct tcg_ctx {
     DevBuf<float> d_mlp, d_feat, d_pred;
     DevBuf<int> d_qrule;
     DevBuf<double> d_expo;
+    DevBuf<VecJob> d_jobs;       // pair-vector jobs of the last preparation
+    DevBuf<VecOut> d_vo;
+    DevBuf<double2> d_dco;
+    DevBuf<double> d_iv;
     long long learned_counts[16] = {0};
     bool learned_on = false;
     DevBuf<unsigned char> d_ood;
     DevBuf<float> d_mlpnorm;
     DevBuf<unsigned long long> d_hist;
     bool direct_ready = false;   // d_q4 / d_Q / primitive Schwarz valid for the direct J/K
+    // pinned staging area of the host-to-device uploads: copies leave it
+    // asynchronously on st; a region is reused only after st has drained
+    char *stage = nullptr;
+    size_t stage_cap = 0, stage_used = 0;
+    // inputs the device t-tables were built from (rules, ncap, lattice): the
+    // tables are rebuilt only when these change
+    std::vector<double> tab_key;
+    // second stream: zeroing of the ERI array while the set-up runs on st
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fill = nullptr;
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -161,6 +175,39 @@ struct tcg_ctx {
         hipError_t e_ = (expr);                                                                    \
         if (e_ != hipSuccess) return (ctx)->fail(TCG_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
+
+// v -> b on c->st through the pinned staging area (the host copy into the area
+// is synchronous, so v may change as soon as this returns).  When the area is
+// full the stream is drained and the area restarts at its beginning.
+template <class T>
+static hipError_t upload_async(tcg_ctx *c, DevBuf<T> &b, const std::vector<T> &v) {
+    hipError_t e = hipSuccess;
+    if (!b.p || v.size() > b.n) {
+        e = hipStreamSynchronize(c->st);      // no copy into the old buffer is in flight
+        if (e == hipSuccess) e = b.alloc(v.size());
+        if (e != hipSuccess) return e;
+    }
+    if (v.empty()) return hipSuccess;
+    const size_t bytes = sizeof(T) * v.size();
+    if (c->stage_used + bytes > c->stage_cap) {
+        e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) return e;
+        c->stage_used = 0;
+        if (bytes > c->stage_cap) {
+            if (c->stage) (void)hipHostFree(c->stage);
+            c->stage = nullptr;
+            c->stage_cap = 0;
+            const size_t cap = std::max<size_t>(bytes, size_t(32) << 20);
+            e = hipHostMalloc((void **)&c->stage, cap, hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+            c->stage_cap = cap;
+        }
+    }
+    std::memcpy(c->stage + c->stage_used, v.data(), bytes);
+    e = hipMemcpyAsync(b.p, c->stage + c->stage_used, bytes, hipMemcpyHostToDevice, c->st);
+    c->stage_used += (bytes + 255) & ~size_t(255);
+    return e;
+}
 
 static int lsum3(const int *l) { return l[0] + l[1] + l[2]; }
 
@@ -205,6 +252,13 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
     for (int n : c->rule_nq) nqp = std::max(nqp, ((n + kNodeTile - 1) / kNodeTile) * kNodeTile);
     c->nqp_all = nqp;
     const int nr = (int)c->rule_nq.size(), ndp = c->ndp, ncap = c->ncap, rows = ncap + 5, nh = nqp / kNodeTile;
+    // the tables depend on the rules, ncap and the lattice only; like the
+    // reference's setup_t_tables (clifford_tools.f90:634-636) they are kept
+    // from one build to the next while these stay the same
+    std::vector<double> key = {(double)c->fast, (double)nqp, (double)ncap, (double)ndp, c->a[0], c->a[1], c->a[2]};
+    for (int n : c->rule_nq) key.push_back((double)n);
+    if (key == c->tab_key && c->d_tab.p) return 1;
+    c->tab_key = key;
     tnode.assign((size_t)nr * nqp, 0.0);
     tweight.assign((size_t)nr * nqp, 0.0);
     for (int r = 0; r < nr; ++r) {
@@ -483,9 +537,8 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev3, c->st));
     const int nreal = 3 - A.ndp;
     if (nreal > 0) {
-        HIPCHK(c, c->d_sq.alloc(ns));
         HIPCHK(c, c->d_realval.alloc((size_t)(nr / A.ndp) * nreal * A.nqp));
-        HIPCHK(c, launch_real_rows(A, so, ro, c->d_surv.p, c->d_sq.p, ns, c->d_realval.p, c->st));
+        HIPCHK(c, launch_real_rows(A, so, ro, c->d_surv.p, ns, c->d_realval.p, c->st));
     }
     if (A.Jacc)
         HIPCHK(c, launch_contract_jk(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
@@ -563,9 +616,12 @@ static int run_quartets(tcg_ctx *c, EriArgs A) {
     return 0;
 }
 
-static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, std::vector<double> &out, int prefactor) {
-    HIPCHK(c, c->d_q4.upload(q4));
+// blocks of q4 (each at its q.out) into d_blk; copied to *out unless out is null
+static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, std::vector<double> *out, int prefactor) {
+    HIPCHK(c, upload_async(c, c->d_q4, q4));
+    if (nout > c->d_blk.n) HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, c->d_blk.alloc(nout));
+    HIPCHK(c, launch_fill(c->d_blk.p, nout, 0.0, c->st));     // blocks without survivors stay zero
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
     A.nq4 = (int)q4.size();
@@ -573,8 +629,9 @@ static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, s
     A.screen_prefactor = prefactor;
     int rc = run_quartets(c, A);
     if (rc) return rc;
-    out.resize(nout);
-    HIPCHK(c, hipMemcpyAsync(out.data(), c->d_blk.p, sizeof(double) * nout, hipMemcpyDeviceToHost, c->st));
+    if (!out) return 0;
+    out->resize(nout);
+    HIPCHK(c, hipMemcpyAsync(out->data(), c->d_blk.p, sizeof(double) * nout, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
@@ -586,7 +643,7 @@ static int prepare(tcg_ctx *c) {
     std::vector<double> tab, rtnode, rtw;
     std::vector<int> tablim, tnmax;
     const double tt0 = now_ms();
-    build_tables(c, tab, tablim, tnmax, rtnode, rtw);
+    const bool tables_kept = build_tables(c, tab, tablim, tnmax, rtnode, rtw) == 1;
     c->times[7] = now_ms() - tt0;
     std::vector<VecJob> jobs;
     size_t nvec = 0, niv = 0;
@@ -599,19 +656,21 @@ static int prepare(tcg_ctx *c) {
     c->max_blk = 1;
     for (auto &s : c->sh) c->max_blk = std::max(c->max_blk, s.ncomp);
     c->max_blk = c->max_blk * c->max_blk * c->max_blk * c->max_blk;
-    HIPCHK(c, c->d_tab.upload(tab));
-    HIPCHK(c, c->d_tablim.upload(tablim));
-    HIPCHK(c, c->d_tnmax.upload(tnmax));
-    HIPCHK(c, c->d_tnode.upload(rtnode));
-    HIPCHK(c, c->d_tw.upload(rtw));
-    HIPCHK(c, c->d_rule_nq.upload(c->rule_nq));
-    HIPCHK(c, c->d_sh.upload(c->sh));
-    HIPCHK(c, c->d_lxyz.upload(c->lxyz));
-    HIPCHK(c, c->d_coef.upload(c->coef));
-    HIPCHK(c, c->d_expo.upload(c->expo));
-    HIPCHK(c, c->d_poff.upload(c->poff));
-    HIPCHK(c, c->d_pp.upload(c->pp));
-    HIPCHK(c, c->d_sp.upload(c->sp));
+    if (!tables_kept) {
+        HIPCHK(c, upload_async(c, c->d_tab, tab));
+        HIPCHK(c, upload_async(c, c->d_tablim, tablim));
+        HIPCHK(c, upload_async(c, c->d_tnmax, tnmax));
+        HIPCHK(c, upload_async(c, c->d_tnode, rtnode));
+        HIPCHK(c, upload_async(c, c->d_tw, rtw));
+        HIPCHK(c, upload_async(c, c->d_rule_nq, c->rule_nq));
+    }
+    HIPCHK(c, upload_async(c, c->d_sh, c->sh));
+    HIPCHK(c, upload_async(c, c->d_lxyz, c->lxyz));
+    HIPCHK(c, upload_async(c, c->d_coef, c->coef));
+    HIPCHK(c, upload_async(c, c->d_expo, c->expo));
+    HIPCHK(c, upload_async(c, c->d_poff, c->poff));
+    HIPCHK(c, upload_async(c, c->d_pp, c->pp));
+    HIPCHK(c, upload_async(c, c->d_sp, c->sp));
     {
         // Fourier coefficients d_k of every (job, power pair) on the host, vectors on the GPU
         std::vector<VecOut> vo;
@@ -632,21 +691,17 @@ static int prepare(tcg_ctx *c) {
                     vo.push_back(o);
                 }
         }
-        DevBuf<VecJob> d_jobs;
-        DevBuf<VecOut> d_vo;
-        DevBuf<double2> d_dco;
-        DevBuf<double> d_iv;
-        HIPCHK(c, d_jobs.upload(jobs));
-        HIPCHK(c, d_vo.upload(vo));
-        HIPCHK(c, d_dco.upload(dco));
-        HIPCHK(c, d_iv.alloc(niv));
+        HIPCHK(c, upload_async(c, c->d_jobs, jobs));
+        HIPCHK(c, upload_async(c, c->d_vo, vo));
+        HIPCHK(c, upload_async(c, c->d_dco, dco));
+        if (niv > c->d_iv.n || nvec > c->d_vec.n) HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, c->d_iv.alloc(niv));
         HIPCHK(c, c->d_vec.alloc(nvec));
-        HIPCHK(c, launch_pair_vectors(d_jobs.p, (int)jobs.size(), d_vo.p, (int)vo.size(), d_dco.p, d_iv.p, c->d_vec.p,
-                                      c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, launch_pair_vectors(c->d_jobs.p, (int)jobs.size(), c->d_vo.p, (int)vo.size(), c->d_dco.p, c->d_iv.p,
+                                      c->d_vec.p, c->st));
     }
-    HIPCHK(c, c->d_herm.upload(herm));
-    HIPCHK(c, c->d_relc.upload(c->relc));
+    HIPCHK(c, upload_async(c, c->d_herm, herm));
+    HIPCHK(c, upload_async(c, c->d_relc, c->relc));
     HIPCHK(c, c->d_cnt.alloc(8));
     c->prepared = true;
     return 0;
@@ -819,10 +874,12 @@ static int learned_pass(tcg_ctx *c, EriArgs A) {
 // quartets on the GPU (c->d_q4[0..c->nq4)).  Shared by the stored build and
 // the direct J/K (eri_direct.f90:76-225 does the same set-up).
 static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
-    // ---- 3: Schwarz from the diagonal blocks, replicated over translations ----
+    // ---- 3-4: Schwarz from the diagonal blocks (ERI_toroidal.f90:195-238),
+    // replicated over translations, and the primitive Schwarz table (:300-338):
+    // both lists of diagonal quartets in one pass through the pipeline; the
+    // kernel prefactor screens apply to the shell blocks only (q.prim < 0) ----
     const int nsh = c->nsh, nshuc = c->nshuc, nb = c->nbas, nf = c->nfuc, nc = c->ncell;
     std::vector<Quartet> q4;
-    std::vector<size_t> boff;
     size_t nout = 0;
     for (int SI = 0; SI < nshuc; ++SI)
         for (int SJ = 0; SJ < nsh; ++SJ) {
@@ -832,47 +889,10 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
             q.out = (int)nout;
             q.prim = -1;
             q4.push_back(q);
-            boff.push_back(nout);
             const int m = c->sh[SI].ncomp * c->sh[SJ].ncomp;
             nout += (size_t)m * m;
         }
-    std::vector<double> blocks;
-    int rc = run_blocks(c, q4, nout, blocks, 1);
-    if (rc) return rc;
-    std::vector<double> Q((size_t)nb * nb, 0.0), Qsh((size_t)nsh * nsh, 0.0);
-    for (size_t x = 0; x < q4.size(); ++x) {
-        const int SI = q4[x].s[0], SJ = q4[x].s[1];
-        const int nI = c->sh[SI].ncomp, nJ = c->sh[SJ].ncomp;
-        for (int a = 0; a < nI; ++a)
-            for (int b = 0; b < nJ; ++b) {
-                const double v = std::sqrt(std::fabs(blocks[boff[x] + a + nI * (b + nJ * (a + nI * b))]));
-                const int i = c->sh[SI].first_ao + a, j = c->sh[SJ].first_ao + b;
-                const int li = i % nf, cj = j / nf, lj = j % nf;
-                for (int ctr = 0; ctr < nc; ++ctr) {
-                    // translate (i in cell 0, j) by +ctr
-                    const int r1 = ctr * nf + li;
-                    const int r2 = c->relc[(size_t)c->relc[(size_t)ctr * nc] * nc + cj] * nf + lj;
-                    Q[(size_t)r1 + (size_t)nb * r2] = v;
-                    Q[(size_t)r2 + (size_t)nb * r1] = v;
-                }
-            }
-    }
-    for (int S1 = 0; S1 < nsh; ++S1)
-        for (int S2 = 0; S2 < nsh; ++S2) {
-            double m = 0.0;
-            for (int a = 0; a < c->sh[S1].ncomp; ++a)
-                for (int b = 0; b < c->sh[S2].ncomp; ++b)
-                    m = std::max(m, Q[(size_t)(c->sh[S1].first_ao + a) + (size_t)nb * (c->sh[S2].first_ao + b)]);
-            Qsh[(size_t)S1 + (size_t)nsh * S2] = m;
-        }
-    double Qmax = 0.0;
-    for (double v : Qsh) Qmax = std::max(Qmax, v);
-
-    // ---- 4: primitive Schwarz table, kernel prefactor screens off ----
-    q4.clear();
-    boff.clear();
-    nout = 0;
-    std::vector<int> pidx;
+    const size_t nshell_q = q4.size();
     for (int SI = 0; SI < nshuc; ++SI)
         for (int SJ = 0; SJ < nsh; ++SJ) {
             const int rp = rep_pair(c, SI, SJ);
@@ -884,39 +904,39 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
                 q.out = (int)nout;
                 q.prim = p;
                 q4.push_back(q);
-                boff.push_back(nout);
-                pidx.push_back(s.first + p);
                 const int m = c->sh[SI].ncomp * c->sh[SJ].ncomp;
                 nout += (size_t)m * m;
             }
         }
-    rc = run_blocks(c, q4, nout, blocks, 0);
+    // blocks stay on the GPU; Q, Qsh, the largest Qsh and the primitive ps are built there
+    int rc = run_blocks(c, q4, nout, nullptr, 1);
     if (rc) return rc;
-    for (size_t x = 0; x < q4.size(); ++x) {
-        const int nI = c->sh[q4[x].s[0]].ncomp, nJ = c->sh[q4[x].s[1]].ncomp;
-        double m = 0.0;
-        for (int a = 0; a < nI; ++a)
-            for (int b = 0; b < nJ; ++b) m = std::max(m, std::sqrt(std::fabs(blocks[boff[x] + a + nI * (b + nJ * (a + nI * b))])));
-        c->pp[pidx[x]].ps = m;
-    }
-    HIPCHK(c, c->d_pp.upload(c->pp));
-    HIPCHK(c, c->d_Q.upload(Q));
+    const size_t nbb = (size_t)nb * nb, nss = (size_t)nsh * nsh;
+    if (nbb > c->d_Q.n || nss > c->d_Qsh.n || !c->d_qmax.p) HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, c->d_Q.alloc(nbb));
+    HIPCHK(c, c->d_Qsh.alloc(nss));
+    HIPCHK(c, c->d_qmax.alloc(1));
+    HIPCHK(c, launch_fill(c->d_Q.p, nbb, 0.0, c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_qmax.p, 0, sizeof(unsigned long long), c->st));
+    HIPCHK(c, launch_schwarz_tables(c->d_q4.p, (int)q4.size(), (int)nshell_q, c->d_sh.p, c->d_sp.p, c->d_blk.p,
+                                    c->d_relc.p, nf, nb, nc, nsh, c->d_Q.p, c->d_Qsh.p, c->d_pp.p, c->d_qmax.p, c->st));
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
     t2 = now_ms();
 
-    // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened, on the GPU ----
-    HIPCHK(c, c->d_Qsh.upload(Qsh));
-    HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
-    HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, Qmax, nshuc, nsh, nc, c->d_cnt.p + 3, nullptr, 0,
-                                   c->st));
+    // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened, on the GPU;
+    // appended into the capacity of an earlier enumeration, counted and redone only when it overflows ----
     unsigned long long nq4 = 0;
-    HIPCHK(c, hipMemcpyAsync(&nq4, c->d_cnt.p + 3, sizeof nq4, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (nq4 > (unsigned long long)0x7fffffff) return c->fail(TCG_E_NOMEM, "too many shell quartets (%llu)", nq4);
-    HIPCHK(c, c->d_q4.alloc(nq4));
-    HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
-    HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, Qmax, nshuc, nsh, nc, c->d_cnt.p + 3, c->d_q4.p,
-                                   nq4, c->st));
+    for (int pass = 0; pass < 2; ++pass) {
+        const unsigned long long cap = c->d_q4.p ? c->d_q4.n : 0;
+        HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
+        HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, c->d_qmax.p, nshuc, nsh, nc, c->d_cnt.p + 3,
+                                       cap ? c->d_q4.p : nullptr, cap, c->st));
+        HIPCHK(c, hipMemcpyAsync(&nq4, c->d_cnt.p + 3, sizeof nq4, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (nq4 > (unsigned long long)0x7fffffff) return c->fail(TCG_E_NOMEM, "too many shell quartets (%llu)", nq4);
+        if (nq4 <= cap) break;
+        HIPCHK(c, c->d_q4.alloc(nq4));
+    }
     c->stats[0] = (long long)nq4;
     c->nq4 = (int)nq4;
     t3 = now_ms();
@@ -1061,6 +1081,22 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     if (!c) return TCG_E_ARG;
     if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_build before tcg_set_basis");
     const double t0 = now_ms();
+    const int nb = c->nbas, nf = c->nfuc;
+    const int n1 = sao ? nf : nb;
+    const size_t count = (size_t)n1 * nb * nb * nb;
+    // the ERI array is zeroed on the second stream (after the work queued so far on
+    // st, which may still read it) while the tables, pairs, Schwarz pass and
+    // quartet list are built on st
+    if (!c->st2) {
+        HIPCHK(c, hipStreamCreate(&c->st2));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_fill, hipEventDisableTiming));
+    }
+    HIPCHK(c, c->d_eri.alloc(count));
+    c->cols_ready = false;
+    HIPCHK(c, hipEventRecord(c->ev_fill, c->st));
+    HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fill, 0));
+    HIPCHK(c, launch_fill(c->d_eri.p, count, 0.0, c->st2));
+    HIPCHK(c, hipEventRecord(c->ev_fill, c->st2));
     int rc = prepare(c);
     if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 3 * sizeof(unsigned long long), c->st));
@@ -1069,15 +1105,10 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     double t2 = 0.0, t3 = 0.0;
     rc = setup_quartets(c, t2, t3);
     if (rc) return rc;
-    const int nb = c->nbas, nf = c->nfuc;
     const unsigned long long nq4 = (unsigned long long)c->nq4;
 
     // ---- 6: quartets on the GPU, scattered into the ERI array ----
-    const int n1 = sao ? nf : nb;
-    const size_t count = (size_t)n1 * nb * nb * nb;
-    HIPCHK(c, c->d_eri.alloc(count));
-    c->cols_ready = false;
-    HIPCHK(c, launch_fill(c->d_eri.p, count, 0.0, c->st));
+    HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_fill, 0));
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
     A.nq4 = (int)nq4;
@@ -1527,7 +1558,7 @@ int tcg_eri_blocks(tcg_ctx *c, int n, const int *shells, double *out, long long 
     }
     offsets[n] = (long long)nout;
     std::vector<double> blocks;
-    rc = run_blocks(c, q4, nout, blocks, 1);
+    rc = run_blocks(c, q4, nout, &blocks, 1);
     if (rc) return rc;
     std::copy(blocks.begin(), blocks.end(), out);
     return TCG_OK;
@@ -1634,11 +1665,11 @@ int tcg_fast_quad_self_check(tcg_ctx *c, int natom, const double *xyz, const int
     std::vector<double> b1, b2, V1, V2;
     int rc = prepare(c);
     if (rc) return rc;
-    if ((rc = run_blocks(c, q4, nout, b1, 1))) return rc;
+    if ((rc = run_blocks(c, q4, nout, &b1, 1))) return rc;
     if ((rc = nuclear_rows(c, natom, xyz, Z, V1, nullptr))) return rc;
     if ((rc = tcg_set_quadrature(c, 0, nq_full))) return rc;
     if ((rc = prepare(c))) return rc;
-    if ((rc = run_blocks(c, q4, nout, b2, 1))) return rc;
+    if ((rc = run_blocks(c, q4, nout, &b2, 1))) return rc;
     if ((rc = nuclear_rows(c, natom, xyz, Z, V2, nullptr))) return rc;
     double de = 0.0, dvv = 0.0;
     for (size_t x = 0; x < nout; ++x) de = std::max(de, std::fabs(b1[x] - b2[x]));
@@ -1681,7 +1712,12 @@ int tcg_destroy(tcg_ctx *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->ev3) (void)hipEventDestroy(c->ev3);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
+    if (c->ev_fill) (void)hipEventDestroy(c->ev_fill);
     if (c->st) (void)hipStreamDestroy(c->st);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
+    if (c->stage) (void)hipHostFree(c->stage);
     delete c;
     return TCG_OK;
 }

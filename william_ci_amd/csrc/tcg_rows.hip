@@ -80,7 +80,8 @@ __global__ __launch_bounds__(64) void k_surv(EriArgs A, int pass, long long *__r
                 const PrimPair &pb = A.pp[bp.first + ij];
                 const PrimPair &pk = A.pp[kp.first + kl];
                 fac = (A.fac_const * pb.fac) * pk.fac;
-                if (A.screen_prefactor && fabs(fac) * pb.cmax * pk.cmax <= kScreenTol) ok = false;
+                // prefactor screen: not in the primitive Schwarz table (quartets of one primitive pair)
+                if (A.screen_prefactor && Q.prim < 0 && fabs(fac) * pb.cmax * pk.cmax <= kScreenTol) ok = false;
                 if (A.screen_prim && pb.ps * pk.ps < kPrimTol) ok = false;
             }
             const unsigned long long m = __ballot(ok);
@@ -334,104 +335,119 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
 }
 
 // ---------------------------------------------------------------------------
-// real directions: one thread per (surviving primitive quartet, real direction,
-// node): the McMurchie-Davidson factor of every power combination,
+// real directions: one wavefront per shell quartet, lanes over the nodes, a
+// loop over (surviving primitive quartet, real direction): the
+// McMurchie-Davidson factor of every power combination,
 //   I(t) = sum_N S_N g_N(t),  S_N = sum (-1)^tau E_tau E'_nu,
 //   g_0 = pi sqrt(D) e^{-w R^2}, g_{N+1} = 2 w R g_N - 2 N w g_{N-1}
 // (ERI_shell_block_toroidal.f90:228-236,302-335), written next to the
 // periodic rows of the same quartet: with R = roff[q] / ndp (the quartet's
 // first (group, combination) index) and g the group inside the quartet,
 //   realval[(R * nreal + (g * nreal + dr) * ncomb + combo) * nqp + k].
+// The quartet's shells, rule and nodes are read once; the survivor and pair
+// records of the next step are loaded before the current step's stores.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void k_real_rows(EriArgs A, const long long *__restrict__ soff, const long long *__restrict__ roff,
-                            const Surv *__restrict__ surv, const int *__restrict__ sq, long long ns,
-                            double *__restrict__ realval) {
-    // one workgroup per (survivor, real direction), one thread per node
-    const int nreal = 3 - A.ndp, nqp = A.nqp;
-    const int k = threadIdx.x;
-    if (k >= nqp) return;
-    const int drr = nreal == 1 ? 0 : (int)(blockIdx.x & 1);
-    const long long g = nreal == 1 ? (long long)blockIdx.x : (long long)(blockIdx.x >> 1);
-    const int dr = A.ndp + drr;
-    const int qi = sq[g];
-    const int rule = quartet_rule(A, qi);
-    if (k >= rule_nodes(A, rule)) return;
-    const Quartet *Q = A.q4 + qi;
-    const int lA = A.sh[Q->s[0]].l, lB = A.sh[Q->s[1]].l, lC = A.sh[Q->s[2]].l, lD = A.sh[Q->s[3]].l;
-    const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
-    const int NN = lA + lB + lC + lD;
-    const Surv S = surv[g];
-    const PrimPair &pb = A.pp[S.pb];
-    const PrimPair &pk = A.pp[S.pk];
-    const double p = pb.q, q = pk.q;
-    const double tk = A.tnode[rule * nqp + k], t2 = tk * tk;
-    const double D2 = 1.0 / (p * q + (p + q) * t2);
-    const double w = p * q * t2 * D2;
-    const double rs = 3.14159265358979323846 * sqrt(D2);
-    const double Rr = pb.cen[dr] - (pk.cen[dr] + Q->shift[dr]);
-    double gt[4 * 2 + 1];
-    gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
-    {
-        double gm1 = 0.0, gc = gt[0];
+__global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__restrict__ soff,
+                                                  const long long *__restrict__ roff,
+                                                  const Surv *__restrict__ surv, double *__restrict__ realval) {
+    const int nreal = 3 - A.ndp, nqp = A.nqp, lane = threadIdx.x;
+    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+        const Quartet *Q = A.q4 + qi;
+        const int rule = quartet_rule(A, qi);
+        const int nq = rule_nodes(A, rule);
+        const int lA = A.sh[Q->s[0]].l, lB = A.sh[Q->s[1]].l, lC = A.sh[Q->s[2]].l, lD = A.sh[Q->s[3]].l;
+        const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
+        const int NN = lA + lB + lC + lD;
+        const long long s0 = soff[qi], ns = soff[qi + 1] - s0;
+        const long long R0 = roff[qi] / A.ndp;
+        double shift[3] = {Q->shift[0], Q->shift[1], Q->shift[2]};
+        // nodes of this lane (k = lane, lane + 64)
+        double t2v[2];
 #pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
-            gt[n + 1] = (n < NN) ? gp1 : 0.0;
-            gm1 = gc;
-            gc = gp1;
+        for (int h = 0; h < 2; ++h) {
+            const int k = lane + 64 * h;
+            const double tk = k < nq ? A.tnode[rule * nqp + k] : 0.0;
+            t2v[h] = tk * tk;
         }
-    }
-    const double *Eb = A.herm + pb.eoff + (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
-    const double *Ek = A.herm + pk.eoff + (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
-    const long long gl = g - soff[qi];
-    const long long R0 = roff[qi] / A.ndp;
-    double *out = realval + ((size_t)(R0 * nreal + (gl * nreal + drr) * ncomb)) * nqp + k;
-    for (int combo = 0; combo < ncomb; ++combo) {
-        int c = combo;
-        const int ia = c % (lA + 1); c /= (lA + 1);
-        const int ib = c % (lB + 1); c /= (lB + 1);
-        const int ic = c % (lC + 1);
-        const int id = c / (lC + 1);
-        const double *eb = Eb + (ia * (lB + 1) + ib) * (2 * kMaxL + 1);
-        const double *ek = Ek + (ic * (lD + 1) + id) * (2 * kMaxL + 1);
-        double v = 0.0;
+        const long long nit = ns * nreal;
+        Surv S = ns > 0 ? surv[s0] : Surv{};
+        for (long long it = 0; it < nit; ++it) {
+            const long long gl = it / nreal;
+            const int drr = (int)(it - gl * nreal);
+            const int dr = A.ndp + drr;
+            const PrimPair &pb = A.pp[S.pb];
+            const PrimPair &pk = A.pp[S.pk];
+            const double p = pb.q, q = pk.q;
+            const double Rr = pb.cen[dr] - (pk.cen[dr] + shift[dr]);
+            const double *Eb = A.herm + pb.eoff + (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
+            const double *Ek = A.herm + pk.eoff + (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
+            // the survivor of the next step, loaded ahead
+            if (drr == nreal - 1 && gl + 1 < ns) S = surv[s0 + gl + 1];
+            double *out = realval + ((size_t)(R0 * nreal + (gl * nreal + drr) * ncomb)) * nqp;
 #pragma unroll
-        for (int n = 0; n <= 4; ++n) {
-            if (n > ia + ib) break;
-            const double se = ((n & 1) ? -1.0 : 1.0) * eb[n];
+            for (int h = 0; h < 2; ++h) {
+                const int k = lane + 64 * h;
+                if (k >= nq) break;
+                const double t2 = t2v[h];
+                const double D2 = 1.0 / (p * q + (p + q) * t2);
+                const double w = p * q * t2 * D2;
+                const double rs = 3.14159265358979323846 * sqrt(D2);
+                double gt[4 * 2 + 1];
+                gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
+                {
+                    double gm1 = 0.0, gc = gt[0];
 #pragma unroll
-            for (int m = 0; m <= 4; ++m) {
-                if (m > ic + id) break;
-                v += se * ek[m] * gt[n + m];
+                    for (int n = 0; n < 8; ++n) {
+                        const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
+                        gt[n + 1] = (n < NN) ? gp1 : 0.0;
+                        gm1 = gc;
+                        gc = gp1;
+                    }
+                }
+                for (int combo = 0; combo < ncomb; ++combo) {
+                    int c = combo;
+                    const int ia = c % (lA + 1); c /= (lA + 1);
+                    const int ib = c % (lB + 1); c /= (lB + 1);
+                    const int ic = c % (lC + 1);
+                    const int id = c / (lC + 1);
+                    const double *eb = Eb + (ia * (lB + 1) + ib) * (2 * kMaxL + 1);
+                    const double *ek = Ek + (ic * (lD + 1) + id) * (2 * kMaxL + 1);
+                    double v = 0.0;
+#pragma unroll
+                    for (int n = 0; n <= 4; ++n) {
+                        if (n > ia + ib) break;
+                        const double se = ((n & 1) ? -1.0 : 1.0) * eb[n];
+#pragma unroll
+                        for (int m = 0; m <= 4; ++m) {
+                            if (m > ic + id) break;
+                            v += se * ek[m] * gt[n + m];
+                        }
+                    }
+                    out[(size_t)combo * nqp + k] = v;
+                }
             }
         }
-        out[(size_t)combo * nqp] = v;
     }
 }
 
-__global__ __launch_bounds__(64) void k_surv_quartet(const long long *__restrict__ soff, int nq4, int *__restrict__ sq) {
-    for (int qi = blockIdx.x; qi < nq4; qi += gridDim.x)
-        for (long long g = soff[qi] + threadIdx.x; g < soff[qi + 1]; g += 64) sq[g] = qi;
-}
-
-hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv, int *sq,
+hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                             long long ns, double *realval, hipStream_t st) {
-    if (a.ndp >= 3 || ns <= 0) return hipSuccess;
+    if (a.ndp >= 3 || ns <= 0 || a.nq4 <= 0) return hipSuccess;
     const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
-    hipLaunchKernelGGL(k_surv_quartet, dim3(grid), dim3(64), 0, st, soff, a.nq4, sq);
-    const long long nb = ns * (3 - a.ndp);          // workgroups: (survivor, real direction)
-    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-    const int bs = (a.nqp + 63) / 64 * 64;
-    hipLaunchKernelGGL(k_real_rows, dim3((unsigned)nb), dim3(bs), 0, st, a, soff, roff, surv, sq, ns, realval);
+    hipLaunchKernelGGL(k_real_rows, dim3(grid), dim3(64), 0, st, a, soff, roff, surv, realval);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // contraction, one wavefront per shell quartet: the block
 //   blk(a,b,c,d) = sum_g fac_g cA cB cC cD sum_k w_k Ix Iy Iz
-// accumulated into blk (zeroed by the caller) with LDS atomics, tasks
-// (primitive quartet g, component) spread over the 64 lanes; the node rows
-// (16-byte aligned, nqp a multiple of 16) are read two nodes per load.
+// accumulated into blk (zeroed by the caller) with LDS atomics.  Tasks
+// (primitive quartet g, component) go to groups of 8 lanes, 8 tasks per
+// wavefront at a time; the 8 lanes of a task read consecutive node pairs of
+// its three rows (16-byte loads, a task's 8 lanes cover 128 contiguous bytes
+// of a row), so one load instruction touches 8 cache lines rather than 64,
+// and the partial sums meet by 3 exchanges inside the group.  Node rows are
+// 16-byte aligned (nqp a multiple of 16).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q, int rule, long s0, long ns,
                                               long r0, const Surv *__restrict__ surv,
@@ -447,46 +463,58 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
     const int nq2 = nq >> 1;
     const double2 *tw2 = reinterpret_cast<const double2 *>(tw);
     const int ntask = (int)ns * nblk;             // <= 16^4 primitive quartets x 6^4 components
-    for (int task = lane; task < ntask; task += 64) {
-        const int g = task / nblk;
-        const int comp = task - g * nblk;
-        int t = comp;
-        const int ca = t % nA; t /= nA;
-        const int cb = t % nB; t /= nB;
-        const int cc = t % nC;
-        const int cd = t / nC;
-        const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
-        const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
-        const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
-        const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
-        const double *I3[3];
+    const int sub = lane & 7, slot = lane >> 3;
+    for (int t0 = 0; t0 < ntask; t0 += 8) {
+        const int task = t0 + slot;
+        const bool act = task < ntask;
+        int g = 0, comp = 0, ca = 0, cb = 0, cc = 0, cd = 0;
+        double val = 0.0;
+        if (act) {
+            g = task / nblk;
+            comp = task - g * nblk;
+            int t = comp;
+            ca = t % nA; t /= nA;
+            cb = t % nB; t /= nB;
+            cc = t % nC;
+            cd = t / nC;
+            const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
+            const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
+            const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
+            const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
+            const double *I3[3];
 #pragma unroll
-        for (int dd = 0; dd < 3; ++dd) {
-            const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
-            if (dd < ndp)
-                I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
-            else
-                I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
+            for (int dd = 0; dd < 3; ++dd) {
+                const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
+                if (dd < ndp)
+                    I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
+                else
+                    I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
+            }
+            const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
+            const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
+            const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
+            double v0 = 0.0, v1 = 0.0;
+            for (int k2 = sub; k2 < nq2; k2 += 8) {
+                const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
+                v0 += (x.x * w.x) * y.x * z.x;
+                v1 += (x.y * w.y) * y.y * z.y;
+            }
+            val = v0 + v1;
+            if ((nq & 1) && sub == 0) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
         }
-        const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
-        const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
-        const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
-        double v0 = 0.0, v1 = 0.0;
-        for (int k2 = 0; k2 < nq2; ++k2) {
-            const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
-            v0 += (x.x * w.x) * y.x * z.x;
-            v1 += (x.y * w.y) * y.y * z.y;
+        val += __shfl_xor(val, 1, 64);
+        val += __shfl_xor(val, 2, 64);
+        val += __shfl_xor(val, 4, 64);
+        if (act && sub == 0) {
+            const Surv S = surv[s0 + g];
+            const PrimPair &pb = A.pp[S.pb];
+            const PrimPair &pk = A.pp[S.pk];
+            const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
+            const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
+            const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
+            const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
+            atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
         }
-        double val = v0 + v1;
-        if (nq & 1) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
-        const Surv S = surv[s0 + g];
-        const PrimPair &pb = A.pp[S.pb];
-        const PrimPair &pk = A.pp[S.pk];
-        const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
-        const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
-        const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
-        const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
-        atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
     }
     if (A.counters && lane == 0)
         atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)ncomb * nreal * 2 * nq *
@@ -772,6 +800,12 @@ hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_rebase, dim3((n + 255) / 256), dim3(256), 0, st, src, n, dst);
     return hipGetLastError();
+}
+
+// owner[g] = the shell quartet of survivor g
+__global__ __launch_bounds__(64) void k_surv_quartet(const long long *__restrict__ soff, int nq4, int *__restrict__ sq) {
+    for (int qi = blockIdx.x; qi < nq4; qi += gridDim.x)
+        for (long long g = soff[qi] + threadIdx.x; g < soff[qi + 1]; g += 64) sq[g] = qi;
 }
 
 hipError_t launch_surv_owner(const long long *soff, int nq4, int *owner, hipStream_t st) {

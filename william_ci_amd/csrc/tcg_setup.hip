@@ -131,13 +131,83 @@ hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, const VecOut *vo, 
     return hipGetLastError();
 }
 
+// Schwarz tables from the diagonal blocks (ab|ab) of one pass
+// (src/Toroidal/ERI_toroidal.f90:200-236 and :300-338).  Quartets [0, nshell)
+// are shell blocks (SI in the unit cell, SJ anywhere): Q(i + R, j + R) =
+// sqrt|(ij|ij)| for every lattice translation R, each entry written once from
+// the pair whose first function lies in the unit cell.  The rest are one
+// primitive pair each: ps of that pair = max over components.  One workgroup
+// of 64 per quartet.
+__global__ __launch_bounds__(64) void k_schwarz_q(const Quartet *__restrict__ q4, int nshell, const Shell *__restrict__ sh,
+                                                  const ShellPair *__restrict__ sp, const double *__restrict__ blk,
+                                                  const int *__restrict__ relc, int nf, int nb, int nc,
+                                                  double *__restrict__ Q, PrimPair *__restrict__ pp) {
+    const int x = blockIdx.x;
+    const Quartet q = q4[x];
+    const Shell A = sh[q.s[0]], B = sh[q.s[1]];
+    const int nI = A.ncomp, nJ = B.ncomp;
+    const double *b = blk + q.out;
+    if (x >= nshell) {
+        double m = 0.0;
+        for (int t = threadIdx.x; t < nI * nJ; t += 64) {
+            const int a = t % nI, bb = t / nI;
+            m = fmax(m, sqrt(fabs(b[a + nI * (bb + nJ * (a + nI * bb))])));
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if (threadIdx.x == 0) pp[sp[q.bra].first + q.prim].ps = m;
+        return;
+    }
+    for (int t = threadIdx.x; t < nI * nJ * nc; t += 64) {
+        const int ab = t % (nI * nJ), ctr = t / (nI * nJ);
+        const int a = ab % nI, bb = ab / nI;
+        const double v = sqrt(fabs(b[a + nI * (bb + nJ * (a + nI * bb))]));
+        const int i = A.first_ao + a, j = B.first_ao + bb;
+        const int li = i % nf, cj = j / nf, lj = j % nf;
+        const int r1 = ctr * nf + li;                                   // i moved to cell ctr
+        const int r2 = relc[(size_t)relc[(size_t)ctr * nc] * nc + cj] * nf + lj;   // j moved with it
+        Q[(size_t)r1 + (size_t)nb * r2] = v;
+    }
+}
+
+// Qsh(S1, S2) = max of Q over the two shells' components; *qmax_bits = the
+// largest Qsh as the bit pattern of a non-negative double (ordered as integers).
+__global__ __launch_bounds__(64) void k_schwarz_sh(const Shell *__restrict__ sh, int nsh, int nb, const double *__restrict__ Q,
+                                                   double *__restrict__ Qsh, unsigned long long *__restrict__ qmax_bits) {
+    const long long t = (long long)blockIdx.x * 64 + threadIdx.x;
+    double m = 0.0;
+    if (t < (long long)nsh * nsh) {
+        const int S1 = (int)(t % nsh), S2 = (int)(t / nsh);
+        const Shell A = sh[S1], B = sh[S2];
+        for (int b = 0; b < B.ncomp; ++b)
+            for (int a = 0; a < A.ncomp; ++a) m = fmax(m, Q[(size_t)(A.first_ao + a) + (size_t)nb * (B.first_ao + b)]);
+        Qsh[t] = m;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (threadIdx.x == 0) atomicMax(qmax_bits, (unsigned long long)__double_as_longlong(m));
+}
+
+hipError_t launch_schwarz_tables(const Quartet *q4, int nquart, int nshell, const Shell *sh, const ShellPair *sp,
+                                 const double *blk, const int *relc, int nf, int nb, int nc, int nsh, double *Q,
+                                 double *Qsh, PrimPair *pp, unsigned long long *qmax_bits, hipStream_t st) {
+    if (nquart <= 0 || nsh <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_schwarz_q, dim3(nquart), dim3(64), 0, st, q4, nshell, sh, sp, blk, relc, nf, nb, nc, Q, pp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long long n = (long long)nsh * nsh;
+    hipLaunchKernelGGL(k_schwarz_sh, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, sh, nsh, nb, Q, Qsh, qmax_bits);
+    return hipGetLastError();
+}
+
 // Unique shell quartets (src/Toroidal/ERI_toroidal.f90:342-401): SI in the
 // unit cell, the local indices of SJ, SK, SL not below SI, both Schwarz
 // screens, and (SI,SJ,SK,SL) the smallest member of its orbit under the 8
 // index permutations x the lattice translations.  One thread per candidate
 // (SI, SJ, SK, SL); pass 0 counts (one atomic per wavefront), pass 1 appends.
 __global__ void k_enum_quartets(const Shell *__restrict__ sh, const int *__restrict__ relc,
-                                const double *__restrict__ Qsh, double Qmax, int nshuc, int nsh, int ncell,
+                                const double *__restrict__ Qsh, const unsigned long long *__restrict__ qmax_bits,
+                                int nshuc, int nsh, int ncell,
                                 unsigned long long *__restrict__ count, Quartet *__restrict__ out,
                                 unsigned long long cap) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -151,6 +221,7 @@ __global__ void k_enum_quartets(const Shell *__restrict__ sh, const int *__restr
         keep = sh[SJ].loc >= SI && sh[SK].loc >= SI && sh[SL].loc >= SI;
     }
     if (keep) {
+        const double Qmax = __longlong_as_double((long long)*qmax_bits);
         const double qij = Qsh[(size_t)SI + (size_t)nsh * SJ];
         keep = qij * Qmax >= kSchwarz && qij * Qsh[(size_t)SK + (size_t)nsh * SL] >= kSchwarz;
     }
@@ -189,13 +260,14 @@ __global__ void k_enum_quartets(const Shell *__restrict__ sh, const int *__restr
     out[pos] = Q;
 }
 
-hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, double Qmax, int nshuc,
+hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, const unsigned long long *qmax_bits,
+                                int nshuc,
                                 int nsh, int ncell, unsigned long long *count, Quartet *out,
                                 unsigned long long cap, hipStream_t st) {
     const long long n = (long long)nshuc * nsh * nsh * nsh;
     const long long blocks = (n + 255) / 256;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_enum_quartets, dim3((unsigned)blocks), dim3(256), 0, st, sh, relc, Qsh, Qmax, nshuc, nsh,
+    hipLaunchKernelGGL(k_enum_quartets, dim3((unsigned)blocks), dim3(256), 0, st, sh, relc, Qsh, qmax_bits, nshuc, nsh,
                        ncell, count, out, cap);
     return hipGetLastError();
 }
