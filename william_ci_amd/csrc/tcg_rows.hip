@@ -334,18 +334,86 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
     return hipSuccess;
 }
 
+// LDS writes of the wavefront visible to its other lanes (single-wavefront
+// workgroups; no wait for outstanding global loads)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // ---------------------------------------------------------------------------
-// real directions: one wavefront per shell quartet, lanes over the nodes, a
-// loop over (surviving primitive quartet, real direction): the
-// McMurchie-Davidson factor of every power combination,
-//   I(t) = sum_N S_N g_N(t),  S_N = sum (-1)^tau E_tau E'_nu,
-//   g_0 = pi sqrt(D) e^{-w R^2}, g_{N+1} = 2 w R g_N - 2 N w g_{N-1}
-// (ERI_shell_block_toroidal.f90:228-236,302-335), written next to the
-// periodic rows of the same quartet: with R = roff[q] / ndp (the quartet's
-// first (group, combination) index) and g the group inside the quartet,
+// real directions (ERI_shell_block_toroidal.f90:228-236,302-335): at node t,
+// for a surviving primitive quartet with exponent sums p, q and distance R,
+//   g_0 = pi sqrt(D) e^{-w R^2}, g_{N+1} = 2 w R g_N - 2 N w g_{N-1},
+//   D = 1/(pq + (p+q) t^2), w = pq t^2 D,
+// and the McMurchie-Davidson factor of every power combination
+//   I = sum_{n,m} (-1)^n E_n E'_m g_{n+m},
+// taken as sum_n (-1)^n E_n h_n with h_n = sum_m E'_m g_{n+m} (each h_n
+// shared by the bra combinations of one ket combination).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void real_gt(double p, double q, double t2, double Rr, int NN, double gt[9]) {
+    const double D2 = 1.0 / (p * q + (p + q) * t2);
+    const double w = p * q * t2 * D2;
+    const double rs = 3.14159265358979323846 * sqrt(D2);
+    gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
+    double gm1 = 0.0, gc = gt[0];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
+        gt[n + 1] = (n < NN) ? gp1 : 0.0;
+        gm1 = gc;
+        gc = gp1;
+    }
+}
+
+// store(combo, value) for combo = ia + (lA+1)(ib + (lB+1)(ic + (lC+1) id));
+// Eb, Ek: Hermite E of the bra / ket pair in this direction,
+// [(ia*(lB+1)+ib)*kE + n], n <= 4 used (kE = 2 kMaxL + 1 as stored)
+template <class F>
+__device__ __forceinline__ void real_combos(int lA, int lB, int lC, int lD, const double *__restrict__ Eb,
+                                            const double *__restrict__ Ek, const double gt[9], F &&store,
+                                            int kE = 2 * kMaxL + 1) {
+    const int nab = lA + lB;
+    for (int id = 0; id <= lD; ++id)
+        for (int ic = 0; ic <= lC; ++ic) {
+            const double *ek = Ek + (ic * (lD + 1) + id) * kE;
+            const int ncd = ic + id;
+            double em[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) em[m] = m <= ncd ? ek[m] : 0.0;
+            double h[5];
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                double sacc = 0.0;
+                if (n <= nab) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        if (m <= ncd) sacc += em[m] * gt[n + m];
+                }
+                h[n] = sacc;
+            }
+            const int kc = ic + (lC + 1) * id;
+            for (int ib = 0; ib <= lB; ++ib)
+                for (int ia = 0; ia <= lA; ++ia) {
+                    const double *eb = Eb + (ia * (lB + 1) + ib) * kE;
+                    const int nb2 = ia + ib;
+                    double v = 0.0;
+#pragma unroll
+                    for (int n = 0; n < 5; ++n)
+                        if (n <= nb2) v += ((n & 1) ? -eb[n] : eb[n]) * h[n];
+                    store(ia + (lA + 1) * (ib + (lB + 1) * kc), v);
+                }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// real-direction rows for the contraction of quartets with d shells (the
+// s/p contraction computes them itself): one wavefront per shell quartet,
+// lanes over the nodes, a loop over (surviving primitive quartet, real
+// direction); written next to the periodic rows of the same quartet: with
+// R = roff[q] / ndp (the quartet's first (group, combination) index) and g
+// the group inside the quartet,
 //   realval[(R * nreal + (g * nreal + dr) * ncomb + combo) * nqp + k].
-// The quartet's shells, rule and nodes are read once; the survivor and pair
-// records of the next step are loaded before the current step's stores.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__restrict__ soff,
                                                   const long long *__restrict__ roff,
@@ -360,8 +428,7 @@ __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__
         const int NN = lA + lB + lC + lD;
         const long long s0 = soff[qi], ns = soff[qi + 1] - s0;
         const long long R0 = roff[qi] / A.ndp;
-        double shift[3] = {Q->shift[0], Q->shift[1], Q->shift[2]};
-        // nodes of this lane (k = lane, lane + 64)
+        const double shift[3] = {Q->shift[0], Q->shift[1], Q->shift[2]};
         double t2v[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -370,61 +437,26 @@ __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__
             t2v[h] = tk * tk;
         }
         const long long nit = ns * nreal;
-        Surv S = ns > 0 ? surv[s0] : Surv{};
         for (long long it = 0; it < nit; ++it) {
             const long long gl = it / nreal;
             const int drr = (int)(it - gl * nreal);
             const int dr = A.ndp + drr;
+            const Surv S = surv[s0 + gl];
             const PrimPair &pb = A.pp[S.pb];
             const PrimPair &pk = A.pp[S.pk];
-            const double p = pb.q, q = pk.q;
             const double Rr = pb.cen[dr] - (pk.cen[dr] + shift[dr]);
             const double *Eb = A.herm + pb.eoff + (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
             const double *Ek = A.herm + pk.eoff + (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
-            // the survivor of the next step, loaded ahead
-            if (drr == nreal - 1 && gl + 1 < ns) S = surv[s0 + gl + 1];
             double *out = realval + ((size_t)(R0 * nreal + (gl * nreal + drr) * ncomb)) * nqp;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = lane + 64 * h;
-                if (k >= nq) break;
-                const double t2 = t2v[h];
-                const double D2 = 1.0 / (p * q + (p + q) * t2);
-                const double w = p * q * t2 * D2;
-                const double rs = 3.14159265358979323846 * sqrt(D2);
-                double gt[4 * 2 + 1];
-                gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
-                {
-                    double gm1 = 0.0, gc = gt[0];
-#pragma unroll
-                    for (int n = 0; n < 8; ++n) {
-                        const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
-                        gt[n + 1] = (n < NN) ? gp1 : 0.0;
-                        gm1 = gc;
-                        gc = gp1;
-                    }
-                }
-                for (int combo = 0; combo < ncomb; ++combo) {
-                    int c = combo;
-                    const int ia = c % (lA + 1); c /= (lA + 1);
-                    const int ib = c % (lB + 1); c /= (lB + 1);
-                    const int ic = c % (lC + 1);
-                    const int id = c / (lC + 1);
-                    const double *eb = Eb + (ia * (lB + 1) + ib) * (2 * kMaxL + 1);
-                    const double *ek = Ek + (ic * (lD + 1) + id) * (2 * kMaxL + 1);
-                    double v = 0.0;
-#pragma unroll
-                    for (int n = 0; n <= 4; ++n) {
-                        if (n > ia + ib) break;
-                        const double se = ((n & 1) ? -1.0 : 1.0) * eb[n];
-#pragma unroll
-                        for (int m = 0; m <= 4; ++m) {
-                            if (m > ic + id) break;
-                            v += se * ek[m] * gt[n + m];
-                        }
-                    }
-                    out[(size_t)combo * nqp + k] = v;
-                }
+                if (h * 64 >= nq) break;
+                double gt[9];
+                real_gt(pb.q, pk.q, t2v[h], Rr, NN, gt);
+                real_combos(lA, lB, lC, lD, Eb, Ek, gt, [&](int combo, double v) {
+                    if (k < nq) out[(size_t)combo * nqp + k] = v;
+                });
             }
         }
     }
@@ -441,21 +473,31 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
 // ---------------------------------------------------------------------------
 // contraction, one wavefront per shell quartet: the block
 //   blk(a,b,c,d) = sum_g fac_g cA cB cC cD sum_k w_k Ix Iy Iz
-// accumulated into blk (zeroed by the caller) with LDS atomics.  Tasks
-// (primitive quartet g, component) go to groups of 8 lanes, 8 tasks per
-// wavefront at a time; the 8 lanes of a task read consecutive node pairs of
-// its three rows (16-byte loads, a task's 8 lanes cover 128 contiguous bytes
-// of a row), so one load instruction touches 8 cache lines rather than 64,
-// and the partial sums meet by 3 exchanges inside the group.  Node rows are
-// 16-byte aligned (nqp a multiple of 16).
+// accumulated into blk (zeroed by the caller) with LDS atomics, one lane per
+// task (primitive quartet g, component).  Nothing in a task waits on a chain
+// of loads: the Cartesian powers of the four shells are packed into
+// one word per shell up front, the row addresses follow from (g, component),
+// and the primitive indices from the pair index (pairs of a shell pair are
+// ordered i major), so only the survivor record precedes the coefficient
+// loads.  Node rows are 16-byte aligned (nqp a multiple of 16) and read two
+// nodes per load.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long shell_powers(const EriArgs &A, const Shell &S) {
+    unsigned long long w = 0;
+    for (int c = 0; c < S.ncomp; ++c) {
+        const int *p = A.ao_lxyz + 3 * (S.first_ao + c);
+        w |= (unsigned long long)(p[0] | (p[1] << 2) | (p[2] << 4)) << (6 * c);
+    }
+    return w;
+}
+
 __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q, int rule, long s0, long ns,
                                               long r0, const Surv *__restrict__ surv,
                                               const double *__restrict__ rowval, const double *__restrict__ realval,
                                               double *blk, int lane) {
     const int nqp = A.nqp, nq = rule_nodes(A, rule), ndp = A.ndp, nreal = 3 - ndp;
     const double *tw = A.tweight + (size_t)rule * nqp;
-    const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
+    const Shell sA = A.sh[Q.s[0]], sB = A.sh[Q.s[1]], sC = A.sh[Q.s[2]], sD = A.sh[Q.s[3]];
     const int lA = sA.l, lB = sB.l, lC = sC.l, lD = sD.l;
     const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
     const int ncomb = (lA + 1) * (lB + 1) * (lC + 1) * (lD + 1);
@@ -463,58 +505,51 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
     const int nq2 = nq >> 1;
     const double2 *tw2 = reinterpret_cast<const double2 *>(tw);
     const int ntask = (int)ns * nblk;             // <= 16^4 primitive quartets x 6^4 components
-    const int sub = lane & 7, slot = lane >> 3;
-    for (int t0 = 0; t0 < ntask; t0 += 8) {
-        const int task = t0 + slot;
-        const bool act = task < ntask;
-        int g = 0, comp = 0, ca = 0, cb = 0, cc = 0, cd = 0;
-        double val = 0.0;
-        if (act) {
-            g = task / nblk;
-            comp = task - g * nblk;
-            int t = comp;
-            ca = t % nA; t /= nA;
-            cb = t % nB; t /= nB;
-            cc = t % nC;
-            cd = t / nC;
-            const int *pa = A.ao_lxyz + 3 * (sA.first_ao + ca);
-            const int *pbw = A.ao_lxyz + 3 * (sB.first_ao + cb);
-            const int *pc = A.ao_lxyz + 3 * (sC.first_ao + cc);
-            const int *pd = A.ao_lxyz + 3 * (sD.first_ao + cd);
-            const double *I3[3];
+    const unsigned long long pwA = shell_powers(A, sA), pwB = shell_powers(A, sB), pwC = shell_powers(A, sC),
+                             pwD = shell_powers(A, sD);
+    const int pb0 = A.sp[Q.bra].first, pk0 = A.sp[Q.ket].first;   // first primitive pair of bra / ket
+    const int npB = sB.nprim, npD = sD.nprim;
+    const long R0 = r0 / ndp;
+    for (int task = lane; task < ntask; task += 64) {
+        const int g = task / nblk;
+        const int cp = task - g * nblk;
+        const Surv S = surv[s0 + g];
+        int t = cp;
+        const int ca = t % nA; t /= nA;
+        const int cb = t % nB; t /= nB;
+        const int cc = t % nC;
+        const int cd = t / nC;
+        const unsigned pa = (unsigned)(pwA >> (6 * ca)), pbw = (unsigned)(pwB >> (6 * cb)),
+                       pc = (unsigned)(pwC >> (6 * cc)), pd = (unsigned)(pwD >> (6 * cd));
+        const double *I3[3];
 #pragma unroll
-            for (int dd = 0; dd < 3; ++dd) {
-                const int cx = pa[dd] + (lA + 1) * (pbw[dd] + (lB + 1) * (pc[dd] + (lC + 1) * pd[dd]));
-                if (dd < ndp)
-                    I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
-                else
-                    I3[dd] = realval + ((size_t)((r0 / ndp) * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
-            }
-            const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
-            const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
-            const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
-            double v0 = 0.0, v1 = 0.0;
-            for (int k2 = sub; k2 < nq2; k2 += 8) {
-                const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
-                v0 += (x.x * w.x) * y.x * z.x;
-                v1 += (x.y * w.y) * y.y * z.y;
-            }
-            val = v0 + v1;
-            if ((nq & 1) && sub == 0) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
+        for (int dd = 0; dd < 3; ++dd) {
+            const int sh = 2 * dd;
+            const int cx = ((pa >> sh) & 3) + (lA + 1) * (((pbw >> sh) & 3) +
+                           (lB + 1) * (((pc >> sh) & 3) + (lC + 1) * ((pd >> sh) & 3)));
+            if (dd < ndp)
+                I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
+            else
+                I3[dd] = realval + ((size_t)(R0 * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
         }
-        val += __shfl_xor(val, 1, 64);
-        val += __shfl_xor(val, 2, 64);
-        val += __shfl_xor(val, 4, 64);
-        if (act && sub == 0) {
-            const Surv S = surv[s0 + g];
-            const PrimPair &pb = A.pp[S.pb];
-            const PrimPair &pk = A.pp[S.pk];
-            const double cA = A.coef[A.ao_poff[sA.first_ao + ca] + pb.pi];
-            const double cB = A.coef[A.ao_poff[sB.first_ao + cb] + pb.pj];
-            const double cC = A.coef[A.ao_poff[sC.first_ao + cc] + pk.pi];
-            const double cD = A.coef[A.ao_poff[sD.first_ao + cd] + pk.pj];
-            atomicAdd(&blk[comp], S.fac * cA * cB * cC * cD * val);
+        const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
+        const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
+        const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
+        double v0 = 0.0, v1 = 0.0;
+        for (int k2 = 0; k2 < nq2; ++k2) {
+            const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
+            v0 += (x.x * w.x) * y.x * z.x;
+            v1 += (x.y * w.y) * y.y * z.y;
         }
+        double val = v0 + v1;
+        if (nq & 1) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
+        const int ib = S.pb - pb0, ik = S.pk - pk0;
+        const int pi = ib / npB, pj = ib - pi * npB, qk = ik / npD, ql = ik - qk * npD;
+        const double cA = A.coef[sA.poff + ca * sA.nprim + pi];
+        const double cB = A.coef[sB.poff + cb * sB.nprim + pj];
+        const double cC = A.coef[sC.poff + cc * sC.nprim + qk];
+        const double cD = A.coef[sD.poff + cd * sD.nprim + ql];
+        atomicAdd(&blk[cp], S.fac * cA * cB * cC * cD * val);
     }
     if (A.counters && lane == 0)
         atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)ncomb * nreal * 2 * nq *
@@ -537,6 +572,34 @@ __device__ __forceinline__ void stored_position(const EriArgs &A, const int f4[4
 // written out (Schwarz passes) or scattered to the 8 permutations translated
 // to the unit cell (src/Toroidal/ERI_toroidal.f90:416-454).
 // ---------------------------------------------------------------------------
+// the finished block: written out (Schwarz passes) or scattered to the 8
+// permutations translated to the unit cell
+__device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, const Shell &sA, const Shell &sB,
+                                            const Shell &sC, const Shell &sD, const double *blk, int lane) {
+    const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
+    const int nblk = nA * nB * nC * nD;
+    if (A.blk_out && Q.out >= 0) {
+        for (int i = lane; i < nblk; i += 64) A.blk_out[Q.out + i] = blk[i];
+    } else if (A.eri) {
+        const size_t nb = A.nbas, n1 = A.n1;
+        for (int task = lane; task < nblk * 8; task += 64) {
+            const int comp = task >> 3, ip = task & 7;
+            int t = comp;
+            const int ca = t % nA; t /= nA;
+            const int cb = t % nB; t /= nB;
+            const int cc = t % nC;
+            const int cd = t / nC;
+            const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
+            if (A.Qao[(size_t)f4[0] + nb * f4[1]] * A.Qao[(size_t)f4[2] + nb * f4[3]] < kSchwarz) continue;
+            int pp[4];
+            stored_position(A, f4, ip, pp);
+            const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * (size_t)pp[3]))
+                                           : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * (size_t)pp[3]));
+            A.eri[idx] = blk[comp];
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__restrict__ soff,
                                                  const long long *__restrict__ roff, const Surv *__restrict__ surv,
                                                  const double *__restrict__ rowval,
@@ -546,34 +609,14 @@ __global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__r
     for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
         const Quartet Q = A.q4[qi];
         const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
-        const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
-        const int nblk = nA * nB * nC * nD;
+        const int nblk = sA.ncomp * sB.ncomp * sC.ncomp * sD.ncomp;
         const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
         __syncthreads();
         for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
         __syncthreads();
         quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, lane);
         __syncthreads();
-        if (A.blk_out && Q.out >= 0) {
-            for (int i = lane; i < nblk; i += 64) A.blk_out[Q.out + i] = blk[i];
-        } else if (A.eri) {
-            const size_t nb = A.nbas, n1 = A.n1;
-            for (int task = lane; task < nblk * 8; task += 64) {
-                const int comp = task >> 3, ip = task & 7;
-                int t = comp;
-                const int ca = t % nA; t /= nA;
-                const int cb = t % nB; t /= nB;
-                const int cc = t % nC;
-                const int cd = t / nC;
-                const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
-                if (A.Qao[(size_t)f4[0] + nb * f4[1]] * A.Qao[(size_t)f4[2] + nb * f4[3]] < kSchwarz) continue;
-                int pp[4];
-                stored_position(A, f4, ip, pp);
-                const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * (size_t)pp[3]))
-                                               : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * (size_t)pp[3]));
-                A.eri[idx] = blk[comp];
-            }
-        }
+        write_block(A, Q, sA, sB, sC, sD, blk, lane);
     }
 }
 
@@ -619,10 +662,6 @@ __device__ __forceinline__ bool first_generator(const EriArgs &A, const Shell &s
     return true;
 }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long *__restrict__ soff,
                                                      const long long *__restrict__ roff,
@@ -757,12 +796,12 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                            const double *rowval, const double *realval, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
+    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
     const size_t lds = contract_lds_bytes(a);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_contract, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
     hipLaunchKernelGGL(k_contract, dim3(grid), dim3(64), lds, st, a, soff, roff, surv, rowval, realval);
     return hipGetLastError();
 }

@@ -34,65 +34,94 @@ __device__ double d_i0_scaled(double x) {
     return sum / sqrt(2.0 * 3.14159265358979323846 * x);
 }
 
-// exp(-A) I_n(A), n = 0..N+lA+lB, one thread per (primitive pair, periodic
-// direction), by Miller's recurrence started at M = N + lA + lB with
-// I_{M+1} = 0 (the reference's bessel_scaled_vector_fast), rescaled against
-// overflow and normalised to exp(-A) I_0(A).
-__global__ void k_bessel_jobs(const VecJob *__restrict__ jobs, int njobs, double *__restrict__ ivbuf) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= njobs) return;
-    const VecJob J = jobs[j];
-    const int M = J.N + J.lA + J.lB;
-    double *iv = ivbuf + J.ivoff;
-    if (J.A < 1e-10) {
-        for (int n = 0; n <= M; ++n) iv[n] = 0.0;
-        iv[0] = 1.0;
-        return;
+// exp(-A) I_n(A), n = 0..M, M = N + lA + lB, one wavefront per (primitive
+// pair, periodic direction), by Miller's recurrence
+//   w_{k-1} = w_{k+1} + (2k/A) w_k,  w_{M+1} = 0,  w_M = 1
+// (the reference's bessel_scaled_vector_fast start), normalised to
+// exp(-A) I_0(A).  The M steps are split into 64 consecutive segments, one
+// per lane, top segment at lane 0: each lane forms the 2x2 transfer matrix of
+// its segment, (w_{lo}, w_{lo+1}) = T (w_{hi}, w_{hi+1}); an inclusive scan
+// of these products over the lanes (scaled by powers of two, exponent kept
+// apart) gives every segment its starting pair, and each lane then runs its
+// own segment.  Inside one segment the values grow by less than
+// (1 + 2M/A)^ceil(M/64), far inside the double range for the M = nmax(A) + l
+// used here, so only the scan needs the exponents.
+struct Mat2 {
+    double a, b, c, d;
+    int e;                                       // value = [[a b] [c d]] * 2^e
+};
+
+__device__ __forceinline__ Mat2 mat2_mul(const Mat2 &P, const Mat2 &Q) {
+    Mat2 R{P.a * Q.a + P.b * Q.c, P.a * Q.b + P.b * Q.d, P.c * Q.a + P.d * Q.c, P.c * Q.b + P.d * Q.d, P.e + Q.e};
+    const double m = fmax(fmax(fabs(R.a), fabs(R.b)), fmax(fabs(R.c), fabs(R.d)));
+    if (m > 0.0) {
+        int ex;
+        frexp(m, &ex);
+        R.a = ldexp(R.a, -ex);
+        R.b = ldexp(R.b, -ex);
+        R.c = ldexp(R.c, -ex);
+        R.d = ldexp(R.d, -ex);
+        R.e += ex;
     }
-    const double r = 2.0 / J.A;
-    double up = 0.0, cur = 1e-280;     // I_{k+1}, I_k
-    iv[M] = cur;
-    // A rescale by 1e-250 at index m applies to iv[m..M]; the positions are kept and the
-    // factors applied in the final pass (the same multiplications in the same order as
-    // rescaling the stored values at once), up to kMaxRescale of them.
-    constexpr int kMaxRescale = 48;
-    int rpos[kMaxRescale];
-    int nres = 0;
-    for (int k = M; k >= 1; --k) {
-        const double dn = up + (double)k * r * cur;
-        iv[k - 1] = dn;
-        up = cur;
-        cur = dn;
-        if (dn > 1e250) {
-            if (nres < kMaxRescale) {
-                rpos[nres++] = k - 1;
-            } else {
-                for (int m = k - 1; m <= M; ++m) iv[m] *= 1e-250;
-            }
-            up *= 1e-250;
-            cur *= 1e-250;
+    return R;
+}
+
+__device__ __forceinline__ Mat2 mat2_shfl_up(const Mat2 &P, int delta) {
+    return Mat2{__shfl_up(P.a, delta, 64), __shfl_up(P.b, delta, 64), __shfl_up(P.c, delta, 64),
+                __shfl_up(P.d, delta, 64), __shfl_up(P.e, delta, 64)};
+}
+
+__global__ __launch_bounds__(64) void k_bessel_jobs(const VecJob *__restrict__ jobs, int njobs, double *__restrict__ ivbuf) {
+    const int lane = threadIdx.x;
+    for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+        const VecJob J = jobs[jb];
+        const int M = J.N + J.lA + J.lB;
+        double *iv = ivbuf + J.ivoff;
+        if (J.A < 1e-10) {
+            for (int n = lane; n <= M; n += 64) iv[n] = n == 0 ? 1.0 : 0.0;
+            continue;
         }
-    }
-    // a value at index n takes the rescales recorded at positions <= n; positions were
-    // recorded in decreasing order, so those are rpos[q..nres)
-    double v0 = iv[0];
-    for (int t = 0; t < nres; ++t)
-        if (rpos[t] <= 0) v0 *= 1e-250;
-    const double s = d_i0_scaled(J.A) / v0;
-    // sixteen loads in flight at a time (the pass would otherwise wait on every load)
-    int q = nres;
-    for (int n0 = 0; n0 <= M; n0 += 16) {
-        double v[16];
+        const double r = 2.0 / J.A;
+        const int seg = (M + 63) / 64;
+        const int khi = M - lane * seg, klo = khi - seg > 0 ? khi - seg : 0;
+        // transfer matrix of this lane's segment
+        Mat2 T{1.0, 0.0, 0.0, 1.0, 0};
+        for (int k = khi; k > klo; --k) {
+            const double ak = (double)k * r;
+            const Mat2 n{ak * T.a + T.c, ak * T.b + T.d, T.a, T.b, 0};
+            T = n;
+        }
+        T = mat2_mul(T, Mat2{1.0, 0.0, 0.0, 1.0, 0});          // scaled, exponent apart
+        // inclusive scan, lane j: T_j T_{j-1} ... T_0
+        Mat2 P = T;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = n0 + j <= M ? iv[n0 + j] : 0.0;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int n = n0 + j;
-            if (n <= M) {
-                while (q > 0 && rpos[q - 1] <= n) --q;
-                double x = v[j];
-                for (int t = q; t < nres; ++t) x *= 1e-250;
-                iv[n] = x * s;
+        for (int dl = 1; dl < 64; dl <<= 1) {
+            const Mat2 Q = mat2_shfl_up(P, dl);
+            if (lane >= dl) P = mat2_mul(P, Q);
+        }
+        Mat2 S = mat2_shfl_up(P, 1);
+        if (lane == 0) S = Mat2{1.0, 0.0, 0.0, 1.0, 0};
+        // walk: (w_khi, w_khi+1) = S (1, 0); the lane covering k = 1 ends at w_0
+        const int jbot = (M - 1) / seg;
+        double u0 = S.a, u1 = S.c;
+        for (int k = khi; k > klo; --k) {
+            const double dn = u1 + (double)k * r * u0;
+            u1 = u0;
+            u0 = dn;
+        }
+        const double w0 = __shfl(u0, jbot, 64);
+        const int e0 = __shfl(S.e, jbot, 64);
+        const double sc = d_i0_scaled(J.A) / w0;
+        const int de = S.e - e0;
+        if (khi >= 1) {
+            u0 = S.a;
+            u1 = S.c;
+            iv[khi] = ldexp(u0 * sc, de);
+            for (int k = khi; k > klo; --k) {
+                const double dn = u1 + (double)k * r * u0;
+                u1 = u0;
+                u0 = dn;
+                if (k - 1 > klo || klo == 0) iv[k - 1] = ldexp(dn * sc, de);
             }
         }
     }
@@ -124,7 +153,7 @@ __global__ void k_pair_combine(const VecOut *__restrict__ vo, const VecJob *__re
 hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, const VecOut *vo, int nvo, const double2 *dco,
                                double *ivbuf, double2 *vec, hipStream_t st) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bessel_jobs, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, ivbuf);
+    hipLaunchKernelGGL(k_bessel_jobs, dim3(njobs < 65536 ? njobs : 65536), dim3(64), 0, st, jobs, njobs, ivbuf);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nvo <= 0) return e;
     hipLaunchKernelGGL(k_pair_combine, dim3(nvo), dim3(128), 0, st, vo, jobs, ivbuf, dco, vec);
