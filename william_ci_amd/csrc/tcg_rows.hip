@@ -59,11 +59,11 @@ __device__ __forceinline__ void quartet_shape(const EriArgs &A, const Quartet &Q
 // ---------------------------------------------------------------------------
 // survivors: pass 0 counts (scount, rcount), pass 1 writes at soff
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_surv(EriArgs A, int pass, long long *__restrict__ scount,
+__global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__restrict__ scount,
                                              long long *__restrict__ rcount, const long long *__restrict__ soff,
                                              Surv *__restrict__ surv) {
-    const int lane = threadIdx.x;
-    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);     // four wavefronts, one quartet each
+    for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
         const Quartet Q = A.q4[qi];
         const ShellPair bp = A.sp[Q.bra], kp = A.sp[Q.ket];
         const long ncand = (Q.prim >= 0) ? 1 : (long)bp.count * kp.count;
@@ -107,20 +107,21 @@ __global__ __launch_bounds__(64) void k_surv(EriArgs A, int pass, long long *__r
 hipError_t launch_surv(const EriArgs &a, int pass, long long *scount, long long *rcount, const long long *soff,
                        Surv *surv, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
-    hipLaunchKernelGGL(k_surv, dim3(grid), dim3(64), 0, st, a, pass, scount, rcount, soff, surv);
+    const int nwg = (a.nq4 + 3) / 4;
+    const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
+    hipLaunchKernelGGL(k_surv, dim3(grid), dim3(256), 0, st, a, pass, scount, rcount, soff, surv);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // rows
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_rows_emit(EriArgs A, const long long *__restrict__ soff,
+__global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *__restrict__ soff,
                                                   const long long *__restrict__ roff, const Surv *__restrict__ surv,
                                                   RowRec *__restrict__ rows, unsigned *__restrict__ keys,
                                                   int *__restrict__ vals) {
-    const int lane = threadIdx.x;
-    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);     // four wavefronts, one quartet each
+    for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
         const Quartet Q = A.q4[qi];
         int l[4], ncomb;
         quartet_shape(A, Q, l, ncomb);
@@ -159,22 +160,23 @@ __global__ __launch_bounds__(64) void k_rows_emit(EriArgs A, const long long *__
 hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                             RowRec *rows, unsigned *keys, int *vals, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
-    hipLaunchKernelGGL(k_rows_emit, dim3(grid), dim3(64), 0, st, a, soff, roff, surv, rows, keys, vals);
+    const int nwg = (a.nq4 + 3) / 4;
+    const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
+    hipLaunchKernelGGL(k_rows_emit, dim3(grid), dim3(256), 0, st, a, soff, roff, surv, rows, keys, vals);
     return hipGetLastError();
 }
 
 // one wavefront per tile of 16 sorted rows; several tiles per wavefront.
 // kNT = node tiles of 16 (2 for the 32-node rule, 3 for 40 nodes).
 template <int kNT>
-__global__ __launch_bounds__(64) void k_rows_mfma(EriArgs A, const RowRec *__restrict__ rows,
+__global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__restrict__ rows,
                                                   const int *__restrict__ order, long long nrows, int hbase,
                                                   double *__restrict__ rowval) {
-    const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nqp = A.nqp, nht = A.nqp / kNodeTile;
     const int tabrows = A.ncap + 5;
     const long long ntiles = (nrows + 15) / 16;
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (long long tile = (long long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long long)gridDim.x * 4) {
         const long long pos = tile * 16 + r;
         const bool valid = pos < nrows;
         int ridx = -1, nm = -1, d = 0x7fffffff;         // d: direction | rule << 8
@@ -318,15 +320,16 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
                             hipStream_t st) {
     if (nrows <= 0) return hipSuccess;
     const long long ntiles = (nrows + 15) / 16;
-    const int grid = ntiles < (1 << 20) ? (int)ntiles : (1 << 20);
+    const long long nwg = (ntiles + 3) / 4;                 // four wavefronts per workgroup, a tile each
+    const int grid = nwg < (1 << 20) ? (int)nwg : (1 << 20);
     const int nht = a.nqp / kNodeTile;
     // up to four node tiles per launch (more nodes: one launch per group of four)
     for (int h0 = 0; h0 < nht; h0 += 4) {
         switch (nht - h0 < 4 ? nht - h0 : 4) {
-        case 1: hipLaunchKernelGGL(k_rows_mfma<1>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
-        case 2: hipLaunchKernelGGL(k_rows_mfma<2>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
-        case 3: hipLaunchKernelGGL(k_rows_mfma<3>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
-        default: hipLaunchKernelGGL(k_rows_mfma<4>, dim3(grid), dim3(64), 0, st, a, rows, order, nrows, h0, rowval); break;
+        case 1: hipLaunchKernelGGL(k_rows_mfma<1>, dim3(grid), dim3(256), 0, st, a, rows, order, nrows, h0, rowval); break;
+        case 2: hipLaunchKernelGGL(k_rows_mfma<2>, dim3(grid), dim3(256), 0, st, a, rows, order, nrows, h0, rowval); break;
+        case 3: hipLaunchKernelGGL(k_rows_mfma<3>, dim3(grid), dim3(256), 0, st, a, rows, order, nrows, h0, rowval); break;
+        default: hipLaunchKernelGGL(k_rows_mfma<4>, dim3(grid), dim3(256), 0, st, a, rows, order, nrows, h0, rowval); break;
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -334,8 +337,8 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
     return hipSuccess;
 }
 
-// LDS writes of the wavefront visible to its other lanes (single-wavefront
-// workgroups; no wait for outstanding global loads)
+// LDS writes of the wavefront visible to its other lanes (for LDS that one
+// wavefront owns; no wait for outstanding global loads)
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -472,15 +475,18 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
 
 // ---------------------------------------------------------------------------
 // contraction, one wavefront per shell quartet: the block
-//   blk(a,b,c,d) = sum_g fac_g cA cB cC cD sum_k w_k Ix Iy Iz
-// accumulated into blk (zeroed by the caller) with LDS atomics, one lane per
-// task (primitive quartet g, component).  Nothing in a task waits on a chain
-// of loads: the Cartesian powers of the four shells are packed into
-// one word per shell up front, the row addresses follow from (g, component),
-// and the primitive indices from the pair index (pairs of a shell pair are
-// ordered i major), so only the survivor record precedes the coefficient
-// loads.  Node rows are 16-byte aligned (nqp a multiple of 16) and read two
-// nodes per load.
+//   blk(a,b,c,d) = sum_g fac_g cA cB cC cD sum_k w_k Ix Iy Iz.
+// Each lane owns one component and a residue class of the surviving
+// primitive quartets g: with fewer than 64 components, G = 64 / ncomp lanes
+// share a component (g = lane / ncomp, + G, ...), their sums meet through
+// part[] (64 doubles of LDS per wavefront) in a fixed order; with 64 or more,
+// lanes loop over the components and every g.  No atomics: the block comes
+// out the same on every run.  Nothing in a task waits on a chain of loads:
+// the Cartesian powers of the four shells are packed into one word per shell
+// up front, the row addresses follow from (g, component), and the primitive
+// indices from the pair index (pairs of a shell pair are ordered i major).
+// Node rows are 16-byte aligned (nqp a multiple of 16) and read two nodes per
+// load.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long shell_powers(const EriArgs &A, const Shell &S) {
     unsigned long long w = 0;
@@ -494,7 +500,7 @@ __device__ __forceinline__ unsigned long long shell_powers(const EriArgs &A, con
 __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q, int rule, long s0, long ns,
                                               long r0, const Surv *__restrict__ surv,
                                               const double *__restrict__ rowval, const double *__restrict__ realval,
-                                              double *blk, int lane) {
+                                              double *blk, double *part, int lane) {
     const int nqp = A.nqp, nq = rule_nodes(A, rule), ndp = A.ndp, nreal = 3 - ndp;
     const double *tw = A.tweight + (size_t)rule * nqp;
     const Shell sA = A.sh[Q.s[0]], sB = A.sh[Q.s[1]], sC = A.sh[Q.s[2]], sD = A.sh[Q.s[3]];
@@ -504,16 +510,13 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
     const int nblk = nA * nB * nC * nD;
     const int nq2 = nq >> 1;
     const double2 *tw2 = reinterpret_cast<const double2 *>(tw);
-    const int ntask = (int)ns * nblk;             // <= 16^4 primitive quartets x 6^4 components
     const unsigned long long pwA = shell_powers(A, sA), pwB = shell_powers(A, sB), pwC = shell_powers(A, sC),
                              pwD = shell_powers(A, sD);
     const int pb0 = A.sp[Q.bra].first, pk0 = A.sp[Q.ket].first;   // first primitive pair of bra / ket
     const int npB = sB.nprim, npD = sD.nprim;
     const long R0 = r0 / ndp;
-    for (int task = lane; task < ntask; task += 64) {
-        const int g = task / nblk;
-        const int cp = task - g * nblk;
-        const Surv S = surv[s0 + g];
+    // one component, its survivors g0, g0 + gstep, ...: the coefficient-weighted node sums
+    auto comp_sum = [&](int cp, long g0, long gstep) {
         int t = cp;
         const int ca = t % nA; t /= nA;
         const int cb = t % nB; t /= nB;
@@ -521,35 +524,54 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
         const int cd = t / nC;
         const unsigned pa = (unsigned)(pwA >> (6 * ca)), pbw = (unsigned)(pwB >> (6 * cb)),
                        pc = (unsigned)(pwC >> (6 * cc)), pd = (unsigned)(pwD >> (6 * cd));
-        const double *I3[3];
+        int cxyz[3];
 #pragma unroll
         for (int dd = 0; dd < 3; ++dd) {
             const int sh = 2 * dd;
-            const int cx = ((pa >> sh) & 3) + (lA + 1) * (((pbw >> sh) & 3) +
-                           (lB + 1) * (((pc >> sh) & 3) + (lC + 1) * ((pd >> sh) & 3)));
-            if (dd < ndp)
-                I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cx) * nqp;
-            else
-                I3[dd] = realval + ((size_t)(R0 * nreal + (g * nreal + (dd - ndp)) * ncomb + cx)) * nqp;
+            cxyz[dd] = ((pa >> sh) & 3) + (lA + 1) * (((pbw >> sh) & 3) +
+                       (lB + 1) * (((pc >> sh) & 3) + (lC + 1) * ((pd >> sh) & 3)));
         }
-        const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
-        const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
-        const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
-        double v0 = 0.0, v1 = 0.0;
-        for (int k2 = 0; k2 < nq2; ++k2) {
-            const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
-            v0 += (x.x * w.x) * y.x * z.x;
-            v1 += (x.y * w.y) * y.y * z.y;
+        const int oA = sA.poff + ca * sA.nprim, oB = sB.poff + cb * sB.nprim, oC = sC.poff + cc * sC.nprim,
+                  oD = sD.poff + cd * sD.nprim;
+        double acc = 0.0;
+        for (long g = g0; g < ns; g += gstep) {
+            const Surv S = surv[s0 + g];
+            const double *I3[3];
+#pragma unroll
+            for (int dd = 0; dd < 3; ++dd) {
+                if (dd < ndp)
+                    I3[dd] = rowval + (size_t)(r0 + (g * ndp + dd) * ncomb + cxyz[dd]) * nqp;
+                else
+                    I3[dd] = realval + ((size_t)(R0 * nreal + (g * nreal + (dd - ndp)) * ncomb + cxyz[dd])) * nqp;
+            }
+            const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
+            const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
+            const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
+            double v0 = 0.0, v1 = 0.0;
+            for (int k2 = 0; k2 < nq2; ++k2) {
+                const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
+                v0 += (x.x * w.x) * y.x * z.x;
+                v1 += (x.y * w.y) * y.y * z.y;
+            }
+            double val = v0 + v1;
+            if (nq & 1) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
+            const int ib = S.pb - pb0, ik = S.pk - pk0;
+            const int pi = ib / npB, pj = ib - pi * npB, qk = ik / npD, ql = ik - qk * npD;
+            acc += S.fac * A.coef[oA + pi] * A.coef[oB + pj] * A.coef[oC + qk] * A.coef[oD + ql] * val;
         }
-        double val = v0 + v1;
-        if (nq & 1) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
-        const int ib = S.pb - pb0, ik = S.pk - pk0;
-        const int pi = ib / npB, pj = ib - pi * npB, qk = ik / npD, ql = ik - qk * npD;
-        const double cA = A.coef[sA.poff + ca * sA.nprim + pi];
-        const double cB = A.coef[sB.poff + cb * sB.nprim + pj];
-        const double cC = A.coef[sC.poff + cc * sC.nprim + qk];
-        const double cD = A.coef[sD.poff + cd * sD.nprim + ql];
-        atomicAdd(&blk[cp], S.fac * cA * cB * cC * cD * val);
+        return acc;
+    };
+    if (nblk >= 64) {
+        for (int cp = lane; cp < nblk; cp += 64) blk[cp] = comp_sum(cp, 0, 1);
+    } else {
+        const int G = 64 / nblk, cp = lane % nblk, gs = lane / nblk;
+        part[lane] = gs < G ? comp_sum(cp, gs, G) : 0.0;
+        wave_sync();
+        if (lane < nblk) {
+            double sum = 0.0;
+            for (int j = 0; j < G; ++j) sum += part[lane + j * nblk];
+            blk[lane] = sum;
+        }
     }
     if (A.counters && lane == 0)
         atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)ncomb * nreal * 2 * nq *
@@ -600,27 +622,27 @@ __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, 
     }
 }
 
-__global__ __launch_bounds__(64) void k_contract(EriArgs A, const long long *__restrict__ soff,
-                                                 const long long *__restrict__ roff, const Surv *__restrict__ surv,
-                                                 const double *__restrict__ rowval,
-                                                 const double *__restrict__ realval) {
-    extern __shared__ double blk[];                        // [max_blk]
-    const int lane = threadIdx.x;
-    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+__global__ __launch_bounds__(256) void k_contract(EriArgs A, const long long *__restrict__ soff,
+                                                  const long long *__restrict__ roff, const Surv *__restrict__ surv,
+                                                  const double *__restrict__ rowval,
+                                                  const double *__restrict__ realval) {
+    // four wavefronts per workgroup, one shell quartet each; per wavefront in LDS a
+    // block of max_blk and 64 partial sums
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double *blk = lds + (size_t)wv * (A.max_blk + 64), *part = blk + A.max_blk;
+    for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
         const Quartet Q = A.q4[qi];
         const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
-        const int nblk = sA.ncomp * sB.ncomp * sC.ncomp * sD.ncomp;
         const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
-        __syncthreads();
-        for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
-        __syncthreads();
-        quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, lane);
-        __syncthreads();
+        wave_sync();
+        quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, part, lane);
+        wave_sync();
         write_block(A, Q, sA, sB, sC, sD, blk, lane);
     }
 }
 
-size_t contract_lds_bytes(const EriArgs &a) { return sizeof(double) * (size_t)a.max_blk; }
+size_t contract_lds_bytes(const EriArgs &a) { return 4 * sizeof(double) * ((size_t)a.max_blk + 64); }
 
 // ---------------------------------------------------------------------------
 // Direct J/K (src/module/eri_direct.f90:262-374): every stored element of a
@@ -667,15 +689,16 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                                                      const long long *__restrict__ roff,
                                                      const Surv *__restrict__ surv, const double *__restrict__ rowval,
                                                      const double *__restrict__ realval, int jk_lds) {
-    // LDS: [4][max_blk] blocks, [4][2][36] per-wave J/K sub-blocks, then J[nfuc*nbas],
-    // K[nfuc*nbas] when jk_lds
+    // LDS: [4][max_blk] blocks, [4][2][36] per-wave J/K sub-blocks, [4][64] partial
+    // sums of the contraction, then J[nfuc*nbas], K[nfuc*nbas] when jk_lds
     extern __shared__ double lds[];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nf = A.nfuc, nb = A.nbas;
     const size_t njk = (size_t)nf * nb;
     double *blk = lds + (size_t)wv * A.max_blk;
     double *Jw = lds + 4 * (size_t)A.max_blk + 72 * wv, *Kw = Jw + 36;
-    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 72 : A.Jacc;
+    double *part = lds + 4 * (size_t)A.max_blk + 4 * 72 + 64 * wv;
+    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 72 + 4 * 64 : A.Jacc;
     double *Kl = jk_lds ? Jl + njk : A.Kacc;
     if (jk_lds) {
         for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
@@ -688,9 +711,7 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         const int nblk = nA * nB * nC * nD;
         const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
         wave_sync();
-        for (int i = lane; i < nblk; i += 64) blk[i] = 0.0;
-        wave_sync();
-        quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, lane);
+        quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, part, lane);
         wave_sync();
         if (jk_lds) {
             // J/K in LDS: every counted element adds there directly
@@ -770,7 +791,7 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                               const double *rowval, const double *realval, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 72);
+    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 72 + 4 * 64);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
     const size_t lds_max = 160 * 1024;
     // TCG_JK_GLOBAL=1 forces the global-memory accumulation (the path of systems
@@ -796,13 +817,14 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                            const double *rowval, const double *realval, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
+    const int nwg = (a.nq4 + 3) / 4;
+    const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
     const size_t lds = contract_lds_bytes(a);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_contract, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(64), lds, st, a, soff, roff, surv, rowval, realval);
+    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, a, soff, roff, surv, rowval, realval);
     return hipGetLastError();
 }
 
