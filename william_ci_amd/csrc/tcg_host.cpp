@@ -398,6 +398,18 @@ static int build_pairs(tcg_ctx *c, std::vector<VecJob> &jobs, size_t &nvec, size
     c->sp.assign((size_t)c->nshuc * c->nsh, ShellPair{0, 0});
     jobs.clear();
     herm.clear();
+    {
+        size_t npair = 0, nherm = 0;
+        for (int S1 = 0; S1 < c->nshuc; ++S1)
+            for (int S2 = 0; S2 < c->nsh; ++S2) {
+                const size_t n = (size_t)c->sh[S1].nprim * c->sh[S2].nprim;
+                npair += n;
+                nherm += n * (3 - c->ndp) * (c->sh[S1].l + 1) * (c->sh[S2].l + 1) * (2 * kMaxL + 1);
+            }
+        c->pp.reserve(npair);
+        jobs.reserve(npair * c->ndp);
+        herm.reserve(nherm);
+    }
     nvec = 0;
     niv = 0;
     const int nreal = 3 - c->ndp;
@@ -675,6 +687,8 @@ static int prepare(tcg_ctx *c) {
         // Fourier coefficients d_k of every (job, power pair) on the host, vectors on the GPU
         std::vector<VecOut> vo;
         std::vector<double2> dco;
+        vo.reserve(jobs.size() * (kMaxL + 1) * (kMaxL + 1));
+        dco.reserve(jobs.size() * 16);
         std::complex<double> dk[4 * kMaxL + 1];
         for (size_t j = 0; j < jobs.size(); ++j) {
             const VecJob &J = jobs[j];

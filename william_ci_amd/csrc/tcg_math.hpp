@@ -173,19 +173,23 @@ inline void hermite_E(int i, int j, double p, double PA, double PB, double *E /*
 // (src/module/clifford_tools.f90:578-607): repeated multiplication by
 // sin(u+ph) = e^{i ph} e^{iu}/(2i) - e^{-i ph} e^{-iu}/(2i).
 inline void sin_product_fourier(int i, int l, double phA, double phB, std::complex<double> *d) {
-    const int K = i + l;
-    std::vector<std::complex<double>> cur(2 * K + 3), nxt(2 * K + 3);
+    const int K = i + l;                 // <= 4 kMaxL (callers' d holds 4 kMaxL + 1 values)
+    constexpr int kCap = 4 * kMaxL + 3;
+    std::complex<double> buf[2][kCap];
+    std::complex<double> *cur = buf[0], *nxt = buf[1];
+    for (int k = 0; k < 2 * K + 3; ++k) cur[k] = 0.0;
     cur[K + 1] = 1.0;   // index k + K + 1
+    const double sA = std::sin(phA), cA = std::cos(phA), sB = std::sin(phB), cB = std::cos(phB);
     for (int m = 1; m <= K; ++m) {
-        double ph = (m <= i) ? phA : phB;
-        std::complex<double> cp = std::complex<double>(std::sin(ph), -std::cos(ph)) * 0.5;   // e^{i ph}/(2i)
-        std::complex<double> cm = std::complex<double>(std::sin(ph), std::cos(ph)) * 0.5;    // -e^{-i ph}/(2i)
-        std::fill(nxt.begin(), nxt.end(), std::complex<double>(0.0));
+        const double sn = (m <= i) ? sA : sB, cs = (m <= i) ? cA : cB;
+        const std::complex<double> cp = std::complex<double>(sn, -cs) * 0.5;   // e^{i ph}/(2i)
+        const std::complex<double> cm = std::complex<double>(sn, cs) * 0.5;    // -e^{-i ph}/(2i)
+        for (int k = 0; k < 2 * K + 3; ++k) nxt[k] = 0.0;
         for (int k = -(m - 1); k <= m - 1; ++k) {
             nxt[k + 1 + K + 1] += cp * cur[k + K + 1];
             nxt[k - 1 + K + 1] += cm * cur[k + K + 1];
         }
-        cur.swap(nxt);
+        std::swap(cur, nxt);
     }
     for (int k = -K; k <= K; ++k) d[k + K] = cur[k + K + 1];
 }
