@@ -760,58 +760,91 @@ static double real_overlap_factor(int j, int m, double p, double PA, double PB) 
 }
 
 static int one_electron_rows(tcg_ctx *c, std::vector<double> &S, std::vector<double> &T) {
-    const int nf = c->nfuc, nb = c->nbas;
-    std::vector<int> ao_shell(nb);
-    for (int sI = 0; sI < c->nsh; ++sI)
-        for (int x = 0; x < c->sh[sI].ncomp; ++x) ao_shell[c->sh[sI].first_ao + x] = sI;
-    std::vector<double> iv;
-    for (int i = 0; i < nf; ++i)
-        for (int j = 0; j < nb; ++j) {
-            const Shell &A = c->sh[ao_shell[i]], &B = c->sh[ao_shell[j]];
-            const int *li = &c->lxyz[3 * i], *lj = &c->lxyz[3 * j];
-            double sv = 0.0, tv = 0.0;
+    // shell pairs (A in the unit cell, B anywhere), spread over host threads; the Bessel
+    // vector of a primitive pair and periodic direction is computed once per shell pair
+    // (orders up to lA + lB + 3) and serves every component pair of the two shells
+    const int nf = c->nfuc, nb = c->nbas, nsh = c->nsh;
+    std::vector<int> uc;
+    for (int sI = 0; sI < nsh; ++sI)
+        if (c->sh[sI].first_ao < nf) uc.push_back(sI);
+    const int ntask = (int)uc.size() * nsh;
+    std::atomic<int> next(0);
+    auto work = [&] {
+        std::vector<double> iv[3], scratch;
+        for (int t = next++; t < ntask; t = next++) {
+            const Shell &A = c->sh[uc[t / nsh]], &B = c->sh[t % nsh];
+            for (int a = 0; a < A.ncomp; ++a)
+                for (int b = 0; b < B.ncomp; ++b) {
+                    S[(size_t)(A.first_ao + a) + (size_t)nb * (B.first_ao + b)] = 0.0;
+                    T[(size_t)(A.first_ao + a) + (size_t)nb * (B.first_ao + b)] = 0.0;
+                }
             for (int pi = 0; pi < A.nprim; ++pi)
                 for (int pj = 0; pj < B.nprim; ++pj) {
                     const double e1 = c->expo[A.poff + pi], e2 = c->expo[B.poff + pj];
                     const double esum = e1 + e2, mu = e1 * e2 / esum;
-                    const double cc = c->coef[c->poff[i] + pi] * c->coef[c->poff[j] + pj];
-                    double Sd[3], Dd[3], pref = 1.0;
+                    double pref = 1.0, Aarg[3] = {0, 0, 0}, phA[3] = {0, 0, 0}, phB[3] = {0, 0, 0};
+                    double PA[3] = {0, 0, 0}, PB[3] = {0, 0, 0};
                     for (int d = 0; d < 3; ++d) {
-                        const int l1 = li[d], l2 = lj[d];
                         if (d < c->ndp) {
                             double pe, cen;
-                            const double a = c->a[d], L = c->L[d];
-                            clifford_pair(e1, e2, A.xyz[d], B.xyz[d], L, a, &pe, &cen);
-                            pref *= std::exp(-2.0 * (esum - pe) / (a * a));
-                            const double Aarg = 2.0 * pe / (a * a);
-                            const int nmax = l1 + l2 + 3;
-                            iv.resize(nmax + 1);
-                            bessel_miller(nmax, Aarg, nmax + 30 + (int)std::sqrt(80.0 * Aarg), iv.data());
-                            const double phA = a * (cen - A.xyz[d]), phB = a * (cen - B.xyz[d]);
-                            Sd[d] = clifford_overlap_factor(l1, l2, Aarg, phA, phB, a, L, iv, false);
-                            double D = 0.0;
-                            if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * clifford_overlap_factor(l1, l2 - 2, Aarg, phA, phB, a, L, iv, false);
-                            if (l2 >= 1) D -= (double)(l2 * l2) * a * a * Sd[d];
-                            // cos(a(x - B)) g_B^l: the l+0 factor times cos(u + phB), with a^-(i+l) as for S(i,l)
-                            D -= 2.0 * e2 * (2 * l2 + 1) * clifford_overlap_factor(l1, l2, Aarg, phA, phB, a, L, iv, true);
-                            D += 4.0 * e2 * e2 * clifford_overlap_factor(l1, l2 + 2, Aarg, phA, phB, a, L, iv, false);
-                            Dd[d] = D;
+                            const double ad = c->a[d], L = c->L[d];
+                            clifford_pair(e1, e2, A.xyz[d], B.xyz[d], L, ad, &pe, &cen);
+                            pref *= std::exp(-2.0 * (esum - pe) / (ad * ad));
+                            Aarg[d] = 2.0 * pe / (ad * ad);
+                            const int nmax = A.l + B.l + 3;
+                            iv[d].resize(nmax + 1);
+                            bessel_miller(nmax, Aarg[d], nmax + 30 + (int)std::sqrt(80.0 * Aarg[d]), iv[d].data(), scratch);
+                            phA[d] = ad * (cen - A.xyz[d]);
+                            phB[d] = ad * (cen - B.xyz[d]);
                         } else {
                             const double P = (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum;
                             pref *= std::exp(-mu * (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]));
-                            const double PA = P - A.xyz[d], PB = P - B.xyz[d];
-                            Sd[d] = real_overlap_factor(l1, l2, esum, PA, PB);
-                            double D = -2.0 * e2 * (2 * l2 + 1) * Sd[d] + 4.0 * e2 * e2 * real_overlap_factor(l1, l2 + 2, esum, PA, PB);
-                            if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * real_overlap_factor(l1, l2 - 2, esum, PA, PB);
-                            Dd[d] = D;
+                            PA[d] = P - A.xyz[d];
+                            PB[d] = P - B.xyz[d];
                         }
                     }
-                    sv += cc * pref * Sd[0] * Sd[1] * Sd[2];
-                    tv -= 0.5 * cc * pref * (Dd[0] * Sd[1] * Sd[2] + Sd[0] * Dd[1] * Sd[2] + Sd[0] * Sd[1] * Dd[2]);
+                    for (int ca = 0; ca < A.ncomp; ++ca)
+                        for (int cb = 0; cb < B.ncomp; ++cb) {
+                            const int i = A.first_ao + ca, j = B.first_ao + cb;
+                            const int *li = &c->lxyz[3 * i], *lj = &c->lxyz[3 * j];
+                            const double cc = c->coef[c->poff[i] + pi] * c->coef[c->poff[j] + pj];
+                            double Sd[3], Dd[3];
+                            for (int d = 0; d < 3; ++d) {
+                                const int l1 = li[d], l2 = lj[d];
+                                if (d < c->ndp) {
+                                    const double ad = c->a[d], L = c->L[d];
+                                    Sd[d] = clifford_overlap_factor(l1, l2, Aarg[d], phA[d], phB[d], ad, L, iv[d], false);
+                                    double D = 0.0;
+                                    if (l2 >= 2)
+                                        D += (double)(l2 * (l2 - 1)) *
+                                             clifford_overlap_factor(l1, l2 - 2, Aarg[d], phA[d], phB[d], ad, L, iv[d], false);
+                                    if (l2 >= 1) D -= (double)(l2 * l2) * ad * ad * Sd[d];
+                                    // cos(a(x - B)) g_B^l: the l+0 factor times cos(u + phB), with a^-(i+l) as for S(i,l)
+                                    D -= 2.0 * e2 * (2 * l2 + 1) *
+                                         clifford_overlap_factor(l1, l2, Aarg[d], phA[d], phB[d], ad, L, iv[d], true);
+                                    D += 4.0 * e2 * e2 *
+                                         clifford_overlap_factor(l1, l2 + 2, Aarg[d], phA[d], phB[d], ad, L, iv[d], false);
+                                    Dd[d] = D;
+                                } else {
+                                    Sd[d] = real_overlap_factor(l1, l2, esum, PA[d], PB[d]);
+                                    double D = -2.0 * e2 * (2 * l2 + 1) * Sd[d] +
+                                               4.0 * e2 * e2 * real_overlap_factor(l1, l2 + 2, esum, PA[d], PB[d]);
+                                    if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * real_overlap_factor(l1, l2 - 2, esum, PA[d], PB[d]);
+                                    Dd[d] = D;
+                                }
+                            }
+                            const size_t ij = (size_t)i + (size_t)nb * j;
+                            S[ij] += cc * pref * Sd[0] * Sd[1] * Sd[2];
+                            T[ij] -= 0.5 * cc * pref * (Dd[0] * Sd[1] * Sd[2] + Sd[0] * Dd[1] * Sd[2] + Sd[0] * Sd[1] * Dd[2]);
+                        }
                 }
-            S[(size_t)i + (size_t)nb * j] = sv;
-            T[(size_t)i + (size_t)nb * j] = tv;
         }
+    };
+    const unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < nthr && (int)k < ntask; ++k) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
     return 0;
 }
 

@@ -41,13 +41,13 @@ inline double i0_scaled(double x) {
 // normalised to exp(-A) I_0(A).  start = N reproduces the reference's
 // bessel_scaled_vector_fast (src/module/clifford_tools.f90:429-461), start =
 // N + 30 + sqrt(80A) its bessel_scaled_vector (:386-421).
-inline void bessel_miller(int N, double A, int n0, double *out) {
+inline void bessel_miller(int N, double A, int n0, double *out, std::vector<double> &w) {
     if (A < 1e-12 || (n0 == N && A < 1e-10)) {
         for (int n = 0; n <= N; ++n) out[n] = 0.0;
         out[0] = 1.0;
         return;
     }
-    std::vector<double> w(n0 + 2, 0.0);
+    w.assign(n0 + 2, 0.0);
     w[n0] = 1e-280;
     double r = 2.0 / A;
     for (int k = n0; k >= 1; --k) {
@@ -57,6 +57,11 @@ inline void bessel_miller(int N, double A, int n0, double *out) {
     }
     double s = i0_scaled(A) / w[0];
     for (int n = 0; n <= N; ++n) out[n] = w[n] * s;
+}
+
+inline void bessel_miller(int N, double A, int n0, double *out) {
+    std::vector<double> w;
+    bessel_miller(N, A, n0, out, w);
 }
 
 // exp(-A) I_n(A), n = 0..N, as the reference's bessel_scaled_vector
