@@ -17,8 +17,9 @@
 //                 with v_mfma_f64_16x16x4f64 (F on the VALU as the A operand,
 //                 the t-table as B), loads two steps ahead of use.  Result:
 //                 the node values of every row (nqp per row), in HBM.
-//   k_real_rows   one workgroup per (primitive quartet, real direction), a thread per node:
-//                 the Hermite-form factors of all power combinations.
+//   k_real_rows   one wavefront per shell quartet, a lane per node, over its
+//                 (primitive quartet, real direction) pairs: the Hermite-form
+//                 factors of all power combinations.
 //   k_contract    one wavefront per shell quartet: sum_k w_k Ix Iy Iz, coefficients, the shell
 //                 block in LDS, then the 8-permutation scatter translated to
 //                 the unit cell (src/Toroidal/ERI_toroidal.f90:416-454) or the
@@ -410,8 +411,7 @@ __device__ __forceinline__ void real_combos(int lA, int lB, int lC, int lD, cons
 }
 
 // ---------------------------------------------------------------------------
-// real-direction rows for the contraction of quartets with d shells (the
-// s/p contraction computes them itself): one wavefront per shell quartet,
+// real-direction rows for the contraction: one wavefront per shell quartet,
 // lanes over the nodes, a loop over (surviving primitive quartet, real
 // direction); written next to the periodic rows of the same quartet: with
 // R = roff[q] / ndp (the quartet's first (group, combination) index) and g
