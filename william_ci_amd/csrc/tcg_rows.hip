@@ -17,7 +17,7 @@
 //                 with v_mfma_f64_16x16x4f64 (F on the VALU as the A operand,
 //                 the t-table as B), loads two steps ahead of use.  Result:
 //                 the node values of every row (nqp per row), in HBM.
-//   k_real_rows   one wavefront per shell quartet, a lane per node, over its
+//   k_real_rows   a few wavefronts per shell quartet, a lane per node, over its
 //                 (primitive quartet, real direction) pairs: the Hermite-form
 //                 factors of all power combinations.
 //   k_contract    one wavefront per shell quartet: sum_k w_k Ix Iy Iz, coefficients, the shell
@@ -411,9 +411,10 @@ __device__ __forceinline__ void real_combos(int lA, int lB, int lC, int lD, cons
 }
 
 // ---------------------------------------------------------------------------
-// real-direction rows for the contraction: one wavefront per shell quartet,
-// lanes over the nodes, a loop over (surviving primitive quartet, real
-// direction); written next to the periodic rows of the same quartet: with
+// real-direction rows for the contraction: gridDim.y wavefronts per shell
+// quartet (one for long quartet lists), lanes over the nodes, each taking every
+// gridDim.y-th (surviving primitive quartet, real direction) step; written next
+// to the periodic rows of the same quartet: with
 // R = roff[q] / ndp (the quartet's first (group, combination) index) and g
 // the group inside the quartet,
 //   realval[(R * nreal + (g * nreal + dr) * ncomb + combo) * nqp + k].
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__
             t2v[h] = tk * tk;
         }
         const long long nit = ns * nreal;
-        for (long long it = 0; it < nit; ++it) {
+        for (long long it = blockIdx.y; it < nit; it += gridDim.y) {
             const long long gl = it / nreal;
             const int drr = (int)(it - gl * nreal);
             const int dr = A.ndp + drr;
@@ -469,7 +470,11 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
                             long long ns, double *realval, hipStream_t st) {
     if (a.ndp >= 3 || ns <= 0 || a.nq4 <= 0) return hipSuccess;
     const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
-    hipLaunchKernelGGL(k_real_rows, dim3(grid), dim3(64), 0, st, a, soff, roff, surv, realval);
+    // short quartet lists (the Schwarz passes: few quartets, up to nprim^4 survivors each)
+    // spread the (survivor, direction) steps of a quartet over 16 wavefronts; long lists
+    // keep one wavefront per quartet (measured faster there)
+    const int gy = a.nq4 < 16384 ? 16 : 1;
+    hipLaunchKernelGGL(k_real_rows, dim3(grid, gy), dim3(64), 0, st, a, soff, roff, surv, realval);
     return hipGetLastError();
 }
 
