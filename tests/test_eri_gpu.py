@@ -120,3 +120,25 @@ def test_jk_stored_physicist_order(sao):
     for J, K in ((Jp, Kp), (Jh, Kh), (Jo, Ko)):
         assert np.abs(J - Jc).max() < 1e-12 * np.abs(Jc).max()
         assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
+
+
+@pytest.mark.parametrize("name", ["h10_ring", "h6_d_1d", "h2_3x3_2d", "h2_2x2x2_d_3d"])
+def test_sp_contraction_matches_general(name, monkeypatch):
+    """The lanes-over-nodes contraction of the s/p quartets (one to three p
+    shells, uniform p coefficients) gives the integrals of the general
+    contraction: the same sums in another order, so equal to round-off."""
+    from william_ci_amd.engine import Engine
+    s = CASES[name]()
+    coef = normalised_coef(s)
+    print(f"case {name} s/p form vs general", flush=True)
+    monkeypatch.setenv("TCG_SP_MIN_QUARTETS", "0")       # the s/p kernels also on these short lists
+    eng = Engine(s, coef)
+    E1 = eng.eri_build()
+    eng.close()
+    monkeypatch.setenv("TCG_GENERAL_CONTRACT", "1")
+    eng = Engine(s, coef)
+    E2 = eng.eri_build()
+    eng.close()
+    d = np.abs(E1 - E2).max()
+    print(f"  max |diff| {d:.2e}", flush=True)
+    assert d <= 1e-13 * max(1.0, np.abs(E2).max())

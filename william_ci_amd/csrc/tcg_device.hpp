@@ -30,7 +30,7 @@ struct EriArgs {
     // every rule padded to nqp nodes:
     const double *tab;                // [rule][d][ncap+1+4][nqp] c_n exp(-C)I_n(C), 0 beyond t_nmax(k)
     const int *tab_lim;               // [rule][d][nqp/16]: last n with a nonzero entry in that node tile
-    const int *tnmax;                 // [rule][d][nqp]: truncation order of each node (-1 in the padding)
+    const long long *tsum;            // [rule][d][ncap+1]: sum over the rule's nodes of min(n, t_nmax(k)) + 1
     const double *tnode, *tweight;    // [rule][nqp] (0 in the padding)
     const int *rule_nq;               // [rule] node count (nullptr: one rule of nq nodes)
     const int *qrule;                 // [nq4] rule of each quartet (nullptr: rule 0)
@@ -48,6 +48,8 @@ struct EriArgs {
     int screen_prim;                  // primitive Schwarz screen (keyword prim_screen)
     int max_ncomb;                    // largest (lA+1)(lB+1)(lC+1)(lD+1) over the quartets
     int max_blk;                      // largest nA nB nC nD over the quartets
+    int sp_contract;                  // s/p quartets (at most three p shells) by the s/p contraction kernels
+    int sp_min_quartets;              // ... in quartet lists of at least this many quartets
     // block mode
     double *blk_out;
     // scatter mode
@@ -84,8 +86,9 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
                             long long ns, double *realval, hipStream_t st);
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                               const double *rowval, const double *realval, hipStream_t st);
+// lists: 4 * a.nq4 ints, cnt: 4 counters (work space of the s/p contraction)
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                           const double *rowval, const double *realval, hipStream_t st);
+                           const double *rowval, const double *realval, int *lists, unsigned *cnt, hipStream_t st);
 hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, int nchunk, int *bounds, hipStream_t st);
 hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_t st);
 hipError_t launch_translate_rows(double *eri, int nb, int nf, int ncell, const int *relc, hipStream_t st);

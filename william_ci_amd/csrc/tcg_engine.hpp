@@ -65,7 +65,9 @@ struct Surv {
 // One row of the quadrature contraction: (primitive quartet, periodic
 // direction, power combination).
 struct RowRec {
-    double phi;          // a_d (P_d - Q_d)
+    double c1, s1;       // e^{i phi}, phi = a_d (P_d - Q_d)
+    double c4, s4;       // e^{4 i phi}
+    double c64, s64;     // e^{64 i phi}
     double sgn;          // a_d^-(ia+ib+ic+id)
     int gofs, hofs;      // offsets of the bra / ket Fourier vectors
     int nm;              // last order: min(N_bra, N_ket)

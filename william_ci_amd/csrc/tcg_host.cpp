@@ -93,7 +93,8 @@ struct tcg_ctx {
     // device
     hipStream_t st = nullptr;
     DevBuf<double> d_tab, d_tnode, d_tw, d_coef, d_herm, d_blk, d_Q, d_Qsh, d_eri, d_P, d_J, d_K, d_jwork;
-    DevBuf<int> d_tablim, d_tnmax, d_lxyz, d_poff, d_relc;
+    DevBuf<int> d_tablim, d_lxyz, d_poff, d_relc;
+    DevBuf<long long> d_tsum;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     bool time_rows = false;    // record ev2/ev3 around the row MFMA kernel
     DevBuf<Shell> d_sh;
@@ -107,7 +108,8 @@ struct tcg_ctx {
     DevBuf<Surv> d_surv;
     DevBuf<RowRec> d_rows;
     DevBuf<unsigned> d_keys, d_keys2;
-    DevBuf<int> d_vals, d_order;
+    DevBuf<int> d_vals, d_order, d_cls;
+    DevBuf<unsigned> d_clscnt;
     DevBuf<double> d_rowval, d_realval;
     DevBuf<int> d_sq;
     DevBuf<char> d_tmp;
@@ -126,6 +128,8 @@ struct tcg_ctx {
     long long stats[8] = {0};
     double times[8] = {0};
     int max_ncomb = 1, max_blk = 1;
+    int sp_contract = 0;         // the s/p contraction kernels take the quartets of s and p shells
+    int sp_min_quartets = 16384; // ... of quartet lists at least this long
     bool prepared = false;
     int nq4 = 0;                 // unique shell quartets in d_q4 (after setup_quartets)
     std::vector<int> rule_nq;    // node count of each rule in the tables (rule 0 = the run's rule)
@@ -233,10 +237,10 @@ static void rule_nodes(int fast, int nq, int nqp, std::vector<double> &tnode, st
 // Tables of every rule the quartets may use: rule 0 = the run's rule
 // (fast_quad / nquad_point), rules 1.. = the learned orders; all padded to
 // the largest node count c->nqp_all.  Layout [rule][d][n][node] (t-table),
-// [rule][d][node tile] (last nonzero order), [rule][d][node] (t_nmax),
+// [rule][d][node tile] (last nonzero order), [rule][d][n] (flop count of a row of last order n),
 // [rule][node] (nodes, weights).  The Bessel columns are independent: they
 // are spread over host threads.
-static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim, std::vector<int> &tnmax,
+static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &tablim, std::vector<long long> &tsum,
                         std::vector<double> &tnode, std::vector<double> &tweight) {
     int lmax = 0;
     double emax = 0.0, amin = 1e300;
@@ -272,7 +276,7 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
         }
     }
     tab.assign((size_t)nr * ndp * rows * nqp, 0.0);
-    tnmax.assign((size_t)nr * ndp * nqp, -1);
+    std::vector<int> tnmax((size_t)nr * ndp * nqp, -1);
     // columns (rule, d, node), each a scaled Bessel vector up to ncap
     std::vector<int> cols;
     for (int r = 0; r < nr; ++r)
@@ -304,6 +308,16 @@ static int build_tables(tcg_ctx *c, std::vector<double> &tab, std::vector<int> &
             int &lim = tablim[(size_t)rd * nh + k / kNodeTile];
             lim = std::max(lim, tnmax[(size_t)rd * nqp + k]);
         }
+    // flop count of a row of last order n: sum over the rule's nodes of min(n, t_nmax(k)) + 1
+    tsum.assign((size_t)nr * ndp * (ncap + 1), 0);
+    for (int rd = 0; rd < nr * ndp; ++rd) {
+        const int nq = c->rule_nq[rd / ndp];
+        for (int n = 0; n <= ncap; ++n) {
+            long long s = 0;
+            for (int k = 0; k < nq; ++k) s += std::min(n, tnmax[(size_t)rd * nqp + k]) + 1;
+            tsum[(size_t)rd * (ncap + 1) + n] = s;
+        }
+    }
     return 0;
 }
 
@@ -499,7 +513,7 @@ static EriArgs base_args(tcg_ctx *c) {
     for (int d = 0; d < c->ndp; ++d) A.fac_const *= c->L[d] * c->L[d];
     A.tab = c->d_tab.p;
     A.tab_lim = c->d_tablim.p;
-    A.tnmax = c->d_tnmax.p;
+    A.tsum = c->d_tsum.p;
     A.tnode = c->d_tnode.p;
     A.tweight = c->d_tw.p;
     A.sh = c->d_sh.p;
@@ -514,6 +528,8 @@ static EriArgs base_args(tcg_ctx *c) {
     A.screen_prim = 0;
     A.max_ncomb = c->max_ncomb;
     A.max_blk = c->max_blk;
+    A.sp_contract = c->sp_contract;
+    A.sp_min_quartets = c->sp_min_quartets;
     A.nbas = c->nbas;
     A.nfuc = c->nfuc;
     A.ncell = c->ncell;
@@ -554,8 +570,12 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     }
     if (A.Jacc)
         HIPCHK(c, launch_contract_jk(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
-    else
-        HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
+    else {
+        HIPCHK(c, c->d_cls.alloc(4 * (size_t)A.nq4));
+        HIPCHK(c, c->d_clscnt.alloc(4));
+        HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->d_cls.p, c->d_clscnt.p,
+                                  c->st));
+    }
     if (c->time_rows) {
         float ms = 0.f;
         HIPCHK(c, hipEventSynchronize(c->ev3));
@@ -653,9 +673,10 @@ static int prepare(tcg_ctx *c) {
     if (c->prepared) return 0;
     // ---- 1-2: tables and pair data ----
     std::vector<double> tab, rtnode, rtw;
-    std::vector<int> tablim, tnmax;
+    std::vector<int> tablim;
+    std::vector<long long> tsum;
     const double tt0 = now_ms();
-    const bool tables_kept = build_tables(c, tab, tablim, tnmax, rtnode, rtw) == 1;
+    const bool tables_kept = build_tables(c, tab, tablim, tsum, rtnode, rtw) == 1;
     c->times[7] = now_ms() - tt0;
     std::vector<VecJob> jobs;
     size_t nvec = 0, niv = 0;
@@ -668,10 +689,16 @@ static int prepare(tcg_ctx *c) {
     c->max_blk = 1;
     for (auto &s : c->sh) c->max_blk = std::max(c->max_blk, s.ncomp);
     c->max_blk = c->max_blk * c->max_blk * c->max_blk * c->max_blk;
+    // (TCG_GENERAL_CONTRACT set: every quartet through the general contraction)
+    c->sp_contract = getenv("TCG_GENERAL_CONTRACT") ? 0 : 1;
+    // (TCG_SP_MIN_QUARTETS: the shortest quartet list for the s/p kernels)
+    if (const char *e = getenv("TCG_SP_MIN_QUARTETS")) c->sp_min_quartets = atoi(e);
+    for (auto &s : c->sh)
+        if (s.l == 1 && s.ncomp != 3) c->sp_contract = 0;
     if (!tables_kept) {
         HIPCHK(c, upload_async(c, c->d_tab, tab));
         HIPCHK(c, upload_async(c, c->d_tablim, tablim));
-        HIPCHK(c, upload_async(c, c->d_tnmax, tnmax));
+        HIPCHK(c, upload_async(c, c->d_tsum, tsum));
         HIPCHK(c, upload_async(c, c->d_tnode, rtnode));
         HIPCHK(c, upload_async(c, c->d_tw, rtw));
         HIPCHK(c, upload_async(c, c->d_rule_nq, c->rule_nq));

@@ -147,7 +147,11 @@ __global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *_
             R.hofs = pk.voff[d] + (ic * (l[3] + 1) + id) * (kN + 1);
             const double ad = (d == 0) ? A.a[0] : (d == 1) ? A.a[1] : A.a[2];
             const double shd = A.q4[qi].shift[d];
-            R.phi = ad * (pb.cen[d] - (pk.cen[d] + shd));
+            // the phase factors of the row, once per row (the MFMA kernel needs no sincos)
+            const double phi = ad * (pb.cen[d] - (pk.cen[d] + shd));
+            sincos(phi, &R.s1, &R.c1);
+            sincos(4.0 * phi, &R.s4, &R.c4);
+            sincos(64.0 * phi, &R.s64, &R.c64);
             R.sgn = pow(ad, -(double)(ia + ib + ic + id));
             const int rule = quartet_rule(A, qi);
             R.d = d | (rule << 8);
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
         const long long pos = tile * 16 + r;
         const bool valid = pos < nrows;
         int ridx = -1, nm = -1, d = 0x7fffffff;         // d: direction | rule << 8
-        double phi = 0.0, sgn = 1.0;
+        double sgn = 1.0, c1 = 1.0, s1 = 0.0, c4 = 1.0, s4 = 0.0, c64 = 1.0, s64 = 0.0;
         const double2 *Gp = A.vec, *Hp = A.vec;
         if (valid) {
             ridx = order[pos];
@@ -189,27 +193,31 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
             nm = R.nm;
             d = R.d;
             sgn = R.sgn;
-            phi = R.phi;
+            c1 = R.c1;
+            s1 = R.s1;
+            c4 = R.c4;
+            s4 = R.s4;
+            c64 = R.c64;
+            s64 = R.s64;
             Gp = A.vec + R.gofs;
             Hp = A.vec + R.hofs;
         }
         if (A.counters && kq == 0 && valid && hbase == 0) {
-            long s = 0;
-            const int rl = d >> 8, dl = d & 0xff, nql = rule_nodes(A, rl);
-#pragma unroll 1
-            for (int k = 0; k < nql; ++k) {
-                const int tk = A.tnmax[(rl * A.ndp + dl) * nqp + k];
-                s += (nm < tk ? nm : tk) + 1;
-            }
+            const int rl = d >> 8, dl = d & 0xff;
+            const long long s = A.tsum[(size_t)(rl * A.ndp + dl) * (A.ncap + 1) + (nm < A.ncap ? nm : A.ncap)];
             atomicAdd(&A.counters[4], (unsigned long long)(2 * s + 10 * (nm + 1)));
         }
         // e^{i n phi}: rotation by e^{4 i phi} inside each block of 64 orders; at a block start
-        // the lane's base phase advances by e^{64 i phi} (both from sincos once per tile),
-        // so the rounding error stays within a few tens of ulps for n up to ~1300
-        double s4, c4, sn0, cn0, s64, c64;
-        sincos(4.0 * phi, &s4, &c4);
-        sincos(64.0 * phi, &s64, &c64);
-        sincos((double)kq * phi, &sn0, &cn0);
+        // the lane's base phase advances by e^{64 i phi} (the row's phase factors, from sincos
+        // once per row), so the rounding error stays within a few tens of ulps for n up to
+        // ~1300; the lane's own e^{i kq phi} is a product of at most three e^{i phi}
+        double cn0 = 1.0, sn0 = 0.0;
+#pragma unroll
+        for (int q = 1; q <= 3; ++q) {
+            const double tc = cn0 * c1 - sn0 * s1, ts = sn0 * c1 + cn0 * s1;
+            cn0 = q <= kq ? tc : cn0;
+            sn0 = q <= kq ? ts : sn0;
+        }
         // the (direction, rule) combinations present in the tile, ascending
 #pragma unroll 1
         for (int dd = wmin(d); dd != 0x7fffffff; dd = wmin(d > dd ? d : 0x7fffffff)) {
@@ -583,6 +591,204 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
                                                                 (lA + lB + lC + lD + 1) + 3ull * nq * nblk));
 }
 
+// ---------------------------------------------------------------------------
+// contraction of a quartet of s and p shells by a group of 16 lanes over the
+// nodes, four quartets per wavefront.
+// With P p shells (rank j = 0..P-1 in the order A, B, C, D) the power
+// combination of a direction has one bit per p shell (set when the shell's
+// component points along that direction): ncomb = 2^P, and a component
+// quartet is a digit string (dir_0 .. dir_{P-1}), dir in {x, y, z}, with
+//   cx = sum_j [dir_j = x] 2^j, cy = sum_j [dir_j = y] 2^j, cz = sum_j [dir_j = z] 2^j,
+// all known at compile time.  Per surviving primitive quartet the group loads
+// the 3 * 2^P node rows it needs once, in runs of 16 consecutive nodes (one
+// 128-byte line per load), and adds Ix Iy Iz into 3^P accumulators per lane;
+// one butterfly sum per component at the end.  The contraction coefficients go
+// into the rows: a p shell's coefficient for its component along direction d
+// multiplies the rows of direction d whose combination has that shell's bit
+// set, so every product Ix Iy Iz carries each p shell's coefficient exactly
+// once; the s shells' coefficients, the prefactor and the node weight scale the
+// x rows.  A quartet's block is a chain of dependent loads (quartet, shells,
+// offsets, survivors, coefficients, rows, then the scatter's tables): four
+// quartets per wavefront keep four such chains in flight.  The sums run in a
+// fixed order: the block is the same on every run.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int pow3(int p) { return p == 0 ? 1 : 3 * pow3(p - 1); }
+
+// component (0..2) of a p shell pointing along each direction: 2 bits per direction
+__device__ __forceinline__ unsigned p_shell_dirmap(unsigned long long pw) {
+    unsigned m = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const unsigned w = (unsigned)(pw >> (6 * c));
+        const unsigned dir = (w & 3) ? 0u : ((w >> 2) & 3) ? 1u : 2u;
+        m |= (unsigned)c << (2 * dir);
+    }
+    return m;
+}
+
+// the block of one s/p quartet by the 16 lanes of a group (gl = 0..15): the
+// rule's nodes (at most 64) in four runs of 16
+template <int P>
+__device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const Quartet &Q, long s0, long ns,
+                                                 long r0, const Surv *__restrict__ surv,
+                                                 const double *__restrict__ rowval,
+                                                 const double *__restrict__ realval, double *blk, int gl) {
+    constexpr int NC = 1 << P, NT = pow3(P);
+    const int rule = quartet_rule(A, qi);
+    const int nqp = A.nqp, nq = rule_nodes(A, rule), ndp = A.ndp, nreal = 3 - ndp;
+    const double *tw = A.tweight + (size_t)rule * nqp;
+    const Shell sA = A.sh[Q.s[0]], sB = A.sh[Q.s[1]], sC = A.sh[Q.s[2]], sD = A.sh[Q.s[3]];
+    const int pb0 = A.sp[Q.bra].first, pk0 = A.sp[Q.ket].first;
+    const int npB = sB.nprim, npD = sD.nprim;
+    const long R0 = r0 / ndp;
+    // the p shells in rank order: position (0..3 = A..D), direction -> component map,
+    // coefficient offset and primitive count
+    int ppos[4] = {0, 0, 0, 0}, pof[4] = {0, 0, 0, 0}, pnp[4] = {1, 1, 1, 1};
+    unsigned dm[4] = {0, 0, 0, 0};
+    {
+        const Shell *sh4[4] = {&sA, &sB, &sC, &sD};
+        int j = 0;
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+            if (sh4[x]->l == 1) {
+                ppos[j & 3] = x;
+                pof[j & 3] = sh4[x]->poff;
+                pnp[j & 3] = sh4[x]->nprim;
+                dm[j & 3] = p_shell_dirmap(shell_powers(A, *sh4[x]));
+                ++j;
+            }
+    }
+    // the four node runs: node (lanes past the rule's last node: the last node,
+    // weight 0) and weight
+    int kr[4];
+    double w[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const int k = 16 * h + gl;
+        kr[h] = k < nq ? k : nq - 1;
+        w[h] = k < nq ? tw[kr[h]] : 0.0;
+    }
+    const int nh = (nq + 15) >> 4;
+    double acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+    // survivors in batches of 16: lane gl reads survivor gl of the batch and forms its
+    // coefficients (s shells and prefactor; each p shell's along x, y, z); the steps
+    // take them from that lane of the group (shuffle), U survivors and UH node runs
+    // per step with their row loads in flight together (a step past the batch's end
+    // repeats its last survivor with coefficient 0)
+    constexpr int U = P == 0 ? 2 : 1, UH = P <= 1 ? 4 : P == 2 ? 2 : 1;
+    const int gbase = threadIdx.x & 48;
+    for (long gb = 0; gb < ns; gb += 16) {
+        const int nb = ns - gb < 16 ? (int)(ns - gb) : 16;
+        double cs_l, cpd_l[3][P > 0 ? P : 1];
+        {
+            const Surv S = surv[s0 + gb + (gl < nb ? gl : nb - 1)];
+            const int ib = S.pb - pb0, ik = S.pk - pk0;
+            const int pi = ib / npB, pj = ib - pi * npB, qk = ik / npD, ql = ik - qk * npD;
+            double cs = gl < nb ? S.fac : 0.0;
+            cs *= sA.l == 0 ? A.coef[sA.poff + pi] : 1.0;
+            cs *= sB.l == 0 ? A.coef[sB.poff + pj] : 1.0;
+            cs *= sC.l == 0 ? A.coef[sC.poff + qk] : 1.0;
+            cs *= sD.l == 0 ? A.coef[sD.poff + ql] : 1.0;
+            cs_l = cs;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int pr = ppos[j] == 0 ? pi : ppos[j] == 1 ? pj : ppos[j] == 2 ? qk : ql;
+                const double *cj = A.coef + pof[j] + pr;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) cpd_l[d][j] = cj[(int)((dm[j] >> (2 * d)) & 3) * pnp[j]];
+            }
+        }
+        for (int i0 = 0; i0 < nb; i0 += U) {
+#pragma unroll
+            for (int v = 0; v < U; ++v) {
+                const int i = i0 + v;                          // <= 15: U divides 16
+                const double cs = __shfl(cs_l, gbase + i, 64);
+                double cpd[3][P > 0 ? P : 1];
+#pragma unroll
+                for (int j = 0; j < P; ++j)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) cpd[d][j] = __shfl(cpd_l[d][j], gbase + i, 64);
+                const long g = gb + (i < nb ? i : nb - 1);
+                const double *rp[3];
+#pragma unroll
+                for (int dd = 0; dd < 3; ++dd)
+                    rp[dd] = dd < ndp ? rowval + (size_t)(r0 + (g * ndp + dd) * NC) * nqp
+                                      : realval + (size_t)(R0 * nreal + (g * nreal + (dd - ndp)) * NC) * nqp;
+#pragma unroll UH
+                for (int h = 0; h < 4; ++h) {
+                    if (h >= nh) break;
+                    double X[NC], Y[NC], Z[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        X[c] = rp[0][(size_t)c * nqp + kr[h]];
+                        Y[c] = rp[1][(size_t)c * nqp + kr[h]];
+                        Z[c] = rp[2][(size_t)c * nqp + kr[h]];
+                    }
+                    const double cw = cs * w[h];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) X[c] *= cw;
+                    // p shell j: its direction-d coefficient into the direction-d rows with bit j set
+#pragma unroll
+                    for (int j = 0; j < P; ++j)
+#pragma unroll
+                        for (int c = 0; c < NC; ++c)
+                            if (c & (1 << j)) {
+                                X[c] *= cpd[0][j];
+                                Y[c] *= cpd[1][j];
+                                Z[c] *= cpd[2][j];
+                            }
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        int cx = 0, cy = 0, cz = 0, u = t;
+#pragma unroll
+                        for (int j = 0; j < P; ++j) {
+                            const int dir = u % 3;
+                            u /= 3;
+                            cx |= (dir == 0) << j;
+                            cy |= (dir == 1) << j;
+                            cz |= (dir == 2) << j;
+                        }
+                        acc[t] = fma(X[cx] * Y[cy], Z[cz], acc[t]);
+                    }
+                }
+            }
+        }
+    }
+    // group sums and the block index of each digit string
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        double v = acc[t];
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        int cp = 0, u = t, st = 1;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int dir = u % 3;
+            u /= 3;
+            cp += (int)((dm[j] >> (2 * dir)) & 3) * st;
+            st *= 3;
+        }
+        if (gl == 0) blk[cp] = v;
+    }
+    if (A.counters && gl == 0) {
+        atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)NC * nreal * 2 * nq *
+                                                                (sA.l + sB.l + sC.l + sD.l + 1) + 3ull * nq * NT));
+    }
+}
+
+// the quartets the s/p form takes: s and p shells, at most three p shells
+// (four: 81 accumulators and 48 row values per lane do not fit the registers),
+// and rules of at most 64 nodes
+__device__ __forceinline__ int sp_form(const EriArgs &A, const Quartet &Q, int qi) {
+    if (!A.sp_contract || rule_nodes(A, quartet_rule(A, qi)) > 64) return -1;
+    const int lA = A.sh[Q.s[0]].l, lB = A.sh[Q.s[1]].l, lC = A.sh[Q.s[2]].l, lD = A.sh[Q.s[3]].l;
+    if (lA > 1 || lB > 1 || lC > 1 || lD > 1) return -1;
+    const int P = lA + lB + lC + lD;
+    return P <= 3 ? P : -1;
+}
+
 // the stored position (ERI_toroidal.f90:433-449 index order, first index in
 // the unit cell) of component quartet f4 under permutation ip
 __device__ __forceinline__ void stored_position(const EriArgs &A, const int f4[4], int ip, int pp[4]) {
@@ -601,15 +807,17 @@ __device__ __forceinline__ void stored_position(const EriArgs &A, const int f4[4
 // ---------------------------------------------------------------------------
 // the finished block: written out (Schwarz passes) or scattered to the 8
 // permutations translated to the unit cell
+// (by nl lanes, lane = 0..nl-1)
 __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, const Shell &sA, const Shell &sB,
-                                            const Shell &sC, const Shell &sD, const double *blk, int lane) {
+                                            const Shell &sC, const Shell &sD, const double *blk, int lane,
+                                            int nl = 64) {
     const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
     const int nblk = nA * nB * nC * nD;
     if (A.blk_out && Q.out >= 0) {
-        for (int i = lane; i < nblk; i += 64) A.blk_out[Q.out + i] = blk[i];
+        for (int i = lane; i < nblk; i += nl) A.blk_out[Q.out + i] = blk[i];
     } else if (A.eri) {
         const size_t nb = A.nbas, n1 = A.n1;
-        for (int task = lane; task < nblk * 8; task += 64) {
+        for (int task = lane; task < nblk * 8; task += nl) {
             const int comp = task >> 3, ip = task & 7;
             int t = comp;
             const int ca = t % nA; t /= nA;
@@ -641,9 +849,67 @@ __global__ __launch_bounds__(256) void k_contract(EriArgs A, const long long *__
         const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
         const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
         wave_sync();
+        if (sp_form(A, Q, qi) >= 0) continue;    // k_contract_sp's
         quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, part, lane);
         wave_sync();
         write_block(A, Q, sA, sB, sC, sD, blk, lane);
+    }
+}
+
+// the s/p quartets into one list per P (lists[P * nq4 + i], cnt[P] entries): per
+// workgroup of 1024 the wavefronts' counts meet in LDS, one atomic per workgroup and P
+__global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict__ lists,
+                                                      unsigned *__restrict__ cnt) {
+    __shared__ unsigned wcnt[16][4], wbase[16][4];
+    const int qi = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int P = qi < A.nq4 ? sp_form(A, A.q4[qi], qi) : -1;
+    unsigned long long m[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        m[p] = __ballot(P == p);
+        if (lane == 0) wcnt[wv][p] = (unsigned)__popcll(m[p]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int p = threadIdx.x;
+        unsigned tot = 0;
+        for (int w = 0; w < 16; ++w) {
+            wbase[w][p] = tot;
+            tot += wcnt[w][p];
+        }
+        const unsigned b0 = tot ? atomicAdd(&cnt[p], tot) : 0u;
+        for (int w = 0; w < 16; ++w) wbase[w][p] += b0;
+    }
+    __syncthreads();
+    const unsigned long long mm = P == 0 ? m[0] : P == 1 ? m[1] : P == 2 ? m[2] : m[3];
+    if (P >= 0) lists[(size_t)P * A.nq4 + wbase[wv][P] + __popcll(mm & ((1ull << lane) - 1))] = qi;
+}
+
+// the s/p quartets with P p shells (list of cnt entries), 16 lanes per quartet,
+// 16 quartets per workgroup; one launch per P keeps each kernel's registers to
+// what its P needs
+template <int P>
+__global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__restrict__ list,
+                                                     const unsigned *__restrict__ cnt,
+                                                     const long long *__restrict__ soff,
+                                                     const long long *__restrict__ roff,
+                                                     const Surv *__restrict__ surv, const double *__restrict__ rowval,
+                                                     const double *__restrict__ realval) {
+    __shared__ double lds[16 * pow3(P)];
+    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    double *blk = lds + grp * pow3(P);
+    const long n = *cnt;
+    for (long e0 = (long)blockIdx.x * 16; e0 < n; e0 += (long)gridDim.x * 16) {
+        const long e = e0 + grp;
+        const int qi = e < n ? list[e] : 0;
+        const Quartet Q = A.q4[qi];
+        if (e < n) {
+            const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
+            quartet_block_sp<P>(A, qi, Q, s0, ns, r0, surv, rowval, realval, blk, gl);
+        }
+        wave_sync();
+        if (e < n) write_block(A, Q, A.sh[Q.s[0]], A.sh[Q.s[1]], A.sh[Q.s[2]], A.sh[Q.s[3]], blk, gl, 16);
+        wave_sync();
     }
 }
 
@@ -820,7 +1086,7 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
 }
 
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                           const double *rowval, const double *realval, hipStream_t st) {
+                           const double *rowval, const double *realval, int *lists, unsigned *cnt, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
     const int nwg = (a.nq4 + 3) / 4;
     const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
@@ -829,7 +1095,28 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
         hipError_t e = hipFuncSetAttribute((const void *)k_contract, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, a, soff, roff, surv, rowval, realval);
+    // short quartet lists (the Schwarz passes: few quartets, up to nprim^4 survivors
+    // each) stay with the general kernel, one wavefront per quartet
+    EriArgs b = a;
+    if (b.nq4 < b.sp_min_quartets) b.sp_contract = 0;
+    if (b.sp_contract) {
+        // s/p quartets: lists by P, then one kernel per P (grids for the largest list)
+        hipError_t e = hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_sp_classify, dim3((a.nq4 + 1023) / 1024), dim3(1024), 0, st, b, lists, cnt);
+        const int g16 = (a.nq4 + 15) / 16, gsp = g16 < (1 << 20) ? g16 : (1 << 20);
+        hipLaunchKernelGGL(k_contract_sp<0>, dim3(gsp), dim3(256), 0, st, b, lists, cnt, soff, roff, surv, rowval,
+                           realval);
+        hipLaunchKernelGGL(k_contract_sp<1>, dim3(gsp), dim3(256), 0, st, b, lists + (size_t)a.nq4, cnt + 1, soff,
+                           roff, surv, rowval, realval);
+        hipLaunchKernelGGL(k_contract_sp<2>, dim3(gsp), dim3(256), 0, st, b, lists + 2 * (size_t)a.nq4, cnt + 2, soff,
+                           roff, surv, rowval, realval);
+        hipLaunchKernelGGL(k_contract_sp<3>, dim3(gsp), dim3(256), 0, st, b, lists + 3 * (size_t)a.nq4, cnt + 3, soff,
+                           roff, surv, rowval, realval);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval);
     return hipGetLastError();
 }
 
