@@ -626,9 +626,9 @@ __device__ __forceinline__ unsigned p_shell_dirmap(unsigned long long pw) {
     return m;
 }
 
-// the block of one s/p quartet by the 16 lanes of a group (gl = 0..15): the
-// rule's nodes (at most 64) in four runs of 16
-template <int P>
+// the block of one s/p quartet by the GL lanes of a group (gl = 0..GL-1): the
+// rule's nodes (at most 64) in 64 / GL runs of GL
+template <int P, int GL>
 __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const Quartet &Q, long s0, long ns,
                                                  long r0, const Surv *__restrict__ surv,
                                                  const double *__restrict__ rowval,
@@ -658,29 +658,29 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
                 ++j;
             }
     }
-    // the four node runs: node (lanes past the rule's last node: the last node,
-    // weight 0) and weight
-    int kr[4];
-    double w[4];
+    // the node runs: node (lanes past the rule's last node: the last node, weight 0)
+    // and weight
+    constexpr int NR = 64 / GL;
+    int kr[NR];
+    double w[NR];
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        const int k = 16 * h + gl;
+    for (int h = 0; h < NR; ++h) {
+        const int k = GL * h + gl;
         kr[h] = k < nq ? k : nq - 1;
         w[h] = k < nq ? tw[kr[h]] : 0.0;
     }
-    const int nh = (nq + 15) >> 4;
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-    // survivors in batches of 16: lane gl reads survivor gl of the batch and forms its
+    // survivors in batches of GL: lane gl reads survivor gl of the batch and forms its
     // coefficients (s shells and prefactor; each p shell's along x, y, z); the steps
     // take them from that lane of the group (shuffle), U survivors and UH node runs
     // per step with their row loads in flight together (a step past the batch's end
     // repeats its last survivor with coefficient 0)
     constexpr int U = P == 0 ? 2 : 1, UH = P <= 1 ? 4 : P == 2 ? 2 : 1;
-    const int gbase = threadIdx.x & 48;
-    for (long gb = 0; gb < ns; gb += 16) {
-        const int nb = ns - gb < 16 ? (int)(ns - gb) : 16;
+    const int gbase = threadIdx.x & (64 - GL);
+    for (long gb = 0; gb < ns; gb += GL) {
+        const int nb = ns - gb < GL ? (int)(ns - gb) : GL;
         double cs_l, cpd_l[3][P > 0 ? P : 1];
         {
             const Surv S = surv[s0 + gb + (gl < nb ? gl : nb - 1)];
@@ -701,58 +701,70 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
             }
         }
         for (int i0 = 0; i0 < nb; i0 += U) {
+            // the U survivors' coefficients and row addresses
+            double cs[U], cpd[U][3][P > 0 ? P : 1];
+            const double *rp[U][3];
 #pragma unroll
             for (int v = 0; v < U; ++v) {
-                const int i = i0 + v;                          // <= 15: U divides 16
-                const double cs = __shfl(cs_l, gbase + i, 64);
-                double cpd[3][P > 0 ? P : 1];
+                const int i = i0 + v;                          // < GL: U divides GL
+                cs[v] = __shfl(cs_l, gbase + i, 64);
 #pragma unroll
                 for (int j = 0; j < P; ++j)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) cpd[d][j] = __shfl(cpd_l[d][j], gbase + i, 64);
+                    for (int d = 0; d < 3; ++d) cpd[v][d][j] = __shfl(cpd_l[d][j], gbase + i, 64);
                 const long g = gb + (i < nb ? i : nb - 1);
-                const double *rp[3];
 #pragma unroll
                 for (int dd = 0; dd < 3; ++dd)
-                    rp[dd] = dd < ndp ? rowval + (size_t)(r0 + (g * ndp + dd) * NC) * nqp
-                                      : realval + (size_t)(R0 * nreal + (g * nreal + (dd - ndp)) * NC) * nqp;
-#pragma unroll UH
-                for (int h = 0; h < 4; ++h) {
-                    if (h >= nh) break;
-                    double X[NC], Y[NC], Z[NC];
+                    rp[v][dd] = dd < ndp ? rowval + (size_t)(r0 + (g * ndp + dd) * NC) * nqp
+                                         : realval + (size_t)(R0 * nreal + (g * nreal + (dd - ndp)) * NC) * nqp;
+            }
+            // UH node runs at a time: every load first (runs past the rule's last node
+            // load its last node, weight 0), then the products
+            for (int h0 = 0; h0 < NR; h0 += UH) {
+                double X[U][UH][NC], Y[U][UH][NC], Z[U][UH][NC];
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        X[c] = rp[0][(size_t)c * nqp + kr[h]];
-                        Y[c] = rp[1][(size_t)c * nqp + kr[h]];
-                        Z[c] = rp[2][(size_t)c * nqp + kr[h]];
-                    }
-                    const double cw = cs * w[h];
+                for (int v = 0; v < U; ++v)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) X[c] *= cw;
-                    // p shell j: its direction-d coefficient into the direction-d rows with bit j set
+                    for (int hh = 0; hh < UH; ++hh)
 #pragma unroll
-                    for (int j = 0; j < P; ++j)
-#pragma unroll
-                        for (int c = 0; c < NC; ++c)
-                            if (c & (1 << j)) {
-                                X[c] *= cpd[0][j];
-                                Y[c] *= cpd[1][j];
-                                Z[c] *= cpd[2][j];
-                            }
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        int cx = 0, cy = 0, cz = 0, u = t;
-#pragma unroll
-                        for (int j = 0; j < P; ++j) {
-                            const int dir = u % 3;
-                            u /= 3;
-                            cx |= (dir == 0) << j;
-                            cy |= (dir == 1) << j;
-                            cz |= (dir == 2) << j;
+                        for (int c = 0; c < NC; ++c) {
+                            const int k = kr[h0 + hh];
+                            X[v][hh][c] = rp[v][0][(size_t)c * nqp + k];
+                            Y[v][hh][c] = rp[v][1][(size_t)c * nqp + k];
+                            Z[v][hh][c] = rp[v][2][(size_t)c * nqp + k];
                         }
-                        acc[t] = fma(X[cx] * Y[cy], Z[cz], acc[t]);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int v = 0; v < U; ++v)
+#pragma unroll
+                    for (int hh = 0; hh < UH; ++hh) {
+                        const double cw = cs[v] * w[h0 + hh];
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) X[v][hh][c] *= cw;
+                        // p shell j: its direction-d coefficient into the direction-d rows with bit j set
+#pragma unroll
+                        for (int j = 0; j < P; ++j)
+#pragma unroll
+                            for (int c = 0; c < NC; ++c)
+                                if (c & (1 << j)) {
+                                    X[v][hh][c] *= cpd[v][0][j];
+                                    Y[v][hh][c] *= cpd[v][1][j];
+                                    Z[v][hh][c] *= cpd[v][2][j];
+                                }
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) {
+                            int cx = 0, cy = 0, cz = 0, u = t;
+#pragma unroll
+                            for (int j = 0; j < P; ++j) {
+                                const int dir = u % 3;
+                                u /= 3;
+                                cx |= (dir == 0) << j;
+                                cy |= (dir == 1) << j;
+                                cz |= (dir == 2) << j;
+                            }
+                            acc[t] = fma(X[v][hh][cx] * Y[v][hh][cy], Z[v][hh][cz], acc[t]);
+                        }
                     }
-                }
             }
         }
     }
@@ -761,7 +773,7 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
     for (int t = 0; t < NT; ++t) {
         double v = acc[t];
 #pragma unroll
-        for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        for (int off = GL / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         int cp = 0, u = t, st = 1;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
@@ -885,32 +897,42 @@ __global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict
     if (P >= 0) lists[(size_t)P * A.nq4 + wbase[wv][P] + __popcll(mm & ((1ull << lane) - 1))] = qi;
 }
 
-// the s/p quartets with P p shells (list of cnt entries), 16 lanes per quartet,
-// 16 quartets per workgroup; one launch per P keeps each kernel's registers to
-// what its P needs
-template <int P>
+// the s/p quartets with P p shells (list of cnt entries), GL lanes per quartet,
+// 256 / GL quartets per workgroup; one launch per P keeps each kernel's registers
+// to what its P needs
+template <int P, int GL>
 __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__restrict__ list,
                                                      const unsigned *__restrict__ cnt,
                                                      const long long *__restrict__ soff,
                                                      const long long *__restrict__ roff,
                                                      const Surv *__restrict__ surv, const double *__restrict__ rowval,
                                                      const double *__restrict__ realval) {
-    __shared__ double lds[16 * pow3(P)];
-    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    constexpr int NG = 256 / GL;
+    __shared__ double lds[NG * pow3(P)];
+    const int grp = threadIdx.x / GL, gl = threadIdx.x & (GL - 1);
     double *blk = lds + grp * pow3(P);
     const long n = *cnt;
-    for (long e0 = (long)blockIdx.x * 16; e0 < n; e0 += (long)gridDim.x * 16) {
+    for (long e0 = (long)blockIdx.x * NG; e0 < n; e0 += (long)gridDim.x * NG) {
         const long e = e0 + grp;
         const int qi = e < n ? list[e] : 0;
         const Quartet Q = A.q4[qi];
         if (e < n) {
             const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
-            quartet_block_sp<P>(A, qi, Q, s0, ns, r0, surv, rowval, realval, blk, gl);
+            quartet_block_sp<P, GL>(A, qi, Q, s0, ns, r0, surv, rowval, realval, blk, gl);
         }
         wave_sync();
-        if (e < n) write_block(A, Q, A.sh[Q.s[0]], A.sh[Q.s[1]], A.sh[Q.s[2]], A.sh[Q.s[3]], blk, gl, 16);
+        if (e < n) write_block(A, Q, A.sh[Q.s[0]], A.sh[Q.s[1]], A.sh[Q.s[2]], A.sh[Q.s[3]], blk, gl, GL);
         wave_sync();
     }
+}
+
+template <int P, int GL>
+static void launch_sp(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
+                      const long long *roff, const Surv *surv, const double *rowval, const double *realval,
+                      hipStream_t st) {
+    const int ng = (b.nq4 + 256 / GL - 1) / (256 / GL), grid = ng < (1 << 20) ? ng : (1 << 20);
+    hipLaunchKernelGGL((k_contract_sp<P, GL>), dim3(grid), dim3(256), 0, st, b, lists + (size_t)P * b.nq4, cnt + P,
+                       soff, roff, surv, rowval, realval);
 }
 
 size_t contract_lds_bytes(const EriArgs &a) { return 4 * sizeof(double) * ((size_t)a.max_blk + 64); }
@@ -1104,15 +1126,10 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
         hipError_t e = hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_sp_classify, dim3((a.nq4 + 1023) / 1024), dim3(1024), 0, st, b, lists, cnt);
-        const int g16 = (a.nq4 + 15) / 16, gsp = g16 < (1 << 20) ? g16 : (1 << 20);
-        hipLaunchKernelGGL(k_contract_sp<0>, dim3(gsp), dim3(256), 0, st, b, lists, cnt, soff, roff, surv, rowval,
-                           realval);
-        hipLaunchKernelGGL(k_contract_sp<1>, dim3(gsp), dim3(256), 0, st, b, lists + (size_t)a.nq4, cnt + 1, soff,
-                           roff, surv, rowval, realval);
-        hipLaunchKernelGGL(k_contract_sp<2>, dim3(gsp), dim3(256), 0, st, b, lists + 2 * (size_t)a.nq4, cnt + 2, soff,
-                           roff, surv, rowval, realval);
-        hipLaunchKernelGGL(k_contract_sp<3>, dim3(gsp), dim3(256), 0, st, b, lists + 3 * (size_t)a.nq4, cnt + 3, soff,
-                           roff, surv, rowval, realval);
+        launch_sp<0, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
+        launch_sp<1, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
+        launch_sp<2, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
+        launch_sp<3, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
