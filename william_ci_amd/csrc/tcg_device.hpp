@@ -86,7 +86,7 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
                             long long ns, double *realval, hipStream_t st);
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                               const double *rowval, const double *realval, hipStream_t st);
-// lists: 4 * a.nq4 ints, cnt: 4 counters (work space of the s/p contraction)
+// lists: 5 * a.nq4 ints, cnt: 5 counters (work space of the s/p contraction)
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                            const double *rowval, const double *realval, int *lists, unsigned *cnt, hipStream_t st);
 hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, int nchunk, int *bounds, hipStream_t st);

@@ -571,8 +571,8 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     if (A.Jacc)
         HIPCHK(c, launch_contract_jk(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
     else {
-        HIPCHK(c, c->d_cls.alloc(4 * (size_t)A.nq4));
-        HIPCHK(c, c->d_clscnt.alloc(4));
+        HIPCHK(c, c->d_cls.alloc(5 * (size_t)A.nq4));
+        HIPCHK(c, c->d_clscnt.alloc(5));
         HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->d_cls.p, c->d_clscnt.p,
                                   c->st));
     }

@@ -150,7 +150,10 @@ __global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *_
             // the phase factors of the row, once per row (the MFMA kernel needs no sincos)
             const double phi = ad * (pb.cen[d] - (pk.cen[d] + shd));
             sincos(phi, &R.s1, &R.c1);
-            sincos(4.0 * phi, &R.s4, &R.c4);
+            // e^{4 i phi} by two doublings of e^{i phi} (a few ulps); e^{64 i phi} directly
+            const double c2 = (R.c1 - R.s1) * (R.c1 + R.s1), s2 = 2.0 * R.s1 * R.c1;
+            R.c4 = (c2 - s2) * (c2 + s2);
+            R.s4 = 2.0 * s2 * c2;
             sincos(64.0 * phi, &R.s64, &R.c64);
             R.sgn = pow(ad, -(double)(ia + ib + ic + id));
             const int rule = quartet_rule(A, qi);
@@ -208,9 +211,9 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
             atomicAdd(&A.counters[4], (unsigned long long)(2 * s + 10 * (nm + 1)));
         }
         // e^{i n phi}: rotation by e^{4 i phi} inside each block of 64 orders; at a block start
-        // the lane's base phase advances by e^{64 i phi} (the row's phase factors, from sincos
-        // once per row), so the rounding error stays within a few tens of ulps for n up to
-        // ~1300; the lane's own e^{i kq phi} is a product of at most three e^{i phi}
+        // the lane's base phase advances by e^{64 i phi} (the row's phase factors, formed once
+        // per row by the emit kernel), so the rounding error stays within a few tens of ulps
+        // for n up to ~1300; the lane's own e^{i kq phi} is a product of at most three e^{i phi}
         double cn0 = 1.0, sn0 = 0.0;
 #pragma unroll
         for (int q = 1; q <= 3; ++q) {
@@ -346,6 +349,13 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
     return hipSuccess;
 }
 
+// lane l's value of v (l wavefront-uniform)
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // LDS writes of the wavefront visible to its other lanes (for LDS that one
 // wavefront owns; no wait for outstanding global loads)
 __device__ __forceinline__ void wave_sync() {
@@ -448,27 +458,49 @@ __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__
             const double tk = k < nq ? A.tnode[rule * nqp + k] : 0.0;
             t2v[h] = tk * tk;
         }
-        const long long nit = ns * nreal;
-        for (long long it = blockIdx.y; it < nit; it += gridDim.y) {
-            const long long gl = it / nreal;
-            const int drr = (int)(it - gl * nreal);
-            const int dr = A.ndp + drr;
-            const Surv S = surv[s0 + gl];
-            const PrimPair &pb = A.pp[S.pb];
-            const PrimPair &pk = A.pp[S.pk];
-            const double Rr = pb.cen[dr] - (pk.cen[dr] + shift[dr]);
-            const double *Eb = A.herm + pb.eoff + (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
-            const double *Ek = A.herm + pk.eoff + (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
-            double *out = realval + ((size_t)(R0 * nreal + (gl * nreal + drr) * ncomb)) * nqp;
+        // survivors in batches of 64: lane i reads survivor i of the batch (exponent
+        // sums, distance per real direction, Hermite offsets); the steps take them from
+        // that lane (readlane), so a step waits only on its Hermite coefficients
+        for (long long gb = 0; gb < ns; gb += 64) {
+            const int nb = ns - gb < 64 ? (int)(ns - gb) : 64;
+            double bq, kq, Rl[2] = {0.0, 0.0};
+            int ebo, eko;
+            {
+                const Surv S = surv[s0 + gb + (lane < nb ? lane : nb - 1)];
+                const PrimPair &pb = A.pp[S.pb];
+                const PrimPair &pk = A.pp[S.pk];
+                bq = pb.q;
+                kq = pk.q;
+                ebo = pb.eoff;
+                eko = pk.eoff;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int k = lane + 64 * h;
-                if (h * 64 >= nq) break;
-                double gt[9];
-                real_gt(pb.q, pk.q, t2v[h], Rr, NN, gt);
-                real_combos(lA, lB, lC, lD, Eb, Ek, gt, [&](int combo, double v) {
-                    if (k < nq) out[(size_t)combo * nqp + k] = v;
-                });
+                for (int drr = 0; drr < 2; ++drr)
+                    if (drr < nreal) {
+                        const int dr = A.ndp + drr;
+                        Rl[drr] = pb.cen[dr] - (pk.cen[dr] + shift[dr]);
+                    }
+            }
+            const int nit = nb * nreal;
+            for (int it = blockIdx.y; it < nit; it += gridDim.y) {
+                const int i = it / nreal, drr = it - i * nreal;
+                const long long gl = gb + i;
+                const double pq = readlane_d(bq, i), qq = readlane_d(kq, i);
+                const double Rr = readlane_d(drr == 0 ? Rl[0] : Rl[1], i);
+                const double *Eb = A.herm + __builtin_amdgcn_readlane(ebo, i) +
+                                   (size_t)drr * (lA + 1) * (lB + 1) * (2 * kMaxL + 1);
+                const double *Ek = A.herm + __builtin_amdgcn_readlane(eko, i) +
+                                   (size_t)drr * (lC + 1) * (lD + 1) * (2 * kMaxL + 1);
+                double *out = realval + ((size_t)(R0 * nreal + (gl * nreal + drr) * ncomb)) * nqp;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int k = lane + 64 * h;
+                    if (h * 64 >= nq) break;
+                    double gt[9];
+                    real_gt(pq, qq, t2v[h], Rr, NN, gt);
+                    real_combos(lA, lB, lC, lD, Eb, Ek, gt, [&](int combo, double v) {
+                        if (k < nq) out[(size_t)combo * nqp + k] = v;
+                    });
+                }
             }
         }
     }
@@ -850,39 +882,46 @@ __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, 
 __global__ __launch_bounds__(256) void k_contract(EriArgs A, const long long *__restrict__ soff,
                                                   const long long *__restrict__ roff, const Surv *__restrict__ surv,
                                                   const double *__restrict__ rowval,
-                                                  const double *__restrict__ realval) {
-    // four wavefronts per workgroup, one shell quartet each; per wavefront in LDS a
-    // block of max_blk and 64 partial sums
+                                                  const double *__restrict__ realval, const int *__restrict__ list,
+                                                  const unsigned *__restrict__ cnt) {
+    // four wavefronts per workgroup, one shell quartet each (every quartet, or the
+    // cnt entries of list); per wavefront in LDS a block of max_blk and 64 partial sums
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double *blk = lds + (size_t)wv * (A.max_blk + 64), *part = blk + A.max_blk;
-    for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
+    const int n = list ? (int)*cnt : A.nq4;
+    for (int e = blockIdx.x * 4 + wv; e < n; e += gridDim.x * 4) {
+        const int qi = list ? list[e] : e;
         const Quartet Q = A.q4[qi];
         const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
         const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
         wave_sync();
-        if (sp_form(A, Q, qi) >= 0) continue;    // k_contract_sp's
         quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, part, lane);
         wave_sync();
         write_block(A, Q, sA, sB, sC, sD, blk, lane);
     }
 }
 
-// the s/p quartets into one list per P (lists[P * nq4 + i], cnt[P] entries): per
-// workgroup of 1024 the wavefronts' counts meet in LDS, one atomic per workgroup and P
+// the quartets into one list per s/p form P = 0..3 and one (P = 4) of the rest
+// (lists[P * nq4 + i], cnt[P] entries): per workgroup of 1024 the wavefronts'
+// counts meet in LDS, one atomic per workgroup and list
 __global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict__ lists,
                                                       unsigned *__restrict__ cnt) {
-    __shared__ unsigned wcnt[16][4], wbase[16][4];
+    __shared__ unsigned wcnt[16][5], wbase[16][5];
     const int qi = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int P = qi < A.nq4 ? sp_form(A, A.q4[qi], qi) : -1;
-    unsigned long long m[4];
+    int P = -1;
+    if (qi < A.nq4) {
+        P = sp_form(A, A.q4[qi], qi);
+        if (P < 0) P = 4;
+    }
+    unsigned long long m[5];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
+    for (int p = 0; p < 5; ++p) {
         m[p] = __ballot(P == p);
         if (lane == 0) wcnt[wv][p] = (unsigned)__popcll(m[p]);
     }
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < 5) {
         const int p = threadIdx.x;
         unsigned tot = 0;
         for (int w = 0; w < 16; ++w) {
@@ -893,7 +932,7 @@ __global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict
         for (int w = 0; w < 16; ++w) wbase[w][p] += b0;
     }
     __syncthreads();
-    const unsigned long long mm = P == 0 ? m[0] : P == 1 ? m[1] : P == 2 ? m[2] : m[3];
+    const unsigned long long mm = P == 0 ? m[0] : P == 1 ? m[1] : P == 2 ? m[2] : P == 3 ? m[3] : m[4];
     if (P >= 0) lists[(size_t)P * A.nq4 + wbase[wv][P] + __popcll(mm & ((1ull << lane) - 1))] = qi;
 }
 
@@ -1123,17 +1162,20 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
     if (b.nq4 < b.sp_min_quartets) b.sp_contract = 0;
     if (b.sp_contract) {
         // s/p quartets: lists by P, then one kernel per P (grids for the largest list)
-        hipError_t e = hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), st);
+        hipError_t e = hipMemsetAsync(cnt, 0, 5 * sizeof(unsigned), st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_sp_classify, dim3((a.nq4 + 1023) / 1024), dim3(1024), 0, st, b, lists, cnt);
         launch_sp<0, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
         launch_sp<1, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
         launch_sp<2, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
         launch_sp<3, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        // the rest (d shells, four p shells) from list 4
+        hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval,
+                           lists + 4 * (size_t)b.nq4, cnt + 4);
+        return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval);
+    hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval, nullptr,
+                       nullptr);
     return hipGetLastError();
 }
 
