@@ -88,6 +88,7 @@ struct Shell {
     int poff;            // offset into expo/coef of the first AO's primitives
     int cell;            // cell index
     int loc;             // local index inside the cell
+    unsigned dirmap;     // p shell: component along direction d in bits 2d..2d+1 (0 otherwise)
     double xyz[3];
 };
 

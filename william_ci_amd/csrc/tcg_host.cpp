@@ -345,6 +345,10 @@ static int build_shells(tcg_ctx *c) {
         s.nprim = c->poff[i + 1] - c->poff[i];
         s.poff = c->poff[i];
         for (int d = 0; d < 3; ++d) s.xyz[d] = c->center[3 * i + d];
+        if (lt == 1)
+            for (int x = 0; x < nc; ++x)
+                for (int d = 0; d < 3; ++d)
+                    if (c->lxyz[3 * (i + x) + d] == 1) s.dirmap |= (unsigned)x << (2 * d);
         if (lt > 2) return c->fail(TCG_E_ARG, "shell at AO %d: l = %d; this engine handles s, p, d", i, lt);
         if (s.nprim > kMaxPrim) return c->fail(TCG_E_ARG, "shell at AO %d: %d primitives (max %d)", i, s.nprim, kMaxPrim);
         c->sh.push_back(s);

@@ -646,18 +646,6 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int pow3(int p) { return p == 0 ? 1 : 3 * pow3(p - 1); }
 
-// component (0..2) of a p shell pointing along each direction: 2 bits per direction
-__device__ __forceinline__ unsigned p_shell_dirmap(unsigned long long pw) {
-    unsigned m = 0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const unsigned w = (unsigned)(pw >> (6 * c));
-        const unsigned dir = (w & 3) ? 0u : ((w >> 2) & 3) ? 1u : 2u;
-        m |= (unsigned)c << (2 * dir);
-    }
-    return m;
-}
-
 // the block of one s/p quartet by the GL lanes of a group (gl = 0..GL-1): the
 // rule's nodes (at most 64) in 64 / GL runs of GL
 template <int P, int GL>
@@ -686,7 +674,7 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
                 ppos[j & 3] = x;
                 pof[j & 3] = sh4[x]->poff;
                 pnp[j & 3] = sh4[x]->nprim;
-                dm[j & 3] = p_shell_dirmap(shell_powers(A, *sh4[x]));
+                dm[j & 3] = sh4[x]->dirmap;
                 ++j;
             }
     }
