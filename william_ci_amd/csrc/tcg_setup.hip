@@ -234,7 +234,7 @@ hipError_t launch_schwarz_tables(const Quartet *q4, int nquart, int nshell, cons
 // screens, and (SI,SJ,SK,SL) the smallest member of its orbit under the 8
 // index permutations x the lattice translations.  One thread per candidate
 // (SI, SJ, SK, SL); pass 0 counts (one atomic per wavefront), pass 1 appends.
-__global__ void k_enum_quartets(const Shell *__restrict__ sh, const int *__restrict__ relc,
+__global__ __launch_bounds__(1024) void k_enum_quartets(const Shell *__restrict__ sh, const int *__restrict__ relc,
                                 const double *__restrict__ Qsh, const unsigned long long *__restrict__ qmax_bits,
                                 int nshuc, int nsh, int ncell,
                                 unsigned long long *__restrict__ count, Quartet *__restrict__ out,
@@ -268,15 +268,25 @@ __global__ void k_enum_quartets(const Shell *__restrict__ sh, const int *__restr
             }
         }
     }
+    // positions: per workgroup the wavefronts' counts meet in LDS, one atomic per workgroup
+    __shared__ unsigned wcnt[16];
+    __shared__ unsigned long long wbase[16];
     const unsigned long long m = __ballot(keep);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    if (lane == 0) wcnt[wv] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < nwv; ++w) {
+            wbase[w] = tot;
+            tot += wcnt[w];
+        }
+        const unsigned long long b0 = tot ? atomicAdd(count, tot) : 0ull;
+        for (int w = 0; w < nwv; ++w) wbase[w] += b0;
+    }
+    __syncthreads();
     if (!keep || !out) return;
-    const unsigned long long pos = base + __popcll(m & ((1ull << lane) - 1ull));
+    const unsigned long long pos = wbase[wv] + __popcll(m & ((1ull << lane) - 1ull));
     if (pos >= cap) return;
     const int lK = sh[SK].loc;
     Quartet Q;
@@ -294,9 +304,9 @@ hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *
                                 int nsh, int ncell, unsigned long long *count, Quartet *out,
                                 unsigned long long cap, hipStream_t st) {
     const long long n = (long long)nshuc * nsh * nsh * nsh;
-    const long long blocks = (n + 255) / 256;
+    const long long blocks = (n + 1023) / 1024;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_enum_quartets, dim3((unsigned)blocks), dim3(256), 0, st, sh, relc, Qsh, qmax_bits, nshuc, nsh,
+    hipLaunchKernelGGL(k_enum_quartets, dim3((unsigned)blocks), dim3(1024), 0, st, sh, relc, Qsh, qmax_bits, nshuc, nsh,
                        ncell, count, out, cap);
     return hipGetLastError();
 }
