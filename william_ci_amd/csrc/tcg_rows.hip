@@ -184,6 +184,7 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
     const int nqp = A.nqp, nht = A.nqp / kNodeTile;
     const int tabrows = A.ncap + 5;
     const long long ntiles = (nrows + 15) / 16;
+    long long flops = 0;                              // this lane's rows' flop count (one atomic per wavefront)
     for (long long tile = (long long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long long)gridDim.x * 4) {
         const long long pos = tile * 16 + r;
         const bool valid = pos < nrows;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
         if (A.counters && kq == 0 && valid && hbase == 0) {
             const int rl = d >> 8, dl = d & 0xff;
             const long long s = A.tsum[(size_t)(rl * A.ndp + dl) * (A.ncap + 1) + (nm < A.ncap ? nm : A.ncap)];
-            atomicAdd(&A.counters[4], (unsigned long long)(2 * s + 10 * (nm + 1)));
+            flops += 2 * s + 10 * (nm + 1);
         }
         // e^{i n phi}: rotation by e^{4 i phi} inside each block of 64 orders; at a block start
         // the lane's base phase advances by e^{64 i phi} (the row's phase factors, formed once
@@ -325,6 +326,11 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
                 }
             }
         }
+    }
+    if (A.counters) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) flops += __shfl_xor(flops, off, 64);
+        if (lane == 0 && flops) atomicAdd(&A.counters[4], (unsigned long long)flops);
     }
 }
 
