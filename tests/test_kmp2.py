@@ -29,6 +29,6 @@ def test_kmp2_oracle_matches_reference_and_gamma():
     # Gamma supercell MP2 with the supercell RHF on the full ERI: the same energy
     Ef, _ = o.eri_full(sao=False)
     ch = Ef.transpose(3, 2, 1, 0)
-    _, e, C, _, _ = orhf.rhf_gamma(ch, S, T, V, s.n_occ, scf.nuclear_repulsion(s))
+    _, e, C, _, _ = orhf.rhf_gamma(ch, S, T, V, s.n_occ, r.nre())
     eg = orhf.mp2_gamma(ch, e, C, s.n_occ)
     assert abs(ec - eg) < 1e-9

@@ -43,7 +43,9 @@ def test_full_eri_matches_reference():
     assert Eg.shape == Er.shape and np.abs(Eg - Er).max() <= 1e-10
 
 
+@needs_ref
 def test_gamma_rhf_matches_oracle_and_bloch():
+    from oracle import ref
     from william_ci_amd import scf
     s = tsys.workload("h10_ring")
     coef = normalised_coef(s)
@@ -52,11 +54,14 @@ def test_gamma_rhf_matches_oracle_and_bloch():
     eng.eri_build(sao=True, to_host=False)          # the SAO layout for the Bloch-basis RHF
     e_bloch = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P)).energy
     eng.close()
-    Eo, _ = orc.OracleSystem(s, coef).eri_full(sao=False)
     print(f"  E gamma {res.energy:.12f}  E bloch {e_bloch:.12f}  iterations {res.iterations}")
     assert res.converged and abs(res.energy - e_bloch) < 1e-9
-    ch = Eo.transpose(3, 2, 1, 0)
-    e_or = orhf.rhf_gamma(ch, S, T, V, s.n_occ, scf.nuclear_repulsion(s))[0]
+    # the pinned Gamma oracle on the reference's own integrals and NRE
+    r = ref.RefSystem(s, sao=False)
+    Sr, Tr, Vr = r.one_electron()
+    ch = r.eri_full().transpose(3, 2, 1, 0)
+    e_or = orhf.rhf_gamma(ch, Sr, Tr, Vr, s.n_occ, r.nre())[0]
+    print(f"  E oracle (reference integrals) {e_or:.12f}  diff {res.energy - e_or:.2e}")
     assert abs(res.energy - e_or) < 1e-9
 
 

@@ -24,6 +24,7 @@ int ctx_shells(tcg_ctx *c, std::vector<int> &first_ao, std::vector<int> &ncomp, 
 // Everything the quartet kernel reads.
 struct EriArgs {
     int ndp, nq, nqp, ncap;           // periodic directions, nodes, nodes padded to 16, Bessel cap
+    int nm_bits;                      // bits of a row's order field in the sort key (ncap + 1 fits)
     double a[3], L[3];
     double fac_const;                 // 2/sqrt(pi) prod_d L_d^2
     // quadrature rules (rule 0 = the run's rule; more with learned per-quartet orders),

@@ -509,6 +509,8 @@ static EriArgs base_args(tcg_ctx *c) {
     A.rule_nq = c->d_rule_nq.p;
     A.qrule = nullptr;
     A.ncap = c->ncap;
+    A.nm_bits = 1;
+    while ((1 << A.nm_bits) <= c->ncap) ++A.nm_bits;   // a row's order is at most ncap
     A.fac_const = 2.0 / std::sqrt(kPi);
     for (int d = 0; d < 3; ++d) {
         A.a[d] = c->a[d];
@@ -559,7 +561,9 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     HIPCHK(c, launch_rows_emit(A, so, ro, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
     int rbits = 0;
     while ((1 << rbits) < (int)c->rule_nq.size()) ++rbits;
-    const int end_bit = 18 + (A.qrule ? rbits : 0);
+    const int end_bit = A.nm_bits + 2 + (A.qrule ? rbits : 0);
+    if (end_bit > 32) return c->fail(TCG_E_ARG, "row sort key needs %d bits (ncap %d, %d rules)", end_bit, c->ncap,
+                                     (int)c->rule_nq.size());
     size_t tb = sort_temp_bytes((int)nr, end_bit);
     HIPCHK(c, c->d_tmp.alloc(tb));
     HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, end_bit,

@@ -10,7 +10,7 @@
 //   k_rows_emit   one ROW per (surviving primitive quartet, periodic direction d,
 //                 power combination): offsets of its bra/ket Fourier vectors,
 //                 length nm = min(N_bra, N_ket), phase phi, power sum; sort key
-//                 (d, nm).
+//                 (rule, d, nm).
 //   (radix sort of the rows by key, tcg_sort.hip)
 //   k_rows_mfma   one wavefront per 16 consecutive sorted rows:
 //                 I_d(t_k) = sum_n Re[conj(G_n) H_n e^{in phi}] T_d[n][k]
@@ -159,7 +159,9 @@ __global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *_
             const int rule = quartet_rule(A, qi);
             R.d = d | (rule << 8);
             rows[r0 + rl] = R;
-            keys[r0 + rl] = ((unsigned)rule << 18) | ((unsigned)d << 16) | (unsigned)R.nm;
+            // key (rule, direction, order): the order field holds ncap + 1 (nm_bits), the
+            // direction two bits above it, the rule above both
+            keys[r0 + rl] = ((unsigned)rule << (A.nm_bits + 2)) | ((unsigned)d << A.nm_bits) | (unsigned)R.nm;
             vals[r0 + rl] = (int)(r0 + rl);
         }
     }

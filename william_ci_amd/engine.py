@@ -183,9 +183,18 @@ class Engine:
         Pf = np.ascontiguousarray(np.asarray(P, dtype=np.float64).T)
         J = np.zeros(nb * nb)
         K = np.zeros(nb * nb)
-        ep = None if eri is None else np.ascontiguousarray(eri).ctypes.data
+        Ea = None
+        if eri is not None:
+            # the caller's array in the reference's memory order (Fortran
+            # ERI(n1,nBas,nBas,nBas) = C-ordered E[l,k,j,i]); the bound copy
+            # lives until the call has returned
+            Ea = np.ascontiguousarray(eri, dtype=np.float64)
+            if Ea.size != n1 * nb ** 3:
+                raise EngineError(f"jk_stored: eri has {Ea.size} elements, expected n1*nBas^3 = {n1 * nb ** 3}")
+        ep = None if Ea is None else Ea.ctypes.data
         self._check(lib().tcg_jk_stored_order(self.h, n1, nb, Pf.ravel(), ep, int(physicist), J, K),
                     "tcg_jk_stored")
+        del Ea
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
 
     # -- AO_to_MO_ERI_inplace + MP2 ---------------------------------------------
