@@ -76,6 +76,17 @@ int tcg_jk_stored(tcg_ctx *ctx, int n1, int nbas, const double *P, const double 
 int tcg_jk_stored_order(tcg_ctx *ctx, int n1, int nbas, const double *P, const double *eri, int physicist,
                         double *J, double *K);
 
+/* Coulomb and exchange from the resident tensor (tcg_eri_build, or the last
+ * caller array tcg_jk_stored uploaded), with NO device allocation.  No
+ * counterpart in the reference: an extra form of tcg_jk_stored (eri == NULL)
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, layout, order and
+ * result as tcg_jk_stored with eri == NULL (to rounding: the sums meet in a
+ * different order).  Its device memory is one static scratch array of 32 MiB
+ * per GPU, resolved by tcg_create: nbas^2 + 2 n1 nbas must not exceed 4194304
+ * (else TCG_E_ARG).  Calls that overlap on one GPU share that array and race. */
+int tcg_jk_stored_noalloc(tcg_ctx *ctx, int n1, int nbas, const double *P, double *J, double *K);
+
 /* Direct J/K: the integrals are recomputed at every call and contracted on
  * the fly, nothing of size nBas^4 is stored.  Replaces
  *   direct_init(ndp,nBas,AO)        src/module/eri_direct.f90:76-225

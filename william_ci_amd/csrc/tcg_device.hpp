@@ -66,6 +66,9 @@ struct EriArgs {
 };
 
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
+// address of the allocation-free J/K's scratch array on the current device (once per device)
+hipError_t noalloc_setup();
+size_t noalloc_scratch_doubles();
 hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
                           const int *pair_rep, int max_prim_pairs, double *V, hipStream_t st);
 hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, const VecOut *vo, int nvo, const double2 *dco,

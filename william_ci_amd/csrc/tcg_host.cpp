@@ -1079,6 +1079,24 @@ int ctx_view(const tcg_ctx *c, CtxView *v) {
     return TCG_OK;
 }
 
+// The resident ERI tensor (tcg_eri_build) and the engine's stream, for the
+// allocation-free J/K (tcg_noalloc.hip).
+int ctx_resident_eri(const tcg_ctx *c, const double **eri, size_t *count, int *n1, int *physicist, int *nbas,
+                     hipStream_t *st) {
+    if (!c || !c->d_eri.p || c->eri_count == 0) return TCG_E_STATE;
+    *eri = c->d_eri.p;
+    *count = c->eri_count;
+    *n1 = c->eri_n1;
+    *physicist = c->eri_physicist ? 1 : 0;
+    *nbas = c->nbas;
+    *st = c->st;
+    return TCG_OK;
+}
+
+void ctx_set_error(tcg_ctx *c, const char *msg) {
+    if (c) c->err = msg;
+}
+
 int ctx_shells(tcg_ctx *c, std::vector<int> &first_ao, std::vector<int> &ncomp, std::vector<int> &cell) {
     if (!c || c->nbas == 0) return TCG_E_STATE;
     first_ao.clear();
@@ -1124,6 +1142,12 @@ int tcg_create(int ndp, const double L[3], const int ncell[3], int nquad, tcg_ct
         }
     }
     if (hipStreamCreate(&c->st) != hipSuccess) {
+        delete c;
+        return TCG_E_DEVICE;
+    }
+    // the allocation-free J/K's scratch array: its address on this device
+    if (noalloc_setup() != hipSuccess) {
+        (void)hipStreamDestroy(c->st);
         delete c;
         return TCG_E_DEVICE;
     }

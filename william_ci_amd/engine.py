@@ -45,6 +45,7 @@ def lib():
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
         L.tcg_eri_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_longlong)]
+        L.tcg_jk_stored_noalloc.argtypes = [_vp, _ci, _ci, _dp, _dp, _dp]
         L.tcg_direct_init.argtypes = [_vp]
         L.tcg_direct_jk.argtypes = [_vp, _ci, _dp, _dp, _dp]
         L.tcg_set_quadrature.argtypes = [_vp, _ci, _ci]
@@ -195,6 +196,17 @@ class Engine:
         self._check(lib().tcg_jk_stored_order(self.h, n1, nb, Pf.ravel(), ep, int(physicist), J, K),
                     "tcg_jk_stored")
         del Ea
+        return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
+
+    def jk_stored_noalloc(self, P: np.ndarray, n1: int | None = None):
+        """jk_stored from the resident tensor through tcg_jk_stored_noalloc (no
+        device allocation; J/K rows and P in the library's static scratch)."""
+        nb = self.s.nbas
+        n1 = self.s.nfuc if n1 is None else n1
+        Pf = np.ascontiguousarray(np.asarray(P, dtype=np.float64).T)
+        J = np.zeros(nb * nb)
+        K = np.zeros(nb * nb)
+        self._check(lib().tcg_jk_stored_noalloc(self.h, n1, nb, Pf.ravel(), J, K), "tcg_jk_stored_noalloc")
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
 
     # -- AO_to_MO_ERI_inplace + MP2 ---------------------------------------------
