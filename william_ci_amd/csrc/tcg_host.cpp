@@ -163,6 +163,13 @@ struct tcg_ctx {
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fill = nullptr;
 
+    // TCG_PROFILE=1: host wall-clock marks of the build's sub-steps, printed to stderr
+    bool prof = false;
+    std::vector<std::pair<const char *, double>> marks;
+    void mark(const char *what) {
+        if (prof) marks.emplace_back(what, now_ms());
+    }
+
     int fail(int code, const char *fmt, ...) {
         char buf[512];
         va_list ap;
@@ -559,6 +566,7 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     HIPCHK(c, c->d_rowval.alloc((size_t)nr * A.nqp));
     HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, so, c->d_surv.p, c->st));
     HIPCHK(c, launch_rows_emit(A, so, ro, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
+    c->mark("  surv+emit launched");
     int rbits = 0;
     while ((1 << rbits) < (int)c->rule_nq.size()) ++rbits;
     const int end_bit = A.nm_bits + 2 + (A.qrule ? rbits : 0);
@@ -579,8 +587,8 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     if (A.Jacc)
         HIPCHK(c, launch_contract_jk(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
     else {
-        HIPCHK(c, c->d_cls.alloc(5 * (size_t)A.nq4));
-        HIPCHK(c, c->d_clscnt.alloc(5));
+        HIPCHK(c, c->d_cls.alloc(9 * (size_t)A.nq4));      // one list per s/p form and one of the rest
+        HIPCHK(c, c->d_clscnt.alloc(9));
         HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->d_cls.p, c->d_clscnt.p,
                                   c->st));
     }
@@ -618,6 +626,7 @@ static int run_quartets(tcg_ctx *c, EriArgs A) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->last_surv = ns;
     c->last_rows = nr;
+    c->mark("  counts");
     if (nr == 0) return 0;
     // rows per chunk: row record, sort keys/values, node values, real-direction values
     const int nreal = 3 - A.ndp;
@@ -685,12 +694,14 @@ static int prepare(tcg_ctx *c) {
     std::vector<long long> tsum;
     const double tt0 = now_ms();
     const bool tables_kept = build_tables(c, tab, tablim, tsum, rtnode, rtw) == 1;
+    c->mark("tables");
     c->times[7] = now_ms() - tt0;
     std::vector<VecJob> jobs;
     size_t nvec = 0, niv = 0;
     std::vector<double> herm;
     int rc = build_pairs(c, jobs, nvec, niv, herm);
     if (rc) return rc;
+    c->mark("pairs (host)");
     c->max_ncomb = 1;
     for (auto &s : c->sh) c->max_ncomb = std::max(c->max_ncomb, s.l + 1);
     c->max_ncomb = c->max_ncomb * c->max_ncomb * c->max_ncomb * c->max_ncomb;
@@ -718,6 +729,7 @@ static int prepare(tcg_ctx *c) {
     HIPCHK(c, upload_async(c, c->d_poff, c->poff));
     HIPCHK(c, upload_async(c, c->d_pp, c->pp));
     HIPCHK(c, upload_async(c, c->d_sp, c->sp));
+    c->mark("uploads");
     {
         // Fourier coefficients d_k of every (job, power pair) on the host, vectors on the GPU
         std::vector<VecOut> vo;
@@ -749,6 +761,7 @@ static int prepare(tcg_ctx *c) {
         HIPCHK(c, launch_pair_vectors(c->d_jobs.p, (int)jobs.size(), c->d_vo.p, (int)vo.size(), c->d_dco.p, c->d_iv.p,
                                       c->d_vec.p, c->st));
     }
+    c->mark("fourier + pair vectors");
     HIPCHK(c, upload_async(c, c->d_herm, herm));
     HIPCHK(c, upload_async(c, c->d_relc, c->relc));
     HIPCHK(c, c->d_cnt.alloc(8));
@@ -991,8 +1004,10 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
             }
         }
     // blocks stay on the GPU; Q, Qsh, the largest Qsh and the primitive ps are built there
+    c->mark("schwarz list (host)");
     int rc = run_blocks(c, q4, nout, nullptr, 1);
     if (rc) return rc;
+    c->mark("schwarz blocks");
     const size_t nbb = (size_t)nb * nb, nss = (size_t)nsh * nsh;
     if (nbb > c->d_Q.n || nss > c->d_Qsh.n || !c->d_qmax.p) HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, c->d_Q.alloc(nbb));
@@ -1004,6 +1019,7 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
                                     c->d_relc.p, nf, nb, nc, nsh, c->d_Q.p, c->d_Qsh.p, c->d_pp.p, c->d_qmax.p, c->st));
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
     t2 = now_ms();
+    c->mark("schwarz tables");
 
     // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened, on the GPU;
     // appended into the capacity of an earlier enumeration, counted and redone only when it overflows ----
@@ -1022,6 +1038,7 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
     c->stats[0] = (long long)nq4;
     c->nq4 = (int)nq4;
     t3 = now_ms();
+    c->mark("quartet list");
     return 0;
 
 }
@@ -1187,6 +1204,8 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     if (!c) return TCG_E_ARG;
     if (c->nbas == 0) return c->fail(TCG_E_STATE, "tcg_eri_build before tcg_set_basis");
     const double t0 = now_ms();
+    c->prof = getenv("TCG_PROFILE") != nullptr;
+    c->marks.clear();
     const int nb = c->nbas, nf = c->nfuc;
     const int n1 = sao ? nf : nb;
     const size_t count = (size_t)n1 * nb * nb * nb;
@@ -1203,6 +1222,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fill, 0));
     HIPCHK(c, launch_fill(c->d_eri.p, count, 0.0, c->st2));
     HIPCHK(c, hipEventRecord(c->ev_fill, c->st2));
+    c->mark("fill launched");
     int rc = prepare(c);
     if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 3 * sizeof(unsigned long long), c->st));
@@ -1234,8 +1254,10 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     rc = c->learned_on ? learned_pass(c, A) : run_quartets(c, A);
     c->time_rows = false;
     if (rc) return rc;
+    c->mark("quartets launched");
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    c->mark("quartets done");
     float kms = 0.f, rms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
     c->times[5] = kms;
@@ -1259,6 +1281,17 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     c->times[2] = t3 - t2;
     c->times[3] = t4 - t3;
     c->times[4] = now_ms() - t0;
+    if (c->prof) {
+        c->mark("end");
+        double prev = t0;
+        fprintf(stderr, "tcg_eri_build profile (ms since previous mark):");
+        for (auto &m : c->marks) {
+            fprintf(stderr, " [%s %.3f]", m.first, m.second - prev);
+            prev = m.second;
+        }
+        fprintf(stderr, "\n");
+        c->marks.clear();
+    }
     return TCG_OK;
 }
 

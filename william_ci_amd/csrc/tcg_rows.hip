@@ -57,6 +57,24 @@ __device__ __forceinline__ void quartet_shape(const EriArgs &A, const Quartet &Q
     ncomb = (l[0] + 1) * (l[1] + 1) * (l[2] + 1) * (l[3] + 1);
 }
 
+// the s/p forms: (P, NBP) -> list 0..7 (P p shells, NBP of them in the bra; at
+// most two per pair); list 8 holds the rest
+__host__ __device__ constexpr int sp_list(int P, int NBP) {
+    return P == 0 ? 0 : P == 1 ? 1 + NBP : P == 2 ? 3 + NBP : 5 + NBP;
+}
+constexpr int kSpLists = 9;
+
+// the quartets the s/p form takes: s and p shells, at most three p shells
+// (four: 81 accumulators per lane do not fit the registers), and rules of at
+// most 64 nodes; returns the list, or -1
+__device__ __forceinline__ int sp_form(const EriArgs &A, const Quartet &Q, int qi) {
+    if (!A.sp_contract || rule_nodes(A, quartet_rule(A, qi)) > 64) return -1;
+    const int lA = A.sh[Q.s[0]].l, lB = A.sh[Q.s[1]].l, lC = A.sh[Q.s[2]].l, lD = A.sh[Q.s[3]].l;
+    if (lA > 1 || lB > 1 || lC > 1 || lD > 1) return -1;
+    const int P = lA + lB + lC + lD;
+    return P <= 3 ? sp_list(P, lA + lB) : -1;
+}
+
 // ---------------------------------------------------------------------------
 // survivors: pass 0 counts (scount, rcount), pass 1 writes at soff
 // ---------------------------------------------------------------------------
@@ -451,6 +469,8 @@ __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__
     const int nreal = 3 - A.ndp, nqp = A.nqp, lane = threadIdx.x;
     for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
         const Quartet *Q = A.q4 + qi;
+        // the s/p contraction forms these factors itself
+        if (A.real_fused && sp_form(A, *Q, qi) >= 0) continue;
         const int rule = quartet_rule(A, qi);
         const int nq = rule_nodes(A, rule);
         const int lA = A.sh[Q->s[0]].l, lB = A.sh[Q->s[1]].l, lC = A.sh[Q->s[2]].l, lD = A.sh[Q->s[3]].l;
@@ -522,7 +542,11 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
     // spread the (survivor, direction) steps of a quartet over 16 wavefronts; long lists
     // keep one wavefront per quartet (measured faster there)
     const int gy = a.nq4 < 16384 ? 16 : 1;
-    hipLaunchKernelGGL(k_real_rows, dim3(grid, gy), dim3(64), 0, st, a, soff, roff, surv, realval);
+    // the quartets of the s/p contraction kernels (long lists, stored build) are skipped:
+    // those kernels form the real-direction factors in registers
+    EriArgs b = a;
+    b.real_fused = (a.sp_contract && a.nq4 >= a.sp_min_quartets && !a.Jacc) ? 1 : 0;
+    hipLaunchKernelGGL(k_real_rows, dim3(grid, gy), dim3(64), 0, st, b, soff, roff, surv, realval);
     return hipGetLastError();
 }
 
@@ -634,41 +658,99 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
 // ---------------------------------------------------------------------------
 // contraction of a quartet of s and p shells by a group of 16 lanes over the
 // nodes, four quartets per wavefront.
-// With P p shells (rank j = 0..P-1 in the order A, B, C, D) the power
-// combination of a direction has one bit per p shell (set when the shell's
-// component points along that direction): ncomb = 2^P, and a component
-// quartet is a digit string (dir_0 .. dir_{P-1}), dir in {x, y, z}, with
+// With P p shells (rank j = 0..P-1 in the order A, B, C, D; NBP of them in
+// the bra A, B) the power combination of a direction has one bit per p shell
+// (set when the shell's component points along that direction): ncomb = 2^P,
+// bits 0..NBP-1 the bra's, the rest the ket's; a component quartet is a digit
+// string (dir_0 .. dir_{P-1}), dir in {x, y, z}, with
 //   cx = sum_j [dir_j = x] 2^j, cy = sum_j [dir_j = y] 2^j, cz = sum_j [dir_j = z] 2^j,
 // all known at compile time.  Per surviving primitive quartet the group loads
-// the 3 * 2^P node rows it needs once, in runs of 16 consecutive nodes (one
-// 128-byte line per load), and adds Ix Iy Iz into 3^P accumulators per lane;
-// one butterfly sum per component at the end.  The contraction coefficients go
-// into the rows: a p shell's coefficient for its component along direction d
-// multiplies the rows of direction d whose combination has that shell's bit
-// set, so every product Ix Iy Iz carries each p shell's coefficient exactly
-// once; the s shells' coefficients, the prefactor and the node weight scale the
-// x rows.  A quartet's block is a chain of dependent loads (quartet, shells,
-// offsets, survivors, coefficients, rows, then the scatter's tables): four
-// quartets per wavefront keep four such chains in flight.  The sums run in a
-// fixed order: the block is the same on every run.
+// the 2^P periodic-direction node rows it needs once, in runs of 16
+// consecutive nodes (one 128-byte line per load); the real-direction factors
+// (ERI_shell_block_toroidal.f90:302-335) are formed in registers at each node:
+//   g_0 = pi sqrt(D) e^{-w R^2}, g_{N+1} = 2 w R g_N - 2 N w g_{N-1},
+//   D = 1/(pq + (p+q) t^2), w = pq t^2 D,
+//   I(bra bits, ket bits) = sum_n (-1)^n E_n sum_m E'_m g_{n+m}
+// with the pairs' Hermite coefficients E, E' (in the order the separate
+// real-row kernel sums them).  Ix Iy Iz then go into 3^P accumulators per
+// lane; one butterfly sum per component at the end.  The contraction
+// coefficients go into the factors: a p shell's coefficient for its component
+// along direction d multiplies the direction-d factors whose combination has
+// that shell's bit set, so every product Ix Iy Iz carries each p shell's
+// coefficient exactly once; the s shells' coefficients, the prefactor and the
+// node weight scale the x factors.  A quartet's block is a chain of dependent
+// loads (quartet, shells, offsets, survivors, coefficients, rows, then the
+// scatter's tables): four quartets per wavefront keep four such chains in
+// flight.  The sums run in a fixed order: the block is the same on every run.
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int pow3(int p) { return p == 0 ? 1 : 3 * pow3(p - 1); }
+__host__ __device__ constexpr int popc(int v) { return v == 0 ? 0 : (v & 1) + popc(v >> 1); }
+
+// Hermite coefficients of one pair and real direction for its p-bits (rank
+// order): stored at [(ia (lB+1) + ib) kE + n]; with two p shells ia is bit 0
+// and ib bit 1, with one the index is the bit itself.
+template <int NB>
+__device__ __forceinline__ constexpr int herm_index(int bits) {
+    return NB == 2 ? (((bits & 1) << 1) | (bits >> 1)) : bits;
+}
+
+// the factors of one real direction at one node for every combination c < 2^P
+// (bra bits c & (2^NBP - 1), ket bits c >> NBP)
+template <int P, int NBP>
+__device__ __forceinline__ void real_factors(double p, double q, double t2, double Rr, const double *__restrict__ Eb,
+                                             const double *__restrict__ Ek, double out[1 << P]) {
+    constexpr int NKP = P - NBP, kE = 2 * kMaxL + 1;
+    const double D2 = 1.0 / (p * q + (p + q) * t2);
+    const double w = p * q * t2 * D2;
+    const double rs = 3.14159265358979323846 * sqrt(D2);
+    double g[P + 1];
+    g[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
+    {
+        double gm1 = 0.0, gc = g[0];
+#pragma unroll
+        for (int n = 0; n < P; ++n) {
+            const double gp1 = 2.0 * w * Rr * gc - 2.0 * (double)n * w * gm1;
+            g[n + 1] = gp1;
+            gm1 = gc;
+            gc = gp1;
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < (1 << NKP); ++kb) {
+        const double *ek = Ek + herm_index<NKP>(kb) * kE;
+        double h[NBP + 1];
+#pragma unroll
+        for (int n = 0; n <= NBP; ++n) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m <= popc(kb); ++m) s += ek[m] * g[n + m];
+            h[n] = s;
+        }
+#pragma unroll
+        for (int bb = 0; bb < (1 << NBP); ++bb) {
+            const double *eb = Eb + herm_index<NBP>(bb) * kE;
+            double v = 0.0;
+#pragma unroll
+            for (int n = 0; n <= popc(bb); ++n) v += ((n & 1) ? -eb[n] : eb[n]) * h[n];
+            out[bb | (kb << NBP)] = v;
+        }
+    }
+}
 
 // the block of one s/p quartet by the GL lanes of a group (gl = 0..GL-1): the
-// rule's nodes (at most 64) in 64 / GL runs of GL
-template <int P, int GL>
+// rule's nodes (at most 64) in 64 / GL runs of GL; NDP periodic directions
+template <int NDP, int P, int NBP, int GL>
 __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const Quartet &Q, long s0, long ns,
                                                  long r0, const Surv *__restrict__ surv,
-                                                 const double *__restrict__ rowval,
-                                                 const double *__restrict__ realval, double *blk, int gl) {
-    constexpr int NC = 1 << P, NT = pow3(P);
+                                                 const double *__restrict__ rowval, double *blk, int gl) {
+    constexpr int NC = 1 << P, NT = pow3(P), kE = 2 * kMaxL + 1, NREAL = 3 - NDP;
     const int rule = quartet_rule(A, qi);
-    const int nqp = A.nqp, nq = rule_nodes(A, rule), ndp = A.ndp, nreal = 3 - ndp;
+    const int nqp = A.nqp, nq = rule_nodes(A, rule);
     const double *tw = A.tweight + (size_t)rule * nqp;
+    const double *tn = A.tnode + (size_t)rule * nqp;
     const Shell sA = A.sh[Q.s[0]], sB = A.sh[Q.s[1]], sC = A.sh[Q.s[2]], sD = A.sh[Q.s[3]];
     const int pb0 = A.sp[Q.bra].first, pk0 = A.sp[Q.ket].first;
     const int npB = sB.nprim, npD = sD.nprim;
-    const long R0 = r0 / ndp;
     // the p shells in rank order: position (0..3 = A..D), direction -> component map,
     // coefficient offset and primitive count
     int ppos[4] = {0, 0, 0, 0}, pof[4] = {0, 0, 0, 0}, pnp[4] = {1, 1, 1, 1};
@@ -686,30 +768,34 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
                 ++j;
             }
     }
-    // the node runs: node (lanes past the rule's last node: the last node, weight 0)
-    // and weight
+    // the node runs: node (lanes past the rule's last node: the last node, weight 0),
+    // weight and t^2
     constexpr int NR = 64 / GL;
     int kr[NR];
-    double w[NR];
+    double w[NR], t2[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
         const int k = GL * h + gl;
         kr[h] = k < nq ? k : nq - 1;
         w[h] = k < nq ? tw[kr[h]] : 0.0;
+        t2[h] = tn[kr[h]] * tn[kr[h]];
     }
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = 0.0;
     // survivors in batches of GL: lane gl reads survivor gl of the batch and forms its
-    // coefficients (s shells and prefactor; each p shell's along x, y, z); the steps
-    // take them from that lane of the group (shuffle), U survivors and UH node runs
-    // per step with their row loads in flight together (a step past the batch's end
+    // coefficients (s shells and prefactor; each p shell's along x, y, z), exponent
+    // sums, real-direction distances and Hermite offsets; the steps take them from
+    // that lane of the group (shuffle), U survivors and UH node runs per step with
+    // their periodic row loads in flight together (a step past the batch's end
     // repeats its last survivor with coefficient 0)
     constexpr int U = P == 0 ? 2 : 1, UH = P <= 1 ? 4 : P == 2 ? 2 : 1;
     const int gbase = threadIdx.x & (64 - GL);
+    constexpr int eb_stride = (1 << NBP) * kE, ek_stride = (1 << (P - NBP)) * kE;   // one real direction
     for (long gb = 0; gb < ns; gb += GL) {
         const int nb = ns - gb < GL ? (int)(ns - gb) : GL;
-        double cs_l, cpd_l[3][P > 0 ? P : 1];
+        double cs_l, cpd_l[3][P > 0 ? P : 1], bq_l = 0.0, kq_l = 0.0, R_l[NREAL > 0 ? NREAL : 1];
+        int ebo_l = 0, eko_l = 0;
         {
             const Surv S = surv[s0 + gb + (gl < nb ? gl : nb - 1)];
             const int ib = S.pb - pb0, ik = S.pk - pk0;
@@ -727,57 +813,83 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
 #pragma unroll
                 for (int d = 0; d < 3; ++d) cpd_l[d][j] = cj[(int)((dm[j] >> (2 * d)) & 3) * pnp[j]];
             }
+            if (NREAL > 0) {
+                const PrimPair &ppb = A.pp[S.pb];
+                const PrimPair &ppk = A.pp[S.pk];
+                bq_l = ppb.q;
+                kq_l = ppk.q;
+                ebo_l = ppb.eoff;
+                eko_l = ppk.eoff;
+#pragma unroll
+                for (int drr = 0; drr < NREAL; ++drr)
+                    R_l[drr] = ppb.cen[NDP + drr] - (ppk.cen[NDP + drr] + Q.shift[NDP + drr]);
+            }
         }
         for (int i0 = 0; i0 < nb; i0 += U) {
-            // the U survivors' coefficients and row addresses
-            double cs[U], cpd[U][3][P > 0 ? P : 1];
-            const double *rp[U][3];
+            // the U survivors' coefficients, periodic row addresses, exponent sums,
+            // distances and Hermite coefficients
+            double cs[U], cpd[U][3][P > 0 ? P : 1], pq[U], qq[U], Rr[U][NREAL > 0 ? NREAL : 1];
+            const double *rp[U][NDP];
+            const double *Eb[U], *Ek[U];
 #pragma unroll
             for (int v = 0; v < U; ++v) {
                 const int i = i0 + v;                          // < GL: U divides GL
-                cs[v] = __shfl(cs_l, gbase + i, 64);
+                const int src = gbase + i;
+                cs[v] = __shfl(cs_l, src, 64);
 #pragma unroll
                 for (int j = 0; j < P; ++j)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) cpd[v][d][j] = __shfl(cpd_l[d][j], gbase + i, 64);
+                    for (int d = 0; d < 3; ++d) cpd[v][d][j] = __shfl(cpd_l[d][j], src, 64);
+                if (NREAL > 0) {
+                    pq[v] = __shfl(bq_l, src, 64);
+                    qq[v] = __shfl(kq_l, src, 64);
+#pragma unroll
+                    for (int drr = 0; drr < NREAL; ++drr) Rr[v][drr] = __shfl(R_l[drr], src, 64);
+                    Eb[v] = A.herm + __shfl(ebo_l, src, 64);
+                    Ek[v] = A.herm + __shfl(eko_l, src, 64);
+                }
                 const long g = gb + (i < nb ? i : nb - 1);
 #pragma unroll
-                for (int dd = 0; dd < 3; ++dd)
-                    rp[v][dd] = dd < ndp ? rowval + (size_t)(r0 + (g * ndp + dd) * NC) * nqp
-                                         : realval + (size_t)(R0 * nreal + (g * nreal + (dd - ndp)) * NC) * nqp;
+                for (int dd = 0; dd < NDP; ++dd) rp[v][dd] = rowval + (size_t)(r0 + (g * NDP + dd) * NC) * nqp;
             }
-            // UH node runs at a time: every load first (runs past the rule's last node
-            // load its last node, weight 0), then the products
+            // UH node runs at a time: every periodic row load first (runs past the rule's
+            // last node load its last node, weight 0), then per run the real factors and
+            // the products
             for (int h0 = 0; h0 < NR; h0 += UH) {
-                double X[U][UH][NC], Y[U][UH][NC], Z[U][UH][NC];
+                double Xp[U][UH][NDP][NC];
 #pragma unroll
                 for (int v = 0; v < U; ++v)
 #pragma unroll
                     for (int hh = 0; hh < UH; ++hh)
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) {
-                            const int k = kr[h0 + hh];
-                            X[v][hh][c] = rp[v][0][(size_t)c * nqp + k];
-                            Y[v][hh][c] = rp[v][1][(size_t)c * nqp + k];
-                            Z[v][hh][c] = rp[v][2][(size_t)c * nqp + k];
-                        }
+                        for (int dd = 0; dd < NDP; ++dd)
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) Xp[v][hh][dd][c] = rp[v][dd][(size_t)c * nqp + kr[h0 + hh]];
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int v = 0; v < U; ++v)
 #pragma unroll
                     for (int hh = 0; hh < UH; ++hh) {
+                        double F[3][NC];
+#pragma unroll
+                        for (int dd = 0; dd < NDP; ++dd)
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) F[dd][c] = Xp[v][hh][dd][c];
+#pragma unroll
+                        for (int drr = 0; drr < NREAL; ++drr)
+                            real_factors<P, NBP>(pq[v], qq[v], t2[h0 + hh], Rr[v][drr], Eb[v] + drr * eb_stride,
+                                                 Ek[v] + drr * ek_stride, F[NDP + drr]);
                         const double cw = cs[v] * w[h0 + hh];
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) X[v][hh][c] *= cw;
-                        // p shell j: its direction-d coefficient into the direction-d rows with bit j set
+                        for (int c = 0; c < NC; ++c) F[0][c] *= cw;
+                        // p shell j: its direction-d coefficient into the direction-d factors with bit j set
 #pragma unroll
                         for (int j = 0; j < P; ++j)
 #pragma unroll
                             for (int c = 0; c < NC; ++c)
                                 if (c & (1 << j)) {
-                                    X[v][hh][c] *= cpd[v][0][j];
-                                    Y[v][hh][c] *= cpd[v][1][j];
-                                    Z[v][hh][c] *= cpd[v][2][j];
+#pragma unroll
+                                    for (int d = 0; d < 3; ++d) F[d][c] *= cpd[v][d][j];
                                 }
 #pragma unroll
                         for (int t = 0; t < NT; ++t) {
@@ -790,7 +902,7 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
                                 cy |= (dir == 1) << j;
                                 cz |= (dir == 2) << j;
                             }
-                            acc[t] = fma(X[v][hh][cx] * Y[v][hh][cy], Z[v][hh][cz], acc[t]);
+                            acc[t] = fma(F[0][cx] * F[1][cy], F[2][cz], acc[t]);
                         }
                     }
             }
@@ -813,20 +925,10 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
         if (gl == 0) blk[cp] = v;
     }
     if (A.counters && gl == 0) {
+        const int nreal = NREAL;
         atomicAdd(&A.counters[2], (unsigned long long)ns * ((unsigned long long)NC * nreal * 2 * nq *
                                                                 (sA.l + sB.l + sC.l + sD.l + 1) + 3ull * nq * NT));
     }
-}
-
-// the quartets the s/p form takes: s and p shells, at most three p shells
-// (four: 81 accumulators and 48 row values per lane do not fit the registers),
-// and rules of at most 64 nodes
-__device__ __forceinline__ int sp_form(const EriArgs &A, const Quartet &Q, int qi) {
-    if (!A.sp_contract || rule_nodes(A, quartet_rule(A, qi)) > 64) return -1;
-    const int lA = A.sh[Q.s[0]].l, lB = A.sh[Q.s[1]].l, lC = A.sh[Q.s[2]].l, lD = A.sh[Q.s[3]].l;
-    if (lA > 1 || lB > 1 || lC > 1 || lD > 1) return -1;
-    const int P = lA + lB + lC + lD;
-    return P <= 3 ? P : -1;
 }
 
 // the stored position (ERI_toroidal.f90:433-449 index order, first index in
@@ -898,26 +1000,27 @@ __global__ __launch_bounds__(256) void k_contract(EriArgs A, const long long *__
     }
 }
 
-// the quartets into one list per s/p form P = 0..3 and one (P = 4) of the rest
-// (lists[P * nq4 + i], cnt[P] entries): per workgroup of 1024 the wavefronts'
-// counts meet in LDS, one atomic per workgroup and list
+// the quartets into one list per s/p form (sp_list, 0..7) and one (8) of the
+// rest (lists[L * nq4 + i], cnt[L] entries): per workgroup of 1024 the
+// wavefronts' counts meet in LDS, one atomic per workgroup and list
 __global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict__ lists,
                                                       unsigned *__restrict__ cnt) {
-    __shared__ unsigned wcnt[16][5], wbase[16][5];
+    __shared__ unsigned wcnt[16][kSpLists], wbase[16][kSpLists];
     const int qi = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int P = -1;
+    int L = -1;
     if (qi < A.nq4) {
-        P = sp_form(A, A.q4[qi], qi);
-        if (P < 0) P = 4;
+        L = sp_form(A, A.q4[qi], qi);
+        if (L < 0) L = kSpLists - 1;
     }
-    unsigned long long m[5];
+    unsigned long long mm = 0;
 #pragma unroll
-    for (int p = 0; p < 5; ++p) {
-        m[p] = __ballot(P == p);
-        if (lane == 0) wcnt[wv][p] = (unsigned)__popcll(m[p]);
+    for (int p = 0; p < kSpLists; ++p) {
+        const unsigned long long m = __ballot(L == p);
+        if (L == p) mm = m;
+        if (lane == 0) wcnt[wv][p] = (unsigned)__popcll(m);
     }
     __syncthreads();
-    if (threadIdx.x < 5) {
+    if (threadIdx.x < kSpLists) {
         const int p = threadIdx.x;
         unsigned tot = 0;
         for (int w = 0; w < 16; ++w) {
@@ -928,20 +1031,18 @@ __global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict
         for (int w = 0; w < 16; ++w) wbase[w][p] += b0;
     }
     __syncthreads();
-    const unsigned long long mm = P == 0 ? m[0] : P == 1 ? m[1] : P == 2 ? m[2] : P == 3 ? m[3] : m[4];
-    if (P >= 0) lists[(size_t)P * A.nq4 + wbase[wv][P] + __popcll(mm & ((1ull << lane) - 1))] = qi;
+    if (L >= 0) lists[(size_t)L * A.nq4 + wbase[wv][L] + __popcll(mm & ((1ull << lane) - 1))] = qi;
 }
 
-// the s/p quartets with P p shells (list of cnt entries), GL lanes per quartet,
-// 256 / GL quartets per workgroup; one launch per P keeps each kernel's registers
-// to what its P needs
-template <int P, int GL>
+// the s/p quartets of one form (P p shells, NBP in the bra; list of cnt
+// entries), GL lanes per quartet, 256 / GL quartets per workgroup; one launch
+// per form keeps each kernel's registers to what its form needs
+template <int NDP, int P, int NBP, int GL>
 __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__restrict__ list,
                                                      const unsigned *__restrict__ cnt,
                                                      const long long *__restrict__ soff,
                                                      const long long *__restrict__ roff,
-                                                     const Surv *__restrict__ surv, const double *__restrict__ rowval,
-                                                     const double *__restrict__ realval) {
+                                                     const Surv *__restrict__ surv, const double *__restrict__ rowval) {
     constexpr int NG = 256 / GL;
     __shared__ double lds[NG * pow3(P)];
     const int grp = threadIdx.x / GL, gl = threadIdx.x & (GL - 1);
@@ -953,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__res
         const Quartet Q = A.q4[qi];
         if (e < n) {
             const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
-            quartet_block_sp<P, GL>(A, qi, Q, s0, ns, r0, surv, rowval, realval, blk, gl);
+            quartet_block_sp<NDP, P, NBP, GL>(A, qi, Q, s0, ns, r0, surv, rowval, blk, gl);
         }
         wave_sync();
         if (e < n) write_block(A, Q, A.sh[Q.s[0]], A.sh[Q.s[1]], A.sh[Q.s[2]], A.sh[Q.s[3]], blk, gl, GL);
@@ -961,13 +1062,26 @@ __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__res
     }
 }
 
-template <int P, int GL>
-static void launch_sp(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
-                      const long long *roff, const Surv *surv, const double *rowval, const double *realval,
-                      hipStream_t st) {
+template <int NDP, int P, int NBP, int GL>
+static void launch_sp1(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
+                       const long long *roff, const Surv *surv, const double *rowval, hipStream_t st) {
+    constexpr int L = sp_list(P, NBP);
     const int ng = (b.nq4 + 256 / GL - 1) / (256 / GL), grid = ng < (1 << 20) ? ng : (1 << 20);
-    hipLaunchKernelGGL((k_contract_sp<P, GL>), dim3(grid), dim3(256), 0, st, b, lists + (size_t)P * b.nq4, cnt + P,
-                       soff, roff, surv, rowval, realval);
+    hipLaunchKernelGGL((k_contract_sp<NDP, P, NBP, GL>), dim3(grid), dim3(256), 0, st, b, lists + (size_t)L * b.nq4,
+                       cnt + L, soff, roff, surv, rowval);
+}
+
+template <int NDP>
+static void launch_sp_all(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
+                          const long long *roff, const Surv *surv, const double *rowval, hipStream_t st) {
+    launch_sp1<NDP, 0, 0, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 1, 0, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 1, 1, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 2, 0, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 2, 1, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 2, 2, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 3, 1, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 3, 2, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
 }
 
 size_t contract_lds_bytes(const EriArgs &a) { return 4 * sizeof(double) * ((size_t)a.max_blk + 64); }
@@ -1157,17 +1271,16 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
     EriArgs b = a;
     if (b.nq4 < b.sp_min_quartets) b.sp_contract = 0;
     if (b.sp_contract) {
-        // s/p quartets: lists by P, then one kernel per P (grids for the largest list)
-        hipError_t e = hipMemsetAsync(cnt, 0, 5 * sizeof(unsigned), st);
+        // s/p quartets: lists by form, then one kernel per form (grids for the largest list)
+        hipError_t e = hipMemsetAsync(cnt, 0, kSpLists * sizeof(unsigned), st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_sp_classify, dim3((a.nq4 + 1023) / 1024), dim3(1024), 0, st, b, lists, cnt);
-        launch_sp<0, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
-        launch_sp<1, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
-        launch_sp<2, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
-        launch_sp<3, 16>(b, lists, cnt, soff, roff, surv, rowval, realval, st);
-        // the rest (d shells, four p shells) from list 4
+        if (b.ndp == 1) launch_sp_all<1>(b, lists, cnt, soff, roff, surv, rowval, st);
+        else if (b.ndp == 2) launch_sp_all<2>(b, lists, cnt, soff, roff, surv, rowval, st);
+        else launch_sp_all<3>(b, lists, cnt, soff, roff, surv, rowval, st);
+        // the rest (d shells, four p shells) from the last list
         hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval,
-                           lists + 4 * (size_t)b.nq4, cnt + 4);
+                           lists + (size_t)(kSpLists - 1) * b.nq4, cnt + (kSpLists - 1));
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval, nullptr,
