@@ -51,7 +51,6 @@ struct EriArgs {
     int max_blk;                      // largest nA nB nC nD over the quartets
     int sp_contract;                  // s/p quartets (at most three p shells) by the s/p contraction kernels
     int sp_min_quartets;              // ... in quartet lists of at least this many quartets
-    int real_fused;                   // set by launch_real_rows: the s/p kernels form their real-direction factors
     // block mode
     double *blk_out;
     // scatter mode
@@ -92,8 +91,9 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                               const double *rowval, const double *realval, hipStream_t st);
 // lists: 9 * a.nq4 ints, cnt: 9 counters (work space of the s/p contraction)
+// (also forms the real-direction rows, in realval, of the quartets that need them)
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                           const double *rowval, const double *realval, int *lists, unsigned *cnt, hipStream_t st);
+                           const double *rowval, double *realval, int *lists, unsigned *cnt, hipStream_t st);
 hipError_t launch_chunk_bounds(const long long *roff, int nq4, long long cap, int nchunk, int *bounds, hipStream_t st);
 hipError_t launch_rebase(const long long *src, int n, long long *dst, hipStream_t st);
 hipError_t launch_translate_rows(double *eri, int nb, int nf, int ncell, const int *relc, hipStream_t st);

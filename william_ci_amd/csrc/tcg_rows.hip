@@ -465,12 +465,13 @@ __device__ __forceinline__ void real_combos(int lA, int lB, int lC, int lD, cons
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__restrict__ soff,
                                                   const long long *__restrict__ roff,
-                                                  const Surv *__restrict__ surv, double *__restrict__ realval) {
+                                                  const Surv *__restrict__ surv, double *__restrict__ realval,
+                                                  const int *__restrict__ list, const unsigned *__restrict__ cnt) {
     const int nreal = 3 - A.ndp, nqp = A.nqp, lane = threadIdx.x;
-    for (int qi = blockIdx.x; qi < A.nq4; qi += gridDim.x) {
+    const int nlist = list ? (int)*cnt : A.nq4;
+    for (int e = blockIdx.x; e < nlist; e += gridDim.x) {
+        const int qi = list ? list[e] : e;
         const Quartet *Q = A.q4 + qi;
-        // the s/p contraction forms these factors itself
-        if (A.real_fused && sp_form(A, *Q, qi) >= 0) continue;
         const int rule = quartet_rule(A, qi);
         const int nq = rule_nodes(A, rule);
         const int lA = A.sh[Q->s[0]].l, lB = A.sh[Q->s[1]].l, lC = A.sh[Q->s[2]].l, lD = A.sh[Q->s[3]].l;
@@ -534,20 +535,25 @@ __global__ __launch_bounds__(64) void k_real_rows(EriArgs A, const long long *__
     }
 }
 
-hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                            long long ns, double *realval, hipStream_t st) {
-    if (a.ndp >= 3 || ns <= 0 || a.nq4 <= 0) return hipSuccess;
-    const int grid = a.nq4 < (1 << 20) ? a.nq4 : (1 << 20);
+// every quartet of a (list == nullptr), or the *cnt quartets of list
+static hipError_t real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                            double *realval, const int *list, const unsigned *cnt, hipStream_t st) {
+    if (a.ndp >= 3 || a.nq4 <= 0) return hipSuccess;
+    // a list's length is on the device: a grid of at most 4096 wavefronts walks it
+    const int cap = list ? 4096 : (1 << 20);
+    const int grid = a.nq4 < cap ? a.nq4 : cap;
     // short quartet lists (the Schwarz passes: few quartets, up to nprim^4 survivors each)
     // spread the (survivor, direction) steps of a quartet over 16 wavefronts; long lists
     // keep one wavefront per quartet (measured faster there)
     const int gy = a.nq4 < 16384 ? 16 : 1;
-    // the quartets of the s/p contraction kernels (long lists, stored build) are skipped:
-    // those kernels form the real-direction factors in registers
-    EriArgs b = a;
-    b.real_fused = (a.sp_contract && a.nq4 >= a.sp_min_quartets && !a.Jacc) ? 1 : 0;
-    hipLaunchKernelGGL(k_real_rows, dim3(grid, gy), dim3(64), 0, st, b, soff, roff, surv, realval);
+    hipLaunchKernelGGL(k_real_rows, dim3(grid, gy), dim3(64), 0, st, a, soff, roff, surv, realval, list, cnt);
     return hipGetLastError();
+}
+
+hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
+                            long long ns, double *realval, hipStream_t st) {
+    if (ns <= 0) return hipSuccess;
+    return real_rows(a, soff, roff, surv, realval, nullptr, nullptr, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1257,7 +1263,7 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
 }
 
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                           const double *rowval, const double *realval, int *lists, unsigned *cnt, hipStream_t st) {
+                           const double *rowval, double *realval, int *lists, unsigned *cnt, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
     const int nwg = (a.nq4 + 3) / 4;
     const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
@@ -1275,13 +1281,22 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
         hipError_t e = hipMemsetAsync(cnt, 0, kSpLists * sizeof(unsigned), st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_sp_classify, dim3((a.nq4 + 1023) / 1024), dim3(1024), 0, st, b, lists, cnt);
+        // real-direction rows only for the quartets of the general kernel (the last list)
+        const int *rest = lists + (size_t)(kSpLists - 1) * b.nq4;
+        e = real_rows(b, soff, roff, surv, realval, rest, cnt + (kSpLists - 1), st);
+        if (e != hipSuccess) return e;
         if (b.ndp == 1) launch_sp_all<1>(b, lists, cnt, soff, roff, surv, rowval, st);
         else if (b.ndp == 2) launch_sp_all<2>(b, lists, cnt, soff, roff, surv, rowval, st);
         else launch_sp_all<3>(b, lists, cnt, soff, roff, surv, rowval, st);
-        // the rest (d shells, four p shells) from the last list
-        hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval,
-                           lists + (size_t)(kSpLists - 1) * b.nq4, cnt + (kSpLists - 1));
+        // the rest (d shells, four p shells) from the last list (its length is on the
+        // device: at most 4096 workgroups walk it)
+        hipLaunchKernelGGL(k_contract, dim3(grid < 4096 ? grid : 4096), dim3(256), lds, st, b, soff, roff, surv,
+                           rowval, realval, rest, cnt + (kSpLists - 1));
         return hipGetLastError();
+    }
+    {
+        const hipError_t e = real_rows(b, soff, roff, surv, realval, nullptr, nullptr, st);
+        if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval, nullptr,
                        nullptr);
