@@ -71,6 +71,9 @@ hipError_t noalloc_setup();
 size_t noalloc_scratch_doubles();
 hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nuc_z, int natom, const int *ao_shell,
                           const int *pair_rep, int max_prim_pairs, double *V, hipStream_t st);
+hipError_t launch_pair_setup(const PairBlock *blk, int nblk, const Shell *sh, const double *expo, const double *coef,
+                             const int *ao_poff, int ndp, int ncap, const double L[3], const double a[3], PrimPair *pp,
+                             VecJob *jobs, VecOut *vo, double2 *dco, double *herm, hipStream_t st);
 hipError_t launch_pair_vectors(const VecJob *jobs, int njobs, const VecOut *vo, int nvo, const double2 *dco,
                                double *ivbuf, double2 *vec, hipStream_t st);
 hipError_t launch_schwarz_tables(const Quartet *q4, int nquart, int nshell, const Shell *sh, const ShellPair *sp,

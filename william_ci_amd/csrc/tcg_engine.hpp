@@ -74,6 +74,23 @@ struct RowRec {
     int d;               // periodic direction
 };
 
+// One representative shell pair (S1 in the unit cell, S2 anywhere) of the
+// GPU pair set-up and where its records go: nij = nprim(S1) nprim(S2)
+// primitive pairs (i major), each with ndp Fourier-vector jobs of
+// (l1+1)(l2+1) vectors.  Vector and Bessel storage is laid out for the
+// largest length (ncap + 1 orders per vector, ncap + l1 + l2 + 1 values per
+// job), so every offset is known before the lengths are.
+struct PairBlock {
+    int S1, S2;
+    int first;           // first PrimPair
+    int job0;            // first VecJob: + ij * ndp + d
+    int vo0;             // first VecOut: + (ij * ndp + d) * (l1+1)(l2+1) + combo
+    int vec0;            // first vector element: + (ij * ndp + d) * (l1+1)(l2+1)(ncap+1)
+    long long iv0;       // first Bessel value: + (ij * ndp + d) * (ncap + l1 + l2 + 1)
+    int herm0;           // first Hermite coefficient: + ij * nreal (l1+1)(l2+1)(2 kMaxL + 1)
+    int dco0;            // first Fourier coefficient: + VecOut index in the block * (2(l1+l2)+1)
+};
+
 // One Fourier vector to assemble: its job (Bessel values), coefficient
 // offset (2K+1 complex d_k) and output offset.
 struct VecOut {

@@ -87,8 +87,7 @@ struct tcg_ctx {
     // tables
     int ncap = 0;
     std::vector<double> tnode, tweight;
-    // pair data
-    std::vector<PrimPair> pp;
+    // pair data (the primitive-pair records themselves are built on the GPU)
     std::vector<ShellPair> sp;
     // device
     hipStream_t st = nullptr;
@@ -143,6 +142,7 @@ struct tcg_ctx {
     DevBuf<int> d_qrule;
     DevBuf<double> d_expo;
     DevBuf<VecJob> d_jobs;       // pair-vector jobs of the last preparation
+    DevBuf<PairBlock> d_pblk;    // shell-pair blocks of the GPU pair set-up
     DevBuf<VecOut> d_vo;
     DevBuf<double2> d_dco;
     DevBuf<double> d_iv;
@@ -395,113 +395,52 @@ static inline int rep_pair(const tcg_ctx *c, int S1, int S2) {
 }
 
 // ---------------------------------------------------------------------------
-// pair Fourier vector V_m = sum_k d_k exp(-A) I_{|m-k|}(A), m = 0..N
-// (src/module/clifford_tools.f90:467-549, sign +1)
+// Representative shell pairs (S1 in the unit cell, S2 anywhere) and where the
+// GPU pair set-up (tcg_setup.hip::k_pair_setup) writes their primitive-pair
+// records, Fourier-vector jobs, vectors, Bessel values, coefficients and
+// Hermite coefficients.  Integer work only: every offset follows from the
+// shells' angular momenta and primitive counts, the vectors and Bessel values
+// being laid out for the longest length (ncap).
 // ---------------------------------------------------------------------------
-static void pair_vector(int N, double A, int i, int l, double phA, double phB, double2 *V, std::vector<double> &iv) {
-    iv.resize(N + i + l + 1);
-    bessel_miller(N + i + l, A, N + i + l, iv.data());
-    if (i + l == 0) {
-        for (int m = 0; m <= N; ++m) V[m] = double2{iv[m], 0.0};
-        return;
-    }
-    std::complex<double> d[4 * kMaxL + 1];
-    sin_product_fourier(i, l, phA, phB, d);
-    const int K = i + l;
-    for (int m = 0; m <= N; ++m) {
-        std::complex<double> acc = 0.0;
-        for (int k = -K; k <= K; ++k) acc += d[k + K] * iv[std::abs(m - k)];
-        V[m] = double2{acc.real(), acc.imag()};
-    }
-}
+struct PairLayout {
+    size_t npp = 0, njobs = 0, nvo = 0, nvec = 0, niv = 0, nherm = 0, ndco = 0;
+};
 
-// Primitive-pair records on the host (barycentres, prefactors, Bessel lengths,
-// Hermite coefficients); the Fourier vectors are generated on the GPU from
-// the job list (tcg_setup.hip::k_pair_vectors).
-static int build_pairs(tcg_ctx *c, std::vector<VecJob> &jobs, size_t &nvec, size_t &niv, std::vector<double> &herm) {
-    c->pp.clear();
+static int build_pair_blocks(tcg_ctx *c, std::vector<PairBlock> &blk, PairLayout &L) {
     c->sp.assign((size_t)c->nshuc * c->nsh, ShellPair{0, 0});
-    jobs.clear();
-    herm.clear();
-    {
-        size_t npair = 0, nherm = 0;
-        for (int S1 = 0; S1 < c->nshuc; ++S1)
-            for (int S2 = 0; S2 < c->nsh; ++S2) {
-                const size_t n = (size_t)c->sh[S1].nprim * c->sh[S2].nprim;
-                npair += n;
-                nherm += n * (3 - c->ndp) * (c->sh[S1].l + 1) * (c->sh[S2].l + 1) * (2 * kMaxL + 1);
-            }
-        c->pp.reserve(npair);
-        jobs.reserve(npair * c->ndp);
-        herm.reserve(nherm);
-    }
-    nvec = 0;
-    niv = 0;
-    const int nreal = 3 - c->ndp;
+    blk.clear();
+    blk.reserve((size_t)c->nshuc * c->nsh);
+    L = PairLayout{};
+    const int ndp = c->ndp, nreal = 3 - ndp, kE = 2 * kMaxL + 1;
     for (int S1 = 0; S1 < c->nshuc; ++S1)
         for (int S2 = 0; S2 < c->nsh; ++S2) {
             const Shell &A = c->sh[S1], &B = c->sh[S2];
+            const size_t nij = (size_t)A.nprim * B.nprim, ncomb = (size_t)(A.l + 1) * (B.l + 1);
             ShellPair &spr = c->sp[(size_t)S1 * c->nsh + S2];
-            spr.first = (int)c->pp.size();
-            spr.count = A.nprim * B.nprim;
-            for (int i = 0; i < A.nprim; ++i)
-                for (int j = 0; j < B.nprim; ++j) {
-                    PrimPair P{};
-                    P.pi = i;
-                    P.pj = j;
-                    const double e1 = c->expo[A.poff + i], e2 = c->expo[B.poff + j];
-                    // esum = 0: two constant functions (the exponent-0 s function of the
-                    // density-fitting integrals, df_toroidal.f90:837-844)
-                    const double esum = e1 + e2, mu = esum > 0.0 ? e1 * e2 / esum : 0.0;
-                    P.q = esum;
-                    double fac = 1.0;
-                    for (int d = 0; d < 3; ++d) {
-                        if (d < c->ndp) {
-                            double pe, cen;
-                            clifford_pair(e1, e2, A.xyz[d], B.xyz[d], c->L[d], c->a[d], &pe, &cen);
-                            P.cen[d] = cen;
-                            fac *= std::exp(-2.0 * (esum - pe) / (c->a[d] * c->a[d]));
-                            const double Aarg = 2.0 * pe / (c->a[d] * c->a[d]);
-                            const int N = std::min(nmax_est(Aarg), c->ncap);
-                            P.N[d] = N;
-                            P.voff[d] = (int)nvec;
-                            VecJob J;
-                            J.A = Aarg;
-                            J.phA = c->a[d] * (cen - A.xyz[d]);
-                            J.phB = c->a[d] * (cen - B.xyz[d]);
-                            J.N = N;
-                            J.lA = A.l;
-                            J.lB = B.l;
-                            J.voff = (int)nvec;
-                            J.ivoff = (long)niv;
-                            jobs.push_back(J);
-                            nvec += (size_t)(A.l + 1) * (B.l + 1) * (N + 1);
-                            niv += (size_t)(N + A.l + B.l + 1);
-                        } else {
-                            const double cen = esum > 0.0 ? (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum
-                                                          : 0.5 * (A.xyz[d] + B.xyz[d]);
-                            P.cen[d] = cen;
-                            fac *= std::exp(-mu * (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]));
-                        }
-                    }
-                    P.fac = fac;
-                    P.eoff = (int)herm.size();
-                    herm.resize(herm.size() + (size_t)nreal * (A.l + 1) * (B.l + 1) * (2 * kMaxL + 1), 0.0);
-                    for (int dr = c->ndp; dr < 3; ++dr)
-                        for (int ia = 0; ia <= A.l; ++ia)
-                            for (int ib = 0; ib <= B.l; ++ib)
-                                hermite_E(ia, ib, esum, P.cen[dr] - A.xyz[dr], P.cen[dr] - B.xyz[dr],
-                                          herm.data() + P.eoff +
-                                              ((size_t)((dr - c->ndp) * (A.l + 1) + ia) * (B.l + 1) + ib) * (2 * kMaxL + 1));
-                    double ca = 0.0, cb = 0.0;
-                    for (int x = 0; x < A.ncomp; ++x) ca = std::max(ca, std::fabs(c->coef[c->poff[A.first_ao + x] + i]));
-                    for (int x = 0; x < B.ncomp; ++x) cb = std::max(cb, std::fabs(c->coef[c->poff[B.first_ao + x] + j]));
-                    P.cmax = ca * cb;
-                    P.ps = 1.0;
-                    c->pp.push_back(P);
-                }
+            spr.first = (int)L.npp;
+            spr.count = (int)nij;
+            PairBlock pb;
+            pb.S1 = S1;
+            pb.S2 = S2;
+            pb.first = (int)L.npp;
+            pb.job0 = (int)L.njobs;
+            pb.vo0 = (int)L.nvo;
+            pb.vec0 = (int)L.nvec;
+            pb.iv0 = (long long)L.niv;
+            pb.herm0 = (int)L.nherm;
+            pb.dco0 = (int)L.ndco;
+            blk.push_back(pb);
+            L.npp += nij;
+            L.njobs += nij * ndp;
+            L.nvo += nij * ndp * ncomb;
+            L.nvec += nij * ndp * ncomb * (size_t)(c->ncap + 1);
+            L.niv += nij * ndp * (size_t)(c->ncap + A.l + B.l + 1);
+            L.nherm += nij * nreal * ncomb * kE;
+            L.ndco += nij * ndp * ncomb * (size_t)(2 * (A.l + B.l) + 1);
         }
-    if (nvec > (size_t)0x7fffffff) return c->fail(TCG_E_NOMEM, "pair vectors exceed 2^31 elements");
+    if (L.nvec > (size_t)0x7fffffff || L.nherm > (size_t)0x7fffffff || L.ndco > (size_t)0x7fffffff ||
+        L.nvo > (size_t)0x7fffffff)
+        return c->fail(TCG_E_NOMEM, "pair vectors exceed 2^31 elements (%zu)", L.nvec);
     return 0;
 }
 
@@ -694,12 +633,11 @@ static int prepare(tcg_ctx *c) {
     const bool tables_kept = build_tables(c, tab, tablim, tsum, rtnode, rtw) == 1;
     c->mark("tables");
     c->times[7] = now_ms() - tt0;
-    std::vector<VecJob> jobs;
-    size_t nvec = 0, niv = 0;
-    std::vector<double> herm;
-    int rc = build_pairs(c, jobs, nvec, niv, herm);
+    std::vector<PairBlock> pblk;
+    PairLayout PL;
+    int rc = build_pair_blocks(c, pblk, PL);
     if (rc) return rc;
-    c->mark("pairs (host)");
+    c->mark("pair layout (host)");
     c->max_ncomb = 1;
     for (auto &s : c->sh) c->max_ncomb = std::max(c->max_ncomb, s.l + 1);
     c->max_ncomb = c->max_ncomb * c->max_ncomb * c->max_ncomb * c->max_ncomb;
@@ -725,43 +663,31 @@ static int prepare(tcg_ctx *c) {
     HIPCHK(c, upload_async(c, c->d_coef, c->coef));
     HIPCHK(c, upload_async(c, c->d_expo, c->expo));
     HIPCHK(c, upload_async(c, c->d_poff, c->poff));
-    HIPCHK(c, upload_async(c, c->d_pp, c->pp));
     HIPCHK(c, upload_async(c, c->d_sp, c->sp));
+    HIPCHK(c, upload_async(c, c->d_pblk, pblk));
+    HIPCHK(c, upload_async(c, c->d_relc, c->relc));
     c->mark("uploads");
     {
-        // Fourier coefficients d_k of every (job, power pair) on the host, vectors on the GPU
-        std::vector<VecOut> vo;
-        std::vector<double2> dco;
-        vo.reserve(jobs.size() * (kMaxL + 1) * (kMaxL + 1));
-        dco.reserve(jobs.size() * 16);
-        std::complex<double> dk[4 * kMaxL + 1];
-        for (size_t j = 0; j < jobs.size(); ++j) {
-            const VecJob &J = jobs[j];
-            for (int ia = 0; ia <= J.lA; ++ia)
-                for (int ib = 0; ib <= J.lB; ++ib) {
-                    const int K = ia + ib;
-                    sin_product_fourier(ia, ib, J.phA, J.phB, dk);
-                    VecOut o;
-                    o.job = (int)j;
-                    o.dco = (int)dco.size();
-                    o.voff = J.voff + (ia * (J.lB + 1) + ib) * (J.N + 1);
-                    o.K = K;
-                    for (int k = 0; k <= 2 * K; ++k) dco.push_back(double2{dk[k].real(), dk[k].imag()});
-                    vo.push_back(o);
-                }
-        }
-        HIPCHK(c, upload_async(c, c->d_jobs, jobs));
-        HIPCHK(c, upload_async(c, c->d_vo, vo));
-        HIPCHK(c, upload_async(c, c->d_dco, dco));
-        if (niv > c->d_iv.n || nvec > c->d_vec.n) HIPCHK(c, hipStreamSynchronize(c->st));
-        HIPCHK(c, c->d_iv.alloc(niv));
-        HIPCHK(c, c->d_vec.alloc(nvec));
-        HIPCHK(c, launch_pair_vectors(c->d_jobs.p, (int)jobs.size(), c->d_vo.p, (int)vo.size(), c->d_dco.p, c->d_iv.p,
+        // primitive pairs, jobs, Fourier and Hermite coefficients on the GPU, then the
+        // Bessel values and Fourier vectors
+        const bool grow = PL.npp > c->d_pp.n || PL.njobs > c->d_jobs.n || PL.nvo > c->d_vo.n ||
+                          PL.ndco > c->d_dco.n || PL.nherm > c->d_herm.n || PL.niv > c->d_iv.n ||
+                          PL.nvec > c->d_vec.n;
+        if (grow) HIPCHK(c, hipStreamSynchronize(c->st));      // no kernel still reads the old buffers
+        HIPCHK(c, c->d_pp.alloc(PL.npp));
+        HIPCHK(c, c->d_jobs.alloc(PL.njobs));
+        HIPCHK(c, c->d_vo.alloc(PL.nvo));
+        HIPCHK(c, c->d_dco.alloc(PL.ndco));
+        HIPCHK(c, c->d_herm.alloc(PL.nherm));
+        HIPCHK(c, c->d_iv.alloc(PL.niv));
+        HIPCHK(c, c->d_vec.alloc(PL.nvec));
+        HIPCHK(c, launch_pair_setup(c->d_pblk.p, (int)pblk.size(), c->d_sh.p, c->d_expo.p, c->d_coef.p, c->d_poff.p,
+                                    c->ndp, c->ncap, c->L, c->a, c->d_pp.p, c->d_jobs.p, c->d_vo.p, c->d_dco.p,
+                                    c->d_herm.p, c->st));
+        HIPCHK(c, launch_pair_vectors(c->d_jobs.p, (int)PL.njobs, c->d_vo.p, (int)PL.nvo, c->d_dco.p, c->d_iv.p,
                                       c->d_vec.p, c->st));
     }
-    c->mark("fourier + pair vectors");
-    HIPCHK(c, upload_async(c, c->d_herm, herm));
-    HIPCHK(c, upload_async(c, c->d_relc, c->relc));
+    c->mark("pair set-up + vectors (launched)");
     HIPCHK(c, c->d_cnt.alloc(8));
     c->prepared = true;
     return 0;

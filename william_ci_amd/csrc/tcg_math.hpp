@@ -1,10 +1,15 @@
-// Scalar special functions and primitive-pair geometry of the engine (host
-// side, FP64).  Written from the mathematics; the reference routines they
+// Scalar special functions and primitive-pair geometry of the engine (FP64;
+// the pair geometry, truncation order, Hermite and Fourier coefficients are
+// shared by the host and the GPU pair set-up).  Written from the mathematics; the reference routines they
 // must agree with are cited per function.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cmath>
 #include <complex>
 #include <vector>
+
+#include "tcg_engine.hpp"
 
 namespace tcg {
 
@@ -90,7 +95,7 @@ inline void bessel_scaled_vector(int N, double A, double *out) {
 }
 
 // Truncation order of exp(-A) I_n(A) (src/module/clifford_tools.f90:566-572)
-inline int nmax_est(double A) { return 20 + (int)std::ceil(9.0 * std::sqrt(A > 0.0 ? A : 0.0)); }
+__host__ __device__ inline int nmax_est(double A) { return 20 + (int)ceil(9.0 * sqrt(A > 0.0 ? A : 0.0)); }
 
 // n-point Gauss-Legendre rule on u in [0,1] mapped to t in [0,inf) by
 // t = s u/(1-u), weights w_u s/(1-u)^2, s = 0.6: the reference's fast_quad
@@ -134,31 +139,31 @@ inline void mapped_gauss_legendre(int n, double *t, double *w) {
 //     centre of the tighter primitive (midpoint when e1 ~ e2 to 1e-10);
 //   e1 ~ e2 (1e-10): midpoint, shifted by -L/2 when |x1-x2| > L/2;
 //   p clamped to e1+e2, and snapped to e1+e2 within 1e-14.
-inline void clifford_pair(double e1, double e2, double x1, double x2, double L, double a,
-                          double *p, double *P) {
+__host__ __device__ inline void clifford_pair(double e1, double e2, double x1, double x2, double L, double a,
+                                              double *p, double *P) {
     const double eps = 1e-6;
-    double dx = x1 - x2, adx = std::fabs(dx);
+    double dx = x1 - x2, adx = fabs(dx);
     double r;
-    if (adx > 0.5 * L - eps && adx < 0.5 * L + eps) r = (std::fabs(e1 - e2) < 1e-8) ? 0.0 : std::fabs(e1 - e2);
-    else r = std::sqrt(std::fabs(e1 * e1 + e2 * e2 + 2.0 * e1 * e2 * std::cos(a * dx)));
+    if (adx > 0.5 * L - eps && adx < 0.5 * L + eps) r = (fabs(e1 - e2) < 1e-8) ? 0.0 : fabs(e1 - e2);
+    else r = sqrt(fabs(e1 * e1 + e2 * e2 + 2.0 * e1 * e2 * cos(a * dx)));
     if (r > e1 + e2) r = e1 + e2;
     if (e1 + e2 - r < 1e-14) r = e1 + e2;
     *p = r;
     if (adx < 0.5 * L + eps && adx > 0.5 * L - eps) {
-        if (std::fabs(e1 - e2) < 1e-10) *P = 0.5 * (x1 + x2);
+        if (fabs(e1 - e2) < 1e-10) *P = 0.5 * (x1 + x2);
         else *P = (e1 > e2) ? x1 : x2;
-    } else if (std::fabs(e1 - e2) < 1e-10) {
+    } else if (fabs(e1 - e2) < 1e-10) {
         *P = 0.5 * (x1 + x2) - ((adx > 0.5 * L + eps) ? 0.5 * L : 0.0);
     } else {
-        double c = e1 * std::cos(a * x1) + e2 * std::cos(a * x2);
-        double sn = e1 * std::sin(a * x1) + e2 * std::sin(a * x2);
-        *P = std::atan(sn / c) / a + ((c <= 0.0) ? 0.5 * L : 0.0);
+        double c = e1 * cos(a * x1) + e2 * cos(a * x2);
+        double sn = e1 * sin(a * x1) + e2 * sin(a * x2);
+        *P = atan(sn / c) / a + ((c <= 0.0) ? 0.5 * L : 0.0);
     }
 }
 
 // Hermite expansion of (y-A)^i (y-B)^j exp(-p(y-P)^2) = sum_t E_t d^t/dP^t exp(-p(y-P)^2),
 // t = 0..i+j (McMurchie-Davidson; src/module/clifford_tools.f90:757-785).
-inline void hermite_E(int i, int j, double p, double PA, double PB, double *E /* [i+j+1] */) {
+__host__ __device__ inline void hermite_E(int i, int j, double p, double PA, double PB, double *E /* [i+j+1] */) {
     double cur[2 * 8 + 3] = {0}, nxt[2 * 8 + 3];
     double *c = cur + 1;   // c[-1] = 0
     c[0] = 1.0;
@@ -195,6 +200,36 @@ inline void sin_product_fourier(int i, int l, double phA, double phB, std::compl
             nxt[k - 1 + K + 1] += cm * cur[k + K + 1];
         }
         std::swap(cur, nxt);
+    }
+    for (int k = -K; k <= K; ++k) d[k + K] = cur[k + K + 1];
+}
+
+// The same coefficients as (re, im) pairs, for the host and the GPU pair set-up:
+// d[k + K] = d_k, k = -K..K, K = i + l <= 2 kMaxL.
+__host__ __device__ inline void sin_fourier_coefs(int i, int l, double phA, double phB, double2 *d) {
+    const int K = i + l;
+    constexpr int kCap = 4 * kMaxL + 3;
+    double2 buf0[kCap], buf1[kCap];
+    double2 *cur = buf0, *nxt = buf1;
+    for (int k = 0; k < 2 * K + 3; ++k) cur[k] = double2{0.0, 0.0};
+    cur[K + 1] = double2{1.0, 0.0};   // index k + K + 1
+    const double sA = sin(phA), cA = cos(phA), sB = sin(phB), cB = cos(phB);
+    for (int m = 1; m <= K; ++m) {
+        const double sn = (m <= i) ? sA : sB, cs = (m <= i) ? cA : cB;
+        const double pr = 0.5 * sn, pi = -0.5 * cs;        // e^{i ph}/(2i)
+        const double mr = 0.5 * sn, mi = 0.5 * cs;         // -e^{-i ph}/(2i)
+        for (int k = 0; k < 2 * K + 3; ++k) nxt[k] = double2{0.0, 0.0};
+        for (int k = -(m - 1); k <= m - 1; ++k) {
+            const double2 v = cur[k + K + 1];
+            double2 &up = nxt[k + 1 + K + 1], &dn = nxt[k - 1 + K + 1];
+            up.x += pr * v.x - pi * v.y;
+            up.y += pr * v.y + pi * v.x;
+            dn.x += mr * v.x - mi * v.y;
+            dn.y += mr * v.y + mi * v.x;
+        }
+        double2 *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
     }
     for (int k = -K; k <= K; ++k) d[k + K] = cur[k + K + 1];
 }

@@ -194,18 +194,44 @@ hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long 
     return hipGetLastError();
 }
 
-// one wavefront per tile of 16 sorted rows; several tiles per wavefront.
-// kNT = node tiles of 16 (2 for the 32-node rule, 3 for 40 nodes).
+// One tile = 16 sorted rows; a workgroup takes four consecutive tiles (one
+// per wavefront) at a time.  kNT = node tiles of 16 in this launch (4 for the
+// 64-node rule, 2 for 32 nodes, 3 for 40).
+//
+// Lanes build F_n = Re[conj(G_n) H_n e^{in phi}] (the A operand of
+// v_mfma_f64_16x16x4f64) for 16 rows x 4 orders; the t-table is the B
+// operand, D accumulates 16 rows x 16 nodes over n in steps of 4.  The
+// workgroup's four tiles run one (rule, direction) combination at a time (one
+// in all but the groups where the sorted rows change combination): the table
+// slice of S orders x the launch's nodes is staged in LDS by the whole
+// workgroup, double-buffered (the next slice's global loads in flight while
+// the current one feeds the matrix cores), so each table entry is read from
+// L2 once per workgroup instead of once per wavefront.
+//
+// The node tiles run as slots sorted by their last order, descending: stage
+// c has the c slots with the largest limits active, each stage a loop of
+// unconditional MFMAs.  G and H are loaded two steps ahead; the phase
+// advances by a rotation with a second rotation per 64 orders (no sincos).
 template <int kNT>
 __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__restrict__ rows,
                                                   const int *__restrict__ order, long long nrows, int hbase,
                                                   double *__restrict__ rowval) {
+    constexpr int S = 16;                            // orders per LDS slice
+    constexpr int TW = kNT * kNodeTile;              // nodes of this launch
+    constexpr int W = TW + 4;                        // LDS row stride (padded)
+    constexpr int NE = S * TW / 256;                 // slice entries per thread
+    __shared__ double tl[2][S * W];
+    __shared__ int wdd[4][2], wnm[4];
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nqp = A.nqp, nht = A.nqp / kNodeTile;
-    const int tabrows = A.ncap + 5;
-    const long long ntiles = (nrows + 15) / 16;
+    const int tabrows = A.ncap + 5, tlast = tabrows - 1;
+    const long long ntiles = (nrows + 15) / 16, ngroups = (ntiles + 3) / 4;
     long long flops = 0;                              // this lane's rows' flop count (one atomic per wavefront)
-    for (long long tile = (long long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long long)gridDim.x * 4) {
+    // the groups from the last (the rows sort by ascending order, so the costliest
+    // groups start first and the cheap ones fill the tail)
+    for (long long gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const long long grp = ngroups - 1 - gi;
+        const long long tile = grp * 4 + wv;
         const long long pos = tile * 16 + r;
         const bool valid = pos < nrows;
         int ridx = -1, nm = -1, d = 0x7fffffff;         // d: direction | rule << 8
@@ -242,12 +268,22 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
             cn0 = q <= kq ? tc : cn0;
             sn0 = q <= kq ? ts : sn0;
         }
-        // the (direction, rule) combinations present in the tile, ascending
-#pragma unroll 1
-        for (int dd = wmin(d); dd != 0x7fffffff; dd = wmin(d > dd ? d : 0x7fffffff)) {
+        // smallest combination above lo over the workgroup's four tiles (uniform)
+        auto group_min = [&](int lo) {
+            const int m = wmin(d > lo ? d : 0x7fffffff);
+            if (lane == 0) wdd[wv][0] = m;
+            __syncthreads();
+            const int g = __builtin_amdgcn_readfirstlane(min(min(wdd[0][0], wdd[1][0]), min(wdd[2][0], wdd[3][0])));
+            __syncthreads();
+            return g;
+        };
+        // per combination dd: slots, accumulators, the order loop, the write-out
+        auto run = [&](int dd) {
             const int nmd = (d == dd) ? nm : -1;      // rows of another combination contribute 0
             const int tr = (dd >> 8) * A.ndp + (dd & 0xff);   // table block of this rule and direction
-            const int nmax_t = wmax(nmd);
+            // (wavefront-uniform values in scalar registers: the order loops below are
+            // scalar loops, not exec-masked ones)
+            const int nmax_t = __builtin_amdgcn_readfirstlane(wmax(nmd));
             // node tiles as slots sorted by their last order, descending (uniform values);
             // padding tiles of a rule with fewer nodes have lim -1 and never run
             int sl[kNT], sp[kNT];
@@ -268,7 +304,7 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
                     sp[j - 1] = sw ? pb : pa;
                     sp[j] = sw ? pa : pb;
                 }
-            const double *tab = A.tab + (size_t)tr * tabrows * nqp + hbase * kNodeTile + r;
+            const double *tabg = A.tab + (size_t)tr * tabrows * nqp + hbase * kNodeTile;
             int toff[kNT];
 #pragma unroll
             for (int j = 0; j < kNT; ++j) toff[j] = sp[j] * kNodeTile;
@@ -276,23 +312,16 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
 #pragma unroll
             for (int j = 0; j < kNT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
             double cn = cn0, sn = sn0, cb = cn0, sb = sn0;
-            // Two register sets, each consumed by one step and refilled for the step after
-            // next (loads two steps ahead, no register copies).  Rows past nm and orders past
-            // the table read in-bounds entries (order 0 of the row, the zero rows at the
-            // table's end) whose products the selection of F or the zero table make vanish.
-            const int tlast = tabrows - 1;
+            // Two register sets of G, H, each consumed by one step and refilled for the step
+            // after next.  Rows past nm and orders past the table read in-bounds entries
+            // (order 0 of the row, the zero rows at the table's end) whose products the
+            // selection of F or the zero table make vanish.
             auto gidx = [&](int n) { return n <= nmd ? n : 0; };
             auto tidx = [&](int n) { return (size_t)(n < tlast ? n : tlast) * nqp; };
             double2 gP = Gp[gidx(kq)], hP = Hp[gidx(kq)];
             double2 gQ = Gp[gidx(4 + kq)], hQ = Hp[gidx(4 + kq)];
-            double bP[kNT], bQ[kNT];
-#pragma unroll
-            for (int j = 0; j < kNT; ++j) {
-                bP[j] = tab[tidx(kq) + toff[j]];
-                bQ[j] = tab[tidx(4 + kq) + toff[j]];
-            }
-            // one step of 4 orders m..m+3 (lane order m + kq) with the set (g, h, b); c slots active
-            auto step = [&](int m, int c, double2 &g, double2 &hh, double *b) {
+            int s0 = 0, cur = 0;
+            auto fval = [&](int m, const double2 &g, const double2 &hh) {
                 const int n = m + kq;
                 if (m > 0) {
                     if ((m & 63) == 0) {
@@ -309,27 +338,66 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
                 }
                 const double re = g.x * hh.x + g.y * hh.y;   // conj(g) * h
                 const double im = g.x * hh.y - g.y * hh.x;
-                const double F = (n <= nmd) ? re * cn - im * sn : 0.0;
+                return (n <= nmd) ? re * cn - im * sn : 0.0;
+            };
+            // one step of 4 orders m..m+3 (lane order m + kq); c slots active; the table
+            // from the LDS slice
+            auto step = [&](int m, int c, double2 &g, double2 &hh) {
+                const int n = m + kq;
+                const double F = fval(m, g, hh);
+                const double *brow = &tl[cur][(n - s0) * W + r];
 #pragma unroll
                 for (int j = 0; j < kNT; ++j)
-                    if (j < c) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, b[j], acc[j], 0, 0, 0);
+                    if (j < c) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, brow[toff[j]], acc[j], 0, 0, 0);
                 const int n2 = n + 8;
                 g = Gp[gidx(n2)];
                 hh = Hp[gidx(n2)];
-                const size_t t2 = tidx(n2);
-#pragma unroll
-                for (int j = 0; j < kNT; ++j) b[j] = j < c ? tab[t2 + toff[j]] : 0.0;
             };
-            // stage c: the c slots with the largest limits are active, up to order sl[c-1]
-            // (steps in pairs: one extra step past a limit only adds zero products)
+            // stage loop over [n0, s_end): stage c has the c slots with the largest limits
+            // active, up to order sl[c-1] (steps in pairs: one extra step past a limit only
+            // adds zero products)
             int n0 = 0;
+            auto stages = [&](int s_end) {
 #pragma unroll
-            for (int c = kNT; c >= 1; --c) {
-                const int lend = sl[c - 1];
+                for (int c = kNT; c >= 1; --c) {
+                    const int lend = sl[c - 1];
 #pragma unroll 1
-                for (; n0 <= lend; n0 += 8) {
-                    step(n0, c, gP, hP, bP);
-                    step(n0 + 4, c, gQ, hQ, bQ);
+                    for (; n0 <= lend && n0 < s_end; n0 += 8) {
+                        step(n0, c, gP, hP);
+                        step(n0 + 4, c, gQ, hQ);
+                    }
+                }
+            };
+            {
+                // the workgroup's largest order, then the slices
+                if (lane == 0) wnm[wv] = nmax_t;
+                __syncthreads();
+                const int nmax_wg = __builtin_amdgcn_readfirstlane(max(max(wnm[0], wnm[1]), max(wnm[2], wnm[3])));
+                double stg[NE];
+                auto gload = [&](int sb0) {
+#pragma unroll
+                    for (int i = 0; i < NE; ++i) {
+                        const int e = threadIdx.x + 256 * i;
+                        stg[i] = tabg[tidx(sb0 + e / TW) + e % TW];
+                    }
+                };
+                auto sstore = [&](int buf) {
+#pragma unroll
+                    for (int i = 0; i < NE; ++i) {
+                        const int e = threadIdx.x + 256 * i;
+                        tl[buf][(e / TW) * W + e % TW] = stg[i];
+                    }
+                };
+                gload(0);
+                sstore(0);
+                __syncthreads();
+                for (s0 = 0; s0 <= nmax_wg; s0 += S) {
+                    const bool more = s0 + S <= nmax_wg;
+                    if (more) gload(s0 + S);
+                    stages(s0 + S);
+                    if (more) sstore(cur ^ 1);
+                    __syncthreads();
+                    cur ^= 1;
                 }
             }
             // lane (r, kq) holds D[row = 4v + kq][node = 16h + r]
@@ -345,7 +413,11 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
                         rowval[(size_t)rrow * nqp + hbase * kNodeTile + toff[j] + r] = acc[j][v] * sg;
                 }
             }
-        }
+        };
+        // the (rule, direction) combinations present in the workgroup's tiles, ascending
+        // (one, except where the sorted rows change combination)
+#pragma unroll 1
+        for (int dd = group_min(-1); dd != 0x7fffffff; dd = group_min(dd)) run(dd);
     }
     if (A.counters) {
 #pragma unroll
@@ -359,7 +431,11 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
     if (nrows <= 0) return hipSuccess;
     const long long ntiles = (nrows + 15) / 16;
     const long long nwg = (ntiles + 3) / 4;                 // four wavefronts per workgroup, a tile each
-    const int grid = nwg < (1 << 20) ? (int)nwg : (1 << 20);
+    // at most as many workgroups as the chip holds at once (the LDS path's barriers
+    // keep the four wavefronts of a workgroup together; groups of tiles are walked)
+    // (an XCD-aware walk, each XCD a contiguous eighth of the sorted groups, was
+    // slower: the eighths differ several-fold in cost, the orders growing along the sort)
+    const int grid = nwg < 2048 ? (int)nwg : 2048;
     const int nht = a.nqp / kNodeTile;
     // up to four node tiles per launch (more nodes: one launch per group of four)
     for (int h0 = 0; h0 < nht; h0 += 4) {

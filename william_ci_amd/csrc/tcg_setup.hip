@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tcg_device.hpp"
+#include "tcg_math.hpp"
 
 namespace tcg {
 
@@ -127,9 +128,117 @@ __global__ __launch_bounds__(64) void k_bessel_jobs(const VecJob *__restrict__ j
     }
 }
 
+// Primitive-pair records of the representative shell pairs, on the GPU: one
+// workgroup per shell pair, a thread per primitive pair (i major).  Per pair:
+// prefactor prod_d exp(-2(e1+e2-p_d)/a_d^2) prod_real exp(-mu R^2),
+// barycentres (toroidal per periodic direction, clifford_pair; the ordinary
+// Gaussian centre in the real ones), exponent sum, coefficient bound, Bessel
+// lengths N_d = min(nmax(2 p_d / a_d^2), ncap) and the Fourier-vector jobs with
+// the coefficients d_k of sin^ia(u+phA) sin^ib(u+phB); Hermite E of the real
+// directions (src/module/clifford_tools.f90:467-549,578-607,715-785;
+// src/Toroidal/ERI_shell_block_toroidal.f90:133-209).  ps starts at 1 (the
+// Schwarz pass sets it).
+struct PairSetupArgs {
+    const PairBlock *blk;
+    const Shell *sh;
+    const double *expo, *coef;
+    const int *ao_poff;
+    int ndp, ncap;
+    double L[3], a[3];
+    PrimPair *pp;
+    VecJob *jobs;
+    VecOut *vo;
+    double2 *dco;
+    double *herm;
+};
+
+__global__ __launch_bounds__(64) void k_pair_setup(PairSetupArgs S) {
+    const PairBlock B = S.blk[blockIdx.x];
+    const Shell sA = S.sh[B.S1], sB = S.sh[B.S2];
+    const int nij = sA.nprim * sB.nprim, ndp = S.ndp, nreal = 3 - ndp;
+    const int ncomb = (sA.l + 1) * (sB.l + 1), kE = 2 * kMaxL + 1, dstride = 2 * (sA.l + sB.l) + 1;
+    for (int ij = threadIdx.x; ij < nij; ij += blockDim.x) {
+        const int i = ij / sB.nprim, j = ij - i * sB.nprim;
+        PrimPair P{};
+        P.pi = i;
+        P.pj = j;
+        const double e1 = S.expo[sA.poff + i], e2 = S.expo[sB.poff + j];
+        // esum = 0: two constant functions (the exponent-0 s function of the
+        // density-fitting integrals, df_toroidal.f90:837-844)
+        const double esum = e1 + e2, mu = esum > 0.0 ? e1 * e2 / esum : 0.0;
+        P.q = esum;
+        double fac = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            if (d < ndp) {
+                double pe, cen;
+                clifford_pair(e1, e2, sA.xyz[d], sB.xyz[d], S.L[d], S.a[d], &pe, &cen);
+                P.cen[d] = cen;
+                fac *= exp(-2.0 * (esum - pe) / (S.a[d] * S.a[d]));
+                const double Aarg = 2.0 * pe / (S.a[d] * S.a[d]);
+                const int nm = nmax_est(Aarg);
+                const int N = nm < S.ncap ? nm : S.ncap;
+                const int jl = ij * ndp + d;
+                P.N[d] = N;
+                P.voff[d] = B.vec0 + jl * ncomb * (S.ncap + 1);
+                VecJob J;
+                J.A = Aarg;
+                J.phA = S.a[d] * (cen - sA.xyz[d]);
+                J.phB = S.a[d] * (cen - sB.xyz[d]);
+                J.N = N;
+                J.lA = sA.l;
+                J.lB = sB.l;
+                J.voff = P.voff[d];
+                J.ivoff = B.iv0 + (long long)jl * (S.ncap + sA.l + sB.l + 1);
+                S.jobs[B.job0 + jl] = J;
+                for (int ia = 0; ia <= sA.l; ++ia)
+                    for (int ib = 0; ib <= sB.l; ++ib) {
+                        const int cl = jl * ncomb + ia * (sB.l + 1) + ib;
+                        VecOut o;
+                        o.job = B.job0 + jl;
+                        o.dco = B.dco0 + cl * dstride;
+                        o.voff = J.voff + (ia * (sB.l + 1) + ib) * (N + 1);
+                        o.K = ia + ib;
+                        S.vo[B.vo0 + cl] = o;
+                        sin_fourier_coefs(ia, ib, J.phA, J.phB, S.dco + o.dco);
+                    }
+            } else {
+                const double cen = esum > 0.0 ? (e1 * sA.xyz[d] + e2 * sB.xyz[d]) / esum
+                                              : 0.5 * (sA.xyz[d] + sB.xyz[d]);
+                P.cen[d] = cen;
+                fac *= exp(-mu * (sA.xyz[d] - sB.xyz[d]) * (sA.xyz[d] - sB.xyz[d]));
+            }
+        }
+        P.fac = fac;
+        P.eoff = B.herm0 + ij * nreal * ncomb * kE;
+        double *E = S.herm + P.eoff;
+        for (int x = 0; x < nreal * ncomb * kE; ++x) E[x] = 0.0;
+        for (int dr = ndp; dr < 3; ++dr)
+            for (int ia = 0; ia <= sA.l; ++ia)
+                for (int ib = 0; ib <= sB.l; ++ib)
+                    hermite_E(ia, ib, esum, P.cen[dr] - sA.xyz[dr], P.cen[dr] - sB.xyz[dr],
+                              E + ((size_t)((dr - ndp) * (sA.l + 1) + ia) * (sB.l + 1) + ib) * kE);
+        double ca = 0.0, cb = 0.0;
+        for (int x = 0; x < sA.ncomp; ++x) ca = fmax(ca, fabs(S.coef[S.ao_poff[sA.first_ao + x] + i]));
+        for (int x = 0; x < sB.ncomp; ++x) cb = fmax(cb, fabs(S.coef[S.ao_poff[sB.first_ao + x] + j]));
+        P.cmax = ca * cb;
+        P.ps = 1.0;
+        S.pp[B.first + ij] = P;
+    }
+}
+
+hipError_t launch_pair_setup(const PairBlock *blk, int nblk, const Shell *sh, const double *expo, const double *coef,
+                             const int *ao_poff, int ndp, int ncap, const double L[3], const double a[3], PrimPair *pp,
+                             VecJob *jobs, VecOut *vo, double2 *dco, double *herm, hipStream_t st) {
+    if (nblk <= 0) return hipSuccess;
+    PairSetupArgs S{blk, sh, expo, coef, ao_poff, ndp, ncap, {L[0], L[1], L[2]}, {a[0], a[1], a[2]}, pp, jobs, vo,
+                    dco, herm};
+    hipLaunchKernelGGL(k_pair_setup, dim3(nblk), dim3(64), 0, st, S);
+    return hipGetLastError();
+}
+
 // V_m = sum_k d_k exp(-A) I_|m-k|(A), m = 0..N: one block per vector, the
 // orders m across the threads.  d_k (Fourier coefficients of
-// sin^ia(u+phA) sin^ib(u+phB), k = -K..K) come from the host.
+// sin^ia(u+phA) sin^ib(u+phB), k = -K..K) from the pair set-up.
 __global__ void k_pair_combine(const VecOut *__restrict__ vo, const VecJob *__restrict__ jobs,
                                const double *__restrict__ ivbuf, const double2 *__restrict__ dco,
                                double2 *__restrict__ vec) {
