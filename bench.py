@@ -66,6 +66,11 @@ def cpu_baseline(sysm, quad, seconds: float):
     except Exception:
         ref = None
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    flags = "-O2"
+    if ref is not None and ref.fast_available():
+        # the reference built with its own optimisation flags (its Makefile:37)
+        ref.use_fast_build()
+        flags = "-O3 -funroll-loops -ftree-vectorize -ffast-math -march=x86-64-v3"
     if ref is not None and ref.available():
         r = ref.RefSystem(sysm, nq=nquad, fast_quad=fast)
         n, t0 = 0, time.perf_counter()
@@ -80,7 +85,7 @@ def cpu_baseline(sysm, quad, seconds: float):
                 "cores": threads, "kind": "reference",
                 "sample": f"{n} full ERI builds of {sysm.kind} {sysm.nbas}-function workload, {nquad}-point "
                           f"{'fast' if fast else 'table'} rule "
-                          f"(reference Fortran ERI_integral_toroidal, OpenMP, {el:.1f} s)",
+                          f"(reference Fortran ERI_integral_toroidal, amdflang {flags}, OpenMP, {el:.1f} s)",
                 "seconds_per_build": el / n}
     from oracle import orc
     from william_ci_amd.basis import normalize

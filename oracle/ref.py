@@ -21,8 +21,25 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _ci = ctypes.c_int
 
 
+_LIB_FAST_PATH = os.path.join(_HERE, "_ref_fast", "libtcgref.so")
+
+
 def available() -> bool:
     return os.path.exists(_LIB_PATH)
+
+
+def fast_available() -> bool:
+    return os.path.exists(_LIB_FAST_PATH)
+
+
+def use_fast_build() -> None:
+    """Loads the reference built with its own optimisation flags
+    (oracle/_ref_fast) instead of the -O2 checker build; for timing only.
+    Must come before the first call of lib()."""
+    global _LIB_PATH, _lib
+    if _lib is not None and _LIB_PATH != _LIB_FAST_PATH:
+        raise RuntimeError("the reference library is already loaded")
+    _LIB_PATH = _LIB_FAST_PATH
 
 
 def lib():
