@@ -45,6 +45,7 @@ struct EriArgs {
     const double *herm;
     const Quartet *q4;
     int nq4;
+    const unsigned long long *nq4_dev;   // set: nq4 is a capacity, the list's length is on the device
     int screen_prefactor;             // kernel prefactor screens (shell quartets only, q.prim < 0)
     int screen_prim;                  // primitive Schwarz screen (keyword prim_screen)
     int max_ncomb;                    // largest (lA+1)(lB+1)(lC+1)(lD+1) over the quartets
@@ -66,6 +67,7 @@ struct EriArgs {
 };
 
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
+hipError_t launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t st);
 // address of the allocation-free J/K's scratch array on the current device (once per device)
 hipError_t noalloc_setup();
 size_t noalloc_scratch_doubles();

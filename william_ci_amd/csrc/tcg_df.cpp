@@ -55,8 +55,13 @@ struct DBuf {
         p = nullptr;
         n = 0;
     }
+    // grows the buffer; the old one is freed only after the device has drained
     hipError_t alloc(size_t c) {
         if (c <= n && p) return hipSuccess;
+        if (p) {
+            const hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) return e;
+        }
         release();
         n = c;
         return hipMalloc((void **)&p, sizeof(T) * (c ? c : 1));

@@ -14,6 +14,20 @@ __global__ void k_fill(double *p, size_t n, double v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st) {
     if (n == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;

@@ -48,8 +48,15 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // grows the buffer; the old one is freed only after the device has drained
+    // (work queued earlier, without a synchronisation in between, may still read
+    // it).  A grown buffer has a new address: callers re-read p after alloc.
     hipError_t alloc(size_t count) {
         if (count <= n && p) return hipSuccess;
+        if (p) {
+            const hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) return e;
+        }
         release();
         n = count;
         return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1));
@@ -130,7 +137,8 @@ struct tcg_ctx {
     int sp_contract = 0;         // the s/p contraction kernels take the quartets of s and p shells
     int sp_min_quartets = 16384; // ... of quartet lists at least this long
     bool prepared = false;
-    int nq4 = 0;                 // unique shell quartets in d_q4 (after setup_quartets)
+    int nq4 = 0;                 // unique shell quartets in d_q4 (after setup_quartets; -1: count on the device)
+    int q4_cap = 0;              // largest main enumeration so far (the capacity a deferred one writes into)
     std::vector<int> rule_nq;    // node count of each rule in the tables (rule 0 = the run's rule)
     int nqp_all = 32;            // node stride of the tables (largest padded node count)
     DevBuf<int> d_rule_nq;
@@ -494,8 +502,10 @@ static EriArgs base_args(tcg_ctx *c) {
 // survivors -> rows -> sort by (direction, length) -> MFMA -> contraction.
 // One chunk of quartets (A.q4[0..A.nq4)) whose survivor/row offsets so/ro
 // start at 0: survivors, rows, sort, MFMA rows, real rows, contraction.
+// padded: ns, nr are upper bounds (the exact counts stay on the device); the
+// row keys past the last row sort last, their order entries are -1.
 static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const long long *ro, long long ns,
-                     long long nr) {
+                     long long nr, bool padded = false) {
     HIPCHK(c, c->d_surv.alloc(ns));
     HIPCHK(c, c->d_rows.alloc(nr));
     HIPCHK(c, c->d_keys.alloc(nr));
@@ -503,6 +513,10 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     HIPCHK(c, c->d_vals.alloc(nr));
     HIPCHK(c, c->d_order.alloc(nr));
     HIPCHK(c, c->d_rowval.alloc((size_t)nr * A.nqp));
+    if (padded) {
+        HIPCHK(c, launch_fill_u32(c->d_keys.p, nr, 0xffffffffu, c->st));
+        HIPCHK(c, launch_fill_u32(reinterpret_cast<unsigned *>(c->d_vals.p), nr, 0xffffffffu, c->st));
+    }
     HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, so, c->d_surv.p, c->st));
     HIPCHK(c, launch_rows_emit(A, so, ro, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
     c->mark("  surv+emit launched");
@@ -541,8 +555,19 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
 // Quartets of A.q4[0..A.nq4) through the row pipeline (tcg_rows.hip):
 // survivors -> rows -> sort by (direction, length) -> MFMA -> contraction,
 // in chunks of at most row_cap rows so that the row buffers stay bounded.
-static int run_quartets(tcg_ctx *c, EriArgs A) {
-    const int nq4 = A.nq4;
+// run_quartets' answer when a deferred quartet list outgrew its capacity
+constexpr int kRecount = 1;
+
+// Quartets of A.q4[0..A.nq4) through the row pipeline.
+// ns_ub, nr_ub >= 0: upper bounds of the survivor and row counts, known on the
+// host; the pipeline then runs on buffers of those sizes without reading the
+// exact counts back (no host synchronisation; short lists such as the Schwarz
+// passes).
+// A.nq4_dev set: A.nq4 is the capacity of the quartet list and its length is
+// on the device (a deferred enumeration); it is read back with the counts,
+// and kRecount is returned when it exceeds the capacity.
+static int run_quartets(tcg_ctx *c, EriArgs A, long long ns_ub = -1, long long nr_ub = -1) {
+    int nq4 = A.nq4;
     c->last_surv = c->last_rows = 0;
     c->rows_ms = 0.0;
     if (nq4 <= 0) return 0;
@@ -557,10 +582,24 @@ static int run_quartets(tcg_ctx *c, EriArgs A) {
     HIPCHK(c, c->d_tmp.alloc(tb));
     HIPCHK(c, scan_exclusive(c->d_tmp.p, tb, c->d_scount.p, c->d_soff.p, nq4 + 1, c->st));
     HIPCHK(c, scan_exclusive(c->d_tmp.p, tb, c->d_rcount.p, c->d_roff.p, nq4 + 1, c->st));
+    if (ns_ub >= 0 && nr_ub >= 0 && nr_ub <= (1ll << 24) && !A.nq4_dev) {
+        if (nr_ub == 0) return 0;
+        return run_chunk(c, A, c->d_soff.p, c->d_roff.p, ns_ub > 0 ? ns_ub : 1, nr_ub, true);
+    }
     long long ns = 0, nr = 0;
+    unsigned long long nq4d = 0;
     HIPCHK(c, hipMemcpyAsync(&ns, c->d_soff.p + nq4, sizeof ns, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(&nr, c->d_roff.p + nq4, sizeof nr, hipMemcpyDeviceToHost, c->st));
+    if (A.nq4_dev) HIPCHK(c, hipMemcpyAsync(&nq4d, A.nq4_dev, sizeof nq4d, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (A.nq4_dev) {
+        if (nq4d > (unsigned long long)nq4) return kRecount;   // the list outgrew its capacity
+        nq4 = (int)nq4d;                                       // the counts past it are zero
+        A.nq4 = nq4;
+        A.nq4_dev = nullptr;
+        c->nq4 = nq4;
+        c->stats[0] = nq4;
+    }
     c->last_surv = ns;
     c->last_rows = nr;
     c->mark("  counts");
@@ -613,7 +652,16 @@ static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, s
     A.nq4 = (int)q4.size();
     A.blk_out = c->d_blk.p;
     A.screen_prefactor = prefactor;
-    int rc = run_quartets(c, A);
+    // survivor and row bounds: every primitive quartet of the listed quartets
+    long long ns_ub = 0, nr_ub = 0;
+    for (const Quartet &q : q4) {
+        const long long n = q.prim >= 0 ? 1 : (long long)c->sp[q.bra].count * c->sp[q.ket].count;
+        long long ncomb = 1;
+        for (int x = 0; x < 4; ++x) ncomb *= c->sh[q.s[x]].l + 1;
+        ns_ub += n;
+        nr_ub += n * c->ndp * ncomb;
+    }
+    int rc = run_quartets(c, A, ns_ub, nr_ub);
     if (rc) return rc;
     if (!out) return 0;
     out->resize(nout);
@@ -888,11 +936,44 @@ static int learned_pass(tcg_ctx *c, EriArgs A) {
     return 0;
 }
 
+// Step 5 (ERI_toroidal.f90:342-401): unique shell quartets (translation x 8
+// permutations), Schwarz screened, on the GPU, appended into the capacity of
+// an earlier main enumeration (c->q4_cap).  defer, with such a capacity in
+// place: the count stays on the device (c->nq4 = -1, count in d_cnt[3]);
+// run_quartets reads it with the survivor counts and answers kRecount when it
+// overflowed.  Otherwise counted here, and redone when it overflows.
+static int enumerate_quartets(tcg_ctx *c, bool defer) {
+    const int nsh = c->nsh, nshuc = c->nshuc, nc = c->ncell;
+    if (defer && c->q4_cap > 0 && c->d_q4.p && c->d_q4.n >= (size_t)c->q4_cap) {
+        HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
+        HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, c->d_qmax.p, nshuc, nsh, nc, c->d_cnt.p + 3,
+                                       c->d_q4.p, c->d_q4.n, c->st));
+        c->nq4 = -1;
+        return 0;
+    }
+    unsigned long long nq4 = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const unsigned long long cap = c->d_q4.p ? c->d_q4.n : 0;
+        HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
+        HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, c->d_qmax.p, nshuc, nsh, nc, c->d_cnt.p + 3,
+                                       cap ? c->d_q4.p : nullptr, cap, c->st));
+        HIPCHK(c, hipMemcpyAsync(&nq4, c->d_cnt.p + 3, sizeof nq4, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (nq4 > (unsigned long long)0x7fffffff) return c->fail(TCG_E_NOMEM, "too many shell quartets (%llu)", nq4);
+        if (nq4 <= cap) break;
+        HIPCHK(c, c->d_q4.alloc(nq4));
+    }
+    c->stats[0] = (long long)nq4;
+    c->nq4 = (int)nq4;
+    c->q4_cap = (int)std::max<unsigned long long>(nq4, (unsigned long long)c->q4_cap);
+    return 0;
+}
+
 // Steps 3-5 of the driver (ERI_toroidal.f90:200-401): Schwarz values from
 // the diagonal blocks, the primitive Schwarz table, then the unique shell
 // quartets on the GPU (c->d_q4[0..c->nq4)).  Shared by the stored build and
 // the direct J/K (eri_direct.f90:76-225 does the same set-up).
-static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
+static int setup_quartets(tcg_ctx *c, double &t2, double &t3, bool defer = false) {
     // ---- 3-4: Schwarz from the diagonal blocks (ERI_toroidal.f90:195-238),
     // replicated over translations, and the primitive Schwarz table (:300-338):
     // both lists of diagonal quartets in one pass through the pipeline; the
@@ -945,26 +1026,11 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3) {
     t2 = now_ms();
     c->mark("schwarz tables");
 
-    // ---- 5: unique shell quartets (translation x 8 permutations), Schwarz screened, on the GPU;
-    // appended into the capacity of an earlier enumeration, counted and redone only when it overflows ----
-    unsigned long long nq4 = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-        const unsigned long long cap = c->d_q4.p ? c->d_q4.n : 0;
-        HIPCHK(c, hipMemsetAsync(c->d_cnt.p + 3, 0, sizeof(unsigned long long), c->st));
-        HIPCHK(c, launch_enum_quartets(c->d_sh.p, c->d_relc.p, c->d_Qsh.p, c->d_qmax.p, nshuc, nsh, nc, c->d_cnt.p + 3,
-                                       cap ? c->d_q4.p : nullptr, cap, c->st));
-        HIPCHK(c, hipMemcpyAsync(&nq4, c->d_cnt.p + 3, sizeof nq4, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        if (nq4 > (unsigned long long)0x7fffffff) return c->fail(TCG_E_NOMEM, "too many shell quartets (%llu)", nq4);
-        if (nq4 <= cap) break;
-        HIPCHK(c, c->d_q4.alloc(nq4));
-    }
-    c->stats[0] = (long long)nq4;
-    c->nq4 = (int)nq4;
+    const int rc2 = enumerate_quartets(c, defer);
+    if (rc2) return rc2;
     t3 = now_ms();
     c->mark("quartet list");
     return 0;
-
 }
 
 // V(i, j), i in the unit cell, j anywhere, on the GPU (other rows zero);
@@ -1153,15 +1219,15 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     const double t1 = now_ms();
 
     double t2 = 0.0, t3 = 0.0;
-    rc = setup_quartets(c, t2, t3);
+    rc = setup_quartets(c, t2, t3, !c->learned_on);
     if (rc) return rc;
-    const unsigned long long nq4 = (unsigned long long)c->nq4;
+    const int nq4 = c->nq4;                     // -1: deferred (the count is on the device)
 
     // ---- 6: quartets on the GPU, scattered into the ERI array ----
     HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_fill, 0));
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
-    A.nq4 = (int)nq4;
+    A.nq4 = nq4;
     A.screen_prim = 1;
     A.eri = c->d_eri.p;
     A.n1 = n1;
@@ -1175,7 +1241,22 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
     c->time_rows = true;
-    rc = c->learned_on ? learned_pass(c, A) : run_quartets(c, A);
+    if (c->learned_on) {
+        rc = learned_pass(c, A);
+    } else {
+        if (c->nq4 < 0) {                       // deferred enumeration: its count comes with the survivors'
+            A.nq4 = (int)c->d_q4.n;
+            A.nq4_dev = c->d_cnt.p + 3;
+        }
+        rc = run_quartets(c, A);
+        if (rc == kRecount) {                   // the list outgrew its capacity: count, enlarge, redo
+            if ((rc = enumerate_quartets(c, false))) return rc;
+            A.q4 = c->d_q4.p;                   // (enlarged: a new buffer)
+            A.nq4 = c->nq4;
+            A.nq4_dev = nullptr;
+            rc = run_quartets(c, A);
+        }
+    }
     c->time_rows = false;
     if (rc) return rc;
     c->mark("quartets launched");
@@ -1327,6 +1408,10 @@ int tcg_direct_jk(tcg_ctx *c, int nbas, const double *P, double *J, double *K) {
     HIPCHK(c, launch_fill(c->d_J.p, njk, 0.0, c->st));
     HIPCHK(c, launch_fill(c->d_K.p, njk, 0.0, c->st));
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
+    if (c->nq4 < 0) {                           // a deferred enumeration left its count on the device
+        const int rc0 = enumerate_quartets(c, false);
+        if (rc0) return rc0;
+    }
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
     A.nq4 = c->nq4;

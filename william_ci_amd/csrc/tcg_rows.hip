@@ -82,7 +82,16 @@ __global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__
                                              long long *__restrict__ rcount, const long long *__restrict__ soff,
                                              Surv *__restrict__ surv) {
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);     // four wavefronts, one quartet each
+    // a list whose length is on the device: the entries past it count zero
+    const int nlive = A.nq4_dev ? (int)min((unsigned long long)A.nq4, *A.nq4_dev) : A.nq4;
     for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
+        if (qi >= nlive) {
+            if (!pass && lane == 0) {
+                scount[qi] = 0;
+                rcount[qi] = 0;
+            }
+            continue;
+        }
         const Quartet Q = A.q4[qi];
         const ShellPair bp = A.sp[Q.bra], kp = A.sp[Q.ket];
         const long ncand = (Q.prim >= 0) ? 1 : (long)bp.count * kp.count;
@@ -233,12 +242,13 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
         const long long grp = ngroups - 1 - gi;
         const long long tile = grp * 4 + wv;
         const long long pos = tile * 16 + r;
-        const bool valid = pos < nrows;
-        int ridx = -1, nm = -1, d = 0x7fffffff;         // d: direction | rule << 8
+        // (a list sized by an upper bound ends in padding entries with index -1)
+        int ridx = pos < nrows ? order[pos] : -1;
+        const bool valid = ridx >= 0;
+        int nm = -1, d = 0x7fffffff;                    // d: direction | rule << 8
         double sgn = 1.0, c1 = 1.0, s1 = 0.0, c4 = 1.0, s4 = 0.0, c64 = 1.0, s64 = 0.0;
         const double2 *Gp = A.vec, *Hp = A.vec;
         if (valid) {
-            ridx = order[pos];
             const RowRec R = rows[ridx];
             nm = R.nm;
             d = R.d;
