@@ -343,46 +343,74 @@ hipError_t launch_schwarz_tables(const Quartet *q4, int nquart, int nshell, cons
 // screens, and (SI,SJ,SK,SL) the smallest member of its orbit under the 8
 // index permutations x the lattice translations.  One thread per candidate
 // (SI, SJ, SK, SL); pass 0 counts (one atomic per wavefront), pass 1 appends.
+// One thread per (SI, SJ, SK) and run of kSLRun consecutive SL: the unique
+// quartets (SI in the unit cell, every member's local index >= SI, Schwarz
+// Qsh(SI,SJ) Qsh(SK,SL) >= 1e-13, the smallest member of its orbit under the 8
+// permutations with the first index translated to the unit cell), written in
+// no particular order.  Positions: a wavefront-wide scan of the lanes' counts,
+// then per workgroup the wavefronts' totals meet in LDS, one atomic per
+// workgroup.
+constexpr int kSLRun = 4;
+
 __global__ __launch_bounds__(1024) void k_enum_quartets(const Shell *__restrict__ sh, const int *__restrict__ relc,
                                 const double *__restrict__ Qsh, const unsigned long long *__restrict__ qmax_bits,
                                 int nshuc, int nsh, int ncell,
                                 unsigned long long *__restrict__ count, Quartet *__restrict__ out,
                                 unsigned long long cap) {
+    const int nrun = (nsh + kSLRun - 1) / kSLRun;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool keep = t < (long long)nshuc * nsh * nsh * nsh;
-    int SI = 0, SJ = 0, SK = 0, SL = 0;
-    if (keep) {
-        SL = (int)(t % nsh);
-        SK = (int)((t / nsh) % nsh);
-        SJ = (int)((t / ((long long)nsh * nsh)) % nsh);
-        SI = (int)(t / ((long long)nsh * nsh * nsh));
-        keep = sh[SJ].loc >= SI && sh[SK].loc >= SI && sh[SL].loc >= SI;
+    const bool live = t < (long long)nshuc * nsh * nsh * nrun;
+    int SI = 0, SJ = 0, SK = 0, SL0 = 0;
+    bool pre = live;
+    if (pre) {
+        SL0 = (int)(t % nrun) * kSLRun;
+        SK = (int)((t / nrun) % nsh);
+        SJ = (int)((t / ((long long)nrun * nsh)) % nsh);
+        SI = (int)(t / ((long long)nrun * nsh * nsh));
+        pre = sh[SJ].loc >= SI && sh[SK].loc >= SI;
     }
-    if (keep) {
+    double qij = 0.0;
+    if (pre) {
         const double Qmax = __longlong_as_double((long long)*qmax_bits);
-        const double qij = Qsh[(size_t)SI + (size_t)nsh * SJ];
-        keep = qij * Qmax >= kSchwarz && qij * Qsh[(size_t)SK + (size_t)nsh * SL] >= kSchwarz;
+        qij = Qsh[(size_t)SI + (size_t)nsh * SJ];
+        pre = qij * Qmax >= kSchwarz;
     }
-    if (keep) {
-        const int q[4] = {SI, SJ, SK, SL};
-        const int qc[4] = {0, sh[SJ].cell, sh[SK].cell, sh[SL].cell};
-        const int ql[4] = {SI, sh[SJ].loc, sh[SK].loc, sh[SL].loc};
-        for (int ip = 1; ip < 8 && keep; ++ip) {
-            const int c0 = qc[c_perm8_setup[ip][0]];
-            for (int x = 0; x < 4; ++x) {
-                const int y = c_perm8_setup[ip][x];
-                const int rr = relc[(size_t)c0 * ncell + qc[y]] * nshuc + ql[y];
-                if (rr < q[x]) { keep = false; break; }
-                if (rr > q[x]) break;
+    unsigned keepm = 0;
+    if (pre) {
+        const int cJ = sh[SJ].cell, cK = sh[SK].cell;
+        const int lJ = sh[SJ].loc, lK = sh[SK].loc;
+        for (int u = 0; u < kSLRun; ++u) {
+            const int SL = SL0 + u;
+            if (SL >= nsh) break;
+            if (sh[SL].loc < SI || qij * Qsh[(size_t)SK + (size_t)nsh * SL] < kSchwarz) continue;
+            const int q[4] = {SI, SJ, SK, SL};
+            const int qc[4] = {0, cJ, cK, sh[SL].cell};
+            const int ql[4] = {SI, lJ, lK, sh[SL].loc};
+            bool keep = true;
+            for (int ip = 1; ip < 8 && keep; ++ip) {
+                const int c0 = qc[c_perm8_setup[ip][0]];
+                for (int x = 0; x < 4; ++x) {
+                    const int y = c_perm8_setup[ip][x];
+                    const int rr = relc[(size_t)c0 * ncell + qc[y]] * nshuc + ql[y];
+                    if (rr < q[x]) { keep = false; break; }
+                    if (rr > q[x]) break;
+                }
             }
+            if (keep) keepm |= 1u << u;
         }
     }
-    // positions: per workgroup the wavefronts' counts meet in LDS, one atomic per workgroup
+    // this lane's count, its offset inside the wavefront (inclusive scan), the wavefront total
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int mine = __popc(keepm);
+    int incl = mine;
+#pragma unroll
+    for (int dl = 1; dl < 64; dl <<= 1) {
+        const int v = __shfl_up(incl, dl, 64);
+        if (lane >= dl) incl += v;
+    }
     __shared__ unsigned wcnt[16];
     __shared__ unsigned long long wbase[16];
-    const unsigned long long m = __ballot(keep);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    if (lane == 0) wcnt[wv] = (unsigned)__popcll(m);
+    if (lane == 63) wcnt[wv] = (unsigned)incl;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long tot = 0;
@@ -394,25 +422,29 @@ __global__ __launch_bounds__(1024) void k_enum_quartets(const Shell *__restrict_
         for (int w = 0; w < nwv; ++w) wbase[w] += b0;
     }
     __syncthreads();
-    if (!keep || !out) return;
-    const unsigned long long pos = wbase[wv] + __popcll(m & ((1ull << lane) - 1ull));
-    if (pos >= cap) return;
+    if (!keepm || !out) return;
+    unsigned long long pos = wbase[wv] + (unsigned)(incl - mine);
     const int lK = sh[SK].loc;
-    Quartet Q;
-    Q.s[0] = SI; Q.s[1] = SJ; Q.s[2] = SK; Q.s[3] = SL;
-    Q.bra = SI * nsh + SJ;
-    Q.ket = lK * nsh + relc[(size_t)sh[SK].cell * ncell + sh[SL].cell] * nshuc + sh[SL].loc;
-    for (int d = 0; d < 3; ++d) Q.shift[d] = sh[SK].xyz[d] - sh[lK].xyz[d];
-    Q.out = -1;
-    Q.prim = -1;
-    out[pos] = Q;
+    for (int u = 0; u < kSLRun; ++u) {
+        if (!(keepm & (1u << u))) continue;
+        if (pos >= cap) return;
+        const int SL = SL0 + u;
+        Quartet Q;
+        Q.s[0] = SI; Q.s[1] = SJ; Q.s[2] = SK; Q.s[3] = SL;
+        Q.bra = SI * nsh + SJ;
+        Q.ket = lK * nsh + relc[(size_t)sh[SK].cell * ncell + sh[SL].cell] * nshuc + sh[SL].loc;
+        for (int d = 0; d < 3; ++d) Q.shift[d] = sh[SK].xyz[d] - sh[lK].xyz[d];
+        Q.out = -1;
+        Q.prim = -1;
+        out[pos++] = Q;
+    }
 }
 
 hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *Qsh, const unsigned long long *qmax_bits,
                                 int nshuc,
                                 int nsh, int ncell, unsigned long long *count, Quartet *out,
                                 unsigned long long cap, hipStream_t st) {
-    const long long n = (long long)nshuc * nsh * nsh * nsh;
+    const long long n = (long long)nshuc * nsh * nsh * ((nsh + kSLRun - 1) / kSLRun);
     const long long blocks = (n + 1023) / 1024;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_enum_quartets, dim3((unsigned)blocks), dim3(1024), 0, st, sh, relc, Qsh, qmax_bits, nshuc, nsh,
