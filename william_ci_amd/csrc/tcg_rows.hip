@@ -223,8 +223,10 @@ hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long 
 // advances by a rotation with a second rotation per 64 orders (no sincos).
 template <int kNT>
 __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__restrict__ rows,
-                                                  const int *__restrict__ order, long long nrows, int hbase,
+                                                  const int *__restrict__ order, long long nrows, int hbase0,
                                                   double *__restrict__ rowval) {
+    // node tiles hbase .. hbase + kNT - 1 (blockIdx.y: one group of kNT per grid row)
+    const int hbase = hbase0 + kNT * (int)blockIdx.y;
     constexpr int S = 16;                            // orders per LDS slice
     constexpr int TW = kNT * kNodeTile;              // nodes of this launch
     constexpr int W = TW + 4;                        // LDS row stride (padded)
@@ -447,7 +449,14 @@ hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *ord
     // slower: the eighths differ several-fold in cost, the orders growing along the sort)
     const int grid = nwg < 2048 ? (int)nwg : 2048;
     const int nht = a.nqp / kNodeTile;
-    // up to four node tiles per launch (more nodes: one launch per group of four)
+    // short row lists (the Schwarz passes: few, long rows) give each node tile a grid row
+    // of its own (four times the wavefronts, one MFMA per step each); long lists take up
+    // to four node tiles per wavefront, one launch per group of four
+    if (ntiles < 4 * 2048) {
+        const int g1 = (int)nwg;
+        hipLaunchKernelGGL(k_rows_mfma<1>, dim3(g1, nht), dim3(256), 0, st, a, rows, order, nrows, 0, rowval);
+        return hipGetLastError();
+    }
     for (int h0 = 0; h0 < nht; h0 += 4) {
         switch (nht - h0 < 4 ? nht - h0 : 4) {
         case 1: hipLaunchKernelGGL(k_rows_mfma<1>, dim3(grid), dim3(256), 0, st, a, rows, order, nrows, h0, rowval); break;
