@@ -125,6 +125,52 @@ int tcg_mp2_gamma(tcg_ctx *ctx, int nbas, int nocc, const double *e, const doubl
  * out for |D| < 1e-8, out[4] = GPU wall time in ms. */
 int tcg_kmp2(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out);
 
+/* k-space (Bloch, "SAO") matrices.  M_sao is the (nBas,nBas) complex matrix
+ * of the reference (re, im pairs, column-major), block diagonal in k: block k
+ * (cells and k points in one order, x fastest) holds rows and columns
+ * k nfuc .. k nfuc + nfuc - 1.
+ *
+ * tcg_ao_to_sao replaces transform_AO_to_SAO(nBas,AO,SAO)  src/Tools/SAO_Transfer.f90:1-105:
+ *   M_sao(k nf + a, k nf + b) = sum_R exp(+i k.R) M_ao(a, b + R nf)
+ * (the unit-cell rows of M_ao; all other elements of M_sao are set to zero).
+ * tcg_sao_to_ao replaces transform_SAO_to_AO(nBas,SAO,AO)  :108-233:
+ *   M_ao(a + R nf, b + R' nf) = Re (1/N) sum_k exp(+i k.(R-R')) M_sao(k nf + a, k nf + b),
+ * block elements of modulus below 1e-15 left out (only the diagonal blocks are read).
+ * tcg_heev_blocks replaces diagonalize_block_diagonal_complex_para(nBas,nfuc,Nc,A,e)
+ *   src/Tools/general_tools.f90:309-339 (clean_and_symmetrize + zheev 'V','U'):
+ * every diagonal block is replaced by its eigenvectors (columns, ascending
+ * eigenvalues), e(nBas) block by block; elements outside the blocks are not
+ * touched.  Eigenvectors are fixed up to a phase per column.  nfuc <= 56. */
+int tcg_ao_to_sao(tcg_ctx *ctx, int nbas, const double *M_ao, double *M_sao);
+int tcg_sao_to_ao(tcg_ctx *ctx, int nbas, const double *M_sao, double *M_ao);
+int tcg_heev_blocks(tcg_ctx *ctx, int nbas, double *A_sao, double *e);
+
+/* The whole k-space RHF iteration on the GPU.  Replaces
+ *   RHF_SAO(nBas,nO,S,T,V,Hc,ERI,X,ENuc,EHF,e,coeff_SAO,blk,natom,Zat)
+ *                                    src/Energies/SAO/RHF_SAO.f90:1-553
+ * with guess_AO_RHF_SAO (core guess, level at the cut filled evenly,
+ * src/Energies/SAO/guess_Huckel_SAO.f90:141-192), X_k per block
+ * (get_X_from_overlap_complex_blocks, canonical below 1e-6), DIIS on the
+ * k-space error X^H (FPS - SPF) X (history 8, reset below rcond 1e-14) and the
+ * reference's convergence and stall rules.  Every matrix stays on the GPU;
+ * one iteration reads back a few scalars.
+ *   source    0: J/K from the resident SAO ERI (tcg_eri_build, either order);
+ *             1: integral-direct (tcg_direct_init first)
+ *   nocc      nO, doubly occupied orbitals over the supercell
+ *   S, T, V   (nBas,nBas) one-electron matrices
+ *   max_scf, thresh, diis   maxSCF, the orbital-error threshold, keyword DIIS
+ * Out: e(nBas) orbital energies block by block, ascending in each block;
+ * coeff_sao (nBas,nBas) complex (re, im pairs) or NULL; occ(nBas) 2 / 0 or
+ * NULL; P(nBas,nBas) the last density or NULL;
+ *   out[0] E_HF without E_nuc   out[1..4] ET, EV, EJ, EK   out[5] iterations
+ *   out[6] orbital error        out[7] converged           out[8] stalled
+ *   out[9] HOMO-LUMO gap        out[10] J/K time (ms, GPU) out[11] wall time (ms)
+ * Occupations take the nO lowest levels over all k (ties by block, then band).
+ * nfuc <= 56. */
+int tcg_rhf_sao(tcg_ctx *ctx, int source, int nbas, int nocc, const double *S, const double *T, const double *V,
+                int max_scf, double thresh, int diis, double *e, double *coeff_sao, int *occ, double *P,
+                double *out);
+
 /* Density fitting (keyword density_fitting, SAO path).  A DF context holds
  * the fitted three-index tensor df_B(nfuc, nBas, naux) of one torus system
  * on the GPU.

@@ -97,6 +97,41 @@ module tcg_engine
       integer(c_int), intent(in) :: occ(*)
       real(c_double), intent(out) :: out(5)
     end function
+    ! transform_AO_to_SAO / transform_SAO_to_AO
+    integer(c_int) function tcg_ao_to_sao(ctx, nbas, M_ao, M_sao) bind(C, name="tcg_ao_to_sao")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbas
+      real(c_double), intent(in) :: M_ao(*)
+      complex(c_double_complex), intent(out) :: M_sao(*)
+    end function
+    integer(c_int) function tcg_sao_to_ao(ctx, nbas, M_sao, M_ao) bind(C, name="tcg_sao_to_ao")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbas
+      complex(c_double_complex), intent(in) :: M_sao(*)
+      real(c_double), intent(out) :: M_ao(*)
+    end function
+    ! diagonalize_block_diagonal_complex_para
+    integer(c_int) function tcg_heev_blocks(ctx, nbas, A_sao, e) bind(C, name="tcg_heev_blocks")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbas
+      complex(c_double_complex), intent(inout) :: A_sao(*)
+      real(c_double), intent(out) :: e(*)
+    end function
+    ! RHF_SAO (the whole iteration on the GPU)
+    integer(c_int) function tcg_rhf_sao(ctx, source, nbas, nocc, S, T, V, max_scf, thresh, diis, e, &
+        coeff_sao, occ, P, out) bind(C, name="tcg_rhf_sao")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: source, nbas, nocc, max_scf, diis
+      real(c_double), value :: thresh
+      real(c_double), intent(in) :: S(*), T(*), V(*)
+      real(c_double), intent(out) :: e(*), P(*), out(12)
+      complex(c_double_complex), intent(out) :: coeff_sao(*)
+      integer(c_int), intent(out) :: occ(*)
+    end function
     ! df_init, df_JK, df_kMO + kMP2_calc
     integer(c_int) function tcg_df_create(ctx, nau_cell, aux_center, aux_lxyz, aux_prim_off, aux_expo, &
         aux_coef, df) bind(C, name="tcg_df_create")

@@ -62,8 +62,21 @@ def lib():
         L.ref_mp2.argtypes = [_ci, _ci, _dp, _dp, _dp, ctypes.c_char_p]
         L.ref_kmp2.argtypes = [_ci, np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS"), _dp, _ip,
                                _dp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
+        _cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+        L.ref_ao_to_sao.argtypes = [_ci, _dp, _cp]
+        L.ref_sao_to_ao.argtypes = [_ci, _cp, _dp]
+        L.ref_clean_and_symmetrize.argtypes = [_ci, _cp]
         _lib = L
     return _lib
+
+
+def clean_and_symmetrize(A: np.ndarray) -> np.ndarray:
+    """The reference's clean_and_symmetrize (src/Tools/SAO_Transfer.f90:235) on a
+    copy of the complex square matrix A (row-major numpy)."""
+    n = A.shape[0]
+    buf = np.ascontiguousarray(np.asarray(A, dtype=np.complex128).T).ravel()
+    lib().ref_clean_and_symmetrize(n, buf)
+    return buf.reshape(n, n).T.copy()
 
 
 class RefSystem:
@@ -190,6 +203,20 @@ class RefSystem:
                       r"\s*nuclear attraction \(sample of\s*(\d+)\)\s*([-0-9.E+]+)", text)
         return {"n_quartets": int(m.group(1)), "eri_max_diff": float(m.group(2)), "n_v": int(m.group(3)),
                 "v_max_diff": float(m.group(4)), "fast_quad": bool(out[0]), "nquad": int(out[1]), "text": text}
+
+    def ao_to_sao(self, M):
+        """transform_AO_to_SAO on the real (nBas, nBas) matrix M (row-major numpy)."""
+        nb = self.s.nbas
+        out = np.zeros(nb * nb, dtype=np.complex128)
+        lib().ref_ao_to_sao(nb, np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel(), out)
+        return out.reshape(nb, nb).T.copy()
+
+    def sao_to_ao(self, M):
+        """transform_SAO_to_AO on the complex (nBas, nBas) matrix M (row-major numpy)."""
+        nb = self.s.nbas
+        out = np.zeros(nb * nb)
+        lib().ref_sao_to_ao(nb, np.ascontiguousarray(np.asarray(M, dtype=np.complex128).T).ravel(), out)
+        return out.reshape(nb, nb).T.copy()
 
     def nre(self) -> float:
         geom = np.asfortranarray(self.s.geom).ravel(order="F").copy()

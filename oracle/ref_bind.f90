@@ -21,6 +21,9 @@
 !                              src/Energies/MP2/MP2.f90:1              !
 !   transform_ERI_AO_to_kMO,   src/Energies/SAO/MP2/AO_to_MO_SAO.f90:34 !
 !   kMP2_calc                  src/Energies/SAO/MP2/MP2_SAO.f90:6      !
+!   transform_AO_to_SAO,       src/Tools/SAO_Transfer.f90:1,108,235    !
+!   transform_SAO_to_AO,                                               !
+!   clean_and_symmetrize                                               !
 !---------------------------------------------------------------------!
 module ref_bind
 
@@ -294,6 +297,49 @@ contains
         call kMP2_calc(N_cell, blk, 1, o_hi, omin+1, nfuc, ekmo, 0.d0, 0.d0, ec)
         close(outfile)
       end subroutine ref_kmp2
+
+      ! transform_AO_to_SAO(nBas, AO, SAO) on the unit cell set by ref_init
+      subroutine ref_ao_to_sao(nbas, ao, sao) bind(C, name="ref_ao_to_sao")
+        integer(c_int), value          :: nbas
+        real(c_double), intent(in)     :: ao(nbas,nbas)
+        complex(c_double_complex), intent(out) :: sao(nbas,nbas)
+        interface
+          subroutine transform_AO_to_SAO(nBas, AO_matrix, SAO_matrix)
+            integer, intent(in) :: nBas
+            real(8), intent(in) :: AO_matrix(nBas,nBas)
+            complex(8), intent(out) :: SAO_matrix(nBas,nBas)
+          end subroutine
+        end interface
+        call transform_AO_to_SAO(nbas, ao, sao)
+      end subroutine ref_ao_to_sao
+
+      ! transform_SAO_to_AO(nBas, SAO, AO)
+      subroutine ref_sao_to_ao(nbas, sao, ao) bind(C, name="ref_sao_to_ao")
+        integer(c_int), value          :: nbas
+        complex(c_double_complex), intent(in) :: sao(nbas,nbas)
+        real(c_double), intent(out)    :: ao(nbas,nbas)
+        interface
+          subroutine transform_SAO_to_AO(nBas, M_SAO, M_AO)
+            integer, intent(in) :: nBas
+            complex(8), intent(in) :: M_SAO(nBas,nBas)
+            real(8), intent(out) :: M_AO(nBas,nBas)
+          end subroutine
+        end interface
+        call transform_SAO_to_AO(nbas, sao, ao)
+      end subroutine ref_sao_to_ao
+
+      ! clean_and_symmetrize(N, A) in place
+      subroutine ref_clean_and_symmetrize(n, a) bind(C, name="ref_clean_and_symmetrize")
+        integer(c_int), value          :: n
+        complex(c_double_complex), intent(inout) :: a(n,n)
+        interface
+          subroutine clean_and_symmetrize(N, A)
+            integer, intent(in) :: N
+            complex(8), intent(inout) :: A(N,N)
+          end subroutine
+        end interface
+        call clean_and_symmetrize(n, a)
+      end subroutine ref_clean_and_symmetrize
 
       ! direct J/K on the density P(nBas,nBas): direct_init then direct_JK
       subroutine ref_direct_jk(P, J, K) bind(C, name="ref_direct_jk")

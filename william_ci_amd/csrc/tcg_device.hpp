@@ -149,4 +149,48 @@ size_t jk_work_doubles(int n1, int nbas);
 hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,
                             double *jwork, hipStream_t st);
 
+// ---- k-space RHF algebra (tcg_scf.hip) ----
+constexpr int kMaxBlock = 56;     // largest k block (nfuc): three n x n complex LDS buffers
+constexpr int kMaxDiis = 8;       // DIIS history (src/module/HF_constant.f90 max_diis)
+// unit-cell rows of up to four real matrices, element (a, c) of source s at
+// p[s][a sa[s] + c sc[s]], summed with factors f[s]
+struct RowSources {
+    const double *p[4];
+    long long sa[4], sc[4];
+    double f[4];
+    int n;
+};
+struct DiisSlots {
+    int slot[kMaxDiis];
+    double w[kMaxDiis];
+    int m;
+};
+hipError_t launch_lattice_fwd(const RowSources &src, const double2 *ph, int N, int nf, double2 *out, hipStream_t st);
+hipError_t launch_lattice_inv(const double2 *Mk, const double2 *ph, int N, int nf, int skip, double *C,
+                              double *maxabs, int *nmax, hipStream_t st);
+hipError_t launch_circulant(const double *C, int nf, const int ncl[3], double *M, hipStream_t st);
+hipError_t launch_transpose(const double *A, int n, double *B, hipStream_t st);
+hipError_t launch_rows_sum(const double *A, const double *B, int nf, int nb, double *H, hipStream_t st);
+hipError_t launch_overlap_x(const double2 *Sk, int N, int n, double2 *Xk, int *dead, double *smin, hipStream_t st);
+hipError_t launch_fock_diag(const double2 *Fk, const double2 *Xk, const int *dead, int N, int n, double *e,
+                            double2 *Ck, hipStream_t st);
+hipError_t launch_scf_error(const double2 *Fk, const double2 *Pk, const double2 *Sk, const double2 *Xk, int N, int n,
+                            double2 *err, hipStream_t st);
+hipError_t launch_density_k(const double2 *Ck, const double *w, int N, int n, double2 *Pk, hipStream_t st);
+hipError_t launch_occupy(const double *e, int ne, int nocc, int frac, double *w, int *occ, double *sc,
+                         hipStream_t st);
+int energy_parts_blocks(int nb);
+hipError_t launch_energy(const double *P, const double *Tt, const double *Vt, const double *J, const double *K,
+                         int ldjk, int nf, int nb, double *part, const double *maxes, int nmax, double ncell,
+                         double *sc, hipStream_t st);
+hipError_t launch_diis_dots(const double2 *enew, const double2 *hist, long long len, const DiisSlots &sl, double *out,
+                            hipStream_t st);
+hipError_t launch_diis_combine(const double2 *hist, long long len, const DiisSlots &sl, double2 *out, hipStream_t st);
+// J and K rows of the unit cell for a density already on the device (the GPU
+// SCF loop): source 0 = the resident ERI tensor (leading dimension nbas, rows
+// < n1 written), 1 = integral-direct (leading dimension nfuc).  dJ, dK hold
+// nbas^2 doubles.
+int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *dK, int *ld);
+hipStream_t ctx_stream(const tcg_ctx *c);
+
 }  // namespace tcg

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/tcg.h"
+#include "tcg_devbuf.hpp"
 #include "tcg_device.hpp"
 
 using namespace tcg;
@@ -45,34 +46,6 @@ hipError_t launch_df_kt(const double *Fr, const double *Fi, const double2 *Pk, i
 
 namespace {
 
-template <class T>
-struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    // grows the buffer; the old one is freed only after the device has drained
-    hipError_t alloc(size_t c) {
-        if (c <= n && p) return hipSuccess;
-        if (p) {
-            const hipError_t e = hipDeviceSynchronize();
-            if (e != hipSuccess) return e;
-        }
-        release();
-        n = c;
-        return hipMalloc((void **)&p, sizeof(T) * (c ? c : 1));
-    }
-    hipError_t put(const std::vector<T> &v) {
-        hipError_t e = alloc(v.size());
-        if (e == hipSuccess && !v.empty()) e = hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
-        return e;
-    }
-};
-
 constexpr double kMetricTol = 1e-10;     // pivoted Cholesky of (P|Q)(k), Gaussian aux basis (df_toroidal.f90:68)
 constexpr double kDfSchwarz = 1e-13;     // df_schwarz (df_toroidal.f90:70)
 
@@ -82,12 +55,12 @@ struct tcg_df {
     tcg_ctx *comb = nullptr;             // engine context on the combined basis
     int ndp = 1, nf = 0, nb = 0, nau = 0, N = 1, nfc = 0, ncl[3] = {1, 1, 1};
     std::vector<int> cdiff, cadd;
-    DBuf<int> d_cdiff, d_cadd;
-    DBuf<double> d_B;                    // df_B(nfuc, nBas, naux), column-major
+    DevBuf<int> d_cdiff, d_cadd;
+    DevBuf<double> d_B;                    // df_B(nfuc, nBas, naux), column-major
     // F(a, b0, k, Q, kap) = sum_{Rb, RQ} df_B(a, b0 + Rb, Q RQ) e^{ik Rb} e^{-i kap RQ} (real, imaginary)
-    DBuf<double> d_Fr, d_Fi;
+    DevBuf<double> d_Fr, d_Fi;
     bool f_ready = false;
-    DBuf<double2> d_Pk, d_Y, d_Kt;       // exchange work arrays (kept between SCF iterations)
+    DevBuf<double2> d_Pk, d_Y, d_Kt;       // exchange work arrays (kept between SCF iterations)
     std::vector<int> rank;
     double dmin = 0.0;
     double times[4] = {0, 0, 0, 0};
@@ -143,9 +116,9 @@ static int build_F(tcg_df *df) {
     const int nf = df->nf, N = df->N, nau = df->nau, naux = nau * N;
     std::vector<double> cphi, sphi;
     phase_tables(df->ncl, N, 1.0, cphi, sphi);
-    DBuf<double> dC, dS, Y1r, Y1i;
-    DCHK(df, dC.put(cphi));
-    DCHK(df, dS.put(sphi));
+    DevBuf<double> dC, dS, Y1r, Y1i;
+    DCHK(df, dC.upload(cphi));
+    DCHK(df, dS.upload(sphi));
     // df_B viewed (a, b0, Rb, Q, RQ): first Rb -> k, batched over (Q, RQ)
     const size_t nff = (size_t)nf * nf, nY = nff * N * naux;
     DCHK(df, Y1r.alloc(nY));
@@ -235,8 +208,8 @@ int tcg_df_create(tcg_ctx *ctx, int nau_cell, const double *aux_center, const in
             df->cdiff[(size_t)a * N + b] = idx(ax - bx, ay - by, az - bz);
             df->cadd[(size_t)a * N + b] = idx(ax + bx, ay + by, az + bz);
         }
-    DCHK(df, df->d_cdiff.put(df->cdiff));
-    DCHK(df, df->d_cadd.put(df->cadd));
+    DCHK(df, df->d_cdiff.upload(df->cdiff));
+    DCHK(df, df->d_cadd.upload(df->cadd));
     // shells of the combined basis: kind by position inside the cell
     std::vector<int> sfirst, sncomp, scell;
     if (ctx_shells(df->comb, sfirst, sncomp, scell)) return df->fail(TCG_E_STATE, "combined shells");
@@ -293,14 +266,14 @@ int tcg_df_create(tcg_ctx *ctx, int nau_cell, const double *aux_center, const in
     // ---- V(k) and its factor f(k) on the GPU ----
     std::vector<double> cphi, sphi;
     phase_tables(df->ncl, N, 1.0, cphi, sphi);
-    DBuf<double> dC, dS, dVm, dVr, dVi, dfr, dfi, ddmin;
-    DBuf<int> drank, dmate;
+    DevBuf<double> dC, dS, dVm, dVr, dVi, dfr, dfi, ddmin;
+    DevBuf<int> drank, dmate;
     std::vector<int> mate(N);
     for (int k = 0; k < N; ++k) mate[k] = df->cdiff[k];          // -k (cdiff row 0: 0 - k)
-    DCHK(df, dmate.put(mate));
-    DCHK(df, dC.put(cphi));
-    DCHK(df, dS.put(sphi));
-    DCHK(df, dVm.put(Vm));
+    DCHK(df, dmate.upload(mate));
+    DCHK(df, dC.upload(cphi));
+    DCHK(df, dS.upload(sphi));
+    DCHK(df, dVm.upload(Vm));
     DCHK(df, dVr.alloc((size_t)nau * nau * N));
     DCHK(df, dVi.alloc((size_t)nau * nau * N));
     // Vm as (nau*nau) x N (rows (p, q), columns RQ) times e^{+ik RQ}
@@ -310,7 +283,7 @@ int tcg_df_create(tcg_ctx *ctx, int nau_cell, const double *aux_center, const in
     DCHK(df, dfi.alloc((size_t)nau * nau * N));
     DCHK(df, drank.alloc(N));
     DCHK(df, ddmin.alloc(N));
-    DBuf<double2> dwork;
+    DevBuf<double2> dwork;
     if (nau > 64) DCHK(df, dwork.alloc(2 * (size_t)nau * nau * N));
     DCHK(df, launch_metric_factor(dVr.p, dVi.p, nau, N, dmate.p, kMetricTol, dwork.p, dfr.p, dfi.p, drank.p, ddmin.p,
                                   df->st));
@@ -377,8 +350,8 @@ int tcg_df_create(tcg_ctx *ctx, int nau_cell, const double *aux_center, const in
     const double t2 = now_ms_df();
     // ---- B = T3 V^-1/2 as a convolution over cells ----
     {
-        DBuf<double> dT3, dAr, dAi, dWr, dWi;
-        DCHK(df, dT3.put(T3));
+        DevBuf<double> dT3, dAr, dAi, dWr, dWi;
+        DCHK(df, dT3.upload(T3));
         DCHK(df, dAr.alloc(m1 * N));
         DCHK(df, dAi.alloc(m1 * N));
         // T3 is a (nfuc nBas nau) x N matrix (aux index (p, RP)): Z(k) = sum_RP T3(RP) e^{ik RP}
@@ -437,8 +410,8 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K) {
             for (int l0 = 0; l0 < nf; ++l0)
                 Ps[(size_t)l0 + (size_t)nf * s + nx * R] = P[(size_t)(R * nf + l0) + (size_t)nb * col];
         }
-    DBuf<double> dPs, dG, dgam, dJ;
-    DCHK(df, dPs.put(Ps));
+    DevBuf<double> dPs, dG, dgam, dJ;
+    DCHK(df, dPs.upload(Ps));
     DCHK(df, dG.alloc((size_t)naux * N));
     DCHK(df, launch_df_gamma(df->d_B.p, dPs.p, nf, nb, nau, N, dG.p, df->st));
     std::vector<double> G((size_t)naux * N), gam(naux, 0.0);
@@ -449,7 +422,7 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K) {
         for (int rq = 0; rq < N; ++rq)
             for (int q = 0; q < nau; ++q)
                 gam[(size_t)df->cadd[(size_t)rq * N + R] * nau + q] += G[(size_t)rq * nau + q + (size_t)naux * R];
-    DCHK(df, dgam.put(gam));
+    DCHK(df, dgam.upload(gam));
     DCHK(df, dJ.alloc(nx));
     DCHK(df, launch_df_jvec(df->d_B.p, dgam.p, nf, nb, naux, dJ.p, df->st));
     // K in k space (df_toroidal.f90:893-954): P(k) = sum_R P(0, R) e^{ikR} (its Hermitian part),
@@ -479,7 +452,7 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K) {
                 pk[b + nf * a] = double2{pk[a + nf * b].x, -pk[a + nf * b].y};
             }
     }
-    DCHK(df, df->d_Pk.put(Pk));
+    DCHK(df, df->d_Pk.upload(Pk));
     DCHK(df, df->d_Y.alloc(nff * nau * N * N));
     DCHK(df, df->d_Kt.alloc(nff * N));
     DCHK(df, launch_df_kt(df->d_Fr.p, df->d_Fi.p, df->d_Pk.p, nf, nau, N, df->d_cdiff.p, df->d_Y.p, df->d_Kt.p, df->st));
@@ -534,8 +507,8 @@ int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, 
     if (o_hi == 0 || nv <= 0) return df->fail(TCG_E_ARG, "tcg_df_kmp2: no occupied or no virtual bands");
     int rc = build_F(df);
     if (rc) return rc;
-    DBuf<double2> dCk, dL, dE;
-    DCHK(df, dCk.put(Ck));
+    DevBuf<double2> dCk, dL, dE;
+    DCHK(df, dCk.upload(Ck));
     const size_t nL = (size_t)o_hi * nv * nau * N * N;
     DCHK(df, dL.alloc(nL));
     DCHK(df, launch_df_l(df->d_Fr.p, df->d_Fi.p, dCk.p, nf, nau, N, o_hi, v_lo, df->d_cdiff.p, dL.p, df->st));
@@ -546,11 +519,11 @@ int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, 
     std::vector<double> ek((size_t)N * nf);
     for (size_t i = 0; i < ek.size(); ++i) ek[i] = e[i];
     std::vector<int> oc(occ, occ + (size_t)N * nf);
-    DBuf<double> de, dpart;
-    DBuf<int> docc;
-    DBuf<unsigned long long> dnz;
-    DCHK(df, de.put(ek));
-    DCHK(df, docc.put(oc));
+    DevBuf<double> de, dpart;
+    DevBuf<int> docc;
+    DevBuf<unsigned long long> dnz;
+    DCHK(df, de.upload(ek));
+    DCHK(df, docc.upload(oc));
     const int nblk = 1024;
     DCHK(df, dpart.alloc(2 * nblk));
     DCHK(df, dnz.alloc(1));

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/tcg.h"
+#include "tcg_devbuf.hpp"
 #include "tcg_device.hpp"
 #include "tcg_engine.hpp"
 #include "tcg_math.hpp"
@@ -37,37 +38,6 @@
 using namespace tcg;
 
 namespace {
-
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    // grows the buffer; the old one is freed only after the device has drained
-    // (work queued earlier, without a synchronisation in between, may still read
-    // it).  A grown buffer has a new address: callers re-read p after alloc.
-    hipError_t alloc(size_t count) {
-        if (count <= n && p) return hipSuccess;
-        if (p) {
-            const hipError_t e = hipDeviceSynchronize();
-            if (e != hipSuccess) return e;
-        }
-        release();
-        n = count;
-        return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1));
-    }
-    hipError_t upload(const std::vector<T> &v) {
-        hipError_t e = alloc(v.size());
-        if (e != hipSuccess) return e;
-        if (!v.empty()) e = hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
-        return e;
-    }
-};
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -1065,6 +1035,76 @@ static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, 
     return 0;
 }
 
+// Column lists of the resident ERI for the sparse J/K: the non-zero columns
+// ERI(:,b,c,d), listed by d and by b (built once per tensor and storage order).
+static int stored_cols(tcg_ctx *c, const double *src, int n1, int nbas, int physicist) {
+    if (c->cols_ready && c->cols_phys == physicist) return TCG_OK;
+    // non-zero columns ERI(:,b,c,d), listed by d and by b (once per tensor)
+    const long long ncol = (long long)nbas * nbas * nbas;
+    DevBuf<unsigned char> flags;
+    DevBuf<int> cnt;
+    DevBuf<unsigned> keyb, keyb2;
+    DevBuf<long long> nsel, tmp_list;
+    HIPCHK(c, flags.alloc(ncol));
+    HIPCHK(c, cnt.alloc(2 * (size_t)nbas + 2));
+    HIPCHK(c, keyb.alloc(ncol));
+    HIPCHK(c, nsel.alloc(1));
+    HIPCHK(c, hipMemsetAsync(cnt.p, 0, sizeof(int) * (2 * (size_t)nbas + 2), c->st));
+    HIPCHK(c, launch_col_flags(src, n1, nbas, physicist, flags.p, cnt.p, cnt.p + nbas + 1, keyb.p, c->st));
+    size_t tb = std::max(select_temp_bytes(ncol), scan_int_temp_bytes(nbas + 1));
+    HIPCHK(c, c->d_tmp.alloc(tb));
+    HIPCHK(c, tmp_list.alloc(ncol));
+    HIPCHK(c, select_flagged(c->d_tmp.p, tb, flags.p, tmp_list.p, nsel.p, ncol, c->st));
+    long long nz = 0;
+    HIPCHK(c, hipMemcpyAsync(&nz, nsel.p, sizeof nz, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, c->d_off_d.alloc(nbas + 1));
+    HIPCHK(c, c->d_off_b.alloc(nbas + 1));
+    HIPCHK(c, scan_int_exclusive(c->d_tmp.p, tb, cnt.p, c->d_off_d.p, nbas + 1, c->st));
+    HIPCHK(c, scan_int_exclusive(c->d_tmp.p, tb, cnt.p + nbas + 1, c->d_off_b.p, nbas + 1, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, c->d_list_d.alloc(nz > 0 ? nz : 1));
+    HIPCHK(c, c->d_list_b.alloc(nz > 0 ? nz : 1));
+    if (nz > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->d_list_d.p, tmp_list.p, sizeof(long long) * nz, hipMemcpyDeviceToDevice, c->st));
+        // keys b of the selected columns, then a stable sort by b
+        HIPCHK(c, keyb2.alloc(nz));
+        DevBuf<unsigned> kb_sel;
+        HIPCHK(c, kb_sel.alloc(nz));
+        HIPCHK(c, launch_gather_keys(keyb.p, c->d_list_d.p, nz, kb_sel.p, c->st));
+        const int bits = (int)std::ceil(std::log2((double)nbas + 1.0)) + 1;
+        size_t tb2 = sort_ll_temp_bytes(nz, bits);
+        HIPCHK(c, c->d_tmp.alloc(tb2));
+        HIPCHK(c, sort_pairs_ll(c->d_tmp.p, tb2, kb_sel.p, keyb2.p, c->d_list_d.p, c->d_list_b.p, nz, bits, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->cols_ready = true;
+    c->cols_phys = physicist;
+    return TCG_OK;
+}
+
+// J and K rows of the unit cell (nfuc x nbas, leading dimension nfuc) from the
+// integral-direct build for the density dP on the device.
+static int direct_jk_device(tcg_ctx *c, const double *dP, double *dJ, double *dK) {
+    if (!c->direct_ready || !c->prepared) return c->fail(TCG_E_STATE, "direct J/K: tcg_direct_init not called");
+    const size_t njk = (size_t)c->nfuc * c->nbas;
+    HIPCHK(c, launch_fill(dJ, njk, 0.0, c->st));
+    HIPCHK(c, launch_fill(dK, njk, 0.0, c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
+    if (c->nq4 < 0) {                           // a deferred enumeration left its count on the device
+        const int rc0 = enumerate_quartets(c, false);
+        if (rc0) return rc0;
+    }
+    EriArgs A = base_args(c);
+    A.q4 = c->d_q4.p;
+    A.nq4 = c->nq4;
+    A.screen_prim = 1;
+    A.Qao = c->d_Q.p;
+    A.Pden = dP;
+    A.Jacc = dJ;
+    A.Kacc = dK;
+    return run_quartets(c, A);
+}
+
 // Read access for the density-fitting driver (tcg_df.cpp), same library.
 namespace tcg {
 int ctx_view(const tcg_ctx *c, CtxView *v) {
@@ -1102,6 +1142,29 @@ int ctx_resident_eri(const tcg_ctx *c, const double **eri, size_t *count, int *n
 
 void ctx_set_error(tcg_ctx *c, const char *msg) {
     if (c) c->err = msg;
+}
+
+hipStream_t ctx_stream(const tcg_ctx *c) { return c ? c->st : nullptr; }
+
+int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *dK, int *ld) {
+    if (!c || !dP || !dJ || !dK || !ld) return TCG_E_ARG;
+    if (source == 0) {
+        const int nb = c->nbas, n1 = c->eri_n1;
+        if (!c->d_eri.p || n1 < c->nfuc || n1 > nb || c->eri_count != (size_t)n1 * nb * nb * nb)
+            return c->fail(TCG_E_STATE, "stored J/K: no resident ERI of this basis (call tcg_eri_build)");
+        const int physicist = c->eri_physicist ? 1 : 0;
+        const int rc = stored_cols(c, c->d_eri.p, n1, nb, physicist);
+        if (rc) return rc;
+        HIPCHK(c, launch_jk_sparse(c->d_eri.p, dP, c->d_list_d.p, c->d_off_d.p, c->d_list_b.p, c->d_off_b.p, n1, nb,
+                                   physicist, dJ, dK, c->st));
+        *ld = nb;
+        return TCG_OK;
+    }
+    if (source == 1) {
+        *ld = c->nfuc;
+        return direct_jk_device(c, dP, dJ, dK);
+    }
+    return c->fail(TCG_E_ARG, "J/K source must be 0 (stored) or 1 (direct)");
 }
 
 int ctx_shells(tcg_ctx *c, std::vector<int> &first_ao, std::vector<int> &ncomp, std::vector<int> &cell) {
@@ -1323,51 +1386,10 @@ int tcg_jk_stored_order(tcg_ctx *c, int n1, int nbas, const double *P, const dou
     // a caller's array follows the caller's order; the resident one the order it was built in
     const int physicist = eri ? (phys_arg != 0) : (c->eri_physicist ? 1 : 0);
     if (eri) c->eri_physicist = physicist != 0;
-    if (eri && c->cols_phys != physicist) c->cols_ready = false;
-    if (!eri && c->cols_ready && c->cols_phys != physicist) c->cols_ready = false;
     const size_t nn = (size_t)nbas * nbas;
-    if (!c->cols_ready) {
-        // non-zero columns ERI(:,b,c,d), listed by d and by b (once per tensor)
-        const long long ncol = (long long)nbas * nbas * nbas;
-        DevBuf<unsigned char> flags;
-        DevBuf<int> cnt;
-        DevBuf<unsigned> keyb, keyb2;
-        DevBuf<long long> nsel, tmp_list;
-        HIPCHK(c, flags.alloc(ncol));
-        HIPCHK(c, cnt.alloc(2 * (size_t)nbas + 2));
-        HIPCHK(c, keyb.alloc(ncol));
-        HIPCHK(c, nsel.alloc(1));
-        HIPCHK(c, hipMemsetAsync(cnt.p, 0, sizeof(int) * (2 * (size_t)nbas + 2), c->st));
-        HIPCHK(c, launch_col_flags(src, n1, nbas, physicist, flags.p, cnt.p, cnt.p + nbas + 1, keyb.p, c->st));
-        size_t tb = std::max(select_temp_bytes(ncol), scan_int_temp_bytes(nbas + 1));
-        HIPCHK(c, c->d_tmp.alloc(tb));
-        HIPCHK(c, tmp_list.alloc(ncol));
-        HIPCHK(c, select_flagged(c->d_tmp.p, tb, flags.p, tmp_list.p, nsel.p, ncol, c->st));
-        long long nz = 0;
-        HIPCHK(c, hipMemcpyAsync(&nz, nsel.p, sizeof nz, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, c->d_off_d.alloc(nbas + 1));
-        HIPCHK(c, c->d_off_b.alloc(nbas + 1));
-        HIPCHK(c, scan_int_exclusive(c->d_tmp.p, tb, cnt.p, c->d_off_d.p, nbas + 1, c->st));
-        HIPCHK(c, scan_int_exclusive(c->d_tmp.p, tb, cnt.p + nbas + 1, c->d_off_b.p, nbas + 1, c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        HIPCHK(c, c->d_list_d.alloc(nz > 0 ? nz : 1));
-        HIPCHK(c, c->d_list_b.alloc(nz > 0 ? nz : 1));
-        if (nz > 0) {
-            HIPCHK(c, hipMemcpyAsync(c->d_list_d.p, tmp_list.p, sizeof(long long) * nz, hipMemcpyDeviceToDevice, c->st));
-            // keys b of the selected columns, then a stable sort by b
-            std::vector<long long> dummy;
-            HIPCHK(c, keyb2.alloc(nz));
-            DevBuf<unsigned> kb_sel;
-            HIPCHK(c, kb_sel.alloc(nz));
-            HIPCHK(c, launch_gather_keys(keyb.p, c->d_list_d.p, nz, kb_sel.p, c->st));
-            const int bits = (int)std::ceil(std::log2((double)nbas + 1.0)) + 1;
-            size_t tb2 = sort_ll_temp_bytes(nz, bits);
-            HIPCHK(c, c->d_tmp.alloc(tb2));
-            HIPCHK(c, sort_pairs_ll(c->d_tmp.p, tb2, kb_sel.p, keyb2.p, c->d_list_d.p, c->d_list_b.p, nz, bits, c->st));
-        }
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        c->cols_ready = true;
-        c->cols_phys = physicist;
+    {
+        const int rc = stored_cols(c, src, n1, nbas, physicist);
+        if (rc) return rc;
     }
     HIPCHK(c, c->d_P.alloc(nn));
     HIPCHK(c, c->d_J.alloc(nn));
@@ -1405,23 +1427,8 @@ int tcg_direct_jk(tcg_ctx *c, int nbas, const double *P, double *J, double *K) {
     HIPCHK(c, c->d_J.alloc(njk));
     HIPCHK(c, c->d_K.alloc(njk));
     HIPCHK(c, hipMemcpyAsync(c->d_P.p, P, sizeof(double) * nn, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_fill(c->d_J.p, njk, 0.0, c->st));
-    HIPCHK(c, launch_fill(c->d_K.p, njk, 0.0, c->st));
-    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
-    if (c->nq4 < 0) {                           // a deferred enumeration left its count on the device
-        const int rc0 = enumerate_quartets(c, false);
-        if (rc0) return rc0;
-    }
-    EriArgs A = base_args(c);
-    A.q4 = c->d_q4.p;
-    A.nq4 = c->nq4;
-    A.screen_prim = 1;
-    A.Qao = c->d_Q.p;
-    A.Pden = c->d_P.p;
-    A.Jacc = c->d_J.p;
-    A.Kacc = c->d_K.p;
     const double t0 = now_ms();
-    int rc = run_quartets(c, A);
+    int rc = direct_jk_device(c, c->d_P.p, c->d_J.p, c->d_K.p);
     if (rc) return rc;
     std::vector<double> Jr(njk), Kr(njk);
     HIPCHK(c, hipMemcpyAsync(Jr.data(), c->d_J.p, sizeof(double) * njk, hipMemcpyDeviceToHost, c->st));

@@ -48,6 +48,11 @@ def lib():
         L.tcg_jk_stored_noalloc.argtypes = [_vp, _ci, _ci, _dp, _dp, _dp]
         L.tcg_direct_init.argtypes = [_vp]
         L.tcg_direct_jk.argtypes = [_vp, _ci, _dp, _dp, _dp]
+        L.tcg_ao_to_sao.argtypes = [_vp, _ci, _dp, _dp]
+        L.tcg_sao_to_ao.argtypes = [_vp, _ci, _dp, _dp]
+        L.tcg_heev_blocks.argtypes = [_vp, _ci, _dp, _dp]
+        L.tcg_rhf_sao.argtypes = [_vp, _ci, _ci, _ci, _dp, _dp, _dp, _ci, ctypes.c_double, _ci, _dp, _dp, _ip, _dp,
+                                  _dp]
         L.tcg_set_quadrature.argtypes = [_vp, _ci, _ci]
         L.tcg_get_quadrature.argtypes = [_vp, ctypes.POINTER(_ci), ctypes.POINTER(_ci), _vp, _vp]
         L.tcg_fast_quad_self_check.argtypes = [_vp, _ci, _dp, _ip, _ci, ctypes.POINTER(ctypes.c_double),
@@ -256,6 +261,58 @@ class Engine:
         K = np.zeros(nb * nb)
         self._check(lib().tcg_direct_jk(self.h, nb, Pf.ravel(), J, K), "tcg_direct_jk")
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
+
+    # -- k-space (SAO) algebra ----------------------------------------------------
+    @staticmethod
+    def _cplx_out(buf: np.ndarray, nb: int) -> np.ndarray:
+        return buf.view(np.complex128).reshape(nb, nb).T.copy()      # column-major (re, im) pairs
+
+    @staticmethod
+    def _cplx_in(M: np.ndarray) -> np.ndarray:
+        return np.ascontiguousarray(np.asarray(M, dtype=np.complex128).T).view(np.float64).ravel()
+
+    def ao_to_sao(self, M: np.ndarray) -> np.ndarray:
+        """transform_AO_to_SAO: the k-block-diagonal (nBas, nBas) complex matrix."""
+        nb = self.s.nbas
+        out = np.zeros(2 * nb * nb)
+        Mf = np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel()
+        self._check(lib().tcg_ao_to_sao(self.h, nb, Mf, out), "tcg_ao_to_sao")
+        return self._cplx_out(out, nb)
+
+    def sao_to_ao(self, M: np.ndarray) -> np.ndarray:
+        """transform_SAO_to_AO: the real (nBas, nBas) supercell matrix of the k blocks of M."""
+        nb = self.s.nbas
+        out = np.zeros(nb * nb)
+        self._check(lib().tcg_sao_to_ao(self.h, nb, self._cplx_in(M), out), "tcg_sao_to_ao")
+        return out.reshape(nb, nb).T.copy()
+
+    def heev_blocks(self, A: np.ndarray):
+        """diagonalize_block_diagonal_complex_para: (A with its k blocks replaced by
+        their eigenvectors, e block by block ascending)."""
+        nb = self.s.nbas
+        buf = self._cplx_in(A).copy()
+        e = np.zeros(nb)
+        self._check(lib().tcg_heev_blocks(self.h, nb, buf, e), "tcg_heev_blocks")
+        return self._cplx_out(buf, nb), e
+
+    def rhf_sao(self, S, T, V, nocc: int, source: str = "stored", max_scf: int = 100, thresh: float = 1e-9,
+                diis: bool = True) -> dict:
+        """RHF_SAO with every iteration on the GPU (tcg_rhf_sao).  source: 'stored'
+        (the resident SAO ERI) or 'direct' (after direct_init)."""
+        nb = self.s.nbas
+        src = {"stored": 0, "direct": 1}[source]
+        f = lambda M: np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel()
+        e = np.zeros(nb)
+        C = np.zeros(2 * nb * nb)
+        occ = np.zeros(nb, dtype=np.int32)
+        P = np.zeros(nb * nb)
+        out = np.zeros(12)
+        self._check(lib().tcg_rhf_sao(self.h, src, nb, int(nocc), f(S), f(T), f(V), int(max_scf), float(thresh),
+                                      1 if diis else 0, e, C, occ, P, out), "tcg_rhf_sao")
+        return {"e_hf": out[0], "ET": out[1], "EV": out[2], "EJ": out[3], "EK": out[4],
+                "iterations": int(out[5]), "conv": out[6], "converged": bool(out[7]), "stalled": bool(out[8]),
+                "gap": out[9], "jk_ms": out[10], "wall_ms": out[11],
+                "e": e, "coeff_sao": self._cplx_out(C, nb), "occ": occ, "P": P.reshape(nb, nb).T.copy()}
 
 
 def ERI_integral_toroidal(sysm, coef, physicist: bool = False, sao: bool = True, nquad: int = 32):

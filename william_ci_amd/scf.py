@@ -15,8 +15,10 @@ drop-in; the engine supplies the integrals and the Coulomb/exchange build:
   NRE                    src/Energies/Nuclear_repulsion.f90:1 (chord distance)
   HF_parameter           src/module/HF_constant.f90 (maxSCF 100, orbital error 1e-9)
 
-The dense algebra (nBas^2 to nBas^3, per-k nfuc x nfuc eigenproblems) runs
-in numpy; the J/K build, which dominates, runs on the GPU.
+rhf_sao runs the dense algebra (per-k nfuc x nfuc eigenproblems, lattice
+transforms, DIIS) in numpy around the GPU J/K build; rhf_sao_gpu runs the same
+iteration entirely on the GPU (tcg_rhf_sao) and is the product path; rhf_sao
+stays as its cross-check and for callers that bring their own J/K.
 """
 from __future__ import annotations
 
@@ -279,6 +281,24 @@ def rhf_sao(sysm, S: np.ndarray, T: np.ndarray, V: np.ndarray, jk: Callable[[np.
     return RHFResult(energy=E + enuc, e_nuc=enuc, converged=converged, iterations=it, orbital_error=conv,
                      mo_energies=ek.copy(), mo_coef=Ck.copy(), P=P, occupied=occ.copy(),
                      times={"scf_s": time.perf_counter() - t0, "jk_s": t_jk})
+
+
+def rhf_sao_gpu(sysm, eng, S: np.ndarray, T: np.ndarray, V: np.ndarray, source: str = "stored",
+                diis: bool = True, max_scf: int = MAX_SCF, thresh: float = THRESH) -> RHFResult:
+    """RHF_SAO with the whole iteration on the GPU (engine.rhf_sao, tcg_rhf_sao):
+    the same guess, DIIS and convergence rules as rhf_sao, with J/K from the
+    resident SAO ERI (source='stored') or integral-direct ('direct')."""
+    t0 = time.perf_counter()
+    nf, Nc = sysm.nfuc, sysm.n_cell
+    r = eng.rhf_sao(S, T, V, sysm.n_occ, source=source, max_scf=max_scf, thresh=thresh, diis=diis)
+    enuc = nuclear_repulsion(sysm)
+    C = r["coeff_sao"]
+    Ck = np.stack([C[k * nf:(k + 1) * nf, k * nf:(k + 1) * nf] for k in range(Nc)])
+    return RHFResult(energy=r["e_hf"] + enuc, e_nuc=enuc, converged=r["converged"], iterations=r["iterations"],
+                     orbital_error=r["conv"], mo_energies=r["e"].reshape(Nc, nf), mo_coef=Ck, P=r["P"],
+                     occupied=(r["occ"] == 2).reshape(Nc, nf),
+                     times={"scf_s": time.perf_counter() - t0, "jk_s": r["jk_ms"] * 1e-3,
+                            "gpu_loop_s": r["wall_ms"] * 1e-3, "gap": r["gap"], "stalled": r["stalled"]})
 
 
 @dataclasses.dataclass
