@@ -120,7 +120,7 @@ def _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total, ms_step_pla
     step = (time.perf_counter() - t0) / args.steps
     tm = eng.times()
     t1 = time.perf_counter()
-    rl = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
+    rl = scf.rhf_sao_gpu(s, eng, S, T, V)
     t2 = time.perf_counter()
     ml = eng.kmp2(rl.mo_coef, rl.mo_energies, rl.occupied)
     t3 = time.perf_counter()
@@ -236,10 +236,14 @@ def main():
     t_one = time.perf_counter()
     S, T, V = eng.one_electron()
     t_one = time.perf_counter() - t_one
-    scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P), max_scf=2)      # untimed: first-call costs of the host loop
+    scf.rhf_sao_gpu(s, eng, S, T, V, max_scf=2)                         # untimed: first-call costs
     t_rhf = time.perf_counter()
-    res = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
+    res = scf.rhf_sao_gpu(s, eng, S, T, V)                               # the whole iteration on the GPU
     rhf_wall = time.perf_counter() - t_rhf
+    # untimed cross-check: the host loop (numpy algebra around the GPU J/K)
+    t_host = time.perf_counter()
+    res_host = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P))
+    host_wall = time.perf_counter() - t_host
     # MP2 on the RHF orbitals (SAO run: k-point MP2 from the resident ERI,
     # main.f90:823-847 with transform_ERI_AO_to_kMO + kMP2_calc)
     t_mp2 = time.perf_counter()
@@ -278,7 +282,10 @@ def main():
         "step_ms": {k: round(v, 3) for k, v in tm.items()},
         "rhf": {"energy": res.energy, "converged": res.converged, "iterations": res.iterations,
                 "orbital_error": res.orbital_error, "wall_s": rhf_wall, "one_electron_s": t_one,
-                "jk_s": res.times["jk_s"], "eri_plus_rhf_wall_s": rhf_wall + ms_per_step / 1e3},
+                "jk_s": res.times["jk_s"], "eri_plus_rhf_wall_s": rhf_wall + ms_per_step / 1e3,
+                "loop": "tcg_rhf_sao (J/K, lattice transforms, per-k Jacobi, DIIS, density on the GPU)",
+                "host_loop_check": {"energy": res_host.energy, "dE": res.energy - res_host.energy,
+                                    "iterations": res_host.iterations, "wall_s": host_wall}},
         "mp2": {"kind": "k-point MP2 (transform_ERI_AO_to_kMO + kMP2_calc)", "ec": mp2["ec"],
                 "e_total": res.energy + mp2["ec"], "zero_denominators": mp2["zero_denominators"],
                 "gpu_ms": mp2["gpu_ms"], "wall_s": mp2_wall,

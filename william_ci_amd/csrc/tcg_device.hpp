@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
 #include "tcg_engine.hpp"
@@ -143,7 +144,7 @@ hipError_t launch_col_flags(const double *eri, int n1, int nbas, int physicist, 
                             int *cnt_t, unsigned *key_t, hipStream_t st);
 hipError_t launch_jk_sparse(const double *eri, const double *P, const long long *list_d, const long long *off_d,
                             const long long *list_t, const long long *off_t, int n1, int nbas, int physicist,
-                            double *J, double *K, hipStream_t st);
+                            long long nz, double *J, double *K, hipStream_t st);
 hipError_t launch_gather_keys(const unsigned *keys, const long long *idx, long long n, unsigned *out, hipStream_t st);
 size_t jk_work_doubles(int n1, int nbas);
 hipError_t launch_jk_stored(const double *eri, const double *P, double *J, double *K, int n1, int nbas,
@@ -192,5 +193,7 @@ hipError_t launch_diis_combine(const double2 *hist, long long len, const DiisSlo
 // nbas^2 doubles.
 int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *dK, int *ld);
 hipStream_t ctx_stream(const tcg_ctx *c);
+// state the SCF loop keeps in the context between calls (released with it)
+std::shared_ptr<void> &ctx_scf_work(tcg_ctx *c);
 
 }  // namespace tcg

@@ -164,7 +164,9 @@ __global__ __launch_bounds__(256) void k_col_flags(const double *__restrict__ er
 
 // Column list entries g = b + nbas (c + nbas d); K(a,d) for the block's d, J(a,t)
 // for the block's target t (t = b chemist, t = c physicist: J(a,c) += ERI(a,b,c,d)
-// P(b,d), src/Energies/hartree_exchange_potential.f90:136-146).
+// P(b,d), src/Energies/hartree_exchange_potential.f90:136-146).  Each output
+// column's list is split over gridDim.y workgroups (enough workgroups to fill
+// the chip when the columns are few); their sums are added to the zeroed output.
 // Few rows (n1 <= NA): threads over the list entries, register accumulators
 // per row, one LDS reduction per block.
 template <int NA>
@@ -179,7 +181,10 @@ __global__ __launch_bounds__(256) void k_jk_sparse_small(const double *__restric
     double loc[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) loc[a] = 0.0;
-    for (long long i = off[col] + threadIdx.x; i < off[col + 1]; i += blockDim.x) {
+    const long long o0 = off[col], o1 = off[col + 1];
+    const long long len = o1 - o0, chunk = (len + gridDim.y - 1) / gridDim.y;
+    const long long i0 = o0 + chunk * blockIdx.y, i1 = i0 + chunk < o1 ? i0 + chunk : o1;
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const long long g = list[i];
         const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas), d = (int)(g / ((long long)nbas * nbas));
         double pw;
@@ -191,25 +196,33 @@ __global__ __launch_bounds__(256) void k_jk_sparse_small(const double *__restric
             if (a < n1) loc[a] += e[a] * pw;
     }
 #pragma unroll
-    for (int a = 0; a < NA; ++a)
-        if (a < n1) atomicAdd(&acc[a], loc[a]);
+    for (int a = 0; a < NA; ++a) {
+        double v = loc[a];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (a < n1 && (threadIdx.x & 63) == 0) atomicAdd(&acc[a], v);
+    }
     __syncthreads();
+    if (i1 <= i0) return;
     const double f = is_j ? 1.0 : -0.5;
-    for (int a = threadIdx.x; a < n1; a += blockDim.x) out[a + (size_t)nbas * col] = f * acc[a];
+    for (int a = threadIdx.x; a < n1; a += blockDim.x) atomicAdd(&out[a + (size_t)nbas * col], f * acc[a]);
 }
 
 // Many rows (n1 > 32, the full layout): threads over the rows a (coalesced
-// column reads), the block walks its list.
+// column reads), the block walks its part of the list.
 __global__ __launch_bounds__(256) void k_jk_sparse_rows(const double *__restrict__ eri, const double *__restrict__ P,
                                                         const long long *__restrict__ list,
                                                         const long long *__restrict__ off, int n1, int nbas, int is_j,
                                                         int physicist, double *__restrict__ out) {
     const int col = blockIdx.x;
     const double f = is_j ? 1.0 : -0.5;
+    const long long o0 = off[col], o1 = off[col + 1];
+    const long long len = o1 - o0, chunk = (len + gridDim.y - 1) / gridDim.y;
+    const long long i0 = o0 + chunk * blockIdx.y, i1 = i0 + chunk < o1 ? i0 + chunk : o1;
+    if (i1 <= i0) return;
     for (int a0 = 0; a0 < n1; a0 += blockDim.x) {
         const int a = a0 + threadIdx.x;
         double acc = 0.0;
-        for (long long i = off[col]; i < off[col + 1]; ++i) {
+        for (long long i = i0; i < i1; ++i) {
             const long long g = list[i];
             const int b = (int)(g % nbas), c = (int)((g / nbas) % nbas), d = (int)(g / ((long long)nbas * nbas));
             double pw;
@@ -217,23 +230,31 @@ __global__ __launch_bounds__(256) void k_jk_sparse_rows(const double *__restrict
             else pw = physicist ? P[b + (size_t)nbas * d] : P[c + (size_t)nbas * d];
             if (a < n1) acc += eri[(size_t)a + (size_t)n1 * g] * pw;
         }
-        if (a < n1) out[a + (size_t)nbas * col] = f * acc;
+        if (a < n1) atomicAdd(&out[a + (size_t)nbas * col], f * acc);
     }
 }
 
+__global__ void k_zero_rows(double *__restrict__ out, int n1, int nbas) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < (long long)n1 * nbas) out[(idx % n1) + (size_t)nbas * (idx / n1)] = 0.0;
+}
+
 static hipError_t launch_one(const double *eri, const double *P, const long long *list, const long long *off, int n1,
-                             int nbas, int is_j, int physicist, double *out, hipStream_t st) {
+                             int nbas, int is_j, int physicist, int splits, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)(((long long)n1 * nbas + 255) / 256)), dim3(256), 0, st, out, n1,
+                       nbas);
+    const dim3 grid(nbas, splits);
     if (n1 <= 8)
-        hipLaunchKernelGGL(k_jk_sparse_small<8>, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+        hipLaunchKernelGGL(k_jk_sparse_small<8>, grid, dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
                            physicist, out);
     else if (n1 <= 16)
-        hipLaunchKernelGGL(k_jk_sparse_small<16>, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+        hipLaunchKernelGGL(k_jk_sparse_small<16>, grid, dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
                            physicist, out);
     else if (n1 <= 32)
-        hipLaunchKernelGGL(k_jk_sparse_small<32>, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+        hipLaunchKernelGGL(k_jk_sparse_small<32>, grid, dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
                            physicist, out);
     else
-        hipLaunchKernelGGL(k_jk_sparse_rows, dim3(nbas), dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
+        hipLaunchKernelGGL(k_jk_sparse_rows, grid, dim3(256), 0, st, eri, P, list, off, n1, nbas, is_j,
                            physicist, out);
     return hipGetLastError();
 }
@@ -248,12 +269,19 @@ hipError_t launch_col_flags(const double *eri, int n1, int nbas, int physicist, 
     return hipGetLastError();
 }
 
+// nz = entries of each list (the non-zero columns).  Rows a < n1 of J and K
+// are written; the others are left as they are.
 hipError_t launch_jk_sparse(const double *eri, const double *P, const long long *list_d, const long long *off_d,
                             const long long *list_t, const long long *off_t, int n1, int nbas, int physicist,
-                            double *J, double *K, hipStream_t st) {
-    hipError_t e = launch_one(eri, P, list_d, off_d, n1, nbas, 0, physicist, K, st);
+                            long long nz, double *J, double *K, hipStream_t st) {
+    // about 2048 workgroups per pass, each at least 512 list entries on average
+    long long splits = (2048 + nbas - 1) / nbas;
+    const long long per_col = nz / (nbas > 0 ? nbas : 1);
+    splits = std::min<long long>(splits, std::max<long long>(1, per_col / 512));
+    splits = std::max<long long>(1, std::min<long long>(splits, 64));
+    hipError_t e = launch_one(eri, P, list_d, off_d, n1, nbas, 0, physicist, (int)splits, K, st);
     if (e != hipSuccess) return e;
-    return launch_one(eri, P, list_t, off_t, n1, nbas, 1, physicist, J, st);
+    return launch_one(eri, P, list_t, off_t, n1, nbas, 1, physicist, (int)splits, J, st);
 }
 
 __global__ void k_gather_keys(const unsigned *__restrict__ keys, const long long *__restrict__ idx, long long n,

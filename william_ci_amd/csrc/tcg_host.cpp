@@ -101,6 +101,8 @@ struct tcg_ctx {
     int eri_n1 = 0;
     bool eri_physicist = false;
     int cols_phys = 0;           // storage order the column lists were built for
+    long long cols_nz = 0;       // entries of each column list
+    std::shared_ptr<void> scf_work;   // device buffers of the SCF loop (tcg_scf.cpp)
     long long stats[8] = {0};
     double times[8] = {0};
     int max_ncomb = 1, max_blk = 1;
@@ -1079,6 +1081,7 @@ static int stored_cols(tcg_ctx *c, const double *src, int n1, int nbas, int phys
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->cols_ready = true;
     c->cols_phys = physicist;
+    c->cols_nz = nz;
     return TCG_OK;
 }
 
@@ -1146,6 +1149,8 @@ void ctx_set_error(tcg_ctx *c, const char *msg) {
 
 hipStream_t ctx_stream(const tcg_ctx *c) { return c ? c->st : nullptr; }
 
+std::shared_ptr<void> &ctx_scf_work(tcg_ctx *c) { return c->scf_work; }
+
 int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *dK, int *ld) {
     if (!c || !dP || !dJ || !dK || !ld) return TCG_E_ARG;
     if (source == 0) {
@@ -1156,7 +1161,7 @@ int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *
         const int rc = stored_cols(c, c->d_eri.p, n1, nb, physicist);
         if (rc) return rc;
         HIPCHK(c, launch_jk_sparse(c->d_eri.p, dP, c->d_list_d.p, c->d_off_d.p, c->d_list_b.p, c->d_off_b.p, n1, nb,
-                                   physicist, dJ, dK, c->st));
+                                   physicist, c->cols_nz, dJ, dK, c->st));
         *ld = nb;
         return TCG_OK;
     }
@@ -1398,7 +1403,7 @@ int tcg_jk_stored_order(tcg_ctx *c, int n1, int nbas, const double *P, const dou
     HIPCHK(c, launch_fill(c->d_J.p, nn, 0.0, c->st));
     HIPCHK(c, launch_fill(c->d_K.p, nn, 0.0, c->st));
     HIPCHK(c, launch_jk_sparse(src, c->d_P.p, c->d_list_d.p, c->d_off_d.p, c->d_list_b.p, c->d_off_b.p, n1, nbas,
-                               physicist, c->d_J.p, c->d_K.p, c->st));
+                               physicist, c->cols_nz, c->d_J.p, c->d_K.p, c->st));
     HIPCHK(c, hipMemcpyAsync(J, c->d_J.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(K, c->d_K.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -164,12 +165,35 @@ bool diis_solve(const std::vector<double> &B, int m, std::vector<double> &w) {
     return true;
 }
 
-struct PinnedScalars {
-    double *p = nullptr;
-    ~PinnedScalars() {
-        if (p) (void)hipHostFree(p);
+// Device buffers of the SCF loop, kept in the context between calls (grown as
+// needed), with the pinned read-back area, the J/K timing events and the
+// phase table of the lattice they were built for.
+struct ScfWork {
+    DevBuf<double> dTt, dVt, dH, dP, dJ, dK, dC, dCerr, dmax, dpart, dsc, de, dw, dsmin, dtmp;
+    DevBuf<double2> dph, dSk, dXk, dPk, dErrH, dFH, dFx, dCk;
+    DevBuf<int> ddead, docc;
+    double *hs = nullptr;
+    hipEvent_t ej0 = nullptr, ej1 = nullptr;
+    int ph_key[4] = {0, 0, 0, 0};                // (nx, ny, nz) of the phase table
+    ~ScfWork() {
+        if (hs) (void)hipHostFree(hs);
+        if (ej0) (void)hipEventDestroy(ej0);
+        if (ej1) (void)hipEventDestroy(ej1);
     }
 };
+
+int scf_work(tcg_ctx *c, ScfWork *&w) {
+    std::shared_ptr<void> &slot = ctx_scf_work(c);
+    if (!slot) {
+        auto sw = std::make_shared<ScfWork>();
+        SCHK(c, hipHostMalloc((void **)&sw->hs, sizeof(double) * 32, hipHostMallocDefault));
+        SCHK(c, hipEventCreate(&sw->ej0));
+        SCHK(c, hipEventCreate(&sw->ej1));
+        slot = sw;
+    }
+    w = static_cast<ScfWork *>(slot.get());
+    return TCG_OK;
+}
 
 }  // namespace
 
@@ -268,34 +292,26 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     const size_t nn = (size_t)nb * nb, len = (size_t)N * nf * nf;
 
     // --- set-up: S_k, X_k, T^T, V^T, unit-cell rows of Hc = T + V ---
-    DevBuf<double> dTt, dVt, dH, dP, dJ, dK, dC, dCerr, dmax, dpart, dsc, de, dw, dsmin, dtmp;
-    DevBuf<double2> dph, dSk, dXk, dPk, dErrH, dFH, dFx, dCk;
-    DevBuf<int> ddead, docc;
+    ScfWork *w = nullptr;
+    rc = scf_work(c, w);
+    if (rc) return rc;
+    auto &dTt = w->dTt, &dVt = w->dVt, &dH = w->dH, &dP = w->dP, &dJ = w->dJ, &dK = w->dK, &dC = w->dC;
+    auto &dCerr = w->dCerr, &dmax = w->dmax, &dpart = w->dpart, &dsc = w->dsc, &de = w->de, &dw = w->dw;
+    auto &dsmin = w->dsmin, &dtmp = w->dtmp;
+    auto &dph = w->dph, &dSk = w->dSk, &dXk = w->dXk, &dPk = w->dPk, &dErrH = w->dErrH, &dFH = w->dFH;
+    auto &dFx = w->dFx, &dCk = w->dCk;
+    auto &ddead = w->ddead, &docc = w->docc;
+    // every buffer at its size first (a grown buffer drains the device, so none is
+    // resized while this call's work is queued)
     SCHK(c, dtmp.alloc(nn));
     SCHK(c, dTt.alloc(nn));
     SCHK(c, dVt.alloc(nn));
     SCHK(c, dH.alloc((size_t)nf * nb));
-    SCHK(c, hipMemcpyAsync(dtmp.p, T, sizeof(double) * nn, hipMemcpyHostToDevice, st));
-    SCHK(c, launch_transpose(dtmp.p, nb, dTt.p, st));
-    SCHK(c, dP.alloc(nn));                                 // V goes through dP (free until the guess)
-    SCHK(c, hipMemcpyAsync(dP.p, V, sizeof(double) * nn, hipMemcpyHostToDevice, st));
-    SCHK(c, launch_transpose(dP.p, nb, dVt.p, st));
-    SCHK(c, launch_rows_sum(dtmp.p, dP.p, nf, nb, dH.p, st));
-    SCHK(c, hipStreamSynchronize(st));                     // dtmp is reused for S below
-    SCHK(c, hipMemcpyAsync(dtmp.p, S, sizeof(double) * nn, hipMemcpyHostToDevice, st));
-    SCHK(c, dph.upload(phase_table(L)));
+    SCHK(c, dP.alloc(nn));
     SCHK(c, dSk.alloc(len));
     SCHK(c, dXk.alloc(len));
     SCHK(c, ddead.alloc(nb));
     SCHK(c, dsmin.alloc(N));
-    RowSources srcS{};
-    srcS.n = 1;
-    srcS.p[0] = dtmp.p;
-    srcS.sa[0] = 1;
-    srcS.sc[0] = nb;
-    srcS.f[0] = 1.0;
-    SCHK(c, launch_lattice_fwd(srcS, dph.p, N, nf, dSk.p, st));
-    SCHK(c, launch_overlap_x(dSk.p, N, nf, dXk.p, ddead.p, dsmin.p, st));
     SCHK(c, dJ.alloc(nn));
     SCHK(c, dK.alloc(nn));
     SCHK(c, dPk.alloc(len));
@@ -305,24 +321,36 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     SCHK(c, dCk.alloc(len));
     SCHK(c, dC.alloc(len));
     SCHK(c, dCerr.alloc(len));
-    SCHK(c, dmax.alloc((len + 255) / 256 + 1));
+    SCHK(c, dmax.alloc(N));                                 // one maximum per lattice vector
     SCHK(c, dpart.alloc(4 * (size_t)energy_parts_blocks(nb)));
     SCHK(c, dsc.alloc(32));
     SCHK(c, de.alloc(nb));
     SCHK(c, dw.alloc(nb));
     SCHK(c, docc.alloc(nb));
-    PinnedScalars hs;
-    SCHK(c, hipHostMalloc((void **)&hs.p, sizeof(double) * 32, hipHostMallocDefault));
-    hipEvent_t ej0 = nullptr, ej1 = nullptr;
-    SCHK(c, hipEventCreate(&ej0));
-    SCHK(c, hipEventCreate(&ej1));
-    struct EvGuard {
-        hipEvent_t a, b;
-        ~EvGuard() {
-            if (a) (void)hipEventDestroy(a);
-            if (b) (void)hipEventDestroy(b);
-        }
-    } evg{ej0, ej1};
+    SCHK(c, dph.alloc((size_t)N * N));
+    if (w->ph_key[0] != L.ncl[0] || w->ph_key[1] != L.ncl[1] || w->ph_key[2] != L.ncl[2]) {
+        const std::vector<double2> ph = phase_table(L);
+        SCHK(c, hipMemcpy(dph.p, ph.data(), sizeof(double2) * ph.size(), hipMemcpyHostToDevice));
+        for (int d = 0; d < 3; ++d) w->ph_key[d] = L.ncl[d];
+    }
+    double *const hsp = w->hs;
+    hipEvent_t ej0 = w->ej0, ej1 = w->ej1;
+    SCHK(c, hipMemcpyAsync(dtmp.p, T, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+    SCHK(c, launch_transpose(dtmp.p, nb, dTt.p, st));
+    // V goes through dP (free until the guess)
+    SCHK(c, hipMemcpyAsync(dP.p, V, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+    SCHK(c, launch_transpose(dP.p, nb, dVt.p, st));
+    SCHK(c, launch_rows_sum(dtmp.p, dP.p, nf, nb, dH.p, st));
+    // dtmp is reused for S (stream order: after the transpose and the row sums)
+    SCHK(c, hipMemcpyAsync(dtmp.p, S, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+    RowSources srcS{};
+    srcS.n = 1;
+    srcS.p[0] = dtmp.p;
+    srcS.sa[0] = 1;
+    srcS.sc[0] = nb;
+    srcS.f[0] = 1.0;
+    SCHK(c, launch_lattice_fwd(srcS, dph.p, N, nf, dSk.p, st));
+    SCHK(c, launch_overlap_x(dSk.p, N, nf, dXk.p, ddead.p, dsmin.p, st));
 
     auto new_density = [&](const double2 *Fk, int frac) -> int {
         SCHK(c, launch_fock_diag(Fk, dXk.p, ddead.p, N, nf, de.p, dCk.p, st));
@@ -392,14 +420,14 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
         sl.slot[0] = slot;
         for (int j = 0; j < nkeep; ++j) sl.slot[1 + j] = order[j];
         if (diis) SCHK(c, launch_diis_dots(Ek, dErrH.p, (long long)len, sl, dsc.p + 8, st));
-        SCHK(c, hipMemcpyAsync(hs.p, dsc.p, sizeof(double) * 32, hipMemcpyDeviceToHost, st));
+        SCHK(c, hipMemcpyAsync(hsp, dsc.p, sizeof(double) * 32, hipMemcpyDeviceToHost, st));
         SCHK(c, hipStreamSynchronize(st));
         float jms = 0.0f;
         SCHK(c, hipEventElapsedTime(&jms, ej0, ej1));
         t_jk += jms;
-        for (int q = 0; q < 4; ++q) parts[q] = hs.p[q];
+        for (int q = 0; q < 4; ++q) parts[q] = hsp[q];
         E = parts[0] + parts[1] + parts[2] + parts[3];
-        if (it > 1) conv = hs.p[4];
+        if (it > 1) conv = hsp[4];
         if (conv < thresh) {
             converged = true;
             break;
@@ -418,7 +446,7 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
             // prepend this iteration's error to the history (DIIS_extrapolation :20-24)
             const int m = sl.m;
             std::vector<double> Bn((size_t)m * m);
-            for (int j = 0; j < m; ++j) Bn[j] = Bn[(size_t)m * j] = hs.p[8 + j];
+            for (int j = 0; j < m; ++j) Bn[j] = Bn[(size_t)m * j] = hsp[8 + j];
             const int mo = (int)order.size();
             for (int i = 1; i < m; ++i)
                 for (int j = 1; j < m; ++j) Bn[(size_t)m * i + j] = Bm[(size_t)mo * (i - 1) + (j - 1)];
@@ -450,7 +478,7 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     if (coeff_sao) SCHK(c, hipMemcpyAsync(Ck.data(), dCk.p, sizeof(double2) * len, hipMemcpyDeviceToHost, st));
     if (occ_out) SCHK(c, hipMemcpyAsync(occ_out, docc.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     if (P_out) SCHK(c, hipMemcpyAsync(P_out, dP.p, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
-    SCHK(c, hipMemcpyAsync(hs.p + 24, dsc.p + 24, sizeof(double), hipMemcpyDeviceToHost, st));
+    SCHK(c, hipMemcpyAsync(hsp + 24, dsc.p + 24, sizeof(double), hipMemcpyDeviceToHost, st));
     SCHK(c, hipStreamSynchronize(st));
     if (coeff_sao) scatter_blocks(L, Ck, coeff_sao, true);
     out[0] = E;
@@ -459,7 +487,7 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     out[6] = conv;
     out[7] = converged ? 1.0 : 0.0;
     out[8] = stalled ? 1.0 : 0.0;
-    out[9] = hs.p[24];
+    out[9] = hsp[24];
     out[10] = t_jk;
     out[11] = wall_ms() - t_start;
     return TCG_OK;

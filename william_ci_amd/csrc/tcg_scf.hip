@@ -40,6 +40,7 @@ constexpr double kCleanTol = 1e-15;     // clean_and_symmetrize (SAO_Transfer.f9
 constexpr double kSkipTol = 1e-15;      // transform_SAO_to_AO skips |M_k(a,b)| below this (:194)
 constexpr double kLindep = 1e-6;        // get_X_from_overlap_complex_blocks (Energies_tools.f90:343)
 constexpr int kMaxSweeps = 40;
+constexpr int kOccLds = 8192;          // levels staged in LDS by k_occupy (64 KiB)
 
 // block sum over all threads of the workgroup (blockDim a multiple of 64, <= 1024)
 __device__ double block_sum(double v, double *red) {
@@ -219,53 +220,95 @@ __device__ int diag_rank(const double2 *M, int n, int i) {
 // Lattice Fourier transforms
 // ---------------------------------------------------------------------------
 
+// One workgroup per output k point (per lattice vector dR for the inverse):
+// thread (ab, g) sums the lattice index over the residue class g mod G
+// (G = blockDim / nf^2 groups), the G partial sums are added in LDS.
 // out_k(a,b) = sum_R ph(k,R) sum_s f_s A_s(a, b + R nf)   (transform_AO_to_SAO on the
 // unit-cell rows: element (a, c) of source s at p_s[a sa_s + c sc_s]).
-__global__ __launch_bounds__(256) void k_lattice_fwd(RowSources src, const double2 *__restrict__ ph, int N, int nf,
-                                                     double2 *__restrict__ out) {
-    const int nff = nf * nf;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= N * nff) return;
-    const int k = idx / nff, r = idx % nff, a = r % nf, b = r / nf;
+__device__ inline double2 fwd_term_sum(const RowSources &src, const double2 *phk, int N, int nf, int ab, int g,
+                                       int G) {
+    const int a = ab % nf, b = ab / nf;
     double2 acc{0.0, 0.0};
-    for (int R = 0; R < N; ++R) {
+    for (int R = g; R < N; R += G) {
         const long long col = b + (long long)nf * R;
         double v = 0.0;
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-            if (s < src.n) v += src.f[s] * src.p[s][(long long)a * src.sa[s] + col * src.sc[s]];
-        const double2 e = ph[(size_t)k * N + R];
+        for (int q = 0; q < 4; ++q)
+            if (q < src.n) v += src.f[q] * src.p[q][(long long)a * src.sa[q] + col * src.sc[q]];
+        const double2 e = phk[R];
         acc.x += e.x * v;
         acc.y += e.y * v;
     }
-    out[idx] = acc;
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_lattice_fwd(RowSources src, const double2 *__restrict__ ph, int N, int nf,
+                                                     double2 *__restrict__ out) {
+    __shared__ double2 part[256];
+    const int nff = nf * nf, k = blockIdx.x, t = threadIdx.x;
+    const double2 *phk = ph + (size_t)k * N;
+    double2 *o = out + (size_t)k * nff;
+    if (nff <= (int)blockDim.x) {
+        const int G = blockDim.x / nff, ab = t % nff, g = t / nff;
+        part[t] = g < G ? fwd_term_sum(src, phk, N, nf, ab, g, G) : double2{0.0, 0.0};
+        __syncthreads();
+        if (t < nff) {
+            double2 sm{0.0, 0.0};
+            for (int q = 0; q < G; ++q) {
+                sm.x += part[t + nff * q].x;
+                sm.y += part[t + nff * q].y;
+            }
+            o[t] = sm;
+        }
+    } else {
+        for (int ab = t; ab < nff; ab += blockDim.x) o[ab] = fwd_term_sum(src, phk, N, nf, ab, 0, 1);
+    }
 }
 
 // C[dR](a,b) = Re (1/N) sum_k ph(k,dR) M_k(a,b), elements |M_k(a,b)| < 1e-15 left
 // out when skip (transform_SAO_to_AO, SAO_Transfer.f90:186-225).  maxabs (optional):
-// per-workgroup max |C| into maxabs[blockIdx.x].
+// max |C[dR]| into maxabs[dR].
+__device__ inline double inv_term_sum(const double2 *Mk, const double2 *ph, int N, int nff, int dR, int skip, int ab,
+                                      int g, int G) {
+    double acc = 0.0;
+    for (int k = g; k < N; k += G) {
+        const double2 m = Mk[(size_t)k * nff + ab];
+        if (skip && m.x * m.x + m.y * m.y < kSkipTol * kSkipTol) continue;
+        const double2 e = ph[(size_t)k * N + dR];
+        acc += e.x * m.x - e.y * m.y;
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(256) void k_lattice_inv(const double2 *__restrict__ Mk, const double2 *__restrict__ ph,
                                                      int N, int nf, int skip, double *__restrict__ C,
                                                      double *__restrict__ maxabs) {
+    __shared__ double part[256];
     __shared__ double red[16];
-    const int nff = nf * nf;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    double v = 0.0;
-    if (idx < N * nff) {
-        const int dR = idx / nff, r = idx % nff;
-        double acc = 0.0;
-        for (int k = 0; k < N; ++k) {
-            const double2 m = Mk[(size_t)k * nff + r];
-            if (skip && m.x * m.x + m.y * m.y < kSkipTol * kSkipTol) continue;
-            const double2 e = ph[(size_t)k * N + dR];
-            acc += e.x * m.x - e.y * m.y;
+    const int nff = nf * nf, dR = blockIdx.x, t = threadIdx.x;
+    double *o = C + (size_t)dR * nff;
+    double mx = 0.0;
+    if (nff <= (int)blockDim.x) {
+        const int G = blockDim.x / nff, ab = t % nff, g = t / nff;
+        part[t] = g < G ? inv_term_sum(Mk, ph, N, nff, dR, skip, ab, g, G) : 0.0;
+        __syncthreads();
+        if (t < nff) {
+            double sm = 0.0;
+            for (int q = 0; q < G; ++q) sm += part[t + nff * q];
+            const double v = sm / N;
+            o[t] = v;
+            mx = fabs(v);
         }
-        v = acc / N;
-        C[idx] = v;
+    } else {
+        for (int ab = t; ab < nff; ab += blockDim.x) {
+            const double v = inv_term_sum(Mk, ph, N, nff, dR, skip, ab, 0, 1) / N;
+            o[ab] = v;
+            mx = fmax(mx, fabs(v));
+        }
     }
     if (maxabs) {
-        const double mx = block_max(fabs(v), red);
-        if (threadIdx.x == 0) maxabs[blockIdx.x] = mx;
+        mx = block_max(mx, red);
+        if (t == 0) maxabs[dR] = mx;
     }
 }
 
@@ -507,9 +550,11 @@ __global__ __launch_bounds__(256) void k_density_k(const double2 *__restrict__ C
 // (src/Energies/SAO/guess_Huckel_SAO.f90:141-192): the level at the cut, all
 // values within tol = 1e-8 max(1, |e_cut|) of it, shares the remaining electrons
 // evenly.  w[ne] weights, occ[ne] = 2 / 0 (frac = 0), sc[0] = HOMO-LUMO gap.
-__global__ __launch_bounds__(1024) void k_occupy(const double *__restrict__ e, int ne, int nocc, int frac,
+// The levels are staged in LDS when they fit (ne <= kOccLds).
+__global__ __launch_bounds__(1024) void k_occupy(const double *__restrict__ e_g, int ne, int nocc, int frac,
                                                  double *__restrict__ w, int *__restrict__ occ,
                                                  double *__restrict__ sc) {
+    extern __shared__ double e_l[];
     __shared__ double ecut, enext;
     __shared__ int nlo, nwin;
     if (threadIdx.x == 0) {
@@ -518,7 +563,11 @@ __global__ __launch_bounds__(1024) void k_occupy(const double *__restrict__ e, i
         nlo = 0;
         nwin = 0;
     }
+    const bool staged = ne <= kOccLds;
+    if (staged)
+        for (int i = threadIdx.x; i < ne; i += blockDim.x) e_l[i] = e_g[i];
     __syncthreads();
+    const double *e = staged ? e_l : e_g;
     for (int i = threadIdx.x; i < ne; i += blockDim.x) {
         const double v = e[i];
         int r = 0;
@@ -654,15 +703,14 @@ static hipError_t allow_lds(K kern, size_t lds) {
 }
 
 hipError_t launch_lattice_fwd(const RowSources &src, const double2 *ph, int N, int nf, double2 *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_lattice_fwd, dim3(blocks_for((long long)N * nf * nf)), dim3(256), 0, st, src, ph, N, nf, out);
+    hipLaunchKernelGGL(k_lattice_fwd, dim3(N), dim3(256), 0, st, src, ph, N, nf, out);
     return hipGetLastError();
 }
 
 hipError_t launch_lattice_inv(const double2 *Mk, const double2 *ph, int N, int nf, int skip, double *C,
                               double *maxabs, int *nmax, hipStream_t st) {
-    const unsigned g = blocks_for((long long)N * nf * nf);
-    if (nmax) *nmax = (int)g;
-    hipLaunchKernelGGL(k_lattice_inv, dim3(g), dim3(256), 0, st, Mk, ph, N, nf, skip, C, maxabs);
+    if (nmax) *nmax = N;
+    hipLaunchKernelGGL(k_lattice_inv, dim3(N), dim3(256), 0, st, Mk, ph, N, nf, skip, C, maxabs);
     return hipGetLastError();
 }
 
@@ -716,7 +764,8 @@ hipError_t launch_density_k(const double2 *Ck, const double *w, int N, int n, do
 
 hipError_t launch_occupy(const double *e, int ne, int nocc, int frac, double *w, int *occ, double *sc,
                          hipStream_t st) {
-    hipLaunchKernelGGL(k_occupy, dim3(1), dim3(1024), 0, st, e, ne, nocc, frac, w, occ, sc);
+    const size_t lds = ne <= kOccLds ? sizeof(double) * (size_t)ne : 0;
+    hipLaunchKernelGGL(k_occupy, dim3(1), dim3(1024), lds, st, e, ne, nocc, frac, w, occ, sc);
     return hipGetLastError();
 }
 
