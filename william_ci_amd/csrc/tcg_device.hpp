@@ -115,6 +115,22 @@ hipError_t launch_kmp2_sums(const double2 *ekmo, const double *ek, const int *oc
                             hipStream_t st);
 hipError_t launch_mp2_sums(const double *X, const double *e, int o, int v, long long si, long long sa, long long sj,
                            long long sb, double *part, int nblocks, hipStream_t st);
+// the lattice transform fused with an orbital contraction (tcg_mp2.hip, k_dft_mo)
+struct DftMoArgs {
+    const double *Xr, *Xi;
+    long long I, O;              // inner rows, outer blocks
+    const long long *imap;       // inner row -> offset in X (nullptr: i)
+    long long sJ, sO;            // strides of j = c + nf R and of o in X
+    int nf, N, NR, M;
+    const double *cosm, *sinm;   // [R + NR k]
+    double sgn, scale;
+    const double2 *T;            // [(k nf + c) M + m]
+    double *Yr, *Yi;
+};
+hipError_t launch_dft_mo(const DftMoArgs &A, hipStream_t st);
+hipError_t launch_list_j2_flags(const long long *list, long long n, int nbas, unsigned char *flags, hipStream_t st);
+hipError_t launch_kmo_last(const double *Yr, const double *Yi, const double2 *Ck, int nf, int N, int nx, int ny,
+                           int nz, int o_hi, int v_lo, double2 *out, hipStream_t st);
 int mlp_param_count();
 hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, const Surv *surv, const double *expo,
                            const float *norm, float *feat, unsigned char *ood, hipStream_t st);

@@ -103,6 +103,13 @@ struct tcg_ctx {
     int cols_phys = 0;           // storage order the column lists were built for
     long long cols_nz = 0;       // entries of each column list
     std::shared_ptr<void> scf_work;   // device buffers of the SCF loop (tcg_scf.cpp)
+    // k-point MP2 work buffers (kept between calls, grown as needed)
+    DevBuf<double> km_cos, km_sin, km_cosn, km_sinn, km_e, km_part, km_cos2, km_sin2, km_Ar, km_Ai, km_Br, km_Bi;
+    DevBuf<double2> km_Ck, km_kmo, km_TA, km_TB;
+    DevBuf<int> km_occ;
+    DevBuf<long long> km_imap;
+    DevBuf<unsigned char> km_fl;
+    DevBuf<unsigned long long> km_nz;
     long long stats[8] = {0};
     double times[8] = {0};
     int max_ncomb = 1, max_blk = 1;
@@ -1571,44 +1578,159 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
             cosn[R + (size_t)N * k] = std::cos(th) / N;
             sinn[R + (size_t)N * k] = std::sin(th) / N;
         }
-    DevBuf<double> dcos, dsin, dcosn, dsinn, X1r, X1i, X2r, X2i, Sr, Si, de, dpart;
-    DevBuf<double2> dCk, dekmo;
-    DevBuf<int> docc;
-    DevBuf<unsigned long long> dnz;
-    HIPCHK(c, dcos.upload(cosm));
-    HIPCHK(c, dsin.upload(sinm));
-    HIPCHK(c, dcosn.upload(cosn));
-    HIPCHK(c, dsinn.upload(sinn));
-    HIPCHK(c, dCk.upload(Ck));
-    HIPCHK(c, de.upload(ek));
-    HIPCHK(c, docc.upload(oc));
+    DevBuf<double> X1r, X1i, X2r, X2i, Sr, Si;
+    auto &dcos = c->km_cos, &dsin = c->km_sin, &dcosn = c->km_cosn, &dsinn = c->km_sinn, &de = c->km_e;
+    auto &dpart = c->km_part;
+    auto &dCk = c->km_Ck, &dekmo = c->km_kmo;
+    auto &docc = c->km_occ;
+    auto &dnz = c->km_nz;
+    HIPCHK(c, upload_async(c, dcos, cosm));
+    HIPCHK(c, upload_async(c, dsin, sinm));
+    HIPCHK(c, upload_async(c, dcosn, cosn));
+    HIPCHK(c, upload_async(c, dsinn, sinn));
+    HIPCHK(c, upload_async(c, dCk, Ck));
+    HIPCHK(c, upload_async(c, de, ek));
+    HIPCHK(c, upload_async(c, docc, oc));
+    const int nv = nf - v_lo;
+    const size_t nkmo = (size_t)o_hi * nv * o_hi * nv * N * N * N;
+    hipStream_t st = c->st;
+    const double t0 = now_ms();
+    if (N <= 64) {
+        // Each lattice transform is fused with the orbital contraction of its index pair
+        // (k_dft_mo), occupied index first, so that every later pass runs on the
+        // reduced tensor:
+        //   A: (c R3) -> (r k3), conj C_k3(c, r)   Y_A(a b R2, r k3, j4)
+        //   B: (d R4) -> (s k4), C_k4(d, s)        Y_B(a b R2 r k3, s k4)
+        //   C: (b R2) -> (q k2), C_k2(b, q) / N    Y_C(a, q k2, r k3 s k4)
+        //   D: a -> p, conj C_k1(a, p), k1 = k4 - k3 + k2
+        // with e^{-i k3 R3}, e^{+i k4 R4}, e^{+i k2 R2} (the reference's phase convention).
+        // The bra pair (a in cell 0, b in cell R2) is non-zero only for R2 near cell 0:
+        // the passes run over those cells alone (from the non-zero column lists).
+        {
+            const int rc = stored_cols(c, c->d_eri.p, nf, nb, 0);
+            if (rc) return rc;
+        }
+        auto &dfl = c->km_fl;
+        HIPCHK(c, dfl.alloc(nb));
+        HIPCHK(c, hipMemsetAsync(dfl.p, 0, nb, st));
+        HIPCHK(c, launch_list_j2_flags(c->d_list_d.p, c->cols_nz, nb, dfl.p, st));
+        std::vector<unsigned char> fl(nb);
+        HIPCHK(c, hipMemcpyAsync(fl.data(), dfl.p, nb, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        std::vector<int> cells2;
+        for (int R = 0; R < N; ++R) {
+            bool any = false;
+            for (int b2 = 0; b2 < nf; ++b2) any |= fl[(size_t)R * nf + b2] != 0;
+            if (any) cells2.push_back(R);
+        }
+        if (cells2.empty()) cells2.push_back(0);               // an all-zero tensor: E_c = 0
+        const int n2 = (int)cells2.size();
+        std::vector<double> cos2((size_t)n2 * N), sin2((size_t)n2 * N);
+        for (int k = 0; k < N; ++k)
+            for (int i = 0; i < n2; ++i) {
+                cos2[i + (size_t)n2 * k] = cosm[cells2[i] + (size_t)N * k];
+                sin2[i + (size_t)n2 * k] = sinm[cells2[i] + (size_t)N * k];
+            }
+        std::vector<double2> TA((size_t)N * nf * o_hi), TB((size_t)N * nf * nv);
+        for (int k = 0; k < N; ++k)
+            for (int a = 0; a < nf; ++a) {
+                for (int m = 0; m < o_hi; ++m) {
+                    const double2 v = Ck[(size_t)k * nf * nf + a + (size_t)nf * m];
+                    TA[((size_t)k * nf + a) * o_hi + m] = double2{v.x, -v.y};
+                }
+                for (int m = 0; m < nv; ++m)
+                    TB[((size_t)k * nf + a) * nv + m] = Ck[(size_t)k * nf * nf + a + (size_t)nf * (v_lo + m)];
+            }
+        const long long IA = (long long)nf * nf * n2, IB = IA * o_hi * N;
+        std::vector<long long> imap(IA);
+        for (long long i = 0; i < IA; ++i) {
+            const long long a = i % nf, b2 = (i / nf) % nf, R2 = cells2[i / ((long long)nf * nf)];
+            imap[i] = a + (long long)nf * (b2 + (long long)nf * R2);
+        }
+        auto &dTA = c->km_TA, &dTB = c->km_TB;
+        auto &dcos2 = c->km_cos2, &dsin2 = c->km_sin2;
+        auto &dimap = c->km_imap;
+        HIPCHK(c, upload_async(c, dTA, TA));
+        HIPCHK(c, upload_async(c, dTB, TB));
+        HIPCHK(c, upload_async(c, dcos2, cos2));
+        HIPCHK(c, upload_async(c, dsin2, sin2));
+        HIPCHK(c, upload_async(c, dimap, imap));
+        const size_t nYA = (size_t)IA * o_hi * N * nb, nYB = (size_t)IB * nv * N;
+        const size_t nYC = (size_t)nf * nv * N * o_hi * N * nv * N;
+        auto &Ar = c->km_Ar, &Ai = c->km_Ai, &Br = c->km_Br, &Bi = c->km_Bi;
+        HIPCHK(c, Ar.alloc(std::max(nYA, nYC)));
+        HIPCHK(c, Ai.alloc(std::max(nYA, nYC)));
+        HIPCHK(c, Br.alloc(nYB));
+        HIPCHK(c, Bi.alloc(nYB));
+        DftMoArgs d{};
+        // A: rows (a, b, R2 in cells2) gathered from the tensor, outer j4
+        d.Xr = c->d_eri.p;
+        d.Xi = nullptr;
+        d.I = IA;
+        d.O = nb;
+        d.imap = dimap.p;
+        d.sJ = (long long)nf * nb;
+        d.sO = (long long)nf * nb * nb;
+        d.nf = nf;
+        d.N = N;
+        d.NR = N;
+        d.M = o_hi;
+        d.cosm = dcos.p;
+        d.sinm = dsin.p;
+        d.sgn = -1.0;
+        d.scale = 1.0;
+        d.T = dTA.p;
+        d.Yr = Ar.p;
+        d.Yi = Ai.p;
+        HIPCHK(c, launch_dft_mo(d, st));
+        // B: rows (a b R2, r k3), j4 = d + nf R4 at stride IB
+        d.Xr = Ar.p;
+        d.Xi = Ai.p;
+        d.I = IB;
+        d.O = 1;
+        d.imap = nullptr;
+        d.sJ = IB;
+        d.sO = IB * nf * N;
+        d.M = nv;
+        d.sgn = 1.0;
+        d.T = dTB.p;
+        d.Yr = Br.p;
+        d.Yi = Bi.p;
+        HIPCHK(c, launch_dft_mo(d, st));
+        // C: rows a, j2 = b + nf R2 over cells2 at stride nf, outer (r k3, s k4) at stride IA
+        d.Xr = Br.p;
+        d.Xi = Bi.p;
+        d.I = nf;
+        d.O = (long long)o_hi * N * nv * N;
+        d.sJ = nf;
+        d.sO = IA;
+        d.NR = n2;
+        d.cosm = dcos2.p;
+        d.sinm = dsin2.p;
+        d.scale = 1.0 / N;
+        d.Yr = Ar.p;
+        d.Yi = Ai.p;
+        HIPCHK(c, launch_dft_mo(d, st));
+        HIPCHK(c, dekmo.alloc(nkmo));
+        HIPCHK(c, launch_kmo_last(Ar.p, Ai.p, dCk.p, nf, N, nx, ny, nz, o_hi, v_lo, dekmo.p, st));
+    } else {
+    // lattice transforms as real GEMMs on the whole SAO tensor, then the quarter transforms
     const size_t nX = M1 * N;                  // = nf^2 nb^2 (all elements, k in place of R)
     HIPCHK(c, X1r.alloc(nX));
     HIPCHK(c, X1i.alloc(nX));
-    hipStream_t st = c->st;
-    const double t0 = now_ms();
-    // Each lattice transform is one pass of k_dft_lattice (N <= 64), else real GEMMs.
-    const bool dft = N <= 64;
     // 1) R4 -> k4: X1(a,j2,j3,d,k4) = sum_R4 ERI(a,j2,j3,d R4) e^{+i k4 R4}
-    if (dft) {
-        HIPCHK(c, launch_dft_lattice(c->d_eri.p, nullptr, (long long)M1, 1, N, dcos.p, dsin.p, 1.0, 1.0, X1r.p, X1i.p, st));
-    } else {
-        HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dcos.p, N, 0, 0.0, X1r.p, (long)M1, 0, 1, st));
-        HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dsin.p, N, 0, 0.0, X1i.p, (long)M1, 0, 1, st));
-    }
+    HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dcos.p, N, 0, 0.0, X1r.p, (long)M1, 0, 1, st));
+    HIPCHK(c, launch_dgemm_nn((int)M1, N, N, 1.0, c->d_eri.p, (long)M1, 0, dsin.p, N, 0, 0.0, X1i.p, (long)M1, 0, 1, st));
     // 2) R3 -> k3 (e^{-i k3 R3}), batched over (d, k4): rows (a, j2, c), X2(a,j2,c,k3,d,k4)
     HIPCHK(c, X2r.alloc(nX));
     HIPCHK(c, X2i.alloc(nX));
     {
         const long long rows = (long long)nf * nb * nf, bs = (long long)nf * nb * nb, nbatch = (long long)nf * N;
         // re = X1r cos + X1i sin ; im = X1i cos - X1r sin
-        if (dft) HIPCHK(c, launch_dft_lattice(X1r.p, X1i.p, rows, nbatch, N, dcos.p, dsin.p, -1.0, 1.0, X2r.p, X2i.p, st));
-        else {
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1r.p, (long)rows, bs, dcos.p, N, 0, 0.0, X2r.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1i.p, (long)rows, bs, dsin.p, N, 0, 1.0, X2r.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X1i.p, (long)rows, bs, dcos.p, N, 0, 0.0, X2i.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, -1.0, X1r.p, (long)rows, bs, dsin.p, N, 0, 1.0, X2i.p, (long)rows, bs, nbatch, st));
-        }
     }
     X1r.release();
     X1i.release();
@@ -1617,21 +1739,17 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
     HIPCHK(c, Si.alloc(nX));
     {
         const long long rows = (long long)nf * nf, bs = (long long)nf * nb, nbatch = (long long)nf * N * nf * N;
-        if (dft) HIPCHK(c, launch_dft_lattice(X2r.p, X2i.p, rows, nbatch, N, dcos.p, dsin.p, 1.0, 1.0 / N, Sr.p, Si.p, st));
-        else {
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2r.p, (long)rows, bs, dcosn.p, N, 0, 0.0, Sr.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, -1.0, X2i.p, (long)rows, bs, dsinn.p, N, 0, 1.0, Sr.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2r.p, (long)rows, bs, dsinn.p, N, 0, 0.0, Si.p, (long)rows, bs, nbatch, st));
         HIPCHK(c, launch_dgemm_nn((int)rows, N, N, 1.0, X2i.p, (long)rows, bs, dcosn.p, N, 0, 1.0, Si.p, (long)rows, bs, nbatch, st));
-        }
     }
     X2r.release();
     X2i.release();
     // 4) quarter transforms with the k-block coefficients, (ov|ov) only
-    const int nv = nf - v_lo;
-    const size_t nkmo = (size_t)o_hi * nv * o_hi * nv * N * N * N;
     HIPCHK(c, dekmo.alloc(nkmo));
     HIPCHK(c, launch_kmo(Sr.p, Si.p, dCk.p, nf, N, nx, ny, nz, o_hi, v_lo, dekmo.p, st));
+    }
     // 5) kMP2_calc
     const int nblk = 1024;
     HIPCHK(c, dpart.alloc(2 * nblk));

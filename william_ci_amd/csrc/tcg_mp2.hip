@@ -424,6 +424,204 @@ hipError_t launch_kmo(const double *Sre, const double *Sim, const double2 *Ck, i
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// k_dft_mo: the lattice Fourier transform along one (orbital, cell) index pair
+// j = c + nf R of X fused with the orbital contraction of that pair:
+//   Y[i + I (m + M k) + I M N o] =
+//       scale sum_c T[(k nf + c) M + m] sum_R (cos + i sgn sin)(k.cell_R) X[x(i) + sJ (c + nf R) + sO o]
+// R over the NR input cells (cos/sin tables [R + NR k]), k over N points, x(i) =
+// imap[i] (a gather of the inner rows) or i, for the workgroup's chunk m in
+// [MR blockIdx.y, MR (blockIdx.y + 1)) of the M outputs.  The orbital index is
+// contracted as soon as its transform is in the accumulators, so the transformed
+// tensor with all nf orbitals is never stored.  X real (kCplx false) or split
+// complex; Y split complex.  The rows (i, o) are flattened, 16 per tile, and each
+// DFT is a matrix-core product as in k_dft_lattice (W in LDS).  KT = 4: a wave
+// does every k tile of its own 16 rows; KT = 1: the four waves share 16 rows,
+// wave w doing k tile w (room for more accumulators per k).
+// ---------------------------------------------------------------------------
+
+template <bool kCplx, int KT, int MR>
+__global__ __launch_bounds__(256) void k_dft_mo(DftMoArgs A) {
+    __shared__ double Wc[64][65];
+    __shared__ double Ws[64][65];
+    const int N = A.N, NR = A.NR, nf = A.nf, M = A.M;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int R = e >> 6, k = e & 63;
+        const bool in = R < NR && k < N;
+        Wc[R][k] = in ? A.cosm[R + (size_t)NR * k] : 0.0;
+        Ws[R][k] = in ? A.sgn * A.sinm[R + (size_t)NR * k] : 0.0;
+    }
+    __syncthreads();
+    const long long I = A.I, rows = I * A.O, ntile = (rows + 15) / 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    const int nk = (N + 15) >> 4;
+    const int m0 = MR * blockIdx.y;
+    const int t_lo = KT == 4 ? 0 : wv;
+    const long long tile0 = KT == 4 ? (long long)blockIdx.x * 4 + wv : (long long)blockIdx.x;
+    const long long tstep = KT == 4 ? (long long)gridDim.x * 4 : (long long)gridDim.x;
+    const size_t sR = (size_t)A.sJ * nf;
+    for (long long tile = tile0; tile < ntile; tile += tstep) {
+        const long long r = tile * 16 + lr;
+        const bool rv = r < rows;
+        const long long rr = rv ? r : rows - 1;
+        const long long inner = rr % I, ob = rr / I;
+        const size_t base = (size_t)(A.imap ? A.imap[inner] : inner) + (size_t)A.sO * ob;
+        dbl4 yr[MR][KT], yi[MR][KT];
+#pragma unroll
+        for (int mm = 0; mm < MR; ++mm)
+#pragma unroll
+            for (int tt = 0; tt < KT; ++tt) {
+                yr[mm][tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+                yi[mm][tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+            }
+        for (int c = 0; c < nf; ++c) {
+            dbl4 re[KT], im[KT];
+#pragma unroll
+            for (int tt = 0; tt < KT; ++tt) {
+                re[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+                im[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+            }
+            const size_t bc = base + (size_t)A.sJ * c;
+            for (int R0 = 0; R0 < NR; R0 += 4) {
+                const int R = R0 + lq;
+                const double ar = R < NR ? A.Xr[bc + sR * R] : 0.0;
+                const double ai = (kCplx && R < NR) ? A.Xi[bc + sR * R] : 0.0;
+#pragma unroll
+                for (int tt = 0; tt < KT; ++tt) {
+                    const int t = t_lo + tt;
+                    if (t < nk) {
+                        const double wc = Wc[R][16 * t + lr], ws = Ws[R][16 * t + lr];
+                        re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(wc, ar, re[tt], 0, 0, 0);
+                        im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ws, ar, im[tt], 0, 0, 0);
+                        if (kCplx) {
+                            re[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ws, ai, re[tt], 0, 0, 0);
+                            im[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(wc, ai, im[tt], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            // fold orbital c into the outputs: lane holds k = 16 t + 4 v + lq of row lr
+#pragma unroll
+            for (int tt = 0; tt < KT; ++tt) {
+                const int t = t_lo + tt;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = 16 * t + 4 * v + lq;
+                    if (t >= nk || k >= N) continue;
+                    const double2 *Tk = A.T + ((size_t)k * nf + c) * M;
+#pragma unroll
+                    for (int mm = 0; mm < MR; ++mm) {
+                        const int m = m0 + mm;
+                        if (m >= M) continue;
+                        const double2 w = Tk[m];
+                        yr[mm][tt][v] += w.x * re[tt][v] - w.y * im[tt][v];
+                        yi[mm][tt][v] += w.x * im[tt][v] + w.y * re[tt][v];
+                    }
+                }
+            }
+        }
+        if (rv) {
+            const size_t ob_off = (size_t)inner + (size_t)I * M * N * ob;
+#pragma unroll
+            for (int tt = 0; tt < KT; ++tt) {
+                const int t = t_lo + tt;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = 16 * t + 4 * v + lq;
+                    if (t >= nk || k >= N) continue;
+#pragma unroll
+                    for (int mm = 0; mm < MR; ++mm) {
+                        const int m = m0 + mm;
+                        if (m >= M) continue;
+                        const size_t o = ob_off + (size_t)I * (m + (size_t)M * k);
+                        A.Yr[o] = A.scale * yr[mm][tt][v];
+                        A.Yi[o] = A.scale * yi[mm][tt][v];
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <bool kCplx, int KT, int MR>
+static hipError_t dft_mo_launch(const DftMoArgs &A, hipStream_t st) {
+    const long long ntile = (A.I * A.O + 15) / 16;
+    const long long g = KT == 4 ? (ntile + 3) / 4 : ntile;
+    const dim3 grid((unsigned)std::min<long long>(g, 4096), (unsigned)((A.M + MR - 1) / MR));
+    hipLaunchKernelGGL((k_dft_mo<kCplx, KT, MR>), grid, dim3(256), 0, st, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft_mo(const DftMoArgs &A, hipStream_t st) {
+    if (A.N > 64 || A.N < 1 || A.NR > 64 || A.NR < 1 || A.I <= 0 || A.O <= 0 || A.M <= 0 || A.nf <= 0)
+        return hipErrorInvalidValue;
+    const int nk = (A.N + 15) / 16;
+    const bool cplx = A.Xi != nullptr;
+    if (nk < 3 || A.M <= 2) {                      // every k tile in one wave
+        if (cplx) return A.M == 1 ? dft_mo_launch<true, 4, 1>(A, st) : dft_mo_launch<true, 4, 2>(A, st);
+        return A.M == 1 ? dft_mo_launch<false, 4, 1>(A, st) : dft_mo_launch<false, 4, 2>(A, st);
+    }
+    if (cplx) return A.M <= 4 ? dft_mo_launch<true, 1, 4>(A, st) : dft_mo_launch<true, 1, 8>(A, st);
+    return A.M <= 4 ? dft_mo_launch<false, 1, 4>(A, st) : dft_mo_launch<false, 1, 8>(A, st);
+}
+
+// flags[j2] = 1 for every bra function j2 = g mod nbas of a listed non-zero column g
+__global__ void k_list_j2_flags(const long long *__restrict__ list, long long n, int nbas,
+                                unsigned char *__restrict__ flags) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        flags[list[i] % nbas] = 1;
+}
+
+hipError_t launch_list_j2_flags(const long long *list, long long n, int nbas, unsigned char *flags, hipStream_t st) {
+    const int grid = (int)std::min<long long>((n + 255) / 256, 4096);
+    if (n > 0) hipLaunchKernelGGL(k_list_j2_flags, dim3(grid > 0 ? grid : 1), dim3(256), 0, st, list, n, nbas, flags);
+    return hipGetLastError();
+}
+
+// Last quarter transform of the (ov|ov) blocks: the unit-cell index a with
+// conj C_k1(a, p), k1 = k4 - k3 + k2, from
+//   Y[a + nf (q + nv (k2 + N (r + o_hi (k3 + N (s + nv k4)))))]    (split complex)
+// into out[pq + per (rs + per (k1 + N (k2 + N k3)))], pq = p + o_hi q, rs = r + o_hi s,
+// per = o_hi nv (the layout k_kmp2_sums reads).
+__global__ __launch_bounds__(256) void k_kmo_last(const double *__restrict__ Yr, const double *__restrict__ Yi,
+                                                  const double2 *__restrict__ Ck, int nf, int N, KGrid g, int o_hi,
+                                                  int v_lo, double2 *__restrict__ out) {
+    const int nv = nf - v_lo, per = o_hi * nv;
+    const long long total = (long long)per * per * N * N * N;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r_ = t;
+        const int p = (int)(r_ % o_hi); r_ /= o_hi;
+        const int q = (int)(r_ % nv); r_ /= nv;
+        const int r = (int)(r_ % o_hi); r_ /= o_hi;
+        const int s = (int)(r_ % nv); r_ /= nv;
+        const int k1 = (int)(r_ % N); r_ /= N;
+        const int k2 = (int)(r_ % N);
+        const int k3 = (int)(r_ / N);
+        const int k4 = g.add(k1, k2, k3);
+        const size_t yb = (size_t)nf * (q + (size_t)nv * (k2 + (size_t)N * (r + (size_t)o_hi * (k3 + (size_t)N * (s + (size_t)nv * k4)))));
+        const double2 *C1 = Ck + (size_t)k1 * nf * nf + (size_t)nf * p;
+        double2 acc{0.0, 0.0};
+        for (int a = 0; a < nf; ++a) {
+            const double2 y{Yr[yb + a], Yi[yb + a]};
+            const double2 m = cmulc(C1[a], y);
+            acc.x += m.x;
+            acc.y += m.y;
+        }
+        out[t] = acc;
+    }
+}
+
+hipError_t launch_kmo_last(const double *Yr, const double *Yi, const double2 *Ck, int nf, int N, int nx, int ny,
+                           int nz, int o_hi, int v_lo, double2 *out, hipStream_t st) {
+    KGrid g{nx, ny, nz};
+    const long long per = (long long)o_hi * (nf - v_lo);
+    const long long total = per * per * N * N * N;
+    const int grid = (int)std::min<long long>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_kmo_last, dim3(grid), dim3(256), 0, st, Yr, Yi, Ck, nf, N, g, o_hi, v_lo, out);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_kmp2_sums(const double2 *__restrict__ ekmo, const double *__restrict__ ek,
                                                    const int *__restrict__ occ, int nf, int N, KGrid g, int o_hi,
                                                    int v_lo, double *__restrict__ part,
