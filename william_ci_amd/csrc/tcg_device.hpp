@@ -95,7 +95,8 @@ size_t contract_lds_bytes(const EriArgs &a);
 hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
                             long long ns, double *realval, hipStream_t st);
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                              const double *rowval, const double *realval, hipStream_t st);
+                              const double *rowval, const double *realval, const int *list, const unsigned *cnt,
+                              hipStream_t st);
 // lists: 9 * a.nq4 ints, cnt: 9 counters (work space of the s/p contraction)
 // (also forms the real-direction rows, in realval, of the quartets that need them)
 hipError_t launch_contract(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,

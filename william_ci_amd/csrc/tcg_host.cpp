@@ -513,15 +513,11 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev3, c->st));
     const int nreal = 3 - A.ndp;
     if (nreal > 0) HIPCHK(c, c->d_realval.alloc((size_t)(nr / A.ndp) * nreal * A.nqp));
-    if (A.Jacc) {
-        if (nreal > 0) HIPCHK(c, launch_real_rows(A, so, ro, c->d_surv.p, ns, c->d_realval.p, c->st));
-        HIPCHK(c, launch_contract_jk(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->st));
-    } else {
-        HIPCHK(c, c->d_cls.alloc(9 * (size_t)A.nq4));      // one list per s/p form and one of the rest
-        HIPCHK(c, c->d_clscnt.alloc(9));
-        HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->d_cls.p, c->d_clscnt.p,
-                                  c->st));
-    }
+    // the block of every quartet goes to the ERI array, or (direct) into J and K
+    HIPCHK(c, c->d_cls.alloc(9 * (size_t)A.nq4));          // one list per s/p form and one of the rest
+    HIPCHK(c, c->d_clscnt.alloc(9));
+    HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->d_cls.p, c->d_clscnt.p,
+                              c->st));
     if (c->time_rows) {
         float ms = 0.f;
         HIPCHK(c, hipEventSynchronize(c->ev3));

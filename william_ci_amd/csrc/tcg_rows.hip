@@ -1138,55 +1138,6 @@ __global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict
 // the s/p quartets of one form (P p shells, NBP in the bra; list of cnt
 // entries), GL lanes per quartet, 256 / GL quartets per workgroup; one launch
 // per form keeps each kernel's registers to what its form needs
-template <int NDP, int P, int NBP, int GL>
-__global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__restrict__ list,
-                                                     const unsigned *__restrict__ cnt,
-                                                     const long long *__restrict__ soff,
-                                                     const long long *__restrict__ roff,
-                                                     const Surv *__restrict__ surv, const double *__restrict__ rowval) {
-    constexpr int NG = 256 / GL;
-    __shared__ double lds[NG * pow3(P)];
-    const int grp = threadIdx.x / GL, gl = threadIdx.x & (GL - 1);
-    double *blk = lds + grp * pow3(P);
-    const long n = *cnt;
-    for (long e0 = (long)blockIdx.x * NG; e0 < n; e0 += (long)gridDim.x * NG) {
-        const long e = e0 + grp;
-        const int qi = e < n ? list[e] : 0;
-        const Quartet Q = A.q4[qi];
-        if (e < n) {
-            const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
-            quartet_block_sp<NDP, P, NBP, GL>(A, qi, Q, s0, ns, r0, surv, rowval, blk, gl);
-        }
-        wave_sync();
-        if (e < n) write_block(A, Q, A.sh[Q.s[0]], A.sh[Q.s[1]], A.sh[Q.s[2]], A.sh[Q.s[3]], blk, gl, GL);
-        wave_sync();
-    }
-}
-
-template <int NDP, int P, int NBP, int GL>
-static void launch_sp1(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
-                       const long long *roff, const Surv *surv, const double *rowval, hipStream_t st) {
-    constexpr int L = sp_list(P, NBP);
-    const int ng = (b.nq4 + 256 / GL - 1) / (256 / GL), grid = ng < (1 << 20) ? ng : (1 << 20);
-    hipLaunchKernelGGL((k_contract_sp<NDP, P, NBP, GL>), dim3(grid), dim3(256), 0, st, b, lists + (size_t)L * b.nq4,
-                       cnt + L, soff, roff, surv, rowval);
-}
-
-template <int NDP>
-static void launch_sp_all(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
-                          const long long *roff, const Surv *surv, const double *rowval, hipStream_t st) {
-    launch_sp1<NDP, 0, 0, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-    launch_sp1<NDP, 1, 0, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-    launch_sp1<NDP, 1, 1, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-    launch_sp1<NDP, 2, 0, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-    launch_sp1<NDP, 2, 1, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-    launch_sp1<NDP, 2, 2, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-    launch_sp1<NDP, 3, 1, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-    launch_sp1<NDP, 3, 2, 16>(b, lists, cnt, soff, roff, surv, rowval, st);
-}
-
-size_t contract_lds_bytes(const EriArgs &a) { return 4 * sizeof(double) * ((size_t)a.max_blk + 64); }
-
 // ---------------------------------------------------------------------------
 // Direct J/K (src/module/eri_direct.f90:262-374): every stored element of a
 // block once, J(p1,p2) += E P(p3,p4), K(p1,p4) -= E P(p2,p3) / 2.
@@ -1228,10 +1179,149 @@ __device__ __forceinline__ bool first_generator(const EriArgs &A, const Shell &s
 }
 
 
+// True when a permutation other than the identity, with the translation that
+// brings its first index to cell 0, maps the shell quartet onto itself: only
+// then can two (component, permutation) pairs of the quartet reach one stored
+// position, and only then is first_generator needed.
+__device__ __forceinline__ bool shell_stabilized(const EriArgs &A, const Shell &sA, const Shell &sB, const Shell &sC,
+                                                 const Shell &sD) {
+    const int f[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
+    bool stab = false;
+#pragma unroll
+    for (int g = 1; g < 8; ++g) {
+        int pp[4];
+        stored_position(A, f, g, pp);
+        stab = stab || (pp[0] == f[0] && pp[1] == f[1] && pp[2] == f[2] && pp[3] == f[3]);
+    }
+    return stab;
+}
+
+// J and K of one finished block (the direct path): every counted element adds
+// J(p1,p2) += E P(p3,p4), K(p1,p4) -= E P(p2,p3) / 2 into Jl, Kl (LDS or global,
+// leading dimension nfuc), by nl lanes.
+__device__ __forceinline__ void jk_block(const EriArgs &A, const Shell &sA, const Shell &sB, const Shell &sC,
+                                         const Shell &sD, const double *blk, int lane, int nl, bool stab, double *Jl,
+                                         double *Kl) {
+    const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
+    const int nblk = nA * nB * nC * nD;
+    const int nf = A.nfuc, nb = A.nbas;
+    for (int task = lane; task < nblk * 8; task += nl) {
+        const int comp = task >> 3, ip = task & 7;
+        int t = comp;
+        const int ca = t % nA; t /= nA;
+        const int cb = t % nB; t /= nB;
+        const int cc = t % nC;
+        const int cd = t / nC;
+        const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
+        if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz) continue;
+        int pp[4];
+        stored_position(A, f4, ip, pp);
+        if (stab && !first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
+        const double val = blk[comp];
+        atomicAdd(&Jl[pp[0] + (size_t)nf * pp[1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
+        atomicAdd(&Kl[pp[0] + (size_t)nf * pp[3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
+    }
+}
+
+// JK = 0: the block goes to the ERI array (write_block); JK = 1: its J and K
+// contributions (jk_block), into LDS copies of J and K flushed at the end when
+// jk_lds, else straight into the global arrays.
+template <int NDP, int P, int NBP, int GL, int JK>
+__global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__restrict__ list,
+                                                     const unsigned *__restrict__ cnt,
+                                                     const long long *__restrict__ soff,
+                                                     const long long *__restrict__ roff,
+                                                     const Surv *__restrict__ surv, const double *__restrict__ rowval,
+                                                     int jk_lds) {
+    constexpr int NG = 256 / GL;
+    __shared__ double lds[NG * pow3(P)];
+    extern __shared__ double jkl[];
+    const int grp = threadIdx.x / GL, gl = threadIdx.x & (GL - 1);
+    double *blk = lds + grp * pow3(P);
+    const size_t njk = (size_t)A.nfuc * A.nbas;
+    double *Jl = (JK && jk_lds) ? jkl : A.Jacc, *Kl = (JK && jk_lds) ? jkl + njk : A.Kacc;
+    if (JK && jk_lds) {
+        for (size_t i = threadIdx.x; i < 2 * njk; i += 256) jkl[i] = 0.0;
+        __syncthreads();
+    }
+    const long n = *cnt;
+    for (long e0 = (long)blockIdx.x * NG; e0 < n; e0 += (long)gridDim.x * NG) {
+        const long e = e0 + grp;
+        const int qi = e < n ? list[e] : 0;
+        const Quartet Q = A.q4[qi];
+        if (e < n) {
+            const long s0 = soff[qi], ns = soff[qi + 1] - s0, r0 = roff[qi];
+            quartet_block_sp<NDP, P, NBP, GL>(A, qi, Q, s0, ns, r0, surv, rowval, blk, gl);
+        }
+        wave_sync();
+        if (e < n) {
+            const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
+            if (JK) jk_block(A, sA, sB, sC, sD, blk, gl, GL, shell_stabilized(A, sA, sB, sC, sD), Jl, Kl);
+            else write_block(A, Q, sA, sB, sC, sD, blk, gl, GL);
+        }
+        wave_sync();
+    }
+    if (JK && jk_lds) {
+        __syncthreads();
+        for (size_t i = threadIdx.x; i < njk; i += 256) {
+            if (jkl[i] != 0.0) atomicAdd(&A.Jacc[i], jkl[i]);
+            if (jkl[njk + i] != 0.0) atomicAdd(&A.Kacc[i], jkl[njk + i]);
+        }
+    }
+}
+
+// J/K copies in LDS per workgroup up to this size (TCG_SP_JK_LDS_KB, default 150
+// KiB: LDS atomics beat global ones even at one workgroup per CU); larger ones
+// are accumulated in global memory
+static size_t sp_jk_lds_bytes() {
+    static const size_t v = [] {
+        const char *e = getenv("TCG_SP_JK_LDS_KB");
+        const long kb = e ? atol(e) : 150;
+        return (size_t)(kb < 0 ? 0 : kb) * 1024;
+    }();
+    return v;
+}
+
+template <int NDP, int P, int NBP, int GL, int JK>
+static void launch_sp1(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
+                       const long long *roff, const Surv *surv, const double *rowval, hipStream_t st) {
+    constexpr int L = sp_list(P, NBP);
+    const size_t jk_bytes = JK ? sizeof(double) * 2 * (size_t)b.nfuc * b.nbas : 0;
+    const int jk_lds = JK && jk_bytes <= sp_jk_lds_bytes() ? 1 : 0;
+    int ng = (b.nq4 + 256 / GL - 1) / (256 / GL);
+    ng = ng < (1 << 20) ? ng : (1 << 20);
+    if (jk_lds) {
+        // persistent: each workgroup flushes its copy once
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / (jk_bytes + 4096)));
+        ng = ng < 256 * per_cu ? ng : 256 * per_cu;
+        if (jk_bytes > 64 * 1024)
+            (void)hipFuncSetAttribute((const void *)k_contract_sp<NDP, P, NBP, GL, JK>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)jk_bytes);
+    }
+    hipLaunchKernelGGL((k_contract_sp<NDP, P, NBP, GL, JK>), dim3(ng), dim3(256), jk_lds ? jk_bytes : 0, st, b,
+                       lists + (size_t)L * b.nq4, cnt + L, soff, roff, surv, rowval, jk_lds);
+}
+
+template <int NDP, int JK>
+static void launch_sp_all(const EriArgs &b, const int *lists, const unsigned *cnt, const long long *soff,
+                          const long long *roff, const Surv *surv, const double *rowval, hipStream_t st) {
+    launch_sp1<NDP, 0, 0, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 1, 0, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 1, 1, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 2, 0, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 2, 1, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 2, 2, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 3, 1, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+    launch_sp1<NDP, 3, 2, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
+}
+
+size_t contract_lds_bytes(const EriArgs &a) { return 4 * sizeof(double) * ((size_t)a.max_blk + 64); }
+
 __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long *__restrict__ soff,
                                                      const long long *__restrict__ roff,
                                                      const Surv *__restrict__ surv, const double *__restrict__ rowval,
-                                                     const double *__restrict__ realval, int jk_lds) {
+                                                     const double *__restrict__ realval, int jk_lds,
+                                                     const int *__restrict__ list, const unsigned *__restrict__ cnt) {
     // LDS: [4][max_blk] blocks, [4][2][36] per-wave J/K sub-blocks, [4][64] partial
     // sums of the contraction, then J[nfuc*nbas], K[nfuc*nbas] when jk_lds
     extern __shared__ double lds[];
@@ -1247,7 +1337,9 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
     }
     __syncthreads();
-    for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
+    const int nlist = list ? (int)*cnt : A.nq4;
+    for (int e = blockIdx.x * 4 + wv; e < nlist; e += gridDim.x * 4) {
+        const int qi = list ? list[e] : e;
         const Quartet Q = A.q4[qi];
         const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
         const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
@@ -1256,25 +1348,10 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         wave_sync();
         quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, part, lane);
         wave_sync();
+        const bool stab = shell_stabilized(A, sA, sB, sC, sD);
         if (jk_lds) {
             // J/K in LDS: every counted element adds there directly
-            for (int task = lane; task < nblk * 8; task += 64) {
-                const int comp = task >> 3, ip = task & 7;
-                int t = comp;
-                const int ca = t % nA; t /= nA;
-                const int cb = t % nB; t /= nB;
-                const int cc = t % nC;
-                const int cd = t / nC;
-                const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
-                if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz)
-                    continue;
-                int pp[4];
-                stored_position(A, f4, ip, pp);
-                if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
-                const double val = blk[comp];
-                atomicAdd(&Jl[pp[0] + (size_t)nf * pp[1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
-                atomicAdd(&Kl[pp[0] + (size_t)nf * pp[3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
-            }
+            jk_block(A, sA, sB, sC, sD, blk, lane, 64, stab, Jl, Kl);
             continue;
         }
         // J/K in global memory: one permutation at a time, the counted elements of the
@@ -1302,7 +1379,7 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                     continue;
                 int pp[4];
                 stored_position(A, f4, ip, pp);
-                if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
+                if (stab && !first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
                 const int c4[4] = {ca, cb, cc, cd};
                 const double val = blk[comp];
                 atomicAdd(&Jw[c4[x0] * n1 + c4[x1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
@@ -1332,7 +1409,8 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
 }
 
 hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                              const double *rowval, const double *realval, hipStream_t st) {
+                              const double *rowval, const double *realval, const int *list, const unsigned *cnt,
+                              hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
     const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 72 + 4 * 64);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
@@ -1353,7 +1431,8 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
     const int per_cu = jk_lds ? (int)std::max<size_t>(1, std::min<size_t>(4, lds_max / lds)) : 4;
     const int want = (a.nq4 + 3) / 4;
     const int grid = std::min(want, 256 * per_cu);
-    hipLaunchKernelGGL(k_contract_jk, dim3(grid), dim3(256), lds, st, a, soff, roff, surv, rowval, realval, jk_lds);
+    hipLaunchKernelGGL(k_contract_jk, dim3(grid), dim3(256), lds, st, a, soff, roff, surv, rowval, realval, jk_lds,
+                       list, cnt);
     return hipGetLastError();
 }
 
@@ -1380,11 +1459,16 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
         const int *rest = lists + (size_t)(kSpLists - 1) * b.nq4;
         e = real_rows(b, soff, roff, surv, realval, rest, cnt + (kSpLists - 1), st);
         if (e != hipSuccess) return e;
-        if (b.ndp == 1) launch_sp_all<1>(b, lists, cnt, soff, roff, surv, rowval, st);
-        else if (b.ndp == 2) launch_sp_all<2>(b, lists, cnt, soff, roff, surv, rowval, st);
-        else launch_sp_all<3>(b, lists, cnt, soff, roff, surv, rowval, st);
+        const bool jk = b.Jacc != nullptr;
+        if (b.ndp == 1) jk ? launch_sp_all<1, 1>(b, lists, cnt, soff, roff, surv, rowval, st)
+                           : launch_sp_all<1, 0>(b, lists, cnt, soff, roff, surv, rowval, st);
+        else if (b.ndp == 2) jk ? launch_sp_all<2, 1>(b, lists, cnt, soff, roff, surv, rowval, st)
+                                : launch_sp_all<2, 0>(b, lists, cnt, soff, roff, surv, rowval, st);
+        else jk ? launch_sp_all<3, 1>(b, lists, cnt, soff, roff, surv, rowval, st)
+                : launch_sp_all<3, 0>(b, lists, cnt, soff, roff, surv, rowval, st);
         // the rest (d shells, four p shells) from the last list (its length is on the
         // device: at most 4096 workgroups walk it)
+        if (jk) return launch_contract_jk(b, soff, roff, surv, rowval, realval, rest, cnt + (kSpLists - 1), st);
         hipLaunchKernelGGL(k_contract, dim3(grid < 4096 ? grid : 4096), dim3(256), lds, st, b, soff, roff, surv,
                            rowval, realval, rest, cnt + (kSpLists - 1));
         return hipGetLastError();
@@ -1393,6 +1477,7 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
         const hipError_t e = real_rows(b, soff, roff, surv, realval, nullptr, nullptr, st);
         if (e != hipSuccess) return e;
     }
+    if (b.Jacc) return launch_contract_jk(b, soff, roff, surv, rowval, realval, nullptr, nullptr, st);
     hipLaunchKernelGGL(k_contract, dim3(grid), dim3(256), lds, st, b, soff, roff, surv, rowval, realval, nullptr,
                        nullptr);
     return hipGetLastError();
