@@ -93,3 +93,21 @@ def test_rhf_direct_equals_stored():
     eng.close()
     print(f"  E stored {e_st:.12f}  direct {e_di:.12f}  diff {e_di - e_st:.2e}")
     assert abs(e_di - e_st) < 1e-10
+
+
+def test_direct_jk_unchanged_by_block_requests():
+    """A shell-block request (tcg_eri_blocks) between two direct J/K builds leaves
+    the direct quartet list alone: the second build equals the first."""
+    from william_ci_amd.engine import Engine
+    from william_ci_amd.learned import eri_blocks
+    s = tsys.lattice_2d_h2(3, 3.6, 10.8)
+    print("case h2_3x3_2d direct J/K around a block request", flush=True)
+    eng = Engine(s, normalised_coef(s))
+    eng.direct_init()
+    P = _density(s.nbas, seed=4)
+    J1, K1 = eng.direct_jk(P)
+    q = np.array([[0, 1, 2, 3], [0, 0, 5, 7], [1, 2, 1, 2]], dtype=np.int32)
+    eri_blocks(eng, q)
+    J2, K2 = eng.direct_jk(P)
+    eng.close()
+    assert _rel(J2, J1) < 1e-13 and _rel(K2, K1) < 1e-13

@@ -77,7 +77,8 @@ struct tcg_ctx {
     DevBuf<PrimPair> d_pp;
     DevBuf<ShellPair> d_sp;
     DevBuf<double2> d_vec;
-    DevBuf<Quartet> d_q4;
+    DevBuf<Quartet> d_q4;        // the unique quartet list (ERI build, direct J/K)
+    DevBuf<Quartet> d_q4blk;     // quartets of a block request (Schwarz pass, tcg_eri_blocks, self-check)
     DevBuf<unsigned long long> d_cnt, d_qmax;
     // row pipeline
     DevBuf<long long> d_scount, d_rcount, d_soff, d_roff;
@@ -617,13 +618,15 @@ static int run_quartets(tcg_ctx *c, EriArgs A, long long ns_ub = -1, long long n
 }
 
 // blocks of q4 (each at its q.out) into d_blk; copied to *out unless out is null
+// Blocks of the listed quartets into d_blk (and out).  The list goes to its own
+// buffer: the unique quartet list of the direct J/K and of the build stays valid.
 static int run_blocks(tcg_ctx *c, const std::vector<Quartet> &q4, size_t nout, std::vector<double> *out, int prefactor) {
-    HIPCHK(c, upload_async(c, c->d_q4, q4));
+    HIPCHK(c, upload_async(c, c->d_q4blk, q4));
     if (nout > c->d_blk.n) HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, c->d_blk.alloc(nout));
     HIPCHK(c, launch_fill(c->d_blk.p, nout, 0.0, c->st));     // blocks without survivors stay zero
     EriArgs A = base_args(c);
-    A.q4 = c->d_q4.p;
+    A.q4 = c->d_q4blk.p;
     A.nq4 = (int)q4.size();
     A.blk_out = c->d_blk.p;
     A.screen_prefactor = prefactor;
@@ -995,7 +998,7 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3, bool defer = false
     HIPCHK(c, c->d_qmax.alloc(1));
     HIPCHK(c, launch_fill(c->d_Q.p, nbb, 0.0, c->st));
     HIPCHK(c, hipMemsetAsync(c->d_qmax.p, 0, sizeof(unsigned long long), c->st));
-    HIPCHK(c, launch_schwarz_tables(c->d_q4.p, (int)q4.size(), (int)nshell_q, c->d_sh.p, c->d_sp.p, c->d_blk.p,
+    HIPCHK(c, launch_schwarz_tables(c->d_q4blk.p, (int)q4.size(), (int)nshell_q, c->d_sh.p, c->d_sp.p, c->d_blk.p,
                                     c->d_relc.p, nf, nb, nc, nsh, c->d_Q.p, c->d_Qsh.p, c->d_pp.p, c->d_qmax.p, c->st));
     HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
     t2 = now_ms();

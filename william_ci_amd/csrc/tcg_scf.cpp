@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -290,6 +292,11 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     const double t_start = wall_ms();
     hipStream_t st = ctx_stream(c);
     const size_t nn = (size_t)nb * nb, len = (size_t)N * nf * nf;
+    // TCG_SCF_LOG=1: one line per iteration on stderr (the reference's SCF table)
+    static const bool log_it = [] {
+        const char *e = std::getenv("TCG_SCF_LOG");
+        return e && e[0] == '1';
+    }();
 
     // --- set-up: S_k, X_k, T^T, V^T, unit-cell rows of Hc = T + V ---
     ScfWork *w = nullptr;
@@ -428,6 +435,11 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
         for (int q = 0; q < 4; ++q) parts[q] = hsp[q];
         E = parts[0] + parts[1] + parts[2] + parts[3];
         if (it > 1) conv = hsp[4];
+        if (log_it) {
+            std::fprintf(stderr, "tcg_rhf_sao: SCF %3d  E_el = %.12f  conv = %.3e  (%.1f ms)\n", it, E, conv,
+                         wall_ms() - t_start);
+            std::fflush(stderr);
+        }
         if (conv < thresh) {
             converged = true;
             break;
