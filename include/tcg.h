@@ -267,10 +267,20 @@ int tcg_eri_blocks(tcg_ctx *ctx, int n, const int *shells, double *out, long lon
  *   norm:   24 floats, feature mean[6], std[6], lowest[6], highest[6]
  *   orders: norders (<= 15) ascending node counts (4..128)
  * Trained on Tori1D systems: tcg_eri_build_learned refuses ndp > 1.
- * rule_counts (norders + 1 entries, may be NULL): shell quartets per order,
- * the last entry those on the plain rule. */
+ * rule_counts (norders + 2 entries, may be NULL): shell quartets per order,
+ * then those on the plain rule, then those skipped (tcg_set_learned_skip).
+ *
+ * tcg_set_learned_skip: learned Schwarz-style skipping in the next learned
+ * builds.  A shell quartet all of whose primitive quartets are in the training
+ * range is left out (its elements stay zero) when the bound head's largest
+ * log10 |integral| over them, plus log10 of their number and margin_b, is below
+ * log10(thresh) and (schwarz_cap > 0) its Schwarz product Qsh(bra) Qsh(ket) is
+ * below schwarz_cap: no skipped integral then exceeds schwarz_cap.  The kept
+ * quartets pass the reference's screens as before.  thresh = 0 turns skipping
+ * off (the default). */
 int tcg_set_learned(tcg_ctx *ctx, const float *params, const float *norm, const int *orders, int norders,
                     float margin);
+int tcg_set_learned_skip(tcg_ctx *ctx, double thresh, double schwarz_cap, float margin_b);
 int tcg_eri_build_learned(tcg_ctx *ctx, int sao, int physicist, double *eri, long long *rule_counts);
 
 /* The network itself (no context): parameter count, inference, one

@@ -231,6 +231,13 @@ module tcg_engine
       integer(c_int), value :: norders
       real(c_float), value :: margin
     end function
+    integer(c_int) function tcg_set_learned_skip(ctx, thresh, schwarz_cap, margin_b) &
+        bind(C, name="tcg_set_learned_skip")
+      import :: c_int, c_double, c_float, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: thresh, schwarz_cap
+      real(c_float), value :: margin_b
+    end function
     integer(c_int) function tcg_eri_build_learned(ctx, sao, physicist, eri, rule_counts) &
         bind(C, name="tcg_eri_build_learned")
       import :: c_int, c_ptr

@@ -92,7 +92,7 @@ def test_learned_build_with_infinite_margin_is_plain():
     E1, counts = learned.eri_build_learned(eng, m)
     eng.close()
     print(f"  counts {counts.tolist()}  max |diff| {np.abs(E1 - E0).max():.2e}")
-    assert counts[-1] == counts.sum() and np.abs(E1 - E0).max() <= 1e-15
+    assert counts[len(m.orders)] == counts.sum() and np.abs(E1 - E0).max() <= 1e-15
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(__file__)), "william_ci_amd", "data",
@@ -111,10 +111,47 @@ def test_learned_build_energy_within_microhartree():
     m.install(eng)
     E1, counts = learned.eri_build_learned(eng, m)
     e1 = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P)).energy
+    # spot check: sampled shell quartets against a converged rule (40 mapped points,
+    # the 128-point table to ~1e-14); the network's target is 1e-10 per integral
+    spot = learned.spot_check(eng, E1, n=256)
+    spot0 = learned.spot_check(eng, E0, n=256)
     eng.close()
-    print(f"  counts {counts.tolist()}  max |dERI| {np.abs(E1 - E0).max():.2e}  dE {e1 - e0:.2e}")
+    print(f"  counts {counts.tolist()}  max |dERI| {np.abs(E1 - E0).max():.2e}  dE {e1 - e0:.2e}  "
+          f"spot check {spot:.2e} (plain 64-point table {spot0:.2e})")
     assert np.abs(E1 - E0).max() <= 1e-8
     assert abs(e1 - e0) <= 1e-6
+    assert spot <= 1e-10
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(__file__)), "william_ci_amd", "data",
+                                                    "quartet_orders.npz")), reason="no trained model committed")
+def test_learned_skipping():
+    """Skipping by the bound head at 1e-13 inside the Schwarz band below 1e-10:
+    the skipped quartets are counted, they leave zeros and change nothing else,
+    every element they held is below the cap (the Schwarz inequality), and the
+    energy moves by far less than 1 uEh."""
+    from william_ci_amd import learned, scf
+    from william_ci_amd.engine import Engine
+    s = tsys.workload("h64_chain")
+    print("case learned skipping h64_chain", flush=True)
+    eng = Engine(s, normalised_coef(s))
+    S, T, V = eng.one_electron()
+    m = learned.Model.load()
+    m.install(eng)
+    E1, c1 = learned.eri_build_learned(eng, m)
+    e1 = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P)).energy
+    learned.set_skip(eng, 1e-13)
+    E2, c2 = learned.eri_build_learned(eng, m)
+    e2 = scf.rhf_sao(s, S, T, V, lambda P: eng.jk_stored(P)).energy
+    learned.set_skip(eng, 0.0)
+    eng.close()
+    nord = len(m.orders)
+    d = np.abs(E2 - E1)
+    print(f"  skipped {c2[nord + 1]} of {c2.sum()} shell quartets; max |dERI| {d.max():.2e}; dE {e2 - e1:.2e}")
+    assert c1[nord + 1] == 0 and c2.sum() == c1.sum()
+    assert np.all(E2[E2 != E1] == 0.0)          # a skipped quartet leaves zeros, the rest is unchanged
+    assert d.max() < 1e-10
+    assert abs(e2 - e1) < 1e-8
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(__file__)), "william_ci_amd", "data",

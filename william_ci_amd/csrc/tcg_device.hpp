@@ -136,8 +136,14 @@ int mlp_param_count();
 hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, const Surv *surv, const double *expo,
                            const float *norm, float *feat, unsigned char *ood, hipStream_t st);
 hipError_t launch_surv_owner(const long long *soff, int nq4, int *owner, hipStream_t st);
+constexpr int kRuleSkip = -1;     // a shell quartet the learned bound leaves out
 hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const unsigned char *ood, float margin,
-                        int plain, int *qrule, hipStream_t st);
+                        int plain, float log_skip, float margin_b, const Quartet *q4, const double *Qsh, int nsh,
+                        double cap, int *qrule, hipStream_t st);
+hipError_t launch_rule_keep(int nq4, const int *qrule, unsigned char *keep, hipStream_t st);
+hipError_t launch_gather_quartets(const Quartet *q4, const int *qrule, const long long *idx,
+                                  const unsigned long long *n, long long nmax, Quartet *q4_out, int *rule_out,
+                                  hipStream_t st);
 hipError_t launch_rule_hist(int nq4, const int *qrule, unsigned long long *hist, hipStream_t st);
 hipError_t launch_mlp_fwd(const float *P, const float *X, int n, float *Y, hipStream_t st);
 hipError_t launch_mlp_step(const float *P, const float *X, const float *T, int n, float inv_total, float w0, float w1,

@@ -126,6 +126,11 @@ struct tcg_ctx {
     std::vector<float> mlp_params, mlp_norm;           // network, feature mean/std and limits
     std::vector<int> learned_orders;                   // index -> mapped Gauss-Legendre order
     float learned_margin = 1.0f;
+    double learned_skip = 0.0;         // skip threshold on the learned bound (0: no skipping)
+    double learned_skip_cap = 0.0;     // ... only below this Schwarz product (0: no cap)
+    float learned_margin_b = 1.5f;     // margin on the bound head, log10 units
+    DevBuf<Quartet> d_q4keep;          // the quartets a learned build evaluates (skipped ones removed)
+    DevBuf<int> d_qrule_keep;
     DevBuf<float> d_mlp, d_feat, d_pred;
     DevBuf<int> d_qrule;
     DevBuf<double> d_expo;
@@ -869,7 +874,7 @@ static void fill_translation(const tcg_ctx *c, double *M) {
 static int learned_pass(tcg_ctx *c, EriArgs A) {
     const int nq4 = A.nq4;
     const int nord = (int)c->learned_orders.size();
-    for (int r = 0; r <= nord && r < 16; ++r) c->learned_counts[r] = 0;
+    for (int r = 0; r <= nord + 1 && r < 16; ++r) c->learned_counts[r] = 0;
     if (nq4 <= 0) return 0;
     HIPCHK(c, c->d_scount.alloc(nq4 + 1));
     HIPCHK(c, c->d_rcount.alloc(nq4 + 1));
@@ -898,8 +903,12 @@ static int learned_pass(tcg_ctx *c, EriArgs A) {
                                   c->d_ood.p, c->st));
         HIPCHK(c, launch_mlp_fwd(c->d_mlp.p, c->d_feat.p, (int)ns, c->d_pred.p, c->st));
     }
-    HIPCHK(c, launch_rules(nq4, c->d_soff.p, c->d_pred.p, c->d_ood.p, c->learned_margin, nord, c->d_qrule.p, c->st));
-    // quartets per rule (rule 0 = the plain rule)
+    const bool skipping = c->learned_skip > 0.0;
+    const float log_skip = skipping ? (float)std::log10(c->learned_skip) : -3e30f;
+    HIPCHK(c, launch_rules(nq4, c->d_soff.p, c->d_pred.p, c->d_ood.p, c->learned_margin, nord, log_skip,
+                           c->learned_margin_b, A.q4, c->d_Qsh.p, c->nsh, c->learned_skip_cap, c->d_qrule.p,
+                           c->st));
+    // quartets per rule (rule 0 = the plain rule; slot 15 = kRuleSkip & 15: the skipped ones)
     HIPCHK(c, c->d_hist.alloc(16));
     HIPCHK(c, launch_fill((double *)c->d_hist.p, 16, 0.0, c->st));
     HIPCHK(c, launch_rule_hist(nq4, c->d_qrule.p, c->d_hist.p, c->st));
@@ -907,10 +916,35 @@ static int learned_pass(tcg_ctx *c, EriArgs A) {
     HIPCHK(c, hipMemcpyAsync(hist, c->d_hist.p, sizeof hist, hipMemcpyDeviceToHost, c->st));
     // one pass over all quartets, each with its rule (rows sorted by rule, direction, length)
     A.qrule = c->d_qrule.p;
-    const int rc = run_quartets(c, A);
+    if (skipping) {
+        // the skipped quartets leave the list (their elements stay zero in the array)
+        DevBuf<unsigned char> keep;
+        DevBuf<long long> idx, nsel;
+        HIPCHK(c, keep.alloc(nq4));
+        HIPCHK(c, idx.alloc(nq4));
+        HIPCHK(c, nsel.alloc(1));
+        HIPCHK(c, launch_rule_keep(nq4, c->d_qrule.p, keep.p, c->st));
+        const size_t tb2 = select_temp_bytes(nq4);
+        HIPCHK(c, c->d_tmp.alloc(tb2));
+        HIPCHK(c, select_flagged(c->d_tmp.p, tb2, keep.p, idx.p, nsel.p, nq4, c->st));
+        HIPCHK(c, c->d_q4keep.alloc(nq4));
+        HIPCHK(c, c->d_qrule_keep.alloc(nq4));
+        HIPCHK(c, launch_gather_quartets(A.q4, c->d_qrule.p, idx.p,
+                                         reinterpret_cast<const unsigned long long *>(nsel.p), nq4, c->d_q4keep.p,
+                                         c->d_qrule_keep.p, c->st));
+        long long kept = 0;
+        HIPCHK(c, hipMemcpyAsync(&kept, nsel.p, sizeof kept, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        A.q4 = c->d_q4keep.p;
+        A.nq4 = (int)kept;
+        A.nq4_dev = nullptr;
+        A.qrule = c->d_qrule_keep.p;
+    }
+    const int rc = A.nq4 > 0 ? run_quartets(c, A) : 0;
     if (rc) return rc;
     for (int r = 0; r < nord; ++r) c->learned_counts[r] = (long long)hist[r + 1];
     c->learned_counts[nord] = (long long)hist[0];
+    c->learned_counts[nord + 1] = (long long)hist[15];
     return 0;
 }
 
@@ -1791,6 +1825,16 @@ int tcg_set_learned(tcg_ctx *c, const float *params, const float *norm, const in
     return TCG_OK;
 }
 
+int tcg_set_learned_skip(tcg_ctx *c, double thresh, double schwarz_cap, float margin_b) {
+    if (!c) return TCG_E_ARG;
+    if (!(thresh >= 0.0) || !(schwarz_cap >= 0.0) || !(margin_b >= 0.0f))
+        return c->fail(TCG_E_ARG, "tcg_set_learned_skip: bad arguments");
+    c->learned_skip = thresh;
+    c->learned_skip_cap = schwarz_cap;
+    c->learned_margin_b = margin_b;
+    return TCG_OK;
+}
+
 int tcg_eri_build_learned(tcg_ctx *c, int sao, int physicist, double *eri, long long *rule_counts) {
     if (!c) return TCG_E_ARG;
     if (c->learned_orders.empty()) return c->fail(TCG_E_STATE, "tcg_eri_build_learned before tcg_set_learned");
@@ -1800,7 +1844,7 @@ int tcg_eri_build_learned(tcg_ctx *c, int sao, int physicist, double *eri, long 
     c->learned_on = false;
     if (rc) return rc;
     if (rule_counts)
-        for (size_t r = 0; r <= c->learned_orders.size(); ++r) rule_counts[r] = c->learned_counts[r];
+        for (size_t r = 0; r <= c->learned_orders.size() + 1; ++r) rule_counts[r] = c->learned_counts[r];
     return TCG_OK;
 }
 

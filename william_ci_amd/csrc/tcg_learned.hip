@@ -80,25 +80,63 @@ __global__ __launch_bounds__(256) void k_features(EriArgs A, const int *__restri
     }
 }
 
+// Skipping (log_skip > -1e30): a shell quartet whose primitive quartets all lie in
+// the training range is left out (rule kRuleSkip) when the bound head's largest
+// log10 |integral| over them, plus log10 of their number and margin_b, is below
+// log_skip, and its Schwarz product Qsh(bra) Qsh(ket) is below cap (cap > 0): the
+// learned estimate decides inside the band [1e-13, cap) of the Schwarz screen,
+// whose inequality keeps every skipped integral below cap.
 __global__ __launch_bounds__(64) void k_rules(int nq4, const long long *__restrict__ soff, const float *__restrict__ pred,
                                               const unsigned char *__restrict__ ood, float margin, int plain,
+                                              float log_skip, float margin_b, const Quartet *__restrict__ q4,
+                                              const double *__restrict__ Qsh, int nsh, double cap,
                                               int *__restrict__ qrule) {
     for (int qi = blockIdx.x; qi < nq4; qi += gridDim.x) {
         int r = 0;
+        float bmax = -1e30f;
         for (long long g = soff[qi] + threadIdx.x; g < soff[qi + 1]; g += 64) {
             int rg = plain;
             if (!ood[g]) {
                 const float y = pred[g * 2 + 1] + margin;
                 rg = y <= 0.f ? 0 : (int)ceilf(y);
                 rg = rg > plain ? plain : rg;
+                bmax = fmaxf(bmax, pred[g * 2]);
+            } else {
+                bmax = 1e30f;                        // outside the training range: never skipped
             }
             r = r > rg ? r : rg;
         }
         for (int off = 32; off > 0; off >>= 1) {
             const int o = __shfl_xor(r, off, 64);
             r = r > o ? r : o;
+            bmax = fmaxf(bmax, __shfl_xor(bmax, off, 64));
         }
-        if (threadIdx.x == 0) qrule[qi] = r >= plain ? 0 : r + 1;
+        if (threadIdx.x == 0) {
+            const long long n = soff[qi + 1] - soff[qi];
+            bool skip = n > 0 && log_skip > -1e30f && bmax + log10f((float)n) + margin_b < log_skip;
+            if (skip && cap > 0.0) {
+                const Quartet &Q = q4[qi];
+                skip = Qsh[(size_t)Q.s[0] + (size_t)nsh * Q.s[1]] * Qsh[(size_t)Q.s[2] + (size_t)nsh * Q.s[3]] < cap;
+            }
+            qrule[qi] = skip ? kRuleSkip : (r >= plain ? 0 : r + 1);
+        }
+    }
+}
+
+// keep[i] = 1 when quartet i is evaluated (its rule is not kRuleSkip)
+__global__ void k_rule_keep(int nq4, const int *__restrict__ qrule, unsigned char *__restrict__ keep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq4) keep[i] = qrule[i] == kRuleSkip ? 0 : 1;
+}
+
+// q4_out[j] = q4[idx[j]], rule_out[j] = qrule[idx[j]]
+__global__ void k_gather_quartets(const Quartet *__restrict__ q4, const int *__restrict__ qrule,
+                                  const long long *__restrict__ idx, const unsigned long long *__restrict__ n,
+                                  Quartet *__restrict__ q4_out, int *__restrict__ rule_out) {
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < (long long)*n) {
+        q4_out[j] = q4[idx[j]];
+        rule_out[j] = qrule[idx[j]];
     }
 }
 
@@ -123,10 +161,27 @@ hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, con
 }
 
 hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const unsigned char *ood, float margin,
-                        int plain, int *qrule, hipStream_t st) {
+                        int plain, float log_skip, float margin_b, const Quartet *q4, const double *Qsh, int nsh,
+                        double cap, int *qrule, hipStream_t st) {
     if (nq4 <= 0) return hipSuccess;
     const int grid = nq4 < (1 << 20) ? nq4 : (1 << 20);
-    hipLaunchKernelGGL(k_rules, dim3(grid), dim3(64), 0, st, nq4, soff, pred, ood, margin, plain, qrule);
+    hipLaunchKernelGGL(k_rules, dim3(grid), dim3(64), 0, st, nq4, soff, pred, ood, margin, plain, log_skip, margin_b,
+                       q4, Qsh, nsh, cap, qrule);
+    return hipGetLastError();
+}
+
+hipError_t launch_rule_keep(int nq4, const int *qrule, unsigned char *keep, hipStream_t st) {
+    hipLaunchKernelGGL(k_rule_keep, dim3((nq4 + 255) / 256 > 0 ? (nq4 + 255) / 256 : 1), dim3(256), 0, st, nq4, qrule,
+                       keep);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_quartets(const Quartet *q4, const int *qrule, const long long *idx,
+                                  const unsigned long long *n, long long nmax, Quartet *q4_out, int *rule_out,
+                                  hipStream_t st) {
+    const long long g = (nmax + 255) / 256;
+    hipLaunchKernelGGL(k_gather_quartets, dim3((unsigned)(g > 0 ? g : 1)), dim3(256), 0, st, q4, qrule, idx, n, q4_out,
+                       rule_out);
     return hipGetLastError();
 }
 

@@ -58,6 +58,7 @@ def _lib():
                                       ctypes.c_int, ctypes.c_float]
         L.tcg_eri_build_learned.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                             np.ctypeslib.ndpointer(dtype=np.int64)]
+        L.tcg_set_learned_skip.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_float]
         L.tcg_eri_blocks.argtypes = [ctypes.c_void_p, ctypes.c_int, np.ctypeslib.ndpointer(dtype=np.int32),
                                      np.ctypeslib.ndpointer(dtype=np.float64), np.ctypeslib.ndpointer(dtype=np.int64)]
         L._mlp_bound = True
@@ -279,10 +280,19 @@ def eri_build_learned(eng, model: Model, sao=True, physicist=False, to_host=True
     nb = s.nbas
     n1 = s.nfuc if sao else nb
     out = np.empty(n1 * nb * nb * nb) if to_host else None
-    counts = np.zeros(len(model.orders) + 1, dtype=np.int64)
+    counts = np.zeros(len(model.orders) + 2, dtype=np.int64)      # per order, plain rule, skipped
     rc = _lib().tcg_eri_build_learned(eng.h, int(sao), int(physicist), out.ctypes.data if to_host else None, counts)
     eng._check(rc, "tcg_eri_build_learned")
     return (out.reshape(nb, nb, nb, n1) if to_host else None), counts
+
+
+def set_skip(eng, thresh: float, schwarz_cap: float = 1e-10, margin_b: float = 1.5) -> None:
+    """Learned Schwarz-style skipping in the next learned builds (tcg_set_learned_skip):
+    shell quartets whose predicted largest |integral| (bound head, plus log10 of
+    the primitive-quartet count and margin_b) is below thresh, and whose Schwarz
+    product is below schwarz_cap, are left out; thresh = 0 turns it off."""
+    eng._check(_lib().tcg_set_learned_skip(eng.h, float(thresh), float(schwarz_cap), float(margin_b)),
+               "tcg_set_learned_skip")
 
 
 def eri_blocks(eng, quartets: np.ndarray):
