@@ -307,6 +307,7 @@ def main():
                 "orbital_error": res.orbital_error, "wall_s": rhf_wall, "one_electron_s": t_one,
                 "jk_s": res.times["jk_s"], "eri_plus_rhf_wall_s": rhf_wall + ms_per_step / 1e3,
                 "loop": "tcg_rhf_sao (J/K, lattice transforms, per-k Jacobi, DIIS, density on the GPU)",
+                "loop_ms": res.times["gpu_loop_s"] * 1e3,
                 "host_loop_check": {"energy": res_host.energy, "dE": res.energy - res_host.energy,
                                     "iterations": res_host.iterations, "wall_s": host_wall}},
         "mp2": {"kind": "k-point MP2 (transform_ERI_AO_to_kMO + kMP2_calc)", "ec": mp2["ec"],

@@ -166,7 +166,7 @@ class Engine:
         V = np.zeros(nb * nb)
         self._check(lib().tcg_one_electron(self.h, s.natom, np.ascontiguousarray(s.geom).ravel(),
                                            np.ascontiguousarray(s.Z, dtype=np.int32), S, T, V), "tcg_one_electron")
-        f = lambda M: M.reshape(nb, nb).T.copy()
+        f = lambda M: M.reshape(nb, nb).T                  # the library's column-major arrays (views)
         return f(S), f(T), f(V)
 
     def refresh_basis(self, coef):
@@ -301,7 +301,7 @@ class Engine:
         (the resident SAO ERI) or 'direct' (after direct_init)."""
         nb = self.s.nbas
         src = {"stored": 0, "direct": 1}[source]
-        f = lambda M: np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel()
+        f = lambda M: np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel()   # no copy for F-ordered M
         e = np.zeros(nb)
         C = np.zeros(2 * nb * nb)
         occ = np.zeros(nb, dtype=np.int32)
@@ -312,7 +312,8 @@ class Engine:
         return {"e_hf": out[0], "ET": out[1], "EV": out[2], "EJ": out[3], "EK": out[4],
                 "iterations": int(out[5]), "conv": out[6], "converged": bool(out[7]), "stalled": bool(out[8]),
                 "gap": out[9], "jk_ms": out[10], "wall_ms": out[11],
-                "e": e, "coeff_sao": self._cplx_out(C, nb), "occ": occ, "P": P.reshape(nb, nb).T.copy()}
+                "e": e, "coeff_sao": C.view(np.complex128).reshape(nb, nb).T, "occ": occ,
+                "P": P.reshape(nb, nb).T}
 
 
 def ERI_integral_toroidal(sysm, coef, physicist: bool = False, sao: bool = True, nquad: int = 32):

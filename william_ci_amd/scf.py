@@ -38,15 +38,13 @@ LINDEP_SHIFT = 1e4
 
 def nuclear_repulsion(sysm) -> float:
     """Sum Z_i Z_j / d_ij with the chord distance along the periodic directions."""
-    g, Z = sysm.geom, sysm.Z.astype(np.float64)
-    e = 0.0
-    for i in range(sysm.natom - 1):
-        d = g[i] - g[i + 1:]
-        for k in range(sysm.ndp):
-            a = 2 * math.pi / sysm.L[k]
-            d[:, k] = np.sqrt(2.0 * (1.0 - np.cos(a * d[:, k])) / (a * a))
-        e += float(np.sum(Z[i] * Z[i + 1:] / np.sqrt(np.sum(d * d, axis=1))))
-    return e
+    g, Z = np.asarray(sysm.geom, dtype=np.float64), sysm.Z.astype(np.float64)
+    d = g[:, None, :] - g[None, :, :]                       # all pairs at once
+    for k in range(sysm.ndp):
+        a = 2 * math.pi / sysm.L[k]
+        d[:, :, k] = np.sqrt(2.0 * (1.0 - np.cos(a * d[:, :, k])) / (a * a))
+    i, j = np.triu_indices(sysm.natom, 1)
+    return float(np.sum(Z[i] * Z[j] / np.sqrt(np.sum(d[i, j] ** 2, axis=1))))
 
 
 def canonical_X(S: np.ndarray, thresh: float = LINDEP) -> np.ndarray:
@@ -293,7 +291,7 @@ def rhf_sao_gpu(sysm, eng, S: np.ndarray, T: np.ndarray, V: np.ndarray, source: 
     r = eng.rhf_sao(S, T, V, sysm.n_occ, source=source, max_scf=max_scf, thresh=thresh, diis=diis)
     enuc = nuclear_repulsion(sysm)
     C = r["coeff_sao"]
-    Ck = np.stack([C[k * nf:(k + 1) * nf, k * nf:(k + 1) * nf] for k in range(Nc)])
+    Ck = np.stack([C[k * nf:(k + 1) * nf, k * nf:(k + 1) * nf] for k in range(Nc)])   # copies of the k blocks
     return RHFResult(energy=r["e_hf"] + enuc, e_nuc=enuc, converged=r["converged"], iterations=r["iterations"],
                      orbital_error=r["conv"], mo_energies=r["e"].reshape(Nc, nf), mo_coef=Ck, P=r["P"],
                      occupied=(r["occ"] == 2).reshape(Nc, nf),
