@@ -223,11 +223,11 @@ def main():
         sys.exit(2)
 
     from william_ci_amd import system as tsys
-    from william_ci_amd.basis import normalize
+    from william_ci_amd.engine import normalized_coef
     from william_ci_amd.engine import Engine
 
     s = tsys.workload(args.workload)
-    coef = normalize(s)
+    coef = normalized_coef(s)                   # normalize_AO_toroidal on the GPU
     eng = Engine(s, coef)
     # start-up self-check of the fast rule (the reference runs it before any
     # integral, src/calculation_type/Tori1D.f90:88): it keeps the 32-point rule

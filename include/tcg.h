@@ -215,13 +215,22 @@ int tcg_df_times(tcg_df *df, double *out);
 const char *tcg_df_last_error(tcg_df *df);
 int tcg_df_destroy(tcg_df *df);
 
-/* One-electron matrices S, T, V (nBas,nBas): rows of the unit cell computed,
- * values below 1e-14 zeroed, the other rows filled by translation.  Replaces
+/* One-electron matrices S, T, V (nBas,nBas): rows of the unit cell computed on
+ * the GPU (S, T in closed form per direction, V by the t-quadrature), values below
+ * 1e-14 zeroed, the other rows filled by translation there.  Replaces
  *   overlap_matrix_toroidal_nD            e.g. src/Toroidal/1D/1D_Overlap_toroidal.f90:23
  *   kinetic_matrix_toroidal_nD            e.g. src/Toroidal/1D/1D_Kinetic_toroidal.f90:102
  *   nuclear_attraction_matrix_toroidal_nD e.g. src/Toroidal/1D/1D_Nuclear_attraction_toroidal.f90:169
  * xyz[3*c+d] nuclear positions, Z[c] charges (the supercell's atoms). */
 int tcg_one_electron(tcg_ctx *ctx, int natom, const double *xyz, const int *Z, double *S, double *T, double *V);
+
+/* Normalised contraction coefficients of an AO list (same structure of arrays
+ * as tcg_set_basis, raw coefficients in) on the context's torus.  Replaces
+ *   normalize_AO_toroidal_nD   e.g. src/basis/normalization/normalize_1D.f90:17-79
+ * every primitive normalised to one (closed-form toroidal self-overlap), then
+ * the contraction.  Exponents must be positive. */
+int tcg_normalize(tcg_ctx *ctx, int nbas, const double *center, const int *lxyz, const int *prim_off,
+                  const double *expo, const double *coef_raw, double *coef_out);
 
 /* Quadrature rule of every later call.  Replaces the module state
  *   fast_quad, nquad_point            src/module/precompute_t.f90:9,13

@@ -192,6 +192,16 @@ module tcg_engine
       real(c_double), intent(out) :: S(*), T(*), V(*)
     end function
     ! fast_quad / nquad_point and the start-up self-check
+    ! normalize_AO_toroidal_nD
+    integer(c_int) function tcg_normalize(ctx, nbas, center, lxyz, prim_off, expo, coef_raw, coef_out) &
+        bind(C, name="tcg_normalize")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbas
+      real(c_double), intent(in) :: center(3,*), expo(*), coef_raw(*)
+      integer(c_int), intent(in) :: lxyz(3,*), prim_off(*)
+      real(c_double), intent(out) :: coef_out(*)
+    end function
     integer(c_int) function tcg_set_quadrature(ctx, fast, nquad) bind(C, name="tcg_set_quadrature")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

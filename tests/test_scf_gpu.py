@@ -46,6 +46,26 @@ def test_one_electron_matches_reference(case):
         assert d < 1e-10, (name, d)
 
 
+@needs_ref
+@pytest.mark.parametrize("case", list(SYSTEMS))
+def test_normalization_matches_reference(case):
+    """tcg_normalize (closed-form self-overlaps on the GPU) against the
+    reference's normalize_AO_toroidal_nD and the independent quadrature of
+    tests_support."""
+    from oracle.ref import RefSystem
+    from tests_support import normalised_coef
+    from william_ci_amd.engine import normalized_coef
+    s = SYSTEMS[case]()
+    print(f"case {case} nbas={s.nbas}", flush=True)
+    c = normalized_coef(s)
+    cr = RefSystem(s).coef
+    cq = normalised_coef(s)
+    d = np.abs(c - cr).max() / np.abs(cr).max()
+    print(f"  vs reference {d:.2e}  vs quadrature {np.abs(c - cq).max() / np.abs(cq).max():.2e}", flush=True)
+    assert d < 1e-12
+    assert np.abs(c - cq).max() < 1e-10 * np.abs(cq).max()
+
+
 def test_overlap_diagonal_and_symmetry():
     s = tsys.workload("h10_ring")
     eng = _engine(s)

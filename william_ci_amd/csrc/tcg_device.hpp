@@ -132,6 +132,18 @@ hipError_t launch_dft_mo(const DftMoArgs &A, hipStream_t st);
 hipError_t launch_list_j2_flags(const long long *list, long long n, int nbas, unsigned char *flags, hipStream_t st);
 hipError_t launch_kmo_last(const double *Yr, const double *Yi, const double2 *Ck, int nf, int N, int nx, int ny,
                            int nz, int o_hi, int v_lo, double2 *out, hipStream_t st);
+// overlap/kinetic rows and the AO normalisation (tcg_oneel.hip)
+struct OneStArgs {
+    int ndp, nf, nb;
+    double a[3], L[3];
+    const double *center;     // [nb][3]
+    const int *lxyz, *poff;   // [nb][3], [nb + 1]
+    const double *expo, *coef;
+    double *S, *T;            // (nb, nb) column-major, rows < nf written
+};
+hipError_t launch_one_st(const OneStArgs &g, hipStream_t st);
+hipError_t launch_normalize(const OneStArgs &g, const double *raw, double *out, hipStream_t st);
+hipError_t launch_translate_fill(double *M, int nf, int nb, int nc, const int *relc, hipStream_t st);
 int mlp_param_count();
 hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, const Surv *surv, const double *expo,
                            const float *norm, float *feat, unsigned char *ood, hipStream_t st);

@@ -134,6 +134,7 @@ struct tcg_ctx {
     DevBuf<float> d_mlp, d_feat, d_pred;
     DevBuf<int> d_qrule;
     DevBuf<double> d_expo;
+    DevBuf<double> d_center, d_S1, d_T1;   // AO centres; S and T of the last tcg_one_electron
     DevBuf<VecJob> d_jobs;       // pair-vector jobs of the last preparation
     DevBuf<PairBlock> d_pblk;    // shell-pair blocks of the GPU pair set-up
     DevBuf<VecOut> d_vo;
@@ -693,6 +694,7 @@ static int prepare(tcg_ctx *c) {
     HIPCHK(c, upload_async(c, c->d_lxyz, c->lxyz));
     HIPCHK(c, upload_async(c, c->d_coef, c->coef));
     HIPCHK(c, upload_async(c, c->d_expo, c->expo));
+    HIPCHK(c, upload_async(c, c->d_center, c->center));
     HIPCHK(c, upload_async(c, c->d_poff, c->poff));
     HIPCHK(c, upload_async(c, c->d_sp, c->sp));
     HIPCHK(c, upload_async(c, c->d_pblk, pblk));
@@ -735,138 +737,6 @@ static int prepare(tcg_ctx *c) {
 // (src/module/clifford_tools.f90:295-330).  Real directions: Hermite E_0
 // overlap and m(m-1)S(m-2) - 2beta(2m+1)S(m) + 4beta^2 S(m+2).
 // ---------------------------------------------------------------------------
-static double clifford_overlap_factor(int i, int l, double A, double phA, double phB, double a, double L,
-                                      const std::vector<double> &iv, bool with_cos) {
-    if (l < 0) return 0.0;
-    const int K = i + l;
-    std::complex<double> d[4 * kMaxL + 8];
-    sin_product_fourier(i, l, phA, phB, d);
-    std::complex<double> acc = 0.0;
-    if (!with_cos) {
-        for (int k = -K; k <= K; ++k) acc += d[k + K] * iv[std::abs(k)];
-    } else {
-        // times cos(u + phB) = (e^{i phB} e^{iu} + e^{-i phB} e^{-iu})/2
-        const std::complex<double> ep(std::cos(phB), std::sin(phB));
-        for (int k = -K; k <= K; ++k) {
-            acc += 0.5 * ep * d[k + K] * iv[std::abs(k + 1)];
-            acc += 0.5 * std::conj(ep) * d[k + K] * iv[std::abs(k - 1)];
-        }
-    }
-    return acc.real() * L * std::pow(a, -(double)(i + l));
-}
-
-static double real_overlap_factor(int j, int m, double p, double PA, double PB) {
-    if (m < 0) return 0.0;
-    double E[2 * 8 + 3];
-    hermite_E(j, m, p, PA, PB, E);
-    return E[0] * std::sqrt(kPi / p);
-}
-
-static int one_electron_rows(tcg_ctx *c, std::vector<double> &S, std::vector<double> &T) {
-    // shell pairs (A in the unit cell, B anywhere), spread over host threads; the Bessel
-    // vector of a primitive pair and periodic direction is computed once per shell pair
-    // (orders up to lA + lB + 3) and serves every component pair of the two shells
-    const int nf = c->nfuc, nb = c->nbas, nsh = c->nsh;
-    std::vector<int> uc;
-    for (int sI = 0; sI < nsh; ++sI)
-        if (c->sh[sI].first_ao < nf) uc.push_back(sI);
-    const int ntask = (int)uc.size() * nsh;
-    std::atomic<int> next(0);
-    auto work = [&] {
-        std::vector<double> iv[3], scratch;
-        for (int t = next++; t < ntask; t = next++) {
-            const Shell &A = c->sh[uc[t / nsh]], &B = c->sh[t % nsh];
-            for (int a = 0; a < A.ncomp; ++a)
-                for (int b = 0; b < B.ncomp; ++b) {
-                    S[(size_t)(A.first_ao + a) + (size_t)nb * (B.first_ao + b)] = 0.0;
-                    T[(size_t)(A.first_ao + a) + (size_t)nb * (B.first_ao + b)] = 0.0;
-                }
-            for (int pi = 0; pi < A.nprim; ++pi)
-                for (int pj = 0; pj < B.nprim; ++pj) {
-                    const double e1 = c->expo[A.poff + pi], e2 = c->expo[B.poff + pj];
-                    const double esum = e1 + e2, mu = e1 * e2 / esum;
-                    double pref = 1.0, Aarg[3] = {0, 0, 0}, phA[3] = {0, 0, 0}, phB[3] = {0, 0, 0};
-                    double PA[3] = {0, 0, 0}, PB[3] = {0, 0, 0};
-                    for (int d = 0; d < 3; ++d) {
-                        if (d < c->ndp) {
-                            double pe, cen;
-                            const double ad = c->a[d], L = c->L[d];
-                            clifford_pair(e1, e2, A.xyz[d], B.xyz[d], L, ad, &pe, &cen);
-                            pref *= std::exp(-2.0 * (esum - pe) / (ad * ad));
-                            Aarg[d] = 2.0 * pe / (ad * ad);
-                            const int nmax = A.l + B.l + 3;
-                            iv[d].resize(nmax + 1);
-                            bessel_miller(nmax, Aarg[d], nmax + 30 + (int)std::sqrt(80.0 * Aarg[d]), iv[d].data(), scratch);
-                            phA[d] = ad * (cen - A.xyz[d]);
-                            phB[d] = ad * (cen - B.xyz[d]);
-                        } else {
-                            const double P = (e1 * A.xyz[d] + e2 * B.xyz[d]) / esum;
-                            pref *= std::exp(-mu * (A.xyz[d] - B.xyz[d]) * (A.xyz[d] - B.xyz[d]));
-                            PA[d] = P - A.xyz[d];
-                            PB[d] = P - B.xyz[d];
-                        }
-                    }
-                    for (int ca = 0; ca < A.ncomp; ++ca)
-                        for (int cb = 0; cb < B.ncomp; ++cb) {
-                            const int i = A.first_ao + ca, j = B.first_ao + cb;
-                            const int *li = &c->lxyz[3 * i], *lj = &c->lxyz[3 * j];
-                            const double cc = c->coef[c->poff[i] + pi] * c->coef[c->poff[j] + pj];
-                            double Sd[3], Dd[3];
-                            for (int d = 0; d < 3; ++d) {
-                                const int l1 = li[d], l2 = lj[d];
-                                if (d < c->ndp) {
-                                    const double ad = c->a[d], L = c->L[d];
-                                    Sd[d] = clifford_overlap_factor(l1, l2, Aarg[d], phA[d], phB[d], ad, L, iv[d], false);
-                                    double D = 0.0;
-                                    if (l2 >= 2)
-                                        D += (double)(l2 * (l2 - 1)) *
-                                             clifford_overlap_factor(l1, l2 - 2, Aarg[d], phA[d], phB[d], ad, L, iv[d], false);
-                                    if (l2 >= 1) D -= (double)(l2 * l2) * ad * ad * Sd[d];
-                                    // cos(a(x - B)) g_B^l: the l+0 factor times cos(u + phB), with a^-(i+l) as for S(i,l)
-                                    D -= 2.0 * e2 * (2 * l2 + 1) *
-                                         clifford_overlap_factor(l1, l2, Aarg[d], phA[d], phB[d], ad, L, iv[d], true);
-                                    D += 4.0 * e2 * e2 *
-                                         clifford_overlap_factor(l1, l2 + 2, Aarg[d], phA[d], phB[d], ad, L, iv[d], false);
-                                    Dd[d] = D;
-                                } else {
-                                    Sd[d] = real_overlap_factor(l1, l2, esum, PA[d], PB[d]);
-                                    double D = -2.0 * e2 * (2 * l2 + 1) * Sd[d] +
-                                               4.0 * e2 * e2 * real_overlap_factor(l1, l2 + 2, esum, PA[d], PB[d]);
-                                    if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * real_overlap_factor(l1, l2 - 2, esum, PA[d], PB[d]);
-                                    Dd[d] = D;
-                                }
-                            }
-                            const size_t ij = (size_t)i + (size_t)nb * j;
-                            S[ij] += cc * pref * Sd[0] * Sd[1] * Sd[2];
-                            T[ij] -= 0.5 * cc * pref * (Dd[0] * Sd[1] * Sd[2] + Sd[0] * Dd[1] * Sd[2] + Sd[0] * Sd[1] * Dd[2]);
-                        }
-                }
-        }
-    };
-    const unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (unsigned k = 1; k < nthr && (int)k < ntask; ++k) pool.emplace_back(work);
-    work();
-    for (auto &th : pool) th.join();
-    return 0;
-}
-
-// rows of the other cells from the unit cell's (src/Tools/symmetric_matrix.f90:1-52)
-static void fill_translation(const tcg_ctx *c, double *M) {
-    const int nf = c->nfuc, nb = c->nbas, nc = c->ncell;
-    for (int i = 0; i < nf; ++i)
-        for (int j = 0; j < nb; ++j)
-            if (std::fabs(M[(size_t)i + (size_t)nb * j]) < 1e-14) M[(size_t)i + (size_t)nb * j] = 0.0;
-    for (int i = nf; i < nb; ++i) {
-        const int ci = i / nf, a = i % nf;
-        for (int j = 0; j < nb; ++j) {
-            const int cj = j / nf, b = j % nf;
-            const int src = c->relc[(size_t)ci * nc + cj] * nf + b;
-            M[(size_t)i + (size_t)nb * j] = M[(size_t)a + (size_t)nb * src];
-        }
-    }
-}
-
 // The final quartet pass with learned orders: survivors of the whole list,
 // their features and the network's predictions, one rule per shell quartet
 // (the largest asked for, plain rule outside the training range), then the
@@ -1047,8 +917,27 @@ static int setup_quartets(tcg_ctx *c, double &t2, double &t3, bool defer = false
 
 // V(i, j), i in the unit cell, j anywhere, on the GPU (other rows zero);
 // host_work runs on the host while the kernel executes.
+// Device copy of the AO list for the one-electron kernels
+static OneStArgs one_args(tcg_ctx *c) {
+    OneStArgs g{};
+    g.ndp = c->ndp;
+    g.nf = c->nfuc;
+    g.nb = c->nbas;
+    for (int d = 0; d < 3; ++d) {
+        g.a[d] = c->a[d];
+        g.L[d] = c->L[d];
+    }
+    g.center = c->d_center.p;
+    g.lxyz = c->d_lxyz.p;
+    g.poff = c->d_poff.p;
+    g.expo = c->d_expo.p;
+    g.coef = c->d_coef.p;
+    return g;
+}
+
 template <class F>
-static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, std::vector<double> &V, F host_work) {
+static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, std::vector<double> &V, F host_work,
+                        bool fill = false) {
     const int nb = c->nbas, nf = c->nfuc;
     const size_t nn = (size_t)nb * nb;
     std::vector<int> ao_shell(nb), pair_rep((size_t)nf * nb);
@@ -1070,6 +959,7 @@ static int nuclear_rows(tcg_ctx *c, int natom, const double *xyz, const int *Z, 
     int maxpp = 1;
     for (int rp : pair_rep) maxpp = std::max(maxpp, c->sp[rp].count);
     HIPCHK(c, launch_nuclear(A, d_nxyz.p, d_nz.p, natom, d_aosh.p, d_prep.p, maxpp, d_V.p, c->st));
+    if (fill) HIPCHK(c, launch_translate_fill(d_V.p, nf, nb, c->ncell, c->d_relc.p, c->st));
     if constexpr (!std::is_same<F, std::nullptr_t>::value) host_work();
     V.assign(nn, 0.0);
     HIPCHK(c, hipMemcpyAsync(V.data(), d_V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
@@ -1891,17 +1781,65 @@ int tcg_one_electron(tcg_ctx *c, int natom, const double *xyz, const int *Z, dou
     if (natom <= 0 || !xyz || !Z || !S || !T || !V) return c->fail(TCG_E_ARG, "tcg_one_electron: bad arguments");
     int rc = prepare(c);
     if (rc) return rc;
-    const int nb = c->nbas;
+    const int nb = c->nbas, nf = c->nfuc;
     const size_t nn = (size_t)nb * nb;
-    std::vector<double> Sv(nn, 0.0), Tv(nn, 0.0), Vv;
-    rc = nuclear_rows(c, natom, xyz, Z, Vv, [&]() { one_electron_rows(c, Sv, Tv); });
+    // S and T rows of the unit cell on the GPU, the other rows by translation there too;
+    // V (its own kernel) after them on the same stream
+    HIPCHK(c, c->d_S1.alloc(nn));
+    HIPCHK(c, c->d_T1.alloc(nn));
+    OneStArgs g = one_args(c);
+    g.S = c->d_S1.p;
+    g.T = c->d_T1.p;
+    HIPCHK(c, launch_one_st(g, c->st));
+    HIPCHK(c, launch_translate_fill(c->d_S1.p, nf, nb, c->ncell, c->d_relc.p, c->st));
+    HIPCHK(c, launch_translate_fill(c->d_T1.p, nf, nb, c->ncell, c->d_relc.p, c->st));
+    std::vector<double> Vv;
+    rc = nuclear_rows(c, natom, xyz, Z, Vv, nullptr, true);
     if (rc) return rc;
-    std::copy(Sv.begin(), Sv.end(), S);
-    std::copy(Tv.begin(), Tv.end(), T);
+    HIPCHK(c, hipMemcpyAsync(S, c->d_S1.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(T, c->d_T1.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     std::copy(Vv.begin(), Vv.end(), V);
-    fill_translation(c, S);
-    fill_translation(c, T);
-    fill_translation(c, V);
+    return TCG_OK;
+}
+
+int tcg_normalize(tcg_ctx *c, int nbas, const double *center, const int *lxyz, const int *prim_off,
+                  const double *expo, const double *coef_raw, double *coef_out) {
+    if (!c) return TCG_E_ARG;
+    if (nbas <= 0 || !center || !lxyz || !prim_off || !expo || !coef_raw || !coef_out)
+        return c->fail(TCG_E_ARG, "tcg_normalize: bad arguments");
+    const int np = prim_off[nbas];
+    if (np <= 0) return c->fail(TCG_E_ARG, "tcg_normalize: no primitives");
+    for (int i = 0; i < nbas; ++i) {
+        if (prim_off[i + 1] <= prim_off[i]) return c->fail(TCG_E_ARG, "tcg_normalize: AO %d has no primitive", i);
+        if (lxyz[3 * i] + lxyz[3 * i + 1] + lxyz[3 * i + 2] > kMaxL)
+            return c->fail(TCG_E_ARG, "tcg_normalize: AO %d above l = %d", i, kMaxL);
+    }
+    for (int p = 0; p < np; ++p)
+        if (!(expo[p] > 0.0)) return c->fail(TCG_E_ARG, "tcg_normalize: exponent %d not positive", p);
+    DevBuf<double> dc, de, draw, dout;
+    DevBuf<int> dl, dp;
+    HIPCHK(c, dc.upload(std::vector<double>(center, center + 3 * (size_t)nbas)));
+    HIPCHK(c, dl.upload(std::vector<int>(lxyz, lxyz + 3 * (size_t)nbas)));
+    HIPCHK(c, dp.upload(std::vector<int>(prim_off, prim_off + nbas + 1)));
+    HIPCHK(c, de.upload(std::vector<double>(expo, expo + np)));
+    HIPCHK(c, draw.upload(std::vector<double>(coef_raw, coef_raw + np)));
+    HIPCHK(c, dout.alloc(np));
+    OneStArgs g{};
+    g.ndp = c->ndp;
+    g.nf = nbas;
+    g.nb = nbas;
+    for (int d = 0; d < 3; ++d) {
+        g.a[d] = c->a[d];
+        g.L[d] = c->L[d];
+    }
+    g.center = dc.p;
+    g.lxyz = dl.p;
+    g.poff = dp.p;
+    g.expo = de.p;
+    HIPCHK(c, launch_normalize(g, draw.p, dout.p, c->st));
+    HIPCHK(c, hipMemcpyAsync(coef_out, dout.p, sizeof(double) * np, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return TCG_OK;
 }
 

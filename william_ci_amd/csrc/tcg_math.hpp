@@ -18,7 +18,7 @@ constexpr double kPi = 3.14159265358979323846;
 // exp(-x) I_0(x): power series below 20 (A&S 9.6.10), Hankel asymptotic
 // series above (A&S 9.7.1), summed to the smallest term.  Agrees with the
 // reference's iv_scaled(0,x) (src/module/bessel_iv_log.f90:320) to ~1e-13.
-inline double i0_scaled(double x) {
+__host__ __device__ inline double i0_scaled(double x) {
     if (x <= 0.0) return 1.0;
     if (x <= 20.0) {
         double q = 0.25 * x * x, term = 1.0, sum = 1.0;
@@ -27,7 +27,7 @@ inline double i0_scaled(double x) {
             sum += term;
             if (term < 1e-17 * sum) break;
         }
-        return sum * std::exp(-x);
+        return sum * exp(-x);
     }
     double term = 1.0, sum = 1.0, prev = 1e300;
     for (int k = 1; k < 200; ++k) {
@@ -37,7 +37,7 @@ inline double i0_scaled(double x) {
         term = t;
         sum += term;
     }
-    return sum / std::sqrt(2.0 * kPi * x);
+    return sum / sqrt(2.0 * kPi * x);
 }
 
 // exp(-A) I_n(A), n = 0..N, by Miller's backward recurrence

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tcg_device.hpp"
+#include "tcg_math.hpp"
 
 namespace tcg {
 
@@ -198,6 +199,243 @@ hipError_t launch_nuclear(const EriArgs &a, const double *nuc_xyz, const int *nu
     const size_t lds = sizeof(double) * 3 * 16 * (size_t)a.nqp;
     hipLaunchKernelGGL(k_nuclear, dim3((unsigned)blocks), dim3(64), lds, st, a, nuc_xyz, nuc_z, natom, ao_shell,
                        pair_rep, ngroups, V);
+    return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// Overlap and kinetic rows and the AO normalisation (closed form per direction):
+//   overlap_matrix_toroidal_nD / kinetic_matrix_toroidal_nD
+//       e.g. src/Toroidal/1D/1D_Overlap_toroidal.f90:23, 1D_Kinetic_toroidal.f90:102
+//   normalize_AO_toroidal_nD   src/basis/normalization/normalize_1D.f90:17-79
+// A periodic direction of a primitive pair (e1 at A, e2 at B) is the product of
+// exp(-2(e1+e2-pe)/a^2) and sum_k d_k exp(-Ar) I_|k|(Ar) L a^-(i+l), with
+// Ar = 2 pe / a^2, pe and the barycentre from clifford_pair, d_k the Fourier
+// coefficients of sin^i(u+phA) sin^l(u+phB); a real direction is the Hermite E_0
+// of the Gaussian product times sqrt(pi/p).  The kinetic factor of a direction
+// applies d^2/dx^2 to the ket: on the torus l(l-1) S(i,l-2) - l^2 a^2 S(i,l)
+// - 2 e2 (2l+1) S_cos(i,l) + 4 e2^2 S(i,l+2) (S_cos: the extra cos(a(x-B))).
+// One thread per element; the orders are small (lA + lB + 3), so every element
+// runs Miller's recurrence for its primitive pairs itself.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kOneN = 2 * kMaxL + 3;        // highest Bessel order a kinetic factor reads
+constexpr int kOneK = 2 * kMaxL + 2;        // highest sin-product degree (ket raised by 2)
+
+// exp(-A) I_n(A), n = 0..N, Miller's backward recurrence from n0 with two running
+// values, rescaled against overflow, normalised to exp(-A) I_0(A)
+// (tcg_math.hpp::bessel_miller with the start of bessel_scaled_vector)
+__device__ void bessel_dev(int N, double A, double *out) {
+    for (int n = 0; n <= N; ++n) out[n] = 0.0;
+    if (A < 1e-12) {
+        out[0] = 1.0;
+        return;
+    }
+    const int n0 = N + 30 + (int)sqrt(80.0 * A);
+    double wk1 = 0.0, wk = 1e-280;              // w[k+1], w[k]
+    if (n0 <= N) out[n0] = wk;
+    const double r = 2.0 / A;
+    for (int k = n0; k >= 1; --k) {
+        double wm = wk1 + (double)k * r * wk;
+        if (k - 1 <= N) out[k - 1] = wm;
+        if (wm > 1e250) {
+            wm *= 1e-250;
+            wk *= 1e-250;
+            for (int m = k - 1; m <= N; ++m) out[m] *= 1e-250;
+        }
+        wk1 = wk;
+        wk = wm;
+    }
+    const double sc = i0_scaled(A) / out[0];
+    for (int n = 0; n <= N; ++n) out[n] *= sc;
+}
+
+// d_k, k = -K..K, of sin^i(u+phA) sin^l(u+phB), K = i + l <= kOneK
+__device__ void sin_fourier_dev(int i, int l, double phA, double phB, double2 *d) {
+    const int K = i + l;
+    double2 cur[2 * kOneK + 3], nxt[2 * kOneK + 3];
+    for (int k = 0; k < 2 * K + 3; ++k) cur[k] = double2{0.0, 0.0};
+    cur[K + 1] = double2{1.0, 0.0};
+    const double sA = sin(phA), cA = cos(phA), sB = sin(phB), cB = cos(phB);
+    for (int m = 1; m <= K; ++m) {
+        const double sn = (m <= i) ? sA : sB, cs = (m <= i) ? cA : cB;
+        const double pr = 0.5 * sn, pi = -0.5 * cs;        // e^{i ph}/(2i)
+        const double mr = 0.5 * sn, mi = 0.5 * cs;         // -e^{-i ph}/(2i)
+        for (int k = 0; k < 2 * K + 3; ++k) nxt[k] = double2{0.0, 0.0};
+        for (int k = -(m - 1); k <= m - 1; ++k) {
+            const double2 v = cur[k + K + 1];
+            double2 &up = nxt[k + 1 + K + 1], &dn = nxt[k - 1 + K + 1];
+            up.x += pr * v.x - pi * v.y;
+            up.y += pr * v.y + pi * v.x;
+            dn.x += mr * v.x - mi * v.y;
+            dn.y += mr * v.y + mi * v.x;
+        }
+        for (int k = 0; k < 2 * K + 3; ++k) cur[k] = nxt[k];
+    }
+    for (int k = -K; k <= K; ++k) d[k + K] = cur[k + K + 1];
+}
+
+__device__ double torus_factor(int i, int l, double phA, double phB, double a, double L, const double *iv,
+                               bool with_cos) {
+    if (l < 0) return 0.0;
+    const int K = i + l;
+    double2 d[2 * kOneK + 1];
+    sin_fourier_dev(i, l, phA, phB, d);
+    double acc = 0.0;
+    if (!with_cos) {
+        for (int k = -K; k <= K; ++k) acc += d[k + K].x * iv[k < 0 ? -k : k];
+    } else {
+        // times cos(u + phB) = (e^{i phB} e^{iu} + e^{-i phB} e^{-iu}) / 2: real part
+        const double cb = cos(phB), sb = sin(phB);
+        for (int k = -K; k <= K; ++k) {
+            const double2 v = d[k + K];
+            const int kp = k + 1 < 0 ? -(k + 1) : k + 1, km = k - 1 < 0 ? -(k - 1) : k - 1;
+            acc += 0.5 * (cb * v.x - sb * v.y) * iv[kp];
+            acc += 0.5 * (cb * v.x + sb * v.y) * iv[km];
+        }
+    }
+    return acc * L * pow(a, -(double)(i + l));
+}
+
+__device__ double real_factor(int j, int m, double p, double PA, double PB) {
+    if (m < 0) return 0.0;
+    double E[2 * 8 + 3];
+    hermite_E(j, m, p, PA, PB, E);
+    return E[0] * sqrt(kPi / p);
+}
+
+// overlap and kinetic of one primitive pair of AOs (powers li, lj; centres A, B)
+__device__ void prim_st(const OneStArgs &g, const int *li, const int *lj, const double *A, const double *B, double e1,
+                        double e2, bool kinetic, double *s_out, double *t_out) {
+    const double esum = e1 + e2, mu = e1 * e2 / esum;
+    double pref = 1.0, Sd[3], Dd[3];
+    const int lt = li[0] + li[1] + li[2] + lj[0] + lj[1] + lj[2];
+    for (int d = 0; d < 3; ++d) {
+        const int l1 = li[d], l2 = lj[d];
+        if (d < g.ndp) {
+            const double ad = g.a[d], L = g.L[d];
+            double pe, cen;
+            clifford_pair(e1, e2, A[d], B[d], L, ad, &pe, &cen);
+            pref *= exp(-2.0 * (esum - pe) / (ad * ad));
+            const double Ar = 2.0 * pe / (ad * ad);
+            double iv[kOneN + 1];
+            bessel_dev(lt + 3 < kOneN ? lt + 3 : kOneN, Ar, iv);
+            const double phA = ad * (cen - A[d]), phB = ad * (cen - B[d]);
+            Sd[d] = torus_factor(l1, l2, phA, phB, ad, L, iv, false);
+            if (kinetic) {
+                double D = 0.0;
+                if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * torus_factor(l1, l2 - 2, phA, phB, ad, L, iv, false);
+                if (l2 >= 1) D -= (double)(l2 * l2) * ad * ad * Sd[d];
+                D -= 2.0 * e2 * (2 * l2 + 1) * torus_factor(l1, l2, phA, phB, ad, L, iv, true);
+                D += 4.0 * e2 * e2 * torus_factor(l1, l2 + 2, phA, phB, ad, L, iv, false);
+                Dd[d] = D;
+            }
+        } else {
+            const double P = (e1 * A[d] + e2 * B[d]) / esum;
+            pref *= exp(-mu * (A[d] - B[d]) * (A[d] - B[d]));
+            const double PA = P - A[d], PB = P - B[d];
+            Sd[d] = real_factor(l1, l2, esum, PA, PB);
+            if (kinetic) {
+                double D = -2.0 * e2 * (2 * l2 + 1) * Sd[d] + 4.0 * e2 * e2 * real_factor(l1, l2 + 2, esum, PA, PB);
+                if (l2 >= 2) D += (double)(l2 * (l2 - 1)) * real_factor(l1, l2 - 2, esum, PA, PB);
+                Dd[d] = D;
+            }
+        }
+    }
+    *s_out = pref * Sd[0] * Sd[1] * Sd[2];
+    if (kinetic) *t_out = -0.5 * pref * (Dd[0] * Sd[1] * Sd[2] + Sd[0] * Dd[1] * Sd[2] + Sd[0] * Sd[1] * Dd[2]);
+}
+
+}  // namespace
+
+// S(i, j), T(i, j) for i < nfuc, every j (column-major, leading dimension nbas)
+__global__ __launch_bounds__(256) void k_one_st(OneStArgs g) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)g.nf * g.nb) return;
+    const int i = (int)(idx % g.nf), j = (int)(idx / g.nf);
+    const int *li = g.lxyz + 3 * i, *lj = g.lxyz + 3 * j;
+    const double *A = g.center + 3 * i, *B = g.center + 3 * j;
+    double S = 0.0, T = 0.0;
+    for (int p = g.poff[i]; p < g.poff[i + 1]; ++p)
+        for (int q = g.poff[j]; q < g.poff[j + 1]; ++q) {
+            double s, t;
+            prim_st(g, li, lj, A, B, g.expo[p], g.expo[q], true, &s, &t);
+            const double cc = g.coef[p] * g.coef[q];
+            S += cc * s;
+            T += cc * t;
+        }
+    g.S[(size_t)i + (size_t)g.nb * j] = S;
+    g.T[(size_t)i + (size_t)g.nb * j] = T;
+}
+
+// normalize_AO_toroidal_nD: every primitive of AO i to one, then the
+// contraction: c_p = coef_p / sqrt(S_pp), out = c / sqrt(c^T S c), S the
+// self-overlap of the AO's raw primitives (same centre).  One thread per AO.
+__global__ __launch_bounds__(64) void k_normalize(OneStArgs g, const double *__restrict__ raw,
+                                                  double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nb) return;
+    const int *li = g.lxyz + 3 * i;
+    const double *A = g.center + 3 * i;
+    const int p0 = g.poff[i], np = g.poff[i + 1] - p0;
+    double norm2 = 0.0;
+    for (int p = 0; p < np; ++p) {
+        double spp, dummy;
+        prim_st(g, li, li, A, A, g.expo[p0 + p], g.expo[p0 + p], false, &spp, &dummy);
+        const double cp = raw[p0 + p] / sqrt(spp);
+        for (int q = 0; q < np; ++q) {
+            double sqq, spq;
+            prim_st(g, li, li, A, A, g.expo[p0 + q], g.expo[p0 + q], false, &sqq, &dummy);
+            prim_st(g, li, li, A, A, g.expo[p0 + p], g.expo[p0 + q], false, &spq, &dummy);
+            norm2 += cp * (raw[p0 + q] / sqrt(sqq)) * spq;
+        }
+    }
+    const double nrm = 1.0 / sqrt(norm2);
+    for (int p = 0; p < np; ++p) {
+        double spp, dummy;
+        prim_st(g, li, li, A, A, g.expo[p0 + p], g.expo[p0 + p], false, &spp, &dummy);
+        out[p0 + p] = raw[p0 + p] / sqrt(spp) * nrm;
+    }
+}
+
+// rows of the other cells from the unit cell's (src/Tools/symmetric_matrix.f90:1-52):
+// zero == 1 clears |M(i, j)| < 1e-14 in the unit-cell rows; zero == 0 copies
+// M(a + nf ci, b + nf cj) = M(a, b + nf relc(ci, cj)) for the rows of the other cells
+__global__ __launch_bounds__(256) void k_translate_fill(double *__restrict__ M, int nf, int nb, int nc,
+                                                        const int *__restrict__ relc, int zero) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (zero) {
+        if (idx >= (long long)nf * nb) return;
+        const int i = (int)(idx % nf), j = (int)(idx / nf);
+        double &m = M[(size_t)i + (size_t)nb * j];
+        if (fabs(m) < 1e-14) m = 0.0;
+        return;
+    }
+    if (idx >= (long long)(nb - nf) * nb) return;
+    const int i = nf + (int)(idx % (nb - nf)), j = (int)(idx / (nb - nf));
+    const int ci = i / nf, a = i % nf, cj = j / nf, b = j % nf;
+    const int src = relc[(size_t)ci * nc + cj] * nf + b;
+    M[(size_t)i + (size_t)nb * j] = M[(size_t)a + (size_t)nb * src];
+}
+
+hipError_t launch_one_st(const OneStArgs &g, hipStream_t st) {
+    const long long n = (long long)g.nf * g.nb;
+    hipLaunchKernelGGL(k_one_st, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize(const OneStArgs &g, const double *raw, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_normalize, dim3((g.nb + 63) / 64), dim3(64), 0, st, g, raw, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_translate_fill(double *M, int nf, int nb, int nc, const int *relc, hipStream_t st) {
+    hipLaunchKernelGGL(k_translate_fill, dim3((unsigned)(((long long)nf * nb + 255) / 256)), dim3(256), 0, st, M, nf,
+                       nb, nc, relc, 1);
+    if (nb > nf)
+        hipLaunchKernelGGL(k_translate_fill, dim3((unsigned)(((long long)(nb - nf) * nb + 255) / 256)), dim3(256), 0,
+                           st, M, nf, nb, nc, relc, 0);
     return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ def lib():
         L.tcg_jk_stored_noalloc.argtypes = [_vp, _ci, _ci, _dp, _dp, _dp]
         L.tcg_direct_init.argtypes = [_vp]
         L.tcg_direct_jk.argtypes = [_vp, _ci, _dp, _dp, _dp]
+        L.tcg_normalize.argtypes = [_vp, _ci, _dp, _ip, _ip, _dp, _dp, _dp]
         L.tcg_ao_to_sao.argtypes = [_vp, _ci, _dp, _dp]
         L.tcg_sao_to_ao.argtypes = [_vp, _ci, _dp, _dp]
         L.tcg_heev_blocks.argtypes = [_vp, _ci, _dp, _dp]
@@ -314,6 +315,29 @@ class Engine:
                 "gap": out[9], "jk_ms": out[10], "wall_ms": out[11],
                 "e": e, "coeff_sao": C.view(np.complex128).reshape(nb, nb).T, "occ": occ,
                 "P": P.reshape(nb, nb).T}
+
+
+def normalized_coef(sysm) -> np.ndarray:
+    """normalize_AO_toroidal_nD on the GPU (tcg_normalize): the normalised
+    contraction coefficients of the system's AO list."""
+    L = lib()
+    h = _vp()
+    rc = L.tcg_create(sysm.ndp, np.array(sysm.L, dtype=np.float64), np.array(sysm.ncell, dtype=np.int32), 32,
+                      ctypes.byref(h))
+    if rc != 0:
+        raise EngineError(f"tcg_create failed ({rc})")
+    try:
+        out = np.zeros(len(sysm.coef))
+        rc = L.tcg_normalize(h, sysm.nbas, np.ascontiguousarray(sysm.center, dtype=np.float64).ravel(),
+                             np.ascontiguousarray(sysm.lxyz, dtype=np.int32).ravel(),
+                             np.ascontiguousarray(sysm.prim_off, dtype=np.int32),
+                             np.ascontiguousarray(sysm.expo, dtype=np.float64),
+                             np.ascontiguousarray(sysm.coef, dtype=np.float64), out)
+        if rc != 0:
+            raise EngineError(f"tcg_normalize: {L.tcg_last_error(h).decode()}")
+        return out
+    finally:
+        L.tcg_destroy(h)
 
 
 def ERI_integral_toroidal(sysm, coef, physicist: bool = False, sao: bool = True, nquad: int = 32):
