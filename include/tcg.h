@@ -125,6 +125,21 @@ int tcg_mp2_gamma(tcg_ctx *ctx, int nbas, int nocc, const double *e, const doubl
  * out for |D| < 1e-8, out[4] = GPU wall time in ms. */
 int tcg_kmp2(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out);
 
+/* The same k-point MP2 without the stored SAO tensor (integral-direct).  Replaces
+ *   transform_ERI_direct_kMO(nBas,coeff_SAO,p_lo,p_hi,q_lo,q_hi,ERI_out,iout)
+ *                                src/Energies/SAO/MP2/AO_to_MO_SAO_direct.f90:26-199
+ *   direct_slab(SD,nfuc,nBas,ncell,slab)   src/module/eri_direct.f90:816-869
+ * followed by kMP2_calc.  Needs tcg_direct_init.  The integrals are built in
+ * slabs of the last index, j4 = d + nf R4 for the cells R4 of the slab (the
+ * reference slices by unit-cell shell; the transform is linear in j4 either
+ * way), each slab transformed over its two ket indices and summed; the unique
+ * quartet list is evaluated once per slab.  slab_cells = cells per slab (0:
+ * as many as a quarter of the free device memory holds).  At most 64 cells.
+ * out[0..4] as tcg_kmp2; out[5] integral time, out[6] transform time (ms),
+ * out[7] number of slabs. */
+int tcg_kmp2_direct(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, const int *occ,
+                    int slab_cells, double *out);
+
 /* k-space (Bloch, "SAO") matrices.  M_sao is the (nBas,nBas) complex matrix
  * of the reference (re, im pairs, column-major), block diagonal in k: block k
  * (cells and k points in one order, x fastest) holds rows and columns

@@ -97,6 +97,17 @@ module tcg_engine
       integer(c_int), intent(in) :: occ(*)
       real(c_double), intent(out) :: out(5)
     end function
+    ! transform_ERI_direct_kMO + kMP2_calc (integral-direct)
+    integer(c_int) function tcg_kmp2_direct(ctx, nbas, coeff_sao, e, occ, slab_cells, out) &
+        bind(C, name="tcg_kmp2_direct")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbas, slab_cells
+      complex(c_double_complex), intent(in) :: coeff_sao(*)
+      real(c_double), intent(in) :: e(*)
+      integer(c_int), intent(in) :: occ(*)
+      real(c_double), intent(out) :: out(8)
+    end function
     ! transform_AO_to_SAO / transform_SAO_to_AO
     integer(c_int) function tcg_ao_to_sao(ctx, nbas, M_ao, M_sao) bind(C, name="tcg_ao_to_sao")
       import :: c_int, c_double, c_double_complex, c_ptr

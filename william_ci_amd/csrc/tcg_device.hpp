@@ -58,6 +58,7 @@ struct EriArgs {
     // scatter mode
     double *eri;                      // ERI(n1,nbas,nbas,nbas)
     int n1, nbas, nfuc, ncell, physicist;
+    int j4lo, j4hi;                   // j4hi > 0: only last indices in [j4lo, j4hi), stored at j4 - j4lo
     const double *Qao;                // [nbas*nbas] function-pair Schwarz values
     // direct J/K mode
     const double *Pden;               // density P(nbas,nbas), column-major
@@ -127,6 +128,7 @@ struct DftMoArgs {
     double sgn, scale;
     const double2 *T;            // [(k nf + c) M + m]
     double *Yr, *Yi;
+    int accum;                   // 1: Y += (sums over slabs of the transformed index pair)
 };
 hipError_t launch_dft_mo(const DftMoArgs &A, hipStream_t st);
 hipError_t launch_list_j2_flags(const long long *list, long long n, int nbas, unsigned char *flags, hipStream_t st);

@@ -1448,16 +1448,22 @@ int tcg_mp2_gamma(tcg_ctx *c, int nbas, int nocc, const double *e, const double 
     return TCG_OK;
 }
 
-int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out) {
-    if (!c) return TCG_E_ARG;
-    if (nbas != c->nbas || !coeff_sao || !e || !occ || !out) return c->fail(TCG_E_ARG, "tcg_kmp2: bad arguments");
-    if (c->eri_n1 != c->nfuc || !c->d_eri.p || c->eri_physicist)
-        return c->fail(TCG_E_STATE, "tcg_kmp2 needs the chemist SAO ERI resident (tcg_eri_build with sao = 1)");
+// The k blocks of the coefficients, orbital energies and occupations of a k-point
+// MP2 call, the (ov|ov) ranges set_cc_dims sets (src/module/cc_dims.f90:68-88) and
+// the phase tables e^{+i k.R} (cells and k points x fastest).
+struct KmpInput {
+    std::vector<double2> Ck;                    // [k][nf x nf] column-major blocks
+    std::vector<double> ek, cosm, sinm, cosn, sinn;
+    std::vector<int> oc;
+    int o_hi = 0, v_lo = 0;
+};
+
+static int kmp2_input(tcg_ctx *c, const char *who, const double *coeff_sao, const double *e, const int *occ,
+                      KmpInput &K) {
     const int nf = c->nfuc, N = c->ncell, nb = c->nbas;
-    // k blocks, occupation ranges as set_cc_dims sets them (src/module/cc_dims.f90:68-88)
-    std::vector<double2> Ck((size_t)N * nf * nf);
-    std::vector<double> ek((size_t)N * nf);
-    std::vector<int> oc((size_t)N * nf);
+    K.Ck.assign((size_t)N * nf * nf, double2{0.0, 0.0});
+    K.ek.assign((size_t)N * nf, 0.0);
+    K.oc.assign((size_t)N * nf, 0);
     int o_hi = 0, omin = nf;
     double offblk = 0.0, cmax = 0.0;
     for (int col = 0; col < nb; ++col)
@@ -1467,40 +1473,105 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
             const double m = std::hypot(re, im);
             cmax = std::max(cmax, m);
             if (row / nf != col / nf) offblk = std::max(offblk, m);
-            else Ck[(size_t)(row / nf) * nf * nf + (row % nf) + (size_t)nf * (col % nf)] = double2{re, im};
+            else K.Ck[(size_t)(row / nf) * nf * nf + (row % nf) + (size_t)nf * (col % nf)] = double2{re, im};
         }
     if (offblk > 1e-10 * std::max(1.0, cmax))
-        return c->fail(TCG_E_ARG, "tcg_kmp2: coefficients not k-block ordered (off-block %.3e)", offblk);
+        return c->fail(TCG_E_ARG, "%s: coefficients not k-block ordered (off-block %.3e)", who, offblk);
     for (int k = 0; k < N; ++k) {
         int nok = 0;
         for (int p = 0; p < nf; ++p) {
-            ek[(size_t)k * nf + p] = e[(size_t)k * nf + p];
-            oc[(size_t)k * nf + p] = occ[(size_t)k * nf + p];
+            K.ek[(size_t)k * nf + p] = e[(size_t)k * nf + p];
+            K.oc[(size_t)k * nf + p] = occ[(size_t)k * nf + p];
             const int o = occ[(size_t)k * nf + p];
-            if (o != 0 && o != 2) return c->fail(TCG_E_ARG, "tcg_kmp2: occupation must be 0 or 2");
+            if (o != 0 && o != 2) return c->fail(TCG_E_ARG, "%s: occupation must be 0 or 2", who);
             if (o == 2) {
-                if (p != nok) return c->fail(TCG_E_ARG, "tcg_kmp2: occupied bands are not the lowest ones in a k block");
+                if (p != nok) return c->fail(TCG_E_ARG, "%s: occupied bands are not the lowest ones in a k block", who);
                 ++nok;
             }
         }
         o_hi = std::max(o_hi, nok);
         omin = std::min(omin, nok);
     }
-    const int v_lo = omin;                     // first virtual band (0-based)
-    if (o_hi == 0 || v_lo >= nf) return c->fail(TCG_E_ARG, "tcg_kmp2: no occupied or no virtual bands");
-    const size_t nbl = (size_t)nb, M1 = (size_t)nf * nbl * nbl * nf;     // rows (a, j2, j3, d)
-    // phase matrix e^{+i k.R}, cells and k points x fastest
-    std::vector<double> cosm((size_t)N * N), sinm((size_t)N * N), cosn((size_t)N * N), sinn((size_t)N * N);
+    K.o_hi = o_hi;
+    K.v_lo = omin;                             // first virtual band (0-based)
+    if (o_hi == 0 || K.v_lo >= nf) return c->fail(TCG_E_ARG, "%s: no occupied or no virtual bands", who);
+    K.cosm.resize((size_t)N * N);
+    K.sinm.resize((size_t)N * N);
+    K.cosn.resize((size_t)N * N);
+    K.sinn.resize((size_t)N * N);
     const int nx = c->ncl[0], ny = c->ncl[1], nz = c->ncl[2];
     for (int k = 0; k < N; ++k)
         for (int R = 0; R < N; ++R) {
             const double th = 2.0 * kPi * ((double)(k % nx) * (R % nx) / nx + (double)((k / nx) % ny) * ((R / nx) % ny) / ny +
                                            (double)(k / (nx * ny)) * (R / (nx * ny)) / nz);
-            cosm[R + (size_t)N * k] = std::cos(th);
-            sinm[R + (size_t)N * k] = std::sin(th);
-            cosn[R + (size_t)N * k] = std::cos(th) / N;
-            sinn[R + (size_t)N * k] = std::sin(th) / N;
+            K.cosm[R + (size_t)N * k] = std::cos(th);
+            K.sinm[R + (size_t)N * k] = std::sin(th);
+            K.cosn[R + (size_t)N * k] = std::cos(th) / N;
+            K.sinn[R + (size_t)N * k] = std::sin(th) / N;
         }
+    return 0;
+}
+
+// Orbital factors of the fused transforms: TA = conj C_k(a, m) over the occupied
+// bands, TB = C_k(a, v_lo + m) over the virtual ones, [(k nf + a) M + m].
+static void kmp2_factors(const KmpInput &K, int nf, int N, std::vector<double2> &TA, std::vector<double2> &TB) {
+    const int o_hi = K.o_hi, nv = nf - K.v_lo;
+    TA.resize((size_t)N * nf * o_hi);
+    TB.resize((size_t)N * nf * nv);
+    for (int k = 0; k < N; ++k)
+        for (int a = 0; a < nf; ++a) {
+            for (int m = 0; m < o_hi; ++m) {
+                const double2 v = K.Ck[(size_t)k * nf * nf + a + (size_t)nf * m];
+                TA[((size_t)k * nf + a) * o_hi + m] = double2{v.x, -v.y};
+            }
+            for (int m = 0; m < nv; ++m)
+                TB[((size_t)k * nf + a) * nv + m] = K.Ck[(size_t)k * nf * nf + a + (size_t)nf * (K.v_lo + m)];
+        }
+}
+
+// kMP2_calc on the (ov|ov) blocks in km_kmo: out[0..3] as tcg_kmp2
+static int kmp2_sums(tcg_ctx *c, int o_hi, int v_lo, double *out) {
+    const int nf = c->nfuc, N = c->ncell;
+    const int nblk = 1024;
+    HIPCHK(c, c->km_part.alloc(2 * nblk));
+    HIPCHK(c, c->km_nz.alloc(1));
+    HIPCHK(c, launch_fill((double *)c->km_nz.p, 1, 0.0, c->st));
+    HIPCHK(c, launch_kmp2_sums(c->km_kmo.p, c->km_e.p, c->km_occ.p, nf, N, c->ncl[0], c->ncl[1], c->ncl[2], o_hi, v_lo,
+                               c->km_part.p, nblk, c->km_nz.p, c->st));
+    std::vector<double> part(2 * nblk);
+    unsigned long long nzero = 0;
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->km_part.p, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(&nzero, c->km_nz.p, sizeof nzero, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double E2d = 0.0, E2x = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        E2d += part[2 * b];
+        E2x += part[2 * b + 1];
+    }
+    out[0] = 2.0 * E2d - E2x;
+    out[1] = 2.0 * E2d;
+    out[2] = -E2x;
+    out[3] = (double)nzero;
+    return 0;
+}
+
+int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out) {
+    if (!c) return TCG_E_ARG;
+    if (nbas != c->nbas || !coeff_sao || !e || !occ || !out) return c->fail(TCG_E_ARG, "tcg_kmp2: bad arguments");
+    if (c->eri_n1 != c->nfuc || !c->d_eri.p || c->eri_physicist)
+        return c->fail(TCG_E_STATE, "tcg_kmp2 needs the chemist SAO ERI resident (tcg_eri_build with sao = 1)");
+    const int nf = c->nfuc, N = c->ncell, nb = c->nbas;
+    KmpInput KI;
+    {
+        const int rc = kmp2_input(c, "tcg_kmp2", coeff_sao, e, occ, KI);
+        if (rc) return rc;
+    }
+    const std::vector<double2> &Ck = KI.Ck;
+    const std::vector<double> &cosm = KI.cosm, &sinm = KI.sinm, &cosn = KI.cosn, &sinn = KI.sinn, &ek = KI.ek;
+    const std::vector<int> &oc = KI.oc;
+    const int o_hi = KI.o_hi, v_lo = KI.v_lo;
+    const int nx = c->ncl[0], ny = c->ncl[1], nz = c->ncl[2];
+    const size_t nbl = (size_t)nb, M1 = (size_t)nf * nbl * nbl * nf;     // rows (a, j2, j3, d)
     DevBuf<double> X1r, X1i, X2r, X2i, Sr, Si;
     auto &dcos = c->km_cos, &dsin = c->km_sin, &dcosn = c->km_cosn, &dsinn = c->km_sinn, &de = c->km_e;
     auto &dpart = c->km_part;
@@ -1554,16 +1625,8 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
                 cos2[i + (size_t)n2 * k] = cosm[cells2[i] + (size_t)N * k];
                 sin2[i + (size_t)n2 * k] = sinm[cells2[i] + (size_t)N * k];
             }
-        std::vector<double2> TA((size_t)N * nf * o_hi), TB((size_t)N * nf * nv);
-        for (int k = 0; k < N; ++k)
-            for (int a = 0; a < nf; ++a) {
-                for (int m = 0; m < o_hi; ++m) {
-                    const double2 v = Ck[(size_t)k * nf * nf + a + (size_t)nf * m];
-                    TA[((size_t)k * nf + a) * o_hi + m] = double2{v.x, -v.y};
-                }
-                for (int m = 0; m < nv; ++m)
-                    TB[((size_t)k * nf + a) * nv + m] = Ck[(size_t)k * nf * nf + a + (size_t)nf * (v_lo + m)];
-            }
+        std::vector<double2> TA, TB;
+        kmp2_factors(KI, nf, N, TA, TB);
         const long long IA = (long long)nf * nf * n2, IB = IA * o_hi * N;
         std::vector<long long> imap(IA);
         for (long long i = 0; i < IA; ++i) {
@@ -1674,26 +1737,216 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
     HIPCHK(c, launch_kmo(Sr.p, Si.p, dCk.p, nf, N, nx, ny, nz, o_hi, v_lo, dekmo.p, st));
     }
     // 5) kMP2_calc
-    const int nblk = 1024;
-    HIPCHK(c, dpart.alloc(2 * nblk));
-    HIPCHK(c, dnz.alloc(1));
-    HIPCHK(c, launch_fill((double *)dnz.p, 1, 0.0, st));
-    HIPCHK(c, launch_kmp2_sums(dekmo.p, de.p, docc.p, nf, N, nx, ny, nz, o_hi, v_lo, dpart.p, nblk, dnz.p, st));
-    std::vector<double> part(2 * nblk);
-    unsigned long long nzero = 0;
-    HIPCHK(c, hipMemcpyAsync(part.data(), dpart.p, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(&nzero, dnz.p, sizeof nzero, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    double E2d = 0.0, E2x = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        E2d += part[2 * b];
-        E2x += part[2 * b + 1];
+    {
+        const int rc = kmp2_sums(c, o_hi, v_lo, out);
+        if (rc) return rc;
     }
-    out[0] = 2.0 * E2d - E2x;
-    out[1] = 2.0 * E2d;
-    out[2] = -E2x;
-    out[3] = (double)nzero;
     out[4] = now_ms() - t0;
+    return TCG_OK;
+}
+
+// k-point MP2 without the stored tensor: the SAO integrals are built slab by slab
+// of the last index (j4 = d + nf R4 over a range of cells R4), each slab through
+// the first two fused transforms (R3 -> k3 with the occupied index, R4 -> k4 with
+// the virtual one, the latter summed over the slabs), then the last two transforms
+// and the sums once.  Linear in j4, so the result is tcg_kmp2's on the stored tensor.
+int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, const int *occ, int slab_cells,
+                    double *out) {
+    if (!c) return TCG_E_ARG;
+    if (nbas != c->nbas || !coeff_sao || !e || !occ || !out || slab_cells < 0)
+        return c->fail(TCG_E_ARG, "tcg_kmp2_direct: bad arguments");
+    if (!c->direct_ready || !c->prepared) return c->fail(TCG_E_STATE, "tcg_kmp2_direct: tcg_direct_init not called");
+    const int nf = c->nfuc, N = c->ncell, nb = c->nbas;
+    if (N > 64) return c->fail(TCG_E_ARG, "tcg_kmp2_direct: at most 64 cells (%d)", N);
+    KmpInput KI;
+    {
+        const int rc = kmp2_input(c, "tcg_kmp2_direct", coeff_sao, e, occ, KI);
+        if (rc) return rc;
+    }
+    const int o_hi = KI.o_hi, v_lo = KI.v_lo, nv = nf - v_lo;
+    hipStream_t st = c->st;
+    const double t0 = now_ms();
+    if (c->nq4 < 0) {                           // a deferred enumeration left its count on the device
+        const int rc0 = enumerate_quartets(c, false);
+        if (rc0) return rc0;
+    }
+    // bra cells R2 with any stored element (a in cell 0, b in cell R2): those whose
+    // pair Schwarz value passes the element screen against the largest one
+    std::vector<double> Q((size_t)nb * nb);
+    HIPCHK(c, hipMemcpyAsync(Q.data(), c->d_Q.p, sizeof(double) * Q.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const double qmax = *std::max_element(Q.begin(), Q.end());
+    std::vector<int> cells2;
+    for (int R = 0; R < N; ++R) {
+        bool any = false;
+        for (int b = 0; b < nf && !any; ++b)
+            for (int a = 0; a < nf && !any; ++a) any = Q[(size_t)a + (size_t)nb * (b + (size_t)nf * R)] * qmax >= kSchwarz;
+        if (any) cells2.push_back(R);
+    }
+    if (cells2.empty()) cells2.push_back(0);
+    const int n2 = (int)cells2.size();
+    const long long IA = (long long)nf * nf * n2, IB = IA * o_hi * N;
+    // cells per slab: the caller's, or what a quarter of the free device memory holds
+    // next to the work buffers of the transforms
+    const size_t nYB = (size_t)IB * nv * N, nYC = (size_t)nf * nv * N * o_hi * N * nv * N;
+    const size_t per_cell = (size_t)nf * nb * nb * nf * sizeof(double) + (size_t)IA * o_hi * N * nf * 2 * sizeof(double);
+    int nR = slab_cells;
+    if (nR == 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(c, hipMemGetInfo(&fr, &tot));
+        const size_t fixed = 2 * sizeof(double) * (nYB + nYC);
+        const size_t avail = fr > fixed ? (fr - fixed) / 2 : 0;
+        nR = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, avail / per_cell));
+    }
+    nR = std::min(nR, N);
+    const int nslab = (N + nR - 1) / nR;
+    // tables
+    std::vector<double2> TA, TB;
+    kmp2_factors(KI, nf, N, TA, TB);
+    std::vector<double> cos2((size_t)n2 * N), sin2((size_t)n2 * N);
+    for (int k = 0; k < N; ++k)
+        for (int i = 0; i < n2; ++i) {
+            cos2[i + (size_t)n2 * k] = KI.cosm[cells2[i] + (size_t)N * k];
+            sin2[i + (size_t)n2 * k] = KI.sinm[cells2[i] + (size_t)N * k];
+        }
+    // per slab s: cos/sin of its cells [s nR, s nR + nRs), table [R' + nRs k] at offset s nR N
+    std::vector<double> cosB((size_t)N * N), sinB((size_t)N * N);
+    for (int sl = 0; sl < nslab; ++sl) {
+        const int R0 = sl * nR, nRs = std::min(nR, N - R0);
+        for (int k = 0; k < N; ++k)
+            for (int r = 0; r < nRs; ++r) {
+                cosB[(size_t)R0 * N + r + (size_t)nRs * k] = KI.cosm[(R0 + r) + (size_t)N * k];
+                sinB[(size_t)R0 * N + r + (size_t)nRs * k] = KI.sinm[(R0 + r) + (size_t)N * k];
+            }
+    }
+    std::vector<long long> imap(IA);
+    for (long long i = 0; i < IA; ++i) {
+        const long long a = i % nf, b2 = (i / nf) % nf, R2 = cells2[i / ((long long)nf * nf)];
+        imap[i] = a + (long long)nf * (b2 + (long long)nf * R2);
+    }
+    HIPCHK(c, upload_async(c, c->km_cos, KI.cosm));
+    HIPCHK(c, upload_async(c, c->km_sin, KI.sinm));
+    HIPCHK(c, upload_async(c, c->km_cosn, cosB));
+    HIPCHK(c, upload_async(c, c->km_sinn, sinB));
+    HIPCHK(c, upload_async(c, c->km_Ck, KI.Ck));
+    HIPCHK(c, upload_async(c, c->km_e, KI.ek));
+    HIPCHK(c, upload_async(c, c->km_occ, KI.oc));
+    HIPCHK(c, upload_async(c, c->km_TA, TA));
+    HIPCHK(c, upload_async(c, c->km_TB, TB));
+    HIPCHK(c, upload_async(c, c->km_cos2, cos2));
+    HIPCHK(c, upload_async(c, c->km_sin2, sin2));
+    HIPCHK(c, upload_async(c, c->km_imap, imap));
+    HIPCHK(c, hipStreamSynchronize(st));
+    // work buffers: the slab, Y_A of one slab, Y_B summed over the slabs (then Y_C)
+    DevBuf<double> slab, Ar, Ai;
+    const size_t nslab_el = (size_t)nf * nb * nb * ((size_t)nf * nR);
+    const size_t nYA = (size_t)IA * o_hi * N * ((size_t)nf * nR);
+    HIPCHK(c, slab.alloc(nslab_el));
+    HIPCHK(c, Ar.alloc(std::max(nYA, nYC)));
+    HIPCHK(c, Ai.alloc(std::max(nYA, nYC)));
+    auto &Br = c->km_Br, &Bi = c->km_Bi;
+    HIPCHK(c, Br.alloc(nYB));
+    HIPCHK(c, Bi.alloc(nYB));
+    double t_int = 0.0, t_tr = 0.0;
+    for (int sl = 0; sl < nslab; ++sl) {
+        const int R0 = sl * nR, nRs = std::min(nR, N - R0);
+        const double ta = now_ms();
+        // ---- the slab: SAO elements with j4 in [nf R0, nf (R0 + nRs)) ----
+        HIPCHK(c, launch_fill(slab.p, (size_t)nf * nb * nb * ((size_t)nf * nRs), 0.0, st));
+        HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), st));
+        EriArgs A = base_args(c);
+        A.q4 = c->d_q4.p;
+        A.nq4 = c->nq4;
+        A.screen_prim = 1;
+        A.Qao = c->d_Q.p;
+        A.eri = slab.p;
+        A.n1 = nf;
+        A.physicist = 0;
+        A.j4lo = nf * R0;
+        A.j4hi = nf * (R0 + nRs);
+        int rc = run_quartets(c, A);
+        if (rc) return rc;
+        HIPCHK(c, hipStreamSynchronize(st));
+        const double tb = now_ms();
+        t_int += tb - ta;
+        // ---- A: (c R3) -> (r k3), rows (a b R2 in cells2), outer j4 of the slab ----
+        DftMoArgs d{};
+        d.Xr = slab.p;
+        d.Xi = nullptr;
+        d.I = IA;
+        d.O = (long long)nf * nRs;
+        d.imap = c->km_imap.p;
+        d.sJ = (long long)nf * nb;
+        d.sO = (long long)nf * nb * nb;
+        d.nf = nf;
+        d.N = N;
+        d.NR = N;
+        d.M = o_hi;
+        d.cosm = c->km_cos.p;
+        d.sinm = c->km_sin.p;
+        d.sgn = -1.0;
+        d.scale = 1.0;
+        d.T = c->km_TA.p;
+        d.Yr = Ar.p;
+        d.Yi = Ai.p;
+        HIPCHK(c, launch_dft_mo(d, st));
+        // ---- B: (d R4) -> (s k4) over the slab's cells, summed into Y_B ----
+        d.Xr = Ar.p;
+        d.Xi = Ai.p;
+        d.I = IB;
+        d.O = 1;
+        d.imap = nullptr;
+        d.sJ = IB;
+        d.sO = IB * nf * nRs;
+        d.NR = nRs;
+        d.cosm = c->km_cosn.p + (size_t)R0 * N;
+        d.sinm = c->km_sinn.p + (size_t)R0 * N;
+        d.M = nv;
+        d.sgn = 1.0;
+        d.T = c->km_TB.p;
+        d.Yr = Br.p;
+        d.Yi = Bi.p;
+        d.accum = sl > 0;
+        HIPCHK(c, launch_dft_mo(d, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        t_tr += now_ms() - tb;
+    }
+    slab.release();
+    const double tc = now_ms();
+    // ---- C: (b R2) -> (q k2) / N over cells2; D: a -> p ----
+    DftMoArgs d{};
+    d.Xr = Br.p;
+    d.Xi = Bi.p;
+    d.I = nf;
+    d.O = (long long)o_hi * N * nv * N;
+    d.imap = nullptr;
+    d.sJ = nf;
+    d.sO = IA;
+    d.nf = nf;
+    d.N = N;
+    d.NR = n2;
+    d.M = nv;
+    d.cosm = c->km_cos2.p;
+    d.sinm = c->km_sin2.p;
+    d.sgn = 1.0;
+    d.scale = 1.0 / N;
+    d.T = c->km_TB.p;
+    d.Yr = Ar.p;
+    d.Yi = Ai.p;
+    HIPCHK(c, launch_dft_mo(d, st));
+    const size_t nkmo = (size_t)o_hi * nv * o_hi * nv * N * N * N;
+    HIPCHK(c, c->km_kmo.alloc(nkmo));
+    HIPCHK(c, launch_kmo_last(Ar.p, Ai.p, c->km_Ck.p, nf, N, c->ncl[0], c->ncl[1], c->ncl[2], o_hi, v_lo,
+                              c->km_kmo.p, st));
+    {
+        const int rc = kmp2_sums(c, o_hi, v_lo, out);
+        if (rc) return rc;
+    }
+    t_tr += now_ms() - tc;
+    out[4] = now_ms() - t0;
+    out[5] = t_int;
+    out[6] = t_tr;
+    out[7] = nslab;
     return TCG_OK;
 }
 

@@ -429,6 +429,7 @@ hipError_t launch_kmo(const double *Sre, const double *Sim, const double2 *Ck, i
 // j = c + nf R of X fused with the orbital contraction of that pair:
 //   Y[i + I (m + M k) + I M N o] =
 //       scale sum_c T[(k nf + c) M + m] sum_R (cos + i sgn sin)(k.cell_R) X[x(i) + sJ (c + nf R) + sO o]
+// (Y += with A.accum: the sum over R split into slabs of cells, one call each)
 // R over the NR input cells (cos/sin tables [R + NR k]), k over N points, x(i) =
 // imap[i] (a gather of the inner rows) or i, for the workgroup's chunk m in
 // [MR blockIdx.y, MR (blockIdx.y + 1)) of the M outputs.  The orbital index is
@@ -534,8 +535,13 @@ __global__ __launch_bounds__(256) void k_dft_mo(DftMoArgs A) {
                         const int m = m0 + mm;
                         if (m >= M) continue;
                         const size_t o = ob_off + (size_t)I * (m + (size_t)M * k);
-                        A.Yr[o] = A.scale * yr[mm][tt][v];
-                        A.Yi[o] = A.scale * yi[mm][tt][v];
+                        if (A.accum) {
+                            A.Yr[o] += A.scale * yr[mm][tt][v];
+                            A.Yi[o] += A.scale * yi[mm][tt][v];
+                        } else {
+                            A.Yr[o] = A.scale * yr[mm][tt][v];
+                            A.Yi[o] = A.scale * yi[mm][tt][v];
+                        }
                     }
                 }
             }

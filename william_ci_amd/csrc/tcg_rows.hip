@@ -1071,8 +1071,10 @@ __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, 
             if (A.Qao[(size_t)f4[0] + nb * f4[1]] * A.Qao[(size_t)f4[2] + nb * f4[3]] < kSchwarz) continue;
             int pp[4];
             stored_position(A, f4, ip, pp);
-            const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * (size_t)pp[3]))
-                                           : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * (size_t)pp[3]));
+            if (A.j4hi > 0 && (pp[3] < A.j4lo || pp[3] >= A.j4hi)) continue;   // outside the slab
+            const size_t p3 = (size_t)(pp[3] - A.j4lo);
+            const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * p3))
+                                           : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * p3));
             A.eri[idx] = blk[comp];
         }
     }

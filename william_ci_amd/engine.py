@@ -41,6 +41,7 @@ def lib():
         L.tcg_jk_stored_order.argtypes = [_vp, _ci, _ci, _dp, _vp, _ci, _dp, _dp]
         L.tcg_mp2_gamma.argtypes = [_vp, _ci, _ci, _dp, _dp, _dp]
         L.tcg_kmp2.argtypes = [_vp, _ci, _dp, _dp, _ip, _dp]
+        L.tcg_kmp2_direct.argtypes = [_vp, _ci, _dp, _dp, _ip, _ci, _dp]
         L.tcg_one_electron.argtypes = [_vp, _ci, _dp, _ip, _dp, _dp, _dp]
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
@@ -228,12 +229,9 @@ class Engine:
         return {"ec": out[0], "direct": out[1], "exchange": out[2], "gpu_ms": out[3]}
 
     # -- transform_ERI_AO_to_kMO + kMP2_calc --------------------------------------
-    def kmp2(self, mo_coef: np.ndarray, mo_energies: np.ndarray, occupied: np.ndarray):
-        """k-point MP2 from the resident SAO ERI (eri_build(sao=True)).
-        mo_coef[k] (nfuc x nfuc, columns = bands), mo_energies[k, band],
-        occupied[k, band] (bool), k ordered as the cells (x fastest) -- the
-        fields of scf.rhf_sao's result.  Returns dict(ec, direct, exchange,
-        zero_denominators, gpu_ms)."""
+    def _kblocks(self, mo_coef, mo_energies, occupied):
+        """coeff_SAO (column-major (re, im) pairs, k-block diagonal), e and occ
+        (2 / 0) in the k-block order of the C ABI."""
         s = self.s
         nb, nf = s.nbas, s.nfuc
         N = nb // nf
@@ -243,10 +241,32 @@ class Engine:
         cf = np.ascontiguousarray(C.T).view(np.float64).ravel()           # column-major, (re, im) pairs
         e = np.ascontiguousarray(np.asarray(mo_energies, dtype=np.float64).reshape(N * nf))
         occ = np.ascontiguousarray(np.where(np.asarray(occupied).reshape(N * nf), 2, 0).astype(np.int32))
+        return cf, e, occ
+
+    def kmp2(self, mo_coef: np.ndarray, mo_energies: np.ndarray, occupied: np.ndarray):
+        """k-point MP2 from the resident SAO ERI (eri_build(sao=True)).
+        mo_coef[k] (nfuc x nfuc, columns = bands), mo_energies[k, band],
+        occupied[k, band] (bool), k ordered as the cells (x fastest) -- the
+        fields of scf.rhf_sao's result.  Returns dict(ec, direct, exchange,
+        zero_denominators, gpu_ms)."""
+        nb = self.s.nbas
+        cf, e, occ = self._kblocks(mo_coef, mo_energies, occupied)
         out = np.zeros(5)
         self._check(lib().tcg_kmp2(self.h, nb, cf, e, occ, out), "tcg_kmp2")
         return {"ec": out[0], "direct": out[1], "exchange": out[2], "zero_denominators": int(out[3]),
                 "gpu_ms": out[4]}
+
+    def kmp2_direct(self, mo_coef: np.ndarray, mo_energies: np.ndarray, occupied: np.ndarray, slab_cells: int = 0):
+        """k-point MP2 without the stored tensor (tcg_kmp2_direct, after
+        direct_init): the SAO integrals in slabs of slab_cells cells of the last
+        index (0: sized from the free device memory).  Same arguments and keys as
+        kmp2, plus integral_ms, transform_ms and slabs."""
+        nb, nf = self.s.nbas, self.s.nfuc
+        cf, e, occ = self._kblocks(mo_coef, mo_energies, occupied)
+        out = np.zeros(8)
+        self._check(lib().tcg_kmp2_direct(self.h, nb, cf, e, occ, int(slab_cells), out), "tcg_kmp2_direct")
+        return {"ec": out[0], "direct": out[1], "exchange": out[2], "zero_denominators": int(out[3]),
+                "gpu_ms": out[4], "integral_ms": out[5], "transform_ms": out[6], "slabs": int(out[7])}
 
     # -- direct_init / direct_JK -------------------------------------------------
     def direct_init(self):
