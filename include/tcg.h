@@ -290,7 +290,8 @@ int tcg_eri_blocks(tcg_ctx *ctx, int n, const int *shells, double *out, long lon
  *   params: tcg_mlp_param_count() floats, W1[6][64] b1 W2[64][64] b2 W3[64][2] b3
  *   norm:   24 floats, feature mean[6], std[6], lowest[6], highest[6]
  *   orders: norders (<= 15) ascending node counts (4..128)
- * Trained on Tori1D systems: tcg_eri_build_learned refuses ndp > 1.
+ * The features are defined for 1, 2 and 3 periodic directions (chord
+ * distances along the periodic ones, the shortest periodic length as L).
  * rule_counts (norders + 2 entries, may be NULL): shell quartets per order,
  * then those on the plain rule, then those skipped (tcg_set_learned_skip).
  *
@@ -307,6 +308,14 @@ int tcg_set_learned(tcg_ctx *ctx, const float *params, const float *norm, const 
 int tcg_set_learned_skip(tcg_ctx *ctx, double thresh, double schwarz_cap, float margin_b);
 int tcg_eri_build_learned(tcg_ctx *ctx, int sao, int physicist, double *eri, long long *rule_counts);
 
+/* tcg_direct_init with the learned rules: the unique quartet list of the
+ * direct J/K, each quartet with its learned rule (and learned skipping, when
+ * set), evaluated once here; every later tcg_direct_jk (and the direct source
+ * of tcg_rhf_sao) runs that list.  tcg_direct_init goes back to the plain rule.
+ * rule_counts as tcg_eri_build_learned's.  No counterpart in the reference: a
+ * form of direct_init (src/module/eri_direct.f90:76-225). */
+int tcg_direct_init_learned(tcg_ctx *ctx, long long *rule_counts);
+
 /* The network itself (no context): parameter count, inference, one
  * forward + backward (loss = mean over samples of w0 (y0 - t0)^2 +
  * w1 (y1 - t1)^2; grad receives dloss/dparams) and Adam training on the GPU
@@ -320,7 +329,7 @@ int tcg_mlp_grad(const float *params, const float *X, const float *T, int n, con
 int tcg_mlp_train(float *params, const float *X, const float *T, int n, int epochs, int batch, float lr,
                   unsigned seed, const float w[2], float *loss_hist);
 
-/* Counters of the last tcg_eri_build:
+/* Counters of the last tcg_eri_build or tcg_direct_jk:
  *   out[0] shell quartets evaluated   out[1] primitive quartets evaluated
  *   out[2] rows (primitive quartet x direction x power combination)
  *   out[3] algorithmic FP64 flops of the quartet kernel: per primitive quartet

@@ -265,6 +265,10 @@ module tcg_engine
       type(c_ptr), value :: ctx, eri, rule_counts
       integer(c_int), value :: sao, physicist
     end function
+    integer(c_int) function tcg_direct_init_learned(ctx, rule_counts) bind(C, name="tcg_direct_init_learned")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, rule_counts
+    end function
     integer(c_int) function tcg_mlp_param_count() bind(C, name="tcg_mlp_param_count")
       import :: c_int
     end function

@@ -79,3 +79,38 @@ def test_feature_large_torus_limit():
     assert abs(f[2] - math.log10(1 + rho * (xP - xQ) ** 2)) < 1e-6
     lk = -(e[0] * e[1] / P * 1.0 + e[2] * e[3] / Q * 1.5 ** 2) / math.log(10)
     assert abs(f[5] - lk / 10) < 1e-6
+
+
+def _feat3(e, xyz, L3, ndp, ls=(0, 0, 0, 0)):
+    from william_ci_amd import learned
+    return learned.features(np.array([e]), np.array([xyz], dtype=np.float64), np.array([sum(ls)]),
+                            np.array([L3], dtype=np.float64), ndp)[0]
+
+
+def test_features_more_periodic_directions():
+    """2D/3D tori: with every centre on one line along x, the periodic y and z
+    directions add nothing and the features are the Tori1D ones when x has the
+    shortest length; the features do not depend on which axis is which, and
+    are invariant under a common translation along every periodic direction."""
+    e = (0.3, 1.7, 0.9, 5.0)
+    x = (1.0, 4.0, 7.5, 9.0)
+    f1 = _feat(e, x, 12.0)
+    on_x = [(v, 0.5, -1.0) for v in x]
+    assert np.allclose(_feat3(e, on_x, (12.0, 20.0, 0.0), 2), f1, atol=1e-12)
+    assert np.allclose(_feat3(e, on_x, (12.0, 20.0, 30.0), 3), f1, atol=1e-12)
+    pts = [(1.0, 2.0, 3.0), (4.0, 0.5, 7.0), (7.5, 9.0, 1.0), (9.0, 3.0, 5.5)]
+    L3 = (12.0, 15.0, 18.0)
+    f = _feat3(e, pts, L3, 3)
+    perm = (2, 0, 1)                                # axes relabelled
+    assert np.allclose(_feat3(e, [tuple(p[i] for i in perm) for p in pts], tuple(L3[i] for i in perm), 3), f,
+                       atol=1e-12)
+    assert np.allclose(_feat3(e, [(p[0] + 1.1, p[1] - 0.7, p[2] + 2.3) for p in pts], L3, 3), f, atol=1e-12)
+    # 2D: the real z direction enters as an ordinary distance; large periodic lengths
+    # give the ordinary Gaussian-product features
+    big = _feat3(e, pts, (1e6, 1e6, 0.0), 2)
+    P, Q = e[0] + e[1], e[2] + e[3]
+    cP = [(e[0] * pts[0][d] + e[1] * pts[1][d]) / P for d in range(3)]
+    cQ = [(e[2] * pts[2][d] + e[3] * pts[3][d]) / Q for d in range(3)]
+    rho = P * Q / (P + Q)
+    R2 = sum((cP[d] - cQ[d]) ** 2 for d in range(3))
+    assert abs(big[2] - math.log10(1 + rho * R2)) < 1e-6

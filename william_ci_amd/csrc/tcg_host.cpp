@@ -130,6 +130,10 @@ struct tcg_ctx {
     double learned_skip_cap = 0.0;     // ... only below this Schwarz product (0: no cap)
     float learned_margin_b = 1.5f;     // margin on the bound head, log10 units
     DevBuf<Quartet> d_q4keep;          // the quartets a learned build evaluates (skipped ones removed)
+    bool direct_learned = false;       // the direct J/K runs the learned list below
+    DevBuf<Quartet> d_q4dl;            // ... its quartets (skipped ones removed) and their rules
+    DevBuf<int> d_qrdl;
+    int nq4dl = 0;
     DevBuf<int> d_qrule_keep;
     DevBuf<float> d_mlp, d_feat, d_pred;
     DevBuf<int> d_qrule;
@@ -741,7 +745,12 @@ static int prepare(tcg_ctx *c) {
 // their features and the network's predictions, one rule per shell quartet
 // (the largest asked for, plain rule outside the training range), then the
 // row pipeline once per rule on the quartets of that rule.
-static int learned_pass(tcg_ctx *c, EriArgs A) {
+// The learned rule of every quartet of A.q4[0..A.nq4) (features of the surviving
+// primitive quartets, the network, the largest order per shell quartet): c->d_qrule,
+// per-rule counts in c->learned_counts.  With skipping on, or gather set, the kept
+// quartets and their rules are gathered into q4_out / rule_out (capacity nq4) and
+// A is pointed at them; otherwise A.qrule = c->d_qrule.
+static int learned_rules(tcg_ctx *c, EriArgs &A, bool gather, DevBuf<Quartet> &q4_out, DevBuf<int> &rule_out) {
     const int nq4 = A.nq4;
     const int nord = (int)c->learned_orders.size();
     for (int r = 0; r <= nord + 1 && r < 16; ++r) c->learned_counts[r] = 0;
@@ -786,7 +795,7 @@ static int learned_pass(tcg_ctx *c, EriArgs A) {
     HIPCHK(c, hipMemcpyAsync(hist, c->d_hist.p, sizeof hist, hipMemcpyDeviceToHost, c->st));
     // one pass over all quartets, each with its rule (rows sorted by rule, direction, length)
     A.qrule = c->d_qrule.p;
-    if (skipping) {
+    if (skipping || gather) {
         // the skipped quartets leave the list (their elements stay zero in the array)
         DevBuf<unsigned char> keep;
         DevBuf<long long> idx, nsel;
@@ -797,25 +806,33 @@ static int learned_pass(tcg_ctx *c, EriArgs A) {
         const size_t tb2 = select_temp_bytes(nq4);
         HIPCHK(c, c->d_tmp.alloc(tb2));
         HIPCHK(c, select_flagged(c->d_tmp.p, tb2, keep.p, idx.p, nsel.p, nq4, c->st));
-        HIPCHK(c, c->d_q4keep.alloc(nq4));
-        HIPCHK(c, c->d_qrule_keep.alloc(nq4));
+        HIPCHK(c, q4_out.alloc(nq4));
+        HIPCHK(c, rule_out.alloc(nq4));
         HIPCHK(c, launch_gather_quartets(A.q4, c->d_qrule.p, idx.p,
-                                         reinterpret_cast<const unsigned long long *>(nsel.p), nq4, c->d_q4keep.p,
-                                         c->d_qrule_keep.p, c->st));
+                                         reinterpret_cast<const unsigned long long *>(nsel.p), nq4, q4_out.p,
+                                         rule_out.p, c->st));
         long long kept = 0;
         HIPCHK(c, hipMemcpyAsync(&kept, nsel.p, sizeof kept, hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
-        A.q4 = c->d_q4keep.p;
+        A.q4 = q4_out.p;
         A.nq4 = (int)kept;
         A.nq4_dev = nullptr;
-        A.qrule = c->d_qrule_keep.p;
+        A.qrule = rule_out.p;
+    } else {
+        HIPCHK(c, hipStreamSynchronize(c->st));
     }
-    const int rc = A.nq4 > 0 ? run_quartets(c, A) : 0;
-    if (rc) return rc;
     for (int r = 0; r < nord; ++r) c->learned_counts[r] = (long long)hist[r + 1];
     c->learned_counts[nord] = (long long)hist[0];
     c->learned_counts[nord + 1] = (long long)hist[15];
     return 0;
+}
+
+// A learned ERI build: the rules, then one pass over the kept quartets, each with
+// its rule (rows sorted by rule, direction, length)
+static int learned_pass(tcg_ctx *c, EriArgs A) {
+    int rc = learned_rules(c, A, false, c->d_q4keep, c->d_qrule_keep);
+    if (rc) return rc;
+    return A.nq4 > 0 ? run_quartets(c, A) : 0;
 }
 
 // Step 5 (ERI_toroidal.f90:342-401): unique shell quartets (translation x 8
@@ -1030,12 +1047,28 @@ static int direct_jk_device(tcg_ctx *c, const double *dP, double *dJ, double *dK
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
     A.nq4 = c->nq4;
+    if (c->direct_learned) {                    // per-quartet learned rules (tcg_direct_init_learned)
+        A.q4 = c->d_q4dl.p;
+        A.nq4 = c->nq4dl;
+        A.qrule = c->d_qrdl.p;
+    }
     A.screen_prim = 1;
     A.Qao = c->d_Q.p;
     A.Pden = dP;
     A.Jacc = dJ;
     A.Kacc = dK;
-    return run_quartets(c, A);
+    const int rc = run_quartets(c, A);
+    if (rc) return rc;
+    // counters of this J/K (tcg_eri_stats): quartets, survivors, rows, flops
+    unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(cnt, c->d_cnt.p, sizeof cnt, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->stats[0] = A.nq4;
+    c->stats[1] = c->last_surv;
+    c->stats[2] = c->last_rows;
+    c->stats[3] = (long long)(cnt[2] + cnt[4]);
+    c->stats[4] = (long long)cnt[4];
+    return 0;
 }
 
 // Read access for the density-fitting driver (tcg_df.cpp), same library.
@@ -1349,6 +1382,31 @@ int tcg_direct_init(tcg_ctx *c) {
     rc = setup_quartets(c, t2, t3);
     if (rc) return rc;
     c->direct_ready = true;
+    c->direct_learned = false;
+    return TCG_OK;
+}
+
+int tcg_direct_init_learned(tcg_ctx *c, long long *rule_counts) {
+    if (!c) return TCG_E_ARG;
+    if (c->learned_orders.empty()) return c->fail(TCG_E_STATE, "tcg_direct_init_learned before tcg_set_learned");
+    int rc = tcg_direct_init(c);
+    if (rc) return rc;
+    if (c->nq4 < 0) {
+        rc = enumerate_quartets(c, false);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), c->st));
+    EriArgs A = base_args(c);
+    A.q4 = c->d_q4.p;
+    A.nq4 = c->nq4;
+    A.screen_prim = 1;
+    A.Qao = c->d_Q.p;
+    rc = learned_rules(c, A, true, c->d_q4dl, c->d_qrdl);
+    if (rc) return rc;
+    c->nq4dl = A.nq4;
+    c->direct_learned = true;
+    if (rule_counts)
+        for (size_t r = 0; r <= c->learned_orders.size() + 1; ++r) rule_counts[r] = c->learned_counts[r];
     return TCG_OK;
 }
 
@@ -1981,7 +2039,6 @@ int tcg_set_learned_skip(tcg_ctx *c, double thresh, double schwarz_cap, float ma
 int tcg_eri_build_learned(tcg_ctx *c, int sao, int physicist, double *eri, long long *rule_counts) {
     if (!c) return TCG_E_ARG;
     if (c->learned_orders.empty()) return c->fail(TCG_E_STATE, "tcg_eri_build_learned before tcg_set_learned");
-    if (c->ndp != 1) return c->fail(TCG_E_ARG, "tcg_eri_build_learned: the network is trained on Tori1D systems");
     c->learned_on = true;
     const int rc = tcg_eri_build(c, sao, physicist, eri);
     c->learned_on = false;

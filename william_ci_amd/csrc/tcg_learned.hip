@@ -10,12 +10,14 @@
 //                learned order r.
 //   k_rule_hist  shell quartets per rule.
 //
-// Features of the primitive quartet (e1 A, e2 B | e3 C, e4 D) on a torus of
-// length L (periodic direction x, chord distances, a = 2 pi / L):
+// Features of the primitive quartet (e1 A, e2 B | e3 C, e4 D) on a torus with
+// ndp periodic directions d < ndp of lengths L_d (a_d = 2 pi / L_d, chord
+// distances 2 (1 - cos a_d x) / a_d^2) and real directions d >= ndp, L = the
+// shortest periodic length:
 //   f0 = log10 rho,  rho = P Q / (P + Q),  P = e1 + e2,  Q = e3 + e4
 //   f1 = log10 (rho L^2)
-//   f2 = log10 (1 + rho R^2), R = distance of the pair centres (x: circular
-//        weighted mean, chord; y, z: weighted mean)
+//   f2 = log10 (1 + rho R^2), R = distance of the pair centres (periodic
+//        directions: circular weighted mean, chord; real ones: weighted mean)
 //   f3 = (lA + lB + lC + lD) / 4
 //   f4 = log10 (max(e) L^2)
 //   f5 = max(log10 (K_AB K_CD), -30) / 10,  K_AB = exp(-e1 e2 / P d_AB^2)
@@ -27,24 +29,35 @@ namespace tcg {
 
 __device__ __forceinline__ double chord2(double a, double dx) { return 2.0 * (1.0 - cos(a * dx)) / (a * a); }
 
-__device__ void pair_centre(double a, double e1, const double *A, double e2, const double *B, double *P) {
-    const double s = e1 * sin(a * A[0]) + e2 * sin(a * B[0]);
-    const double cc = e1 * cos(a * A[0]) + e2 * cos(a * B[0]);
-    P[0] = atan2(s, cc) / a;
-    P[1] = (e1 * A[1] + e2 * B[1]) / (e1 + e2);
-    P[2] = (e1 * A[2] + e2 * B[2]) / (e1 + e2);
+__device__ void pair_centre(int ndp, const double *a, double e1, const double *A, double e2, const double *B,
+                            double *P) {
+    for (int d = 0; d < 3; ++d) {
+        if (d < ndp) {
+            const double s = e1 * sin(a[d] * A[d]) + e2 * sin(a[d] * B[d]);
+            const double cc = e1 * cos(a[d] * A[d]) + e2 * cos(a[d] * B[d]);
+            P[d] = atan2(s, cc) / a[d];
+        } else {
+            P[d] = (e1 * A[d] + e2 * B[d]) / (e1 + e2);
+        }
+    }
 }
 
-__device__ double pair_d2(double a, const double *A, const double *B) {
-    const double dy = A[1] - B[1], dz = A[2] - B[2];
-    return chord2(a, A[0] - B[0]) + dy * dy + dz * dz;
+__device__ double pair_d2(int ndp, const double *a, const double *A, const double *B) {
+    double r = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double dx = A[d] - B[d];
+        r += d < ndp ? chord2(a[d], dx) : dx * dx;
+    }
+    return r;
 }
 
 __global__ __launch_bounds__(256) void k_features(EriArgs A, const int *__restrict__ owner, long long ns,
                                                   const Surv *__restrict__ surv, const double *__restrict__ expo,
                                                   const float *__restrict__ norm, float *__restrict__ feat,
                                                   unsigned char *__restrict__ ood) {
-    const double a = A.a[0], L = A.L[0];
+    const int ndp = A.ndp;
+    double L = A.L[0];
+    for (int d = 1; d < ndp; ++d) L = fmin(L, A.L[d]);
     for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < ns; g += (long long)gridDim.x * blockDim.x) {
         const Quartet &Q = A.q4[owner[g]];
         const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
@@ -56,12 +69,12 @@ __global__ __launch_bounds__(256) void k_features(EriArgs A, const int *__restri
             const double e3 = expo[A.ao_poff[sC.first_ao] + pk.pi], e4 = expo[A.ao_poff[sD.first_ao] + pk.pj];
             const double P = e1 + e2, Qe = e3 + e4, rho = P * Qe / (P + Qe);
             double Pc[3], Qc[3];
-            pair_centre(a, e1, sA.xyz, e2, sB.xyz, Pc);
-            pair_centre(a, e3, sC.xyz, e4, sD.xyz, Qc);
-            const double R2 = pair_d2(a, Pc, Qc);
+            pair_centre(ndp, A.a, e1, sA.xyz, e2, sB.xyz, Pc);
+            pair_centre(ndp, A.a, e3, sC.xyz, e4, sD.xyz, Qc);
+            const double R2 = pair_d2(ndp, A.a, Pc, Qc);
             const double emax = fmax(fmax(e1, e2), fmax(e3, e4));
-            const double lk = -(e1 * e2 / P * pair_d2(a, sA.xyz, sB.xyz) + e3 * e4 / Qe * pair_d2(a, sC.xyz, sD.xyz)) /
-                              2.302585092994046;
+            const double lk = -(e1 * e2 / P * pair_d2(ndp, A.a, sA.xyz, sB.xyz) +
+                                e3 * e4 / Qe * pair_d2(ndp, A.a, sC.xyz, sD.xyz)) / 2.302585092994046;
             double f[6];
             f[0] = log10(rho);
             f[1] = log10(rho * L * L);

@@ -1060,17 +1060,30 @@ __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, 
         for (int i = lane; i < nblk; i += nl) A.blk_out[Q.out + i] = blk[i];
     } else if (A.eri) {
         const size_t nb = A.nbas, n1 = A.n1;
+        // nl is a multiple of 8, so a lane keeps one permutation ip for all its tasks;
+        // its stored positions are those of component (0,0,0,0) plus the components'
+        // offsets (a shell lies inside one cell, so the translation is the same for
+        // all its components)
+        const int ip = lane & 7;
+        const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
+        int pb[4], px[4];
+        stored_position(A, f0, ip, pb);
+#pragma unroll
+        for (int x = 0; x < 4; ++x) px[x] = c_perm8_rows[ip][x];
         for (int task = lane; task < nblk * 8; task += nl) {
-            const int comp = task >> 3, ip = task & 7;
+            const int comp = task >> 3;
             int t = comp;
             const int ca = t % nA; t /= nA;
             const int cb = t % nB; t /= nB;
             const int cc = t % nC;
             const int cd = t / nC;
-            const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
-            if (A.Qao[(size_t)f4[0] + nb * f4[1]] * A.Qao[(size_t)f4[2] + nb * f4[3]] < kSchwarz) continue;
+            const int c4[4] = {ca, cb, cc, cd};
+            if (A.Qao[(size_t)(f0[0] + ca) + nb * (f0[1] + cb)] * A.Qao[(size_t)(f0[2] + cc) + nb * (f0[3] + cd)] <
+                kSchwarz)
+                continue;
             int pp[4];
-            stored_position(A, f4, ip, pp);
+#pragma unroll
+            for (int x = 0; x < 4; ++x) pp[x] = pb[x] + c4[px[x]];
             if (A.j4hi > 0 && (pp[3] < A.j4lo || pp[3] >= A.j4hi)) continue;   // outside the slab
             const size_t p3 = (size_t)(pp[3] - A.j4lo);
             const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * p3))
@@ -1198,28 +1211,56 @@ __device__ __forceinline__ bool shell_stabilized(const EriArgs &A, const Shell &
     return stab;
 }
 
-// J and K of one finished block (the direct path): every counted element adds
-// J(p1,p2) += E P(p3,p4), K(p1,p4) -= E P(p2,p3) / 2 into Jl, Kl (LDS or global,
-// leading dimension nfuc), by nl lanes.
+// Zero the elements of a finished block that the element screen drops
+// (Q(p1,p2) Q(p3,p4) < 1e-13, ERI_toroidal.f90:422), by nl lanes: the J/K
+// epilogues then skip zeros instead of screening each (element, permutation).
+__device__ __forceinline__ void screen_block(const EriArgs &A, const Shell &sA, const Shell &sB, const Shell &sC,
+                                             const Shell &sD, double *blk, int lane, int nl) {
+    const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
+    const int nblk = nA * nB * nC * nD;
+    const size_t nb = A.nbas;
+    for (int comp = lane; comp < nblk; comp += nl) {
+        int t = comp;
+        const int ca = t % nA; t /= nA;
+        const int cb = t % nB; t /= nB;
+        const int cc = t % nC;
+        const int cd = t / nC;
+        if (A.Qao[(size_t)(sA.first_ao + ca) + nb * (sB.first_ao + cb)] *
+                A.Qao[(size_t)(sC.first_ao + cc) + nb * (sD.first_ao + cd)] < kSchwarz)
+            blk[comp] = 0.0;
+    }
+}
+
+// J and K of one finished, screened block (the direct path): every counted
+// element adds J(p1,p2) += E P(p3,p4), K(p1,p4) -= E P(p2,p3) / 2 into Jl, Kl (LDS
+// or global, leading dimension nfuc), by nl lanes (a multiple of 8: a lane keeps
+// one permutation, its positions are those of component (0,0,0,0) plus offsets).
 __device__ __forceinline__ void jk_block(const EriArgs &A, const Shell &sA, const Shell &sB, const Shell &sC,
                                          const Shell &sD, const double *blk, int lane, int nl, bool stab, double *Jl,
                                          double *Kl) {
     const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
     const int nblk = nA * nB * nC * nD;
     const int nf = A.nfuc, nb = A.nbas;
+    const int ip = lane & 7;
+    const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
+    int pb[4], px[4];
+    stored_position(A, f0, ip, pb);
+#pragma unroll
+    for (int x = 0; x < 4; ++x) px[x] = c_perm8_rows[ip][x];
     for (int task = lane; task < nblk * 8; task += nl) {
-        const int comp = task >> 3, ip = task & 7;
+        const int comp = task >> 3;
+        const double val = blk[comp];
+        if (val == 0.0) continue;
         int t = comp;
         const int ca = t % nA; t /= nA;
         const int cb = t % nB; t /= nB;
         const int cc = t % nC;
         const int cd = t / nC;
-        const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
-        if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz) continue;
+        const int c4[4] = {ca, cb, cc, cd};
         int pp[4];
-        stored_position(A, f4, ip, pp);
+#pragma unroll
+        for (int x = 0; x < 4; ++x) pp[x] = pb[x] + c4[px[x]];
         if (stab && !first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
-        const double val = blk[comp];
         atomicAdd(&Jl[pp[0] + (size_t)nf * pp[1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
         atomicAdd(&Kl[pp[0] + (size_t)nf * pp[3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
     }
@@ -1258,8 +1299,13 @@ __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__res
         wave_sync();
         if (e < n) {
             const Shell &sA = A.sh[Q.s[0]], &sB = A.sh[Q.s[1]], &sC = A.sh[Q.s[2]], &sD = A.sh[Q.s[3]];
-            if (JK) jk_block(A, sA, sB, sC, sD, blk, gl, GL, shell_stabilized(A, sA, sB, sC, sD), Jl, Kl);
-            else write_block(A, Q, sA, sB, sC, sD, blk, gl, GL);
+            if (JK) {
+                screen_block(A, sA, sB, sC, sD, blk, gl, GL);
+                wave_sync();
+                jk_block(A, sA, sB, sC, sD, blk, gl, GL, shell_stabilized(A, sA, sB, sC, sD), Jl, Kl);
+            } else {
+                write_block(A, Q, sA, sB, sC, sD, blk, gl, GL);
+            }
         }
         wave_sync();
     }
@@ -1324,16 +1370,17 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                                                      const Surv *__restrict__ surv, const double *__restrict__ rowval,
                                                      const double *__restrict__ realval, int jk_lds,
                                                      const int *__restrict__ list, const unsigned *__restrict__ cnt) {
-    // LDS: [4][max_blk] blocks, [4][2][36] per-wave J/K sub-blocks, [4][64] partial
-    // sums of the contraction, then J[nfuc*nbas], K[nfuc*nbas] when jk_lds
+    // LDS: [4][max_blk] blocks, [4][4][36] per-wave J/K sub-blocks and the P sub-blocks
+    // they read, [4][64] partial sums of the contraction, then J[nfuc*nbas],
+    // K[nfuc*nbas] when jk_lds
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nf = A.nfuc, nb = A.nbas;
     const size_t njk = (size_t)nf * nb;
     double *blk = lds + (size_t)wv * A.max_blk;
-    double *Jw = lds + 4 * (size_t)A.max_blk + 72 * wv, *Kw = Jw + 36;
-    double *part = lds + 4 * (size_t)A.max_blk + 4 * 72 + 64 * wv;
-    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 72 + 4 * 64 : A.Jacc;
+    double *Jw = lds + 4 * (size_t)A.max_blk + 144 * wv, *Kw = Jw + 36, *Pj = Jw + 72, *Pk = Jw + 108;
+    double *part = lds + 4 * (size_t)A.max_blk + 4 * 144 + 64 * wv;
+    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 144 + 4 * 64 : A.Jacc;
     double *Kl = jk_lds ? Jl + njk : A.Kacc;
     if (jk_lds) {
         for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
@@ -1350,6 +1397,8 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         wave_sync();
         quartet_block(A, Q, quartet_rule(A, qi), s0, ns, r0, surv, rowval, realval, blk, part, lane);
         wave_sync();
+        screen_block(A, sA, sB, sC, sD, blk, lane, 64);
+        wave_sync();
         const bool stab = shell_stabilized(A, sA, sB, sC, sD);
         if (jk_lds) {
             // J/K in LDS: every counted element adds there directly
@@ -1358,45 +1407,50 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         }
         // J/K in global memory: one permutation at a time, the counted elements of the
         // block add into the (n0 x n1) J and (n0 x n3) K sub-blocks the permutation maps
-        // them to (LDS), which then go out with one atomic per entry instead of one per
-        // element
+        // them to (LDS), reading the (n2 x n3) and (n1 x n2) sub-blocks of P they need
+        // (staged in LDS first); the sub-blocks then go out with one atomic per entry.
+        // A component's stored position is that of component (0,0,0,0), pb, plus its
+        // offsets (a shell lies inside one cell).
         const int ncmp[4] = {nA, nB, nC, nD};
         const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
         for (int ip = 0; ip < 8; ++ip) {
-            const int x0 = c_perm8_rows[ip][0], x1 = c_perm8_rows[ip][1], x3 = c_perm8_rows[ip][3];
-            const int n0 = ncmp[x0], n1 = ncmp[x1], n3 = ncmp[x3];
+            const int x0 = c_perm8_rows[ip][0], x1 = c_perm8_rows[ip][1], x2 = c_perm8_rows[ip][2],
+                      x3 = c_perm8_rows[ip][3];
+            const int n0 = ncmp[x0], n1 = ncmp[x1], n2 = ncmp[x2], n3 = ncmp[x3];
+            int pb[4];
+            stored_position(A, f0, ip, pb);
             if (lane < 36) {
                 Jw[lane] = 0.0;
                 Kw[lane] = 0.0;
             }
+            if (lane < n2 * n3) Pj[lane] = A.Pden[(pb[2] + lane / n3) + (size_t)nb * (pb[3] + lane % n3)];
+            if (lane < n1 * n2) Pk[lane] = A.Pden[(pb[1] + lane / n2) + (size_t)nb * (pb[2] + lane % n2)];
             wave_sync();
             for (int comp = lane; comp < nblk; comp += 64) {
+                const double val = blk[comp];
+                if (val == 0.0) continue;
                 int t = comp;
                 const int ca = t % nA; t /= nA;
                 const int cb = t % nB; t /= nB;
                 const int cc = t % nC;
                 const int cd = t / nC;
-                const int f4[4] = {sA.first_ao + ca, sB.first_ao + cb, sC.first_ao + cc, sD.first_ao + cd};
-                if (A.Qao[(size_t)f4[0] + (size_t)nb * f4[1]] * A.Qao[(size_t)f4[2] + (size_t)nb * f4[3]] < kSchwarz)
-                    continue;
-                int pp[4];
-                stored_position(A, f4, ip, pp);
-                if (stab && !first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
                 const int c4[4] = {ca, cb, cc, cd};
-                const double val = blk[comp];
-                atomicAdd(&Jw[c4[x0] * n1 + c4[x1]], val * A.Pden[pp[2] + (size_t)nb * pp[3]]);
-                atomicAdd(&Kw[c4[x0] * n3 + c4[x3]], -0.5 * val * A.Pden[pp[1] + (size_t)nb * pp[2]]);
+                const int i0 = c4[x0], i1 = c4[x1], i2 = c4[x2], i3 = c4[x3];
+                if (stab) {
+                    const int pp[4] = {pb[0] + i0, pb[1] + i1, pb[2] + i2, pb[3] + i3};
+                    if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
+                }
+                atomicAdd(&Jw[i0 * n1 + i1], val * Pj[i2 * n3 + i3]);
+                atomicAdd(&Kw[i0 * n3 + i3], -0.5 * val * Pk[i1 * n2 + i2]);
             }
             wave_sync();
-            int pb[4];                                     // stored position of components (0,0,0,0)
-            stored_position(A, f0, ip, pb);
-            for (int e = lane; e < n0 * n1; e += 64) {
-                const double v = Jw[e];
-                if (v != 0.0) atomicAdd(&Jl[(pb[0] + e / n1) + (size_t)nf * (pb[1] + e % n1)], v);
+            for (int e2 = lane; e2 < n0 * n1; e2 += 64) {
+                const double v = Jw[e2];
+                if (v != 0.0) atomicAdd(&Jl[(pb[0] + e2 / n1) + (size_t)nf * (pb[1] + e2 % n1)], v);
             }
-            for (int e = lane; e < n0 * n3; e += 64) {
-                const double v = Kw[e];
-                if (v != 0.0) atomicAdd(&Kl[(pb[0] + e / n3) + (size_t)nf * (pb[3] + e % n3)], v);
+            for (int e2 = lane; e2 < n0 * n3; e2 += 64) {
+                const double v = Kw[e2];
+                if (v != 0.0) atomicAdd(&Kl[(pb[0] + e2 / n3) + (size_t)nf * (pb[3] + e2 % n3)], v);
             }
             wave_sync();
         }
@@ -1414,7 +1468,7 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
                               const double *rowval, const double *realval, const int *list, const unsigned *cnt,
                               hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 72 + 4 * 64);
+    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 144 + 4 * 64);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
     const size_t lds_max = 160 * 1024;
     // TCG_JK_GLOBAL=1 forces the global-memory accumulation (the path of systems

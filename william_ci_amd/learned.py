@@ -59,6 +59,7 @@ def _lib():
         L.tcg_eri_build_learned.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                             np.ctypeslib.ndpointer(dtype=np.int64)]
         L.tcg_set_learned_skip.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_float]
+        L.tcg_direct_init_learned.argtypes = [ctypes.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64)]
         L.tcg_eri_blocks.argtypes = [ctypes.c_void_p, ctypes.c_int, np.ctypeslib.ndpointer(dtype=np.int32),
                                      np.ctypeslib.ndpointer(dtype=np.float64), np.ctypeslib.ndpointer(dtype=np.int64)]
         L._mlp_bound = True
@@ -74,54 +75,73 @@ def _chord2(a, dx):
     return 2.0 * (1.0 - np.cos(a * dx)) / (a * a)
 
 
-def features(e, xyz, lsum, L):
+def features(e, xyz, lsum, L, ndp=1):
     """e: (n, 4) exponents, xyz: (n, 4, 3) centres, lsum: (n,) total angular
-    momentum, L: (n,) torus lengths (periodic along x).  Returns (n, 6)."""
+    momentum, L: (n,) torus lengths (one periodic direction, x) or (n, 3)
+    lengths of the periodic directions d < ndp (ndp scalar or (n,)).  Chord
+    distances along the periodic directions, ordinary ones along the others;
+    L in the features is the shortest periodic length.  Returns (n, 6)."""
     e = np.asarray(e, dtype=np.float64)
-    a = 2.0 * math.pi / np.asarray(L, dtype=np.float64)
+    n = len(e)
+    L = np.asarray(L, dtype=np.float64)
+    if L.ndim == 1:
+        L3 = np.zeros((n, 3))
+        L3[:, 0] = L
+    else:
+        L3 = L
+    nd = np.broadcast_to(np.asarray(ndp), (n,))
+    per = np.arange(3)[None, :] < nd[:, None]                    # (n, 3) periodic directions
+    a = 2.0 * math.pi / np.where(per, L3, 1.0)
     P, Q = e[:, 0] + e[:, 1], e[:, 2] + e[:, 3]
     rho = P * Q / (P + Q)
 
     def centre(i, j):
-        s = e[:, i] * np.sin(a * xyz[:, i, 0]) + e[:, j] * np.sin(a * xyz[:, j, 0])
-        c = e[:, i] * np.cos(a * xyz[:, i, 0]) + e[:, j] * np.cos(a * xyz[:, j, 0])
-        x = np.arctan2(s, c) / a
-        yz = (e[:, i, None] * xyz[:, i, 1:] + e[:, j, None] * xyz[:, j, 1:]) / (e[:, i] + e[:, j])[:, None]
-        return x, yz
+        s_ = e[:, i, None] * np.sin(a * xyz[:, i]) + e[:, j, None] * np.sin(a * xyz[:, j])
+        c_ = e[:, i, None] * np.cos(a * xyz[:, i]) + e[:, j, None] * np.cos(a * xyz[:, j])
+        real = (e[:, i, None] * xyz[:, i] + e[:, j, None] * xyz[:, j]) / (e[:, i] + e[:, j])[:, None]
+        return np.where(per, np.arctan2(s_, c_) / a, real)
 
-    def d2(x1, yz1, x2, yz2):
-        return _chord2(a, x1 - x2) + np.sum((yz1 - yz2) ** 2, axis=1)
+    def d2(X, Y):
+        dx = X - Y
+        return np.sum(np.where(per, _chord2(a, dx), dx * dx), axis=1)
 
-    xP, yzP = centre(0, 1)
-    xQ, yzQ = centre(2, 3)
-    R2 = d2(xP, yzP, xQ, yzQ)
-    dAB = d2(xyz[:, 0, 0], xyz[:, 0, 1:], xyz[:, 1, 0], xyz[:, 1, 1:])
-    dCD = d2(xyz[:, 2, 0], xyz[:, 2, 1:], xyz[:, 3, 0], xyz[:, 3, 1:])
+    R2 = d2(centre(0, 1), centre(2, 3))
+    dAB = d2(xyz[:, 0], xyz[:, 1])
+    dCD = d2(xyz[:, 2], xyz[:, 3])
     lk = -(e[:, 0] * e[:, 1] / P * dAB + e[:, 2] * e[:, 3] / Q * dCD) / math.log(10.0)
-    Lv = np.asarray(L, dtype=np.float64)
+    Lv = np.min(np.where(per, L3, np.inf), axis=1)
     return np.stack([np.log10(rho), np.log10(rho * Lv * Lv), np.log10(1.0 + rho * R2), 0.25 * np.asarray(lsum),
                      np.log10(e.max(axis=1) * Lv * Lv), np.maximum(lk, -30.0) / 10.0], axis=1)
 
 
 # -- training data from the plain quadrature -----------------------------------
-def random_torus(rng, nshell: int):
-    """A Tori1D system of one cell: nshell single-primitive shells (l in
-    {0,1,2}), exponents log-uniform in [0.05, 50], centres uniform on the
-    torus (on the x axis, as in the chains), L uniform in [5, 120]."""
-    L = float(rng.uniform(5.0, 120.0))
+def random_torus(rng, nshell: int, ndp: int = 1):
+    """A torus of one cell with ndp periodic directions: nshell
+    single-primitive shells (l in {0,1,2}), exponents log-uniform in [0.05, 50],
+    lengths uniform in [5, 120] per periodic direction, centres uniform on the
+    torus along those directions (and uniform in [-2, 2] bohr along the real
+    ones, except for Tori1D, whose centres lie on the x axis as in the chains)."""
+    Ls = np.zeros(3)
+    Ls[:ndp] = rng.uniform(5.0, 120.0, size=ndp)
     ls = rng.choice([0, 1, 2], size=nshell, p=[0.5, 0.3, 0.2])
     ex = np.exp(rng.uniform(math.log(0.05), math.log(50.0), size=nshell))
-    xs = rng.uniform(0.0, L, size=nshell)
+    xs = np.zeros((nshell, 3))
+    for d in range(3):
+        if d < ndp:
+            xs[:, d] = rng.uniform(0.0, Ls[d], size=nshell)
+        elif ndp > 1:
+            xs[:, d] = rng.uniform(-2.0, 2.0, size=nshell)
     center, lxyz, expo, shell_of = [], [], [], []
     for s in range(nshell):
         for comp in _sys._CART[int(ls[s])]:
-            center.append((xs[s], 0.0, 0.0))
+            center.append(tuple(xs[s]))
             lxyz.append(comp)
             expo.append(ex[s])
             shell_of.append(s)
     nb = len(center)
-    sysm = _sys.TorusSystem(kind="Tori1D", ndp=1, L=(L, 0.0, 0.0), ncell=(1, 1, 1), spacing=L, keywords=[],
-                            geom=np.zeros((1, 3)), Z=np.array([1]), nauc=1,
+    kind = {1: "Tori1D", 2: "Tori2D", 3: "Tori3D"}[ndp]
+    sysm = _sys.TorusSystem(kind=kind, ndp=ndp, L=tuple(float(v) for v in Ls), ncell=(1, 1, 1),
+                            spacing=float(Ls[0]), keywords=[], geom=np.zeros((1, 3)), Z=np.array([1]), nauc=1,
                             center=np.array(center, dtype=np.float64), lxyz=np.array(lxyz, dtype=np.int32),
                             prim_off=np.arange(nb + 1, dtype=np.int32), expo=np.array(expo, dtype=np.float64),
                             coef=np.ones(nb))
@@ -164,19 +184,21 @@ def torus_samples(sysm, shells):
     e = np.stack([shells["e"][i], shells["e"][j], shells["e"][k], shells["e"][l]], axis=1)
     xyz = np.zeros((len(q), 4, 3))
     for c, s in enumerate((i, j, k, l)):
-        xyz[:, c, 0] = shells["x"][s]
+        xyz[:, c, :] = shells["x"][s]
     lsum = shells["l"][i] + shells["l"][j] + shells["l"][k] + shells["l"][l]
-    X = features(e, xyz, lsum, np.full(len(q), sysm.L[0]))
+    X = features(e, xyz, lsum, np.tile(np.asarray(sysm.L, dtype=np.float64), (len(q), 1)), sysm.ndp)
     bound = np.log10(np.maximum(big[q], 1e-30))
     return X, np.stack([bound, order[q].astype(np.float64)], axis=1)
 
 
-def make_dataset(n_tori: int, nshell: int = 12, seed: int = 12345, log=None):
+def make_dataset(n_tori: int, nshell: int = 12, seed: int = 12345, log=None, ndps=(1, 2, 3)):
+    """Features and labels of n_tori random tori, their periodic dimension
+    cycling through ndps."""
     rng = np.random.default_rng(seed)
     Xs, Ts = [], []
     t0 = time.perf_counter()
     for t in range(n_tori):
-        sysm, shells = random_torus(rng, nshell)
+        sysm, shells = random_torus(rng, nshell, ndps[t % len(ndps)])
         X, T = torus_samples(sysm, shells)
         Xs.append(X)
         Ts.append(T)
@@ -286,6 +308,15 @@ def eri_build_learned(eng, model: Model, sao=True, physicist=False, to_host=True
     return (out.reshape(nb, nb, nb, n1) if to_host else None), counts
 
 
+def direct_init_learned(eng, model: Model):
+    """direct_init with the learned rules (model installed on eng): every later
+    direct J/K (and the direct RHF) evaluates each quartet with its learned order;
+    returns the shell quartets per order, the plain rule, and skipped."""
+    counts = np.zeros(len(model.orders) + 2, dtype=np.int64)
+    eng._check(_lib().tcg_direct_init_learned(eng.h, counts), "tcg_direct_init_learned")
+    return counts
+
+
 def set_skip(eng, thresh: float, schwarz_cap: float = 1e-10, margin_b: float = 1.5) -> None:
     """Learned Schwarz-style skipping in the next learned builds (tcg_set_learned_skip):
     shell quartets whose predicted largest |integral| (bound head, plus log10 of
@@ -357,23 +388,40 @@ def spot_check(eng, E, n=256, rule=(True, 40), seed=7):
     return worst
 
 
+def evaluate(model: Model, X, T, margin_b: float = 1.5):
+    """Order and bound statistics of the model on samples (X, T): RMSEs, the
+    fraction of quartets whose order is under-predicted (after the margin), and
+    the fraction whose bound (plus margin_b) falls below the true log10 bound."""
+    Y = forward(model.params, model.normalise(X))
+    pred = np.ceil(np.maximum(Y[:, 1] + model.margin, 0.0)).astype(int)
+    return {"bound_rmse": float(np.sqrt(np.mean((Y[:, 0] - T[:, 0]) ** 2))),
+            "order_rmse": float(np.sqrt(np.mean((Y[:, 1] - T[:, 1]) ** 2))),
+            "under_orders": float(np.mean(pred < T[:, 1])),
+            "under_bound": float(np.mean(Y[:, 0] + margin_b < T[:, 0])), "n": int(len(X))}
+
+
 def main():
-    """Generate the training data with the plain quadrature, train on the GPU,
-    save the model (python -m william_ci_amd.learned [n_tori] [path])."""
+    """Generate the training data with the plain quadrature on tori with 1, 2
+    and 3 periodic directions, train on the GPU, check on held-out tori (another
+    seed) and save the model (python -m william_ci_amd.learned [n_tori] [path])."""
     import sys
-    n_tori = int(sys.argv[1]) if len(sys.argv) > 1 else 800
+    n_tori = int(sys.argv[1]) if len(sys.argv) > 1 else 1200
     path = sys.argv[2] if len(sys.argv) > 2 else MODEL_PATH
     log = lambda m: print(m, flush=True)
     X, T = make_dataset(n_tori, log=log)
     log(f"orders needed: {np.bincount(T[:, 1].astype(int), minlength=len(ORDERS) + 1).tolist()}")
     model, hist = fit(X, T, log=log)
-    Y = forward(model.params, model.normalise(X))
-    pred = np.ceil(np.maximum(Y[:, 1] + model.margin, 0.0)).astype(int)
-    under = np.mean(pred < T[:, 1])
-    log(f"train set: bound rmse {np.sqrt(np.mean((Y[:, 0] - T[:, 0]) ** 2)):.3f}, "
-        f"order rmse {np.sqrt(np.mean((Y[:, 1] - T[:, 1]) ** 2)):.3f}, under-predicted orders {under:.2e}")
-    model.save(path, loss=hist, n_samples=np.int64(len(X)), under=np.float64(under),
-               order_hist=np.bincount(T[:, 1].astype(int), minlength=len(ORDERS) + 1))
+    train = evaluate(model, X, T)
+    log(f"train set: {train}")
+    held = {}
+    for nd in (1, 2, 3):
+        Xh, Th = make_dataset(max(12, n_tori // 12), seed=777 + nd, ndps=(nd,))
+        held[nd] = evaluate(model, Xh, Th)
+        log(f"held-out Tori{nd}D: {held[nd]}")
+    model.save(path, loss=hist, n_samples=np.int64(len(X)), under=np.float64(train["under_orders"]),
+               order_hist=np.bincount(T[:, 1].astype(int), minlength=len(ORDERS) + 1),
+               heldout=np.array([[held[nd][k] for k in ("bound_rmse", "order_rmse", "under_orders", "under_bound",
+                                                        "n")] for nd in (1, 2, 3)]))
     log(f"saved {path}")
 
 
