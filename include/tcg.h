@@ -170,7 +170,9 @@ int tcg_heev_blocks(tcg_ctx *ctx, int nbas, double *A_sao, double *e);
  * reference's convergence and stall rules.  Every matrix stays on the GPU;
  * one iteration reads back a few scalars.
  *   source    0: J/K from the resident SAO ERI (tcg_eri_build, either order);
- *             1: integral-direct (tcg_direct_init first)
+ *             1: integral-direct (tcg_direct_init first);
+ *             2: integral-direct, incremental with density screening
+ *                (tcg_set_direct_screen)
  *   nocc      nO, doubly occupied orbitals over the supercell
  *   S, T, V   (nBas,nBas) one-electron matrices
  *   max_scf, thresh, diis   maxSCF, the orbital-error threshold, keyword DIIS
@@ -224,6 +226,14 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K);
  * followed by kMP2_calc (src/Energies/SAO/MP2/MP2_SAO.f90:6-151); arguments
  * and out[0..4] as tcg_kmp2. */
 int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out);
+
+/* The same in FP32: L (df_L_kap, :1053-1112) with FP32 arithmetic, the
+ * (ov|ov) assembly (the zgemm at df_toroidal.f90:1033) as FP32 matrix-core
+ * GEMMs with FP32 accumulation (one complex GEMM over Q per momentum transfer
+ * k1 - k2), and the kMP2_calc sums in FP32 kept as compensated (hi, lo) FP32
+ * pairs.  Arguments and out[0..4] as tcg_df_kmp2; the difference to
+ * tcg_df_kmp2 is the FP32 rounding of the integrals (~1e-7 relative). */
+int tcg_df_kmp2_fp32(tcg_df *df, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out);
 
 /* out[0] metric + Cholesky, out[1] Schwarz + 3-index integrals, out[2] B (ms). */
 int tcg_df_times(tcg_df *df, double *out);
@@ -315,6 +325,29 @@ int tcg_eri_build_learned(tcg_ctx *ctx, int sao, int physicist, double *eri, lon
  * rule_counts as tcg_eri_build_learned's.  No counterpart in the reference: a
  * form of direct_init (src/module/eri_direct.f90:76-225). */
 int tcg_direct_init_learned(tcg_ctx *ctx, long long *rule_counts);
+
+/* Density screening of the integral-direct J/K and the incremental Fock build
+ * of tcg_rhf_sao's source 2.  No counterpart in the reference (its direct_JK
+ * evaluates every Schwarz-surviving quartet at every iteration): a form of
+ * direct_JK (src/module/eri_direct.f90:262-374) for late SCF iterations.
+ *   tcg_rhf_sao with source 2: a full direct J/K every full_every iterations
+ *   (the first included); in between, J and K of the density change P - P_prev,
+ *   added to the previous J and K, where a shell quartet is left out when its
+ *   bound times the largest element of the six density-change blocks it
+ *   contracts with (pairs AB, CD, AC, AD, BC, BD) is below tau.  The bound is
+ *   the Schwarz product Qsh(AB) Qsh(CD), or (use_learned_bound, with the
+ *   learned list of tcg_direct_init_learned) the smaller of it and the
+ *   network's bound.  Defaults: tau 1e-13 (the reference's Schwarz threshold
+ *   on the product with the density change), full_every 50, Schwarz bound.
+ * tcg_direct_screen_info: tau, full_every and counts[5]: shell quartets
+ * evaluated / listed in the last direct J/K, then since the last
+ * tcg_direct_init or tcg_rhf_sao: evaluated, listed, number of J/K builds. */
+int tcg_set_direct_screen(tcg_ctx *ctx, double tau, int full_every, int use_learned_bound);
+/* One direct J/K with that density screening at tau (tau = 0: tcg_direct_jk):
+ * the building block of an incremental Fock build in a caller's own SCF loop,
+ * called with the density change. */
+int tcg_direct_jk_screened(tcg_ctx *ctx, int nbas, const double *P, double tau, double *J, double *K);
+int tcg_direct_screen_info(tcg_ctx *ctx, double *tau, int *full_every, long long *counts);
 
 /* The network itself (no context): parameter count, inference, one
  * forward + backward (loss = mean over samples of w0 (y0 - t0)^2 +

@@ -180,6 +180,15 @@ module tcg_engine
       integer(c_int), intent(in) :: occ(*)
       real(c_double), intent(out) :: out(5)
     end function
+    integer(c_int) function tcg_df_kmp2_fp32(df, nbas, coeff_sao, e, occ, out) bind(C, name="tcg_df_kmp2_fp32")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: df
+      integer(c_int), value :: nbas
+      complex(c_double_complex), intent(in) :: coeff_sao(*)
+      real(c_double), intent(in) :: e(*)
+      integer(c_int), intent(in) :: occ(*)
+      real(c_double), intent(out) :: out(5)
+    end function
     integer(c_int) function tcg_df_times(df, out) bind(C, name="tcg_df_times")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: df
@@ -268,6 +277,28 @@ module tcg_engine
     integer(c_int) function tcg_direct_init_learned(ctx, rule_counts) bind(C, name="tcg_direct_init_learned")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, rule_counts
+    end function
+    integer(c_int) function tcg_set_direct_screen(ctx, tau, full_every, use_learned_bound) &
+        bind(C, name="tcg_set_direct_screen")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tau
+      integer(c_int), value :: full_every, use_learned_bound
+    end function
+    integer(c_int) function tcg_direct_jk_screened(ctx, nbas, P, tau, J, K) bind(C, name="tcg_direct_jk_screened")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbas
+      real(c_double), value :: tau
+      real(c_double), intent(in) :: P(*)
+      real(c_double), intent(out) :: J(*), K(*)
+    end function
+    integer(c_int) function tcg_direct_screen_info(ctx, tau, full_every, counts) bind(C, name="tcg_direct_screen_info")
+      import :: c_int, c_double, c_long_long, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: tau
+      integer(c_int), intent(out) :: full_every
+      integer(c_long_long), intent(out) :: counts(5)
     end function
     integer(c_int) function tcg_mlp_param_count() bind(C, name="tcg_mlp_param_count")
       import :: c_int

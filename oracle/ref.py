@@ -66,6 +66,7 @@ def lib():
         L.ref_ao_to_sao.argtypes = [_ci, _dp, _cp]
         L.ref_sao_to_ao.argtypes = [_ci, _cp, _dp]
         L.ref_clean_and_symmetrize.argtypes = [_ci, _cp]
+        L.ref_details.argtypes = [_ci, _ci, _dp, _dp, _dp, _dp, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -127,6 +128,14 @@ class RefSystem:
         out = np.zeros(n1 * nb * nb * nb)
         lib().ref_eri_full(out)
         return out.reshape(nb, nb, nb, n1)
+
+    def details(self, dirpath, S, T, V, E):
+        """The reference's details_integrals(_SAO) (keyword Details) writing into
+        dirpath; S, T, V row-major numpy, E as eri_full returns it (E[l,k,j,i])."""
+        nb = self.s.nbas
+        f = lambda M: np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel()
+        lib().ref_details(nb, int(self.sao), f(S), f(T), f(V), np.ascontiguousarray(E, dtype=np.float64).ravel(),
+                          str(dirpath).encode())
 
     def one_electron(self):
         nb = self.s.nbas

@@ -217,6 +217,29 @@ contains
         end select
       end subroutine ref_one_electron
 
+      ! The keyword Details: details_integrals (full ERI(nBas,nBas,nBas,nBas)) or
+      ! details_integrals_SAO (ERI(nfuc,nBas,nBas,nBas)) write OV.dat, KI.dat,
+      ! NA.dat, the *_matrix.dat files and ERI.dat into the directory 'dir'
+      ! (module variable tmp_file_name)
+      subroutine ref_details(nbas, sao, S, T, V, eri, dir) bind(C, name="ref_details")
+        use files, only : tmp_file_name
+        integer(c_int), value      :: nbas, sao
+        real(c_double), intent(in) :: S(nbas,nbas), T(nbas,nbas), V(nbas,nbas), eri(*)
+        character(kind=c_char), intent(in) :: dir(*)
+        integer :: i
+        tmp_file_name = ' '
+        i = 1
+        do while (dir(i) /= c_null_char .and. i <= 100)
+          tmp_file_name(i:i) = dir(i)
+          i = i + 1
+        end do
+        if (sao /= 0) then
+          call details_integrals_SAO(nbas, S, T, V, eri)
+        else
+          call details_integrals(nbas, S, T, V, eri)
+        end if
+      end subroutine ref_details
+
       ! Gamma MP2: AO_to_MO_ERI_inplace on eri (overwritten), then MP2, whose
       ! report goes to the file 'path' (unit outfile)
       subroutine ref_mp2(nbas, nocc, e, cmo, eri, path) bind(C, name="ref_mp2")

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtcg.so")
-SOURCES = ["tcg_eri.hip", "tcg_rows.hip", "tcg_sort.hip", "tcg_oneel.hip", "tcg_mp2.hip", "tcg_mlp.hip", "tcg_learned.hip", "tcg_setup.hip", "tcg_df.hip", "tcg_noalloc.hip", "tcg_scf.hip", "tcg_host.cpp", "tcg_mlp_host.cpp", "tcg_df.cpp", "tcg_scf.cpp"]
+SOURCES = ["tcg_eri.hip", "tcg_rows.hip", "tcg_sort.hip", "tcg_oneel.hip", "tcg_mp2.hip", "tcg_mlp.hip", "tcg_learned.hip", "tcg_setup.hip", "tcg_df.hip", "tcg_noalloc.hip", "tcg_scf.hip", "tcg_screen.hip", "tcg_host.cpp", "tcg_mlp_host.cpp", "tcg_df.cpp", "tcg_scf.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed FP32 instructions stay off (they are not used by this FP64 code either)

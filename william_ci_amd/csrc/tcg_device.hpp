@@ -51,6 +51,8 @@ struct EriArgs {
     int screen_prim;                  // primitive Schwarz screen (keyword prim_screen)
     int max_ncomb;                    // largest (lA+1)(lB+1)(lC+1)(lD+1) over the quartets
     int max_blk;                      // largest nA nB nC nD over the quartets
+    int part_words;                   // LDS words per wavefront after the block in the general contraction:
+                                      // 64 partial sums, a chunk of 64 survivors (96), 4 x max(ncomp nprim)
     int sp_contract;                  // s/p quartets (at most three p shells) by the s/p contraction kernels
     int sp_min_quartets;              // ... in quartet lists of at least this many quartets
     // block mode
@@ -153,7 +155,16 @@ hipError_t launch_surv_owner(const long long *soff, int nq4, int *owner, hipStre
 constexpr int kRuleSkip = -1;     // a shell quartet the learned bound leaves out
 hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const unsigned char *ood, float margin,
                         int plain, float log_skip, float margin_b, const Quartet *q4, const double *Qsh, int nsh,
-                        double cap, int *qrule, hipStream_t st);
+                        double cap, int *qrule, float *lbound, hipStream_t st);
+// density screening of the direct J/K (tcg_screen.hip)
+hipError_t launch_pair_dmax(const double *P, int nb, const Shell *sh, int nshuc, int nsh, double *Dsh,
+                            hipStream_t st);
+hipError_t launch_density_keep(const Quartet *q4, int nq4, const Shell *sh, const double *Qsh, int nsh, int nshuc,
+                               const int *relc, int ncell, const double *Dsh, const float *lbound, double tau,
+                               unsigned char *keep, hipStream_t st);
+hipError_t launch_gather_list(const Quartet *q4, const int *qrule, const float *lb, const long long *idx,
+                              const unsigned long long *n, long long nmax, Quartet *q4_out, int *rule_out,
+                              float *lb_out, hipStream_t st);
 hipError_t launch_rule_keep(int nq4, const int *qrule, unsigned char *keep, hipStream_t st);
 hipError_t launch_gather_quartets(const Quartet *q4, const int *qrule, const long long *idx,
                                   const unsigned long long *n, long long nmax, Quartet *q4_out, int *rule_out,
@@ -229,6 +240,10 @@ hipError_t launch_diis_combine(const double2 *hist, long long len, const DiisSlo
 // < n1 written), 1 = integral-direct (leading dimension nfuc).  dJ, dK hold
 // nbas^2 doubles.
 int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *dK, int *ld);
+void ctx_screen_reset(tcg_ctx *c);
+int ctx_screen_full_every(const tcg_ctx *c);
+hipError_t launch_axpby(const double *a, const double *b, double *out, size_t n, double alpha, double beta,
+                        hipStream_t st);
 hipStream_t ctx_stream(const tcg_ctx *c);
 // state the SCF loop keeps in the context between calls (released with it)
 std::shared_ptr<void> &ctx_scf_work(tcg_ctx *c);

@@ -38,6 +38,10 @@ hipError_t launch_df_gamma(const double *B, const double *Ps, int nf, int nb, in
 hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *J, hipStream_t st);
 hipError_t launch_df_l(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,
                        int v_lo, const int *cdiff, double2 *L, hipStream_t st);
+hipError_t launch_df_mp2_f32(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,
+                             int v_lo, const int *cdiff, const int *cadd, const double *ek, const int *occ,
+                             float2 *L, float2 *eri, float *part, int nblocks, unsigned long long *nzero,
+                             hipStream_t st);
 hipError_t launch_df_eri(const double2 *L, int nau, int N, int o_hi, int nv, const int *cdiff, const int *cadd,
                          double2 *out, hipStream_t st);
 hipError_t launch_df_kt(const double *Fr, const double *Fi, const double2 *Pk, int nf, int nau, int N, const int *cdiff,
@@ -477,7 +481,8 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K) {
     return TCG_OK;
 }
 
-int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out) {
+static int df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, const int *occ, bool fp32,
+                   double *out) {
     if (!df) return TCG_E_ARG;
     if (nbas != df->nb || !coeff_sao || !e || !occ || !out) return df->fail(TCG_E_ARG, "tcg_df_kmp2: bad arguments");
     const int nf = df->nf, nb = df->nb, N = df->N, nau = df->nau, naux = nau * N;
@@ -510,11 +515,7 @@ int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, 
     DevBuf<double2> dCk, dL, dE;
     DCHK(df, dCk.upload(Ck));
     const size_t nL = (size_t)o_hi * nv * nau * N * N;
-    DCHK(df, dL.alloc(nL));
-    DCHK(df, launch_df_l(df->d_Fr.p, df->d_Fi.p, dCk.p, nf, nau, N, o_hi, v_lo, df->d_cdiff.p, dL.p, df->st));
     const size_t nE = (size_t)o_hi * nv * o_hi * nv * N * N * N;
-    DCHK(df, dE.alloc(nE));
-    DCHK(df, launch_df_eri(dL.p, nau, N, o_hi, nv, df->d_cdiff.p, df->d_cadd.p, dE.p, df->st));
     // the k-MP2 sums (kMP2_calc, src/Energies/SAO/MP2/MP2_SAO.f90:6-151)
     std::vector<double> ek((size_t)N * nf);
     for (size_t i = 0; i < ek.size(); ++i) ek[i] = e[i];
@@ -525,6 +526,39 @@ int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, 
     DCHK(df, de.upload(ek));
     DCHK(df, docc.upload(oc));
     const int nblk = 1024;
+    if (fp32) {
+        // FP32: L, the (ov|ov) GEMMs on the FP32 matrix cores, compensated FP32 sums
+        DevBuf<float2> dL32, dE32;
+        DevBuf<float> dp32;
+        DCHK(df, dL32.alloc(nL));
+        DCHK(df, dE32.alloc(nE));
+        DCHK(df, dp32.alloc(4 * nblk));
+        DCHK(df, dnz.alloc(1));
+        DCHK(df, hipMemsetAsync(dnz.p, 0, sizeof(unsigned long long), df->st));
+        DCHK(df, launch_df_mp2_f32(df->d_Fr.p, df->d_Fi.p, dCk.p, nf, nau, N, o_hi, v_lo, df->d_cdiff.p, df->d_cadd.p,
+                                   de.p, docc.p, dL32.p, dE32.p, dp32.p, nblk, dnz.p, df->st));
+        std::vector<float> part(4 * nblk);
+        unsigned long long nzero = 0;
+        DCHK(df, hipMemcpyAsync(part.data(), dp32.p, sizeof(float) * 4 * nblk, hipMemcpyDeviceToHost, df->st));
+        DCHK(df, hipMemcpyAsync(&nzero, dnz.p, sizeof nzero, hipMemcpyDeviceToHost, df->st));
+        DCHK(df, hipStreamSynchronize(df->st));
+        // the blocks' (hi, lo) pairs meet in FP64 on the host
+        double E2d = 0.0, E2x = 0.0;
+        for (int b = 0; b < nblk; ++b) {
+            E2d += (double)part[4 * b] + (double)part[4 * b + 1];
+            E2x += (double)part[4 * b + 2] + (double)part[4 * b + 3];
+        }
+        out[0] = 2.0 * E2d - E2x;
+        out[1] = 2.0 * E2d;
+        out[2] = -E2x;
+        out[3] = (double)nzero;
+        out[4] = now_ms_df() - t0;
+        return TCG_OK;
+    }
+    DCHK(df, dL.alloc(nL));
+    DCHK(df, launch_df_l(df->d_Fr.p, df->d_Fi.p, dCk.p, nf, nau, N, o_hi, v_lo, df->d_cdiff.p, dL.p, df->st));
+    DCHK(df, dE.alloc(nE));
+    DCHK(df, launch_df_eri(dL.p, nau, N, o_hi, nv, df->d_cdiff.p, df->d_cadd.p, dE.p, df->st));
     DCHK(df, dpart.alloc(2 * nblk));
     DCHK(df, dnz.alloc(1));
     DCHK(df, hipMemsetAsync(dnz.p, 0, sizeof(unsigned long long), df->st));
@@ -546,6 +580,14 @@ int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, 
     out[3] = (double)nzero;
     out[4] = now_ms_df() - t0;
     return TCG_OK;
+}
+
+int tcg_df_kmp2(tcg_df *df, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out) {
+    return df_kmp2(df, nbas, coeff_sao, e, occ, false, out);
+}
+
+int tcg_df_kmp2_fp32(tcg_df *df, int nbas, const double *coeff_sao, const double *e, const int *occ, double *out) {
+    return df_kmp2(df, nbas, coeff_sao, e, occ, true, out);
 }
 
 int tcg_df_times(tcg_df *df, double *out) {

@@ -15,6 +15,10 @@
 //                    Kt(kk) = -1/(2N) sum_{k1,Q} M P(k1) M^H,  M = Bh_Q(k1, kk - k1)
 //   k_df_l           L(p, q, Q, k1, k2) = sum_ab conj C_k1(a, p) C_k2(b, q) F(a, b, k2, Q, k2 - k1)
 //   k_df_eri         ERI(p,q,r,s,k1,k2,k3) = 1/N sum_Q L(p,q,Q,k1,k2) L(r,s,Q,k3,k4)
+//   k_df_l_f32, k_df_eri_mfma32, k_kmp2_sums_f32: the same in FP32 -- L with FP32
+//                    arithmetic, the (ov|ov) assembly as FP32 matrix-core GEMMs with
+//                    FP32 accumulation, the energy sums in FP32 with a compensated
+//                    (hi, lo) FP32 pair per thread and per block
 // Cells and k points are numbered x fastest; cdiff[c1 * N + c2] = c1 - c2.
 #include <hip/hip_runtime.h>
 
@@ -490,6 +494,185 @@ __global__ __launch_bounds__(256) void k_df_eri(const double2 *__restrict__ L, i
         }
         out[t] = double2{acc.x * invN, acc.y * invN};
     }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 path of the DF k-MP2.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// L(p, q, Q, k1, k2) as k_df_l, in FP32 arithmetic (FP32 output)
+__global__ __launch_bounds__(256) void k_df_l_f32(const double *__restrict__ Fr, const double *__restrict__ Fi,
+                                                  const double2 *__restrict__ Ck, int nf, int nau, int N, int o_hi,
+                                                  int v_lo, const int *__restrict__ cdiff, float2 *__restrict__ L) {
+    const int nv = nf - v_lo;
+    const size_t total = (size_t)o_hi * nv * nau * N * N;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        size_t r = t;
+        const int p = (int)(r % o_hi); r /= o_hi;
+        const int q = v_lo + (int)(r % nv); r /= nv;
+        const int Q = (int)(r % nau); r /= nau;
+        const int k1 = (int)(r % N);
+        const int k2 = (int)(r / N);
+        const int kap = cdiff[k2 * N + k1];
+        const double2 *C1 = Ck + (size_t)k1 * nf * nf, *C2 = Ck + (size_t)k2 * nf * nf;
+        const size_t fb = (size_t)nf * nf * (k2 + (size_t)N * (Q + (size_t)nau * kap));
+        float ax = 0.f, ay = 0.f;
+        for (int b = 0; b < nf; ++b) {
+            float sx = 0.f, sy = 0.f;
+            for (int a = 0; a < nf; ++a) {
+                const size_t fi = fb + a + (size_t)nf * b;
+                const double2 c = C1[a + (size_t)nf * p];
+                const float cx = (float)c.x, cy = -(float)c.y, fx = (float)Fr[fi], fy = (float)Fi[fi];
+                sx += cx * fx - cy * fy;
+                sy += cx * fy + cy * fx;
+            }
+            const double2 c2 = C2[b + (size_t)nf * q];
+            const float cx = (float)c2.x, cy = (float)c2.y;
+            ax += cx * sx - cy * sy;
+            ay += cx * sy + cy * sx;
+        }
+        L[t] = float2{ax, ay};
+    }
+}
+
+// (ov|ov) on the FP32 matrix cores.  For one momentum transfer dk = k1 - k2 =
+// k4 - k3 (blockIdx.y) the blocks form one complex GEMM over Q:
+//   rows i = (pq, k1), k2 = k1 - dk;  columns j = (rs, k3), k4 = k3 + dk
+//   ERI(i, j) = 1/N sum_Q L(pq, Q, k1, k2) L(rs, Q, k3, k4)
+// in 16 x 16 tiles (one per wavefront), v_mfma_f32_16x16x4f32 with FP32
+// accumulators: Re = sum ArBr - AiBi, Im = sum ArBi + AiBr (four real
+// products per step of 4 Q).  A operand: lane holds row lr of the tile at
+// Q0 + lq; B operand: column lr at Q0 + lq; D: rows 4 lq + v, column lr.
+__global__ __launch_bounds__(256) void k_df_eri_mfma32(const float2 *__restrict__ L, int nau, int N, int per,
+                                                       const int *__restrict__ cdiff, const int *__restrict__ cadd,
+                                                       float2 *__restrict__ out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lq = lane >> 4;
+    const int M = per * N, nt = (M + 15) / 16, dk = blockIdx.y;
+    const float invN = 1.0f / (float)N;
+    for (int tile = blockIdx.x * 4 + wv; tile < nt * nt; tile += gridDim.x * 4) {
+        const int ti = tile % nt, tj = tile / nt;
+        const int i = 16 * ti + lr, j = 16 * tj + lr;
+        const bool iv = i < M, jv = j < M;
+        const int pq = iv ? i % per : 0, k1 = iv ? i / per : 0;
+        const int k2 = cdiff[k1 * N + dk];
+        const int rs = jv ? j % per : 0, k3 = jv ? j / per : 0;
+        const int k4 = cadd[dk * N + k3];
+        const float2 *Ar = L + pq + (size_t)per * nau * (k1 + (size_t)N * k2);
+        const float2 *Bc = L + rs + (size_t)per * nau * (k3 + (size_t)N * k4);
+        floatx4 rr = {0.f, 0.f, 0.f, 0.f}, ii = rr, ri = rr, ir = rr;
+        for (int Q0 = 0; Q0 < nau; Q0 += 4) {
+            const int Q = Q0 + lq;
+            const float2 a = (iv && Q < nau) ? Ar[(size_t)per * Q] : float2{0.f, 0.f};
+            const float2 b = (jv && Q < nau) ? Bc[(size_t)per * Q] : float2{0.f, 0.f};
+            rr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, rr, 0, 0, 0);
+            ii = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, ii, 0, 0, 0);
+            ri = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, ri, 0, 0, 0);
+            ir = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, ir, 0, 0, 0);
+        }
+        if (!jv) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i2 = 16 * ti + 4 * lq + v;
+            if (i2 >= M) continue;
+            const int pq2 = i2 % per, kk1 = i2 / per, kk2 = cdiff[kk1 * N + dk];
+            out[pq2 + (size_t)per * (rs + (size_t)per * (kk1 + (size_t)N * (kk2 + (size_t)N * k3)))] =
+                float2{(rr[v] - ii[v]) * invN, (ri[v] + ir[v]) * invN};
+        }
+    }
+}
+
+// s + t as an unevaluated FP32 pair (hi, lo): Knuth's two-sum of hi and t, the
+// rounding error added to lo
+__device__ __forceinline__ void acc2(float &hi, float &lo, float t) {
+    const float s = hi + t;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (t - bb);
+    hi = s;
+    lo += err;
+}
+
+// kMP2_calc in FP32 (the k_kmp2_sums loop): terms from the FP32 (ov|ov), each
+// sum kept as a compensated (hi, lo) FP32 pair, per thread and over the block;
+// part[4 b ..] = d_hi, d_lo, x_hi, x_lo of block b
+__global__ __launch_bounds__(256) void k_kmp2_sums_f32(const float2 *__restrict__ ekmo, const double *__restrict__ ek,
+                                                       const int *__restrict__ occ, int nf, int N,
+                                                       const int *__restrict__ cadd, const int *__restrict__ cdiff,
+                                                       int o_hi, int v_lo, float *__restrict__ part,
+                                                       unsigned long long *__restrict__ nzero) {
+    __shared__ float sdh[256], sdl[256], sxh[256], sxl[256];
+    const int nv = nf - v_lo;
+    const long long total = (long long)N * N * N * o_hi * o_hi * nv * nv;
+    float dh = 0.f, dl = 0.f, xh = 0.f, xl = 0.f;
+    unsigned long long nz = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        long long r_ = t;
+        const int i = (int)(r_ % o_hi); r_ /= o_hi;
+        const int a = v_lo + (int)(r_ % nv); r_ /= nv;
+        const int j = (int)(r_ % o_hi); r_ /= o_hi;
+        const int b = v_lo + (int)(r_ % nv); r_ /= nv;
+        const int ki = (int)(r_ % N); r_ /= N;
+        const int ka = (int)(r_ % N);
+        const int kj = (int)(r_ / N);
+        const int kb = cadd[cdiff[ki * N + ka] * N + kj];                // ki - ka + kj
+        if (occ[ki * nf + i] != 2 || occ[kj * nf + j] != 2 || occ[ka * nf + a] != 0 || occ[kb * nf + b] != 0) continue;
+        const float D = (float)(ek[ka * nf + a] + ek[kb * nf + b] - ek[ki * nf + i] - ek[kj * nf + j]);
+        const size_t blk = (size_t)o_hi * nv * o_hi * nv;
+        const float2 gd = ekmo[(size_t)i + o_hi * ((size_t)(a - v_lo) + nv * ((size_t)j + (size_t)o_hi * (b - v_lo))) +
+                               blk * ((size_t)ki + (size_t)N * ((size_t)ka + (size_t)N * kj))];
+        if (fabsf(D) < 1e-8f) {
+            ++nz;
+            continue;
+        }
+        const float2 gx = ekmo[(size_t)i + o_hi * ((size_t)(b - v_lo) + nv * ((size_t)j + (size_t)o_hi * (a - v_lo))) +
+                               blk * ((size_t)ki + (size_t)N * ((size_t)kb + (size_t)N * kj))];
+        acc2(dh, dl, -(gd.x * gd.x + gd.y * gd.y) / D);
+        acc2(xh, xl, -(gd.x * gx.x + gd.y * gx.y) / D);
+    }
+    if (nz) atomicAdd(nzero, nz);
+    sdh[threadIdx.x] = dh;
+    sdl[threadIdx.x] = dl;
+    sxh[threadIdx.x] = xh;
+    sxl[threadIdx.x] = xl;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            float h = sdh[threadIdx.x], l = sdl[threadIdx.x] + sdl[threadIdx.x + s];
+            acc2(h, l, sdh[threadIdx.x + s]);
+            sdh[threadIdx.x] = h;
+            sdl[threadIdx.x] = l;
+            h = sxh[threadIdx.x];
+            l = sxl[threadIdx.x] + sxl[threadIdx.x + s];
+            acc2(h, l, sxh[threadIdx.x + s]);
+            sxh[threadIdx.x] = h;
+            sxl[threadIdx.x] = l;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x] = sdh[0];
+        part[4 * blockIdx.x + 1] = sdl[0];
+        part[4 * blockIdx.x + 2] = sxh[0];
+        part[4 * blockIdx.x + 3] = sxl[0];
+    }
+}
+
+hipError_t launch_df_mp2_f32(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,
+                             int v_lo, const int *cdiff, const int *cadd, const double *ek, const int *occ,
+                             float2 *L, float2 *eri, float *part, int nblocks, unsigned long long *nzero,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_df_l_f32, dim3(2048), dim3(256), 0, st, Fr, Fi, Ck, nf, nau, N, o_hi, v_lo, cdiff, L);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int per = o_hi * (nf - v_lo), M = per * N, nt = (M + 15) / 16;
+    const int gx = std::max(1, std::min((nt * nt + 3) / 4, 1024));
+    hipLaunchKernelGGL(k_df_eri_mfma32, dim3(gx, N), dim3(256), 0, st, L, nau, N, per, cdiff, cadd, eri);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_kmp2_sums_f32, dim3(nblocks), dim3(256), 0, st, eri, ek, occ, nf, N, cadd, cdiff, o_hi, v_lo,
+                       part, nzero);
+    return hipGetLastError();
 }
 
 hipError_t launch_df_l(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,

@@ -133,7 +133,20 @@ struct tcg_ctx {
     bool direct_learned = false;       // the direct J/K runs the learned list below
     DevBuf<Quartet> d_q4dl;            // ... its quartets (skipped ones removed) and their rules
     DevBuf<int> d_qrdl;
+    DevBuf<float> d_lb, d_lbdl;        // log10 learned bound per quartet (all / the direct list)
     int nq4dl = 0;
+    // density screening of the direct J/K (incremental Fock builds, tcg_set_direct_screen)
+    double screen_tau = 1e-13;         // skip when bound x largest density element < tau (the
+                                       // reference's Schwarz threshold, ERI_toroidal.f90:64)
+    int screen_full_every = 50;        // a full (unscreened) build every this many iterations
+    int screen_learned = 0;            // 1: the bound is min(Schwarz, learned) when the learned list runs
+    DevBuf<double> d_Dsh;
+    DevBuf<unsigned char> d_keep;
+    DevBuf<long long> d_sidx, d_snsel;
+    DevBuf<Quartet> d_q4scr;
+    DevBuf<int> d_qrscr;
+    long long screen_kept = 0, screen_total = 0;   // quartets of the last direct build, kept / listed
+    long long screen_cum[3] = {0, 0, 0};           // since the last reset: kept, listed, builds
     DevBuf<int> d_qrule_keep;
     DevBuf<float> d_mlp, d_feat, d_pred;
     DevBuf<int> d_qrule;
@@ -478,6 +491,9 @@ static EriArgs base_args(tcg_ctx *c) {
     A.screen_prim = 0;
     A.max_ncomb = c->max_ncomb;
     A.max_blk = c->max_blk;
+    int max_cf = 1;
+    for (const Shell &sh : c->sh) max_cf = std::max(max_cf, sh.ncomp * sh.nprim);
+    A.part_words = 160 + 4 * max_cf;
     A.sp_contract = c->sp_contract;
     A.sp_min_quartets = c->sp_min_quartets;
     A.nbas = c->nbas;
@@ -750,7 +766,8 @@ static int prepare(tcg_ctx *c) {
 // per-rule counts in c->learned_counts.  With skipping on, or gather set, the kept
 // quartets and their rules are gathered into q4_out / rule_out (capacity nq4) and
 // A is pointed at them; otherwise A.qrule = c->d_qrule.
-static int learned_rules(tcg_ctx *c, EriArgs &A, bool gather, DevBuf<Quartet> &q4_out, DevBuf<int> &rule_out) {
+static int learned_rules(tcg_ctx *c, EriArgs &A, bool gather, DevBuf<Quartet> &q4_out, DevBuf<int> &rule_out,
+                         DevBuf<float> *lb_out = nullptr) {
     const int nq4 = A.nq4;
     const int nord = (int)c->learned_orders.size();
     for (int r = 0; r <= nord + 1 && r < 16; ++r) c->learned_counts[r] = 0;
@@ -784,9 +801,10 @@ static int learned_rules(tcg_ctx *c, EriArgs &A, bool gather, DevBuf<Quartet> &q
     }
     const bool skipping = c->learned_skip > 0.0;
     const float log_skip = skipping ? (float)std::log10(c->learned_skip) : -3e30f;
+    if (lb_out) HIPCHK(c, c->d_lb.alloc(nq4));
     HIPCHK(c, launch_rules(nq4, c->d_soff.p, c->d_pred.p, c->d_ood.p, c->learned_margin, nord, log_skip,
                            c->learned_margin_b, A.q4, c->d_Qsh.p, c->nsh, c->learned_skip_cap, c->d_qrule.p,
-                           c->st));
+                           lb_out ? c->d_lb.p : nullptr, c->st));
     // quartets per rule (rule 0 = the plain rule; slot 15 = kRuleSkip & 15: the skipped ones)
     HIPCHK(c, c->d_hist.alloc(16));
     HIPCHK(c, launch_fill((double *)c->d_hist.p, 16, 0.0, c->st));
@@ -808,9 +826,10 @@ static int learned_rules(tcg_ctx *c, EriArgs &A, bool gather, DevBuf<Quartet> &q
         HIPCHK(c, select_flagged(c->d_tmp.p, tb2, keep.p, idx.p, nsel.p, nq4, c->st));
         HIPCHK(c, q4_out.alloc(nq4));
         HIPCHK(c, rule_out.alloc(nq4));
-        HIPCHK(c, launch_gather_quartets(A.q4, c->d_qrule.p, idx.p,
-                                         reinterpret_cast<const unsigned long long *>(nsel.p), nq4, q4_out.p,
-                                         rule_out.p, c->st));
+        if (lb_out) HIPCHK(c, lb_out->alloc(nq4));
+        HIPCHK(c, launch_gather_list(A.q4, c->d_qrule.p, lb_out ? c->d_lb.p : nullptr, idx.p,
+                                     reinterpret_cast<const unsigned long long *>(nsel.p), nq4, q4_out.p, rule_out.p,
+                                     lb_out ? lb_out->p : nullptr, c->st));
         long long kept = 0;
         HIPCHK(c, hipMemcpyAsync(&kept, nsel.p, sizeof kept, hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1034,7 +1053,7 @@ static int stored_cols(tcg_ctx *c, const double *src, int n1, int nbas, int phys
 
 // J and K rows of the unit cell (nfuc x nbas, leading dimension nfuc) from the
 // integral-direct build for the density dP on the device.
-static int direct_jk_device(tcg_ctx *c, const double *dP, double *dJ, double *dK) {
+static int direct_jk_device(tcg_ctx *c, const double *dP, double *dJ, double *dK, double tau = 0.0) {
     if (!c->direct_ready || !c->prepared) return c->fail(TCG_E_STATE, "direct J/K: tcg_direct_init not called");
     const size_t njk = (size_t)c->nfuc * c->nbas;
     HIPCHK(c, launch_fill(dJ, njk, 0.0, c->st));
@@ -1047,17 +1066,52 @@ static int direct_jk_device(tcg_ctx *c, const double *dP, double *dJ, double *dK
     EriArgs A = base_args(c);
     A.q4 = c->d_q4.p;
     A.nq4 = c->nq4;
+    const float *lb = nullptr;
     if (c->direct_learned) {                    // per-quartet learned rules (tcg_direct_init_learned)
         A.q4 = c->d_q4dl.p;
         A.nq4 = c->nq4dl;
         A.qrule = c->d_qrdl.p;
+        if (c->screen_learned) lb = c->d_lbdl.p;
     }
+    if (tau > 0.0 && A.nq4 > 0) {
+        // density screening: the quartets whose bound times the largest element of the
+        // density blocks they contract with is below tau leave this build's list
+        const int nshuc = c->nshuc, nsh = c->nsh, n = A.nq4;
+        HIPCHK(c, c->d_Dsh.alloc((size_t)nshuc * nsh));
+        HIPCHK(c, c->d_keep.alloc(n));
+        HIPCHK(c, c->d_sidx.alloc(n));
+        HIPCHK(c, c->d_snsel.alloc(1));
+        HIPCHK(c, c->d_q4scr.alloc(n));
+        if (A.qrule) HIPCHK(c, c->d_qrscr.alloc(n));
+        HIPCHK(c, launch_pair_dmax(dP, c->nbas, c->d_sh.p, nshuc, nsh, c->d_Dsh.p, c->st));
+        HIPCHK(c, launch_density_keep(A.q4, n, c->d_sh.p, c->d_Qsh.p, nsh, nshuc, c->d_relc.p, c->ncell, c->d_Dsh.p, lb,
+                                      tau, c->d_keep.p, c->st));
+        const size_t tb = select_temp_bytes(n);
+        HIPCHK(c, c->d_tmp.alloc(tb));
+        HIPCHK(c, select_flagged(c->d_tmp.p, tb, c->d_keep.p, c->d_sidx.p, c->d_snsel.p, n, c->st));
+        HIPCHK(c, launch_gather_list(A.q4, A.qrule, nullptr, c->d_sidx.p,
+                                     reinterpret_cast<const unsigned long long *>(c->d_snsel.p), n, c->d_q4scr.p,
+                                     A.qrule ? c->d_qrscr.p : nullptr, nullptr, c->st));
+        long long kept = 0;
+        HIPCHK(c, hipMemcpyAsync(&kept, c->d_snsel.p, sizeof kept, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->screen_total = n;
+        c->screen_kept = kept;
+        A.q4 = c->d_q4scr.p;
+        A.nq4 = (int)kept;
+        if (A.qrule) A.qrule = c->d_qrscr.p;
+    } else {
+        c->screen_total = c->screen_kept = A.nq4;
+    }
+    c->screen_cum[0] += c->screen_kept;
+    c->screen_cum[1] += c->screen_total;
+    c->screen_cum[2] += 1;
     A.screen_prim = 1;
     A.Qao = c->d_Q.p;
     A.Pden = dP;
     A.Jacc = dJ;
     A.Kacc = dK;
-    const int rc = run_quartets(c, A);
+    const int rc = A.nq4 > 0 ? run_quartets(c, A) : 0;
     if (rc) return rc;
     // counters of this J/K (tcg_eri_stats): quartets, survivors, rows, flops
     unsigned long long cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1114,6 +1168,12 @@ hipStream_t ctx_stream(const tcg_ctx *c) { return c ? c->st : nullptr; }
 
 std::shared_ptr<void> &ctx_scf_work(tcg_ctx *c) { return c->scf_work; }
 
+void ctx_screen_reset(tcg_ctx *c) {
+    for (long long &v : c->screen_cum) v = 0;
+}
+
+int ctx_screen_full_every(const tcg_ctx *c) { return c->screen_full_every; }
+
 int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *dK, int *ld) {
     if (!c || !dP || !dJ || !dK || !ld) return TCG_E_ARG;
     if (source == 0) {
@@ -1131,6 +1191,10 @@ int ctx_jk_device(tcg_ctx *c, int source, const double *dP, double *dJ, double *
     if (source == 1) {
         *ld = c->nfuc;
         return direct_jk_device(c, dP, dJ, dK);
+    }
+    if (source == 2) {                         // integral-direct with density screening at tau
+        *ld = c->nfuc;
+        return direct_jk_device(c, dP, dJ, dK, c->screen_tau);
     }
     return c->fail(TCG_E_ARG, "J/K source must be 0 (stored) or 1 (direct)");
 }
@@ -1383,6 +1447,28 @@ int tcg_direct_init(tcg_ctx *c) {
     if (rc) return rc;
     c->direct_ready = true;
     c->direct_learned = false;
+    for (long long &v : c->screen_cum) v = 0;
+    return TCG_OK;
+}
+
+int tcg_set_direct_screen(tcg_ctx *c, double tau, int full_every, int use_learned_bound) {
+    if (!c) return TCG_E_ARG;
+    if (!(tau >= 0.0) || full_every < 1) return c->fail(TCG_E_ARG, "tcg_set_direct_screen: bad arguments");
+    c->screen_tau = tau;
+    c->screen_full_every = full_every;
+    c->screen_learned = use_learned_bound ? 1 : 0;
+    return TCG_OK;
+}
+
+int tcg_direct_screen_info(tcg_ctx *c, double *tau, int *full_every, long long *counts) {
+    if (!c) return TCG_E_ARG;
+    if (tau) *tau = c->screen_tau;
+    if (full_every) *full_every = c->screen_full_every;
+    if (counts) {
+        counts[0] = c->screen_kept;
+        counts[1] = c->screen_total;
+        for (int i = 0; i < 3; ++i) counts[2 + i] = c->screen_cum[i];
+    }
     return TCG_OK;
 }
 
@@ -1401,7 +1487,7 @@ int tcg_direct_init_learned(tcg_ctx *c, long long *rule_counts) {
     A.nq4 = c->nq4;
     A.screen_prim = 1;
     A.Qao = c->d_Q.p;
-    rc = learned_rules(c, A, true, c->d_q4dl, c->d_qrdl);
+    rc = learned_rules(c, A, true, c->d_q4dl, c->d_qrdl, &c->d_lbdl);
     if (rc) return rc;
     c->nq4dl = A.nq4;
     c->direct_learned = true;
@@ -1411,9 +1497,13 @@ int tcg_direct_init_learned(tcg_ctx *c, long long *rule_counts) {
 }
 
 int tcg_direct_jk(tcg_ctx *c, int nbas, const double *P, double *J, double *K) {
+    return tcg_direct_jk_screened(c, nbas, P, 0.0, J, K);
+}
+
+int tcg_direct_jk_screened(tcg_ctx *c, int nbas, const double *P, double tau, double *J, double *K) {
     if (!c) return TCG_E_ARG;
     if (!c->direct_ready || !c->prepared) return c->fail(TCG_E_STATE, "tcg_direct_jk: direct_init not called");
-    if (nbas != c->nbas || !P || !J || !K) return c->fail(TCG_E_ARG, "tcg_direct_jk: bad arguments");
+    if (nbas != c->nbas || !P || !J || !K || !(tau >= 0.0)) return c->fail(TCG_E_ARG, "tcg_direct_jk: bad arguments");
     const int nb = c->nbas, nf = c->nfuc;
     const size_t nn = (size_t)nb * nb, njk = (size_t)nf * nb;
     HIPCHK(c, c->d_P.alloc(nn));
@@ -1421,7 +1511,7 @@ int tcg_direct_jk(tcg_ctx *c, int nbas, const double *P, double *J, double *K) {
     HIPCHK(c, c->d_K.alloc(njk));
     HIPCHK(c, hipMemcpyAsync(c->d_P.p, P, sizeof(double) * nn, hipMemcpyHostToDevice, c->st));
     const double t0 = now_ms();
-    int rc = direct_jk_device(c, c->d_P.p, c->d_J.p, c->d_K.p);
+    int rc = direct_jk_device(c, c->d_P.p, c->d_J.p, c->d_K.p, tau);
     if (rc) return rc;
     std::vector<double> Jr(njk), Kr(njk);
     HIPCHK(c, hipMemcpyAsync(Jr.data(), c->d_J.p, sizeof(double) * njk, hipMemcpyDeviceToHost, c->st));

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(64) void k_rules(int nq4, const long long *__restri
                                               const unsigned char *__restrict__ ood, float margin, int plain,
                                               float log_skip, float margin_b, const Quartet *__restrict__ q4,
                                               const double *__restrict__ Qsh, int nsh, double cap,
-                                              int *__restrict__ qrule) {
+                                              int *__restrict__ qrule, float *__restrict__ lbound) {
     for (int qi = blockIdx.x; qi < nq4; qi += gridDim.x) {
         int r = 0;
         float bmax = -1e30f;
@@ -132,6 +132,9 @@ __global__ __launch_bounds__(64) void k_rules(int nq4, const long long *__restri
                 skip = Qsh[(size_t)Q.s[0] + (size_t)nsh * Q.s[1]] * Qsh[(size_t)Q.s[2] + (size_t)nsh * Q.s[3]] < cap;
             }
             qrule[qi] = skip ? kRuleSkip : (r >= plain ? 0 : r + 1);
+            // log10 of the learned bound of the whole shell quartet (largest predicted
+            // integral, times the number of primitive quartets, times 10^margin_b)
+            if (lbound) lbound[qi] = n > 0 ? bmax + log10f((float)n) + margin_b : -1e30f;
         }
     }
 }
@@ -175,11 +178,11 @@ hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, con
 
 hipError_t launch_rules(int nq4, const long long *soff, const float *pred, const unsigned char *ood, float margin,
                         int plain, float log_skip, float margin_b, const Quartet *q4, const double *Qsh, int nsh,
-                        double cap, int *qrule, hipStream_t st) {
+                        double cap, int *qrule, float *lbound, hipStream_t st) {
     if (nq4 <= 0) return hipSuccess;
     const int grid = nq4 < (1 << 20) ? nq4 : (1 << 20);
     hipLaunchKernelGGL(k_rules, dim3(grid), dim3(64), 0, st, nq4, soff, pred, ood, margin, plain, log_skip, margin_b,
-                       q4, Qsh, nsh, cap, qrule);
+                       q4, Qsh, nsh, cap, qrule, lbound);
     return hipGetLastError();
 }
 

@@ -657,12 +657,14 @@ hipError_t launch_real_rows(const EriArgs &a, const long long *soff, const long 
 // Each lane owns one component and a residue class of the surviving
 // primitive quartets g: with fewer than 64 components, G = 64 / ncomp lanes
 // share a component (g = lane / ncomp, + G, ...), their sums meet through
-// part[] (64 doubles of LDS per wavefront) in a fixed order; with 64 or more,
+// part[] (the first 64 doubles of the per-wavefront staging) in a fixed order; with 64 or more,
 // lanes loop over the components and every g.  No atomics: the block comes
-// out the same on every run.  Nothing in a task waits on a chain of loads:
-// the Cartesian powers of the four shells are packed into one word per shell
-// up front, the row addresses follow from (g, component), and the primitive
-// indices from the pair index (pairs of a shell pair are ordered i major).
+// out the same on every run.  Nothing in a task waits on a chain of global
+// loads: the Cartesian powers of the four shells are packed into one word per
+// shell up front, the row addresses follow from (g, component), and the
+// survivors' prefactors and primitive indices (from the pair index: pairs of a
+// shell pair are ordered i major) and the four shells' contraction
+// coefficients are staged in LDS, 64 survivors at a time.
 // Node rows are 16-byte aligned (nqp a multiple of 16) and read two nodes per
 // load.
 // ---------------------------------------------------------------------------
@@ -691,10 +693,22 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
     const unsigned long long pwA = shell_powers(A, sA), pwB = shell_powers(A, sB), pwC = shell_powers(A, sC),
                              pwD = shell_powers(A, sD);
     const int pb0 = A.sp[Q.bra].first, pk0 = A.sp[Q.ket].first;   // first primitive pair of bra / ket
-    const int npB = sB.nprim, npD = sD.nprim;
+    const int npA = sA.nprim, npB = sB.nprim, npC = sC.nprim, npD = sD.nprim;
     const long R0 = r0 / ndp;
-    // one component, its survivors g0, g0 + gstep, ...: the coefficient-weighted node sums
-    auto comp_sum = [&](int cp, long g0, long gstep) {
+    // staged in LDS after part[64]: the prefactors and primitive indices of a chunk of
+    // up to 64 survivors, and the contraction coefficients of the four shells
+    // (cX[c np + p] = coefficient of component c, primitive p), so that a
+    // (component, survivor) task reads LDS rather than a chain of global loads
+    double *sfac = part + 64;
+    int *sidx = reinterpret_cast<int *>(sfac + 64);            // 64 packed (pi, pj, qk, ql), 4 bits each
+    double *cA = sfac + 96, *cB = cA + nA * npA, *cC = cB + nB * npB, *cD = cC + nC * npC;
+    for (int e = lane; e < nA * npA; e += 64) cA[e] = A.coef[sA.poff + e];
+    for (int e = lane; e < nB * npB; e += 64) cB[e] = A.coef[sB.poff + e];
+    for (int e = lane; e < nC * npC; e += 64) cC[e] = A.coef[sC.poff + e];
+    for (int e = lane; e < nD * npD; e += 64) cD[e] = A.coef[sD.poff + e];
+    // one component, survivors g0, g0 + gstep, ... < gend of the staged chunk
+    // starting at gbase: the coefficient-weighted node sums
+    auto comp_sum = [&](int cp, long gbase, long g0, long gend, long gstep) {
         int t = cp;
         const int ca = t % nA; t /= nA;
         const int cb = t % nB; t /= nB;
@@ -709,11 +723,10 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
             cxyz[dd] = ((pa >> sh) & 3) + (lA + 1) * (((pbw >> sh) & 3) +
                        (lB + 1) * (((pc >> sh) & 3) + (lC + 1) * ((pd >> sh) & 3)));
         }
-        const int oA = sA.poff + ca * sA.nprim, oB = sB.poff + cb * sB.nprim, oC = sC.poff + cc * sC.nprim,
-                  oD = sD.poff + cd * sD.nprim;
+        const double *cfA = cA + ca * npA, *cfB = cB + cb * npB, *cfC = cC + cc * npC, *cfD = cD + cd * npD;
         double acc = 0.0;
-        for (long g = g0; g < ns; g += gstep) {
-            const Surv S = surv[s0 + g];
+        for (long gl = g0; gl < gend; gl += gstep) {
+            const long g = gbase + gl;
             const double *I3[3];
 #pragma unroll
             for (int dd = 0; dd < 3; ++dd) {
@@ -725,25 +738,61 @@ __device__ __forceinline__ void quartet_block(const EriArgs &A, const Quartet &Q
             const double2 *X = reinterpret_cast<const double2 *>(I3[0]);
             const double2 *Y = reinterpret_cast<const double2 *>(I3[1]);
             const double2 *Z = reinterpret_cast<const double2 *>(I3[2]);
+            // eight nodes per step with all of their loads issued before the products
+            // (each lane walks its own rows, so the loads, not the arithmetic, set the pace)
             double v0 = 0.0, v1 = 0.0;
-            for (int k2 = 0; k2 < nq2; ++k2) {
+            int k2 = 0;
+            for (; k2 + 4 <= nq2; k2 += 4) {
+                double2 x[4], y[4], z[4], w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    x[u] = X[k2 + u];
+                    y[u] = Y[k2 + u];
+                    z[u] = Z[k2 + u];
+                    w[u] = tw2[k2 + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    v0 += (x[u].x * w[u].x) * y[u].x * z[u].x;
+                    v1 += (x[u].y * w[u].y) * y[u].y * z[u].y;
+                }
+            }
+            for (; k2 < nq2; ++k2) {
                 const double2 x = X[k2], y = Y[k2], z = Z[k2], w = tw2[k2];
                 v0 += (x.x * w.x) * y.x * z.x;
                 v1 += (x.y * w.y) * y.y * z.y;
             }
             double val = v0 + v1;
             if (nq & 1) val += (I3[0][nq - 1] * tw[nq - 1]) * I3[1][nq - 1] * I3[2][nq - 1];
-            const int ib = S.pb - pb0, ik = S.pk - pk0;
-            const int pi = ib / npB, pj = ib - pi * npB, qk = ik / npD, ql = ik - qk * npD;
-            acc += S.fac * A.coef[oA + pi] * A.coef[oB + pj] * A.coef[oC + qk] * A.coef[oD + ql] * val;
+            const int px = sidx[gl];
+            acc += sfac[gl] * cfA[px & 15] * cfB[(px >> 4) & 15] * cfC[(px >> 8) & 15] * cfD[(px >> 12) & 15] * val;
         }
         return acc;
     };
-    if (nblk >= 64) {
-        for (int cp = lane; cp < nblk; cp += 64) blk[cp] = comp_sum(cp, 0, 1);
-    } else {
-        const int G = 64 / nblk, cp = lane % nblk, gs = lane / nblk;
-        part[lane] = gs < G ? comp_sum(cp, gs, G) : 0.0;
+    const bool wide = nblk >= 64;
+    const int G = wide ? 1 : 64 / nblk, cpl = lane % nblk, gs = lane / nblk;
+    double mine = 0.0;                                    // lane's running sum (fewer than 64 components)
+    if (wide)
+        for (int cp = lane; cp < nblk; cp += 64) blk[cp] = 0.0;
+    for (long gb = 0; gb < ns; gb += 64) {
+        const long nc = ns - gb < 64 ? ns - gb : 64;
+        wave_sync();                                      // the previous chunk's readers are done
+        if (lane < nc) {
+            const Surv S = surv[s0 + gb + lane];
+            const int ib = S.pb - pb0, ik = S.pk - pk0;
+            const int pi = ib / npB, pj = ib - pi * npB, qk = ik / npD, ql = ik - qk * npD;
+            sfac[lane] = S.fac;
+            sidx[lane] = pi | (pj << 4) | (qk << 8) | (ql << 12);
+        }
+        wave_sync();
+        if (wide) {
+            for (int cp = lane; cp < nblk; cp += 64) blk[cp] += comp_sum(cp, gb, 0, nc, 1);
+        } else if (gs < G) {
+            mine += comp_sum(cpl, gb, gs, nc, G);
+        }
+    }
+    if (!wide) {
+        part[lane] = gs < G ? mine : 0.0;
         wave_sync();
         if (lane < nblk) {
             double sum = 0.0;
@@ -1099,10 +1148,10 @@ __global__ __launch_bounds__(256) void k_contract(EriArgs A, const long long *__
                                                   const double *__restrict__ realval, const int *__restrict__ list,
                                                   const unsigned *__restrict__ cnt) {
     // four wavefronts per workgroup, one shell quartet each (every quartet, or the
-    // cnt entries of list); per wavefront in LDS a block of max_blk and 64 partial sums
+    // cnt entries of list); per wavefront in LDS a block of max_blk and the staging (part_words)
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double *blk = lds + (size_t)wv * (A.max_blk + 64), *part = blk + A.max_blk;
+    double *blk = lds + (size_t)wv * (A.max_blk + A.part_words), *part = blk + A.max_blk;
     const int n = list ? (int)*cnt : A.nq4;
     for (int e = blockIdx.x * 4 + wv; e < n; e += gridDim.x * 4) {
         const int qi = list ? list[e] : e;
@@ -1363,7 +1412,7 @@ static void launch_sp_all(const EriArgs &b, const int *lists, const unsigned *cn
     launch_sp1<NDP, 3, 2, 16, JK>(b, lists, cnt, soff, roff, surv, rowval, st);
 }
 
-size_t contract_lds_bytes(const EriArgs &a) { return 4 * sizeof(double) * ((size_t)a.max_blk + 64); }
+size_t contract_lds_bytes(const EriArgs &a) { return 4 * sizeof(double) * ((size_t)a.max_blk + a.part_words); }
 
 __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long *__restrict__ soff,
                                                      const long long *__restrict__ roff,
@@ -1371,7 +1420,7 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                                                      const double *__restrict__ realval, int jk_lds,
                                                      const int *__restrict__ list, const unsigned *__restrict__ cnt) {
     // LDS: [4][max_blk] blocks, [4][4][36] per-wave J/K sub-blocks and the P sub-blocks
-    // they read, [4][64] partial sums of the contraction, then J[nfuc*nbas],
+    // they read, [4][part_words] the contraction's staging, then J[nfuc*nbas],
     // K[nfuc*nbas] when jk_lds
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1379,8 +1428,8 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
     const size_t njk = (size_t)nf * nb;
     double *blk = lds + (size_t)wv * A.max_blk;
     double *Jw = lds + 4 * (size_t)A.max_blk + 144 * wv, *Kw = Jw + 36, *Pj = Jw + 72, *Pk = Jw + 108;
-    double *part = lds + 4 * (size_t)A.max_blk + 4 * 144 + 64 * wv;
-    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 144 + 4 * 64 : A.Jacc;
+    double *part = lds + 4 * (size_t)A.max_blk + 4 * 144 + (size_t)A.part_words * wv;
+    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 144 + 4 * (size_t)A.part_words : A.Jacc;
     double *Kl = jk_lds ? Jl + njk : A.Kacc;
     if (jk_lds) {
         for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
@@ -1468,7 +1517,7 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
                               const double *rowval, const double *realval, const int *list, const unsigned *cnt,
                               hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 144 + 4 * 64);
+    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 144 + 4 * (size_t)a.part_words);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
     const size_t lds_max = 160 * 1024;
     // TCG_JK_GLOBAL=1 forces the global-memory accumulation (the path of systems

@@ -172,6 +172,7 @@ bool diis_solve(const std::vector<double> &B, int m, std::vector<double> &w) {
 // phase table of the lattice they were built for.
 struct ScfWork {
     DevBuf<double> dTt, dVt, dH, dP, dJ, dK, dC, dCerr, dmax, dpart, dsc, de, dw, dsmin, dtmp;
+    DevBuf<double> dPprev, dDP, dJacc, dKacc;    // incremental direct builds: last full-built density, J, K
     DevBuf<double2> dph, dSk, dXk, dPk, dErrH, dFH, dFx, dCk;
     DevBuf<int> ddead, docc;
     double *hs = nullptr;
@@ -288,7 +289,8 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     const int nf = L.nf, nb = L.nb, N = L.N;
     if (nocc < 1 || nocc > nb) return fail(c, TCG_E_ARG, "tcg_rhf_sao: nocc outside 1..nbas");
     if (max_scf < 1 || !(thresh > 0.0)) return fail(c, TCG_E_ARG, "tcg_rhf_sao: max_scf < 1 or thresh <= 0");
-    if (source != 0 && source != 1) return fail(c, TCG_E_ARG, "tcg_rhf_sao: source must be 0 (stored) or 1 (direct)");
+    if (source < 0 || source > 2)
+        return fail(c, TCG_E_ARG, "tcg_rhf_sao: source must be 0 (stored), 1 (direct) or 2 (direct, incremental)");
     const double t_start = wall_ms();
     hipStream_t st = ctx_stream(c);
     const size_t nn = (size_t)nb * nb, len = (size_t)N * nf * nf;
@@ -335,6 +337,14 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     SCHK(c, dw.alloc(nb));
     SCHK(c, docc.alloc(nb));
     SCHK(c, dph.alloc((size_t)N * N));
+    if (source == 2) {
+        SCHK(c, w->dPprev.alloc(nn));
+        SCHK(c, w->dDP.alloc(nn));
+        SCHK(c, w->dJacc.alloc((size_t)nf * nb));
+        SCHK(c, w->dKacc.alloc((size_t)nf * nb));
+        ctx_screen_reset(c);
+    }
+    const int full_every = ctx_screen_full_every(c);
     if (w->ph_key[0] != L.ncl[0] || w->ph_key[1] != L.ncl[1] || w->ph_key[2] != L.ncl[2]) {
         const std::vector<double2> ph = phase_table(L);
         SCHK(c, hipMemcpy(dph.p, ph.data(), sizeof(double2) * ph.size(), hipMemcpyHostToDevice));
@@ -392,8 +402,28 @@ int tcg_rhf_sao(tcg_ctx *c, int source, int nbas, int nocc, const double *S, con
     while (it < max_scf) {
         ++it;
         SCHK(c, hipEventRecord(ej0, st));
-        rc = ctx_jk_device(c, source, dP.p, dJ.p, dK.p, &ld);
-        if (rc) return rc;
+        if (source == 2) {
+            // incremental Fock build: a full direct J/K every full_every iterations; in
+            // between, J and K of the density change P - P_prev with density screening,
+            // added to the last J and K (the direct J/K is linear in P)
+            const size_t njk = (size_t)nf * nb;
+            if ((it - 1) % full_every == 0) {
+                rc = ctx_jk_device(c, 1, dP.p, dJ.p, dK.p, &ld);
+                if (rc) return rc;
+            } else {
+                SCHK(c, launch_axpby(dP.p, w->dPprev.p, w->dDP.p, nn, 1.0, -1.0, st));
+                rc = ctx_jk_device(c, 2, w->dDP.p, dJ.p, dK.p, &ld);
+                if (rc) return rc;
+                SCHK(c, launch_axpby(dJ.p, w->dJacc.p, dJ.p, njk, 1.0, 1.0, st));
+                SCHK(c, launch_axpby(dK.p, w->dKacc.p, dK.p, njk, 1.0, 1.0, st));
+            }
+            SCHK(c, hipMemcpyAsync(w->dPprev.p, dP.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+            SCHK(c, hipMemcpyAsync(w->dJacc.p, dJ.p, sizeof(double) * njk, hipMemcpyDeviceToDevice, st));
+            SCHK(c, hipMemcpyAsync(w->dKacc.p, dK.p, sizeof(double) * njk, hipMemcpyDeviceToDevice, st));
+        } else {
+            rc = ctx_jk_device(c, source, dP.p, dJ.p, dK.p, &ld);
+            if (rc) return rc;
+        }
         SCHK(c, hipEventRecord(ej1, st));
         // slot of this iteration's F and error: a free one, or the oldest
         int slot = 0;

@@ -117,6 +117,7 @@ def _bind():
     L.tcg_df_b.argtypes = [_vp, _dp]
     L.tcg_df_jk.argtypes = [_vp, _ci, _dp, _dp, _dp]
     L.tcg_df_kmp2.argtypes = [_vp, _ci, _dp, _dp, _ip, _dp]
+    L.tcg_df_kmp2_fp32.argtypes = [_vp, _ci, _dp, _dp, _ip, _dp]
     L.tcg_df_times.argtypes = [_vp, _dp]
     L.tcg_df_last_error.argtypes = [_vp]
     L.tcg_df_last_error.restype = ctypes.c_char_p
@@ -186,8 +187,11 @@ class DF:
         self._check(_bind().tcg_df_jk(self.h, nb, Pf.ravel(), J, K), "tcg_df_jk")
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
 
-    def kmp2(self, mo_coef: np.ndarray, mo_energies: np.ndarray, occupied: np.ndarray):
-        """k-point DF-MP2; arguments and result as Engine.kmp2."""
+    def kmp2(self, mo_coef: np.ndarray, mo_energies: np.ndarray, occupied: np.ndarray, fp32: bool = False):
+        """k-point DF-MP2; arguments and result as Engine.kmp2.  fp32: the
+        three-index transform and the (ov|ov) assembly in FP32 (FP32 matrix-core
+        GEMMs, FP32 accumulation) with compensated FP32 energy sums
+        (tcg_df_kmp2_fp32)."""
         s = self.s
         nb, nf = s.nbas, s.nfuc
         N = nb // nf
@@ -198,6 +202,7 @@ class DF:
         e = np.ascontiguousarray(np.asarray(mo_energies, dtype=np.float64).reshape(N * nf))
         occ = np.ascontiguousarray(np.where(np.asarray(occupied).reshape(N * nf), 2, 0).astype(np.int32))
         out = np.zeros(5)
-        self._check(_bind().tcg_df_kmp2(self.h, nb, cf, e, occ, out), "tcg_df_kmp2")
+        fn = _bind().tcg_df_kmp2_fp32 if fp32 else _bind().tcg_df_kmp2
+        self._check(fn(self.h, nb, cf, e, occ, out), "tcg_df_kmp2_fp32" if fp32 else "tcg_df_kmp2")
         return {"ec": out[0], "direct": out[1], "exchange": out[2], "zero_denominators": int(out[3]),
                 "gpu_ms": out[4]}

@@ -42,6 +42,9 @@ def lib():
         L.tcg_mp2_gamma.argtypes = [_vp, _ci, _ci, _dp, _dp, _dp]
         L.tcg_kmp2.argtypes = [_vp, _ci, _dp, _dp, _ip, _dp]
         L.tcg_kmp2_direct.argtypes = [_vp, _ci, _dp, _dp, _ip, _ci, _dp]
+        L.tcg_set_direct_screen.argtypes = [_vp, ctypes.c_double, _ci, _ci]
+        L.tcg_direct_jk_screened.argtypes = [_vp, _ci, _dp, ctypes.c_double, _dp, _dp]
+        L.tcg_direct_screen_info.argtypes = [_vp, _dp, _ip, np.ctypeslib.ndpointer(dtype=np.int64)]
         L.tcg_one_electron.argtypes = [_vp, _ci, _dp, _ip, _dp, _dp, _dp]
         L.tcg_eri_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _ci]
         L.tcg_eri_times.argtypes = [_vp, _dp, _ci]
@@ -268,19 +271,39 @@ class Engine:
         return {"ec": out[0], "direct": out[1], "exchange": out[2], "zero_denominators": int(out[3]),
                 "gpu_ms": out[4], "integral_ms": out[5], "transform_ms": out[6], "slabs": int(out[7])}
 
+    def set_direct_screen(self, tau: float = 1e-13, full_every: int = 50, learned_bound: bool = False):
+        """Density screening of the incremental direct J/K (tcg_set_direct_screen)."""
+        self._check(lib().tcg_set_direct_screen(self.h, float(tau), int(full_every), 1 if learned_bound else 0),
+                    "tcg_set_direct_screen")
+
+    def direct_screen_info(self):
+        """Quartets evaluated / listed in the last direct J/K and since the last
+        reset (tcg_direct_screen_info)."""
+        tau = np.zeros(1)
+        fe = np.zeros(1, dtype=np.int32)
+        c = np.zeros(5, dtype=np.int64)
+        self._check(lib().tcg_direct_screen_info(self.h, tau, fe, c), "tcg_direct_screen_info")
+        return {"tau": float(tau[0]), "full_every": int(fe[0]), "kept": int(c[0]), "listed": int(c[1]),
+                "kept_total": int(c[2]), "listed_total": int(c[3]), "builds": int(c[4])}
+
     # -- direct_init / direct_JK -------------------------------------------------
     def direct_init(self):
         """Schwarz tables and the unique quartet list for the direct J/K."""
         self._check(lib().tcg_direct_init(self.h), "tcg_direct_init")
 
-    def direct_jk(self, P: np.ndarray):
+    def direct_jk(self, P: np.ndarray, tau: float = 0.0):
         """J, K (row-major numpy, rows > nfuc zero) with the integrals
-        recomputed on the GPU and contracted on the fly."""
+        recomputed on the GPU and contracted on the fly; tau > 0: with density
+        screening at tau (tcg_direct_jk_screened)."""
         nb = self.s.nbas
         Pf = np.ascontiguousarray(np.asarray(P, dtype=np.float64).T)
         J = np.zeros(nb * nb)
         K = np.zeros(nb * nb)
-        self._check(lib().tcg_direct_jk(self.h, nb, Pf.ravel(), J, K), "tcg_direct_jk")
+        if tau > 0.0:
+            self._check(lib().tcg_direct_jk_screened(self.h, nb, Pf.ravel(), float(tau), J, K),
+                        "tcg_direct_jk_screened")
+        else:
+            self._check(lib().tcg_direct_jk(self.h, nb, Pf.ravel(), J, K), "tcg_direct_jk")
         return J.reshape(nb, nb).T.copy(), K.reshape(nb, nb).T.copy()
 
     # -- k-space (SAO) algebra ----------------------------------------------------
@@ -319,9 +342,10 @@ class Engine:
     def rhf_sao(self, S, T, V, nocc: int, source: str = "stored", max_scf: int = 100, thresh: float = 1e-9,
                 diis: bool = True) -> dict:
         """RHF_SAO with every iteration on the GPU (tcg_rhf_sao).  source: 'stored'
-        (the resident SAO ERI) or 'direct' (after direct_init)."""
+        (the resident SAO ERI), 'direct' (after direct_init) or 'direct_incremental'
+        (direct, incremental Fock builds with density screening, set_direct_screen)."""
         nb = self.s.nbas
-        src = {"stored": 0, "direct": 1}[source]
+        src = {"stored": 0, "direct": 1, "direct_incremental": 2}[source]
         f = lambda M: np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel()   # no copy for F-ordered M
         e = np.zeros(nb)
         C = np.zeros(2 * nb * nb)
