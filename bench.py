@@ -28,7 +28,10 @@ workload: RHF (k-space, J/K from the resident ERI) and k-point MP2 wall times;
 time, energy differences to the plain path, speed-ups); ``df_mp2`` runs the
 density-fitted path (41 even-tempered auxiliary functions per H): df_init,
 DF-RHF, DF-kMP2, and the DF correlation energy on the exact orbitals against the
-exact k-MP2.
+exact k-MP2 (and in FP32 on the matrix cores); ``learned_direct`` runs the
+integral-direct ERI + RHF + MP2 of the Tori2D 8x8 H2 lattice plain, plain on the
+converged (128-point) quadrature, and learned (per-quartet orders, incremental
+Fock builds with density screening).
 """
 from __future__ import annotations
 
@@ -156,6 +159,65 @@ def _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total, ms_step_pla
             "eri_rhf_mp2_wall_s": total, "speedup_eri_rhf_mp2": plain_total / total}
 
 
+def _learned_direct_leg(workload: str = "h2_8x8"):
+    """Integral-direct ERI + RHF + MP2 on the Tori2D 8x8 lattice of H2 (640
+    functions), the path where the integrals are recomputed every SCF
+    iteration: direct RHF then the direct k-MP2 (no stored tensor), three ways:
+      plain      the reference's rule after its start-up self-check (the 64-point
+                 table here), every quartet at every iteration (direct_JK);
+      plain_128  the same on the 128-point table (the converged quadrature the
+                 network is trained to reproduce: the plain path at the learned
+                 path's accuracy);
+      learned    per-quartet learned orders and the incremental Fock build with
+                 density screening at 1e-13."""
+    from william_ci_amd import learned, scf
+    from william_ci_amd import system as tsys
+    from william_ci_amd.engine import Engine, normalized_coef
+    s = tsys.workload(workload)
+    eng = Engine(s, normalized_coef(s))
+    sc = eng.fast_quad_self_check(64)
+    S, T, V = eng.one_electron()
+    model = learned.Model.load()
+    model.install(eng)
+
+    def run(setup, source):
+        setup()
+        t0 = time.perf_counter()
+        r = scf.rhf_sao_gpu(s, eng, S, T, V, source=source)
+        t1 = time.perf_counter()
+        m = eng.kmp2_direct(r.mo_coef, r.mo_energies, r.occupied)
+        t2 = time.perf_counter()
+        info = eng.direct_screen_info()
+        return {"rhf_energy": r.energy, "iterations": r.iterations, "rhf_s": t1 - t0, "mp2_ec": m["ec"],
+                "mp2_s": t2 - t1, "e_total": r.energy + m["ec"], "wall_s": t2 - t0,
+                "quartets_evaluated": info["kept_total"], "quartets_listed": info["listed_total"]}
+    run(lambda: eng.direct_init(), "direct")                        # warm-up (first-call buffers)
+    plain = run(lambda: eng.direct_init(), "direct")
+    counts = {}
+
+    def learned_setup():
+        counts["c"] = learned.direct_init_learned(eng, model)
+        eng.set_direct_screen(1e-13, 50, False)
+    lrn = run(learned_setup, "direct_incremental")
+    quad = eng.quadrature()
+    eng.set_quadrature(False, 128)
+    p128 = run(lambda: eng.direct_init(), "direct")
+    eng.close()
+    c = counts["c"]
+    nord = len(model.orders)
+    total = int(c.sum())
+    return {"workload": f"{workload} (Tori2D 8x8 H2, {s.nbas} functions)",
+            "self_check": sc, "plain_rule": f"{quad[1]}-point {'fast' if quad[0] else 'table'}",
+            "plain": plain, "plain_128": p128, "learned": lrn,
+            "learned_orders": {"plain_rule": int(c[nord]), **{str(o): int(v) for o, v in zip(model.orders, c[:nord])}},
+            "fallback_fraction": int(c[nord]) / max(total, 1),
+            "speedup_vs_plain": plain["wall_s"] / lrn["wall_s"],
+            "speedup_vs_plain_128": p128["wall_s"] / lrn["wall_s"],
+            "dE_total_vs_plain": lrn["e_total"] - plain["e_total"],
+            "dE_total_vs_plain_128": lrn["e_total"] - p128["e_total"],
+            "dE_total_plain_vs_plain_128": plain["e_total"] - p128["e_total"]}
+
+
 def _df_leg(s, eng, S, T, V, res, mp2):
     """Density-fitted MP2 on the same chain (keyword density_fitting with the
     41-function even-tempered AuxBasis): df_init, DF-RHF, DF-kMP2; and DF-kMP2
@@ -179,6 +241,7 @@ def _df_leg(s, eng, S, T, V, res, mp2):
     md = d.kmp2(rd.mo_coef, rd.mo_energies, rd.occupied)
     t3 = time.perf_counter()
     on_exact = d.kmp2(res.mo_coef, res.mo_energies, res.occupied)
+    fp32 = d.kmp2(res.mo_coef, res.mo_energies, res.occupied, fp32=True)
     info, times = d.info(), d.times()
     d.close()
     return {"aux": "AuxBasis_H_etb41 (s 8: 0.1-40, p 5: 0.2-10, d 3: 0.3-3; 41 per H)",
@@ -189,7 +252,10 @@ def _df_leg(s, eng, S, T, V, res, mp2):
             "kmp2_ec": md["ec"], "kmp2_gpu_ms": md["gpu_ms"], "kmp2_wall_s": t3 - t2,
             "e_total": rd.energy + md["ec"], "df_init_rhf_mp2_wall_s": t3 - t0,
             "ec_on_exact_orbitals": on_exact["ec"], "ec_exact": mp2["ec"],
-            "fit_error_ec": on_exact["ec"] - mp2["ec"]}
+            "fit_error_ec": on_exact["ec"] - mp2["ec"],
+            "fp32": {"kind": "FP32 L, (ov|ov) as FP32 matrix-core GEMMs, compensated FP32 energy sums",
+                     "ec": fp32["ec"], "gpu_ms": fp32["gpu_ms"], "ec_fp64": on_exact["ec"], "gpu_ms_fp64": on_exact["gpu_ms"],
+                     "dE_vs_fp64": fp32["ec"] - on_exact["ec"]}}
 
 
 def main():
@@ -204,6 +270,7 @@ def main():
                     help="keep the 32-point fast rule (skip the start-up fast_quad self-check)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-order leg")
     ap.add_argument("--no-df", action="store_true", help="skip the density-fitted MP2 leg")
+    ap.add_argument("--no-direct", action="store_true", help="skip the learned integral-direct leg (Tori2D 8x8)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -278,6 +345,7 @@ def main():
                                                              ms_step_plain)
     df_part = None if args.no_df else _df_leg(s, eng, S, T, V, res, mp2)
     eng.close()
+    direct_part = None if args.no_direct else _learned_direct_leg()
 
     nb = s.nbas
     unique = nb ** 4 / 8
@@ -320,6 +388,7 @@ def main():
                      "all_quartet_flops": st["flops"]},
     }
     line["learned"] = learned_part
+    line["learned_direct"] = direct_part
     line["df_mp2"] = df_part
     if not args.no_cpu_baseline:
         cb = cpu_baseline(s, quad, args.cpu_seconds)

@@ -133,7 +133,8 @@ int tcg_kmp2(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, c
  * slabs of the last index, j4 = d + nf R4 for the cells R4 of the slab (the
  * reference slices by unit-cell shell; the transform is linear in j4 either
  * way), each slab transformed over its two ket indices and summed; the unique
- * quartet list is evaluated once per slab.  slab_cells = cells per slab (0:
+ * quartet list is evaluated once per slab (with the learned rules after
+ * tcg_direct_init_learned).  slab_cells = cells per slab (0:
  * as many as a quarter of the free device memory holds).  At most 64 cells.
  * out[0..4] as tcg_kmp2; out[5] integral time, out[6] transform time (ms),
  * out[7] number of slabs. */

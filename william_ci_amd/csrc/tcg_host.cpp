@@ -2005,6 +2005,11 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
         EriArgs A = base_args(c);
         A.q4 = c->d_q4.p;
         A.nq4 = c->nq4;
+        if (c->direct_learned) {                // the learned list of tcg_direct_init_learned
+            A.q4 = c->d_q4dl.p;
+            A.nq4 = c->nq4dl;
+            A.qrule = c->d_qrdl.p;
+        }
         A.screen_prim = 1;
         A.Qao = c->d_Q.p;
         A.eri = slab.p;
