@@ -137,3 +137,43 @@ def test_tori3d_direct_rhf_and_density_fitting(tori3d):
     assert abs(EJf - EJ) < 1e-4 * abs(EJ)
     assert abs(EKf - EK) < 1e-3 * abs(EK)
     assert -0.1 * s.n_occ < m["ec"] < 0.0
+
+
+def test_tori2d_8x8_learned_direct_matches_converged_quadrature():
+    """The 640-function lattice with the reference's start-up rule choice (the
+    64-point table): learned per-quartet orders and incremental Fock builds in the
+    direct RHF, then the direct k-MP2, against the plain direct path on the
+    converged quadrature (the 128-point table): E_total within 1 microhartree, no
+    quartet on the fallback rule, and fewer quartets evaluated than the plain
+    path's."""
+    import os
+    from william_ci_amd import learned
+    model_path = learned.MODEL_PATH
+    if not os.path.exists(model_path):
+        pytest.skip("no trained model committed")
+    s = tsys.workload("h2_8x8")
+    print(f"case h2_8x8 learned direct nbas={s.nbas}", flush=True)
+    eng = _engine(s)
+    eng.fast_quad_self_check(64)
+    S, T, V = eng.one_electron()
+    m = learned.Model.load()
+    m.install(eng)
+    counts = learned.direct_init_learned(eng, m)
+    eng.set_direct_screen()
+    rl = scf.rhf_sao_gpu(s, eng, S, T, V, source="direct_incremental")
+    ml = eng.kmp2_direct(rl.mo_coef, rl.mo_energies, rl.occupied)
+    info = eng.direct_screen_info()
+    eng.set_quadrature(False, 128)
+    eng.direct_init()
+    r0 = scf.rhf_sao_gpu(s, eng, S, T, V, source="direct")
+    m0 = eng.kmp2_direct(r0.mo_coef, r0.mo_energies, r0.occupied)
+    info0 = eng.direct_screen_info()
+    eng.close()
+    nord = len(m.orders)
+    el, e0 = rl.energy + ml["ec"], r0.energy + m0["ec"]
+    print(f"  learned E_total {el:.10f} ({rl.iterations} it, {info['kept_total']} quartets)  "
+          f"128-point table {e0:.10f} ({r0.iterations} it, {info0['kept_total']} quartets)  diff {el - e0:.2e}")
+    assert rl.converged and r0.converged
+    assert counts[nord] == 0
+    assert abs(el - e0) < 1e-6
+    assert info["kept_total"] < info0["kept_total"]

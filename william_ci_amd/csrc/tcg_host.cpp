@@ -1934,6 +1934,12 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     if (cells2.empty()) cells2.push_back(0);
     const int n2 = (int)cells2.size();
     const long long IA = (long long)nf * nf * n2, IB = IA * o_hi * N;
+    // the screened direct J/K's scratch lists are rebuilt on demand: give their memory
+    // to the slabs
+    c->d_q4scr.release();
+    c->d_qrscr.release();
+    c->d_keep.release();
+    c->d_sidx.release();
     // cells per slab: the caller's, or what a quarter of the free device memory holds
     // next to the work buffers of the transforms
     const size_t nYB = (size_t)IB * nv * N, nYC = (size_t)nf * nv * N * o_hi * N * nv * N;
