@@ -169,7 +169,7 @@ def _learned_direct_leg(workload: str = "h2_8x8"):
                  network is trained to reproduce: the plain path at the learned
                  path's accuracy);
       learned    per-quartet learned orders and the incremental Fock build with
-                 density screening at 1e-13."""
+                 density screening at 1e-14."""
     from william_ci_amd import learned, scf
     from william_ci_amd import system as tsys
     from william_ci_amd.engine import Engine, normalized_coef
@@ -197,7 +197,7 @@ def _learned_direct_leg(workload: str = "h2_8x8"):
 
     def learned_setup():
         counts["c"] = learned.direct_init_learned(eng, model)
-        eng.set_direct_screen(1e-13, 50, False)
+        eng.set_direct_screen(1e-14, 50, False)
     lrn = run(learned_setup, "direct_incremental")
     quad = eng.quadrature()
     eng.set_quadrature(False, 128)

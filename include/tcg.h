@@ -338,8 +338,7 @@ int tcg_direct_init_learned(tcg_ctx *ctx, long long *rule_counts);
  *   contracts with (pairs AB, CD, AC, AD, BC, BD) is below tau.  The bound is
  *   the Schwarz product Qsh(AB) Qsh(CD), or (use_learned_bound, with the
  *   learned list of tcg_direct_init_learned) the smaller of it and the
- *   network's bound.  Defaults: tau 1e-13 (the reference's Schwarz threshold
- *   on the product with the density change), full_every 50, Schwarz bound.
+ *   network's bound.  Defaults: tau 1e-14, full_every 50, Schwarz bound.
  * tcg_direct_screen_info: tau, full_every and counts[5]: shell quartets
  * evaluated / listed in the last direct J/K, then since the last
  * tcg_direct_init or tcg_rhf_sao: evaluated, listed, number of J/K builds. */

@@ -6,7 +6,7 @@ J and K; and the learned per-quartet orders in the direct J/K
 Bars:
   * incremental vs the plain direct RHF (every quartet at every iteration, the
     reference's direct_JK): the same converged energy to 1e-8 Eh at the
-    default threshold (1e-13 on bound x density change);
+    default threshold (1e-14 on bound x density change);
   * the screened J/K of a density equals the unscreened one within the
     threshold's reach, and with tau = 0 it is the unscreened one exactly in
     the number of quartets;
@@ -41,7 +41,7 @@ def test_incremental_direct_rhf_matches_plain(name):
     S, T, V = eng.one_electron()
     eng.direct_init()
     r0 = scf.rhf_sao_gpu(s, eng, S, T, V, source="direct")
-    eng.set_direct_screen()                                   # defaults: tau 1e-13, full_every 50
+    eng.set_direct_screen()                                   # defaults: tau 1e-14, full_every 50
     r1 = scf.rhf_sao_gpu(s, eng, S, T, V, source="direct_incremental")
     info = eng.direct_screen_info()
     eng.close()

@@ -136,8 +136,7 @@ struct tcg_ctx {
     DevBuf<float> d_lb, d_lbdl;        // log10 learned bound per quartet (all / the direct list)
     int nq4dl = 0;
     // density screening of the direct J/K (incremental Fock builds, tcg_set_direct_screen)
-    double screen_tau = 1e-13;         // skip when bound x largest density element < tau (the
-                                       // reference's Schwarz threshold, ERI_toroidal.f90:64)
+    double screen_tau = 1e-14;         // skip when bound x largest density element < tau
     int screen_full_every = 50;        // a full (unscreened) build every this many iterations
     int screen_learned = 0;            // 1: the bound is min(Schwarz, learned) when the learned list runs
     DevBuf<double> d_Dsh;
@@ -843,6 +842,16 @@ static int learned_rules(tcg_ctx *c, EriArgs &A, bool gather, DevBuf<Quartet> &q
     for (int r = 0; r < nord; ++r) c->learned_counts[r] = (long long)hist[r + 1];
     c->learned_counts[nord] = (long long)hist[0];
     c->learned_counts[nord + 1] = (long long)hist[15];
+    if (gather) {
+        // the direct list keeps its rules: the per-survivor features, predictions and
+        // owners are not needed again (tens of GB on large lattices)
+        c->d_feat.release();
+        c->d_pred.release();
+        c->d_ood.release();
+        c->d_sq.release();
+        c->d_lb.release();
+        c->d_qrule.release();
+    }
     return 0;
 }
 
@@ -1934,22 +1943,32 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     if (cells2.empty()) cells2.push_back(0);
     const int n2 = (int)cells2.size();
     const long long IA = (long long)nf * nf * n2, IB = IA * o_hi * N;
-    // the screened direct J/K's scratch lists are rebuilt on demand: give their memory
-    // to the slabs
+    // the screened direct J/K's scratch lists and the row pipeline's buffers are
+    // rebuilt on demand: give their memory to the slabs
     c->d_q4scr.release();
     c->d_qrscr.release();
     c->d_keep.release();
     c->d_sidx.release();
-    // cells per slab: the caller's, or what a quarter of the free device memory holds
-    // next to the work buffers of the transforms
+    c->d_surv.release();
+    c->d_rows.release();
+    c->d_keys.release();
+    c->d_keys2.release();
+    c->d_vals.release();
+    c->d_order.release();
+    c->d_rowval.release();
+    c->d_realval.release();
+    HIPCHK(c, hipDeviceSynchronize());
+    // cells per slab: the caller's, or what the free device memory holds next to the
+    // work buffers of the transforms and the row pipeline of a slab build (its row
+    // budget, a quarter more for survivors and sort space)
     const size_t nYB = (size_t)IB * nv * N, nYC = (size_t)nf * nv * N * o_hi * N * nv * N;
     const size_t per_cell = (size_t)nf * nb * nb * nf * sizeof(double) + (size_t)IA * o_hi * N * nf * 2 * sizeof(double);
     int nR = slab_cells;
     if (nR == 0) {
         size_t fr = 0, tot = 0;
         HIPCHK(c, hipMemGetInfo(&fr, &tot));
-        const size_t fixed = 2 * sizeof(double) * (nYB + nYC);
-        const size_t avail = fr > fixed ? (fr - fixed) / 2 : 0;
+        const size_t fixed = 2 * sizeof(double) * (nYB + nYC) + (size_t)(1.25 * c->row_budget) + ((size_t)1 << 32);
+        const size_t avail = fr > fixed ? fr - fixed : 0;
         nR = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, avail / per_cell));
     }
     nR = std::min(nR, N);

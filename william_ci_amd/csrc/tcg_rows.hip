@@ -1315,6 +1315,64 @@ __device__ __forceinline__ void jk_block(const EriArgs &A, const Shell &sA, cons
     }
 }
 
+// The same for J and K in global memory, by nl lanes of one quartet: one
+// permutation at a time, the counted elements add into the (n0 x n1) J and
+// (n0 x n3) K sub-blocks the permutation maps them to (w[0..35] and w[36..71],
+// LDS), reading the (n2 x n3) and (n1 x n2) density blocks staged in w[72..143];
+// each sub-block then leaves with one global atomic per entry instead of one per
+// element.  Positions: those of component (0,0,0,0), pb, plus the offsets.
+__device__ __forceinline__ void jk_block_sub(const EriArgs &A, const Shell &sA, const Shell &sB, const Shell &sC,
+                                             const Shell &sD, const double *blk, int lane, int nl, bool stab,
+                                             double *w, double *Jg, double *Kg) {
+    const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
+    const int nblk = nA * nB * nC * nD;
+    const int nf = A.nfuc, nb = A.nbas;
+    const int ncmp[4] = {nA, nB, nC, nD};
+    const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
+    double *Jw = w, *Kw = w + 36, *Pj = w + 72, *Pk = w + 108;
+    for (int ip = 0; ip < 8; ++ip) {
+        const int x0 = c_perm8_rows[ip][0], x1 = c_perm8_rows[ip][1], x2 = c_perm8_rows[ip][2],
+                  x3 = c_perm8_rows[ip][3];
+        const int n0 = ncmp[x0], n1 = ncmp[x1], n2 = ncmp[x2], n3 = ncmp[x3];
+        int pb[4];
+        stored_position(A, f0, ip, pb);
+        for (int e = lane; e < 36; e += nl) {
+            Jw[e] = 0.0;
+            Kw[e] = 0.0;
+            if (e < n2 * n3) Pj[e] = A.Pden[(pb[2] + e / n3) + (size_t)nb * (pb[3] + e % n3)];
+            if (e < n1 * n2) Pk[e] = A.Pden[(pb[1] + e / n2) + (size_t)nb * (pb[2] + e % n2)];
+        }
+        wave_sync();
+        for (int comp = lane; comp < nblk; comp += nl) {
+            const double val = blk[comp];
+            if (val == 0.0) continue;
+            int t = comp;
+            const int ca = t % nA; t /= nA;
+            const int cb = t % nB; t /= nB;
+            const int cc = t % nC;
+            const int cd = t / nC;
+            const int c4[4] = {ca, cb, cc, cd};
+            const int i0 = c4[x0], i1 = c4[x1], i2 = c4[x2], i3 = c4[x3];
+            if (stab) {
+                const int pp[4] = {pb[0] + i0, pb[1] + i1, pb[2] + i2, pb[3] + i3};
+                if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
+            }
+            atomicAdd(&Jw[i0 * n1 + i1], val * Pj[i2 * n3 + i3]);
+            atomicAdd(&Kw[i0 * n3 + i3], -0.5 * val * Pk[i1 * n2 + i2]);
+        }
+        wave_sync();
+        for (int e = lane; e < n0 * n1; e += nl) {
+            const double v = Jw[e];
+            if (v != 0.0) atomicAdd(&Jg[(pb[0] + e / n1) + (size_t)nf * (pb[1] + e % n1)], v);
+        }
+        for (int e = lane; e < n0 * n3; e += nl) {
+            const double v = Kw[e];
+            if (v != 0.0) atomicAdd(&Kg[(pb[0] + e / n3) + (size_t)nf * (pb[3] + e % n3)], v);
+        }
+        wave_sync();
+    }
+}
+
 // JK = 0: the block goes to the ERI array (write_block); JK = 1: its J and K
 // contributions (jk_block), into LDS copies of J and K flushed at the end when
 // jk_lds, else straight into the global arrays.
@@ -1327,6 +1385,7 @@ __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__res
                                                      int jk_lds) {
     constexpr int NG = 256 / GL;
     __shared__ double lds[NG * pow3(P)];
+    __shared__ double jkw[JK ? NG : 1][JK ? 144 : 1];   // per-group J/K sub-blocks (global J/K)
     extern __shared__ double jkl[];
     const int grp = threadIdx.x / GL, gl = threadIdx.x & (GL - 1);
     double *blk = lds + grp * pow3(P);
@@ -1351,7 +1410,9 @@ __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__res
             if (JK) {
                 screen_block(A, sA, sB, sC, sD, blk, gl, GL);
                 wave_sync();
-                jk_block(A, sA, sB, sC, sD, blk, gl, GL, shell_stabilized(A, sA, sB, sC, sD), Jl, Kl);
+                const bool stab = shell_stabilized(A, sA, sB, sC, sD);
+                if (jk_lds) jk_block(A, sA, sB, sC, sD, blk, gl, GL, stab, Jl, Kl);
+                else jk_block_sub(A, sA, sB, sC, sD, blk, gl, GL, stab, jkw[JK ? grp : 0], Jl, Kl);
             } else {
                 write_block(A, Q, sA, sB, sC, sD, blk, gl, GL);
             }
@@ -1365,6 +1426,14 @@ __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__res
             if (jkl[njk + i] != 0.0) atomicAdd(&A.Kacc[i], jkl[njk + i]);
         }
     }
+}
+
+// TCG_JK_GLOBAL=1 forces the global-memory J/K accumulation (the path of systems
+// whose J/K rows exceed the LDS) so that it can be tested on small systems; read
+// at every launch, so a test can switch it within one process
+static bool jk_force_global() {
+    const char *e = getenv("TCG_JK_GLOBAL");
+    return e && e[0] == '1';
 }
 
 // J/K copies in LDS per workgroup up to this size (TCG_SP_JK_LDS_KB, default 150
@@ -1384,7 +1453,7 @@ static void launch_sp1(const EriArgs &b, const int *lists, const unsigned *cnt, 
                        const long long *roff, const Surv *surv, const double *rowval, hipStream_t st) {
     constexpr int L = sp_list(P, NBP);
     const size_t jk_bytes = JK ? sizeof(double) * 2 * (size_t)b.nfuc * b.nbas : 0;
-    const int jk_lds = JK && jk_bytes <= sp_jk_lds_bytes() ? 1 : 0;
+    const int jk_lds = JK && jk_bytes <= sp_jk_lds_bytes() && !jk_force_global() ? 1 : 0;
     int ng = (b.nq4 + 256 / GL - 1) / (256 / GL);
     ng = ng < (1 << 20) ? ng : (1 << 20);
     if (jk_lds) {
@@ -1520,12 +1589,7 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
     const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 144 + 4 * (size_t)a.part_words);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
     const size_t lds_max = 160 * 1024;
-    // TCG_JK_GLOBAL=1 forces the global-memory accumulation (the path of systems
-    // whose J/K rows exceed the LDS) so that it can be tested on small systems
-    static const bool force_global = [] {
-        const char *e = getenv("TCG_JK_GLOBAL");
-        return e && e[0] == '1';
-    }();
+    const bool force_global = jk_force_global();
     const int jk_lds = (blk_bytes + jk_bytes <= lds_max && !force_global) ? 1 : 0;
     const size_t lds = blk_bytes + (jk_lds ? jk_bytes : 0);
     if (lds > 64 * 1024) {

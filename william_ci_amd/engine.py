@@ -271,7 +271,7 @@ class Engine:
         return {"ec": out[0], "direct": out[1], "exchange": out[2], "zero_denominators": int(out[3]),
                 "gpu_ms": out[4], "integral_ms": out[5], "transform_ms": out[6], "slabs": int(out[7])}
 
-    def set_direct_screen(self, tau: float = 1e-13, full_every: int = 50, learned_bound: bool = False):
+    def set_direct_screen(self, tau: float = 1e-14, full_every: int = 50, learned_bound: bool = False):
         """Density screening of the incremental direct J/K (tcg_set_direct_screen)."""
         self._check(lib().tcg_set_direct_screen(self.h, float(tau), int(full_every), 1 if learned_bound else 0),
                     "tcg_set_direct_screen")
