@@ -26,6 +26,7 @@ struct PrimPair {
     double cmax;         // max_comp |c_A(i)| * max_comp |c_B(j)|  (shell-wide max per primitive)
     double ps;           // primitive Schwarz value sqrt|(ij|ij)| (max over components)
     int N[3];            // Bessel length per periodic direction
+    float A[3];          // its Bessel argument 2 p_d / a_d^2 (the row length of a product of two pairs)
     int voff[3];         // offset (complex elements) of the Fourier vectors of direction d:
                          //   vec[voff + combo*(N+1) + n], combo = ia*(lB+1)+ib
     int eoff;            // offset of Hermite E: herm[eoff + ((dr*(lA+1)+ia)*(lB+1)+ib)*(2*kMaxL+1) + t]

@@ -9,7 +9,8 @@
 //                 (ERI_shell_block_toroidal.f90:191,216-219), counted, then listed.
 //   k_rows_emit   one ROW per (surviving primitive quartet, periodic direction d,
 //                 power combination): offsets of its bra/ket Fourier vectors,
-//                 length nm = min(N_bra, N_ket), phase phi, power sum; sort key
+//                 length nm (row_length: the product's cut-off, at most
+//                 min(N_bra, N_ket)), phase phi, power sum; sort key
 //                 (rule, d, nm).
 //   (radix sort of the rows by key, tcg_sort.hip)
 //   k_rows_mfma   one wavefront per 16 consecutive sorted rows:
@@ -144,6 +145,22 @@ hipError_t launch_surv(const EriArgs &a, int pass, long long *scount, long long 
 // ---------------------------------------------------------------------------
 // rows
 // ---------------------------------------------------------------------------
+// Orders of one row F_n = Re[conj(G_n) H_n e^{in phi}]: each pair's Fourier vector
+// is cut where its Bessel factor e^{-A} I_n(A) falls below ~e^{-40.5} of I_0 (N =
+// 20 + 9 sqrt(A), the reference's truncation, clifford_tools.f90:566-572); the
+// product of the two falls there already at n - L = 9 sqrt(A_b A_k / (A_b + A_k))
+// (e^{-n^2/2A} per factor; L = the larger power sum, the shift of the sin-product
+// Fourier coefficients), + 20 as the reference adds, so the row stops at the
+// smaller of that and min(N_bra, N_ket).  The orders left out carry less than
+// 1e-17 of the row's leading terms.
+__device__ __forceinline__ int row_length(int bN, int kN, float Ab, float Ak, int Lb, int Lk) {
+    int nm = bN < kN ? bN : kN;
+    const float s = Ab + Ak;
+    const float h = s > 0.f ? Ab * Ak / s : 0.f;
+    const int cut = 20 + (Lb > Lk ? Lb : Lk) + (int)ceilf(9.0f * sqrtf(h));
+    return cut < nm ? cut : nm;
+}
+
 __global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *__restrict__ soff,
                                                   const long long *__restrict__ roff, const Surv *__restrict__ surv,
                                                   RowRec *__restrict__ rows, unsigned *__restrict__ keys,
@@ -169,7 +186,7 @@ __global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *_
             const int id = t / (l[2] + 1);
             const int bN = pb.N[d], kN = pk.N[d];
             RowRec R;
-            R.nm = bN < kN ? bN : kN;
+            R.nm = row_length(bN, kN, pb.A[d], pk.A[d], l[0] + l[1], l[2] + l[3]);
             R.gofs = pb.voff[d] + (ia * (l[1] + 1) + ib) * (bN + 1);
             R.hofs = pk.voff[d] + (ic * (l[3] + 1) + id) * (kN + 1);
             const double ad = (d == 0) ? A.a[0] : (d == 1) ? A.a[1] : A.a[2];

@@ -179,6 +179,7 @@ __global__ __launch_bounds__(64) void k_pair_setup(PairSetupArgs S) {
                 const int N = nm < S.ncap ? nm : S.ncap;
                 const int jl = ij * ndp + d;
                 P.N[d] = N;
+                P.A[d] = (float)Aarg;
                 P.voff[d] = B.vec0 + jl * ncomb * (S.ncap + 1);
                 VecJob J;
                 J.A = Aarg;
