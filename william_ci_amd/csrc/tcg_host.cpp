@@ -72,7 +72,9 @@ struct tcg_ctx {
     DevBuf<int> d_tablim, d_lxyz, d_poff, d_relc;
     DevBuf<long long> d_tsum;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-    bool time_rows = false;    // record ev2/ev3 around the row MFMA kernel
+    bool time_rows = false;    // record an event pair around every row MFMA launch
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> row_evs;   // ... read after the build's final sync
+    int row_evs_used = 0;
     DevBuf<Shell> d_sh;
     DevBuf<PrimPair> d_pp;
     DevBuf<ShellPair> d_sp;
@@ -534,9 +536,20 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     HIPCHK(c, c->d_tmp.alloc(tb));
     HIPCHK(c, sort_pairs(c->d_tmp.p, tb, c->d_keys.p, c->d_keys2.p, c->d_vals.p, c->d_order.p, (int)nr, end_bit,
                          c->st));
-    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev2, c->st));
+    hipEvent_t rev1 = nullptr;
+    if (c->time_rows) {
+        if (c->row_evs_used == (int)c->row_evs.size()) {
+            std::pair<hipEvent_t, hipEvent_t> e{nullptr, nullptr};
+            HIPCHK(c, hipEventCreate(&e.first));
+            HIPCHK(c, hipEventCreate(&e.second));
+            c->row_evs.push_back(e);
+        }
+        const auto &e = c->row_evs[c->row_evs_used++];
+        HIPCHK(c, hipEventRecord(e.first, c->st));
+        rev1 = e.second;
+    }
     HIPCHK(c, launch_rows_mfma(A, c->d_rows.p, c->d_order.p, nr, c->d_rowval.p, c->st));
-    if (c->time_rows) HIPCHK(c, hipEventRecord(c->ev3, c->st));
+    if (rev1) HIPCHK(c, hipEventRecord(rev1, c->st));
     const int nreal = 3 - A.ndp;
     if (nreal > 0) HIPCHK(c, c->d_realval.alloc((size_t)(nr / A.ndp) * nreal * A.nqp));
     // the block of every quartet goes to the ERI array, or (direct) into J and K
@@ -544,12 +557,18 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     HIPCHK(c, c->d_clscnt.alloc(9));
     HIPCHK(c, launch_contract(A, so, ro, c->d_surv.p, c->d_rowval.p, c->d_realval.p, c->d_cls.p, c->d_clscnt.p,
                               c->st));
-    if (c->time_rows) {
+    return 0;
+}
+
+// Sum of the row MFMA launches' event times of this build (after its final sync)
+static int read_row_times(tcg_ctx *c) {
+    c->rows_ms = 0.0;
+    for (int i = 0; i < c->row_evs_used; ++i) {
         float ms = 0.f;
-        HIPCHK(c, hipEventSynchronize(c->ev3));
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev2, c->ev3));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->row_evs[i].first, c->row_evs[i].second));
         c->rows_ms += ms;
     }
+    c->row_evs_used = 0;
     return 0;
 }
 
@@ -1345,6 +1364,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
     c->time_rows = true;
+    c->row_evs_used = 0;
     if (c->learned_on) {
         rc = learned_pass(c, A);
     } else {
@@ -1371,6 +1391,10 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
     c->times[5] = kms;
     (void)rms;
+    {
+        const int rc6 = read_row_times(c);
+        if (rc6) return rc6;
+    }
     c->times[6] = c->rows_ms;
     // no SAO: rows of the other cells by translation (ERI_toroidal.f90:549-552)
     if (!sao) HIPCHK(c, launch_translate_rows(c->d_eri.p, nb, nf, c->ncell, c->d_relc.p, c->st));
@@ -2401,6 +2425,10 @@ int tcg_destroy(tcg_ctx *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->ev3) (void)hipEventDestroy(c->ev3);
+    for (auto &e : c->row_evs) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     if (c->st) (void)hipStreamSynchronize(c->st);
     if (c->st2) (void)hipStreamSynchronize(c->st2);
     if (c->ev_fill) (void)hipEventDestroy(c->ev_fill);
