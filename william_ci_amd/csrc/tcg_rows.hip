@@ -76,8 +76,27 @@ __device__ __forceinline__ int sp_form(const EriArgs &A, const Quartet &Q, int q
     return P <= 3 ? sp_list(P, lA + lB) : -1;
 }
 
+// A slab build (j4hi > 0) stores only the elements whose last index lies in
+// [j4lo, j4hi), a range of whole cells.  Permutation (p1 p2|p3 p4) of the
+// quartet (A B|C D) stores its block with the first index translated to the unit
+// cell, so its last index lies in cell relc(cell p1, cell p4); the eight
+// permutations pair (p1, p4) as (A,D) (B,D) (A,C) (B,C) (C,B) (D,B) (C,A) (D,A).
+// A quartet none of whose pairs reaches the slab's cells has nothing to store.
+__device__ __forceinline__ bool slab_hit(const EriArgs &A, const Quartet &Q) {
+    if (A.j4hi <= 0) return true;
+    const int r0 = A.j4lo / A.nfuc, r1 = A.j4hi / A.nfuc;
+    const int cA = A.sh[Q.s[0]].cell, cB = A.sh[Q.s[1]].cell, cC = A.sh[Q.s[2]].cell, cD = A.sh[Q.s[3]].cell;
+    auto in = [&](int x, int y) {
+        const int r = A.relc[x * A.ncell + y];
+        return r >= r0 && r < r1;
+    };
+    return in(cA, cD) || in(cB, cD) || in(cA, cC) || in(cB, cC) || in(cC, cB) || in(cD, cB) || in(cC, cA) ||
+           in(cD, cA);
+}
+
 // ---------------------------------------------------------------------------
-// survivors: pass 0 counts (scount, rcount), pass 1 writes at soff
+// survivors: pass 0 counts (scount, rcount), pass 1 writes at soff; a quartet a
+// slab build does not store has none
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__restrict__ scount,
                                              long long *__restrict__ rcount, const long long *__restrict__ soff,
@@ -86,7 +105,7 @@ __global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__
     // a list whose length is on the device: the entries past it count zero
     const int nlive = A.nq4_dev ? (int)min((unsigned long long)A.nq4, *A.nq4_dev) : A.nq4;
     for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
-        if (qi >= nlive) {
+        if (qi >= nlive || !slab_hit(A, A.q4[qi])) {
             if (!pass && lane == 0) {
                 scount[qi] = 0;
                 rcount[qi] = 0;
@@ -1185,12 +1204,14 @@ __global__ __launch_bounds__(256) void k_contract(EriArgs A, const long long *__
 // the quartets into one list per s/p form (sp_list, 0..7) and one (8) of the
 // rest (lists[L * nq4 + i], cnt[L] entries): per workgroup of 1024 the
 // wavefronts' counts meet in LDS, one atomic per workgroup and list
-__global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, int *__restrict__ lists,
-                                                      unsigned *__restrict__ cnt) {
+// (quartets without survivors have a zero block: the outputs start at zero, so
+// they join no list)
+__global__ __launch_bounds__(1024) void k_sp_classify(EriArgs A, const long long *__restrict__ soff,
+                                                      int *__restrict__ lists, unsigned *__restrict__ cnt) {
     __shared__ unsigned wcnt[16][kSpLists], wbase[16][kSpLists];
     const int qi = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int L = -1;
-    if (qi < A.nq4) {
+    if (qi < A.nq4 && soff[qi + 1] > soff[qi]) {
         L = sp_form(A, A.q4[qi], qi);
         if (L < 0) L = kSpLists - 1;
     }
@@ -1640,7 +1661,7 @@ hipError_t launch_contract(const EriArgs &a, const long long *soff, const long l
         // s/p quartets: lists by form, then one kernel per form (grids for the largest list)
         hipError_t e = hipMemsetAsync(cnt, 0, kSpLists * sizeof(unsigned), st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_sp_classify, dim3((a.nq4 + 1023) / 1024), dim3(1024), 0, st, b, lists, cnt);
+        hipLaunchKernelGGL(k_sp_classify, dim3((a.nq4 + 1023) / 1024), dim3(1024), 0, st, b, soff, lists, cnt);
         // real-direction rows only for the quartets of the general kernel (the last list)
         const int *rest = lists + (size_t)(kSpLists - 1) * b.nq4;
         e = real_rows(b, soff, roff, surv, realval, rest, cnt + (kSpLists - 1), st);
