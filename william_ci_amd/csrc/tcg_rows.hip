@@ -448,7 +448,8 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
                     cur ^= 1;
                 }
             }
-            // lane (r, kq) holds D[row = 4v + kq][node = 16h + r]
+            // lane (r, kq) holds D[row = 4v + kq][node = 16h + r]; the node tiles past the
+            // rule's last node (limit -1) are never read and are not written
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * v + kq;
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
                 if (rrow >= 0 && rd == dd) {
 #pragma unroll
                     for (int j = 0; j < kNT; ++j)
-                        rowval[(size_t)rrow * nqp + hbase * kNodeTile + toff[j] + r] = acc[j][v] * sg;
+                        if (sl[j] >= 0) rowval[(size_t)rrow * nqp + hbase * kNodeTile + toff[j] + r] = acc[j][v] * sg;
                 }
             }
         };
@@ -959,6 +960,7 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
     constexpr int NR = 64 / GL;
     int kr[NR];
     double w[NR], t2[NR];
+    const int nrun = (nq + GL - 1) / GL;               // node runs holding the rule's nodes
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
         const int k = GL * h + gl;
@@ -1040,8 +1042,9 @@ __device__ __forceinline__ void quartet_block_sp(const EriArgs &A, int qi, const
             }
             // UH node runs at a time: every periodic row load first (runs past the rule's
             // last node load its last node, weight 0), then per run the real factors and
-            // the products
+            // the products; the runs stop after the rule's last node
             for (int h0 = 0; h0 < NR; h0 += UH) {
+                if (h0 >= nrun) break;
                 double Xp[U][UH][NDP][NC];
 #pragma unroll
                 for (int v = 0; v < U; ++v)
