@@ -51,6 +51,7 @@ struct EriArgs {
     int screen_prim;                  // primitive Schwarz screen (keyword prim_screen)
     int max_ncomb;                    // largest (lA+1)(lB+1)(lC+1)(lD+1) over the quartets
     int max_blk;                      // largest nA nB nC nD over the quartets
+    int sub_words;                    // largest nX nY over two shells: a permutation's J or K sub-block
     int part_words;                   // LDS words per wavefront after the block in the general contraction:
                                       // 64 partial sums, a chunk of 64 survivors (96), 4 x max(ncomp nprim)
     int sp_contract;                  // s/p quartets (at most three p shells) by the s/p contraction kernels

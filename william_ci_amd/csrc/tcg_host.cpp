@@ -492,6 +492,11 @@ static EriArgs base_args(tcg_ctx *c) {
     A.screen_prim = 0;
     A.max_ncomb = c->max_ncomb;
     A.max_blk = c->max_blk;
+    {
+        int nc = 1;
+        for (const Shell &sh : c->sh) nc = std::max(nc, sh.ncomp);
+        A.sub_words = nc * nc;
+    }
     int max_cf = 1;
     for (const Shell &sh : c->sh) max_cf = std::max(max_cf, sh.ncomp * sh.nprim);
     A.part_words = 160 + 4 * max_cf;

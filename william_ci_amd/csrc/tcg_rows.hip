@@ -1131,6 +1131,41 @@ __device__ __forceinline__ void stored_position(const EriArgs &A, const int f4[4
     for (int x = 0; x < 4; ++x) pp[x] = A.relc[c0 * A.ncell + t4[x] / A.nfuc] * A.nfuc + t4[x] % A.nfuc;
 }
 
+// The components comp, comp + step, comp + 2 step, ... of a block, as digits
+// comp = ca + nA (cb + nB (cc + nC cd)): the digits of the start and of the step
+// are split once, each further component adds the step's digits with carries
+// (no division per component)
+struct CompWalk {
+    int ca, cb, cc, cd;
+    int sa, sb, sc, sd;
+    int nA, nB, nC;
+    __device__ __forceinline__ CompWalk(int comp, int step, int nA_, int nB_, int nC_) : nA(nA_), nB(nB_), nC(nC_) {
+        split(comp, ca, cb, cc, cd);
+        split(step, sa, sb, sc, sd);
+    }
+    __device__ __forceinline__ void split(int v, int &a, int &b, int &c, int &d) const {
+        a = v % nA;
+        v /= nA;
+        b = v % nB;
+        v /= nB;
+        c = v % nC;
+        d = v / nC;
+    }
+    // each digit stays below twice its radix before its carry, so one subtraction suffices
+    __device__ __forceinline__ void next() {
+        ca += sa;
+        int k = ca >= nA;
+        ca -= k ? nA : 0;
+        cb += sb + k;
+        k = cb >= nB;
+        cb -= k ? nB : 0;
+        cc += sc + k;
+        k = cc >= nC;
+        cc -= k ? nC : 0;
+        cd += sd + k;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // contraction and output, one wavefront per shell quartet: the block, then
 // written out (Schwarz passes) or scattered to the 8 permutations translated
@@ -1148,35 +1183,35 @@ __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, 
         for (int i = lane; i < nblk; i += nl) A.blk_out[Q.out + i] = blk[i];
     } else if (A.eri) {
         const size_t nb = A.nbas, n1 = A.n1;
-        // nl is a multiple of 8, so a lane keeps one permutation ip for all its tasks;
-        // its stored positions are those of component (0,0,0,0) plus the components'
-        // offsets (a shell lies inside one cell, so the translation is the same for
-        // all its components)
-        const int ip = lane & 7;
+        // a lane takes the components lane, lane + nl, ...: the element screen once per
+        // component, then its stored position under each of the 8 permutations, that
+        // of component (0,0,0,0) plus the component's offsets (a shell lies inside one
+        // cell, so the translation is the same for all its components)
+        constexpr int perm[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                    {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
         const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
-        int pb[4], px[4];
-        stored_position(A, f0, ip, pb);
+        int pb[8][4];
 #pragma unroll
-        for (int x = 0; x < 4; ++x) px[x] = c_perm8_rows[ip][x];
-        for (int task = lane; task < nblk * 8; task += nl) {
-            const int comp = task >> 3;
-            int t = comp;
-            const int ca = t % nA; t /= nA;
-            const int cb = t % nB; t /= nB;
-            const int cc = t % nC;
-            const int cd = t / nC;
-            const int c4[4] = {ca, cb, cc, cd};
-            if (A.Qao[(size_t)(f0[0] + ca) + nb * (f0[1] + cb)] * A.Qao[(size_t)(f0[2] + cc) + nb * (f0[3] + cd)] <
+        for (int ip = 0; ip < 8; ++ip) stored_position(A, f0, ip, pb[ip]);
+        CompWalk cw(lane, nl, nA, nB, nC);
+        for (int comp = lane; comp < nblk; comp += nl, cw.next()) {
+            const int c4[4] = {cw.ca, cw.cb, cw.cc, cw.cd};
+            if (A.Qao[(size_t)(f0[0] + c4[0]) + nb * (f0[1] + c4[1])] *
+                    A.Qao[(size_t)(f0[2] + c4[2]) + nb * (f0[3] + c4[3])] <
                 kSchwarz)
                 continue;
-            int pp[4];
+            const double v = blk[comp];
 #pragma unroll
-            for (int x = 0; x < 4; ++x) pp[x] = pb[x] + c4[px[x]];
-            if (A.j4hi > 0 && (pp[3] < A.j4lo || pp[3] >= A.j4hi)) continue;   // outside the slab
-            const size_t p3 = (size_t)(pp[3] - A.j4lo);
-            const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * p3))
-                                           : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * p3));
-            A.eri[idx] = blk[comp];
+            for (int ip = 0; ip < 8; ++ip) {
+                int pp[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) pp[x] = pb[ip][x] + c4[perm[ip][x]];
+                if (A.j4hi > 0 && (pp[3] < A.j4lo || pp[3] >= A.j4hi)) continue;   // outside the slab
+                const size_t p3 = (size_t)(pp[3] - A.j4lo);
+                const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * p3))
+                                               : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * p3));
+                A.eri[idx] = v;
+            }
         }
     }
 }
@@ -1309,14 +1344,10 @@ __device__ __forceinline__ void screen_block(const EriArgs &A, const Shell &sA, 
     const int nA = sA.ncomp, nB = sB.ncomp, nC = sC.ncomp, nD = sD.ncomp;
     const int nblk = nA * nB * nC * nD;
     const size_t nb = A.nbas;
-    for (int comp = lane; comp < nblk; comp += nl) {
-        int t = comp;
-        const int ca = t % nA; t /= nA;
-        const int cb = t % nB; t /= nB;
-        const int cc = t % nC;
-        const int cd = t / nC;
-        if (A.Qao[(size_t)(sA.first_ao + ca) + nb * (sB.first_ao + cb)] *
-                A.Qao[(size_t)(sC.first_ao + cc) + nb * (sD.first_ao + cd)] < kSchwarz)
+    CompWalk cw(lane, nl, nA, nB, nC);
+    for (int comp = lane; comp < nblk; comp += nl, cw.next()) {
+        if (A.Qao[(size_t)(sA.first_ao + cw.ca) + nb * (sB.first_ao + cw.cb)] *
+                A.Qao[(size_t)(sC.first_ao + cw.cc) + nb * (sD.first_ao + cw.cd)] < kSchwarz)
             blk[comp] = 0.0;
     }
 }
@@ -1529,17 +1560,18 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                                                      const Surv *__restrict__ surv, const double *__restrict__ rowval,
                                                      const double *__restrict__ realval, int jk_lds,
                                                      const int *__restrict__ list, const unsigned *__restrict__ cnt) {
-    // LDS: [4][max_blk] blocks, [4][4][36] per-wave J/K sub-blocks and the P sub-blocks
-    // they read, [4][part_words] the contraction's staging, then J[nfuc*nbas],
-    // K[nfuc*nbas] when jk_lds
+    // LDS: [4][max_blk] blocks, [4][4][sub_words] per-wave J/K sub-blocks and the P
+    // sub-blocks they read, [4][part_words] the contraction's staging, then
+    // J[nfuc*nbas], K[nfuc*nbas] when jk_lds
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nf = A.nfuc, nb = A.nbas;
     const size_t njk = (size_t)nf * nb;
     double *blk = lds + (size_t)wv * A.max_blk;
-    double *Jw = lds + 4 * (size_t)A.max_blk + 144 * wv, *Kw = Jw + 36, *Pj = Jw + 72, *Pk = Jw + 108;
-    double *part = lds + 4 * (size_t)A.max_blk + 4 * 144 + (size_t)A.part_words * wv;
-    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 4 * 144 + 4 * (size_t)A.part_words : A.Jacc;
+    const int sw = A.sub_words;
+    double *Jw = lds + 4 * (size_t)A.max_blk + 4 * sw * wv, *Kw = Jw + sw, *Pj = Jw + 2 * sw, *Pk = Jw + 3 * sw;
+    double *part = lds + 4 * (size_t)A.max_blk + 16 * (size_t)sw + (size_t)A.part_words * wv;
+    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 16 * (size_t)sw + 4 * (size_t)A.part_words : A.Jacc;
     double *Kl = jk_lds ? Jl + njk : A.Kacc;
     if (jk_lds) {
         for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
@@ -1570,30 +1602,31 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         // (staged in LDS first); the sub-blocks then go out with one atomic per entry.
         // A component's stored position is that of component (0,0,0,0), pb, plus its
         // offsets (a shell lies inside one cell).
+        // (the permutation loop unrolled: each permutation's index selections are fixed;
+        // the lane's components walk by digits, split once per quartet)
+        constexpr int perm[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                    {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
         const int ncmp[4] = {nA, nB, nC, nD};
         const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
+        const CompWalk cw0(lane, 64, nA, nB, nC);
+#pragma unroll
         for (int ip = 0; ip < 8; ++ip) {
-            const int x0 = c_perm8_rows[ip][0], x1 = c_perm8_rows[ip][1], x2 = c_perm8_rows[ip][2],
-                      x3 = c_perm8_rows[ip][3];
+            const int x0 = perm[ip][0], x1 = perm[ip][1], x2 = perm[ip][2], x3 = perm[ip][3];
             const int n0 = ncmp[x0], n1 = ncmp[x1], n2 = ncmp[x2], n3 = ncmp[x3];
             int pb[4];
             stored_position(A, f0, ip, pb);
-            if (lane < 36) {
-                Jw[lane] = 0.0;
-                Kw[lane] = 0.0;
-            }
-            if (lane < n2 * n3) Pj[lane] = A.Pden[(pb[2] + lane / n3) + (size_t)nb * (pb[3] + lane % n3)];
-            if (lane < n1 * n2) Pk[lane] = A.Pden[(pb[1] + lane / n2) + (size_t)nb * (pb[2] + lane % n2)];
+            for (int e2 = lane; e2 < n0 * n1; e2 += 64) Jw[e2] = 0.0;
+            for (int e2 = lane; e2 < n0 * n3; e2 += 64) Kw[e2] = 0.0;
+            for (int e2 = lane; e2 < n2 * n3; e2 += 64)
+                Pj[e2] = A.Pden[(pb[2] + e2 / n3) + (size_t)nb * (pb[3] + e2 % n3)];
+            for (int e2 = lane; e2 < n1 * n2; e2 += 64)
+                Pk[e2] = A.Pden[(pb[1] + e2 / n2) + (size_t)nb * (pb[2] + e2 % n2)];
             wave_sync();
-            for (int comp = lane; comp < nblk; comp += 64) {
+            CompWalk cw = cw0;
+            for (int comp = lane; comp < nblk; comp += 64, cw.next()) {
                 const double val = blk[comp];
                 if (val == 0.0) continue;
-                int t = comp;
-                const int ca = t % nA; t /= nA;
-                const int cb = t % nB; t /= nB;
-                const int cc = t % nC;
-                const int cd = t / nC;
-                const int c4[4] = {ca, cb, cc, cd};
+                const int c4[4] = {cw.ca, cw.cb, cw.cc, cw.cd};
                 const int i0 = c4[x0], i1 = c4[x1], i2 = c4[x2], i3 = c4[x3];
                 if (stab) {
                     const int pp[4] = {pb[0] + i0, pb[1] + i1, pb[2] + i2, pb[3] + i3};
@@ -1627,7 +1660,7 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
                               const double *rowval, const double *realval, const int *list, const unsigned *cnt,
                               hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 4 * 144 + 4 * (size_t)a.part_words);
+    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 16 * (size_t)a.sub_words + 4 * (size_t)a.part_words);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
     const size_t lds_max = 160 * 1024;
     const bool force_global = jk_force_global();
