@@ -748,7 +748,7 @@ static int prepare(tcg_ctx *c) {
         // Bessel values and Fourier vectors
         const bool grow = PL.npp > c->d_pp.n || PL.njobs > c->d_jobs.n || PL.nvo > c->d_vo.n ||
                           PL.ndco > c->d_dco.n || PL.nherm > c->d_herm.n || PL.niv > c->d_iv.n ||
-                          PL.nvec > c->d_vec.n;
+                          PL.nvec + (size_t)c->ncap + 64 > c->d_vec.n;
         if (grow) HIPCHK(c, hipStreamSynchronize(c->st));      // no kernel still reads the old buffers
         HIPCHK(c, c->d_pp.alloc(PL.npp));
         HIPCHK(c, c->d_jobs.alloc(PL.njobs));
@@ -756,7 +756,9 @@ static int prepare(tcg_ctx *c) {
         HIPCHK(c, c->d_dco.alloc(PL.ndco));
         HIPCHK(c, c->d_herm.alloc(PL.nherm));
         HIPCHK(c, c->d_iv.alloc(PL.niv));
-        HIPCHK(c, c->d_vec.alloc(PL.nvec));
+        // (the row kernel reads up to ncap + 16 entries past a row's first order, so the
+        // vectors end in a padding that keeps those reads inside the allocation)
+        HIPCHK(c, c->d_vec.alloc(PL.nvec + (size_t)c->ncap + 64));
         HIPCHK(c, launch_pair_setup(c->d_pblk.p, (int)pblk.size(), c->d_sh.p, c->d_expo.p, c->d_coef.p, c->d_poff.p,
                                     c->ndp, c->ncap, c->L, c->a, c->d_pp.p, c->d_jobs.p, c->d_vo.p, c->d_dco.p,
                                     c->d_herm.p, c->st));

@@ -285,7 +285,8 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
         const bool valid = ridx >= 0;
         int nm = -1, d = 0x7fffffff;                    // d: direction | rule << 8
         double sgn = 1.0, c1 = 1.0, s1 = 0.0, c4 = 1.0, s4 = 0.0, c64 = 1.0, s64 = 0.0;
-        const double2 *Gp = A.vec, *Hp = A.vec;
+        // byte offsets of the lane's G and H entries (order kq) in the pair vectors
+        unsigned gob = 16u * (unsigned)kq, hob = 16u * (unsigned)kq;
         if (valid) {
             const RowRec R = rows[ridx];
             nm = R.nm;
@@ -297,8 +298,8 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
             s4 = R.s4;
             c64 = R.c64;
             s64 = R.s64;
-            Gp = A.vec + R.gofs;
-            Hp = A.vec + R.hofs;
+            gob = 16u * (unsigned)(R.gofs + kq);
+            hob = 16u * (unsigned)(R.hofs + kq);
         }
         if (A.counters && kq == 0 && valid && hbase == 0) {
             const int rl = d >> 8, dl = d & 0xff;
@@ -361,13 +362,16 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
             for (int j = 0; j < kNT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
             double cn = cn0, sn = sn0, cb = cn0, sb = sn0;
             // Two register sets of G, H, each consumed by one step and refilled for the step
-            // after next.  Rows past nm and orders past the table read in-bounds entries
-            // (order 0 of the row, the zero rows at the table's end) whose products the
-            // selection of F or the zero table make vanish.
-            auto gidx = [&](int n) { return n <= nmd ? n : 0; };
+            // after next.  Orders past a row's nm read the entries that follow its vector
+            // (the pair vectors end in a padding of ncap + 64 entries, so every such read
+            // stays in the allocation), and orders past the table the zero rows at its end;
+            // the selection of F or the zero table make their products vanish.
+            const char *vb = reinterpret_cast<const char *>(A.vec);
+            auto gl = [&](unsigned m) { return *reinterpret_cast<const double2 *>(vb + gob + 16u * m); };
+            auto hl = [&](unsigned m) { return *reinterpret_cast<const double2 *>(vb + hob + 16u * m); };
             auto tidx = [&](int n) { return (size_t)(n < tlast ? n : tlast) * nqp; };
-            double2 gP = Gp[gidx(kq)], hP = Hp[gidx(kq)];
-            double2 gQ = Gp[gidx(4 + kq)], hQ = Hp[gidx(4 + kq)];
+            double2 gP = gl(0), hP = hl(0);
+            double2 gQ = gl(4), hQ = hl(4);
             int s0 = 0, cur = 0;
             auto fval = [&](int m, const double2 &g, const double2 &hh) {
                 const int n = m + kq;
@@ -388,32 +392,45 @@ __global__ __launch_bounds__(256) void k_rows_mfma(EriArgs A, const RowRec *__re
                 const double im = g.x * hh.y - g.y * hh.x;
                 return (n <= nmd) ? re * cn - im * sn : 0.0;
             };
-            // one step of 4 orders m..m+3 (lane order m + kq); c slots active; the table
-            // from the LDS slice
-            auto step = [&](int m, int c, double2 &g, double2 &hh) {
-                const int n = m + kq;
-                const double F = fval(m, g, hh);
-                const double *brow = &tl[cur][(n - s0) * W + r];
+            // two steps of 4 orders each, n0..n0+3 and n0+4..n0+7 (lane orders n0 + kq,
+            // n0 + 4 + kq; both inside the LDS slice: n0 is a multiple of 8 below its
+            // end), c slots active: the table operands of both steps are read from LDS
+            // first, so that their latency runs under the F assembly; then the MFMAs of
+            // the first step and of the second (the order of every accumulator's sum is
+            // that of two single steps); then the G, H loads of the pair after next
+            int n0 = 0;
+            auto step2 = [&](int c) {
+                const double *brow = &tl[cur][(n0 + kq - s0) * W + r];
+                double b0[kNT], b1[kNT];
 #pragma unroll
                 for (int j = 0; j < kNT; ++j)
-                    if (j < c) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(F, brow[toff[j]], acc[j], 0, 0, 0);
-                const int n2 = n + 8;
-                g = Gp[gidx(n2)];
-                hh = Hp[gidx(n2)];
+                    if (j < c) {
+                        b0[j] = brow[toff[j]];
+                        b1[j] = brow[4 * W + toff[j]];
+                    }
+                __builtin_amdgcn_sched_barrier(0);
+                const double F0 = fval(n0, gP, hP);
+                const double F1 = fval(n0 + 4, gQ, hQ);
+#pragma unroll
+                for (int j = 0; j < kNT; ++j)
+                    if (j < c) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(F0, b0[j], acc[j], 0, 0, 0);
+#pragma unroll
+                for (int j = 0; j < kNT; ++j)
+                    if (j < c) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(F1, b1[j], acc[j], 0, 0, 0);
+                gP = gl((unsigned)(n0 + 8));
+                hP = hl((unsigned)(n0 + 8));
+                gQ = gl((unsigned)(n0 + 12));
+                hQ = hl((unsigned)(n0 + 12));
             };
             // stage loop over [n0, s_end): stage c has the c slots with the largest limits
             // active, up to order sl[c-1] (steps in pairs: one extra step past a limit only
             // adds zero products)
-            int n0 = 0;
             auto stages = [&](int s_end) {
 #pragma unroll
                 for (int c = kNT; c >= 1; --c) {
                     const int lend = sl[c - 1];
 #pragma unroll 1
-                    for (; n0 <= lend && n0 < s_end; n0 += 8) {
-                        step(n0, c, gP, hP);
-                        step(n0 + 4, c, gQ, hQ);
-                    }
+                    for (; n0 <= lend && n0 < s_end; n0 += 8) step2(c);
                 }
             };
             {
