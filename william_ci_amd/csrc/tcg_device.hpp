@@ -61,7 +61,8 @@ struct EriArgs {
     // scatter mode
     double *eri;                      // ERI(n1,nbas,nbas,nbas)
     int n1, nbas, nfuc, ncell, physicist;
-    int j4lo, j4hi;                   // j4hi > 0: only last indices in [j4lo, j4hi), stored at j4 - j4lo
+    const int *slab_pos;              // set: a slab build; the cell of a last index -> its position among the
+                                      // slab's cells (-1: not in the slab); stored at nfuc position + local index
     const double *Qao;                // [nbas*nbas] function-pair Schwarz values
     // direct J/K mode
     const double *Pden;               // density P(nbas,nbas), column-major

@@ -2004,6 +2004,42 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     }
     nR = std::min(nR, N);
     const int nslab = (N + nR - 1) / nR;
+    // the slabs' cells: ordered by the distance of the cell from cell 0 (nearest
+    // image), each cell next to its inverse.  A quartet's permutations store into
+    // the relative cells of its shell pairs and their inverses, near cell 0 for the
+    // quartets that survive the screens, so the first slabs gather most quartets and
+    // each quartet reaches few slabs (the slab builds skip the quartets they do not store)
+    std::vector<int> corder;
+    {
+        const int n0 = c->ncl[0], n1 = c->ncl[1], n2 = c->ncl[2];
+        const double h[3] = {c->L[0] / n0, c->L[1] / n1, c->L[2] / n2};
+        auto coord = [&](int R, int d) {
+            const int n = d == 0 ? n0 : d == 1 ? n1 : n2;
+            const int x = d == 0 ? R % n0 : d == 1 ? (R / n0) % n1 : R / (n0 * n1);
+            return 2 * x <= n ? x : x - n;
+        };
+        auto inverse = [&](int R) {
+            const int x = R % n0, y = (R / n0) % n1, z = R / (n0 * n1);
+            return (n0 - x) % n0 + n0 * ((n1 - y) % n1 + n1 * ((n2 - z) % n2));
+        };
+        std::vector<std::pair<double, int>> cls;
+        for (int R = 0; R < N; ++R) {
+            if (inverse(R) < R) continue;
+            double d2 = 0.0;
+            for (int d = 0; d < 3; ++d) d2 += (coord(R, d) * h[d]) * (coord(R, d) * h[d]);
+            cls.push_back({d2, R});
+        }
+        std::sort(cls.begin(), cls.end());
+        for (const auto &cl : cls) {
+            corder.push_back(cl.second);
+            if (inverse(cl.second) != cl.second) corder.push_back(inverse(cl.second));
+        }
+    }
+    std::vector<int> spos((size_t)nslab * N, -1);
+    for (int sl = 0; sl < nslab; ++sl)
+        for (int r = 0; r < std::min(nR, N - sl * nR); ++r) spos[(size_t)sl * N + corder[sl * nR + r]] = r;
+    DevBuf<int> dspos;
+    HIPCHK(c, dspos.upload(spos));
     // tables
     std::vector<double2> TA, TB;
     kmp2_factors(KI, nf, N, TA, TB);
@@ -2013,14 +2049,14 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
             cos2[i + (size_t)n2 * k] = KI.cosm[cells2[i] + (size_t)N * k];
             sin2[i + (size_t)n2 * k] = KI.sinm[cells2[i] + (size_t)N * k];
         }
-    // per slab s: cos/sin of its cells [s nR, s nR + nRs), table [R' + nRs k] at offset s nR N
+    // per slab s: cos/sin of its cells corder[s nR .. s nR + nRs), table [r + nRs k] at offset s nR N
     std::vector<double> cosB((size_t)N * N), sinB((size_t)N * N);
     for (int sl = 0; sl < nslab; ++sl) {
         const int R0 = sl * nR, nRs = std::min(nR, N - R0);
         for (int k = 0; k < N; ++k)
             for (int r = 0; r < nRs; ++r) {
-                cosB[(size_t)R0 * N + r + (size_t)nRs * k] = KI.cosm[(R0 + r) + (size_t)N * k];
-                sinB[(size_t)R0 * N + r + (size_t)nRs * k] = KI.sinm[(R0 + r) + (size_t)N * k];
+                cosB[(size_t)R0 * N + r + (size_t)nRs * k] = KI.cosm[corder[R0 + r] + (size_t)N * k];
+                sinB[(size_t)R0 * N + r + (size_t)nRs * k] = KI.sinm[corder[R0 + r] + (size_t)N * k];
             }
     }
     std::vector<long long> imap(IA);
@@ -2055,7 +2091,8 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     for (int sl = 0; sl < nslab; ++sl) {
         const int R0 = sl * nR, nRs = std::min(nR, N - R0);
         const double ta = now_ms();
-        // ---- the slab: SAO elements with j4 in [nf R0, nf (R0 + nRs)) ----
+        // ---- the slab: SAO elements whose last index lies in the slab's cells, at
+        // j4 = d + nf r for the cell's position r in the slab ----
         HIPCHK(c, launch_fill(slab.p, (size_t)nf * nb * nb * ((size_t)nf * nRs), 0.0, st));
         HIPCHK(c, hipMemsetAsync(c->d_cnt.p, 0, 8 * sizeof(unsigned long long), st));
         EriArgs A = base_args(c);
@@ -2071,8 +2108,7 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
         A.eri = slab.p;
         A.n1 = nf;
         A.physicist = 0;
-        A.j4lo = nf * R0;
-        A.j4hi = nf * (R0 + nRs);
+        A.slab_pos = dspos.p + (size_t)sl * N;
         int rc = run_quartets(c, A);
         if (rc) return rc;
         HIPCHK(c, hipStreamSynchronize(st));

@@ -76,20 +76,16 @@ __device__ __forceinline__ int sp_form(const EriArgs &A, const Quartet &Q, int q
     return P <= 3 ? sp_list(P, lA + lB) : -1;
 }
 
-// A slab build (j4hi > 0) stores only the elements whose last index lies in
-// [j4lo, j4hi), a range of whole cells.  Permutation (p1 p2|p3 p4) of the
-// quartet (A B|C D) stores its block with the first index translated to the unit
-// cell, so its last index lies in cell relc(cell p1, cell p4); the eight
-// permutations pair (p1, p4) as (A,D) (B,D) (A,C) (B,C) (C,B) (D,B) (C,A) (D,A).
-// A quartet none of whose pairs reaches the slab's cells has nothing to store.
+// A slab build (slab_pos set) stores only the elements whose last index lies in
+// one of the slab's cells.  Permutation (p1 p2|p3 p4) of the quartet (A B|C D)
+// stores its block with the first index translated to the unit cell, so its last
+// index lies in cell relc(cell p1, cell p4); the eight permutations pair
+// (p1, p4) as (A,D) (B,D) (A,C) (B,C) (C,B) (D,B) (C,A) (D,A).  A quartet none of
+// whose pairs reaches the slab's cells has nothing to store.
 __device__ __forceinline__ bool slab_hit(const EriArgs &A, const Quartet &Q) {
-    if (A.j4hi <= 0) return true;
-    const int r0 = A.j4lo / A.nfuc, r1 = A.j4hi / A.nfuc;
+    if (!A.slab_pos) return true;
     const int cA = A.sh[Q.s[0]].cell, cB = A.sh[Q.s[1]].cell, cC = A.sh[Q.s[2]].cell, cD = A.sh[Q.s[3]].cell;
-    auto in = [&](int x, int y) {
-        const int r = A.relc[x * A.ncell + y];
-        return r >= r0 && r < r1;
-    };
+    auto in = [&](int x, int y) { return A.slab_pos[A.relc[x * A.ncell + y]] >= 0; };
     return in(cA, cD) || in(cB, cD) || in(cA, cC) || in(cB, cC) || in(cC, cB) || in(cD, cB) || in(cC, cA) ||
            in(cD, cA);
 }
@@ -1208,8 +1204,16 @@ __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, 
                                     {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
         const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
         int pb[8][4];
+        // the last index's base in the output: its cell's position in the slab (-1:
+        // outside it; a shell lies inside one cell, so this holds for all its components)
+        int p3b[8];
 #pragma unroll
-        for (int ip = 0; ip < 8; ++ip) stored_position(A, f0, ip, pb[ip]);
+        for (int ip = 0; ip < 8; ++ip) {
+            stored_position(A, f0, ip, pb[ip]);
+            const int cl = pb[ip][3] / A.nfuc, off = pb[ip][3] - cl * A.nfuc;
+            const int lc = A.slab_pos ? A.slab_pos[cl] : cl;
+            p3b[ip] = lc >= 0 ? lc * A.nfuc + off : -1;
+        }
         CompWalk cw(lane, nl, nA, nB, nC);
         for (int comp = lane; comp < nblk; comp += nl, cw.next()) {
             const int c4[4] = {cw.ca, cw.cb, cw.cc, cw.cd};
@@ -1223,8 +1227,8 @@ __device__ __forceinline__ void write_block(const EriArgs &A, const Quartet &Q, 
                 int pp[4];
 #pragma unroll
                 for (int x = 0; x < 4; ++x) pp[x] = pb[ip][x] + c4[perm[ip][x]];
-                if (A.j4hi > 0 && (pp[3] < A.j4lo || pp[3] >= A.j4hi)) continue;   // outside the slab
-                const size_t p3 = (size_t)(pp[3] - A.j4lo);
+                if (p3b[ip] < 0) continue;                                   // outside the slab
+                const size_t p3 = (size_t)(p3b[ip] + c4[perm[ip][3]]);
                 const size_t idx = A.physicist ? pp[0] + n1 * (pp[2] + nb * (pp[1] + nb * p3))
                                                : pp[0] + n1 * (pp[1] + nb * (pp[2] + nb * p3));
                 A.eri[idx] = v;
