@@ -241,6 +241,7 @@ def _df_leg(s, eng, S, T, V, res, mp2):
     md = d.kmp2(rd.mo_coef, rd.mo_energies, rd.occupied)
     t3 = time.perf_counter()
     on_exact = d.kmp2(res.mo_coef, res.mo_energies, res.occupied)
+    d.kmp2(res.mo_coef, res.mo_energies, res.occupied, fp32=True)      # first call: the FP32 path's buffers
     fp32 = d.kmp2(res.mo_coef, res.mo_energies, res.occupied, fp32=True)
     info, times = d.info(), d.times()
     d.close()

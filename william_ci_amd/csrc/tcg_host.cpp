@@ -565,6 +565,20 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
     return 0;
 }
 
+// TCG_PROFILE set: the marks of one call, ms since the previous mark, to stderr
+static void print_marks(tcg_ctx *c, const char *what, double t0) {
+    if (!c->prof) return;
+    c->mark("end");
+    double prev = t0;
+    fprintf(stderr, "%s profile (ms since previous mark):", what);
+    for (auto &m : c->marks) {
+        fprintf(stderr, " [%s %.3f]", m.first, m.second - prev);
+        prev = m.second;
+    }
+    fprintf(stderr, "\n");
+    c->marks.clear();
+}
+
 // Sum of the row MFMA launches' event times of this build (after its final sync)
 static int read_row_times(tcg_ctx *c) {
     c->rows_ms = 0.0;
@@ -1421,17 +1435,7 @@ int tcg_eri_build(tcg_ctx *c, int sao, int physicist, double *eri) {
     c->times[2] = t3 - t2;
     c->times[3] = t4 - t3;
     c->times[4] = now_ms() - t0;
-    if (c->prof) {
-        c->mark("end");
-        double prev = t0;
-        fprintf(stderr, "tcg_eri_build profile (ms since previous mark):");
-        for (auto &m : c->marks) {
-            fprintf(stderr, " [%s %.3f]", m.first, m.second - prev);
-            prev = m.second;
-        }
-        fprintf(stderr, "\n");
-        c->marks.clear();
-    }
+    print_marks(c, "tcg_eri_build", t0);
     return TCG_OK;
 }
 
@@ -1487,6 +1491,10 @@ int tcg_direct_init(tcg_ctx *c) {
     if (rc) return rc;
     c->direct_ready = true;
     c->direct_learned = false;
+    c->d_q4dl.release();                 // the learned list of an earlier tcg_direct_init_learned
+    c->d_qrdl.release();
+    c->d_lbdl.release();
+    c->nq4dl = 0;
     for (long long &v : c->screen_cum) v = 0;
     return TCG_OK;
 }
@@ -1934,7 +1942,7 @@ int tcg_kmp2(tcg_ctx *c, int nbas, const double *coeff_sao, const double *e, con
 }
 
 // k-point MP2 without the stored tensor: the SAO integrals are built slab by slab
-// of the last index (j4 = d + nf R4 over a range of cells R4), each slab through
+// of the last index (j4 = d + nf R4 over a set of cells R4), each slab through
 // the first two fused transforms (R3 -> k3 with the occupied index, R4 -> k4 with
 // the virtual one, the latter summed over the slabs), then the last two transforms
 // and the sums once.  Linear in j4, so the result is tcg_kmp2's on the stored tensor.
@@ -1989,18 +1997,35 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     c->d_rowval.release();
     c->d_realval.release();
     HIPCHK(c, hipDeviceSynchronize());
-    // cells per slab: the caller's, or what the free device memory holds next to the
-    // work buffers of the transforms and the row pipeline of a slab build (its row
-    // budget, a quarter more for survivors and sort space)
+    c->prof = getenv("TCG_PROFILE") != nullptr;
+    c->marks.clear();
+    c->mark("release scratch");
+    // The slab builds run the row pipeline in chunks of a smaller row budget (8 GB:
+    // more chunks, each still millions of rows), which leaves the memory to the slabs
+    struct BudgetScope {
+        tcg_ctx *c;
+        double saved;
+        ~BudgetScope() { c->row_budget = saved; }
+    } budget_scope{c, c->row_budget};
+    if (!c->row_budget_set) c->row_budget = std::min(c->row_budget, 8e9);
+    // cells per slab: the caller's, or what the free device memory holds next to Y_B
+    // (summed over the slabs) and the row pipeline of a slab build (its row budget, a
+    // quarter more for survivors and sort space); per cell the slab and Y_A.  Y_C
+    // takes the slab's memory after the last slab.
     const size_t nYB = (size_t)IB * nv * N, nYC = (size_t)nf * nv * N * o_hi * N * nv * N;
     const size_t per_cell = (size_t)nf * nb * nb * nf * sizeof(double) + (size_t)IA * o_hi * N * nf * 2 * sizeof(double);
     int nR = slab_cells;
-    if (nR == 0) {
+    {
         size_t fr = 0, tot = 0;
         HIPCHK(c, hipMemGetInfo(&fr, &tot));
-        const size_t fixed = 2 * sizeof(double) * (nYB + nYC) + (size_t)(1.25 * c->row_budget) + ((size_t)1 << 32);
-        const size_t avail = fr > fixed ? fr - fixed : 0;
-        nR = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, avail / per_cell));
+        const size_t fixed = 2 * sizeof(double) * nYB + (size_t)(1.25 * c->row_budget) + ((size_t)1 << 32);
+        if (fr < 2 * sizeof(double) * (nYB + nYC) + ((size_t)1 << 30))
+            return c->fail(TCG_E_NOMEM, "tcg_kmp2_direct: the transforms need %.1f GB, %.1f GB free",
+                           2e-9 * sizeof(double) * (nYB + nYC), 1e-9 * fr);
+        if (nR == 0) {
+            const size_t avail = fr > fixed ? fr - fixed : 0;
+            nR = (int)std::max<size_t>(1, std::min<size_t>((size_t)N, avail / per_cell));
+        }
     }
     nR = std::min(nR, N);
     const int nslab = (N + nR - 1) / nR;
@@ -2077,16 +2102,18 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     HIPCHK(c, upload_async(c, c->km_sin2, sin2));
     HIPCHK(c, upload_async(c, c->km_imap, imap));
     HIPCHK(c, hipStreamSynchronize(st));
+    c->mark("tables");
     // work buffers: the slab, Y_A of one slab, Y_B summed over the slabs (then Y_C)
     DevBuf<double> slab, Ar, Ai;
     const size_t nslab_el = (size_t)nf * nb * nb * ((size_t)nf * nR);
     const size_t nYA = (size_t)IA * o_hi * N * ((size_t)nf * nR);
     HIPCHK(c, slab.alloc(nslab_el));
-    HIPCHK(c, Ar.alloc(std::max(nYA, nYC)));
-    HIPCHK(c, Ai.alloc(std::max(nYA, nYC)));
+    HIPCHK(c, Ar.alloc(nYA));
+    HIPCHK(c, Ai.alloc(nYA));
     auto &Br = c->km_Br, &Bi = c->km_Bi;
     HIPCHK(c, Br.alloc(nYB));
     HIPCHK(c, Bi.alloc(nYB));
+    c->mark("slab and Y buffers");
     double t_int = 0.0, t_tr = 0.0;
     for (int sl = 0; sl < nslab; ++sl) {
         const int R0 = sl * nR, nRs = std::min(nR, N - R0);
@@ -2112,6 +2139,7 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
         int rc = run_quartets(c, A);
         if (rc) return rc;
         HIPCHK(c, hipStreamSynchronize(st));
+        c->mark("slab integrals");
         const double tb = now_ms();
         t_int += tb - ta;
         // ---- A: (c R3) -> (r k3), rows (a b R2 in cells2), outer j4 of the slab ----
@@ -2154,9 +2182,20 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
         d.accum = sl > 0;
         HIPCHK(c, launch_dft_mo(d, st));
         HIPCHK(c, hipStreamSynchronize(st));
+        c->mark("slab transforms");
         t_tr += now_ms() - tb;
     }
-    slab.release();
+    // Y_C in the slab's memory when it holds both parts (a fresh allocation of that
+    // size right after freeing the slab can take seconds), else in Y_A's buffers
+    double *Cr = slab.p, *Ci = slab.p + nYC;
+    if (slab.n < 2 * nYC) {
+        slab.release();
+        HIPCHK(c, Ar.alloc(nYC));
+        HIPCHK(c, Ai.alloc(nYC));
+        Cr = Ar.p;
+        Ci = Ai.p;
+    }
+    c->mark("Y_C buffers");
     const double tc = now_ms();
     // ---- C: (b R2) -> (q k2) / N over cells2; D: a -> p ----
     DftMoArgs d{};
@@ -2176,12 +2215,12 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     d.sgn = 1.0;
     d.scale = 1.0 / N;
     d.T = c->km_TB.p;
-    d.Yr = Ar.p;
-    d.Yi = Ai.p;
+    d.Yr = Cr;
+    d.Yi = Ci;
     HIPCHK(c, launch_dft_mo(d, st));
     const size_t nkmo = (size_t)o_hi * nv * o_hi * nv * N * N * N;
     HIPCHK(c, c->km_kmo.alloc(nkmo));
-    HIPCHK(c, launch_kmo_last(Ar.p, Ai.p, c->km_Ck.p, nf, N, c->ncl[0], c->ncl[1], c->ncl[2], o_hi, v_lo,
+    HIPCHK(c, launch_kmo_last(Cr, Ci, c->km_Ck.p, nf, N, c->ncl[0], c->ncl[1], c->ncl[2], o_hi, v_lo,
                               c->km_kmo.p, st));
     {
         const int rc = kmp2_sums(c, o_hi, v_lo, out);
@@ -2192,6 +2231,7 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     out[5] = t_int;
     out[6] = t_tr;
     out[7] = nslab;
+    print_marks(c, "tcg_kmp2_direct", t0);
     return TCG_OK;
 }
 
