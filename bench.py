@@ -29,9 +29,10 @@ time, energy differences to the plain path, speed-ups); ``df_mp2`` runs the
 density-fitted path (41 even-tempered auxiliary functions per H): df_init,
 DF-RHF, DF-kMP2, and the DF correlation energy on the exact orbitals against the
 exact k-MP2 (and in FP32 on the matrix cores); ``learned_direct`` runs the
-integral-direct ERI + RHF + MP2 of the Tori2D 8x8 H2 lattice plain, plain on the
-converged (128-point) quadrature, and learned (per-quartet orders, incremental
-Fock builds with density screening).
+integral-direct ERI + RHF + MP2 of the Tori2D 8x8 H2 lattice (or, with
+--direct-workload h2_4x4x4_d, of the Tori3D 4x4x4 H2 + d lattice) plain, plain on
+the converged (128-point) quadrature, and learned (per-quartet orders,
+incremental Fock builds with density screening).
 """
 from __future__ import annotations
 
@@ -159,10 +160,15 @@ def _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total, ms_step_pla
             "eri_rhf_mp2_wall_s": total, "speedup_eri_rhf_mp2": plain_total / total}
 
 
+DIRECT_WORKLOADS = {"h2_8x8": "Tori2D 8x8 H2", "h2_4x4x4_d": "Tori3D 4x4x4 H2 + d"}
+
+
 def _learned_direct_leg(workload: str = "h2_8x8"):
-    """Integral-direct ERI + RHF + MP2 on the Tori2D 8x8 lattice of H2 (640
-    functions), the path where the integrals are recomputed every SCF
-    iteration: direct RHF then the direct k-MP2 (no stored tensor), three ways:
+    """Integral-direct ERI + RHF + MP2 on a lattice of H2 (default the Tori2D
+    8x8 lattice, 640 functions; --direct-workload h2_4x4x4_d: the Tori3D 4x4x4
+    lattice with d functions, 1408 functions, a few minutes), the path where the
+    integrals are recomputed every SCF iteration: direct RHF then the direct
+    k-MP2 (no stored tensor), three ways:
       plain      the reference's rule after its start-up self-check (the 64-point
                  table here), every quartet at every iteration (direct_JK);
       plain_128  the same on the 128-point table (the converged quadrature the
@@ -191,14 +197,20 @@ def _learned_direct_leg(workload: str = "h2_8x8"):
         return {"rhf_energy": r.energy, "iterations": r.iterations, "rhf_s": t1 - t0, "mp2_ec": m["ec"],
                 "mp2_s": t2 - t1, "e_total": r.energy + m["ec"], "wall_s": t2 - t0,
                 "quartets_evaluated": info["kept_total"], "quartets_listed": info["listed_total"]}
-    run(lambda: eng.direct_init(), "direct")                        # warm-up (first-call buffers)
-    plain = run(lambda: eng.direct_init(), "direct")
     counts = {}
 
     def learned_setup():
         counts["c"] = learned.direct_init_learned(eng, model)
         eng.set_direct_screen(1e-14, 50, False)
-    lrn = run(learned_setup, "direct_incremental")
+    if workload == "h2_8x8":
+        run(lambda: eng.direct_init(), "direct")                    # warm-up (first-call buffers)
+        plain = run(lambda: eng.direct_init(), "direct")
+        lrn = run(learned_setup, "direct_incremental")
+    else:
+        # (minutes per path: no warm-up; the learned path runs first and carries the
+        # first-call allocations)
+        lrn = run(learned_setup, "direct_incremental")
+        plain = run(lambda: eng.direct_init(), "direct")
     quad = eng.quadrature()
     eng.set_quadrature(False, 128)
     p128 = run(lambda: eng.direct_init(), "direct")
@@ -206,7 +218,7 @@ def _learned_direct_leg(workload: str = "h2_8x8"):
     c = counts["c"]
     nord = len(model.orders)
     total = int(c.sum())
-    return {"workload": f"{workload} (Tori2D 8x8 H2, {s.nbas} functions)",
+    return {"workload": f"{workload} ({DIRECT_WORKLOADS[workload]}, {s.nbas} functions)",
             "self_check": sc, "plain_rule": f"{quad[1]}-point {'fast' if quad[0] else 'table'}",
             "plain": plain, "plain_128": p128, "learned": lrn,
             "learned_orders": {"plain_rule": int(c[nord]), **{str(o): int(v) for o, v in zip(model.orders, c[:nord])}},
@@ -271,7 +283,9 @@ def main():
                     help="keep the 32-point fast rule (skip the start-up fast_quad self-check)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-order leg")
     ap.add_argument("--no-df", action="store_true", help="skip the density-fitted MP2 leg")
-    ap.add_argument("--no-direct", action="store_true", help="skip the learned integral-direct leg (Tori2D 8x8)")
+    ap.add_argument("--no-direct", action="store_true", help="skip the learned integral-direct leg")
+    ap.add_argument("--direct-workload", default="h2_8x8", choices=sorted(DIRECT_WORKLOADS),
+                    help="lattice of the integral-direct leg (h2_4x4x4_d: a few minutes)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -346,7 +360,7 @@ def main():
                                                              ms_step_plain)
     df_part = None if args.no_df else _df_leg(s, eng, S, T, V, res, mp2)
     eng.close()
-    direct_part = None if args.no_direct else _learned_direct_leg()
+    direct_part = None if args.no_direct else _learned_direct_leg(args.direct_workload)
 
     nb = s.nbas
     unique = nb ** 4 / 8
