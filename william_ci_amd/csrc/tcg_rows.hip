@@ -543,11 +543,24 @@ __device__ __forceinline__ void wave_sync() {
 //   I = sum_{n,m} (-1)^n E_n E'_m g_{n+m},
 // taken as sum_n (-1)^n E_n h_n with h_n = sum_m E'_m g_{n+m} (each h_n
 // shared by the bra combinations of one ket combination).
+// sqrt(D) by the hardware reciprocal square root estimate of pq + (p+q) t^2
+// refined by two Newton steps y <- y (3 - x y^2) / 2, each of which squares the
+// relative error (about 1 ulp after the second); D = y^2.  This replaces a
+// division and a square root per node and direction.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
 __device__ __forceinline__ void real_gt(double p, double q, double t2, double Rr, int NN, double gt[9]) {
-    const double D2 = 1.0 / (p * q + (p + q) * t2);
+    const double y = rsqrt_nr(p * q + (p + q) * t2);
+    const double D2 = y * y;
     const double w = p * q * t2 * D2;
-    const double rs = 3.14159265358979323846 * sqrt(D2);
+    const double rs = 3.14159265358979323846 * y;
     gt[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
     double gm1 = 0.0, gc = gt[0];
 #pragma unroll
@@ -900,9 +913,10 @@ template <int P, int NBP>
 __device__ __forceinline__ void real_factors(double p, double q, double t2, double Rr, const double *__restrict__ Eb,
                                              const double *__restrict__ Ek, double out[1 << P]) {
     constexpr int NKP = P - NBP, kE = 2 * kMaxL + 1;
-    const double D2 = 1.0 / (p * q + (p + q) * t2);
+    const double y = rsqrt_nr(p * q + (p + q) * t2);
+    const double D2 = y * y;
     const double w = p * q * t2 * D2;
-    const double rs = 3.14159265358979323846 * sqrt(D2);
+    const double rs = 3.14159265358979323846 * y;
     double g[P + 1];
     g[0] = (Rr == 0.0) ? rs : rs * exp(-w * Rr * Rr);
     {
