@@ -1425,7 +1425,8 @@ __device__ __forceinline__ void jk_block(const EriArgs &A, const Shell &sA, cons
 // The same for J and K in global memory, by nl lanes of one quartet: one
 // permutation at a time, the counted elements add into the (n0 x n1) J and
 // (n0 x n3) K sub-blocks the permutation maps them to (w[0..35] and w[36..71],
-// LDS), reading the (n2 x n3) and (n1 x n2) density blocks staged in w[72..143];
+// LDS), reading the (n2 x n3) and (n1 x n2) density blocks staged in w[72..143]
+// (and the transposed (n2 x n3) block in w[144..179]);
 // each sub-block then leaves with one global atomic per entry instead of one per
 // element.  Positions: those of component (0,0,0,0), pb, plus the offsets.
 __device__ __forceinline__ void jk_block_sub(const EriArgs &A, const Shell &sA, const Shell &sB, const Shell &sC,
@@ -1436,17 +1437,24 @@ __device__ __forceinline__ void jk_block_sub(const EriArgs &A, const Shell &sA, 
     const int nf = A.nfuc, nb = A.nbas;
     const int ncmp[4] = {nA, nB, nC, nD};
     const int f0[4] = {sA.first_ao, sB.first_ao, sC.first_ao, sD.first_ao};
-    double *Jw = w, *Kw = w + 36, *Pj = w + 72, *Pk = w + 108;
+    double *Jw = w, *Kw = w + 36, *Pj = w + 72, *Pk = w + 108, *Pjt = w + 144;
     for (int ip = 0; ip < 8; ++ip) {
         const int x0 = c_perm8_rows[ip][0], x1 = c_perm8_rows[ip][1], x2 = c_perm8_rows[ip][2],
                   x3 = c_perm8_rows[ip][3];
         const int n0 = ncmp[x0], n1 = ncmp[x1], n2 = ncmp[x2], n3 = ncmp[x3];
+        // J terms of permutations ip and ip + 2 (ip & 2 == 0) meet in one J element (see
+        // k_contract_jk): without stabilizers the first adds both, the second none
+        const bool jpair = !stab && (ip & 2) == 0, jskip = !stab && (ip & 2) != 0;
         int pb[4];
         stored_position(A, f0, ip, pb);
         for (int e = lane; e < 36; e += nl) {
             Jw[e] = 0.0;
             Kw[e] = 0.0;
-            if (e < n2 * n3) Pj[e] = A.Pden[(pb[2] + e / n3) + (size_t)nb * (pb[3] + e % n3)];
+            if (e < n2 * n3) {
+                const int a2 = pb[2] + e / n3, a3 = pb[3] + e % n3;
+                Pj[e] = A.Pden[a2 + (size_t)nb * a3];
+                Pjt[e] = A.Pden[a3 + (size_t)nb * a2];
+            }
             if (e < n1 * n2) Pk[e] = A.Pden[(pb[1] + e / n2) + (size_t)nb * (pb[2] + e % n2)];
         }
         wave_sync();
@@ -1464,14 +1472,18 @@ __device__ __forceinline__ void jk_block_sub(const EriArgs &A, const Shell &sA, 
                 const int pp[4] = {pb[0] + i0, pb[1] + i1, pb[2] + i2, pb[3] + i3};
                 if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
             }
-            atomicAdd(&Jw[i0 * n1 + i1], val * Pj[i2 * n3 + i3]);
+            if (jpair)
+                atomicAdd(&Jw[i0 * n1 + i1], val * (Pj[i2 * n3 + i3] + Pjt[i2 * n3 + i3]));
+            else if (!jskip)
+                atomicAdd(&Jw[i0 * n1 + i1], val * Pj[i2 * n3 + i3]);
             atomicAdd(&Kw[i0 * n3 + i3], -0.5 * val * Pk[i1 * n2 + i2]);
         }
         wave_sync();
-        for (int e = lane; e < n0 * n1; e += nl) {
-            const double v = Jw[e];
-            if (v != 0.0) atomicAdd(&Jg[(pb[0] + e / n1) + (size_t)nf * (pb[1] + e % n1)], v);
-        }
+        if (!jskip)
+            for (int e = lane; e < n0 * n1; e += nl) {
+                const double v = Jw[e];
+                if (v != 0.0) atomicAdd(&Jg[(pb[0] + e / n1) + (size_t)nf * (pb[1] + e % n1)], v);
+            }
         for (int e = lane; e < n0 * n3; e += nl) {
             const double v = Kw[e];
             if (v != 0.0) atomicAdd(&Kg[(pb[0] + e / n3) + (size_t)nf * (pb[3] + e % n3)], v);
@@ -1492,7 +1504,7 @@ __global__ __launch_bounds__(256) void k_contract_sp(EriArgs A, const int *__res
                                                      int jk_lds) {
     constexpr int NG = 256 / GL;
     __shared__ double lds[NG * pow3(P)];
-    __shared__ double jkw[JK ? NG : 1][JK ? 144 : 1];   // per-group J/K sub-blocks (global J/K)
+    __shared__ double jkw[JK ? NG : 1][JK ? 180 : 1];   // per-group J/K sub-blocks (global J/K)
     extern __shared__ double jkl[];
     const int grp = threadIdx.x / GL, gl = threadIdx.x & (GL - 1);
     double *blk = lds + grp * pow3(P);
@@ -1595,7 +1607,7 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                                                      const Surv *__restrict__ surv, const double *__restrict__ rowval,
                                                      const double *__restrict__ realval, int jk_lds,
                                                      const int *__restrict__ list, const unsigned *__restrict__ cnt) {
-    // LDS: [4][max_blk] blocks, [4][4][sub_words] per-wave J/K sub-blocks and the P
+    // LDS: [4][max_blk] blocks, [4][5][sub_words] per-wave J/K sub-blocks and the P
     // sub-blocks they read, [4][part_words] the contraction's staging, then
     // J[nfuc*nbas], K[nfuc*nbas] when jk_lds
     extern __shared__ double lds[];
@@ -1604,9 +1616,10 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
     const size_t njk = (size_t)nf * nb;
     double *blk = lds + (size_t)wv * A.max_blk;
     const int sw = A.sub_words;
-    double *Jw = lds + 4 * (size_t)A.max_blk + 4 * sw * wv, *Kw = Jw + sw, *Pj = Jw + 2 * sw, *Pk = Jw + 3 * sw;
-    double *part = lds + 4 * (size_t)A.max_blk + 16 * (size_t)sw + (size_t)A.part_words * wv;
-    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 16 * (size_t)sw + 4 * (size_t)A.part_words : A.Jacc;
+    double *Jw = lds + 4 * (size_t)A.max_blk + 5 * sw * wv, *Kw = Jw + sw, *Pj = Jw + 2 * sw, *Pk = Jw + 3 * sw,
+           *Pjt = Jw + 4 * sw;
+    double *part = lds + 4 * (size_t)A.max_blk + 20 * (size_t)sw + (size_t)A.part_words * wv;
+    double *Jl = jk_lds ? lds + 4 * (size_t)A.max_blk + 20 * (size_t)sw + 4 * (size_t)A.part_words : A.Jacc;
     double *Kl = jk_lds ? Jl + njk : A.Kacc;
     if (jk_lds) {
         for (size_t i = threadIdx.x; i < 2 * njk; i += 256) Jl[i] = 0.0;
@@ -1648,12 +1661,24 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
         for (int ip = 0; ip < 8; ++ip) {
             const int x0 = perm[ip][0], x1 = perm[ip][1], x2 = perm[ip][2], x3 = perm[ip][3];
             const int n0 = ncmp[x0], n1 = ncmp[x1], n2 = ncmp[x2], n3 = ncmp[x3];
+            // Permutations ip and ip + 2 (ip & 2 == 0: (ab|cd) and (ab|dc), (ba|cd) and
+            // (ba|dc), ...) store at positions that differ only by swapping the last two
+            // indices, so their J terms go to one J element: E P(p3,p4) + E P(p4,p3).
+            // Without stabilizers every (component, permutation) counts, and the first
+            // of the pair adds both terms (pjt: the transposed density block); with
+            // stabilizers each permutation adds its own, as counted.
+            const bool jpair = !stab && (ip & 2) == 0, jskip = !stab && (ip & 2) != 0;
             int pb[4];
             stored_position(A, f0, ip, pb);
-            for (int e2 = lane; e2 < n0 * n1; e2 += 64) Jw[e2] = 0.0;
+            if (!jskip)
+                for (int e2 = lane; e2 < n0 * n1; e2 += 64) Jw[e2] = 0.0;
             for (int e2 = lane; e2 < n0 * n3; e2 += 64) Kw[e2] = 0.0;
-            for (int e2 = lane; e2 < n2 * n3; e2 += 64)
-                Pj[e2] = A.Pden[(pb[2] + e2 / n3) + (size_t)nb * (pb[3] + e2 % n3)];
+            if (!jskip)
+                for (int e2 = lane; e2 < n2 * n3; e2 += 64) {
+                    const int a2 = pb[2] + e2 / n3, a3 = pb[3] + e2 % n3;
+                    Pj[e2] = A.Pden[a2 + (size_t)nb * a3];
+                    if (jpair) Pjt[e2] = A.Pden[a3 + (size_t)nb * a2];
+                }
             for (int e2 = lane; e2 < n1 * n2; e2 += 64)
                 Pk[e2] = A.Pden[(pb[1] + e2 / n2) + (size_t)nb * (pb[2] + e2 % n2)];
             wave_sync();
@@ -1667,14 +1692,18 @@ __global__ __launch_bounds__(256) void k_contract_jk(EriArgs A, const long long 
                     const int pp[4] = {pb[0] + i0, pb[1] + i1, pb[2] + i2, pb[3] + i3};
                     if (!first_generator(A, sA, sB, sC, sD, comp, ip, pp)) continue;
                 }
-                atomicAdd(&Jw[i0 * n1 + i1], val * Pj[i2 * n3 + i3]);
+                if (jpair)
+                    atomicAdd(&Jw[i0 * n1 + i1], val * (Pj[i2 * n3 + i3] + Pjt[i2 * n3 + i3]));
+                else if (!jskip)
+                    atomicAdd(&Jw[i0 * n1 + i1], val * Pj[i2 * n3 + i3]);
                 atomicAdd(&Kw[i0 * n3 + i3], -0.5 * val * Pk[i1 * n2 + i2]);
             }
             wave_sync();
-            for (int e2 = lane; e2 < n0 * n1; e2 += 64) {
-                const double v = Jw[e2];
-                if (v != 0.0) atomicAdd(&Jl[(pb[0] + e2 / n1) + (size_t)nf * (pb[1] + e2 % n1)], v);
-            }
+            if (!jskip)
+                for (int e2 = lane; e2 < n0 * n1; e2 += 64) {
+                    const double v = Jw[e2];
+                    if (v != 0.0) atomicAdd(&Jl[(pb[0] + e2 / n1) + (size_t)nf * (pb[1] + e2 % n1)], v);
+                }
             for (int e2 = lane; e2 < n0 * n3; e2 += 64) {
                 const double v = Kw[e2];
                 if (v != 0.0) atomicAdd(&Kl[(pb[0] + e2 / n3) + (size_t)nf * (pb[3] + e2 % n3)], v);
@@ -1695,7 +1724,7 @@ hipError_t launch_contract_jk(const EriArgs &a, const long long *soff, const lon
                               const double *rowval, const double *realval, const int *list, const unsigned *cnt,
                               hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 16 * (size_t)a.sub_words + 4 * (size_t)a.part_words);
+    const size_t blk_bytes = sizeof(double) * (4 * (size_t)a.max_blk + 20 * (size_t)a.sub_words + 4 * (size_t)a.part_words);
     const size_t jk_bytes = sizeof(double) * 2 * (size_t)a.nfuc * a.nbas;
     const size_t lds_max = 160 * 1024;
     const bool force_global = jk_force_global();
