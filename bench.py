@@ -147,6 +147,26 @@ def _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total, ms_step_pla
     E_l, _ = learned.eri_build_learned(eng, model, to_host=True)
     spot = learned.spot_check(eng, E_l, n=256)
     del E_l
+    # the plain path at the learned path's accuracy: the 128-point table (the
+    # converged quadrature the network reproduces), ERI + RHF + MP2 timed alike
+    quad = eng.quadrature()
+    eng.set_quadrature(False, 128)
+    for _ in range(max(1, args.warmup)):
+        eng.refresh_basis(coef)
+        eng.eri_build(to_host=False)
+    t5 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.refresh_basis(coef)
+        eng.eri_build(to_host=False)
+    step128 = (time.perf_counter() - t5) / args.steps
+    t6 = time.perf_counter()
+    r128 = scf.rhf_sao_gpu(s, eng, S, T, V)
+    t7 = time.perf_counter()
+    m128 = eng.kmp2(r128.mo_coef, r128.mo_energies, r128.occupied)
+    t8 = time.perf_counter()
+    total128 = step128 + (t7 - t6) + (t8 - t7)
+    eng.set_quadrature(bool(quad[0]), int(quad[1]))
+    eng.refresh_basis(coef)
     return {"orders": list(model.orders), "margin": model.margin,
             "shell_quartets_per_rule": {"plain": int(counts[nord]),
                                         **{str(o): int(c) for o, c in zip(model.orders, counts[:nord])}},
@@ -157,7 +177,11 @@ def _learned_leg(args, s, coef, eng, S, T, V, res, mp2, plain_total, ms_step_pla
             "speedup_eri": ms_step_plain / step,
             "rhf_energy": rl.energy, "rhf_iterations": rl.iterations, "mp2_ec": ml["ec"],
             "dE_rhf": rl.energy - res.energy, "dE_total": (rl.energy + ml["ec"]) - (res.energy + mp2["ec"]),
-            "eri_rhf_mp2_wall_s": total, "speedup_eri_rhf_mp2": plain_total / total}
+            "eri_rhf_mp2_wall_s": total, "speedup_eri_rhf_mp2": plain_total / total,
+            "plain_128": {"ms_per_step": step128 * 1e3, "rhf_energy": r128.energy, "mp2_ec": m128["ec"],
+                          "eri_rhf_mp2_wall_s": total128},
+            "speedup_eri_vs_plain_128": step128 / step, "speedup_eri_rhf_mp2_vs_plain_128": total128 / total,
+            "dE_total_vs_plain_128": (rl.energy + ml["ec"]) - (r128.energy + m128["ec"])}
 
 
 DIRECT_WORKLOADS = {"h2_8x8": "Tori2D 8x8 H2", "h2_4x4x4_d": "Tori3D 4x4x4 H2 + d"}
