@@ -93,7 +93,7 @@ hipError_t launch_enum_quartets(const Shell *sh, const int *relc, const double *
 hipError_t launch_surv(const EriArgs &a, int pass, long long *scount, long long *rcount, const long long *soff,
                        Surv *surv, hipStream_t st);
 hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                            RowRec *rows, unsigned *keys, int *vals, hipStream_t st);
+                            RowRec *rows, unsigned *keys, int *vals, long long nrows, hipStream_t st);
 hipError_t launch_rows_mfma(const EriArgs &a, const RowRec *rows, const int *order, long long nrows, double *rowval,
                             hipStream_t st);
 size_t contract_lds_bytes(const EriArgs &a);

@@ -530,7 +530,7 @@ static int run_chunk(tcg_ctx *c, const EriArgs &A, const long long *so, const lo
         HIPCHK(c, launch_fill_u32(reinterpret_cast<unsigned *>(c->d_vals.p), nr, 0xffffffffu, c->st));
     }
     HIPCHK(c, launch_surv(A, 1, nullptr, nullptr, so, c->d_surv.p, c->st));
-    HIPCHK(c, launch_rows_emit(A, so, ro, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, c->st));
+    HIPCHK(c, launch_rows_emit(A, so, ro, c->d_surv.p, c->d_rows.p, c->d_keys.p, c->d_vals.p, nr, c->st));
     c->mark("  surv+emit launched");
     int rbits = 0;
     while ((1 << rbits) < (int)c->rule_nq.size()) ++rbits;

@@ -176,10 +176,81 @@ __device__ __forceinline__ int row_length(int bN, int kN, float Ab, float Ak, in
     return cut < nm ? cut : nm;
 }
 
+// the last quartet whose first row is at most R: roff ascends, roff[0] = 0 <= R <
+// roff[nq4]; quartets without rows share their offset with the next one, so the
+// quartet found has rows
+__device__ __forceinline__ int row_owner(const long long *__restrict__ roff, int nq4, long long R) {
+    int lo = 0, hi = nq4;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (roff[mid] <= R)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// one thread per row (rows of all quartets, roff[nq4] of them): its quartet by
+// binary search over the row offsets, then its record, key and value
 __global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *__restrict__ soff,
                                                   const long long *__restrict__ roff, const Surv *__restrict__ surv,
                                                   RowRec *__restrict__ rows, unsigned *__restrict__ keys,
                                                   int *__restrict__ vals) {
+    const long long total = roff[A.nq4];
+    for (long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x; row < total;
+         row += (long long)gridDim.x * blockDim.x) {
+        const int qi = row_owner(roff, A.nq4, row);
+        const Quartet &Q = A.q4[qi];
+        int l[4], ncomb;
+        quartet_shape(A, Q, l, ncomb);
+        const long s0 = soff[qi];
+        const long r0 = roff[qi], rl = row - r0;
+        {
+            const int combo = (int)(rl % ncomb);
+            const int d = (int)((rl / ncomb) % A.ndp);
+            const long g = rl / ((long)ncomb * A.ndp);
+            const Surv S = surv[s0 + g];
+            const PrimPair &pb = A.pp[S.pb];
+            const PrimPair &pk = A.pp[S.pk];
+            int t = combo;
+            const int ia = t % (l[0] + 1); t /= (l[0] + 1);
+            const int ib = t % (l[1] + 1); t /= (l[1] + 1);
+            const int ic = t % (l[2] + 1);
+            const int id = t / (l[2] + 1);
+            const int bN = pb.N[d], kN = pk.N[d];
+            RowRec R;
+            R.nm = row_length(bN, kN, pb.A[d], pk.A[d], l[0] + l[1], l[2] + l[3]);
+            R.gofs = pb.voff[d] + (ia * (l[1] + 1) + ib) * (bN + 1);
+            R.hofs = pk.voff[d] + (ic * (l[3] + 1) + id) * (kN + 1);
+            const double ad = (d == 0) ? A.a[0] : (d == 1) ? A.a[1] : A.a[2];
+            const double shd = Q.shift[d];
+            // the phase factors of the row, once per row (the MFMA kernel needs no sincos)
+            const double phi = ad * (pb.cen[d] - (pk.cen[d] + shd));
+            sincos(phi, &R.s1, &R.c1);
+            // e^{4 i phi} by two doublings of e^{i phi} (a few ulps); e^{64 i phi} directly
+            const double c2 = (R.c1 - R.s1) * (R.c1 + R.s1), s2 = 2.0 * R.s1 * R.c1;
+            R.c4 = (c2 - s2) * (c2 + s2);
+            R.s4 = 2.0 * s2 * c2;
+            sincos(64.0 * phi, &R.s64, &R.c64);
+            R.sgn = pow(ad, -(double)(ia + ib + ic + id));
+            const int rule = quartet_rule(A, qi);
+            R.d = d | (rule << 8);
+            rows[r0 + rl] = R;
+            // key (rule, direction, order): the order field holds ncap + 1 (nm_bits), the
+            // direction two bits above it, the rule above both
+            keys[r0 + rl] = ((unsigned)rule << (A.nm_bits + 2)) | ((unsigned)d << A.nm_bits) | (unsigned)R.nm;
+            vals[r0 + rl] = (int)(r0 + rl);
+        }
+    }
+}
+
+// the same with one wavefront per quartet, a lane per row (quartets of many rows:
+// no search, the quartet's data read once per wavefront)
+__global__ __launch_bounds__(256) void k_rows_emit_q(EriArgs A, const long long *__restrict__ soff,
+                                                    const long long *__restrict__ roff, const Surv *__restrict__ surv,
+                                                    RowRec *__restrict__ rows, unsigned *__restrict__ keys,
+                                                    int *__restrict__ vals) {
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);     // four wavefronts, one quartet each
     for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
         const Quartet Q = A.q4[qi];
@@ -227,10 +298,19 @@ __global__ __launch_bounds__(256) void k_rows_emit(EriArgs A, const long long *_
 }
 
 hipError_t launch_rows_emit(const EriArgs &a, const long long *soff, const long long *roff, const Surv *surv,
-                            RowRec *rows, unsigned *keys, int *vals, hipStream_t st) {
-    if (a.nq4 <= 0) return hipSuccess;
-    const int nwg = (a.nq4 + 3) / 4;
-    const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
+                            RowRec *rows, unsigned *keys, int *vals, long long nrows, hipStream_t st) {
+    if (a.nq4 <= 0 || nrows <= 0) return hipSuccess;
+    // nrows: the row count or an upper bound (the kernels stop at roff[nq4]).  Lists
+    // of 32 or more rows per quartet on average keep a wavefront per quartet; shorter
+    // ones (most quartets of s and p shells) a thread per row
+    if (nrows >= 32ll * a.nq4) {
+        const int nwg = (a.nq4 + 3) / 4;
+        const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
+        hipLaunchKernelGGL(k_rows_emit_q, dim3(grid), dim3(256), 0, st, a, soff, roff, surv, rows, keys, vals);
+        return hipGetLastError();
+    }
+    const long long nwg = (nrows + 255) / 256;
+    const int grid = nwg < (1 << 20) ? (int)nwg : (1 << 20);
     hipLaunchKernelGGL(k_rows_emit, dim3(grid), dim3(256), 0, st, a, soff, roff, surv, rows, keys, vals);
     return hipGetLastError();
 }
