@@ -1982,24 +1982,6 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     if (cells2.empty()) cells2.push_back(0);
     const int n2 = (int)cells2.size();
     const long long IA = (long long)nf * nf * n2, IB = IA * o_hi * N;
-    // the screened direct J/K's scratch lists and the row pipeline's buffers are
-    // rebuilt on demand: give their memory to the slabs
-    c->d_q4scr.release();
-    c->d_qrscr.release();
-    c->d_keep.release();
-    c->d_sidx.release();
-    c->d_surv.release();
-    c->d_rows.release();
-    c->d_keys.release();
-    c->d_keys2.release();
-    c->d_vals.release();
-    c->d_order.release();
-    c->d_rowval.release();
-    c->d_realval.release();
-    HIPCHK(c, hipDeviceSynchronize());
-    c->prof = getenv("TCG_PROFILE") != nullptr;
-    c->marks.clear();
-    c->mark("release scratch");
     // The slab builds run the row pipeline in chunks of a smaller row budget (8 GB:
     // more chunks, each still millions of rows), which leaves the memory to the slabs
     struct BudgetScope {
@@ -2014,11 +1996,35 @@ int tcg_kmp2_direct(tcg_ctx *c, int nbas, const double *coeff_sao, const double 
     // takes the slab's memory after the last slab.
     const size_t nYB = (size_t)IB * nv * N, nYC = (size_t)nf * nv * N * o_hi * N * nv * N;
     const size_t per_cell = (size_t)nf * nb * nb * nf * sizeof(double) + (size_t)IA * o_hi * N * nf * 2 * sizeof(double);
+    const size_t fixed = 2 * sizeof(double) * nYB + (size_t)(1.25 * c->row_budget) + ((size_t)1 << 32);
     int nR = slab_cells;
     {
         size_t fr = 0, tot = 0;
         HIPCHK(c, hipMemGetInfo(&fr, &tot));
-        const size_t fixed = 2 * sizeof(double) * nYB + (size_t)(1.25 * c->row_budget) + ((size_t)1 << 32);
+        // the screened direct J/K's scratch lists and the row pipeline's buffers are
+        // rebuilt on demand: their memory goes to the slabs when the slabs' cells do
+        // not fit beside them (they stay otherwise: allocating many GB right after
+        // freeing them can take seconds, here and in the next J/K)
+        const size_t want = (size_t)(slab_cells > 0 ? std::min(slab_cells, N) : N);
+        if (fr < fixed + want * per_cell) {
+            c->d_q4scr.release();
+            c->d_qrscr.release();
+            c->d_keep.release();
+            c->d_sidx.release();
+            c->d_surv.release();
+            c->d_rows.release();
+            c->d_keys.release();
+            c->d_keys2.release();
+            c->d_vals.release();
+            c->d_order.release();
+            c->d_rowval.release();
+            c->d_realval.release();
+            HIPCHK(c, hipDeviceSynchronize());
+            HIPCHK(c, hipMemGetInfo(&fr, &tot));
+        }
+        c->prof = getenv("TCG_PROFILE") != nullptr;
+        c->marks.clear();
+        c->mark("release scratch");
         if (fr < 2 * sizeof(double) * (nYB + nYC) + ((size_t)1 << 30))
             return c->fail(TCG_E_NOMEM, "tcg_kmp2_direct: the transforms need %.1f GB, %.1f GB free",
                            2e-9 * sizeof(double) * (nYB + nYC), 1e-9 * fr);
