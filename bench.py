@@ -199,7 +199,9 @@ def _learned_direct_leg(workload: str = "h2_8x8"):
                  network is trained to reproduce: the plain path at the learned
                  path's accuracy);
       learned    per-quartet learned orders and the incremental Fock build with
-                 density screening at 1e-14."""
+                 density screening at 1e-14 (bound: the Schwarz product);
+    and learned_bound_screen: the learned path with the network's integral bound
+    in the density screening."""
     from william_ci_amd import learned, scf
     from william_ci_amd import system as tsys
     from william_ci_amd.engine import Engine, normalized_coef
@@ -210,7 +212,9 @@ def _learned_direct_leg(workload: str = "h2_8x8"):
     model = learned.Model.load()
     model.install(eng)
 
-    def run(setup, source):
+    def run(setup, source, label):
+        # (a progress line per path on stderr: the 3D lattice's paths take minutes)
+        print(f"bench: direct leg {workload}: {label}", file=sys.stderr, flush=True)
         setup()
         t0 = time.perf_counter()
         r = scf.rhf_sao_gpu(s, eng, S, T, V, source=source)
@@ -226,32 +230,40 @@ def _learned_direct_leg(workload: str = "h2_8x8"):
     def learned_setup():
         counts["c"] = learned.direct_init_learned(eng, model)
         eng.set_direct_screen(1e-14, 50, False)
+    def learned_bound_setup():
+        learned.direct_init_learned(eng, model)
+        eng.set_direct_screen(1e-14, 50, True)
     if workload == "h2_8x8":
-        run(lambda: eng.direct_init(), "direct")                    # warm-up (first-call buffers)
-        plain = run(lambda: eng.direct_init(), "direct")
-        lrn = run(learned_setup, "direct_incremental")
+        run(lambda: eng.direct_init(), "direct", "warm-up")                    # first-call buffers
+        plain = run(lambda: eng.direct_init(), "direct", "plain")
+        lrn = run(learned_setup, "direct_incremental", "learned")
     else:
         # (minutes per path: no warm-up; the learned path runs first and carries the
         # first-call allocations)
-        lrn = run(learned_setup, "direct_incremental")
-        plain = run(lambda: eng.direct_init(), "direct")
+        lrn = run(learned_setup, "direct_incremental", "learned")
+        plain = run(lambda: eng.direct_init(), "direct", "plain")
+    # the same learned path with the network's integral bound in the density screening
+    # (learned Schwarz-style skipping of the incremental builds)
+    lrn_b = run(learned_bound_setup, "direct_incremental", "learned, learned bound")
     quad = eng.quadrature()
     eng.set_quadrature(False, 128)
-    p128 = run(lambda: eng.direct_init(), "direct")
+    p128 = run(lambda: eng.direct_init(), "direct", "plain, 128-point table")
     eng.close()
     c = counts["c"]
     nord = len(model.orders)
     total = int(c.sum())
     return {"workload": f"{workload} ({DIRECT_WORKLOADS[workload]}, {s.nbas} functions)",
             "self_check": sc, "plain_rule": f"{quad[1]}-point {'fast' if quad[0] else 'table'}",
-            "plain": plain, "plain_128": p128, "learned": lrn,
+            "plain": plain, "plain_128": p128, "learned": lrn, "learned_bound_screen": lrn_b,
             "learned_orders": {"plain_rule": int(c[nord]), **{str(o): int(v) for o, v in zip(model.orders, c[:nord])}},
             "fallback_fraction": int(c[nord]) / max(total, 1),
             "speedup_vs_plain": plain["wall_s"] / lrn["wall_s"],
             "speedup_vs_plain_128": p128["wall_s"] / lrn["wall_s"],
             "dE_total_vs_plain": lrn["e_total"] - plain["e_total"],
             "dE_total_vs_plain_128": lrn["e_total"] - p128["e_total"],
-            "dE_total_plain_vs_plain_128": plain["e_total"] - p128["e_total"]}
+            "dE_total_plain_vs_plain_128": plain["e_total"] - p128["e_total"],
+            "speedup_vs_plain_128_learned_bound": p128["wall_s"] / lrn_b["wall_s"],
+            "dE_total_vs_plain_128_learned_bound": lrn_b["e_total"] - p128["e_total"]}
 
 
 def _df_leg(s, eng, S, T, V, res, mp2):
