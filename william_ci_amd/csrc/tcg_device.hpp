@@ -147,8 +147,10 @@ struct OneStArgs {
     const double *expo, *coef;
     double *S, *T;            // (nb, nb) column-major, rows < nf written
 };
-hipError_t launch_one_st(const OneStArgs &g, hipStream_t st);
-hipError_t launch_normalize(const OneStArgs &g, const double *raw, double *out, hipStream_t st);
+// work: 2 nf nb mp^2 (S, T) and nb mp^2 doubles; mp = most primitives of an AO
+hipError_t launch_one_st(const OneStArgs &g, int mp, double *work, hipStream_t st);
+hipError_t launch_normalize(const OneStArgs &g, const double *raw, double *out, int mp, double *work,
+                            hipStream_t st);
 hipError_t launch_translate_fill(double *M, int nf, int nb, int nc, const int *relc, hipStream_t st);
 int mlp_param_count();
 hipError_t launch_features(const EriArgs &a, const int *owner, long long ns, const Surv *surv, const double *expo,

@@ -2333,7 +2333,11 @@ int tcg_one_electron(tcg_ctx *c, int natom, const double *xyz, const int *Z, dou
     OneStArgs g = one_args(c);
     g.S = c->d_S1.p;
     g.T = c->d_T1.p;
-    HIPCHK(c, launch_one_st(g, c->st));
+    int mp = 1;
+    for (int i = 0; i < nb; ++i) mp = std::max(mp, c->poff[i + 1] - c->poff[i]);
+    DevBuf<double> work;
+    HIPCHK(c, work.alloc(2 * (size_t)nf * nb * mp * mp));
+    HIPCHK(c, launch_one_st(g, mp, work.p, c->st));
     HIPCHK(c, launch_translate_fill(c->d_S1.p, nf, nb, c->ncell, c->d_relc.p, c->st));
     HIPCHK(c, launch_translate_fill(c->d_T1.p, nf, nb, c->ncell, c->d_relc.p, c->st));
     std::vector<double> Vv;
@@ -2380,7 +2384,11 @@ int tcg_normalize(tcg_ctx *c, int nbas, const double *center, const int *lxyz, c
     g.lxyz = dl.p;
     g.poff = dp.p;
     g.expo = de.p;
-    HIPCHK(c, launch_normalize(g, draw.p, dout.p, c->st));
+    int mp = 1;
+    for (int i = 0; i < nbas; ++i) mp = std::max(mp, prim_off[i + 1] - prim_off[i]);
+    DevBuf<double> work;
+    HIPCHK(c, work.alloc((size_t)nbas * mp * mp));
+    HIPCHK(c, launch_normalize(g, draw.p, dout.p, mp, work.p, c->st));
     HIPCHK(c, hipMemcpyAsync(coef_out, dout.p, sizeof(double) * np, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return TCG_OK;

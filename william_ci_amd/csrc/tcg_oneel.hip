@@ -349,21 +349,40 @@ __device__ void prim_st(const OneStArgs &g, const int *li, const int *lj, const 
 
 }  // namespace
 
-// S(i, j), T(i, j) for i < nfuc, every j (column-major, leading dimension nbas)
-__global__ __launch_bounds__(256) void k_one_st(OneStArgs g) {
+// S(i, j), T(i, j) for i < nfuc, every j (column-major, leading dimension nbas), in
+// two launches: one thread per primitive pair (p, q) of an AO pair (i, j) (every
+// prim_st runs a Bessel recurrence of up to ~1000 terms on long tori, so a thread
+// per AO pair left most of the GPU idle), then per AO pair the contraction sum in
+// the order p, then q.  w: slot ((i + nf j) mp + p') mp + q' holds (S, T) of the
+// pair's primitives p', q' (their positions in their AOs; mp = most primitives of
+// an AO).
+__global__ __launch_bounds__(256) void k_one_st_prim(OneStArgs g, int mp, double *__restrict__ w) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)g.nf * g.nb * mp * mp) return;
+    const int qq = (int)(idx % mp), pp = (int)((idx / mp) % mp);
+    const long long ij = idx / ((long long)mp * mp);
+    const int i = (int)(ij % g.nf), j = (int)(ij / g.nf);
+    const int p = g.poff[i] + pp, q = g.poff[j] + qq;
+    if (p >= g.poff[i + 1] || q >= g.poff[j + 1]) return;
+    double s, t;
+    prim_st(g, g.lxyz + 3 * i, g.lxyz + 3 * j, g.center + 3 * i, g.center + 3 * j, g.expo[p], g.expo[q], true, &s,
+            &t);
+    w[2 * idx] = s;
+    w[2 * idx + 1] = t;
+}
+
+__global__ __launch_bounds__(256) void k_one_st(OneStArgs g, int mp, const double *__restrict__ w) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)g.nf * g.nb) return;
     const int i = (int)(idx % g.nf), j = (int)(idx / g.nf);
-    const int *li = g.lxyz + 3 * i, *lj = g.lxyz + 3 * j;
-    const double *A = g.center + 3 * i, *B = g.center + 3 * j;
+    const double *wij = w + 2 * (size_t)idx * mp * mp;
     double S = 0.0, T = 0.0;
     for (int p = g.poff[i]; p < g.poff[i + 1]; ++p)
         for (int q = g.poff[j]; q < g.poff[j + 1]; ++q) {
-            double s, t;
-            prim_st(g, li, lj, A, B, g.expo[p], g.expo[q], true, &s, &t);
+            const size_t slot = (size_t)(p - g.poff[i]) * mp + (q - g.poff[j]);
             const double cc = g.coef[p] * g.coef[q];
-            S += cc * s;
-            T += cc * t;
+            S += cc * wij[2 * slot];
+            T += cc * wij[2 * slot + 1];
         }
     g.S[(size_t)i + (size_t)g.nb * j] = S;
     g.T[(size_t)i + (size_t)g.nb * j] = T;
@@ -371,32 +390,33 @@ __global__ __launch_bounds__(256) void k_one_st(OneStArgs g) {
 
 // normalize_AO_toroidal_nD: every primitive of AO i to one, then the
 // contraction: c_p = coef_p / sqrt(S_pp), out = c / sqrt(c^T S c), S the
-// self-overlap of the AO's raw primitives (same centre).  One thread per AO.
-__global__ __launch_bounds__(64) void k_normalize(OneStArgs g, const double *__restrict__ raw,
-                                                  double *__restrict__ out) {
+// self-overlap of the AO's raw primitives (same centre).  The self-overlaps by one
+// thread per primitive pair (w: slot (i mp + p') mp + q'), then one thread per AO.
+__global__ __launch_bounds__(256) void k_norm_prim(OneStArgs g, int mp, double *__restrict__ w) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)g.nb * mp * mp) return;
+    const int qq = (int)(idx % mp), pp = (int)((idx / mp) % mp), i = (int)(idx / ((long long)mp * mp));
+    const int p0 = g.poff[i];
+    if (p0 + pp >= g.poff[i + 1] || p0 + qq >= g.poff[i + 1]) return;
+    double s, dummy;
+    prim_st(g, g.lxyz + 3 * i, g.lxyz + 3 * i, g.center + 3 * i, g.center + 3 * i, g.expo[p0 + pp], g.expo[p0 + qq],
+            false, &s, &dummy);
+    w[idx] = s;
+}
+
+__global__ __launch_bounds__(64) void k_normalize(OneStArgs g, const double *__restrict__ raw, int mp,
+                                                  const double *__restrict__ w, double *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.nb) return;
-    const int *li = g.lxyz + 3 * i;
-    const double *A = g.center + 3 * i;
     const int p0 = g.poff[i], np = g.poff[i + 1] - p0;
+    const double *wi = w + (size_t)i * mp * mp;
     double norm2 = 0.0;
     for (int p = 0; p < np; ++p) {
-        double spp, dummy;
-        prim_st(g, li, li, A, A, g.expo[p0 + p], g.expo[p0 + p], false, &spp, &dummy);
-        const double cp = raw[p0 + p] / sqrt(spp);
-        for (int q = 0; q < np; ++q) {
-            double sqq, spq;
-            prim_st(g, li, li, A, A, g.expo[p0 + q], g.expo[p0 + q], false, &sqq, &dummy);
-            prim_st(g, li, li, A, A, g.expo[p0 + p], g.expo[p0 + q], false, &spq, &dummy);
-            norm2 += cp * (raw[p0 + q] / sqrt(sqq)) * spq;
-        }
+        const double cp = raw[p0 + p] / sqrt(wi[p * mp + p]);
+        for (int q = 0; q < np; ++q) norm2 += cp * (raw[p0 + q] / sqrt(wi[q * mp + q])) * wi[p * mp + q];
     }
     const double nrm = 1.0 / sqrt(norm2);
-    for (int p = 0; p < np; ++p) {
-        double spp, dummy;
-        prim_st(g, li, li, A, A, g.expo[p0 + p], g.expo[p0 + p], false, &spp, &dummy);
-        out[p0 + p] = raw[p0 + p] / sqrt(spp) * nrm;
-    }
+    for (int p = 0; p < np; ++p) out[p0 + p] = raw[p0 + p] / sqrt(wi[p * mp + p]) * nrm;
 }
 
 // rows of the other cells from the unit cell's (src/Tools/symmetric_matrix.f90:1-52):
@@ -419,14 +439,22 @@ __global__ __launch_bounds__(256) void k_translate_fill(double *__restrict__ M, 
     M[(size_t)i + (size_t)nb * j] = M[(size_t)a + (size_t)nb * src];
 }
 
-hipError_t launch_one_st(const OneStArgs &g, hipStream_t st) {
+hipError_t launch_one_st(const OneStArgs &g, int mp, double *work, hipStream_t st) {
     const long long n = (long long)g.nf * g.nb;
-    hipLaunchKernelGGL(k_one_st, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_one_st_prim, dim3((unsigned)((n * mp * mp + 255) / 256)), dim3(256), 0, st, g, mp, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_one_st, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, mp, work);
     return hipGetLastError();
 }
 
-hipError_t launch_normalize(const OneStArgs &g, const double *raw, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_normalize, dim3((g.nb + 63) / 64), dim3(64), 0, st, g, raw, out);
+hipError_t launch_normalize(const OneStArgs &g, const double *raw, double *out, int mp, double *work,
+                            hipStream_t st) {
+    const long long n = (long long)g.nb * mp * mp;
+    hipLaunchKernelGGL(k_norm_prim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, mp, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_normalize, dim3((g.nb + 63) / 64), dim3(64), 0, st, g, raw, mp, work, out);
     return hipGetLastError();
 }
 
