@@ -147,6 +147,8 @@ struct OneStArgs {
     const double *expo, *coef;
     double *S, *T;            // (nb, nb) column-major, rows < nf written
 };
+// chunks of the auxiliary index in the DF J pass (its partial sums: kDfJChunks nf nb doubles)
+constexpr int kDfJChunks = 64;
 // work: 2 nf nb mp^2 (S, T) and nb mp^2 doubles; mp = most primitives of an AO
 hipError_t launch_one_st(const OneStArgs &g, int mp, double *work, hipStream_t st);
 hipError_t launch_normalize(const OneStArgs &g, const double *raw, double *out, int mp, double *work,

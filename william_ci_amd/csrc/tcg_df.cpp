@@ -35,7 +35,8 @@ hipError_t launch_metric_factor(const double *Vre, const double *Vim, int n, int
                                 double2 *work, double *fre, double *fim, int *rank, double *dmin, hipStream_t st);
 hipError_t launch_df_gamma(const double *B, const double *Ps, int nf, int nb, int nau, int N, double *G,
                            hipStream_t st);
-hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *J, hipStream_t st);
+hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *part, double *J,
+                          hipStream_t st);
 hipError_t launch_df_l(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,
                        int v_lo, const int *cdiff, double2 *L, hipStream_t st);
 hipError_t launch_df_mp2_f32(const double *Fr, const double *Fi, const double2 *Ck, int nf, int nau, int N, int o_hi,
@@ -61,6 +62,7 @@ struct tcg_df {
     std::vector<int> cdiff, cadd;
     DevBuf<int> d_cdiff, d_cadd;
     DevBuf<double> d_B;                    // df_B(nfuc, nBas, naux), column-major
+    DevBuf<double> d_jpart;                // partial sums of the J pass
     // F(a, b0, k, Q, kap) = sum_{Rb, RQ} df_B(a, b0 + Rb, Q RQ) e^{ik Rb} e^{-i kap RQ} (real, imaginary)
     DevBuf<double> d_Fr, d_Fi;
     bool f_ready = false;
@@ -428,7 +430,8 @@ int tcg_df_jk(tcg_df *df, int nbas, const double *P, double *J, double *K) {
                 gam[(size_t)df->cadd[(size_t)rq * N + R] * nau + q] += G[(size_t)rq * nau + q + (size_t)naux * R];
     DCHK(df, dgam.upload(gam));
     DCHK(df, dJ.alloc(nx));
-    DCHK(df, launch_df_jvec(df->d_B.p, dgam.p, nf, nb, naux, dJ.p, df->st));
+    DCHK(df, df->d_jpart.alloc(kDfJChunks * (size_t)nx));  // the J pass's partial sums
+    DCHK(df, launch_df_jvec(df->d_B.p, dgam.p, nf, nb, naux, df->d_jpart.p, dJ.p, df->st));
     // K in k space (df_toroidal.f90:893-954): P(k) = sum_R P(0, R) e^{ikR} (its Hermitian part),
     // Kt(kk) = -1/(2N) sum_{k1,Q} M P(k1) M^H with M = Bh_Q(k1, kk - k1) = F(k1, Q, kap = k1 - kk),
     // K(mu0, s0 + R) = 1/N Re sum_kk Kt(kk) e^{-i kk R}.  Only the first nfuc rows of P enter.

@@ -10,7 +10,7 @@
 //                    (:589-592), so that B = 1/N Re sum_k W(k) e^{-ikR} is a
 //                    factor of the fitted integrals (W(-k) = conj W(k)).
 //   k_df_gamma       G(Q, R) = sum_x B(x, Q) Ps(x, R)  (x = (mu0, nu) pairs)
-//   k_df_jvec        J(x) = sum_Q B(x, Q) gamma(Q)
+//   k_df_jvec        J(x) = sum_Q B(x, Q) gamma(Q) (partial sums over chunks of Q, then their sum)
 //   k_df_kt_y/_k     the exchange in k space (df_toroidal.f90:893-954):
 //                    Kt(kk) = -1/(2N) sum_{k1,Q} M P(k1) M^H,  M = Bh_Q(k1, kk - k1)
 //   k_df_l           L(p, q, Q, k1, k2) = sum_ab conj C_k1(a, p) C_k2(b, q) F(a, b, k2, Q, k2 - k1)
@@ -318,12 +318,26 @@ __global__ __launch_bounds__(256) void k_df_gamma(const double *__restrict__ B, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_df_jvec(const double *__restrict__ B, const double *__restrict__ gam, long nx,
-                                                 int naux, double *__restrict__ J) {
+// J(x) = sum_Q B(x, Q) gamma(Q): nx rows are too few threads to stream B (nx naux
+// doubles) at HBM speed, so the Q range is split over kJChunks workgroup rows, each
+// writing its partial sums; a second pass adds the chunks in order (the same J on
+// every run)
+constexpr int kJChunks = kDfJChunks;
+__global__ __launch_bounds__(256) void k_df_jvec_part(const double *__restrict__ B, const double *__restrict__ gam,
+                                                      long nx, int naux, double *__restrict__ part) {
+    const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    const int per = (naux + kJChunks - 1) / kJChunks, q0 = blockIdx.y * per, q1 = min(naux, q0 + per);
+    double s = 0.0;
+    for (int Q = q0; Q < q1; ++Q) s += B[x + nx * (size_t)Q] * gam[Q];
+    part[x + nx * (size_t)blockIdx.y] = s;
+}
+
+__global__ __launch_bounds__(256) void k_df_jvec(const double *__restrict__ part, long nx, double *__restrict__ J) {
     const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= nx) return;
     double s = 0.0;
-    for (int Q = 0; Q < naux; ++Q) s += B[x + nx * (size_t)Q] * gam[Q];
+    for (int c = 0; c < kJChunks; ++c) s += part[x + nx * (size_t)c];
     J[x] = s;
 }
 
@@ -333,9 +347,15 @@ hipError_t launch_df_gamma(const double *B, const double *Ps, int nf, int nb, in
     return hipGetLastError();
 }
 
-hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *J, hipStream_t st) {
+// part: kJChunks nf nb doubles
+hipError_t launch_df_jvec(const double *B, const double *gam, int nf, int nb, int naux, double *part, double *J,
+                          hipStream_t st) {
     const long nx = (long)nf * nb;
-    hipLaunchKernelGGL(k_df_jvec, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, st, B, gam, nx, naux, J);
+    const unsigned gx = (unsigned)((nx + 255) / 256);
+    hipLaunchKernelGGL(k_df_jvec_part, dim3(gx, kJChunks), dim3(256), 0, st, B, gam, nx, naux, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_df_jvec, dim3(gx), dim3(256), 0, st, part, nx, J);
     return hipGetLastError();
 }
 
