@@ -97,12 +97,15 @@ __device__ __forceinline__ bool slab_hit(const EriArgs &A, const Quartet &Q) {
 __global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__restrict__ scount,
                                              long long *__restrict__ rcount, const long long *__restrict__ soff,
                                              Surv *__restrict__ surv) {
-    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);     // four wavefronts, one quartet each
+    // 16 lanes per quartet, four quartets per wavefront (most quartets have few
+    // candidate primitive quartets; a group's survivors keep their candidate order)
+    const int lane = threadIdx.x & 63, gl = lane & 15, grp = lane >> 4;
+    const int q0 = blockIdx.x * 16 + (threadIdx.x >> 4);
     // a list whose length is on the device: the entries past it count zero
     const int nlive = A.nq4_dev ? (int)min((unsigned long long)A.nq4, *A.nq4_dev) : A.nq4;
-    for (int qi = blockIdx.x * 4 + wv; qi < A.nq4; qi += gridDim.x * 4) {
+    for (int qi = q0; qi < A.nq4; qi += gridDim.x * 16) {
         if (qi >= nlive || !slab_hit(A, A.q4[qi])) {
-            if (!pass && lane == 0) {
+            if (!pass && gl == 0) {
                 scount[qi] = 0;
                 rcount[qi] = 0;
             }
@@ -113,8 +116,8 @@ __global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__
         const long ncand = (Q.prim >= 0) ? 1 : (long)bp.count * kp.count;
         long n = 0;
         const long base_out = pass ? soff[qi] : 0;
-        for (long base = 0; base < ncand; base += 64) {
-            const long c = base + lane;
+        for (long base = 0; base < ncand; base += 16) {
+            const long c = base + gl;
             bool ok = c < ncand;
             int ij = 0, kl = 0;
             double fac = 0.0;
@@ -128,18 +131,20 @@ __global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__
                 if (A.screen_prefactor && Q.prim < 0 && fabs(fac) * pb.cmax * pk.cmax <= kScreenTol) ok = false;
                 if (A.screen_prim && pb.ps * pk.ps < kPrimTol) ok = false;
             }
-            const unsigned long long m = __ballot(ok);
+            // this group's 16 bits of the wavefront's ballot (lanes of groups that left
+            // the loop are inactive and contribute none)
+            const unsigned m = (unsigned)((__ballot(ok) >> (16 * grp)) & 0xffffull);
             if (pass && ok) {
-                const long pos = n + __popcll(m & ((1ull << lane) - 1ull));
-                Surv s;
-                s.pb = bp.first + ij;
-                s.pk = kp.first + kl;
-                s.fac = fac;
-                surv[base_out + pos] = s;
+                const long pos = n + __popc(m & ((1u << gl) - 1u));
+                Surv sv;
+                sv.pb = bp.first + ij;
+                sv.pk = kp.first + kl;
+                sv.fac = fac;
+                surv[base_out + pos] = sv;
             }
-            n += __popcll(m);
+            n += __popc(m);
         }
-        if (!pass && lane == 0) {
+        if (!pass && gl == 0) {
             int l[4], ncomb;
             quartet_shape(A, Q, l, ncomb);
             scount[qi] = n;
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(256) void k_surv(EriArgs A, int pass, long long *__
 hipError_t launch_surv(const EriArgs &a, int pass, long long *scount, long long *rcount, const long long *soff,
                        Surv *surv, hipStream_t st) {
     if (a.nq4 <= 0) return hipSuccess;
-    const int nwg = (a.nq4 + 3) / 4;
+    const int nwg = (a.nq4 + 15) / 16;
     const int grid = nwg < (1 << 20) ? nwg : (1 << 20);
     hipLaunchKernelGGL(k_surv, dim3(grid), dim3(256), 0, st, a, pass, scount, rcount, soff, surv);
     return hipGetLastError();
