@@ -92,7 +92,8 @@ int tcg_jk_stored_noalloc(tcg_ctx *ctx, int n1, int nbas, const double *P, doubl
  *   direct_init(ndp,nBas,AO)        src/module/eri_direct.f90:76-225
  *   direct_JK(nBas,P,J,K)           src/module/eri_direct.f90:262-374
  * tcg_direct_init: Schwarz, primitive Schwarz and the unique quartet list
- * (again after every tcg_set_basis / tcg_set_quadrature).
+ * (again after every tcg_set_basis / tcg_set_quadrature); the direct calls that
+ * follow use the plain rule (a learned list of tcg_direct_init_learned is freed).
  * tcg_direct_jk: J(mu,nu) += E P(la,si), K(mu,si) -= E P(nu,la) / 2 over every
  * stored element E = ERI(mu,nu,la,si) of the SAO layout, each counted once;
  * rows mu <= nfuc written, the rest zero; P, J, K are (nBas,nBas). */
@@ -132,10 +133,12 @@ int tcg_kmp2(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, c
  * followed by kMP2_calc.  Needs tcg_direct_init.  The integrals are built in
  * slabs of the last index, j4 = d + nf R4 for the cells R4 of the slab (the
  * reference slices by unit-cell shell; the transform is linear in j4 either
- * way), each slab transformed over its two ket indices and summed; the unique
- * quartet list is evaluated once per slab (with the learned rules after
- * tcg_direct_init_learned).  slab_cells = cells per slab (0:
- * as many as a quarter of the free device memory holds).  At most 64 cells.
+ * way), each slab transformed over its two ket indices and summed.  A slab
+ * takes cells in order of distance from cell 0 (each next to its inverse); per
+ * slab the unique quartets that store into its cells are evaluated (with the
+ * learned rules after tcg_direct_init_learned).  slab_cells = cells per slab (0:
+ * as many as the free device memory holds next to the transforms' buffers and
+ * an 8 GB row budget).  At most 64 cells.
  * out[0..4] as tcg_kmp2; out[5] integral time, out[6] transform time (ms),
  * out[7] number of slabs. */
 int tcg_kmp2_direct(tcg_ctx *ctx, int nbas, const double *coeff_sao, const double *e, const int *occ,
