@@ -175,3 +175,49 @@ def test_learned_build_chunked_equals_one_pass(monkeypatch):
     E2, c2 = learned.eri_build_learned(eng, m)
     eng.close()
     assert np.array_equal(c1, c2) and np.abs(E1 - E2).max() <= 1e-14
+
+
+def test_learned_gain_against_converged_quadrature():
+    """The learned orders against the plain path at the learned path's accuracy
+    (the 128-point table, the converged quadrature the network reproduces) on
+    the Tori1D H64 chain: fewer row flops (counted by the row kernel) and a
+    shorter ERI build, with E_total = E_RHF + E_c(k-MP2) within 1 microhartree.
+    Bars: flops at least 2x fewer, build time at least 1.5x shorter (the bench
+    measures about 2.2x), |dE_total| <= 1e-6 Eh."""
+    import time
+    from william_ci_amd import learned, scf
+    from william_ci_amd.engine import Engine
+    s = tsys.workload("h64_chain")
+    print("case learned vs 128-point table, h64_chain", flush=True)
+    coef = normalised_coef(s)
+    eng = Engine(s, coef)
+    S, T, V = eng.one_electron()
+
+    def timed(build):
+        build()                                              # first call: buffers
+        t0 = time.perf_counter()
+        for _ in range(5):
+            eng.refresh_basis(coef)
+            build()
+        return (time.perf_counter() - t0) / 5
+
+    def energy():
+        r = scf.rhf_sao_gpu(s, eng, S, T, V)
+        return r.energy + eng.kmp2(r.mo_coef, r.mo_energies, r.occupied)["ec"]
+
+    eng.set_quadrature(False, 128)
+    t128 = timed(lambda: eng.eri_build(to_host=False))
+    f128 = eng.stats()["row_flops"]
+    e128 = energy()
+    eng.set_quadrature(False, 64)
+    m = learned.Model.load()
+    m.install(eng)
+    tl = timed(lambda: learned.eri_build_learned(eng, m, to_host=False))
+    fl = eng.stats()["row_flops"]
+    el = energy()
+    eng.close()
+    print(f"  128-point: {t128 * 1e3:.2f} ms, {f128:.3e} row flops;  learned: {tl * 1e3:.2f} ms, {fl:.3e} row flops;"
+          f"  dE_total {el - e128:.2e}")
+    assert f128 >= 2.0 * fl
+    assert t128 >= 1.5 * tl
+    assert abs(el - e128) <= 1e-6
